@@ -1,0 +1,236 @@
+/*
+ * anacapa_hip.h — C ABI of the MI355X (gfx950) wavefront Path+NEE backend.
+ *
+ * This library is a drop-in for the renderer's GPU backend slot, the role
+ * CudaPathIntegrator plays in the reference (src/gpu/cuda/CudaPathIntegrator.h:21-52,
+ * behind IIntegrator, include/anacapa/integrator/IIntegrator.h:172-198).
+ * Every entry point takes plain pointers and sizes; all device memory is owned
+ * by the context.  Every call returns an int status (0 = OK, see AHIP_E_*);
+ * the message of the last failure is available from ahip_last_error().
+ * Calls on one context are serialised internally; they are synchronous at the
+ * ABI boundary (results are complete when the call returns).
+ */
+#ifndef ANACAPA_HIP_H
+#define ANACAPA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AHIP_ABI_VERSION 1
+
+/* status codes */
+#define AHIP_OK              0
+#define AHIP_E_INVALID_ARG   1
+#define AHIP_E_NO_DEVICE     2
+#define AHIP_E_HIP           3
+#define AHIP_E_NOT_PREPARED  4
+#define AHIP_E_UNSUPPORTED   5
+#define AHIP_E_INJECTED      6
+#define AHIP_E_OUT_OF_MEMORY 7
+
+/* material types (reference: src/shading/Lambertian.h:29-99,
+ * src/shading/StandardSurface.h:115-691, src/shading/MarschnerHair.h) */
+#define AHIP_MAT_LAMBERT   0
+#define AHIP_MAT_EMISSIVE  1
+#define AHIP_MAT_STANDARD  2
+
+/* light types (reference: src/shading/lights/AreaLight.h:118-226,
+ * src/shading/lights/DomeLight.h:78-388) */
+#define AHIP_LIGHT_AREA 0
+#define AHIP_LIGHT_DOME 1
+
+/* ahip_render_params.flags */
+#define AHIP_FLAG_FIX_DOUBLE_COSINE 1u  /* opt out of the reference's extra cos in NEE's BSDF branch
+                                          (src/integrator/PathIntegrator.cpp:257-258) */
+
+typedef struct ahip_material {
+    uint32_t type;              /* AHIP_MAT_* */
+    float    albedo[3];         /* Lambert/Emissive albedo; StandardSurface base_color */
+    float    emission[3];       /* Emissive Le; StandardSurface emission_color */
+    float    emission_weight;   /* StandardSurface `emission` (0 = none) */
+    float    base;              /* StandardSurface Params (StandardSurface.h:117-148) */
+    float    metalness;
+    float    roughness;
+    float    specular;
+    float    specular_color[3];
+    float    specular_ior;
+    float    coat;
+    float    coat_roughness;
+    float    transmission;
+    int32_t  roughness_texture; /* -1 = constant roughness; otherwise index of a texture */
+    uint32_t pad[2];
+} ahip_material;
+
+typedef struct ahip_light {
+    uint32_t type;              /* AHIP_LIGHT_* */
+    float    center[3];         /* area: quad centre */
+    float    u_axis[3];         /* area: half-extent vectors */
+    float    v_axis[3];
+    float    Le[3];             /* area: emitted radiance */
+    float    intensity;         /* dome: radiance scale */
+    float    rot[9];            /* dome: world->envmap rows (DomeLight.h:105-112) */
+} ahip_light;
+
+/* Flat scene.  Mirrors GeometryPool / SceneView (include/anacapa/accel/GeometryPool.h:118-197,
+ * IIntegrator.h:150-157): meshes are ranges of one vertex array and one index array;
+ * indices are local to their mesh. Materials are indexed by mesh (SceneView::materials). */
+typedef struct ahip_scene_desc {
+    uint32_t        num_meshes;
+    const uint32_t* mesh_vertex_offset;   /* [num_meshes+1] first vertex of each mesh */
+    const uint32_t* mesh_index_offset;    /* [num_meshes+1] first index (multiple of 3) */
+    const uint32_t* mesh_normal_count;    /* [num_meshes] normals present per mesh (<= vertices), may be NULL */
+    const uint32_t* mesh_uv_count;        /* [num_meshes] uvs present per mesh, may be NULL */
+    const int32_t*  mesh_material;        /* [num_meshes] material index, -1 = none */
+    const float*    positions;            /* [num_vertices*3] world space */
+    const float*    normals;              /* [num_vertices*3] or NULL */
+    const float*    uvs;                  /* [num_vertices*2] or NULL */
+    const uint32_t* indices;              /* [num_indices] */
+    uint32_t        num_vertices;
+    uint32_t        num_indices;
+
+    uint32_t             num_materials;
+    const ahip_material* materials;
+    uint32_t             num_lights;
+    const ahip_light*    lights;          /* the dome (if any) is also the environment light */
+    int32_t              env_light;       /* index of the dome in lights[], -1 = constant background */
+    float                env_radiance[3]; /* SceneView::envRadiance when env_light < 0 */
+    uint32_t             env_width;       /* dome equirect RGB f32 image, row 0 = zenith */
+    uint32_t             env_height;
+    const float*         env_pixels;
+
+    uint32_t             num_textures;    /* single-channel f32 textures (roughness) */
+    const uint32_t*      texture_width;
+    const uint32_t*      texture_height;
+    const float* const*  texture_pixels;
+} ahip_scene_desc;
+
+/* Camera fields of the reference pinhole Camera (IIntegrator.h:27-74). */
+typedef struct ahip_camera {
+    float    origin[3];
+    float    lower_left[3];
+    float    horizontal[3];
+    float    vertical[3];
+    uint32_t width;
+    uint32_t height;
+} ahip_camera;
+
+typedef struct ahip_render_params {
+    uint32_t max_depth;   /* PathIntegrator maxDepth, inclusive (PathIntegrator.cpp:80) */
+    uint32_t min_depth;   /* Russian roulette from bounce >= min_depth (PathIntegrator.cpp:156) */
+    uint64_t seed;        /* HaltonSampler seed (HaltonSampler.h:24) */
+    uint32_t flags;       /* AHIP_FLAG_* */
+    uint32_t pad;
+} ahip_render_params;
+
+/* One camera view of the batched persistent form. */
+typedef struct ahip_view {
+    ahip_camera camera;
+    uint32_t    sample_start;
+    uint32_t    sample_count;
+    uint32_t    max_depth;
+    uint32_t    pad;
+    float*      accum;        /* host output [width*height*5] r,g,b,w,lumSq sums, may be NULL */
+} ahip_view;
+
+typedef struct ahip_stats {
+    uint64_t rays_closest;       /* continuation + MIS rays traced (closest hit) */
+    uint64_t rays_shadow;        /* NEE shadow rays traced (any hit) */
+    uint64_t samples;            /* camera samples rendered */
+    uint64_t nodes_visited;      /* filled by ahip_count_traffic only */
+    uint64_t prims_tested;
+    double   trace_ms;           /* sum of HIP-event durations of the traversal kernels */
+    double   shade_ms;           /* sum of HIP-event durations of generate/shade/accumulate */
+    uint64_t trace_launches;
+    uint64_t shade_launches;
+    double   render_ms;          /* wall time of the last render call on the stream (HIP events) */
+} ahip_stats;
+
+typedef struct ahip_context ahip_context;
+
+/* --- lifetime (CudaPathIntegrator ctor/isValid, CudaPathIntegrator.cu:211-227; IBackend.h:13-20) --- */
+int         ahip_create(int device_ordinal, ahip_context** out);
+void        ahip_destroy(ahip_context* ctx);
+int         ahip_is_valid(const ahip_context* ctx);
+const char* ahip_device_name(const ahip_context* ctx);
+const char* ahip_backend_type(void);
+const char* ahip_last_error(const ahip_context* ctx);
+int         ahip_abi_version(void);
+
+/* --- scene upload (CudaPathIntegrator::prepare, CudaPathIntegrator.cu:240-317):
+ *     builds the SAH-12 BVH of BVHBackend::commit (src/accel/BVHBackend.cpp:17-228)
+ *     and copies geometry, materials, lights and the HDRI into HBM. --- */
+int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* scene);
+/* BVH sizes after prepare: nodes (64-B records) and leaf-ordered triangles. */
+int ahip_bvh_info(ahip_context* ctx, uint32_t* num_nodes, uint32_t* num_tris, uint32_t* max_depth);
+/* Dump the binary (reference-layout) BVH built on the host: per node
+ * {bmin[3], dataA, bmax[3], dataB} exactly as BVHNode (BVHBackend.h:25-39), and the
+ * leaf primitive order (BVHBackend m_primIndices). Either pointer may be NULL. */
+int ahip_bvh_dump(ahip_context* ctx, float* nodes32, uint32_t* prim_indices);
+
+/* --- rendering (IIntegrator::renderFrame / renderTile, IIntegrator.h:181-197) --- */
+int ahip_set_camera(ahip_context* ctx, const ahip_camera* cam);
+int ahip_set_params(ahip_context* ctx, const ahip_render_params* params);
+int ahip_clear_accum(ahip_context* ctx);    /* CudaPathIntegrator::clearAccum (.h:47) */
+/* Accumulate samples [sample_start, sample_start+sample_count) of every pixel into the
+ * persistent device buffer (CudaPathIntegrator::renderFrame, .cu:379-444). */
+int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height,
+                      uint32_t sample_start, uint32_t sample_count);
+/* Copy the accumulation buffer: [width*height*5] = r,g,b sums, weight, sum of luminance^2
+ * (GpuAccumPixel, src/gpu/cuda/shaders/SharedTypes.h:127-130). */
+int ahip_download_accum(ahip_context* ctx, float* out);
+/* Render one tile into out[w*h*5] with the same layout (CudaPathIntegrator::renderTile, .cu:449-489). */
+int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                     uint32_t sample_start, uint32_t sample_count, float* out);
+/* Batched persistent form: many views share one BVH; one graph-captured bounce loop. */
+int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views);
+
+/* --- acceleration boundary (IAccelerationStructure::traceBatch / occludedBatch,
+ *     include/anacapa/accel/IAccelerationStructure.h:75-87) ---
+ * rays: [n*8] = origin xyz, direction xyz, tMin, tMax (Ray, include/anacapa/core/Types.h:253-266).
+ * hits: one ahip_hit per ray; mesh_id == ~0u on a miss. */
+typedef struct ahip_hit {
+    float    t, u, v;      /* ray parameter and barycentrics of the closest hit */
+    uint32_t mesh_id;      /* SurfaceInteraction::meshID (IAccelerationStructure.h:21) */
+    uint32_t prim_id;      /* triangle index within its mesh (SurfaceInteraction::primID) */
+    uint32_t tri_index;    /* index in the flattened (commit-order) triangle array */
+} ahip_hit;
+int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits);
+int ahip_occluded_rays(ahip_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
+
+/* --- sampler (HaltonSampler::get1D, src/sampling/HaltonSampler.h:43-50): dimension dim of
+ *     sample idx at pixel (px,py); dims >= 128 come from the per-pixel PCG stream. --- */
+int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const uint32_t* py,
+                     const uint32_t* idx, const uint32_t* dim, uint32_t n, float* out);
+
+/* --- differentiation: path-replay adjoint of an image L2 loss w.r.t. per-triangle albedo
+ *     and the roughness texture (no reference counterpart; see header of adjoint kernels). --- */
+int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
+                 uint32_t sample_count, const float* dL_dimage,
+                 float* d_albedo, float* d_roughness_tex);
+
+/* --- measurement --- */
+int ahip_get_stats(ahip_context* ctx, ahip_stats* out);
+int ahip_reset_stats(ahip_context* ctx);
+/* Re-run the last render_frame workload with per-ray node/primitive counters
+ * (a separate, untimed pass) and fill nodes_visited/prims_tested. */
+int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height,
+                       uint32_t sample_start, uint32_t sample_count);
+/* Launch-shape knobs (autotuner): paths per wavefront pass and tile width of the
+ * pixel->path swizzle; 0 keeps the current value. */
+int ahip_set_launch(ahip_context* ctx, uint32_t max_paths, uint32_t tile);
+
+/* --- failure injection for the CPU-fallback tests (RenderSession.cpp:441-464, 546-547) --- */
+int ahip_fail_next_launch(ahip_context* ctx);
+
+/* --- host helper: reference pinhole camera (Camera::makePinhole, IIntegrator.h:49-74) --- */
+int ahip_camera_pinhole(const float from[3], const float at[3], const float up[3],
+                        float vfov_degrees, uint32_t width, uint32_t height, ahip_camera* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANACAPA_HIP_H */

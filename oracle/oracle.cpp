@@ -1,0 +1,958 @@
+// oracle.cpp — CPU restatement of the reference Path+NEE hot path.
+//
+// ORACLE / TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and bench.py's
+// cpu_baseline leg as the checker, never by the product library.
+//
+// A plain, single-ray restatement of the reference algorithms, each function citing the
+// reference file:line it follows (paths relative to the reference checkout).  Compiled
+// with -ffp-contract=off so it is a fixed IEEE program; the only fused multiply-adds are
+// the explicit ones the reference's own -mfma build produces where bit-exactness depends
+// on them (Halton accumulation).  Scope: triangle meshes, Lambert / Emissive / opaque
+// StandardSurface materials, quad AreaLight and equirect DomeLight (constant textures plus
+// a single-channel roughness texture with this build's own bilinear lookup).
+//
+// Parity pins (see tests/): PCG vs the published pcg32 known answer and vs the reference
+// build; Halton, camera rays, BVH topology, closest hits and Cornell-box radiance vs the
+// reference build in oracle/_ref; GGX helpers and Distribution1D vs the reference's
+// header-inline functions; StandardSurface mixture and DomeLight sampling by closed-form
+// invariants (pdf normalisation, sample/evaluate consistency, constant-dome pdf).
+// Parity unpinned: the roughness-texture lookup (reference uses OIIO's TextureSystem,
+// src/shading/Texture.cpp:35-83, which is not available here).
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+
+namespace orc {
+
+struct SI {                                                                           // IAccelerationStructure.h:12-30
+    V3 p, n, ng, dpdu, dpdv;
+    V2 uv;
+    float t = 0.f;
+    uint32_t meshID = ~0u, primID = ~0u, tri = ~0u;
+};
+static void fillSI(const Tri& tr, float t, float u, float v, SI& si) {               // BVHBackend.cpp:285-317
+    float w = 1.f - u - v;
+    si.t = t;
+    si.p = tr.v0 + tr.e1 * u + tr.e2 * v;
+    si.ng = tr.n;
+    si.n = safeNormalize(tr.sn0 * w + tr.sn1 * u + tr.sn2 * v);
+    si.dpdu = tr.e1;
+    si.dpdv = tr.e2;
+    if (dot(si.n, si.ng) < 0.f) si.n = -si.n;
+    si.uv.x = tr.uv0.x * w + tr.uv1.x * u + tr.uv2.x * v;
+    si.uv.y = tr.uv0.y * w + tr.uv1.y * u + tr.uv2.y * v;
+    si.meshID = tr.meshID;
+    si.primID = tr.primID;
+}
+
+// ---------------------------------------------------------------- shading frame (ShadingContext.h:28-59)
+struct Ctx {
+    V3 p, n, ng, t, bt;
+    V2 uv;
+    bool front = true;
+    Ctx(const SI& si, V3 rayDir) {
+        p = si.p; ng = si.ng; uv = si.uv;
+        front = dot(-rayDir, ng) > 0.f;
+        n = front ? si.n : -si.n;
+        ng = front ? si.ng : -si.ng;
+        onb(n, t, bt);
+    }
+    V3 toLocal(V3 v) const { return {dot(v, t), dot(v, bt), dot(v, n)}; }
+    V3 toWorld(V3 v) const { return t * v.x + bt * v.y + n * v.z; }
+};
+
+// ---------------------------------------------------------------- textures (this build's lookup)
+struct Texture {
+    uint32_t w = 0, h = 0;
+    std::vector<float> px;
+    // Bilinear, periodic wrap in u and v, V flipped as Texture.cpp:67-68 does before the lookup.
+    float sample(V2 uv) const {
+        float s = uv.x, tt = 1.f - uv.y;
+        float fx = s * (float)w - 0.5f, fy = tt * (float)h - 0.5f;
+        float x0f = std::floor(fx), y0f = std::floor(fy);
+        float ax = fx - x0f, ay = fy - y0f;
+        int x0 = (int)x0f, y0 = (int)y0f;
+        auto wrap = [](int i, int n) { int r = i % n; return r < 0 ? r + n : r; };
+        int xa = wrap(x0, (int)w), xb = wrap(x0 + 1, (int)w), ya = wrap(y0, (int)h), yb = wrap(y0 + 1, (int)h);
+        float p00 = px[(size_t)ya * w + xa], p01 = px[(size_t)ya * w + xb];
+        float p10 = px[(size_t)yb * w + xa], p11 = px[(size_t)yb * w + xb];
+        return (1.f - ay) * ((1.f - ax) * p00 + ax * p01) + ay * ((1.f - ax) * p10 + ax * p11);
+    }
+};
+
+// ---------------------------------------------------------------- materials
+struct BSDFSample { V3 f, wi; float pdf = 0.f; bool delta = false;
+                    bool valid() const { return pdf > 0.f && !isBlack(f); } };        // IMaterial.h:26-40
+struct BSDFEval { V3 f; float pdf = 0.f; };
+
+static inline V3 cosineHemisphere(V2 u) {                                             // Lambertian.h:12-20
+    float r = std::sqrt(u.x), phi = 2.f * kPi * u.y;
+    return {r * std::cos(phi), r * std::sin(phi), std::sqrt(fmaxs(0.f, 1.f - u.x))};
+}
+static inline float cosPdf(float c) { return fmaxs(0.f, c) * kInvPi; }               // Lambertian.h:22-24
+
+// GGX helpers, StandardSurface.h:20-76
+static inline float D_GGX(float cosH, float a2) {
+    if (cosH <= 0.f) return 0.f;
+    float den = cosH * cosH * (a2 - 1.f) + 1.f;
+    return a2 / (kPi * den * den);
+}
+static inline float G1(float absCos, float a2) {
+    if (absCos <= 0.f) return 0.f;
+    float c2 = absCos * absCos;
+    return 2.f * absCos / (absCos + std::sqrt(a2 + (1.f - a2) * c2));
+}
+static inline float G2(float co, float ci, float a2) { return G1(std::abs(co), a2) * G1(std::abs(ci), a2); }
+static inline V3 sampleGGXh(V2 u, float a2) {
+    float phi = 2.f * kPi * u.x;
+    float c2 = (1.f - u.y) / fmaxs(1.f + (a2 - 1.f) * u.y, 1e-10f);
+    float ch = std::sqrt(fmaxs(0.f, c2)), sh = std::sqrt(fmaxs(0.f, 1.f - c2));
+    return {sh * std::cos(phi), sh * std::sin(phi), ch};
+}
+static inline float pdfGGXr(float cosH, float a2, float dotVH) {
+    if (cosH <= 0.f || dotVH <= 0.f) return 0.f;
+    return D_GGX(cosH, a2) * cosH / (4.f * dotVH);
+}
+static inline float F0ior(float ior) { float r = (1.f - ior) / (1.f + ior); return r * r; }
+static inline float schlickD(float c, float f0) {
+    float x = 1.f - std::abs(c);
+    float x5 = x * x * x * x * x;
+    return f0 + (1.f - f0) * x5;
+}
+static inline V3 schlickC(float c, V3 f0) {
+    float x = 1.f - std::abs(c);
+    float x5 = x * x * x * x * x;
+    return f0 + (V3{1.f, 1.f, 1.f} - f0) * x5;
+}
+static inline float fresnelDielectric(float cosI, float eta) {                        // StandardSurface.h:84-92
+    float s2i = fmaxs(0.f, 1.f - cosI * cosI);
+    float s2t = s2i / (eta * eta);
+    if (s2t >= 1.f) return 1.f;
+    float ct = std::sqrt(1.f - s2t);
+    float rs = (cosI - eta * ct) / (cosI + eta * ct);
+    float rp = (eta * cosI - ct) / (eta * cosI + ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+struct Material {
+    ahip_material m;
+    const Texture* roughTex = nullptr;
+    // derived (StandardSurface.h:152-160)
+    float coatA2 = 0.f, f0d = 0.f, coatF0 = 0.f;
+    void init() {
+        float ca = fmaxs(1e-4f, m.coat_roughness * m.coat_roughness);
+        coatA2 = ca * ca;
+        f0d = F0ior(m.specular_ior);
+        coatF0 = F0ior(1.5f);
+    }
+    V3 albedo() const { return {m.albedo[0], m.albedo[1], m.albedo[2]}; }
+    float rough(V2 uv) const { return roughTex ? roughTex->sample(uv) : m.roughness; }  // evalTOV
+    bool isDelta() const {                                                            // StandardSurface.h:162-169
+        if (m.type != AHIP_MAT_STANDARD) return false;
+        float r = m.roughness, mm = m.metalness, s = m.specular;
+        if (r < 0.001f && m.transmission > 0.001f && mm < 0.001f) return true;
+        return r < 0.001f && (mm > 0.999f || s > 0.f);
+    }
+    bool flagsSpecular() const { return m.type == AHIP_MAT_STANDARD && m.roughness < 0.001f; }
+    V3 Le() const {
+        if (m.type == AHIP_MAT_EMISSIVE) return {m.emission[0], m.emission[1], m.emission[2]};  // Lambertian.h:93-95
+        if (m.type == AHIP_MAT_STANDARD && m.emission_weight > 0.f)                         // StandardSurface.h:183-187
+            return V3{m.emission[0], m.emission[1], m.emission[2]} * m.emission_weight;
+        return {};
+    }
+
+    // layer weights, StandardSurface.h:355-371
+    void weights(float woz, float metal, float spec, float& wc, float& wm, float& ws, float& wd, bool& ok) const {
+        float coatF = schlickD(woz, coatF0), specF = schlickD(woz, f0d);
+        wc = m.coat * coatF;
+        wm = metal * (1.f - wc);
+        ws = spec * (1.f - metal) * specF * (1.f - wc);
+        wd = m.base * (1.f - metal) * (1.f - spec * specF) * (1.f - wc);
+        float sum = wc + wm + ws + wd;
+        ok = sum > 0.f;
+        if (!ok) return;
+        float inv = 1.f / sum;
+        wc *= inv; wm *= inv; ws *= inv; wd *= inv;
+    }
+    // evalGGX, StandardSurface.h:578-603
+    static V3 evalGGX(V3 wo, V3 wi, float a2, V3 f0, float& pf) {
+        pf = 0.f;
+        if (wo.z <= 0.f || wi.z <= 0.f) return {};
+        V3 wh = safeNormalize(wo + wi);
+        if (wh.z <= 0.f) return {};
+        float dVH = fmaxs(0.f, dot(wo, wh));
+        V3 F = schlickC(dVH, f0);
+        float D = D_GGX(wh.z, a2), G = G2(wo.z, wi.z, a2);
+        float co = std::abs(wo.z), ci = std::abs(wi.z);
+        if (co < 1e-6f || ci < 1e-6f) return {};
+        V3 bsdf = F * (D * G / (4.f * co * ci));
+        pf = pdfGGXr(wh.z, a2, dVH);
+        return bsdf;
+    }
+    // evalCombined, StandardSurface.h:608-661
+    V3 evalCombined(V3 wo, V3 wi, float wc, float wm, float ws, float wd, V3 base, float spec, float a2,
+                    float& pdf) const {
+        V3 f;
+        pdf = 0.f;
+        if (!(wo.z * wi.z > 0.f)) return {};
+        float pf;
+        if (wc > 0.f) { V3 fc = evalGGX(wo, wi, coatA2, V3{coatF0, coatF0, coatF0}, pf); f = f + fc * wc; pdf += pf * wc; }
+        if (wm > 0.f) { V3 fm = evalGGX(wo, wi, a2, base * m.base, pf); f = f + fm * wm; pdf += pf * wm; }
+        if (ws > 0.f) {
+            V3 f0 = V3{m.specular_color[0], m.specular_color[1], m.specular_color[2]} * (spec * f0d);
+            V3 fs = evalGGX(wo, wi, a2, f0, pf);
+            f = f + fs * ws; pdf += pf * ws;
+        }
+        if (wd > 0.f) {
+            float ci = std::abs(wi.z);
+            V3 fd = base * m.base * kInvPi;
+            f = f + fd * wd; pdf += ci * kInvPi * wd;
+        }
+        return f;
+    }
+
+    BSDFSample sample(const Ctx& c, V3 wo, V2 u, float uc) const {
+        BSDFSample s;
+        if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:36-51
+            V3 wl = cosineHemisphere(u);
+            if (c.toLocal(wo).z < 0.f) wl.z = -wl.z;
+            s.wi = c.toWorld(wl);
+            s.pdf = cosPdf(std::abs(wl.z));
+            s.f = albedo() * (kInvPi * std::abs(wl.z));
+            return s;
+        }
+        // StandardSurface::sample, opaque branch (StandardSurface.h:231-241, 351-405)
+        V3 wo_l = c.toLocal(wo);
+        if (wo_l.z <= 0.f) return {};
+        V3 base = albedo();
+        float metal = m.metalness, spec = m.specular, r = rough(c.uv);
+        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
+        float wc, wm, ws, wd;
+        bool ok;
+        weights(wo_l.z, metal, spec, wc, wm, ws, wd, ok);
+        if (!ok) return {};
+        BSDFSample res;
+        auto sGGX = [&](float a2x, V3 f0) {                                           // StandardSurface.h:516-549
+            BSDFSample q;
+            V3 wh = sampleGGXh(u, a2x);
+            if (wh.z <= 0.f) return q;
+            float dVH = dot(wo_l, wh);
+            if (dVH <= 0.f) return q;
+            V3 wl = wh * (2.f * dVH) - wo_l;
+            if (wl.z <= 0.f) return q;
+            V3 F = schlickC(dVH, f0);
+            float D = D_GGX(wh.z, a2x), G = G2(wo_l.z, wl.z, a2x);
+            float co = std::abs(wo_l.z), ci = std::abs(wl.z);
+            if (co < 1e-6f || ci < 1e-6f) return q;
+            V3 bsdf = F * (D * G / (4.f * co));
+            q.wi = c.toWorld(wl);
+            q.f = bsdf * ci;
+            q.pdf = pdfGGXr(wh.z, a2x, dVH);
+            return q;
+        };
+        if (uc < wc) res = sGGX(coatA2, V3{coatF0, coatF0, coatF0});
+        else if (uc < wc + wm) res = sGGX(a2, base * m.base);
+        else if (uc < wc + wm + ws)
+            res = sGGX(a2, V3{m.specular_color[0], m.specular_color[1], m.specular_color[2]} * (spec * f0d));
+        else {                                                                        // StandardSurface.h:554-573
+            float phi = 2.f * kPi * u.x, rr = std::sqrt(u.y), z = std::sqrt(fmaxs(0.f, 1.f - u.y));
+            V3 wl{rr * std::cos(phi), rr * std::sin(phi), z};
+            if (wo_l.z < 0.f) wl.z = -wl.z;
+            float ci = std::abs(wl.z);
+            res.wi = c.toWorld(wl);
+            res.f = base * m.base * (kInvPi * ci);
+            res.pdf = ci * kInvPi;
+        }
+        if (!res.valid()) return {};
+        float pdf;
+        V3 wil = c.toLocal(res.wi);
+        V3 fc = evalCombined(wo_l, wil, wc, wm, ws, wd, base, spec, a2, pdf);
+        res.f = fc * std::abs(c.toLocal(res.wi).z);
+        res.pdf = pdf;
+        res.delta = flagsSpecular();                                                  // result.flags = flags()
+        return res;
+    }
+
+    BSDFEval evaluate(const Ctx& c, V3 wo, V3 wi) const {
+        BSDFEval e;
+        V3 wil = c.toLocal(wi), wol = c.toLocal(wo);
+        if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:53-66
+            if (!(wil.z * wol.z > 0.f)) return e;
+            e.f = albedo() * kInvPi;
+            e.pdf = cosPdf(std::abs(wil.z));
+            return e;
+        }
+        // StandardSurface::evaluate opaque branch (StandardSurface.h:410-417, 448-503)
+        if (!(wol.z * wil.z > 0.f)) return e;
+        V3 base = albedo();
+        float metal = m.metalness, spec = m.specular, r = rough(c.uv);
+        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
+        float wc, wm, ws, wd;
+        bool ok;
+        weights(wol.z, metal, spec, wc, wm, ws, wd, ok);
+        if (!ok) return e;
+        e.f = evalCombined(wol, wil, wc, wm, ws, wd, base, spec, a2, e.pdf);
+        return e;
+    }
+};
+
+// ---------------------------------------------------------------- lights
+struct LightSample { V3 Li, wi; float pdf = 0.f, dist = 0.f; };
+
+struct Dist1D {                                                                       // DomeLight.h:19-55
+    std::vector<float> func, cdf;
+    float integral = 0.f;
+    uint32_t n = 0;
+    void build(const float* v, uint32_t cnt) {
+        n = cnt;
+        func.assign(v, v + cnt);
+        cdf.resize(cnt + 1);
+        cdf[0] = 0.f;
+        for (uint32_t i = 0; i < cnt; ++i) cdf[i + 1] = cdf[i] + func[i];
+        integral = cdf[cnt];
+        if (integral > 0.f) {
+            float inv = 1.f / integral;
+            for (auto& c : cdf) c *= inv;
+        }
+    }
+    uint32_t sample(float u, float& pdf, float& ur) const {
+        auto it = std::lower_bound(cdf.begin(), cdf.end(), u);
+        int idx = (int)(it - cdf.begin()) - 1;
+        idx = std::max(0, std::min(idx, (int)n - 1));
+        float bw = cdf[idx + 1] - cdf[idx];
+        ur = (bw > 1e-12f) ? (u - cdf[idx]) / bw : 0.f;
+        ur = fmaxs(0.f, fmins(ur, 1.f - 1e-7f));
+        pdf = (integral > 0.f) ? func[idx] / integral : 0.f;
+        return (uint32_t)idx;
+    }
+};
+
+struct Dome {                                                                         // DomeLight.h:78-388
+    uint32_t W = 0, H = 0;
+    std::vector<float> px;
+    std::vector<Dist1D> cond;
+    Dist1D marg;
+    float intensity = 1.f;
+    V3 rot[3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+
+    void build() {                                                                    // :228-259
+        if (W == 0 || H == 0) { W = H = 1; px = {1.f, 1.f, 1.f}; }
+        std::vector<float> rowSums(H), rowF(W);
+        cond.resize(H);
+        for (uint32_t r = 0; r < H; ++r) {
+            float theta = kPi * ((float)r + 0.5f) / (float)H;
+            float st = std::sin(theta);
+            for (uint32_t c = 0; c < W; ++c) {
+                const float* p = &px[((size_t)r * W + c) * 3];
+                float lum = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+                rowF[c] = lum * st;
+            }
+            cond[r].build(rowF.data(), W);
+            rowSums[r] = cond[r].integral;
+        }
+        marg.build(rowSums.data(), H);
+    }
+    V3 toEnv(V3 w) const { return {dot(rot[0], w), dot(rot[1], w), dot(rot[2], w)}; }
+    V3 fromEnv(V3 v) const {
+        return {rot[0].x * v.x + rot[1].x * v.y + rot[2].x * v.z, rot[0].y * v.x + rot[1].y * v.y + rot[2].y * v.z,
+                rot[0].z * v.x + rot[1].z * v.y + rot[2].z * v.z};
+    }
+    V3 sampleDir(V2 u, float& pdf) const {                                            // :264-295
+        float pr, pc, ur, uc;
+        uint32_t row = marg.sample(u.y, pr, ur);
+        uint32_t col = cond[row].sample(u.x, pc, uc);
+        float v = ((float)row + ur) / (float)H;
+        float uu = ((float)col + uc) / (float)W;
+        float theta = v * kPi, phi = uu * k2Pi;
+        float st = std::sin(theta), ct = std::cos(theta);
+        V3 e{st * std::sin(phi), ct, st * std::cos(phi)};
+        V3 dir = fromEnv(e);
+        float sa = fmaxs(st, 1e-6f);
+        pdf = (pr * pc * (float)(W * H)) / (k2Pi * kPi * sa);
+        if (pdf <= 0.f) pdf = 0.f;
+        return dir;
+    }
+    float evalPdf(V3 w) const {                                                       // :300-321
+        V3 l = toEnv(w);
+        float theta = std::acos(fmaxs(-1.f, fmins(1.f, l.y)));
+        float phi = std::atan2(l.x, l.z);
+        if (phi < 0.f) phi += k2Pi;
+        float uu = phi / k2Pi, v = theta / kPi;
+        uint32_t col = std::min((uint32_t)(uu * (float)W), W - 1u);
+        uint32_t row = std::min((uint32_t)(v * (float)H), H - 1u);
+        float st = fmaxs(std::sin(theta), 1e-6f);
+        float pr = marg.integral > 0.f ? marg.func[row] / marg.integral : 0.f;
+        float pc = cond[row].integral > 0.f ? cond[row].func[col] / cond[row].integral : 0.f;
+        return (pr * pc * (float)(W * H)) / (k2Pi * kPi * st);
+    }
+    V3 eval(V3 w) const {                                                             // :326-364
+        V3 l = toEnv(w);
+        float theta = std::acos(fmaxs(-1.f, fmins(1.f, l.y)));
+        float phi = std::atan2(l.x, l.z);
+        if (phi < 0.f) phi += k2Pi;
+        float uu = phi / k2Pi, v = theta / kPi;
+        float fc = uu * (float)W - 0.5f, fr = v * (float)H - 0.5f;
+        int c0 = (int)std::floor(fc), r0 = (int)std::floor(fr);
+        float tc = fc - (float)c0, tr = fr - (float)r0;
+        auto cc = [&](int c) { return (uint32_t)((c % (int)W + (int)W) % (int)W); };
+        auto cr = [&](int r) { return (uint32_t)std::max(0, std::min(r, (int)H - 1)); };
+        uint32_t c1 = cc(c0 + 1), c0u = cc(c0), r1u = cr(r0 + 1), r0u = cr(r0);
+        const float* p00 = &px[((size_t)r0u * W + c0u) * 3];
+        const float* p01 = &px[((size_t)r0u * W + c1) * 3];
+        const float* p10 = &px[((size_t)r1u * W + c0u) * 3];
+        const float* p11 = &px[((size_t)r1u * W + c1) * 3];
+        V3 s;
+        for (int ch = 0; ch < 3; ++ch) {
+            s[ch] = (1.f - tr) * ((1.f - tc) * p00[ch] + tc * p01[ch]) + tr * ((1.f - tc) * p10[ch] + tc * p11[ch]);
+            s[ch] = fmaxs(0.f, s[ch]);
+        }
+        return s;
+    }
+};
+
+struct Light {
+    uint32_t type = 0;
+    V3 center, uA, vA, Le, normal;
+    float area = 1.f;
+    const Dome* dome = nullptr;
+    void init() {                                                                     // AreaLight.h:123-131
+        normal = safeNormalize(cross(uA, vA));
+        area = length(cross(uA, vA)) * 4.f;
+    }
+    LightSample sample(V3 p, V2 u) const {
+        LightSample s;
+        if (type == AHIP_LIGHT_DOME) {                                                // DomeLight.h:117-129
+            float pdf;
+            V3 wi = dome->sampleDir(u, pdf);
+            if (pdf <= 0.f) return s;
+            s.Li = dome->eval(wi) * dome->intensity;
+            s.wi = wi; s.pdf = pdf; s.dist = 1e10f;
+            return s;
+        }
+        V3 pos = center + uA * (2.f * u.x - 1.f) + vA * (2.f * u.y - 1.f);            // AreaLight.h:133-157
+        V3 tl = pos - p;
+        float d2 = lengthSq(tl), d = std::sqrt(d2);
+        V3 wi = tl * (1.f / d);
+        float cosL = dot(-wi, normal);
+        if (cosL <= 0.f) return s;
+        s.Li = Le; s.wi = wi; s.pdf = d2 / (cosL * area); s.dist = d * (1.f - 1e-4f);
+        return s;
+    }
+    float pdf(V3 p, V3 wi) const {
+        if (type == AHIP_LIGHT_DOME) return dome->evalPdf(wi);
+        float den = dot(wi, -normal);                                                 // AreaLight.h:159-180
+        if (den < 1e-6f) return 0.f;
+        float t = dot(center - p, -normal) / den;
+        if (t <= 0.f) return 0.f;
+        V3 hp = p + wi * t, off = hp - center;
+        float ul = length(uA), vl = length(vA);
+        float up = dot(off, uA) / (ul * ul), vp = dot(off, vA) / (vl * vl);
+        if (std::abs(up) > 1.f || std::abs(vp) > 1.f) return 0.f;
+        return t * t / (den * area);
+    }
+};
+
+// ---------------------------------------------------------------- scene + integrator
+struct Scene {
+    Bvh bvh;
+    std::vector<Material> mats;
+    std::vector<int32_t> meshMat;
+    std::vector<Light> lights;
+    std::vector<Texture> textures;
+    Dome dome;
+    bool hasEnv = false;
+    V3 envRad;
+    Camera cam;
+    uint32_t maxDepth = 8, minDepth = 2;
+    uint64_t seed = 0;
+    bool fixDoubleCos = false;
+
+    const Material* matOf(uint32_t mesh) const {
+        if (mesh >= meshMat.size()) return nullptr;
+        int32_t m = meshMat[mesh];
+        return m >= 0 ? &mats[m] : nullptr;
+    }
+    bool traceSI(const Ray& r, SI& si) const {
+        float t, u, v;
+        uint32_t h = bvh.trace(r, t, u, v);
+        if (h == ~0u) return false;
+        fillSI(bvh.tris[h], t, u, v, si);
+        si.tri = h;
+        return true;
+    }
+    V3 envLe(V3 d) const { return hasEnv ? dome.eval(d) * dome.intensity : envRad; }
+
+    // shadowTransmittance (integrator/ShadowRay.h:26-61) for opaque materials
+    V3 shadowTr(Ray ray) const {
+        V3 T{1.f, 1.f, 1.f};
+        for (int i = 0; i < 8; ++i) {
+            SI si;
+            if (!traceSI(ray, si)) break;
+            if (!matOf(si.meshID)) return {};
+            return {};   // transmittanceColor() is black for every supported material
+        }
+        return T;
+    }
+
+    static float power(float f, float g) { f = 1.f * f; g = 1.f * g; return (f * f) / (f * f + g * g); }  // PathIntegrator.h:55-59
+
+    // PathIntegrator::estimateDirect (PathIntegrator.cpp:166-265)
+    V3 estimateDirect(const SI& si, const Material& mat, V3 wo, const Light& light, Halton& smp) const {
+        V3 Ld;
+        Ctx ctx(si, -wo);
+        {
+            V2 u = smp.get2D();
+            LightSample ls = light.sample(si.p, u);
+            if (ls.pdf > 0.f && !isBlack(ls.Li)) {
+                BSDFEval be = mat.evaluate(ctx, wo, ls.wi);
+                if (!isBlack(be.f)) {
+                    Ray sr = spawnRayTo(si.p, si.n, si.p + ls.wi * ls.dist);
+                    V3 Tr = shadowTr(sr);
+                    if (!isBlack(Tr)) {
+                        float w = power(ls.pdf, be.pdf);
+                        float cosI = absDot(ls.wi, si.n);
+                        Ld = Ld + be.f * ls.Li * Tr * cosI * w / ls.pdf;
+                    }
+                }
+            }
+        }
+        {
+            // mat.sample(ctx, wo, get2D(), get1D()): g++ evaluates the arguments right to left.
+            float uc = smp.get1D();
+            V2 u = smp.get2D();
+            BSDFSample bs = mat.sample(ctx, wo, u, uc);
+            if (bs.valid()) {
+                float lp = light.pdf(si.p, bs.wi);
+                if (lp > 0.f) {
+                    float w = bs.delta ? 1.f : power(bs.pdf, lp);
+                    Ray sr = spawnRay(si.p, si.ng, bs.wi);
+                    sr.tMax = 1e10f;
+                    V3 Tr{1.f, 1.f, 1.f};
+                    SI hs;
+                    bool hit = false;
+                    for (int step = 0; step < 8; ++step) {
+                        hit = traceSI(sr, hs);
+                        if (!hit) break;
+                        if (!matOf(hs.meshID)) { Tr = {}; break; }
+                        break;   // opaque / emitter
+                    }
+                    V3 Li;
+                    if (!hit) Li = envLe(bs.wi);
+                    else if (const Material* hm = matOf(hs.meshID)) Li = hm->Le();
+                    if (!isBlack(Li) && !isBlack(Tr)) {
+                        float cosI = fixDoubleCos ? 1.f : absDot(bs.wi, si.n);
+                        Ld = Ld + bs.f * Li * Tr * cosI * w / bs.pdf;
+                    }
+                }
+            }
+        }
+        return Ld;
+    }
+
+    // PathIntegrator::Li (PathIntegrator.cpp:71-164)
+    V3 Li(Ray r, Halton& smp, uint32_t* rays = nullptr) const {
+        V3 L, beta{1.f, 1.f, 1.f};
+        bool specular = false;
+        for (uint32_t bounce = 0; bounce <= maxDepth; ++bounce) {
+            SI si;
+            if (rays) ++*rays;
+            if (!traceSI(r, si)) {
+                if (bounce == 0 || specular) L = L + beta * envLe(r.d);
+                break;
+            }
+            const Material* mat = matOf(si.meshID);
+            if (!mat) break;
+            V3 wo = -r.d;
+            Ctx ctx(si, r.d);
+            V3 Le = mat->Le();
+            if (!isBlack(Le) && (bounce == 0 || specular)) L = L + beta * Le;
+            if (!mat->isDelta() && !lights.empty()) {
+                uint32_t li = (uint32_t)(smp.get1D() * (float)lights.size());
+                li = std::min(li, (uint32_t)(lights.size() - 1));
+                V3 Ld = estimateDirect(si, *mat, wo, lights[li], smp);
+                L = L + beta * Ld * (float)lights.size();
+            }
+            float uc = smp.get1D();
+            V2 u = smp.get2D();
+            BSDFSample bs = mat->sample(ctx, wo, u, uc);
+            if (!bs.valid()) break;
+            specular = bs.delta;
+            beta = beta * (bs.f / bs.pdf);
+            r = spawnRay(si.p, si.n, bs.wi);
+            if (bounce >= minDepth) {
+                float q = 1.f - fmins(maxComponent(beta), 0.95f);
+                if (smp.get1D() < q) break;
+                beta = beta * (1.f / (1.f - q));
+            }
+        }
+        return L;
+    }
+
+    // PathIntegrator::renderTile (PathIntegrator.cpp:8-69): out[w*h*5] = r,g,b (pre-weighted), w, lumSq
+    void renderTile(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float* out) const {
+        Halton smp;
+        smp.seed = seed;
+        for (uint32_t ty = 0; ty < h; ++ty)
+            for (uint32_t tx = 0; tx < w; ++tx) {
+                uint32_t px = x0 + tx, py = y0 + ty;
+                V3 acc;
+                float lsq = 0.f;
+                for (uint32_t s = 0; s < ns; ++s) {
+                    smp.start(px, py, s0 + s);
+                    V2 jit = smp.get2D();
+                    smp.get2D();   // lens
+                    smp.get1D();   // time
+                    Ray ray = generateRay(cam, px, py, jit.x, jit.y);
+                    V3 L = Li(ray, smp);
+                    if (isFinite(L)) {
+                        acc = acc + L;
+                        float lum = luminance(L);
+                        lsq += lum * lum;
+                    }
+                }
+                float inv = 1.f / (float)ns, wt = (float)ns;
+                float* o = out + 5 * (ty * w + tx);
+                o[0] = acc.x * inv * wt;
+                o[1] = acc.y * inv * wt;
+                o[2] = acc.z * inv * wt;
+                o[3] = wt;
+                o[4] = lsq;
+            }
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+int orc_pcg_u32(uint64_t seed, uint64_t seq, uint32_t n, uint32_t* out) {
+    Pcg r(seed, seq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+    return 0;
+}
+
+float orc_radical_inverse(uint32_t base, uint64_t n) { return radicalInverse(base, n); }
+
+int orc_halton(uint64_t seed, uint32_t n, const uint32_t* px, const uint32_t* py, const uint32_t* idx,
+               const uint32_t* dim, float* out) {
+    Halton h;
+    h.seed = seed;
+    for (uint32_t i = 0; i < n; ++i) {
+        h.start(px[i], py[i], idx[i]);
+        if (dim[i] < 128) out[i] = h.scrambled(dim[i], idx[i]);
+        else {
+            float v = 0.f;
+            for (uint32_t k = 128; k <= dim[i]; ++k) v = h.rng.nextFloat();
+            out[i] = v;
+        }
+    }
+    return 0;
+}
+
+int orc_halton_seq(uint64_t seed, uint32_t px, uint32_t py, uint32_t idx, uint32_t nd, float* out) {
+    Halton h;
+    h.seed = seed;
+    h.start(px, py, idx);
+    for (uint32_t d = 0; d < nd; ++d) out[d] = h.get1D();
+    return 0;
+}
+
+int orc_make_pinhole(const float* f, const float* a, const float* u, float vfov, uint32_t w, uint32_t h,
+                     ahip_camera* out) {
+    Camera c = makePinhole(V3{f[0], f[1], f[2]}, V3{a[0], a[1], a[2]}, V3{u[0], u[1], u[2]}, vfov, w, h);
+    for (int i = 0; i < 3; ++i) {
+        out->origin[i] = c.origin[i]; out->lower_left[i] = c.llc[i];
+        out->horizontal[i] = c.hor[i]; out->vertical[i] = c.ver[i];
+    }
+    out->width = w; out->height = h;
+    return 0;
+}
+
+static Camera toCam(const ahip_camera* c) {
+    Camera k;
+    k.origin = {c->origin[0], c->origin[1], c->origin[2]};
+    k.llc = {c->lower_left[0], c->lower_left[1], c->lower_left[2]};
+    k.hor = {c->horizontal[0], c->horizontal[1], c->horizontal[2]};
+    k.ver = {c->vertical[0], c->vertical[1], c->vertical[2]};
+    k.w = c->width; k.h = c->height;
+    return k;
+}
+
+int orc_generate_rays(const ahip_camera* c, uint32_t n, const uint32_t* px, const uint32_t* py, const float* jx,
+                      const float* jy, float* rays) {
+    Camera k = toCam(c);
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r = generateRay(k, px[i], py[i], jx[i], jy[i]);
+        for (int j = 0; j < 3; ++j) { rays[6 * i + j] = r.o[j]; rays[6 * i + 3 + j] = r.d[j]; }
+    }
+    return 0;
+}
+
+void* orc_scene_create(const ahip_scene_desc* d) {
+    auto s = std::make_unique<Scene>();
+    s->bvh.commit(d);
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        Texture t;
+        t.w = d->texture_width[i]; t.h = d->texture_height[i];
+        t.px.assign(d->texture_pixels[i], d->texture_pixels[i] + (size_t)t.w * t.h);
+        s->textures.push_back(std::move(t));
+    }
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        Material m;
+        m.m = d->materials[i];
+        if (m.m.type == AHIP_MAT_STANDARD && m.m.transmission > 0.001f) return nullptr;   // opaque only
+        if (m.m.roughness_texture >= 0) {
+            if ((uint32_t)m.m.roughness_texture >= d->num_textures) return nullptr;
+            m.roughTex = &s->textures[m.m.roughness_texture];
+        }
+        m.init();
+        s->mats.push_back(m);
+    }
+    s->meshMat.assign(d->mesh_material, d->mesh_material + d->num_meshes);
+    if (d->env_light >= 0) {
+        const ahip_light& l = d->lights[d->env_light];
+        s->dome.W = d->env_width; s->dome.H = d->env_height;
+        s->dome.px.assign(d->env_pixels, d->env_pixels + (size_t)d->env_width * d->env_height * 3);
+        s->dome.intensity = l.intensity;
+        for (int r = 0; r < 3; ++r) s->dome.rot[r] = V3{l.rot[3 * r], l.rot[3 * r + 1], l.rot[3 * r + 2]};
+        s->dome.build();
+        s->hasEnv = true;
+    }
+    s->envRad = V3{d->env_radiance[0], d->env_radiance[1], d->env_radiance[2]};
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const ahip_light& l = d->lights[i];
+        Light L;
+        L.type = l.type;
+        if (l.type == AHIP_LIGHT_AREA) {
+            L.center = {l.center[0], l.center[1], l.center[2]};
+            L.uA = {l.u_axis[0], l.u_axis[1], l.u_axis[2]};
+            L.vA = {l.v_axis[0], l.v_axis[1], l.v_axis[2]};
+            L.Le = {l.Le[0], l.Le[1], l.Le[2]};
+            L.init();
+        } else {
+            if ((int32_t)i != d->env_light) return nullptr;
+            L.dome = &s->dome;
+        }
+        s->lights.push_back(L);
+    }
+    return s.release();
+}
+
+void orc_scene_destroy(void* h) { delete static_cast<Scene*>(h); }
+
+int orc_set_render(void* h, const ahip_camera* cam, uint32_t maxDepth, uint32_t minDepth, uint64_t seed,
+                   uint32_t flags) {
+    auto* s = static_cast<Scene*>(h);
+    if (cam) s->cam = toCam(cam);
+    s->maxDepth = maxDepth; s->minDepth = minDepth; s->seed = seed;
+    s->fixDoubleCos = (flags & AHIP_FLAG_FIX_DOUBLE_COSINE) != 0;
+    return 0;
+}
+
+int orc_bvh_sizes(void* h, uint32_t* nn, uint32_t* np) {
+    auto* s = static_cast<Scene*>(h);
+    *nn = (uint32_t)s->bvh.nodes.size();
+    *np = (uint32_t)s->bvh.prims.size();
+    return 0;
+}
+int orc_bvh_dump(void* h, uint32_t* nodes32, uint32_t* prims) {
+    auto* s = static_cast<Scene*>(h);
+    std::memcpy(nodes32, s->bvh.nodes.data(), s->bvh.nodes.size() * sizeof(Node));
+    std::memcpy(prims, s->bvh.prims.data(), s->bvh.prims.size() * 4);
+    return 0;
+}
+
+// rays[n*8]; hits as ahip_hit; optional per-ray node / triangle visit counts.
+int orc_trace(void* h, const float* rays, uint32_t n, ahip_hit* hits, uint32_t* nodeCount, uint32_t* triCount) {
+    auto* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = rays + 8 * i;
+        Ray r;
+        r.o = {q[0], q[1], q[2]}; r.d = {q[3], q[4], q[5]}; r.tMin = q[6]; r.tMax = q[7];
+        float t = 0.f, u = 0.f, v = 0.f;
+        uint64_t nv = 0, tv = 0;
+        uint32_t k = s->bvh.trace(r, t, u, v, &nv, &tv);
+        if (nodeCount) nodeCount[i] = (uint32_t)nv;
+        if (triCount) triCount[i] = (uint32_t)tv;
+        ahip_hit& o = hits[i];
+        if (k == ~0u) { o = {0.f, 0.f, 0.f, ~0u, ~0u, ~0u}; continue; }
+        o = {t, u, v, s->bvh.tris[k].meshID, s->bvh.tris[k].primID, k};
+    }
+    return 0;
+}
+
+int orc_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
+    auto* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = rays + 8 * i;
+        Ray r;
+        r.o = {q[0], q[1], q[2]}; r.d = {q[3], q[4], q[5]}; r.tMin = q[6]; r.tMax = q[7];
+        out[i] = s->bvh.occluded(r) ? 1 : 0;
+    }
+    return 0;
+}
+
+int orc_render_tile(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns, float* out) {
+    static_cast<Scene*>(h)->renderTile(x0, y0, w, hh, s0, ns, out);
+    return 0;
+}
+
+// Per-sample radiance for a list of (px, py, sampleIndex): out[n*3]; non-finite kept as is.
+int orc_render_samples(void* h, uint32_t n, const uint32_t* px, const uint32_t* py, const uint32_t* idx, float* out,
+                       uint32_t* raysOut) {
+    auto* s = static_cast<Scene*>(h);
+    Halton smp;
+    smp.seed = s->seed;
+    for (uint32_t i = 0; i < n; ++i) {
+        smp.start(px[i], py[i], idx[i]);
+        V2 jit = smp.get2D();
+        smp.get2D();
+        smp.get1D();
+        Ray ray = generateRay(s->cam, px[i], py[i], jit.x, jit.y);
+        uint32_t rc = 0;
+        V3 L = s->Li(ray, smp, &rc);
+        out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+        if (raysOut) raysOut[i] = rc;
+    }
+    return 0;
+}
+
+// Multi-threaded whole-frame render over 64x64 tiles (the RenderSession tile loop,
+// RenderSession.cpp:511-527) for rows [row0, row1): out[W*H*5]. *seconds = wall time.
+int orc_render_frame(void* h, uint32_t row0, uint32_t row1, uint32_t spp, uint32_t nthreads, float* out,
+                     double* seconds) {
+    auto* s = static_cast<Scene*>(h);
+    const uint32_t W = s->cam.w, H = s->cam.h, ts = 64;
+    row1 = std::min(row1, H);
+    std::vector<uint32_t> tx, ty;
+    for (uint32_t y = row0; y < row1; y += ts)
+        for (uint32_t x = 0; x < W; x += ts) { tx.push_back(x); ty.push_back(y); }
+    if (nthreads == 0) nthreads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&] {
+        std::vector<float> buf(ts * ts * 5);
+        for (;;) {
+            uint32_t i = next.fetch_add(1);
+            if (i >= tx.size()) break;
+            uint32_t w = std::min(ts, W - tx[i]), hh = std::min(ts, row1 - ty[i]);
+            s->renderTile(tx[i], ty[i], w, hh, 0, spp, buf.data());
+            for (uint32_t yy = 0; yy < hh; ++yy)
+                std::memcpy(out + 5 * ((size_t)(ty[i] + yy) * W + tx[i]), buf.data() + 5 * yy * w, sizeof(float) * 5 * w);
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t i = 0; i < nthreads; ++i) th.emplace_back(work);
+    for (auto& t : th) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+// Material sample / evaluate probes on a flat +Z frame for the invariant tests.
+// in[n*6]: wo (world), u.x, u.y, uc; out[n*9]: wi, f, pdf, valid, evalPdf.
+int orc_material_probe(const ahip_material* mat, uint32_t n, const float* in, float* out) {
+    Material m;
+    m.m = *mat;
+    m.init();
+    SI si;
+    si.n = {0, 0, 1}; si.ng = {0, 0, 1};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 6 * i;
+        V3 wo{q[0], q[1], q[2]};
+        Ctx c(si, -wo);
+        BSDFSample bs = m.sample(c, wo, V2{q[3], q[4]}, q[5]);
+        BSDFEval be = m.evaluate(c, wo, bs.wi);
+        float* o = out + 9 * i;
+        o[0] = bs.wi.x; o[1] = bs.wi.y; o[2] = bs.wi.z;
+        o[3] = bs.f.x; o[4] = bs.f.y; o[5] = bs.f.z;
+        o[6] = bs.pdf; o[7] = bs.valid() ? 1.f : 0.f; o[8] = be.pdf;
+    }
+    return 0;
+}
+
+// evaluate(wo, wi) on a flat +Z frame: in[n*6] = wo, wi; out[n*4] = f, pdf.
+int orc_material_eval(const ahip_material* mat, uint32_t n, const float* in, float* out) {
+    Material m;
+    m.m = *mat;
+    m.init();
+    SI si;
+    si.n = {0, 0, 1}; si.ng = {0, 0, 1};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 6 * i;
+        V3 wo{q[0], q[1], q[2]}, wi{q[3], q[4], q[5]};
+        Ctx c(si, -wo);
+        BSDFEval be = m.evaluate(c, wo, wi);
+        out[4 * i] = be.f.x; out[4 * i + 1] = be.f.y; out[4 * i + 2] = be.f.z; out[4 * i + 3] = be.pdf;
+    }
+    return 0;
+}
+
+int orc_ggx_helper(int which, uint32_t n, const float* a, const float* b, const float* c, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = D_GGX(a[i], b[i]); break;
+            case 1: out[i] = G1(a[i], b[i]); break;
+            case 2: out[i] = schlickD(a[i], b[i]); break;
+            case 3: out[i] = fresnelDielectric(a[i], b[i]); break;
+            case 4: out[i] = pdfGGXr(a[i], b[i], c[i]); break;
+            case 5: out[i] = F0ior(a[i]); break;
+            case 6: {
+                V3 w = sampleGGXh(V2{a[i], b[i]}, c[i]);
+                out[3 * i] = w.x; out[3 * i + 1] = w.y; out[3 * i + 2] = w.z;
+                break;
+            }
+            default: return 1;
+        }
+    }
+    return 0;
+}
+
+int orc_distribution1d(const float* values, uint32_t count, uint32_t ns, const float* us, uint32_t* idx, float* pdf,
+                       float* ur, float* cdfOut, float* integral) {
+    Dist1D d;
+    d.build(values, count);
+    for (uint32_t k = 0; k < ns; ++k) idx[k] = d.sample(us[k], pdf[k], ur[k]);
+    if (cdfOut) std::memcpy(cdfOut, d.cdf.data(), sizeof(float) * (count + 1));
+    if (integral) *integral = d.integral;
+    return 0;
+}
+
+// Dome probes: sample directions for u[n*2] -> out[n*5] = dir, pdf, evalPdf(dir);
+// and evaluate radiance for dirs[n*3] -> rgb[n*3].
+int orc_dome_sample(void* h, uint32_t n, const float* u, float* out) {
+    auto* s = static_cast<Scene*>(h);
+    if (!s->hasEnv) return 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        float pdf;
+        V3 d = s->dome.sampleDir(V2{u[2 * i], u[2 * i + 1]}, pdf);
+        out[5 * i] = d.x; out[5 * i + 1] = d.y; out[5 * i + 2] = d.z;
+        out[5 * i + 3] = pdf; out[5 * i + 4] = s->dome.evalPdf(d);
+    }
+    return 0;
+}
+int orc_dome_eval(void* h, uint32_t n, const float* dirs, float* rgb, float* pdf) {
+    auto* s = static_cast<Scene*>(h);
+    if (!s->hasEnv) return 1;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 d{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+        V3 c = s->dome.eval(d) * s->dome.intensity;
+        rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
+        if (pdf) pdf[i] = s->dome.evalPdf(d);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+// ref_driver.cpp — test-infrastructure driver around the reference's own CPU sources.
+//
+// ORACLE / TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/ together
+// with the unmodified reference translation units src/accel/BVHBackend.cpp and
+// src/integrator/PathIntegrator.cpp, compiled where they lie under the reference checkout
+// with the reference's Release flags (cmake/compiler_options.cmake:10-51).  Only tests/,
+// the smoke check and bench.py's cpu_baseline leg load it.
+//
+// What it can drive: PCGRng, HaltonSampler, Camera::makePinhole, BVHBackend (commit/trace),
+// PathIntegrator::renderTile with LambertianMaterial / EmissiveMaterial / AreaLight and a
+// constant background, and the RenderSession non-adaptive tile loop on the reference
+// ThreadPool (RenderSession.cpp:511-547).  StandardSurface (needs evalTOV from the
+// OIIO-only Texture.cpp), DomeLight (loadImage lives in the OIIO-only DomeLight.cpp) and
+// CurveBrute (needs spdlog) are not linkable here without stand-ins, so they are not built.
+//
+// The reference BVH keeps its nodes private; to dump the built tree for topology checks
+// this one translation unit reads them through an access macro applied only to the
+// reference headers it includes (no reference code is changed or replaced).
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <optional>
+#include <queue>
+#include <string>
+#include <thread>
+#include <vector>
+
+#define private public
+#include "accel/BVHBackend.h"
+#undef private
+#include "integrator/PathIntegrator.h"
+#include "sampling/HaltonSampler.h"
+#include "sampling/PCGRng.h"
+#include "shading/Lambertian.h"
+#include "shading/lights/AreaLight.h"
+#include "shading/lights/DomeLight.h"   // Distribution1D only (header-inline)
+#include "shading/StandardSurface.h"    // free GGX helpers only (header-inline)
+#include "render/ThreadPool.h"
+#include <anacapa/film/Film.h>
+
+#include "../include/anacapa_hip.h"
+
+using namespace anacapa;
+
+namespace {
+
+struct RefScene {
+    GeometryPool pool;
+    std::unique_ptr<BVHBackend> accel;
+    std::vector<std::unique_ptr<IMaterial>> mats;
+    std::vector<std::unique_ptr<ILight>> lights;
+    SceneView view;
+};
+
+Vec3f v3(const float* p) { return Vec3f{p[0], p[1], p[2]}; }
+
+Camera toCamera(const ahip_camera* c) {
+    Camera cam;
+    cam.origin = v3(c->origin);
+    cam.lowerLeftCorner = v3(c->lower_left);
+    cam.horizontal = v3(c->horizontal);
+    cam.vertical = v3(c->vertical);
+    cam.basisU = Vec3f{1, 0, 0};
+    cam.basisV = Vec3f{0, 1, 0};
+    cam.imageWidth = c->width;
+    cam.imageHeight = c->height;
+    return cam;
+}
+
+void writeTile(const TileBuffer& tb, float* out) {
+    for (uint32_t i = 0; i < tb.width * tb.height; ++i) {
+        out[i * 5 + 0] = tb.pixels[i].r;
+        out[i * 5 + 1] = tb.pixels[i].g;
+        out[i * 5 + 2] = tb.pixels[i].b;
+        out[i * 5 + 3] = tb.pixels[i].weight;
+        out[i * 5 + 4] = tb.sumLumSq[i];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_pcg_u32(uint64_t seed, uint64_t seq, uint32_t n, uint32_t* out) {
+    PCGRng rng(seed, seq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = rng.nextUint32();
+    return 0;
+}
+
+int ref_pcg_float(uint64_t seed, uint32_t n, float* out) {
+    PCGRng rng(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = rng.nextFloat();
+    return 0;
+}
+
+// First ndims values of get1D() after startPixelSample(px, py, idx).
+int ref_halton_seq(uint64_t seed, uint32_t px, uint32_t py, uint32_t idx, uint32_t ndims, float* out) {
+    HaltonSampler s(1, seed);
+    s.startPixelSample(px, py, idx);
+    for (uint32_t d = 0; d < ndims; ++d) out[d] = s.get1D();
+    return 0;
+}
+
+// Same stream consumed through get2D() pairs (checks get2D = {get1D(), get1D()} order).
+int ref_halton_seq2d(uint64_t seed, uint32_t px, uint32_t py, uint32_t idx, uint32_t npairs, float* out) {
+    HaltonSampler s(1, seed);
+    s.startPixelSample(px, py, idx);
+    for (uint32_t d = 0; d < npairs; ++d) {
+        Vec2f v = s.get2D();
+        out[2 * d] = v.x;
+        out[2 * d + 1] = v.y;
+    }
+    return 0;
+}
+
+int ref_make_pinhole(const float* from, const float* at, const float* up, float vfov, uint32_t w,
+                     uint32_t h, ahip_camera* out) {
+    Camera c = Camera::makePinhole(v3(from), v3(at), v3(up), vfov, w, h);
+    for (int i = 0; i < 3; ++i) {
+        out->origin[i] = c.origin[i];
+        out->lower_left[i] = c.lowerLeftCorner[i];
+        out->horizontal[i] = c.horizontal[i];
+        out->vertical[i] = c.vertical[i];
+    }
+    out->width = c.imageWidth;
+    out->height = c.imageHeight;
+    return 0;
+}
+
+// Camera::generateRay for a pinhole camera: rays[n*6] = origin, direction.
+int ref_generate_rays(const ahip_camera* c, uint32_t n, const uint32_t* px, const uint32_t* py,
+                      const float* jx, const float* jy, float* rays) {
+    Camera cam = toCamera(c);
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r = cam.generateRay(px[i], py[i], jx[i], jy[i], 0.5f, 0.5f, 0.f);
+        for (int k = 0; k < 3; ++k) {
+            rays[i * 6 + k] = r.origin[k];
+            rays[i * 6 + 3 + k] = r.direction[k];
+        }
+    }
+    return 0;
+}
+
+void* ref_scene_create(const ahip_scene_desc* d) {
+    auto s = std::make_unique<RefScene>();
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        MeshDesc mesh;
+        uint32_t v0 = d->mesh_vertex_offset[m], v1 = d->mesh_vertex_offset[m + 1];
+        uint32_t i0 = d->mesh_index_offset[m], i1 = d->mesh_index_offset[m + 1];
+        for (uint32_t v = v0; v < v1; ++v) mesh.positions.push_back(v3(d->positions + 3 * v));
+        uint32_t nn = d->mesh_normal_count ? d->mesh_normal_count[m] : 0;
+        for (uint32_t v = 0; v < nn; ++v) mesh.normals.push_back(v3(d->normals + 3 * (v0 + v)));
+        uint32_t nu = d->mesh_uv_count ? d->mesh_uv_count[m] : 0;
+        for (uint32_t v = 0; v < nu; ++v)
+            mesh.uvs.push_back(Vec2f{d->uvs[2 * (v0 + v)], d->uvs[2 * (v0 + v) + 1]});
+        for (uint32_t i = i0; i < i1; ++i) mesh.indices.push_back(d->indices[i]);
+        s->pool.addMesh(std::move(mesh));
+    }
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const ahip_material& m = d->materials[i];
+        if (m.type == AHIP_MAT_LAMBERT)
+            s->mats.push_back(std::make_unique<LambertianMaterial>(v3(m.albedo)));
+        else if (m.type == AHIP_MAT_EMISSIVE)
+            s->mats.push_back(std::make_unique<EmissiveMaterial>(v3(m.albedo), v3(m.emission)));
+        else
+            return nullptr;  // StandardSurface is not linkable in the reference build
+    }
+    for (uint32_t m = 0; m < d->num_meshes; ++m) {
+        int32_t mi = d->mesh_material[m];
+        s->view.materials.push_back(mi >= 0 ? s->mats[mi].get() : nullptr);
+    }
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const ahip_light& l = d->lights[i];
+        if (l.type != AHIP_LIGHT_AREA) return nullptr;
+        s->lights.push_back(std::make_unique<AreaLight>(v3(l.center), v3(l.u_axis), v3(l.v_axis), v3(l.Le)));
+        s->view.lights.push_back(s->lights.back().get());
+    }
+    if (d->env_light >= 0) return nullptr;
+    s->view.envRadiance = v3(d->env_radiance);
+    s->accel = std::make_unique<BVHBackend>(s->pool);
+    s->accel->commit();
+    s->view.accel = s->accel.get();
+    return s.release();
+}
+
+void ref_scene_destroy(void* h) { delete static_cast<RefScene*>(h); }
+
+int ref_bvh_sizes(void* h, uint32_t* nnodes, uint32_t* nprims) {
+    auto* s = static_cast<RefScene*>(h);
+    *nnodes = (uint32_t)s->accel->m_nodes.size();
+    *nprims = (uint32_t)s->accel->m_primIndices.size();
+    return 0;
+}
+
+// nodes32: 8 words per node = bmin[3], dataA, bmax[3], dataB (bit patterns).
+int ref_bvh_dump(void* h, uint32_t* nodes32, uint32_t* prims) {
+    auto* s = static_cast<RefScene*>(h);
+    const auto& nodes = s->accel->m_nodes;
+    std::memcpy(nodes32, nodes.data(), nodes.size() * sizeof(BVHNode));
+    std::memcpy(prims, s->accel->m_primIndices.data(), s->accel->m_primIndices.size() * 4);
+    return 0;
+}
+
+// rays[n*8] = o, d, tMin, tMax. out[n*12] floats: t, meshID(bits), primID(bits), p[3], n[3], ng[3];
+// meshID bits = ~0u on a miss.
+int ref_trace(void* h, const float* rays, uint32_t n, float* out) {
+    auto* s = static_cast<RefScene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        Ray ray(v3(r), v3(r + 3), r[6], r[7]);
+        TraceResult tr = s->accel->trace(ray);
+        float* o = out + 12 * i;
+        uint32_t mesh = tr.hit ? tr.si.meshID : ~0u, prim = tr.hit ? tr.si.primID : ~0u;
+        o[0] = tr.hit ? tr.si.t : 0.f;
+        std::memcpy(o + 1, &mesh, 4);
+        std::memcpy(o + 2, &prim, 4);
+        for (int k = 0; k < 3; ++k) {
+            o[3 + k] = tr.si.p[k];
+            o[6 + k] = tr.si.n[k];
+            o[9 + k] = tr.si.ng[k];
+        }
+    }
+    return 0;
+}
+
+int ref_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
+    auto* s = static_cast<RefScene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        out[i] = s->accel->occluded(Ray(v3(r), v3(r + 3), r[6], r[7])) ? 1 : 0;
+    }
+    return 0;
+}
+
+// PathIntegrator::renderTile on one tile; out[w*h*5] = r,g,b (pre-weighted sums), weight, sumLumSq.
+int ref_render_tile(void* h, const ahip_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh,
+                    uint32_t sampleStart, uint32_t sampleCount, uint32_t maxDepth, uint64_t seed,
+                    float* out) {
+    auto* s = static_cast<RefScene*>(h);
+    s->view.camera = toCamera(cam);
+    PathIntegrator integ(maxDepth);
+    integ.prepare(s->view);
+    HaltonSampler base(sampleCount, seed);
+    auto sampler = base.clone();
+    TileBuffer tb(x0, y0, w, hh);
+    TileRequest tile{x0, y0, w, hh, sampleStart, sampleCount};
+    integ.renderTile(s->view, tile, cam->width, cam->height, *sampler, tb);
+    writeTile(tb, out);
+    return 0;
+}
+
+// RenderSession::render non-adaptive CPU branch (RenderSession.cpp:465-547): 64x64 tiles
+// (scheduleTiles, RenderSession.cpp:380-397), ThreadPool(nthreads) (0 = hardware
+// concurrency), HaltonSampler(spp, seed) cloned per tile, PathIntegrator(maxDepth) ->
+// renderTile -> TileBuffer, merged here into out[W*H*5].  *seconds = steady_clock around
+// the parallel tile loop only (RenderSession.cpp:479, 665-667).
+int ref_render_frame(void* h, const ahip_camera* cam, uint32_t spp, uint32_t maxDepth, uint64_t seed,
+                     uint32_t nthreads, uint32_t tileSize, float* out, double* seconds,
+                     uint32_t* threadsUsed) {
+    auto* s = static_cast<RefScene*>(h);
+    s->view.camera = toCamera(cam);
+    const uint32_t W = cam->width, H = cam->height;
+    PathIntegrator integ(maxDepth);
+    integ.prepare(s->view);
+    HaltonSampler base(spp, seed);
+    ThreadPool pool(nthreads);
+    if (threadsUsed) *threadsUsed = pool.numThreads();
+    std::vector<TileRequest> tiles;
+    for (uint32_t y = 0; y < H; y += tileSize)
+        for (uint32_t x = 0; x < W; x += tileSize)
+            tiles.push_back({x, y, std::min(tileSize, W - x), std::min(tileSize, H - y), 0, spp});
+    std::memset(out, 0, sizeof(float) * W * H * 5);
+    std::mutex mergeMutex;
+    auto t0 = std::chrono::steady_clock::now();
+    pool.parallelFor((uint32_t)tiles.size(), [&](uint32_t ti) {
+        const TileRequest& t = tiles[ti];
+        auto sampler = base.clone();
+        TileBuffer tb(t.x0, t.y0, t.width, t.height);
+        integ.renderTile(s->view, t, W, H, *sampler, tb);
+        std::lock_guard<std::mutex> lk(mergeMutex);
+        for (uint32_t ty = 0; ty < t.height; ++ty)
+            for (uint32_t tx = 0; tx < t.width; ++tx) {
+                uint32_t fi = (t.y0 + ty) * W + (t.x0 + tx), li = ty * t.width + tx;
+                out[fi * 5 + 0] += tb.pixels[li].r;
+                out[fi * 5 + 1] += tb.pixels[li].g;
+                out[fi * 5 + 2] += tb.pixels[li].b;
+                out[fi * 5 + 3] += tb.pixels[li].weight;
+                out[fi * 5 + 4] += tb.sumLumSq[li];
+            }
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+// Distribution1D (DomeLight.h:19-55): build over values, then sample at us[k].
+int ref_distribution1d(const float* values, uint32_t count, uint32_t nsamples, const float* us,
+                       uint32_t* idx, float* pdf, float* uRemapped, float* cdfOut, float* integral) {
+    Distribution1D d;
+    d.build(values, count);
+    for (uint32_t k = 0; k < nsamples; ++k) idx[k] = d.sample(us[k], pdf[k], uRemapped[k]);
+    if (cdfOut) std::memcpy(cdfOut, d.cdf.data(), sizeof(float) * (count + 1));
+    if (integral) *integral = d.integral;
+    return 0;
+}
+
+// Free GGX / Fresnel helpers of StandardSurface.h:20-92, evaluated elementwise.
+// which: 0 D_GGX(a, b), 1 G1_Smith(a, b), 2 schlickDielectric(a, b), 3 fresnelDielectric(a, b),
+//        4 pdfGGX_reflection(a, b, c), 5 F0_fromIOR(a); 6 sampleGGX_halfvector((a,b), c) -> 3 outs.
+int ref_ggx_helper(int which, uint32_t n, const float* a, const float* b, const float* c, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = D_GGX(a[i], b[i]); break;
+            case 1: out[i] = G1_Smith(a[i], b[i]); break;
+            case 2: out[i] = schlickDielectric(a[i], b[i]); break;
+            case 3: out[i] = fresnelDielectric(a[i], b[i]); break;
+            case 4: out[i] = pdfGGX_reflection(a[i], b[i], c[i]); break;
+            case 5: out[i] = F0_fromIOR(a[i]); break;
+            case 6: {
+                Vec3f w = sampleGGX_halfvector(Vec2f{a[i], b[i]}, c[i]);
+                out[3 * i] = w.x; out[3 * i + 1] = w.y; out[3 * i + 2] = w.z;
+                break;
+            }
+            default: return 1;
+        }
+    }
+    return 0;
+}
+
+// Lambertian sample / evaluate through the reference material and ShadingContext.
+// in[n*9]: normal n, wo, (u.x, u.y, unused); out[n*8]: wi, f(rgb), pdf, valid, evalPdf.
+int ref_lambert(const float* albedo, uint32_t n, const float* in, float* out) {
+    LambertianMaterial mat(v3(albedo));
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 9 * i;
+        SurfaceInteraction si;
+        si.n = v3(q);
+        si.ng = si.n;
+        Vec3f wo = v3(q + 3);
+        ShadingContext ctx(si, -wo);
+        BSDFSample bs = mat.sample(ctx, wo, Vec2f{q[6], q[7]}, 0.f);
+        BSDFEval be = mat.evaluate(ctx, wo, bs.wi);
+        float* o = out + 8 * i;
+        o[0] = bs.wi.x; o[1] = bs.wi.y; o[2] = bs.wi.z;
+        o[3] = bs.f.x; o[4] = bs.f.y; o[5] = bs.pdf;
+        o[6] = bs.isValid() ? 1.f : 0.f;
+        o[7] = be.pdf;
+    }
+    return 0;
+}
+
+}  // extern "C"

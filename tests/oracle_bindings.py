@@ -1,0 +1,334 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the reference build
+(oracle/_ref/libanacapa_ref.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from anacapa_amd import _abi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libanacapa_ref.so")
+
+VP = C.c_void_p
+P = A.ptr
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path)
+
+
+class Oracle:
+    """This repository's C++ restatement of the reference CPU path (oracle/oracle.cpp)."""
+
+    def __init__(self):
+        self.lib = _load(ORACLE_SO)
+        if self.lib is None:
+            raise RuntimeError("oracle/liboracle.so not built (make -C oracle)")
+        L = self.lib
+        L.orc_scene_create.restype = VP
+        L.orc_scene_create.argtypes = [C.POINTER(A.SceneDesc)]
+        L.orc_scene_destroy.argtypes = [VP]
+        L.orc_radical_inverse.restype = C.c_float
+        L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint64]
+        for n in ("orc_pcg_u32",):
+            getattr(L, n).argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, VP]
+        L.orc_halton.argtypes = [C.c_uint64, C.c_uint32, VP, VP, VP, VP, VP]
+        L.orc_halton_seq.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP]
+        L.orc_make_pinhole.argtypes = [VP, VP, VP, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(A.Camera)]
+        L.orc_generate_rays.argtypes = [C.POINTER(A.Camera), C.c_uint32, VP, VP, VP, VP, VP]
+        L.orc_set_render.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32]
+        L.orc_bvh_sizes.argtypes = [VP, VP, VP]
+        L.orc_bvh_dump.argtypes = [VP, VP, VP]
+        L.orc_trace.argtypes = [VP, VP, C.c_uint32, VP, VP, VP]
+        L.orc_occluded.argtypes = [VP, VP, C.c_uint32, VP]
+        L.orc_render_tile.argtypes = [VP] + [C.c_uint32] * 6 + [VP]
+        L.orc_render_samples.argtypes = [VP, C.c_uint32, VP, VP, VP, VP, VP]
+        L.orc_render_frame.argtypes = [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP]
+        L.orc_material_probe.argtypes = [C.POINTER(A.Material), C.c_uint32, VP, VP]
+        L.orc_material_eval.argtypes = [C.POINTER(A.Material), C.c_uint32, VP, VP]
+        L.orc_ggx_helper.argtypes = [C.c_int, C.c_uint32, VP, VP, VP, VP]
+        L.orc_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
+        L.orc_dome_sample.argtypes = [VP, C.c_uint32, VP, VP]
+        L.orc_dome_eval.argtypes = [VP, C.c_uint32, VP, VP, VP]
+
+    # ---- stateless
+    def pcg(self, seed, seq, n):
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_pcg_u32(seed, seq, n, P(out))
+        return out
+
+    def halton(self, seed, px, py, idx, dim):
+        px, py, idx, dim = (np.ascontiguousarray(a, np.uint32) for a in (px, py, idx, dim))
+        out = np.zeros(len(px), np.float32)
+        self.lib.orc_halton(seed, len(px), P(px), P(py), P(idx), P(dim), P(out))
+        return out
+
+    def halton_seq(self, seed, px, py, idx, nd):
+        out = np.zeros(nd, np.float32)
+        self.lib.orc_halton_seq(seed, px, py, idx, nd, P(out))
+        return out
+
+    def pinhole(self, frm, at, up, vfov, w, h):
+        cam = A.Camera()
+        f, a, u = (np.array(v, np.float32) for v in (frm, at, up))
+        self.lib.orc_make_pinhole(P(f), P(a), P(u), vfov, w, h, C.byref(cam))
+        return cam
+
+    def generate_rays(self, cam, px, py, jx, jy):
+        px, py = (np.ascontiguousarray(a, np.uint32) for a in (px, py))
+        jx, jy = (np.ascontiguousarray(a, np.float32) for a in (jx, jy))
+        out = np.zeros((len(px), 6), np.float32)
+        self.lib.orc_generate_rays(C.byref(cam), len(px), P(px), P(py), P(jx), P(jy), P(out))
+        return out
+
+    def ggx_helper(self, which, a, b=None, c=None):
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(a if b is None else b, np.float32)
+        c = np.ascontiguousarray(a if c is None else c, np.float32)
+        out = np.zeros(len(a) * (3 if which == 6 else 1), np.float32)
+        self.lib.orc_ggx_helper(which, len(a), P(a), P(b), P(c), P(out))
+        return out
+
+    def distribution1d(self, values, us):
+        values = np.ascontiguousarray(values, np.float32)
+        us = np.ascontiguousarray(us, np.float32)
+        n = len(us)
+        idx, pdf, ur = np.zeros(n, np.uint32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        cdf, integ = np.zeros(len(values) + 1, np.float32), np.zeros(1, np.float32)
+        self.lib.orc_distribution1d(P(values), len(values), n, P(us), P(idx), P(pdf), P(ur), P(cdf), P(integ))
+        return idx, pdf, ur, cdf, float(integ[0])
+
+    def material_probe(self, mat, wo, u):
+        """wo [n,3], u [n,3] (u.x, u.y, uComponent) -> dict of wi, f, pdf, valid, eval_pdf."""
+        q = np.ascontiguousarray(np.concatenate([wo, u], 1), np.float32)
+        out = np.zeros((len(q), 9), np.float32)
+        self.lib.orc_material_probe(C.byref(mat), len(q), P(q), P(out))
+        return dict(wi=out[:, :3], f=out[:, 3:6], pdf=out[:, 6], valid=out[:, 7] > 0, eval_pdf=out[:, 8])
+
+    def material_eval(self, mat, wo, wi):
+        q = np.ascontiguousarray(np.concatenate([wo, wi], 1), np.float32)
+        out = np.zeros((len(q), 4), np.float32)
+        self.lib.orc_material_eval(C.byref(mat), len(q), P(q), P(out))
+        return out[:, :3], out[:, 3]
+
+    # ---- scenes
+    def scene(self, sc):
+        return OracleScene(self, sc)
+
+
+class OracleScene:
+    def __init__(self, orc: Oracle, sc):
+        self.orc, self.sc = orc, sc
+        self.desc, self._keep = sc.desc()
+        self.h = orc.lib.orc_scene_create(C.byref(self.desc))
+        if not self.h:
+            raise RuntimeError("oracle rejected the scene")
+        self.set_render()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.orc.lib.orc_scene_destroy(self.h)
+            self.h = None
+
+    def set_render(self, cam=None, max_depth=None, min_depth=2, seed=None, flags=0):
+        sc = self.sc
+        if cam is None:
+            cam = self.orc.pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height)
+        self.cam = cam
+        self.orc.lib.orc_set_render(self.h, C.byref(cam), sc.max_depth if max_depth is None else max_depth,
+                                    min_depth, sc.seed if seed is None else seed, flags)
+
+    def bvh(self):
+        nn, npr = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        self.orc.lib.orc_bvh_sizes(self.h, P(nn), P(npr))
+        nodes = np.zeros((int(nn[0]), 8), np.uint32)
+        prims = np.zeros(int(npr[0]), np.uint32)
+        self.orc.lib.orc_bvh_dump(self.h, P(nodes), P(prims))
+        return nodes, prims
+
+    def trace(self, rays, counts=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros(len(rays), A.HIT_DTYPE)
+        nc = np.zeros(len(rays), np.uint32) if counts else None
+        tc = np.zeros(len(rays), np.uint32) if counts else None
+        self.orc.lib.orc_trace(self.h, P(rays), len(rays), P(hits), P(nc), P(tc))
+        return (hits, nc, tc) if counts else hits
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(len(rays), np.uint8)
+        self.orc.lib.orc_occluded(self.h, P(rays), len(rays), P(out))
+        return out
+
+    def render_tile(self, x0, y0, w, h, s0, ns):
+        out = np.zeros((h, w, 5), np.float32)
+        self.orc.lib.orc_render_tile(self.h, x0, y0, w, h, s0, ns, P(out))
+        return out
+
+    def render_samples(self, px, py, idx):
+        px, py, idx = (np.ascontiguousarray(a, np.uint32) for a in (px, py, idx))
+        out = np.zeros((len(px), 3), np.float32)
+        rc = np.zeros(len(px), np.uint32)
+        self.orc.lib.orc_render_samples(self.h, len(px), P(px), P(py), P(idx), P(out), P(rc))
+        return out, rc
+
+    def render_frame(self, spp, row0=0, row1=None, threads=0):
+        W, H = self.cam.width, self.cam.height
+        out = np.zeros((H, W, 5), np.float32)
+        sec = np.zeros(1, np.float64)
+        self.orc.lib.orc_render_frame(self.h, row0, H if row1 is None else row1, spp, threads, P(out), P(sec))
+        return out, float(sec[0])
+
+    def dome_sample(self, u):
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros((len(u), 5), np.float32)
+        assert self.orc.lib.orc_dome_sample(self.h, len(u), P(u), P(out)) == 0
+        return out[:, :3], out[:, 3], out[:, 4]
+
+    def dome_eval(self, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        rgb = np.zeros((len(dirs), 3), np.float32)
+        pdf = np.zeros(len(dirs), np.float32)
+        assert self.orc.lib.orc_dome_eval(self.h, len(dirs), P(dirs), P(rgb), P(pdf)) == 0
+        return rgb, pdf
+
+
+class Reference:
+    """The reference's own BVHBackend / PathIntegrator / Halton, built by oracle/Makefile."""
+
+    def __init__(self):
+        self.lib = _load(REF_SO)
+        if self.lib is None:
+            raise RuntimeError("oracle/_ref/libanacapa_ref.so not built")
+        L = self.lib
+        L.ref_pcg_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, VP]
+        L.ref_pcg_float.argtypes = [C.c_uint64, C.c_uint32, VP]
+        L.ref_halton_seq.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP]
+        L.ref_halton_seq2d.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP]
+        L.ref_make_pinhole.argtypes = [VP, VP, VP, C.c_float, C.c_uint32, C.c_uint32, C.POINTER(A.Camera)]
+        L.ref_generate_rays.argtypes = [C.POINTER(A.Camera), C.c_uint32, VP, VP, VP, VP, VP]
+        L.ref_scene_create.restype = VP
+        L.ref_scene_create.argtypes = [C.POINTER(A.SceneDesc)]
+        L.ref_scene_destroy.argtypes = [VP]
+        L.ref_bvh_sizes.argtypes = [VP, VP, VP]
+        L.ref_bvh_dump.argtypes = [VP, VP, VP]
+        L.ref_trace.argtypes = [VP, VP, C.c_uint32, VP]
+        L.ref_occluded.argtypes = [VP, VP, C.c_uint32, VP]
+        L.ref_render_tile.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP]
+        L.ref_render_frame.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
+                                       C.c_uint32, C.c_uint32, VP, VP, VP]
+        L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
+        L.ref_ggx_helper.argtypes = [C.c_int, C.c_uint32, VP, VP, VP, VP]
+        L.ref_lambert.argtypes = [VP, C.c_uint32, VP, VP]
+
+    def pcg(self, seed, seq, n):
+        out = np.zeros(n, np.uint32)
+        self.lib.ref_pcg_u32(seed, seq, n, P(out))
+        return out
+
+    def halton_seq(self, seed, px, py, idx, nd):
+        out = np.zeros(nd, np.float32)
+        self.lib.ref_halton_seq(seed, px, py, idx, nd, P(out))
+        return out
+
+    def halton_seq2d(self, seed, px, py, idx, npairs):
+        out = np.zeros(2 * npairs, np.float32)
+        self.lib.ref_halton_seq2d(seed, px, py, idx, npairs, P(out))
+        return out
+
+    def pinhole(self, frm, at, up, vfov, w, h):
+        cam = A.Camera()
+        f, a, u = (np.array(v, np.float32) for v in (frm, at, up))
+        self.lib.ref_make_pinhole(P(f), P(a), P(u), vfov, w, h, C.byref(cam))
+        return cam
+
+    def generate_rays(self, cam, px, py, jx, jy):
+        px, py = (np.ascontiguousarray(a, np.uint32) for a in (px, py))
+        jx, jy = (np.ascontiguousarray(a, np.float32) for a in (jx, jy))
+        out = np.zeros((len(px), 6), np.float32)
+        self.lib.ref_generate_rays(C.byref(cam), len(px), P(px), P(py), P(jx), P(jy), P(out))
+        return out
+
+    def ggx_helper(self, which, a, b=None, c=None):
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(a if b is None else b, np.float32)
+        c = np.ascontiguousarray(a if c is None else c, np.float32)
+        out = np.zeros(len(a) * (3 if which == 6 else 1), np.float32)
+        self.lib.ref_ggx_helper(which, len(a), P(a), P(b), P(c), P(out))
+        return out
+
+    def distribution1d(self, values, us):
+        values = np.ascontiguousarray(values, np.float32)
+        us = np.ascontiguousarray(us, np.float32)
+        n = len(us)
+        idx, pdf, ur = np.zeros(n, np.uint32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        cdf, integ = np.zeros(len(values) + 1, np.float32), np.zeros(1, np.float32)
+        self.lib.ref_distribution1d(P(values), len(values), n, P(us), P(idx), P(pdf), P(ur), P(cdf), P(integ))
+        return idx, pdf, ur, cdf, float(integ[0])
+
+    def scene(self, sc):
+        return RefScene(self, sc)
+
+
+class RefScene:
+    def __init__(self, ref: Reference, sc):
+        self.ref, self.sc = ref, sc
+        self.desc, self._keep = sc.desc()
+        self.h = ref.lib.ref_scene_create(C.byref(self.desc))
+        if not self.h:
+            raise RuntimeError("reference build cannot represent this scene")
+        self.cam = ref.pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.ref.lib.ref_scene_destroy(self.h)
+            self.h = None
+
+    def bvh(self):
+        nn, npr = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        self.ref.lib.ref_bvh_sizes(self.h, P(nn), P(npr))
+        nodes = np.zeros((int(nn[0]), 8), np.uint32)
+        prims = np.zeros(int(npr[0]), np.uint32)
+        self.ref.lib.ref_bvh_dump(self.h, P(nodes), P(prims))
+        return nodes, prims
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(rays), 12), np.float32)
+        self.ref.lib.ref_trace(self.h, P(rays), len(rays), P(out))
+        mesh = out[:, 1].view(np.uint32).copy()
+        prim = out[:, 2].view(np.uint32).copy()
+        return dict(t=out[:, 0].copy(), mesh_id=mesh, prim_id=prim, p=out[:, 3:6], n=out[:, 6:9], ng=out[:, 9:12])
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(len(rays), np.uint8)
+        self.ref.lib.ref_occluded(self.h, P(rays), len(rays), P(out))
+        return out
+
+    def render_tile(self, x0, y0, w, h, s0, ns, max_depth=None, seed=None, cam=None):
+        out = np.zeros((h, w, 5), np.float32)
+        sc = self.sc
+        self.ref.lib.ref_render_tile(self.h, C.byref(cam or self.cam), x0, y0, w, h, s0, ns,
+                                     sc.max_depth if max_depth is None else max_depth,
+                                     sc.seed if seed is None else seed, P(out))
+        return out
+
+    def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None):
+        cam = cam or self.cam
+        out = np.zeros((cam.height, cam.width, 5), np.float32)
+        sec = np.zeros(1, np.float64)
+        used = np.zeros(1, np.uint32)
+        self.ref.lib.ref_render_frame(self.h, C.byref(cam), spp, self.sc.max_depth if max_depth is None else max_depth,
+                                      self.sc.seed, threads, tile, P(out), P(sec), P(used))
+        return out, float(sec[0]), int(used[0])
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
