@@ -1,0 +1,752 @@
+// abi.cpp — the extern "C" boundary (include/anacapa_hip.h): context, scene upload and the
+// host side of the wavefront bounce loop.
+//
+// The context plays CudaPathIntegrator's role (src/gpu/cuda/CudaPathIntegrator.cu:165-489):
+// it owns one HIP stream and every device allocation, copies the scene into HBM in prepare,
+// keeps a persistent accumulation buffer that successive render_frame calls add into, and
+// reports failures as status codes plus a message instead of aborting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/anacapa_hip.h"
+#include "bvh_build.h"
+#include "device_types.h"
+#include "host_scene.h"
+#include "kernels.h"
+
+using namespace ahip;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0;   // 0 trace, 1 shade
+};
+
+}  // namespace
+
+struct ahip_context {
+    std::mutex mu;
+    int device = -1;
+    bool valid = false;
+    bool failNext = false;
+    std::string name;
+    std::string err;
+    hipStream_t stream = nullptr;
+    uint32_t numCU = 0;
+
+    // scene
+    bool prepared = false;
+    BVHOutput bvh;
+    DevBuf dNodes, dTris, dShade, dMats, dMeshMat, dLights, dTexDesc, dDomePx, dDomeMF, dDomeMC, dDomeCF, dDomeCC,
+        dDomeCI;
+    std::vector<DevBuf*> texBufs;
+    DScene dsc{};
+
+    // render settings
+    ahip_camera cam{};
+    bool hasCam = false;
+    ahip_render_params params{8, 2, 0, 0, 0};
+    uint32_t maxPaths = 1u << 22;
+
+    // persistent accumulation
+    DevBuf dAccum, dAccumLsq;
+    uint32_t accW = 0, accH = 0;
+
+    // wavefront queues
+    DevBuf qRo[2], qRd[2], qBeta[2], qHit, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
+    uint32_t qCap = 0, qCtrSlots = 0;
+    uint32_t traceBlocks = 0, shadeBlocks = 0, stackCap = 0;
+
+    // statistics
+    ahip_stats stats{};
+    std::vector<EventPair> events;
+    size_t eventsUsed = 0;
+    hipEvent_t renderBeg = nullptr, renderEnd = nullptr;
+    bool lastRenderValid = false;
+    uint32_t lastW = 0, lastH = 0, lastS0 = 0, lastNs = 0;
+
+    ~ahip_context() {
+        for (auto* b : texBufs) delete b;
+        for (auto& e : events) {
+            if (e.a) hipEventDestroy(e.a);
+            if (e.b) hipEventDestroy(e.b);
+        }
+        if (renderBeg) hipEventDestroy(renderBeg);
+        if (renderEnd) hipEventDestroy(renderEnd);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int fail(ahip_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(ctx, _e == hipErrorOutOfMemory ? AHIP_E_OUT_OF_MEMORY : AHIP_E_HIP, \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                 \
+    } while (0)
+
+template <class T>
+int upload(ahip_context* ctx, DevBuf& b, const T* src, size_t n) {
+    HIPCHK(b.alloc(std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) HIPCHK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return AHIP_OK;
+}
+
+int checkInjected(ahip_context* ctx) {
+    if (ctx->failNext) {
+        ctx->failNext = false;
+        return fail(ctx, AHIP_E_INJECTED, "injected launch failure (ahip_fail_next_launch)");
+    }
+    return AHIP_OK;
+}
+
+EventPair& nextEvent(ahip_context* ctx, int kind) {
+    if (ctx->eventsUsed == ctx->events.size()) {
+        EventPair e;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        ctx->events.push_back(e);
+    }
+    EventPair& e = ctx->events[ctx->eventsUsed++];
+    e.kind = kind;
+    return e;
+}
+
+int ensureQueues(ahip_context* ctx, uint32_t cap) {
+    uint32_t slots = (ctx->params.max_depth + 2u) * 4u;
+    if (cap <= ctx->qCap && slots <= ctx->qCtrSlots && ctx->qOvfRef.p) return AHIP_OK;
+    cap = std::max(cap, ctx->qCap);
+    size_t f4 = (size_t)cap * sizeof(float4);
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(ctx->qRo[k].alloc(f4));
+        HIPCHK(ctx->qRd[k].alloc(f4));
+        HIPCHK(ctx->qBeta[k].alloc(f4));
+    }
+    HIPCHK(ctx->qHit.alloc(f4));
+    HIPCHK(ctx->qSo.alloc(f4));
+    HIPCHK(ctx->qSd.alloc(f4));
+    HIPCHK(ctx->qSc.alloc(f4));
+    HIPCHK(ctx->qMo.alloc(f4));
+    HIPCHK(ctx->qMd.alloc(f4));
+    HIPCHK(ctx->qMc.alloc(f4));
+    HIPCHK(ctx->qL.alloc(f4));
+    HIPCHK(ctx->qCtr.alloc(slots * sizeof(uint32_t)));
+    HIPCHK(ctx->qTotals.alloc(8 * sizeof(unsigned long long)));
+    ctx->qCap = cap;
+    ctx->qCtrSlots = slots;
+    return AHIP_OK;
+}
+
+int ensureStack(ahip_context* ctx) {
+    uint32_t K = lds_stack_entries();
+    uint32_t capEntries = ctx->bvh.maxDepth + 2u;
+    ctx->stackCap = capEntries;
+    size_t extra = capEntries > K ? capEntries - K : 1u;
+    size_t threads = (size_t)ctx->traceBlocks * 128u;
+    HIPCHK(ctx->qOvfRef.alloc(extra * threads * sizeof(uint32_t)));
+    HIPCHK(ctx->qOvfT.alloc(extra * threads * sizeof(float)));
+    return AHIP_OK;
+}
+
+WaveQueues queues(ahip_context* ctx) {
+    WaveQueues q{};
+    for (int k = 0; k < 2; ++k) {
+        q.ro[k] = ctx->qRo[k].as<float4>();
+        q.rd[k] = ctx->qRd[k].as<float4>();
+        q.beta[k] = ctx->qBeta[k].as<float4>();
+    }
+    q.hit = ctx->qHit.as<float4>();
+    q.so = ctx->qSo.as<float4>();
+    q.sd = ctx->qSd.as<float4>();
+    q.sc = ctx->qSc.as<float4>();
+    q.mo = ctx->qMo.as<float4>();
+    q.md = ctx->qMd.as<float4>();
+    q.mc = ctx->qMc.as<float4>();
+    q.L = ctx->qL.as<float4>();
+    q.counters = ctx->qCtr.as<uint32_t>();
+    q.ovfRef = ctx->qOvfRef.as<uint32_t>();
+    q.ovfT = ctx->qOvfT.as<float>();
+    q.stackCap = ctx->stackCap;
+    q.cap = ctx->qCap;
+    return q;
+}
+
+DCamera toDCam(const ahip_camera& c) {
+    DCamera d;
+    for (int i = 0; i < 3; ++i) {
+        d.origin[i] = c.origin[i];
+        d.llc[i] = c.lower_left[i];
+        d.hor[i] = c.horizontal[i];
+        d.ver[i] = c.vertical[i];
+    }
+    d.w = c.width;
+    d.h = c.height;
+    return d;
+}
+
+// Render samples [s0, s0+ns) of every pixel of region (x0, y0, w, h) and add them into
+// acc / lsq (region-sized).  count != nullptr runs the counter-enabled traversal copies too.
+int renderRegion(ahip_context* ctx, const ahip_camera& cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                 uint32_t s0, uint32_t ns, float4* acc, float* lsq, unsigned long long* countSums) {
+    if (w == 0 || h == 0 || ns == 0) return AHIP_OK;
+    const uint32_t tilesX = (w + 7u) / 8u, tilesY = (h + 7u) / 8u;
+    const uint32_t slots = tilesX * tilesY * 64u;
+    uint32_t perPass = std::max(1u, std::min(ns, ctx->maxPaths / std::max(1u, slots)));
+    uint64_t cap64 = (uint64_t)perPass * slots;
+    if (cap64 > 0x7fffffffull) return fail(ctx, AHIP_E_INVALID_ARG, "region too large for one pass");
+    int st = ensureQueues(ctx, (uint32_t)cap64);
+    if (st) return st;
+    WaveQueues q = queues(ctx);
+    const uint32_t maxDepth = ctx->params.max_depth;
+    PassDesc pd{};
+    pd.cam = toDCam(cam);
+    pd.x0 = x0; pd.y0 = y0; pd.w = w; pd.h = h;
+    pd.tilesX = tilesX;
+    pd.slotsPerSample = slots;
+    pd.seed = ctx->params.seed;
+    pd.maxDepth = maxDepth;
+    pd.minDepth = ctx->params.min_depth;
+    pd.flags = ctx->params.flags;
+    pd.accum = acc;
+    pd.accumLumSq = lsq;
+    hipStream_t s = ctx->stream;
+    for (uint32_t done = 0; done < ns; done += perPass) {
+        pd.sampleStart = s0 + done;
+        pd.samples = std::min(perPass, ns - done);
+        uint32_t paths = pd.samples * slots;
+        launchFillU32(q.counters, ctx->qCtrSlots, 0u, s);
+        {
+            EventPair& e = nextEvent(ctx, 1);
+            hipEventRecord(e.a, s);
+            uint32_t gb = std::min((paths + 255u) / 256u, ctx->shadeBlocks);
+            launchGenerate(pd, q, std::max(gb, 1u), s);
+            hipEventRecord(e.b, s);
+        }
+        uint32_t cur = 0;
+        for (uint32_t b = 0; b <= maxDepth; ++b) {
+            if (countSums) launchCount(ctx->dsc, q, b, cur, 0, ctx->traceBlocks, countSums, s);
+            {
+                EventPair& e = nextEvent(ctx, 0);
+                hipEventRecord(e.a, s);
+                launchExtend(ctx->dsc, q, b, cur, ctx->traceBlocks, s);
+                hipEventRecord(e.b, s);
+            }
+            {
+                EventPair& e = nextEvent(ctx, 1);
+                hipEventRecord(e.a, s);
+                launchShade(ctx->dsc, pd, q, b, cur, ctx->shadeBlocks, s);
+                hipEventRecord(e.b, s);
+            }
+            if (countSums) {
+                launchCount(ctx->dsc, q, b, cur, 1, ctx->traceBlocks, countSums, s);
+                launchCount(ctx->dsc, q, b, cur, 2, ctx->traceBlocks, countSums, s);
+            }
+            {
+                EventPair& e = nextEvent(ctx, 0);
+                hipEventRecord(e.a, s);
+                launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s);
+                launchMis(ctx->dsc, q, b, ctx->traceBlocks, s);
+                hipEventRecord(e.b, s);
+            }
+            cur ^= 1u;
+        }
+        {
+            EventPair& e = nextEvent(ctx, 1);
+            hipEventRecord(e.a, s);
+            launchAccumulate(pd, q, s);
+            hipEventRecord(e.b, s);
+        }
+        tallyCounters(q.counters, maxDepth, ctx->qTotals.as<unsigned long long>(), s);
+        ctx->stats.samples += (uint64_t)w * h * pd.samples;
+    }
+    HIPCHK(hipGetLastError());
+    return AHIP_OK;
+}
+
+// Collect per-launch event timings and ray totals after the stream has been synchronised.
+int finishStats(ahip_context* ctx) {
+    for (size_t i = 0; i < ctx->eventsUsed; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->events[i].a, ctx->events[i].b));
+        if (ctx->events[i].kind == 0) {
+            ctx->stats.trace_ms += ms;
+            ctx->stats.trace_launches += 1;
+        } else {
+            ctx->stats.shade_ms += ms;
+            ctx->stats.shade_launches += 1;
+        }
+    }
+    ctx->eventsUsed = 0;
+    unsigned long long tot[8];
+    HIPCHK(hipMemcpy(tot, ctx->qTotals.p, sizeof(tot), hipMemcpyDeviceToHost));
+    // tot[0] continuation rays, tot[1] shadow rays, tot[2] MIS rays, tot[3] stack overflows
+    ctx->stats.rays_closest = tot[0] + (ctx->dsc.hasEmitters ? tot[2] : 0ull);
+    ctx->stats.rays_shadow = tot[1] + (ctx->dsc.hasEmitters ? 0ull : tot[2]);
+    if (tot[3]) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ahip_abi_version(void) { return AHIP_ABI_VERSION; }
+const char* ahip_backend_type(void) { return "HIP"; }
+
+int ahip_create(int device_ordinal, ahip_context** out) {
+    if (!out) return AHIP_E_INVALID_ARG;
+    auto* ctx = new ahip_context();
+    *out = ctx;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(ctx, AHIP_E_NO_DEVICE, "no HIP device available");
+    if (device_ordinal < 0 || device_ordinal >= n) return fail(ctx, AHIP_E_INVALID_ARG, "device ordinal out of range");
+    ctx->device = device_ordinal;
+    HIPCHK(hipSetDevice(device_ordinal));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device_ordinal));
+    ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    ctx->numCU = (uint32_t)prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ctx->renderBeg));
+    HIPCHK(hipEventCreate(&ctx->renderEnd));
+    ctx->traceBlocks = std::max(1u, ctx->numCU) * 10u;
+    ctx->shadeBlocks = std::max(1u, ctx->numCU) * 8u;
+    HaltonTables T;
+    fillHaltonTables(T);
+    setHaltonTables(T, ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->valid = true;
+    return AHIP_OK;
+}
+
+void ahip_destroy(ahip_context* ctx) {
+    if (!ctx) return;
+    if (ctx->valid) {
+        hipSetDevice(ctx->device);
+        hipStreamSynchronize(ctx->stream);
+    }
+    delete ctx;
+}
+
+int ahip_is_valid(const ahip_context* ctx) { return ctx && ctx->valid ? 1 : 0; }
+const char* ahip_device_name(const ahip_context* ctx) { return ctx ? ctx->name.c_str() : ""; }
+const char* ahip_last_error(const ahip_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+#define ENTER()                                                                   \
+    if (!ctx) return AHIP_E_INVALID_ARG;                                          \
+    std::lock_guard<std::mutex> _lk(ctx->mu);                                     \
+    if (!ctx->valid) return fail(ctx, AHIP_E_NO_DEVICE, "context is not valid"); \
+    HIPCHK(hipSetDevice(ctx->device))
+
+int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
+    ENTER();
+    if (!d) return fail(ctx, AHIP_E_INVALID_ARG, "null scene");
+    ctx->prepared = false;
+    HostScene hs;
+    std::string err;
+    int st = buildHostScene(d, hs, err);
+    if (st) return fail(ctx, st, err);
+    if (buildBVH(d, ctx->bvh, err)) return fail(ctx, AHIP_E_INVALID_ARG, err);
+    DScene& S = ctx->dsc;
+    S = DScene{};
+    if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes.data(), ctx->bvh.nodes.size()))) return st;
+    if ((st = upload(ctx, ctx->dTris, ctx->bvh.tris.data(), ctx->bvh.tris.size()))) return st;
+    if ((st = upload(ctx, ctx->dShade, ctx->bvh.shade.data(), ctx->bvh.shade.size()))) return st;
+    if ((st = upload(ctx, ctx->dMats, hs.mats.data(), hs.mats.size()))) return st;
+    if ((st = upload(ctx, ctx->dMeshMat, hs.meshMat.data(), hs.meshMat.size()))) return st;
+    if ((st = upload(ctx, ctx->dLights, hs.lights.data(), hs.lights.size()))) return st;
+    for (auto* b : ctx->texBufs) delete b;
+    ctx->texBufs.clear();
+    std::vector<DTexture> tex;
+    for (size_t i = 0; i < hs.texPixels.size(); ++i) {
+        auto* b = new DevBuf();
+        ctx->texBufs.push_back(b);
+        if ((st = upload(ctx, *b, hs.texPixels[i].data(), hs.texPixels[i].size()))) return st;
+        tex.push_back(DTexture{b->as<float>(), hs.texW[i], hs.texH[i]});
+    }
+    if ((st = upload(ctx, ctx->dTexDesc, tex.data(), tex.size()))) return st;
+    S.nodes = ctx->dNodes.as<DNode>();
+    S.tris = ctx->dTris.as<DTri>();
+    S.shade = ctx->dShade.as<DTriShade>();
+    S.mats = ctx->dMats.as<DMat>();
+    S.meshMat = ctx->dMeshMat.as<int32_t>();
+    S.lights = ctx->dLights.as<DLight>();
+    S.textures = ctx->dTexDesc.as<DTexture>();
+    S.numMeshes = d->num_meshes;
+    S.numLights = (uint32_t)hs.lights.size();
+    S.numNodes = (uint32_t)ctx->bvh.nodes.size();
+    S.numTris = ctx->bvh.numTris;
+    S.hasBVH = ctx->bvh.numTris > 0 ? 1u : 0u;
+    S.rootRef = ctx->bvh.rootRef;
+    std::memcpy(S.rootMin, ctx->bvh.rootMin, 12);
+    std::memcpy(S.rootMax, ctx->bvh.rootMax, 12);
+    S.hasEmitters = hs.hasEmitters ? 1u : 0u;
+    std::memcpy(S.envRadiance, d->env_radiance, 12);
+    if (hs.hasDome) {
+        const HostDome& D = hs.dome;
+        if ((st = upload(ctx, ctx->dDomePx, D.px4.data(), D.px4.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeMF, D.margFunc.data(), D.margFunc.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeMC, D.margCdf.data(), D.margCdf.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeCF, D.condFunc.data(), D.condFunc.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeCC, D.condCdf.data(), D.condCdf.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeCI, D.condInt.data(), D.condInt.size()))) return st;
+        S.hasEnv = 1u;
+        S.dome.px = ctx->dDomePx.as<float>();
+        S.dome.margFunc = ctx->dDomeMF.as<float>();
+        S.dome.margCdf = ctx->dDomeMC.as<float>();
+        S.dome.condFunc = ctx->dDomeCF.as<float>();
+        S.dome.condCdf = ctx->dDomeCC.as<float>();
+        S.dome.condInt = ctx->dDomeCI.as<float>();
+        S.dome.margInt = D.margInt;
+        S.dome.W = D.W;
+        S.dome.H = D.H;
+        S.dome.intensity = D.intensity;
+        std::memcpy(S.dome.rot, D.rot, sizeof(D.rot));
+    }
+    if ((st = ensureStack(ctx))) return st;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->prepared = true;
+    return AHIP_OK;
+}
+
+int ahip_bvh_info(ahip_context* ctx, uint32_t* num_nodes, uint32_t* num_tris, uint32_t* max_depth) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (num_nodes) *num_nodes = (uint32_t)ctx->bvh.nodes.size();
+    if (num_tris) *num_tris = ctx->bvh.numTris;
+    if (max_depth) *max_depth = ctx->bvh.maxDepth;
+    return AHIP_OK;
+}
+
+int ahip_bvh_dump(ahip_context* ctx, float* nodes32, uint32_t* prim_indices) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (nodes32) std::memcpy(nodes32, ctx->bvh.refNodes.data(), ctx->bvh.refNodes.size() * sizeof(RefNode));
+    if (prim_indices) std::memcpy(prim_indices, ctx->bvh.refOrder.data(), ctx->bvh.refOrder.size() * 4);
+    return AHIP_OK;
+}
+
+int ahip_set_camera(ahip_context* ctx, const ahip_camera* cam) {
+    ENTER();
+    if (!cam || cam->width == 0 || cam->height == 0) return fail(ctx, AHIP_E_INVALID_ARG, "bad camera");
+    ctx->cam = *cam;
+    ctx->hasCam = true;
+    return AHIP_OK;
+}
+
+int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
+    ENTER();
+    if (!p || p->max_depth > 1000u) return fail(ctx, AHIP_E_INVALID_ARG, "bad render params");
+    ctx->params = *p;
+    return AHIP_OK;
+}
+
+int ahip_set_launch(ahip_context* ctx, uint32_t max_paths, uint32_t tile) {
+    ENTER();
+    if (max_paths) ctx->maxPaths = std::max(64u, max_paths);
+    (void)tile;
+    return AHIP_OK;
+}
+
+int ahip_clear_accum(ahip_context* ctx) {
+    ENTER();
+    if (ctx->dAccum.p) {
+        launchFillF32(ctx->dAccum.as<float>(), (size_t)ctx->accW * ctx->accH * 4, 0.f, ctx->stream);
+        launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)ctx->accW * ctx->accH, 0.f, ctx->stream);
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return AHIP_OK;
+}
+
+static int renderCall(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns,
+                      float4* acc, float* lsq, unsigned long long* counts) {
+    int st = checkInjected(ctx);
+    if (st) return st;
+    launchFillU32(reinterpret_cast<uint32_t*>(ctx->qTotals.p), 16, 0u, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
+    st = renderRegion(ctx, ctx->cam, x0, y0, w, h, s0, ns, acc, lsq, counts);
+    if (st) return st;
+    HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
+    ctx->stats.render_ms = ms;
+    return finishStats(ctx);
+}
+
+int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
+                      uint32_t sample_count) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
+    if (width != ctx->cam.width || height != ctx->cam.height)
+        return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");
+    int st = ensureQueues(ctx, 64);
+    if (st) return st;
+    if (ctx->accW != width || ctx->accH != height || !ctx->dAccum.p) {
+        HIPCHK(ctx->dAccum.alloc((size_t)width * height * sizeof(float4)));
+        HIPCHK(ctx->dAccumLsq.alloc((size_t)width * height * sizeof(float)));
+        ctx->accW = width;
+        ctx->accH = height;
+        launchFillF32(ctx->dAccum.as<float>(), (size_t)width * height * 4, 0.f, ctx->stream);
+        launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
+    }
+    st = renderCall(ctx, 0, 0, width, height, sample_start, sample_count, ctx->dAccum.as<float4>(),
+                    ctx->dAccumLsq.as<float>(), nullptr);
+    if (st) return st;
+    ctx->lastRenderValid = true;
+    ctx->lastW = width; ctx->lastH = height; ctx->lastS0 = sample_start; ctx->lastNs = sample_count;
+    return AHIP_OK;
+}
+
+static int downloadRegion(ahip_context* ctx, const DevBuf& acc, const DevBuf& lsq, size_t n, float* out) {
+    std::vector<float> a(n * 4), l(n);
+    HIPCHK(hipMemcpy(a.data(), acc.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(l.data(), lsq.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        out[5 * i + 0] = a[4 * i + 0];
+        out[5 * i + 1] = a[4 * i + 1];
+        out[5 * i + 2] = a[4 * i + 2];
+        out[5 * i + 3] = a[4 * i + 3];
+        out[5 * i + 4] = l[i];
+    }
+    return AHIP_OK;
+}
+
+int ahip_download_accum(ahip_context* ctx, float* out) {
+    ENTER();
+    if (!out) return fail(ctx, AHIP_E_INVALID_ARG, "null output");
+    if (!ctx->dAccum.p) return fail(ctx, AHIP_E_NOT_PREPARED, "nothing rendered");
+    return downloadRegion(ctx, ctx->dAccum, ctx->dAccumLsq, (size_t)ctx->accW * ctx->accH, out);
+}
+
+int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
+                     uint32_t sample_count, float* out) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
+    if (!out || w == 0 || h == 0) return fail(ctx, AHIP_E_INVALID_ARG, "bad tile");
+    w = std::min(w, ctx->cam.width > x0 ? ctx->cam.width - x0 : 0u);
+    h = std::min(h, ctx->cam.height > y0 ? ctx->cam.height - y0 : 0u);
+    if (w == 0 || h == 0) return fail(ctx, AHIP_E_INVALID_ARG, "tile outside the film");
+    int st = ensureQueues(ctx, 64);
+    if (st) return st;
+    DevBuf acc, lsq;
+    HIPCHK(acc.alloc((size_t)w * h * sizeof(float4)));
+    HIPCHK(lsq.alloc((size_t)w * h * sizeof(float)));
+    launchFillF32(acc.as<float>(), (size_t)w * h * 4, 0.f, ctx->stream);
+    launchFillF32(lsq.as<float>(), (size_t)w * h, 0.f, ctx->stream);
+    st = renderCall(ctx, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(), lsq.as<float>(), nullptr);
+    if (st) return st;
+    return downloadRegion(ctx, acc, lsq, (size_t)w * h, out);
+}
+
+int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (!views && num_views) return fail(ctx, AHIP_E_INVALID_ARG, "null views");
+    ahip_camera saved = ctx->cam;
+    bool savedHas = ctx->hasCam;
+    ahip_render_params savedP = ctx->params;
+    int st = ensureQueues(ctx, 64);
+    if (st) return st;
+    for (uint32_t v = 0; v < num_views; ++v) {
+        const ahip_view& V = views[v];
+        uint32_t w = V.camera.width, h = V.camera.height;
+        if (!w || !h) return fail(ctx, AHIP_E_INVALID_ARG, "bad view");
+        ctx->cam = V.camera;
+        ctx->params.max_depth = V.max_depth ? V.max_depth : savedP.max_depth;
+        DevBuf acc, lsq;
+        HIPCHK(acc.alloc((size_t)w * h * sizeof(float4)));
+        HIPCHK(lsq.alloc((size_t)w * h * sizeof(float)));
+        launchFillF32(acc.as<float>(), (size_t)w * h * 4, 0.f, ctx->stream);
+        launchFillF32(lsq.as<float>(), (size_t)w * h, 0.f, ctx->stream);
+        st = ensureQueues(ctx, 64);
+        if (!st) st = renderCall(ctx, 0, 0, w, h, V.sample_start, V.sample_count, acc.as<float4>(), lsq.as<float>(), nullptr);
+        if (!st && V.accum) st = downloadRegion(ctx, acc, lsq, (size_t)w * h, V.accum);
+        if (st) break;
+    }
+    ctx->cam = saved;
+    ctx->hasCam = savedHas;
+    ctx->params = savedP;
+    return st;
+}
+
+int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (n == 0) return AHIP_OK;
+    if (!rays || !hits) return fail(ctx, AHIP_E_INVALID_ARG, "null arrays");
+    int st = checkInjected(ctx);
+    if (st) return st;
+    DevBuf dr, dh, dflag;
+    HIPCHK(dr.alloc((size_t)n * 32));
+    HIPCHK(dh.alloc((size_t)n * sizeof(ahip_hit)));
+    HIPCHK(dflag.alloc(4));
+    HIPCHK(hipMemcpyAsync(dr.p, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream));
+    launchFillU32(dflag.as<uint32_t>(), 1, 0u, ctx->stream);
+    launchTraceBatch(ctx->dsc, dr.as<float>(), n, dh.as<ahip_hit>(), nullptr, false, ctx->qOvfRef.as<uint32_t>(),
+                     ctx->qOvfT.as<float>(), ctx->stackCap, ctx->traceBlocks, dflag.as<uint32_t>(), nullptr,
+                     ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hits, dh.p, (size_t)n * sizeof(ahip_hit), hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, dflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (flag) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
+int ahip_occluded_rays(ahip_context* ctx, const float* rays, uint32_t n, uint8_t* occluded) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (n == 0) return AHIP_OK;
+    if (!rays || !occluded) return fail(ctx, AHIP_E_INVALID_ARG, "null arrays");
+    int st = checkInjected(ctx);
+    if (st) return st;
+    DevBuf dr, dout, dflag;
+    HIPCHK(dr.alloc((size_t)n * 32));
+    HIPCHK(dout.alloc(n));
+    HIPCHK(dflag.alloc(4));
+    HIPCHK(hipMemcpyAsync(dr.p, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream));
+    launchFillU32(dflag.as<uint32_t>(), 1, 0u, ctx->stream);
+    launchTraceBatch(ctx->dsc, dr.as<float>(), n, nullptr, dout.as<uint8_t>(), true, ctx->qOvfRef.as<uint32_t>(),
+                     ctx->qOvfT.as<float>(), ctx->stackCap, ctx->traceBlocks, dflag.as<uint32_t>(), nullptr,
+                     ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(occluded, dout.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, dflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (flag) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
+int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx,
+                     const uint32_t* dim, uint32_t n, float* out) {
+    ENTER();
+    if (n == 0) return AHIP_OK;
+    if (!px || !py || !idx || !dim || !out) return fail(ctx, AHIP_E_INVALID_ARG, "null arrays");
+    for (uint32_t i = 0; i < n; ++i)
+        if (dim[i] > 4096u) return fail(ctx, AHIP_E_INVALID_ARG, "dimension beyond 4096");
+    DevBuf b;
+    HIPCHK(b.alloc((size_t)n * 20));
+    uint32_t* d = b.as<uint32_t>();
+    HIPCHK(hipMemcpyAsync(d, px, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d + n, py, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d + 2 * (size_t)n, idx, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d + 3 * (size_t)n, dim, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    launchHalton(seed, d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n, n, reinterpret_cast<float*>(d + 4 * (size_t)n),
+                 ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d + 4 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AHIP_OK;
+}
+
+int ahip_adjoint(ahip_context* ctx, uint32_t, uint32_t, uint32_t, uint32_t, const float*, float*, float*) {
+    ENTER();
+    return fail(ctx, AHIP_E_UNSUPPORTED, "adjoint is not built yet");
+}
+
+int ahip_get_stats(ahip_context* ctx, ahip_stats* out) {
+    if (!ctx || !out) return AHIP_E_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *out = ctx->stats;
+    return AHIP_OK;
+}
+
+int ahip_reset_stats(ahip_context* ctx) {
+    if (!ctx) return AHIP_E_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->stats = ahip_stats{};
+    return AHIP_OK;
+}
+
+int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
+                       uint32_t sample_count) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (!ctx->hasCam || width != ctx->cam.width || height != ctx->cam.height)
+        return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");
+    int st = ensureQueues(ctx, 64);
+    if (st) return st;
+    DevBuf acc, lsq, sums;
+    HIPCHK(acc.alloc((size_t)width * height * sizeof(float4)));
+    HIPCHK(lsq.alloc((size_t)width * height * sizeof(float)));
+    HIPCHK(sums.alloc(2 * sizeof(unsigned long long)));
+    launchFillF32(acc.as<float>(), (size_t)width * height * 4, 0.f, ctx->stream);
+    launchFillF32(lsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
+    launchFillU32(sums.as<uint32_t>(), 4, 0u, ctx->stream);
+    ahip_stats saved = ctx->stats;
+    st = renderCall(ctx, 0, 0, width, height, sample_start, sample_count, acc.as<float4>(), lsq.as<float>(),
+                    sums.as<unsigned long long>());
+    unsigned long long s[2] = {0, 0};
+    if (!st) {
+        hipError_t e = hipMemcpy(s, sums.p, sizeof(s), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = fail(ctx, AHIP_E_HIP, hipGetErrorString(e));
+    }
+    ahip_stats counted = ctx->stats;
+    ctx->stats = saved;
+    ctx->stats.nodes_visited = s[0];
+    ctx->stats.prims_tested = s[1];
+    ctx->stats.rays_closest = counted.rays_closest;
+    ctx->stats.rays_shadow = counted.rays_shadow;
+    return st;
+}
+
+int ahip_fail_next_launch(ahip_context* ctx) {
+    if (!ctx) return AHIP_E_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->failNext = true;
+    return AHIP_OK;
+}
+
+int ahip_camera_pinhole(const float from[3], const float at[3], const float up[3], float vfov_degrees, uint32_t width,
+                        uint32_t height, ahip_camera* out) {
+    if (!from || !at || !up || !out || width == 0 || height == 0) return AHIP_E_INVALID_ARG;
+    return makePinholeCamera(from, at, up, vfov_degrees, width, height, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,372 @@
+// device_shading.h — materials, lights and the shading frame on gfx950.
+//
+// Lambert / Emissive: src/shading/Lambertian.h:12-99.
+// StandardSurface (opaque layered branch: coat GGX + metal GGX + dielectric GGX + diffuse,
+// sample one lobe then report the full-mixture f and pdf): src/shading/StandardSurface.h:20-92,
+// 152-169, 231-241, 351-405, 410-503, 516-661.
+// AreaLight: src/shading/lights/AreaLight.h:123-180.  DomeLight (equirect, 2D piecewise-constant
+// importance sampling, bilinear lookup): src/shading/lights/DomeLight.h:19-55, 117-133, 228-364.
+// ShadingContext: include/anacapa/shading/ShadingContext.h:28-59.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/anacapa_hip.h"
+#include "device_math.h"
+#include "device_types.h"
+
+namespace ahip {
+
+struct Frame {
+    V3 n, ng, t, bt;
+    bool front;
+};
+__device__ __forceinline__ Frame makeFrame(V3 n, V3 ng, V3 rayDir) {
+    Frame f;
+    f.front = dot(-rayDir, ng) > 0.f;
+    f.n = f.front ? n : -n;
+    f.ng = f.front ? ng : -ng;
+    onb(f.n, f.t, f.bt);
+    return f;
+}
+__device__ __forceinline__ V3 toLocal(const Frame& f, V3 v) { return V3{dot(v, f.t), dot(v, f.bt), dot(v, f.n)}; }
+__device__ __forceinline__ V3 toWorld(const Frame& f, V3 v) { return f.t * v.x + f.bt * v.y + f.n * v.z; }
+
+struct BSample {
+    V3 f, wi;
+    float pdf;
+    bool delta;
+};
+__device__ __forceinline__ bool bsValid(const BSample& s) { return s.pdf > 0.f && !isBlack(s.f); }
+
+// ---- GGX helpers (StandardSurface.h:20-76)
+__device__ __forceinline__ float dGGX(float cosH, float a2) {
+    if (cosH <= 0.f) return 0.f;
+    float den = cosH * cosH * (a2 - 1.f) + 1.f;
+    return a2 / (kPi * den * den);
+}
+__device__ __forceinline__ float g1Smith(float c, float a2) {
+    if (c <= 0.f) return 0.f;
+    return 2.f * c / (c + sqrtf(a2 + (1.f - a2) * c * c));
+}
+__device__ __forceinline__ float g2Smith(float co, float ci, float a2) {
+    return g1Smith(fabsf(co), a2) * g1Smith(fabsf(ci), a2);
+}
+__device__ __forceinline__ V3 sampleHalf(V2 u, float a2) {
+    float phi = 2.f * kPi * u.x;
+    float c2 = (1.f - u.y) / stdmax(1.f + (a2 - 1.f) * u.y, 1e-10f);
+    float ch = sqrtf(stdmax(0.f, c2)), sh = sqrtf(stdmax(0.f, 1.f - c2));
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return V3{sh * cp, sh * sp, ch};
+}
+__device__ __forceinline__ float pdfReflect(float cosH, float a2, float dotVH) {
+    if (cosH <= 0.f || dotVH <= 0.f) return 0.f;
+    return dGGX(cosH, a2) * cosH / (4.f * dotVH);
+}
+__device__ __forceinline__ float schlickScalar(float c, float f0) {
+    float x = 1.f - fabsf(c);
+    float x5 = x * x * x * x * x;
+    return f0 + (1.f - f0) * x5;
+}
+__device__ __forceinline__ V3 schlickColor(float c, V3 f0) {
+    float x = 1.f - fabsf(c);
+    float x5 = x * x * x * x * x;
+    return f0 + (V3{1.f, 1.f, 1.f} - f0) * x5;
+}
+
+__device__ __forceinline__ float texBilinear(const DTexture& t, V2 uv) {
+    float s = uv.x, tt = 1.f - uv.y;
+    float fx = s * (float)t.w - 0.5f, fy = tt * (float)t.h - 0.5f;
+    float x0f = floorf(fx), y0f = floorf(fy);
+    float ax = fx - x0f, ay = fy - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f;
+    int W = (int)t.w, H = (int)t.h;
+    int xa = x0 % W; xa = xa < 0 ? xa + W : xa;
+    int xb = (x0 + 1) % W; xb = xb < 0 ? xb + W : xb;
+    int ya = y0 % H; ya = ya < 0 ? ya + H : ya;
+    int yb = (y0 + 1) % H; yb = yb < 0 ? yb + H : yb;
+    float p00 = t.px[(size_t)ya * t.w + xa], p01 = t.px[(size_t)ya * t.w + xb];
+    float p10 = t.px[(size_t)yb * t.w + xa], p11 = t.px[(size_t)yb * t.w + xb];
+    return (1.f - ay) * ((1.f - ax) * p00 + ax * p01) + ay * ((1.f - ax) * p10 + ax * p11);
+}
+
+struct MatEval {
+    const DMat* m;
+    float rough;   // evalTOV(roughness, uv)
+};
+
+// Layer weights at wo (StandardSurface.h:355-371); false when they sum to zero.
+__device__ __forceinline__ bool layerWeights(const DMat& m, float woz, float& wc, float& wm, float& ws, float& wd) {
+    float coatF = schlickScalar(woz, m.coatF0), specF = schlickScalar(woz, m.f0d);
+    wc = m.coat * coatF;
+    wm = m.metal * (1.f - wc);
+    ws = m.spec * (1.f - m.metal) * specF * (1.f - wc);
+    wd = m.base * (1.f - m.metal) * (1.f - m.spec * specF) * (1.f - wc);
+    float sum = wc + wm + ws + wd;
+    if (!(sum > 0.f)) return false;
+    float inv = 1.f / sum;
+    wc *= inv; wm *= inv; ws *= inv; wd *= inv;
+    return true;
+}
+
+__device__ __forceinline__ V3 evalLobe(V3 wo, V3 wi, float a2, V3 f0, float& pf) {   // StandardSurface.h:578-603
+    pf = 0.f;
+    if (wo.z <= 0.f || wi.z <= 0.f) return V3{0.f, 0.f, 0.f};
+    V3 wh = safeNormalize(wo + wi);
+    if (wh.z <= 0.f) return V3{0.f, 0.f, 0.f};
+    float dVH = stdmax(0.f, dot(wo, wh));
+    V3 F = schlickColor(dVH, f0);
+    float D = dGGX(wh.z, a2), G = g2Smith(wo.z, wi.z, a2);
+    float co = fabsf(wo.z), ci = fabsf(wi.z);
+    if (co < 1e-6f || ci < 1e-6f) return V3{0.f, 0.f, 0.f};
+    pf = pdfReflect(wh.z, a2, dVH);
+    return F * (D * G / (4.f * co * ci));
+}
+
+__device__ __forceinline__ V3 evalMixture(const DMat& m, V3 wo, V3 wi, float wc, float wm, float ws, float wd,
+                                          float a2, float& pdf) {                       // StandardSurface.h:608-661
+    V3 f{0.f, 0.f, 0.f};
+    pdf = 0.f;
+    if (!(wo.z * wi.z > 0.f)) return f;
+    float pf;
+    V3 base = ld3(m.albedo);
+    if (wc > 0.f) { V3 c = evalLobe(wo, wi, m.coatA2, V3{m.coatF0, m.coatF0, m.coatF0}, pf); f = f + c * wc; pdf += pf * wc; }
+    if (wm > 0.f) { V3 c = evalLobe(wo, wi, a2, base * m.base, pf); f = f + c * wm; pdf += pf * wm; }
+    if (ws > 0.f) { V3 c = evalLobe(wo, wi, a2, ld3(m.specColor) * (m.spec * m.f0d), pf); f = f + c * ws; pdf += pf * ws; }
+    if (wd > 0.f) {
+        float ci = fabsf(wi.z);
+        f = f + base * m.base * kInvPi * wd;
+        pdf += ci * kInvPi * wd;
+    }
+    return f;
+}
+
+__device__ __forceinline__ float matRoughness(const DScene& sc, const DMat& m, V2 uv) {
+    return m.roughTex >= 0 ? texBilinear(sc.textures[m.roughTex], uv) : m.rough;
+}
+
+// IMaterial::sample
+__device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V2 u, float uc) {
+    BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    if (m.type != AHIP_MAT_STANDARD) {                                   // Lambertian.h:36-51
+        float r = sqrtf(u.x), phi = 2.f * kPi * u.y;
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        V3 wl{r * cp, r * sp, sqrtf(stdmax(0.f, 1.f - u.x))};
+        if (toLocal(fr, wo).z < 0.f) wl.z = -wl.z;
+        s.wi = toWorld(fr, wl);
+        float c = fabsf(wl.z);
+        s.pdf = stdmax(0.f, c) * kInvPi;
+        s.f = ld3(m.albedo) * (kInvPi * c);
+        return s;
+    }
+    V3 wol = toLocal(fr, wo);
+    if (wol.z <= 0.f) return s;
+    float r = matRoughness(sc, m, uv);
+    float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
+    float wc, wm, ws, wd;
+    if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return s;
+    V3 base = ld3(m.albedo);
+    BSample res{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    if (uc < wc + wm + ws) {
+        float a2x;
+        V3 f0;
+        if (uc < wc) { a2x = m.coatA2; f0 = V3{m.coatF0, m.coatF0, m.coatF0}; }
+        else if (uc < wc + wm) { a2x = a2; f0 = base * m.base; }
+        else { a2x = a2; f0 = ld3(m.specColor) * (m.spec * m.f0d); }
+        V3 wh = sampleHalf(u, a2x);                                        // StandardSurface.h:516-549
+        if (wh.z <= 0.f) return s;
+        float dVH = dot(wol, wh);
+        if (dVH <= 0.f) return s;
+        V3 wl = wh * (2.f * dVH) - wol;
+        if (wl.z <= 0.f) return s;
+        V3 F = schlickColor(dVH, f0);
+        float D = dGGX(wh.z, a2x), G = g2Smith(wol.z, wl.z, a2x);
+        float co = fabsf(wol.z), ci = fabsf(wl.z);
+        if (co < 1e-6f || ci < 1e-6f) return s;
+        res.wi = toWorld(fr, wl);
+        res.f = F * (D * G / (4.f * co)) * ci;
+        res.pdf = pdfReflect(wh.z, a2x, dVH);
+    } else {                                                               // StandardSurface.h:554-573
+        float phi = 2.f * kPi * u.x, rr = sqrtf(u.y), z = sqrtf(stdmax(0.f, 1.f - u.y));
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        V3 wl{rr * cp, rr * sp, z};
+        if (wol.z < 0.f) wl.z = -wl.z;
+        float ci = fabsf(wl.z);
+        res.wi = toWorld(fr, wl);
+        res.f = base * m.base * (kInvPi * ci);
+        res.pdf = ci * kInvPi;
+    }
+    if (!bsValid(res)) return s;
+    V3 wil = toLocal(fr, res.wi);
+    float pdf;
+    V3 fc = evalMixture(m, wol, wil, wc, wm, ws, wd, a2, pdf);
+    res.f = fc * fabsf(wil.z);
+    res.pdf = pdf;
+    res.delta = m.specularFlag != 0u;
+    return res;
+}
+
+// IMaterial::evaluate: f without cosine, forward pdf.
+__device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V3 wi, float& pdf) {
+    V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
+    pdf = 0.f;
+    if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
+    if (m.type != AHIP_MAT_STANDARD) {
+        pdf = stdmax(0.f, fabsf(wil.z)) * kInvPi;
+        return ld3(m.albedo) * kInvPi;
+    }
+    float r = matRoughness(sc, m, uv);
+    float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
+    float wc, wm, ws, wd;
+    if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return V3{0.f, 0.f, 0.f};
+    return evalMixture(m, wol, wil, wc, wm, ws, wd, a2, pdf);
+}
+
+// ---------------------------------------------------------------- dome light
+__device__ __forceinline__ uint32_t lowerBound(const float* cdf, uint32_t n, float u) {
+    uint32_t lo = 0, count = n;
+    while (count > 0u) {
+        uint32_t step = count >> 1;
+        if (cdf[lo + step] < u) { lo += step + 1u; count -= step + 1u; }
+        else count = step;
+    }
+    return lo;
+}
+
+// Distribution1D::sample (DomeLight.h:40-54) over func[n], cdf[n+1].
+__device__ __forceinline__ uint32_t dist1dSample(const float* func, const float* cdf, float integral, uint32_t n,
+                                                 float u, float& pdf, float& ur) {
+    int idx = (int)lowerBound(cdf, n + 1u, u) - 1;
+    idx = idx < 0 ? 0 : (idx > (int)n - 1 ? (int)n - 1 : idx);
+    float bw = cdf[idx + 1] - cdf[idx];
+    ur = (bw > 1e-12f) ? (u - cdf[idx]) / bw : 0.f;
+    ur = stdmax(0.f, stdmin(ur, 1.f - 1e-7f));
+    pdf = (integral > 0.f) ? func[idx] / integral : 0.f;
+    return (uint32_t)idx;
+}
+
+__device__ __forceinline__ V3 domeToEnv(const DDome& d, V3 w) {
+    return V3{d.rot[0] * w.x + d.rot[1] * w.y + d.rot[2] * w.z, d.rot[3] * w.x + d.rot[4] * w.y + d.rot[5] * w.z,
+              d.rot[6] * w.x + d.rot[7] * w.y + d.rot[8] * w.z};
+}
+__device__ __forceinline__ V3 domeFromEnv(const DDome& d, V3 v) {
+    return V3{d.rot[0] * v.x + d.rot[3] * v.y + d.rot[6] * v.z, d.rot[1] * v.x + d.rot[4] * v.y + d.rot[7] * v.z,
+              d.rot[2] * v.x + d.rot[5] * v.y + d.rot[8] * v.z};
+}
+
+__device__ __forceinline__ void domeAngles(const DDome& d, V3 w, float& theta, float& phi) {
+    V3 l = domeToEnv(d, w);
+    theta = acosf(stdmax(-1.f, stdmin(1.f, l.y)));
+    phi = atan2f(l.x, l.z);
+    if (phi < 0.f) phi += k2Pi;
+}
+
+// DomeLight::evalEnvmap (DomeLight.h:326-364), times intensity.
+__device__ V3 domeEval(const DDome& d, V3 w) {
+    float theta, phi;
+    domeAngles(d, w, theta, phi);
+    float uu = phi / k2Pi, v = theta / kPi;
+    float fc = uu * (float)d.W - 0.5f, fr = v * (float)d.H - 0.5f;
+    int c0 = (int)floorf(fc), r0 = (int)floorf(fr);
+    float tc = fc - (float)c0, tr = fr - (float)r0;
+    int W = (int)d.W, H = (int)d.H;
+    uint32_t c0u = (uint32_t)((c0 % W + W) % W), c1 = (uint32_t)(((c0 + 1) % W + W) % W);
+    uint32_t r0u = (uint32_t)max(0, min(r0, H - 1)), r1u = (uint32_t)max(0, min(r0 + 1, H - 1));
+    const float4* tex = reinterpret_cast<const float4*>(d.px);
+    float4 p00 = tex[(size_t)r0u * d.W + c0u], p01 = tex[(size_t)r0u * d.W + c1];
+    float4 p10 = tex[(size_t)r1u * d.W + c0u], p11 = tex[(size_t)r1u * d.W + c1];
+    V3 s;
+    s.x = stdmax(0.f, (1.f - tr) * ((1.f - tc) * p00.x + tc * p01.x) + tr * ((1.f - tc) * p10.x + tc * p11.x));
+    s.y = stdmax(0.f, (1.f - tr) * ((1.f - tc) * p00.y + tc * p01.y) + tr * ((1.f - tc) * p10.y + tc * p11.y));
+    s.z = stdmax(0.f, (1.f - tr) * ((1.f - tc) * p00.z + tc * p01.z) + tr * ((1.f - tc) * p10.z + tc * p11.z));
+    return s * d.intensity;
+}
+
+// DomeLight::evalPdf (DomeLight.h:300-321)
+__device__ float domePdf(const DDome& d, V3 w) {
+    float theta, phi;
+    domeAngles(d, w, theta, phi);
+    float uu = phi / k2Pi, v = theta / kPi;
+    uint32_t col = min((uint32_t)(uu * (float)d.W), d.W - 1u);
+    uint32_t row = min((uint32_t)(v * (float)d.H), d.H - 1u);
+    float st = stdmax(sinf(theta), 1e-6f);
+    float pr = d.margInt > 0.f ? d.margFunc[row] / d.margInt : 0.f;
+    float ci = d.condInt[row];
+    float pc = ci > 0.f ? d.condFunc[(size_t)row * d.W + col] / ci : 0.f;
+    return (pr * pc * (float)(d.W * d.H)) / (k2Pi * kPi * st);
+}
+
+// DomeLight::sampleDirection (DomeLight.h:264-295)
+__device__ V3 domeSampleDir(const DDome& d, V2 u, float& pdf) {
+    float pr, pc, ur, uc;
+    uint32_t row = dist1dSample(d.margFunc, d.margCdf, d.margInt, d.H, u.y, pr, ur);
+    uint32_t col = dist1dSample(d.condFunc + (size_t)row * d.W, d.condCdf + (size_t)row * (d.W + 1u), d.condInt[row],
+                                d.W, u.x, pc, uc);
+    float v = ((float)row + ur) / (float)d.H;
+    float uu = ((float)col + uc) / (float)d.W;
+    float theta = v * kPi, phi = uu * k2Pi;
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    V3 dir = domeFromEnv(d, V3{st * sp, ct, st * cp});
+    float sa = stdmax(st, 1e-6f);
+    pdf = (pr * pc * (float)(d.W * d.H)) / (k2Pi * kPi * sa);
+    if (pdf <= 0.f) pdf = 0.f;
+    return dir;
+}
+
+struct LSample {
+    V3 Li, wi;
+    float pdf, dist;
+};
+
+// ILight::sample
+__device__ LSample lightSample(const DScene& sc, const DLight& L, V3 p, V2 u) {
+    LSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f};
+    if (L.type == AHIP_LIGHT_DOME) {
+        float pdf;
+        V3 wi = domeSampleDir(sc.dome, u, pdf);
+        if (pdf <= 0.f) return s;
+        s.Li = domeEval(sc.dome, wi);
+        s.wi = wi;
+        s.pdf = pdf;
+        s.dist = 1e10f;
+        return s;
+    }
+    V3 pos = ld3(L.center) + ld3(L.uA) * (2.f * u.x - 1.f) + ld3(L.vA) * (2.f * u.y - 1.f);
+    V3 tl = pos - p;
+    float d2 = tl.x * tl.x + tl.y * tl.y + tl.z * tl.z;
+    float dist = sqrtf(d2);
+    V3 wi = tl * (1.f / dist);
+    float cosL = dot(-wi, ld3(L.normal));
+    if (cosL <= 0.f) return s;
+    s.Li = ld3(L.Le);
+    s.wi = wi;
+    s.pdf = d2 / (cosL * L.area);
+    s.dist = dist * (1.f - 1e-4f);
+    return s;
+}
+
+// ILight::pdf
+__device__ float lightPdf(const DScene& sc, const DLight& L, V3 p, V3 wi) {
+    if (L.type == AHIP_LIGHT_DOME) return domePdf(sc.dome, wi);
+    V3 nn = -ld3(L.normal);
+    float den = dot(wi, nn);
+    if (den < 1e-6f) return 0.f;
+    float t = dot(ld3(L.center) - p, nn) / den;
+    if (t <= 0.f) return 0.f;
+    V3 off = (p + wi * t) - ld3(L.center);
+    float up = dot(off, ld3(L.uA)) / (L.uLen * L.uLen);
+    float vp = dot(off, ld3(L.vA)) / (L.vLen * L.vLen);
+    if (fabsf(up) > 1.f || fabsf(vp) > 1.f) return 0.f;
+    return t * t / (den * L.area);
+}
+
+__device__ __forceinline__ V3 envLe(const DScene& sc, V3 d) {
+    return sc.hasEnv ? domeEval(sc.dome, d) : ld3(sc.envRadiance);
+}
+
+}  // namespace ahip

@@ -1,0 +1,118 @@
+// device_types.h — HBM layouts shared by the host code and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace ahip {
+
+// A stack / child reference packs a leaf flag, a leaf primitive count (1..8) and an index:
+//   bit 31      leaf flag
+//   bits 28..30 count-1 (leaves hold at most 4 triangles, BVHBackend.h:67)
+//   bits 0..27  node index (interior) or first triangle index (leaf)
+constexpr uint32_t kRefLeaf = 0x80000000u;
+constexpr uint32_t kRefIndexMask = 0x0fffffffu;
+constexpr uint32_t kMaxIndex = kRefIndexMask;
+__host__ __device__ inline uint32_t makeLeafRef(uint32_t first, uint32_t count) {
+    return kRefLeaf | ((count - 1u) << 28) | first;
+}
+__host__ __device__ inline bool refIsLeaf(uint32_t r) { return (r & kRefLeaf) != 0u; }
+__host__ __device__ inline uint32_t refIndex(uint32_t r) { return r & kRefIndexMask; }
+__host__ __device__ inline uint32_t refCount(uint32_t r) { return ((r >> 28) & 7u) + 1u; }
+
+// One 64-byte record per interior node of the reference binary BVH (BVHBackend.h:25-39):
+// the boxes of BOTH children, so a visit costs one aligned 64-B fetch and tests two boxes.
+struct alignas(64) DNode {
+    float lmin[3], lmax[3];   // left child box   (bytes  0..23)
+    float rmin[3], rmax[3];   // right child box  (bytes 24..47)
+    uint32_t left, right;     // child references (bytes 48..55)
+    uint32_t pad[2];
+};
+static_assert(sizeof(DNode) == 64, "DNode must be 64 bytes");
+
+// Leaf-ordered triangle record used by intersection (BVHTriangle v0/e1/e2/n, BVHBackend.h:49-59).
+struct alignas(64) DTri {
+    float v0[3];
+    float e1[3];
+    float e2[3];
+    float n[3];               // geometric normal (normalised cross(e1, e2))
+    uint32_t meshID, primID;  // SurfaceInteraction ids
+    uint32_t commitIndex;     // index in commit order (reference m_tris)
+    uint32_t pad;
+};
+static_assert(sizeof(DTri) == 64, "DTri must be 64 bytes");
+
+// Shading attributes, fetched only for the closest hit.
+struct alignas(64) DTriShade {
+    float sn0[3], sn1[3], sn2[3];
+    float uv0[2], uv1[2], uv2[2];
+    uint32_t pad;
+};
+static_assert(sizeof(DTriShade) == 64, "DTriShade must be 64 bytes");
+
+struct DMat {
+    uint32_t type;            // AHIP_MAT_*
+    uint32_t isDelta;         // StandardSurface::isDelta (StandardSurface.h:162-169)
+    uint32_t specularFlag;    // flags() & Specular (roughness < 0.001)
+    int32_t roughTex;         // -1 or texture slot
+    float albedo[3];          // Lambert albedo / StandardSurface base_color
+    float Le[3];              // emitted radiance (already * emission weight)
+    float base, metal, spec, rough;
+    float specColor[3];
+    float coat, coatA2, f0d, coatF0;
+    uint32_t hasLe;
+};
+
+struct DLight {
+    uint32_t type;            // AHIP_LIGHT_*
+    float center[3], uA[3], vA[3], Le[3], normal[3];
+    float area, uLen, vLen;
+};
+
+struct DTexture {
+    const float* px;
+    uint32_t w, h;
+};
+
+struct DDome {
+    const float* px;          // W*H texels of 4 floats (rgb + pad), 16-byte aligned
+    const float* margFunc;    // [H]
+    const float* margCdf;     // [H+1]
+    const float* condFunc;    // [H*W]
+    const float* condCdf;     // [H*(W+1)]
+    const float* condInt;     // [H]
+    float margInt;
+    uint32_t W, H;
+    float intensity;
+    float rot[9];
+};
+
+// Halton bases and their reciprocals (1.f / (float)prime, the reference's invB), in constant memory.
+struct HaltonTables {
+    uint32_t prime[128];
+    float invPrime[128];
+};
+
+struct DCamera {
+    float origin[3], llc[3], hor[3], ver[3];
+    uint32_t w, h;
+};
+
+// Everything a kernel needs to see of the scene (passed by value as a kernel argument).
+struct DScene {
+    const DNode* nodes;
+    const DTri* tris;
+    const DTriShade* shade;
+    const DMat* mats;
+    const int32_t* meshMat;
+    const DLight* lights;
+    const DTexture* textures;
+    uint32_t numMeshes, numLights, numNodes, numTris;
+    float rootMin[3], rootMax[3];
+    uint32_t rootRef;         // reference of the root (a leaf ref when the whole BVH is one leaf)
+    uint32_t hasBVH;
+    uint32_t hasEnv;          // dome present
+    uint32_t hasEmitters;     // any material with non-zero Le
+    float envRadiance[3];
+    DDome dome;
+};
+
+}  // namespace ahip

@@ -1,0 +1,241 @@
+// host_scene.cpp — materials, lights, dome distributions, Halton tables and the pinhole camera,
+// computed on the host.  Compiled by g++ with the reference's Release flags (like bvh_build.cpp)
+// so the float arithmetic of the dome's distribution build and the camera set-up contracts as
+// the reference's does.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#define __host__
+#define __device__
+#include "device_types.h"
+#include "host_scene.h"
+
+namespace ahip {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float k2Pi = 6.28318530717958647692f;
+
+// First 128 primes (the Halton bases of src/sampling/HaltonSampler.h:68-79).
+void firstPrimes(uint32_t* out, int n) {
+    int k = 0;
+    for (uint32_t c = 2; k < n; ++c) {
+        bool prime = true;
+        for (uint32_t d = 2; d * d <= c; ++d)
+            if (c % d == 0) { prime = false; break; }
+        if (prime) out[k++] = c;
+    }
+}
+
+struct V {
+    float x, y, z;
+};
+V sub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V mul(V a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+V crs(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+float lenv(V a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+V nrm(V a) { return mul(a, 1.f / lenv(a)); }
+V nrmSafe(V a) {
+    float l = lenv(a);
+    return l > 1e-8f ? mul(a, 1.f / l) : V{0.f, 0.f, 1.f};
+}
+V ld(const float* p) { return {p[0], p[1], p[2]}; }
+void st(float* p, V v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// Distribution1D::build (src/shading/lights/DomeLight.h:25-37)
+float buildDist(const float* f, uint32_t n, float* func, float* cdf) {
+    std::memcpy(func, f, n * sizeof(float));
+    cdf[0] = 0.f;
+    for (uint32_t i = 0; i < n; ++i) cdf[i + 1] = cdf[i] + func[i];
+    float integral = cdf[n];
+    if (integral > 0.f) {
+        float inv = 1.f / integral;
+        for (uint32_t i = 0; i <= n; ++i) cdf[i] *= inv;
+    }
+    return integral;
+}
+
+}  // namespace
+
+void fillHaltonTables(HaltonTables& t) {
+    firstPrimes(t.prime, 128);
+    for (int i = 0; i < 128; ++i) t.invPrime[i] = 1.f / static_cast<float>(t.prime[i]);
+}
+
+// Camera::makePinhole (include/anacapa/integrator/IIntegrator.h:49-74)
+int makePinholeCamera(const float from[3], const float at[3], const float up[3], float vfov, uint32_t width,
+                      uint32_t height, ahip_camera* out) {
+    float theta = vfov * 3.14159265f / 180.f;
+    float h = std::tan(theta * 0.5f);
+    float aspect = static_cast<float>(width) / static_cast<float>(height);
+    V f = ld(from), a = ld(at), u0 = ld(up);
+    V w = nrm(sub(f, a));
+    V u = nrm(crs(u0, w));
+    V v = crs(w, u);
+    float vpH = 2.f * h, vpW = aspect * vpH;
+    st(out->origin, f);
+    st(out->horizontal, mul(u, vpW));
+    st(out->vertical, mul(v, vpH));
+    V llc = sub(sub(sub(f, mul(u, vpW * 0.5f)), mul(v, vpH * 0.5f)), w);
+    st(out->lower_left, llc);
+    out->width = width;
+    out->height = height;
+    return AHIP_OK;
+}
+
+int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
+    out = HostScene{};
+    if (d->num_meshes && (!d->mesh_vertex_offset || !d->mesh_index_offset || !d->mesh_material || !d->positions ||
+                          !d->indices)) {
+        err = "scene arrays missing";
+        return AHIP_E_INVALID_ARG;
+    }
+    if (d->num_meshes && d->mesh_vertex_offset[d->num_meshes] > d->num_vertices) {
+        err = "mesh_vertex_offset beyond num_vertices";
+        return AHIP_E_INVALID_ARG;
+    }
+    // textures
+    for (uint32_t i = 0; i < d->num_textures; ++i) {
+        uint32_t w = d->texture_width[i], h = d->texture_height[i];
+        if (!w || !h || !d->texture_pixels[i]) {
+            err = "empty texture";
+            return AHIP_E_INVALID_ARG;
+        }
+        out.texPixels.emplace_back(d->texture_pixels[i], d->texture_pixels[i] + (size_t)w * h);
+        out.texW.push_back(w);
+        out.texH.push_back(h);
+    }
+    // materials (Lambertian.h:29-99, StandardSurface.h:152-187)
+    std::vector<bool> emits(d->num_materials, false);
+    for (uint32_t i = 0; i < d->num_materials; ++i) {
+        const ahip_material& m = d->materials[i];
+        DMat D{};
+        D.type = m.type;
+        D.roughTex = -1;
+        std::memcpy(D.albedo, m.albedo, 12);
+        if (m.type == AHIP_MAT_LAMBERT || m.type == AHIP_MAT_EMISSIVE) {
+            if (m.type == AHIP_MAT_EMISSIVE) std::memcpy(D.Le, m.emission, 12);
+        } else if (m.type == AHIP_MAT_STANDARD) {
+            if (m.transmission > 0.001f) {
+                err = "transmissive standard_surface is not supported by this backend";
+                return AHIP_E_UNSUPPORTED;
+            }
+            D.base = m.base;
+            D.metal = m.metalness;
+            D.spec = m.specular;
+            D.rough = m.roughness;
+            std::memcpy(D.specColor, m.specular_color, 12);
+            D.coat = m.coat;
+            float ca = std::max(1e-4f, m.coat_roughness * m.coat_roughness);
+            D.coatA2 = ca * ca;
+            float r = (1.f - m.specular_ior) / (1.f + m.specular_ior);
+            D.f0d = r * r;
+            float rc = (1.f - 1.5f) / (1.f + 1.5f);
+            D.coatF0 = rc * rc;
+            float rr = m.roughness, mm = m.metalness, ss = m.specular;
+            D.isDelta = (rr < 0.001f && (mm > 0.999f || ss > 0.f)) ? 1u : 0u;
+            D.specularFlag = rr < 0.001f ? 1u : 0u;
+            if (m.emission_weight > 0.f)
+                for (int k = 0; k < 3; ++k) D.Le[k] = m.emission[k] * m.emission_weight;
+            if (m.roughness_texture >= 0) {
+                if ((uint32_t)m.roughness_texture >= d->num_textures) {
+                    err = "roughness_texture out of range";
+                    return AHIP_E_INVALID_ARG;
+                }
+                D.roughTex = m.roughness_texture;
+            }
+        } else {
+            err = "unknown material type";
+            return AHIP_E_INVALID_ARG;
+        }
+        D.hasLe = (D.Le[0] != 0.f || D.Le[1] != 0.f || D.Le[2] != 0.f) ? 1u : 0u;
+        emits[i] = D.hasLe != 0u;
+        out.mats.push_back(D);
+    }
+    out.meshMat.assign(d->mesh_material, d->mesh_material + d->num_meshes);
+    for (int32_t mi : out.meshMat) {
+        if (mi >= (int32_t)d->num_materials) {
+            err = "mesh_material out of range";
+            return AHIP_E_INVALID_ARG;
+        }
+        if (mi >= 0 && emits[mi]) out.hasEmitters = true;
+    }
+    // lights (AreaLight.h:123-131)
+    for (uint32_t i = 0; i < d->num_lights; ++i) {
+        const ahip_light& l = d->lights[i];
+        DLight L{};
+        L.type = l.type;
+        if (l.type == AHIP_LIGHT_AREA) {
+            std::memcpy(L.center, l.center, 12);
+            std::memcpy(L.uA, l.u_axis, 12);
+            std::memcpy(L.vA, l.v_axis, 12);
+            std::memcpy(L.Le, l.Le, 12);
+            V c = crs(ld(l.u_axis), ld(l.v_axis));
+            st(L.normal, nrmSafe(c));
+            L.area = lenv(c) * 4.f;
+            L.uLen = lenv(ld(l.u_axis));
+            L.vLen = lenv(ld(l.v_axis));
+        } else if (l.type == AHIP_LIGHT_DOME) {
+            if ((int32_t)i != d->env_light) {
+                err = "a dome light must be the scene's env_light";
+                return AHIP_E_INVALID_ARG;
+            }
+        } else {
+            err = "unknown light type";
+            return AHIP_E_INVALID_ARG;
+        }
+        out.lights.push_back(L);
+    }
+    if (d->env_light >= 0) {
+        if ((uint32_t)d->env_light >= d->num_lights || d->lights[d->env_light].type != AHIP_LIGHT_DOME) {
+            err = "env_light must index a dome light";
+            return AHIP_E_INVALID_ARG;
+        }
+        // DomeLight::buildDistribution (DomeLight.h:228-259); an absent image is the 1x1 white fallback
+        HostDome& D = out.dome;
+        const ahip_light& l = d->lights[d->env_light];
+        D.intensity = l.intensity;
+        std::memcpy(D.rot, l.rot, sizeof(D.rot));
+        std::vector<float> rgb;
+        if (d->env_pixels && d->env_width && d->env_height) {
+            D.W = d->env_width;
+            D.H = d->env_height;
+            rgb.assign(d->env_pixels, d->env_pixels + (size_t)D.W * D.H * 3);
+        } else {
+            D.W = D.H = 1;
+            rgb = {1.f, 1.f, 1.f};
+        }
+        D.px4.resize((size_t)D.W * D.H * 4);
+        for (size_t i = 0; i < (size_t)D.W * D.H; ++i) {
+            D.px4[4 * i] = rgb[3 * i];
+            D.px4[4 * i + 1] = rgb[3 * i + 1];
+            D.px4[4 * i + 2] = rgb[3 * i + 2];
+            D.px4[4 * i + 3] = 0.f;
+        }
+        D.condFunc.resize((size_t)D.W * D.H);
+        D.condCdf.resize((size_t)(D.W + 1) * D.H);
+        D.condInt.resize(D.H);
+        std::vector<float> row(D.W), rowSums(D.H);
+        for (uint32_t r = 0; r < D.H; ++r) {
+            float theta = kPi * (r + 0.5f) / static_cast<float>(D.H);
+            float s = std::sin(theta);
+            for (uint32_t c = 0; c < D.W; ++c) {
+                const float* p = &rgb[((size_t)r * D.W + c) * 3];
+                float lum = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+                row[c] = lum * s;
+            }
+            D.condInt[r] = buildDist(row.data(), D.W, &D.condFunc[(size_t)r * D.W], &D.condCdf[(size_t)r * (D.W + 1)]);
+            rowSums[r] = D.condInt[r];
+        }
+        D.margFunc.resize(D.H);
+        D.margCdf.resize(D.H + 1);
+        D.margInt = buildDist(rowSums.data(), D.H, D.margFunc.data(), D.margCdf.data());
+        out.hasDome = true;
+        (void)k2Pi;
+    }
+    return AHIP_OK;
+}
+
+}  // namespace ahip

@@ -1,0 +1,42 @@
+// host_scene.h — host-side conversion of ahip_scene_desc into the device records.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/anacapa_hip.h"
+#include "device_types.h"
+
+namespace ahip {
+
+struct HaltonTables;
+
+struct HostDome {
+    uint32_t W = 0, H = 0;
+    std::vector<float> px4;          // W*H*4 (rgb + pad)
+    std::vector<float> margFunc, margCdf, condFunc, condCdf, condInt;
+    float margInt = 0.f;
+    float intensity = 1.f;
+    float rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+};
+
+struct HostScene {
+    std::vector<DMat> mats;
+    std::vector<int32_t> meshMat;
+    std::vector<DLight> lights;
+    std::vector<std::vector<float>> texPixels;
+    std::vector<uint32_t> texW, texH;
+    HostDome dome;
+    bool hasDome = false;
+    bool hasEmitters = false;
+};
+
+// Validates the description and builds materials, lights, textures and the dome's sampling
+// distributions.  Returns an AHIP_E_* status.
+int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err);
+void fillHaltonTables(HaltonTables& t);
+int makePinholeCamera(const float from[3], const float at[3], const float up[3], float vfov, uint32_t w, uint32_t h,
+                      ahip_camera* out);
+
+}  // namespace ahip

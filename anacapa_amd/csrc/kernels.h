@@ -1,0 +1,67 @@
+// kernels.h — launch interface between the host context (abi.cpp) and kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/anacapa_hip.h"
+#include "device_sampling.h"
+#include "device_types.h"
+
+namespace ahip {
+
+// One wavefront pass: `samples` consecutive sample indices of every pixel of a region.
+struct PassDesc {
+    DCamera cam;
+    uint32_t x0, y0, w, h;       // region of the film
+    uint32_t tilesX;             // 8x8 tiles across the region
+    uint32_t slotsPerSample;     // tilesX * tilesY * 64
+    uint32_t sampleStart;        // sample index of the pass's first sample
+    uint32_t samples;
+    uint64_t seed;
+    uint32_t maxDepth, minDepth, flags, pad;
+    float4* accum;               // region [w*h]: r, g, b sums and weight
+    float* accumLumSq;           // region [w*h]
+};
+
+// Device queues of the bounce loop (all structure-of-arrays, 16-byte elements).
+struct WaveQueues {
+    float4* ro[2];    // continuation ray origin | path id
+    float4* rd[2];    // direction | sampler dimension (bit 31: specular bounce)
+    float4* beta[2];  // throughput
+    float4* hit;      // t, u, v | leaf-order triangle (~0 = miss)
+    float4* so;       // shadow ray origin | path id
+    float4* sd;       // direction | tMax
+    float4* sc;       // contribution if unoccluded
+    float4* mo;       // MIS ray origin | path id
+    float4* md;       // direction | tMax
+    float4* mc;       // weight; times emitted radiance
+    float4* L;        // per path radiance
+    uint32_t* counters;  // [(maxDepth + 2) * 4]
+    uint32_t* ovfRef;    // traversal stack overflow (entries past the LDS part)
+    float* ovfT;
+    uint32_t stackCap;   // total stack entries available per thread
+    uint32_t cap;        // path capacity
+};
+
+void setHaltonTables(const HaltonTables& t, hipStream_t s);
+uint32_t lds_stack_entries();
+void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+void launchFillF32(float* p, size_t n, float v, hipStream_t s);
+void launchHalton(uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx, const uint32_t* dim,
+                  uint32_t n, float* out, hipStream_t s);
+void tallyCounters(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, hipStream_t s);
+void launchGenerate(const PassDesc& pd, const WaveQueues& q, uint32_t blocks, hipStream_t s);
+void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
+                  hipStream_t s);
+void launchShade(const DScene& sc, const PassDesc& pd, const WaveQueues& q, uint32_t bounce, uint32_t cur,
+                 uint32_t blocks, hipStream_t s);
+void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
+void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
+void launchAccumulate(const PassDesc& pd, const WaveQueues& q, hipStream_t s);
+void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
+                      uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,
+                      uint32_t* counts, hipStream_t s);
+void launchCount(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, int kind, uint32_t blocks,
+                 unsigned long long* sums, hipStream_t s);
+
+}  // namespace ahip

@@ -1,0 +1,519 @@
+// kernels.hip — gfx950 kernels of the wavefront Path+NEE integrator.
+//
+// One camera sample is one path.  A render is a sequence of passes over up to `cap` paths;
+// each pass runs, for bounce = 0..maxDepth:
+//   extend  — closest hit of every live path's ray (queue compacted by the previous shade)
+//   shade   — PathIntegrator::Li's per-bounce body (src/integrator/PathIntegrator.cpp:80-161):
+//             environment on a miss, emission, uniform light pick, the two NEE strategies of
+//             estimateDirect (PathIntegrator.cpp:166-265) emitted as queued rays, continuation
+//             BSDF sample and Russian roulette; survivors are appended to the next queue
+//   shadow  — any-hit of the NEE light-sample rays; unoccluded ones add their contribution
+//   mis     — the BSDF-sampled NEE rays: closest hit (or any hit when no surface emits) and
+//             the MIS-weighted emitter / environment radiance
+// then `accumulate` folds the pass's per-path radiance into the pixel sums exactly as
+// PathIntegrator::renderTile does (finite samples only, luminance^2, weight = sample count).
+//
+// Queues are appended with one atomic per wavefront: a 64-lane ballot, a popcount prefix and a
+// broadcast of the base slot.  Every kernel is a block-uniform grid-stride loop whose trip
+// count comes from a device counter, so launch shapes are static and the host never reads a
+// queue length (the same launches can be replayed from a HIP graph).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/anacapa_hip.h"
+#include "device_math.h"
+#include "device_sampling.h"
+#include "device_shading.h"
+#include "device_traverse.h"
+#include "device_types.h"
+#include "kernels.h"
+
+namespace ahip {
+
+__constant__ HaltonTables c_halton;
+
+// Counter slots per bounce: [0] continuation rays entering this bounce, [1] shadow rays,
+// [2] MIS rays, [3] stack-overflow flag.
+__device__ __forceinline__ uint32_t* ctr(const WaveQueues& q, uint32_t bounce, uint32_t k) {
+    return q.counters + bounce * 4u + k;
+}
+
+__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
+    uint64_t mask = __ballot(want ? 1 : 0);
+    if (mask == 0ull) return 0u;
+    uint32_t lane = __lane_id();
+    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    uint64_t below = (lane == 0u) ? 0ull : (mask & (~0ull >> (64u - lane)));
+    return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+
+// p0 -> pixel of this pass (8x8 pixel tiles, 64 consecutive paths = one tile).
+__device__ __forceinline__ bool pathPixel(const PassDesc& pd, uint32_t p0, uint32_t& px, uint32_t& py,
+                                          uint32_t& sample) {
+    uint32_t slot = p0 % pd.slotsPerSample, j = p0 / pd.slotsPerSample;
+    uint32_t tile = slot >> 6, lane = slot & 63u;
+    uint32_t tx = tile % pd.tilesX, ty = tile / pd.tilesX;
+    uint32_t lx = tx * 8u + (lane & 7u), ly = ty * 8u + (lane >> 3);
+    px = pd.x0 + lx;
+    py = pd.y0 + ly;
+    sample = pd.sampleStart + j;
+    return lx < pd.w && ly < pd.h;
+}
+
+// --------------------------------------------------------------------------- utility kernels
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_halton_eval(uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx,
+                              const uint32_t* dim, uint32_t n, float* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = haltonSample(c_halton, seed, px[i], py[i], idx[i], dim[i]);
+}
+
+// --------------------------------------------------------------------------- generate
+// Camera rays: HaltonSampler::startPixelSample + get2D (jitter) + get2D (lens) + get1D (time)
+// and Camera::generateRay for a pinhole (PathIntegrator.cpp:34-38; IIntegrator.h:90-108).
+__global__ __launch_bounds__(256) void k_generate(PassDesc pd, WaveQueues q) {
+    const uint32_t total = pd.slotsPerSample * pd.samples;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+        uint32_t p0 = base + threadIdx.x;
+        uint32_t px = 0, py = 0, s = 0;
+        bool valid = p0 < total && pathPixel(pd, p0, px, py, s);
+        V3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f};
+        if (valid) {
+            float jx = haltonSample(c_halton, pd.seed, px, py, s, 0u);
+            float jy = haltonSample(c_halton, pd.seed, px, py, s, 1u);
+            float fs = ((float)px + jx) / (float)pd.cam.w;
+            float ft = ((float)(pd.cam.h - 1u - py) + jy) / (float)pd.cam.h;
+            V3 target = ld3(pd.cam.llc) + ld3(pd.cam.hor) * fs + ld3(pd.cam.ver) * ft;
+            o = ld3(pd.cam.origin);
+            d = normalize(target - o);
+            q.L[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        uint32_t slot = waveAppend(ctr(q, 0, 0), valid);
+        if (valid) {
+            q.ro[0][slot] = f4(o, __uint_as_float(p0));
+            q.rd[0][slot] = f4(d, __uint_as_float(5u));   // dims 0..4 consumed by the camera sample
+            q.beta[0][slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- extend
+template <uint32_t BS, uint32_t K>
+__global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+    __shared__ uint32_t lref[K * BS];
+    __shared__ float lt[K * BS];
+    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, 0);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        float4 a = q.ro[cur][i], b = q.rd[cur][i];
+        TRay r = makeTRay(v3(a), v3(b), 1e-4f, __int_as_float(0x7f800000));
+        float t = 0.f, u = 0.f, v = 0.f;
+        uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        q.hit[i] = make_float4(t, u, v, __uint_as_float(h));
+    }
+}
+
+// --------------------------------------------------------------------------- shade
+__global__ __launch_bounds__(256) void k_shade(DScene sc, PassDesc pd, WaveQueues q, uint32_t bounce,
+                                               uint32_t cur) {
+    const uint32_t count = *ctr(q, bounce, 0);
+    const uint32_t nxt = cur ^ 1u;
+    const float nL = (float)sc.numLights;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < count;
+        bool cont = false, wantShadow = false, wantMis = false;
+        uint32_t p0 = 0, dim = 0;
+        V3 beta{0.f, 0.f, 0.f}, rayO{0.f, 0.f, 0.f}, rayD{0.f, 0.f, 0.f};
+        V3 shO{0.f, 0.f, 0.f}, shD{0.f, 0.f, 0.f}, shC{0.f, 0.f, 0.f};
+        float shTmax = 0.f;
+        V3 miO{0.f, 0.f, 0.f}, miD{0.f, 0.f, 0.f}, miC{0.f, 0.f, 0.f};
+        bool specularNext = false;
+        if (active) {
+            float4 ra = q.ro[cur][i], rb = q.rd[cur][i], bb = q.beta[cur][i], hh = q.hit[i];
+            p0 = __float_as_uint(ra.w);
+            uint32_t meta = __float_as_uint(rb.w);
+            dim = meta & 0x7fffffffu;
+            bool specular = (meta >> 31) != 0u;
+            beta = v3(bb);
+            V3 dir = v3(rb);
+            uint32_t px, py, sidx;
+            pathPixel(pd, p0, px, py, sidx);
+            uint32_t tri = __float_as_uint(hh.w);
+            V3 Ladd{0.f, 0.f, 0.f};
+            if (tri == ~0u) {
+                if (bounce == 0u || specular) Ladd = beta * envLe(sc, dir);   // PathIntegrator.cpp:83-91
+            } else {
+                // fillSurfaceInteraction (BVHBackend.cpp:285-317)
+                const DTri* T = sc.tris + tri;
+                float tt = hh.x, u = hh.y, v = hh.z, w = 1.f - u - v;
+                V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2), ng = ld3(T->n);
+                uint32_t meshID = T->meshID;
+                const DTriShade* SH = sc.shade + tri;
+                V3 p = v0 + e1 * u + e2 * v;
+                V3 n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
+                if (dot(n, ng) < 0.f) n = -n;
+                V2 uv{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v, SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
+                (void)tt;
+                int32_t mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
+                if (mi >= 0) {
+                    const DMat& m = sc.mats[mi];
+                    V3 wo = -dir;
+                    Frame fr = makeFrame(n, ng, dir);
+                    if (m.hasLe && (bounce == 0u || specular)) Ladd = beta * ld3(m.Le);   // :128-132
+                    if (!m.isDelta && sc.numLights > 0u) {                                // :134-144
+                        float ul = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        uint32_t li = (uint32_t)(ul * nL);
+                        li = li < sc.numLights - 1u ? li : sc.numLights - 1u;
+                        const DLight& L = sc.lights[li];
+                        Frame fe = makeFrame(n, ng, -wo);   // estimateDirect's ShadingContext(si, -wo)
+                        // light-sampling strategy (PathIntegrator.cpp:176-198)
+                        V2 u2;
+                        u2.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        u2.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        LSample ls = lightSample(sc, L, p, u2);
+                        if (ls.pdf > 0.f && !isBlack(ls.Li)) {
+                            float bpdf;
+                            V3 bf = matEval(sc, m, fe, uv, wo, ls.wi, bpdf);
+                            if (!isBlack(bf)) {
+                                // spawnRayTo(si.p, si.n, si.p + wi*dist) (Types.h:277-284)
+                                V3 target = p + ls.wi * ls.dist;
+                                V3 sd = target - p;
+                                float dist = len(sd);
+                                sd = sd * (1.f / dist);
+                                V3 off = n * 1e-4f;
+                                if (dot(sd, n) < 0.f) off = -off;
+                                float f = ls.pdf, g = bpdf;
+                                float wgt = (f * f) / (f * f + g * g);
+                                float cosI = fabsf(dot(ls.wi, n));
+                                V3 Ld = bf * ls.Li * cosI * wgt / ls.pdf;
+                                shO = p + off;
+                                shD = sd;
+                                shTmax = dist * (1.f - 1e-4f);
+                                shC = beta * Ld * nL;
+                                wantShadow = true;
+                            }
+                        }
+                        // BSDF-sampling strategy (PathIntegrator.cpp:200-262); g++ evaluates
+                        // sample(ctx, wo, get2D(), get1D()) right to left: uComponent first.
+                        float uc = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        V2 ub;
+                        ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        BSample bs = matSample(sc, m, fe, uv, wo, ub, uc);
+                        if (bsValid(bs)) {
+                            float lp = lightPdf(sc, L, p, bs.wi);
+                            if (lp > 0.f) {
+                                float f = bs.pdf, g = lp;
+                                float wgt = bs.delta ? 1.f : (f * f) / (f * f + g * g);
+                                V3 off = ng * 1e-4f;
+                                if (dot(bs.wi, ng) < 0.f) off = -off;
+                                float cosI = (pd.flags & AHIP_FLAG_FIX_DOUBLE_COSINE) ? 1.f : fabsf(dot(bs.wi, n));
+                                miO = p + off;
+                                miD = bs.wi;
+                                miC = beta * (bs.f * cosI * wgt / bs.pdf) * nL;
+                                wantMis = true;
+                            }
+                        }
+                    }
+                    if (bounce < pd.maxDepth) {
+                        // continuation (PathIntegrator.cpp:146-160)
+                        float uc = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        V2 ub;
+                        ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                        BSample bs = matSample(sc, m, fr, uv, wo, ub, uc);
+                        if (bsValid(bs)) {
+                            specularNext = bs.delta;
+                            beta = beta * (bs.f / bs.pdf);
+                            V3 off = n * 1e-4f;
+                            if (dot(bs.wi, n) < 0.f) off = -off;
+                            rayO = p + off;
+                            rayD = bs.wi;
+                            cont = true;
+                            if (bounce >= pd.minDepth) {
+                                float qq = 1.f - stdmin(maxc(beta), 0.95f);
+                                if (haltonSample(c_halton, pd.seed, px, py, sidx, dim++) < qq) cont = false;
+                                else beta = beta * (1.f / (1.f - qq));
+                            }
+                        }
+                    }
+                }
+            }
+            if (!isBlack(Ladd)) {
+                float4 l = q.L[p0];
+                q.L[p0] = make_float4(l.x + Ladd.x, l.y + Ladd.y, l.z + Ladd.z, 0.f);
+            }
+        }
+        uint32_t sc0 = waveAppend(ctr(q, bounce + 1u, 0), cont);
+        if (cont) {
+            q.ro[nxt][sc0] = f4(rayO, __uint_as_float(p0));
+            q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? 0x80000000u : 0u)));
+            q.beta[nxt][sc0] = f4(beta, 0.f);
+        }
+        uint32_t sc1 = waveAppend(ctr(q, bounce, 1), wantShadow);
+        if (wantShadow) {
+            q.so[sc1] = f4(shO, __uint_as_float(p0));
+            q.sd[sc1] = f4(shD, shTmax);
+            q.sc[sc1] = f4(shC, 0.f);
+        }
+        uint32_t sc2 = waveAppend(ctr(q, bounce, 2), wantMis);
+        if (wantMis) {
+            q.mo[sc2] = f4(miO, __uint_as_float(p0));
+            q.md[sc2] = f4(miD, 1e10f);
+            q.mc[sc2] = f4(miC, 0.f);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- shadow (any hit)
+template <uint32_t BS, uint32_t K>
+__global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t bounce) {
+    __shared__ uint32_t lref[K * BS];
+    __shared__ float lt[K * BS];
+    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, 1);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        float4 a = q.so[i], b = q.sd[i];
+        TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
+        float t, u, v;
+        uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        if (h == ~0u) {
+            uint32_t p0 = __float_as_uint(a.w);
+            float4 c = q.sc[i], l = q.L[p0];
+            q.L[p0] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- MIS (BSDF-sampled NEE)
+template <uint32_t BS, uint32_t K>
+__global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bounce) {
+    __shared__ uint32_t lref[K * BS];
+    __shared__ float lt[K * BS];
+    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, 2);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        float4 a = q.mo[i], b = q.md[i];
+        V3 d = v3(b);
+        TRay r = makeTRay(v3(a), d, 1e-4f, b.w);
+        float t, u, v;
+        V3 Li{0.f, 0.f, 0.f};
+        if (sc.hasEmitters) {
+            uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            if (h == ~0u) Li = envLe(sc, d);
+            else {
+                uint32_t mesh = sc.tris[h].meshID;
+                int32_t mi = mesh < sc.numMeshes ? sc.meshMat[mesh] : -1;
+                if (mi >= 0 && sc.mats[mi].hasLe) Li = ld3(sc.mats[mi].Le);
+            }
+        } else {
+            // no surface emits: only an escaping ray carries radiance, so any hit decides
+            uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            if (h == ~0u) Li = envLe(sc, d);
+        }
+        if (!isBlack(Li)) {
+            uint32_t p0 = __float_as_uint(a.w);
+            float4 c = q.mc[i], l = q.L[p0];
+            V3 add = v3(c) * Li;
+            q.L[p0] = make_float4(l.x + add.x, l.y + add.y, l.z + add.z, 0.f);
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- accumulate
+// PathIntegrator::renderTile's per-pixel sums (PathIntegrator.cpp:27-57) for this pass.
+__global__ __launch_bounds__(256) void k_accumulate(PassDesc pd, WaveQueues q) {
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < pd.slotsPerSample;
+         slot += gridDim.x * blockDim.x) {
+        uint32_t px, py, s;
+        if (!pathPixel(pd, slot, px, py, s)) continue;
+        V3 acc{0.f, 0.f, 0.f};
+        float lsq = 0.f;
+        for (uint32_t j = 0; j < pd.samples; ++j) {
+            float4 l = q.L[j * pd.slotsPerSample + slot];
+            V3 L = v3(l);
+            if (finite3(L)) {
+                acc = acc + L;
+                float lum = luminance(L);
+                lsq += lum * lum;
+            }
+        }
+        size_t o = (size_t)(py - pd.y0) * pd.w + (px - pd.x0);
+        float4 a = pd.accum[o];
+        pd.accum[o] = make_float4(a.x + acc.x, a.y + acc.y, a.z + acc.z, a.w + (float)pd.samples);
+        pd.accumLumSq[o] += lsq;
+    }
+}
+
+// --------------------------------------------------------------------------- ray-batch entry points
+template <uint32_t BS, uint32_t K, bool ANY>
+__global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays, uint32_t n, ahip_hit* hits,
+                                                    uint8_t* occ, uint32_t* ovfRef, float* ovfT, uint32_t cap,
+                                                    uint32_t* overflow, uint32_t* counts) {
+    __shared__ uint32_t lref[K * BS];
+    __shared__ float lt[K * BS];
+    TStack S{lref, lt, ovfRef, ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS, cap};
+    for (uint32_t base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= n) break;
+        const float* q = rays + 8 * (size_t)i;
+        TRay r = makeTRay(V3{q[0], q[1], q[2]}, V3{q[3], q[4], q[5]}, q[6], q[7]);
+        float t = 0.f, u = 0.f, v = 0.f;
+        TraceCounters tc{0, 0};
+        uint32_t h;
+        if (counts) h = traverse<ANY, true>(sc, r, S, t, u, v, tc, overflow);
+        else h = traverse<ANY, false>(sc, r, S, t, u, v, tc, overflow);
+        if (counts) { counts[2 * i] = tc.nodes; counts[2 * i + 1] = tc.tris; }
+        if (ANY) {
+            occ[i] = h != ~0u ? 1 : 0;
+        } else {
+            ahip_hit o;
+            if (h == ~0u) o = ahip_hit{0.f, 0.f, 0.f, ~0u, ~0u, ~0u};
+            else o = ahip_hit{t, u, v, sc.tris[h].meshID, sc.tris[h].primID, sc.tris[h].commitIndex};
+            hits[i] = o;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------- traffic counting
+// Counter-enabled copies of extend / shadow / mis: per-ray node and triangle visits summed with
+// one atomic per wavefront.  Used by a separate, untimed pass.
+template <uint32_t BS, uint32_t K, int KIND>
+__global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur,
+                                              unsigned long long* sums) {
+    __shared__ uint32_t lref[K * BS];
+    __shared__ float lt[K * BS];
+    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, KIND);
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        TraceCounters tc{0, 0};
+        if (i < count) {
+            float4 a, b;
+            if (KIND == 0) { a = q.ro[cur][i]; b = q.rd[cur][i]; b.w = __int_as_float(0x7f800000); }
+            else if (KIND == 1) { a = q.so[i]; b = q.sd[i]; }
+            else { a = q.mo[i]; b = q.md[i]; }
+            TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
+            float t, u, v;
+            bool any = KIND == 1 || (KIND == 2 && !sc.hasEmitters);
+            if (any) traverse<true, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            else traverse<false, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        }
+        unsigned long long nodes = tc.nodes, trisT = tc.tris;
+        for (int off = 32; off > 0; off >>= 1) {
+            nodes += __shfl_down(nodes, off);
+            trisT += __shfl_down(trisT, off);
+        }
+        if (__lane_id() == 0u) {
+            atomicAdd(&sums[0], nodes);
+            atomicAdd(&sums[1], trisT);
+        }
+    }
+}
+
+// Per-pass ray totals: [0] continuation rays, [1] shadow rays, [2] MIS rays, [3] stack overflows.
+__global__ void k_tally(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals) {
+    if (blockIdx.x != 0u || threadIdx.x >= 4u) return;
+    unsigned long long s = 0ull;
+    for (uint32_t b = 0; b <= maxDepth; ++b) s += counters[b * 4u + threadIdx.x];
+    totals[threadIdx.x] += s;
+}
+
+// --------------------------------------------------------------------------- host-side launchers
+void setHaltonTables(const HaltonTables& t, hipStream_t s) {
+    hipMemcpyToSymbolAsync(HIP_SYMBOL(c_halton), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
+}
+
+constexpr uint32_t kTraceBS = 128;
+constexpr uint32_t kStackK = 16;
+
+uint32_t lds_stack_entries() { return kStackK; }
+
+void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks > 1024u) blocks = 1024u;
+    if (blocks == 0u) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launchFillF32(float* p, size_t n, float v, hipStream_t s) {
+    size_t blocks = (n + 255u) / 256u;
+    if (blocks > 4096u) blocks = 4096u;
+    if (blocks == 0u) return;
+    hipLaunchKernelGGL(k_fill_f32, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+}
+void launchHalton(uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx, const uint32_t* dim,
+                  uint32_t n, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_halton_eval, dim3((n + 255u) / 256u), dim3(256), 0, s, seed, px, py, idx, dim, n, out);
+}
+void tallyCounters(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, hipStream_t s) {
+    hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, counters, maxDepth, totals);
+}
+void launchGenerate(const PassDesc& pd, const WaveQueues& q, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, pd, q);
+}
+void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
+                  hipStream_t s) {
+    hipLaunchKernelGGL((k_extend<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+}
+void launchShade(const DScene& sc, const PassDesc& pd, const WaveQueues& q, uint32_t bounce, uint32_t cur,
+                 uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, pd, q, bounce, cur);
+}
+void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+}
+void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_mis<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+}
+void launchAccumulate(const PassDesc& pd, const WaveQueues& q, hipStream_t s) {
+    uint32_t blocks = (pd.slotsPerSample + 255u) / 256u;
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, pd, q);
+}
+void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
+                      uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,
+                      uint32_t* counts, hipStream_t s) {
+    if (any)
+        hipLaunchKernelGGL((k_trace_batch<kTraceBS, kStackK, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, rays, n,
+                           hits, occ, ovfRef, ovfT, cap, overflow, counts);
+    else
+        hipLaunchKernelGGL((k_trace_batch<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, rays, n,
+                           hits, occ, ovfRef, ovfT, cap, overflow, counts);
+}
+void launchCount(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, int kind, uint32_t blocks,
+                 unsigned long long* sums, hipStream_t s) {
+    if (kind == 0)
+        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 0>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
+    else if (kind == 1)
+        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 1>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
+    else
+        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 2>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
+}
+
+}  // namespace ahip

@@ -1,0 +1,176 @@
+"""Python mirror of the reference's GPU backend interface, over the C ABI.
+
+``HipPathIntegrator`` follows CudaPathIntegrator (src/gpu/cuda/CudaPathIntegrator.h:21-52)
+and IIntegrator (include/anacapa/integrator/IIntegrator.h:172-198): construct, ``is_valid``,
+``prepare(scene)``, ``clear_accum``, ``render_frame`` (adds into a persistent device buffer),
+``render_tile`` and the acceleration-structure batch calls (IAccelerationStructure::traceBatch /
+occludedBatch).  Errors raise ``BackendError`` carrying the ABI status and message; the
+renderer-side fallback to the CPU integrator is exercised in the tests through
+``is_valid`` / ``render_frame`` failures exactly as RenderSession does (RenderSession.cpp:441-464,
+546-547).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+
+class BackendError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"ahip status {status}: {message}")
+        self.status = status
+
+
+class HipPathIntegrator:
+    def __init__(self, device: int = 0, lib: C.CDLL | None = None):
+        self.lib = lib or A.load_library()
+        self._ctx = C.c_void_p()
+        self.create_status = self.lib.ahip_create(device, C.byref(self._ctx))
+        self._keep = None
+        self.scene = None
+        self.width = self.height = 0
+
+    # -------------------------------------------------------------- lifetime
+    def close(self):
+        if self._ctx:
+            self.lib.ahip_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def is_valid(self) -> bool:
+        return bool(self._ctx) and bool(self.lib.ahip_is_valid(self._ctx))
+
+    def name(self) -> str:
+        return self.lib.ahip_device_name(self._ctx).decode()
+
+    def backend_type(self) -> str:
+        return self.lib.ahip_backend_type().decode()
+
+    def last_error(self) -> str:
+        return self.lib.ahip_last_error(self._ctx).decode()
+
+    def _check(self, status: int):
+        if status != A.AHIP_OK:
+            raise BackendError(status, self.last_error())
+
+    # -------------------------------------------------------------- scene / settings
+    def prepare(self, scene, camera: A.Camera | None = None, max_depth=None, min_depth=None, seed=None,
+                flags: int = 0):
+        desc, keep = scene.desc()
+        self._check(self.lib.ahip_prepare(self._ctx, C.byref(desc)))
+        self._keep = keep
+        self.scene = scene
+        if camera is None:
+            camera = self.pinhole(scene.cam_from, scene.cam_at, scene.cam_up, scene.vfov, scene.width, scene.height)
+        self.set_camera(camera)
+        self.set_params(scene.max_depth if max_depth is None else max_depth,
+                        scene.min_depth if min_depth is None else min_depth,
+                        scene.seed if seed is None else seed, flags)
+
+    def pinhole(self, frm, at, up, vfov, w, h) -> A.Camera:
+        cam = A.Camera()
+        f, a, u = (np.ascontiguousarray(v, np.float32) for v in (frm, at, up))
+        self._check(self.lib.ahip_camera_pinhole(f.ctypes.data_as(A.F32P), a.ctypes.data_as(A.F32P),
+                                                 u.ctypes.data_as(A.F32P), vfov, w, h, C.byref(cam)))
+        return cam
+
+    def set_camera(self, cam: A.Camera):
+        self._check(self.lib.ahip_set_camera(self._ctx, C.byref(cam)))
+        self.camera = cam
+        self.width, self.height = cam.width, cam.height
+
+    def set_params(self, max_depth=8, min_depth=2, seed=0, flags=0):
+        p = A.RenderParams(max_depth, min_depth, seed, flags, 0)
+        self._check(self.lib.ahip_set_params(self._ctx, C.byref(p)))
+
+    def set_launch(self, max_paths=0, tile=0):
+        self._check(self.lib.ahip_set_launch(self._ctx, max_paths, tile))
+
+    def bvh_info(self):
+        n, t, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self.lib.ahip_bvh_info(self._ctx, C.byref(n), C.byref(t), C.byref(d)))
+        return n.value, t.value, d.value
+
+    def bvh_dump(self):
+        """Reference-layout binary BVH (8 words per node) and the reference prim index list."""
+        nodes_flat, ntris, _ = self.bvh_info()
+        # the binary tree has 2*interior+1 nodes when the root is interior, else one leaf
+        nbin = 2 * nodes_flat + 1 if nodes_flat else (1 if ntris else 0)
+        nodes = np.zeros((nbin, 8), np.uint32)
+        prims = np.zeros(2 * ntris, np.uint32)
+        self._check(self.lib.ahip_bvh_dump(self._ctx, A.ptr(nodes), A.ptr(prims)))
+        return nodes, prims
+
+    # -------------------------------------------------------------- rendering
+    def clear_accum(self):
+        self._check(self.lib.ahip_clear_accum(self._ctx))
+
+    def render_frame(self, sample_start: int, sample_count: int, width=None, height=None):
+        self._check(self.lib.ahip_render_frame(self._ctx, width or self.width, height or self.height,
+                                               sample_start, sample_count))
+
+    def download_accum(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 5), np.float32)
+        self._check(self.lib.ahip_download_accum(self._ctx, A.ptr(out)))
+        return out
+
+    def render_tile(self, x0, y0, w, h, sample_start, sample_count) -> np.ndarray:
+        w = min(w, self.width - x0)
+        h = min(h, self.height - y0)
+        out = np.zeros((h, w, 5), np.float32)
+        self._check(self.lib.ahip_render_tile(self._ctx, x0, y0, w, h, sample_start, sample_count, A.ptr(out)))
+        return out
+
+    def render_batch(self, views):
+        arr = (A.View * len(views))(*views)
+        self._check(self.lib.ahip_render_batch(self._ctx, arr, len(views)))
+
+    # -------------------------------------------------------------- accel / sampler boundary
+    def trace_rays(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros(len(rays), A.HIT_DTYPE)
+        self._check(self.lib.ahip_trace_rays(self._ctx, A.ptr(rays), len(rays), A.ptr(hits)))
+        return hits
+
+    def occluded_rays(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(len(rays), np.uint8)
+        self._check(self.lib.ahip_occluded_rays(self._ctx, A.ptr(rays), len(rays), A.ptr(out)))
+        return out
+
+    def halton(self, seed, px, py, idx, dim) -> np.ndarray:
+        px, py, idx, dim = (np.ascontiguousarray(a, np.uint32) for a in (px, py, idx, dim))
+        out = np.zeros(len(px), np.float32)
+        self._check(self.lib.ahip_halton_eval(self._ctx, seed, A.ptr(px), A.ptr(py), A.ptr(idx), A.ptr(dim),
+                                              len(px), A.ptr(out)))
+        return out
+
+    # -------------------------------------------------------------- measurement / testing
+    def stats(self) -> dict:
+        s = A.Stats()
+        self._check(self.lib.ahip_get_stats(self._ctx, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in A.Stats._fields_}
+
+    def reset_stats(self):
+        self._check(self.lib.ahip_reset_stats(self._ctx))
+
+    def count_traffic(self, sample_start, sample_count):
+        self._check(self.lib.ahip_count_traffic(self._ctx, self.width, self.height, sample_start, sample_count))
+        return self.stats()
+
+    def fail_next_launch(self):
+        self._check(self.lib.ahip_fail_next_launch(self._ctx))
+
+
+def resolve(accum: np.ndarray) -> np.ndarray:
+    """Film pixel value = pre-weighted sum / weight (PixelAccumulator::resolve, Film.h:46-55)."""
+    w = accum[..., 3:4]
+    return np.where(w > 0, accum[..., :3] / np.maximum(w, 1e-30), 0.0)

@@ -1,0 +1,172 @@
+"""HIP path vs the oracle (and the reference vectors) through the C ABI, on the GPU.
+
+Bars: bit-exact for the sampler, BVH topology and closest-hit records (ids, t, u, v) against the
+oracle, whose traversal is a fixed IEEE program; hit ids equal to the reference's except on
+recorded ties (|dt|/t < 1e-6); per-pixel radiance at identical sample indices within 1e-3
+relative (FP32) on at least 99% of pixels, with the image means within 1e-3.
+"""
+import numpy as np
+import pytest
+
+from anacapa_amd import scenes
+from anacapa_amd import _abi as A
+from rays import random_rays, camera_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def _hits_equal(a, b):
+    for k in ("mesh_id", "prim_id"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("t", "u", "v"):
+        m = a["mesh_id"] != 0xFFFFFFFF
+        assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
+
+
+def _radiance_close(got, ref, rel=1e-3, frac=0.99):
+    g = got[..., :3].astype(np.float64)
+    r = ref[..., :3].astype(np.float64)
+    assert np.array_equal(got[..., 3], ref[..., 3])          # sample counts
+    scale = max(np.abs(r).mean(), 1e-6)
+    err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
+    ok = err <= rel
+    assert ok.mean() >= frac, f"{(~ok).sum()} of {ok.size} pixels differ by more than {rel} (max {err.max()})"
+    assert abs(g.mean() / r.mean() - 1) < rel
+
+
+# ---------------------------------------------------------------- sampler
+def test_halton_dense_grid_bit_exact(gpu, oracle):
+    px, py, idx, dim = np.meshgrid(np.arange(0, 37, 5), np.arange(0, 29, 7), np.array([0, 1, 2, 7, 63, 64, 1000, 65535,
+                                                                                         2 ** 31 + 5]),
+                                   np.arange(0, 176), indexing="ij")
+    px, py, idx, dim = (a.reshape(-1).astype(np.uint32) for a in (px, py, idx, dim))
+    print(f"[halton] {len(px)} values", flush=True)
+    for seed in (0, 7, 0xdeadbeef12345):
+        g = gpu.halton(seed, px, py, idx, dim)
+        o = oracle.halton(seed, px, py, idx, dim)
+        assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), seed
+
+
+def test_halton_matches_reference_vectors(gpu, golden):
+    trip, vals = golden["halton_trip"], golden["halton_vals"]
+    n = vals.shape[1]
+    g = gpu.halton(0, np.repeat(trip[:, 0], n), np.repeat(trip[:, 1], n), np.repeat(trip[:, 2], n),
+                   np.tile(np.arange(n), len(trip))).reshape(vals.shape)
+    assert np.array_equal(g.view(np.uint32), vals.view(np.uint32))
+
+
+# ---------------------------------------------------------------- acceleration structure
+@pytest.mark.parametrize("name", ["cornell", "soup", "small_ggx"])
+def test_bvh_and_hits_bit_exact_vs_oracle(gpu, oracle, name):
+    sc = {"cornell": scenes.cornell, "soup": scenes.triangle_soup, "small_ggx": scenes.small_ggx}[name]()
+    print(f"[bvh] {name}: {sc.num_triangles} triangles", flush=True)
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    n_g, p_g = gpu.bvh_dump()
+    n_o, p_o = os_.bvh()
+    assert np.array_equal(n_g, n_o) and np.array_equal(p_g, p_o)
+    rays = np.concatenate([random_rays(sc, 6000, seed=11), camera_rays(os_.cam, 2000, seed=12)])
+    _hits_equal(gpu.trace_rays(rays), os_.trace(rays))
+    assert np.array_equal(gpu.occluded_rays(rays), os_.occluded(rays))
+    short = rays.copy()
+    short[:, 7] = np.random.default_rng(3).uniform(0.01, 2.0, len(rays)).astype(np.float32)
+    _hits_equal(gpu.trace_rays(short), os_.trace(short))
+    assert np.array_equal(gpu.occluded_rays(short), os_.occluded(short))
+
+
+@pytest.mark.parametrize("name", ["cornell", "soup"])
+def test_hits_match_reference_vectors(gpu, golden, name):
+    sc = scenes.cornell() if name == "cornell" else scenes.triangle_soup()
+    gpu.prepare(sc)
+    h = gpu.trace_rays(golden[f"{name}_rays"])
+    t_ref, mesh, prim = golden[f"{name}_hit_t"], golden[f"{name}_hit_mesh"], golden[f"{name}_hit_prim"]
+    assert np.array_equal(h["mesh_id"] == 0xFFFFFFFF, mesh == 0xFFFFFFFF)
+    diff = (h["mesh_id"] != mesh) | (h["prim_id"] != prim)
+    for i in np.nonzero(diff)[0]:
+        assert abs(h["t"][i] - t_ref[i]) <= 1e-6 * abs(t_ref[i])
+    assert diff.sum() <= len(mesh) // 100
+    assert np.array_equal(gpu.occluded_rays(golden[f"{name}_rays"]), golden[f"{name}_occ"])
+
+
+def test_empty_and_degenerate_inputs(gpu):
+    sc = scenes.cornell(width=16, height=16)
+    gpu.prepare(sc)
+    assert len(gpu.trace_rays(np.zeros((0, 8), np.float32))) == 0
+    # a ray starting outside and pointing away misses; a ray with tMax < tMin misses
+    r = np.array([[5, 5, 5, 1, 0, 0, 1e-4, np.inf], [0.5, 0.5, -1, 0, 0, 1, 1e-4, 1e-5]], np.float32)
+    h = gpu.trace_rays(r)
+    assert (h["mesh_id"] == 0xFFFFFFFF).all()
+
+
+# ---------------------------------------------------------------- integrator
+def test_cornell_radiance_vs_oracle_and_reference(gpu, oracle, golden):
+    sc = scenes.cornell()
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    got = gpu.render_tile(200, 220, 48, 40, 0, 8)
+    _radiance_close(got, os_.render_tile(200, 220, 48, 40, 0, 8))
+    _radiance_close(got, golden["cornell_tile_200_220_48x40_s0_n8"])
+    gpu.set_params(max_depth=3, min_depth=2, seed=0)
+    got = gpu.render_tile(0, 0, 32, 32, 3, 5)
+    _radiance_close(got, golden["cornell_tile_0_0_32x32_s3_n5_d3"])
+
+
+def test_small_ggx_radiance_vs_oracle(gpu, oracle):
+    sc = scenes.small_ggx()
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    got = gpu.render_tile(0, 0, sc.width, sc.height, 0, 4)
+    ref = os_.render_tile(0, 0, sc.width, sc.height, 0, 4)
+    _radiance_close(got, ref)
+
+
+def test_render_frame_accumulates_disjoint_sample_ranges(gpu):
+    sc = scenes.cornell(width=64, height=48)
+    gpu.prepare(sc)
+    gpu.clear_accum()
+    gpu.render_frame(0, 3)
+    gpu.render_frame(3, 2)
+    a = gpu.download_accum()
+    gpu.clear_accum()
+    gpu.render_frame(0, 5)
+    b = gpu.download_accum()
+    assert np.array_equal(a[..., 3], np.full((48, 64), 5.0, np.float32))
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+    t = gpu.render_tile(0, 0, 64, 48, 0, 5)
+    np.testing.assert_allclose(t, b, rtol=1e-5, atol=1e-6)
+
+
+def test_small_pass_capacity_gives_same_image(gpu):
+    sc = scenes.cornell(width=40, height=24)
+    gpu.prepare(sc)
+    a = gpu.render_tile(0, 0, 40, 24, 0, 6)
+    gpu.set_launch(max_paths=1000)
+    b = gpu.render_tile(0, 0, 40, 24, 0, 6)
+    gpu.set_launch(max_paths=1 << 22)
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-7)
+
+
+def test_injected_failure_then_recovery(gpu):
+    """renderFrame failing must surface as an error (RenderSession then renders tiles on the CPU,
+    RenderSession.cpp:546-547) and leave the backend usable."""
+    from anacapa_amd.integrator import BackendError
+    sc = scenes.cornell(width=16, height=16)
+    gpu.prepare(sc)
+    gpu.fail_next_launch()
+    with pytest.raises(BackendError) as e:
+        gpu.render_frame(0, 1)
+    assert e.value.status == A.AHIP_E_INJECTED
+    gpu.clear_accum()
+    gpu.render_frame(0, 1)
+    assert gpu.download_accum()[..., 3].min() == 1.0
+
+
+def test_unsupported_material_is_rejected(gpu):
+    from anacapa_amd.integrator import BackendError
+    sc = scenes.cornell(width=16, height=16)
+    m = scenes.standard_surface((1, 1, 1))
+    m.transmission = 1.0
+    sc.materials[0] = m
+    with pytest.raises(BackendError) as e:
+        gpu.prepare(sc)
+    assert e.value.status == A.AHIP_E_UNSUPPORTED
