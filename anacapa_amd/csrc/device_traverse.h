@@ -64,20 +64,31 @@ __device__ __forceinline__ bool slab(const float* bmin, const float* bmax, const
     return tmax > tmin;
 }
 
+// Contraction-free vector helpers for the intersection arithmetic (the pragma is lexical, so the
+// general helpers of device_math.h, which may fuse, are not used here).
+__device__ __forceinline__ float dotNC(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ V3 crossNC(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
 __device__ __forceinline__ bool hitTri(const DTri& T, const TRay& r, float tMaxCur, float& t, float& u, float& v) {
 #pragma clang fp contract(off)
     V3 e1 = ld3(T.e1), e2 = ld3(T.e2), v0 = ld3(T.v0);
-    V3 h = cross(r.dir, e2);
-    float a = dot(e1, h);
+    V3 h = crossNC(r.dir, e2);
+    float a = dotNC(e1, h);
     if (fabsf(a) < 1e-9f) return false;
     float f = 1.f / a;
-    V3 s = r.o - v0;
-    u = f * dot(s, h);
+    V3 s = V3{r.o.x - v0.x, r.o.y - v0.y, r.o.z - v0.z};
+    u = f * dotNC(s, h);
     if (u < 0.f || u > 1.f) return false;
-    V3 q = cross(s, e1);
-    v = f * dot(r.dir, q);
+    V3 q = crossNC(s, e1);
+    v = f * dotNC(r.dir, q);
     if (v < 0.f || u + v > 1.f) return false;
-    t = f * dot(e2, q);
+    t = f * dotNC(e2, q);
     return t >= r.tMin && t <= tMaxCur;
 }
 
