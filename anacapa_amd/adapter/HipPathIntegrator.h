@@ -1,0 +1,44 @@
+// HipPathIntegrator.h — the reference-side adapter: an IIntegrator that forwards to the
+// MI355X backend through include/anacapa_hip.h.  It takes CudaPathIntegrator's place
+// (src/gpu/cuda/CudaPathIntegrator.h:21-52) and is compiled inside the reference renderer,
+// against its headers; RenderSession selects it the way it selects the CUDA backend
+// (src/render/RenderSession.cpp:441-452).
+#pragma once
+
+#include <anacapa/integrator/IIntegrator.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+
+struct ahip_context;
+
+namespace anacapa {
+
+class HipPathIntegrator : public IIntegrator {
+public:
+    explicit HipPathIntegrator(int device = 0);
+    ~HipPathIntegrator() override;
+
+    void prepare(const SceneView& scene) override;
+    bool renderFrame(const SceneView& scene, uint32_t filmWidth, uint32_t filmHeight, uint32_t sampleStart,
+                     uint32_t sampleCount, Film& film) override;
+    void renderTile(const SceneView& scene, const TileRequest& tile, uint32_t filmWidth, uint32_t filmHeight,
+                    ISampler& sampler, TileBuffer& out) override;
+
+    bool isValid() const;
+    void clearAccum();
+    std::string name() const;
+    std::string lastError() const;
+
+private:
+    bool ensureCamera(const SceneView& scene, uint32_t w, uint32_t h);
+
+    ahip_context* m_ctx = nullptr;
+    mutable std::mutex m_mutex;      // renderTile is called concurrently (RenderSession.cpp:515-521)
+    bool m_prepared = false;
+    uint32_t m_maxDepth;
+    uint32_t m_camW = 0, m_camH = 0;
+};
+
+}  // namespace anacapa

@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the untimed node/triangle counting pass")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one elementwise copy of this many MiB (read + write), the known-size kernel "
+                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -59,6 +62,16 @@ def main():
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:  # torch is plumbing only; the library synchronises its own stream
         sync = lambda: None  # noqa: E731
+
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)   # one elementwise kernel: reads and writes copy_probe_mb MiB
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
+        del src, dst
 
     t0 = time.time()
     sc = scenes.ggx_spheres(width=args.width, height=args.height, spp=args.spp, max_depth=args.depth)
