@@ -26,6 +26,7 @@ AHIP_E_OUT_OF_MEMORY = 7
 AHIP_MAT_LAMBERT = 0
 AHIP_MAT_EMISSIVE = 1
 AHIP_MAT_STANDARD = 2
+AHIP_MAT_HAIR = 3
 AHIP_LIGHT_AREA = 0
 AHIP_LIGHT_DOME = 1
 AHIP_FLAG_FIX_DOUBLE_COSINE = 1
@@ -43,6 +44,8 @@ class Material(C.Structure):
         ("specular", C.c_float), ("specular_color", F3), ("specular_ior", C.c_float),
         ("coat", C.c_float), ("coat_roughness", C.c_float), ("transmission", C.c_float),
         ("roughness_texture", C.c_int32), ("pad", C.c_uint32 * 2),
+        ("hair_sigma_a", F3), ("hair_eta", C.c_float), ("hair_beta_m", C.c_float), ("hair_beta_n", C.c_float),
+        ("hair_alpha", C.c_float), ("hair_pad", C.c_uint32),
     ]
 
 
@@ -65,6 +68,8 @@ class SceneDesc(C.Structure):
         ("env_width", C.c_uint32), ("env_height", C.c_uint32), ("env_pixels", F32P),
         ("num_textures", C.c_uint32), ("texture_width", U32P), ("texture_height", U32P),
         ("texture_pixels", C.POINTER(F32P)),
+        ("num_strands", C.c_uint32), ("strand_cv_offset", U32P), ("curve_cvs", F32P), ("strand_width", F32P),
+        ("strand_mesh_index", U32P), ("strand_color", F32P),
     ]
 
 

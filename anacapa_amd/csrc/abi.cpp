@@ -73,6 +73,8 @@ struct ahip_context {
     DevBuf dNodes, dTris, dShade, dMats, dMeshMat, dLights, dTexDesc, dDomePx, dDomeMF, dDomeMC, dDomeCF, dDomeCC,
         dDomeCI;
     std::vector<DevBuf*> texBufs;
+    CurveOutput curves;
+    DevBuf dCNodes, dSegs, dSegIds, dStrandInfo;
     DScene dsc{};
 
     // render settings
@@ -86,7 +88,7 @@ struct ahip_context {
     uint32_t accW = 0, accH = 0;
 
     // wavefront queues
-    DevBuf qRo[2], qRd[2], qBeta[2], qHit, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
+    DevBuf qRo[2], qRd[2], qBeta[2], qHit, qHitAux, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
     uint32_t qCap = 0, qCtrSlots = 0;
     uint32_t traceBlocks = 0, shadeBlocks = 0, stackCap = 0;
 
@@ -163,6 +165,7 @@ int ensureQueues(ahip_context* ctx, uint32_t cap) {
         HIPCHK(ctx->qBeta[k].alloc(f4));
     }
     HIPCHK(ctx->qHit.alloc(f4));
+    HIPCHK(ctx->qHitAux.alloc(f4));
     HIPCHK(ctx->qSo.alloc(f4));
     HIPCHK(ctx->qSd.alloc(f4));
     HIPCHK(ctx->qSc.alloc(f4));
@@ -179,7 +182,7 @@ int ensureQueues(ahip_context* ctx, uint32_t cap) {
 
 int ensureStack(ahip_context* ctx) {
     uint32_t K = lds_stack_entries();
-    uint32_t capEntries = ctx->bvh.maxDepth + 2u;
+    uint32_t capEntries = std::max(ctx->bvh.maxDepth, ctx->curves.maxDepth) + 2u;
     ctx->stackCap = capEntries;
     size_t extra = capEntries > K ? capEntries - K : 1u;
     size_t threads = (size_t)ctx->traceBlocks * 128u;
@@ -196,6 +199,7 @@ WaveQueues queues(ahip_context* ctx) {
         q.beta[k] = ctx->qBeta[k].as<float4>();
     }
     q.hit = ctx->qHit.as<float4>();
+    q.hitAux = ctx->qHitAux.as<float4>();
     q.so = ctx->qSo.as<float4>();
     q.sd = ctx->qSd.as<float4>();
     q.sc = ctx->qSc.as<float4>();
@@ -388,8 +392,18 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     int st = buildHostScene(d, hs, err);
     if (st) return fail(ctx, st, err);
     if (buildBVH(d, ctx->bvh, err)) return fail(ctx, AHIP_E_INVALID_ARG, err);
+    if (buildCurves(d, ctx->curves, err)) return fail(ctx, AHIP_E_INVALID_ARG, err);
     DScene& S = ctx->dsc;
     S = DScene{};
+    if ((st = upload(ctx, ctx->dCNodes, ctx->curves.nodes.data(), ctx->curves.nodes.size()))) return st;
+    if ((st = upload(ctx, ctx->dSegs, ctx->curves.segs.data(), ctx->curves.segs.size()))) return st;
+    if ((st = upload(ctx, ctx->dSegIds, ctx->curves.segIds.data(), ctx->curves.segIds.size()))) return st;
+    if ((st = upload(ctx, ctx->dStrandInfo, ctx->curves.strandInfo.data(), ctx->curves.strandInfo.size()))) return st;
+    S.cnodes = ctx->dCNodes.as<CNode>();
+    S.segs = ctx->dSegs.as<DSeg>();
+    S.segIds = ctx->dSegIds.as<uint32_t>();
+    S.strandInfo = ctx->dStrandInfo.as<float>();
+    S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
     if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes.data(), ctx->bvh.nodes.size()))) return st;
     if ((st = upload(ctx, ctx->dTris, ctx->bvh.tris.data(), ctx->bvh.tris.size()))) return st;
     if ((st = upload(ctx, ctx->dShade, ctx->bvh.shade.data(), ctx->bvh.shade.size()))) return st;

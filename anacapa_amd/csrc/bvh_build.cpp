@@ -308,4 +308,136 @@ int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err) {
     return 0;
 }
 
+// ---------------------------------------------------------------- curves
+// Median split on the longest centroid axis with std::nth_element, leaves of at most 4
+// segments, children stored depth-first with explicit left/right indices, segment boxes = hull
+// of the 4 control points grown by the larger end half-width (CurveBrute.cpp:247-396).
+namespace {
+
+struct SegBox {
+    uint32_t rec;               // index into the commit-order segment list
+    float lo[3], hi[3], c[3];
+};
+
+struct CurveBuilder {
+    std::vector<SegBox>& w;
+    std::vector<CNode>& nodes;
+    std::vector<uint32_t>& leafOrder;
+    uint32_t maxDepth = 0;
+
+    uint32_t build(uint32_t s, uint32_t e, uint32_t depth) {
+        maxDepth = std::max(maxDepth, depth);
+        uint32_t idx = static_cast<uint32_t>(nodes.size());
+        nodes.push_back(CNode{});
+        float lo[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+        float hi[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+        for (uint32_t i = s; i < e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                if (w[i].lo[k] < lo[k]) lo[k] = w[i].lo[k];
+                if (w[i].hi[k] > hi[k]) hi[k] = w[i].hi[k];
+            }
+        if (e - s <= 4) {
+            CNode& N = nodes[idx];
+            std::memcpy(N.bmin, lo, 12);
+            std::memcpy(N.bmax, hi, 12);
+            N.lp = static_cast<uint32_t>(leafOrder.size());
+            N.rc = (e - s) | 0x80000000u;
+            for (uint32_t i = s; i < e; ++i) leafOrder.push_back(w[i].rec);
+            return idx;
+        }
+        float cl[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+        float ch[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+        for (uint32_t i = s; i < e; ++i)
+            for (int k = 0; k < 3; ++k) {
+                if (w[i].c[k] < cl[k]) cl[k] = w[i].c[k];
+                if (w[i].c[k] > ch[k]) ch[k] = w[i].c[k];
+            }
+        float ext[3] = {ch[0] - cl[0], ch[1] - cl[1], ch[2] - cl[2]};
+        int axis = 0;
+        if (ext[1] > ext[0]) axis = 1;
+        if (ext[2] > ext[axis]) axis = 2;
+        uint32_t mid = (s + e) / 2;
+        std::nth_element(w.begin() + s, w.begin() + mid, w.begin() + e,
+                         [axis](const SegBox& a, const SegBox& b) { return a.c[axis] < b.c[axis]; });
+        uint32_t l = build(s, mid, depth + 1);
+        uint32_t r = build(mid, e, depth + 1);
+        CNode& N = nodes[idx];
+        std::memcpy(N.bmin, lo, 12);
+        std::memcpy(N.bmax, hi, 12);
+        N.lp = l;
+        N.rc = r;
+        return idx;
+    }
+};
+
+float widthAt(float root, float tip, float v) { return root * (1.f - v) + tip * v; }   // CurvePool.h:45-54
+
+}  // namespace
+
+int buildCurves(const ahip_scene_desc* d, CurveOutput& out, std::string& err) {
+    out = CurveOutput{};
+    if (!d->num_strands) return 0;
+    if (!d->strand_cv_offset || !d->curve_cvs || !d->strand_width || !d->strand_mesh_index) {
+        err = "strand arrays missing";
+        return 1;
+    }
+    std::vector<DSeg> all;
+    std::vector<uint32_t> ids;
+    std::vector<SegBox> work;
+    out.strandInfo.resize(4 * (size_t)d->num_strands);
+    for (uint32_t s = 0; s < d->num_strands; ++s) {
+        uint32_t c0 = d->strand_cv_offset[s], c1 = d->strand_cv_offset[s + 1];
+        if (c1 < c0) {
+            err = "strand_cv_offset not increasing";
+            return 1;
+        }
+        uint32_t ncv = c1 - c0, N = ncv < 4 ? 0u : (ncv - 1) / 3;
+        float root = d->strand_width[2 * s], tip = d->strand_width[2 * s + 1];
+        float* info = &out.strandInfo[4 * (size_t)s];
+        for (int k = 0; k < 3; ++k) info[k] = d->strand_color ? d->strand_color[3 * (size_t)s + k] : 1.f;
+        uint32_t mesh = d->strand_mesh_index[s];
+        std::memcpy(&info[3], &mesh, 4);
+        for (uint32_t g = 0; g < N; ++g) {
+            DSeg S;
+            for (int k = 0; k < 4; ++k) std::memcpy(&S.P[3 * k], d->curve_cvs + 3 * (size_t)(c0 + 3 * g + k), 12);
+            float v0 = float(g) / float(N), v3 = float(g + 1) / float(N);
+            float wa = widthAt(root, tip, v0), wb = widthAt(root, tip, v3);
+            S.hw0 = wa * 0.5f;
+            S.hw3 = wb * 0.5f;
+            S.tOff = v0;
+            S.tScale = v3 - v0;
+            float mh = std::max(wa, wb) * 0.5f;
+            SegBox b;
+            b.rec = static_cast<uint32_t>(all.size());
+            for (int k = 0; k < 3; ++k) {
+                float p0 = S.P[k], p1 = S.P[3 + k], p2 = S.P[6 + k], p3 = S.P[9 + k];
+                b.lo[k] = std::min({p0, p1, p2, p3}) - mh;
+                b.hi[k] = std::max({p0, p1, p2, p3}) + mh;
+                b.c[k] = (b.lo[k] + b.hi[k]) * 0.5f;
+            }
+            all.push_back(S);
+            ids.push_back(s);
+            ids.push_back(g);
+            work.push_back(b);
+        }
+    }
+    if (work.empty()) return 0;
+    if (work.size() >= 0x7fffffffu) {
+        err = "too many curve segments";
+        return 1;
+    }
+    std::vector<uint32_t> order;
+    CurveBuilder B{work, out.nodes, order};
+    B.build(0, static_cast<uint32_t>(work.size()), 1);
+    out.maxDepth = B.maxDepth;
+    out.segs.reserve(order.size());
+    out.segIds.reserve(2 * order.size());
+    for (uint32_t r : order) {
+        out.segs.push_back(all[r]);
+        out.segIds.push_back(ids[2 * r]);
+        out.segIds.push_back(ids[2 * r + 1]);
+    }
+    return 0;
+}
+
 }  // namespace ahip

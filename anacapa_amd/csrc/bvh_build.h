@@ -35,4 +35,17 @@ struct BVHOutput {
 
 int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err);
 
+struct CNode;
+struct DSeg;
+
+struct CurveOutput {
+    std::vector<CNode> nodes;          // median-split BVH (CurveBrute.cpp:247-320), depth-first
+    std::vector<DSeg> segs;            // leaf order
+    std::vector<uint32_t> segIds;      // strand, segment per record
+    std::vector<float> strandInfo;     // colour rgb + meshID bits per strand
+    uint32_t maxDepth = 0;
+};
+
+int buildCurves(const ahip_scene_desc* d, CurveOutput& out, std::string& err);
+
 }  // namespace ahip

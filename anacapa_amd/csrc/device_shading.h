@@ -19,6 +19,7 @@ namespace ahip {
 
 struct Frame {
     V3 n, ng, t, bt;
+    V3 color;      // per-strand colour (curves); white for surfaces
     bool front;
 };
 __device__ __forceinline__ Frame makeFrame(V3 n, V3 ng, V3 rayDir) {
@@ -27,6 +28,41 @@ __device__ __forceinline__ Frame makeFrame(V3 n, V3 ng, V3 rayDir) {
     f.n = f.front ? n : -n;
     f.ng = f.front ? ng : -ng;
     onb(f.n, f.t, f.bt);
+    f.color = V3{1.f, 1.f, 1.f};
+    return f;
+}
+// Curve hit: tangent frame around the fiber (ShadingContext.h:39-44).
+__device__ __forceinline__ Frame makeCurveFrame(V3 n, V3 tang, V3 rayDir, V3 color) {
+    Frame f;
+    f.front = dot(-rayDir, n) > 0.f;
+    f.n = f.front ? n : -n;
+    f.ng = f.n;
+    f.t = tang;
+    f.bt = safeNormalize(cross(f.n, tang));
+    f.color = color;
+    return f;
+}
+
+// MarschnerHairMaterial::effectiveSigmaA (MarschnerHair.h:430-450)
+__device__ __forceinline__ V3 hairSigmaA(const DMat& m, V3 c) {
+    if (c.x > 0.98f && c.y > 0.98f && c.z > 0.98f) return ld3(m.hairSigma);
+    float lr = powf(fmaxf(c.x, 0.f), 2.2f), lg = powf(fmaxf(c.y, 0.f), 2.2f), lb = powf(fmaxf(c.z, 0.f), 2.2f);
+    if (lr < 0.01f && lg < 0.01f && lb < 0.01f) return ld3(m.hairSigma);
+    return V3{-logf(fmaxf(lr, 0.001f)), -logf(fmaxf(lg, 0.001f)), -logf(fmaxf(lb, 0.001f))};
+}
+
+// Kajiya-Kay, the hair BSDF the reference evaluates at run time (MarschnerHair.h:240, 395-422).
+__device__ __forceinline__ V3 hairKK(const DMat& m, const struct Frame& fr, V3 wo, V3 wi, float& pdf) {
+    V3 sa = hairSigmaA(m, fr.color);
+    V3 f{expf(-sa.x) * kInvPi, expf(-sa.y) * kInvPi, expf(-sa.z) * kInvPi};
+    float dTL = dot(fr.t, wi), dTV = dot(fr.t, wo);
+    float sTL = sqrtf(fmaxf(0.f, 1.f - dTL * dTL)), sTV = sqrtf(fmaxf(0.f, 1.f - dTV * dTV));
+    float spec = dTL * dTV + sTL * sTV;
+    if (spec > 0.f) {
+        float s = 0.15f * (powf(spec, 20.f) * kInvPi);
+        f = f + V3{s, s, s};
+    }
+    pdf = fmaxf(0.f, dot(wi, fr.n)) * kInvPi;
     return f;
 }
 __device__ __forceinline__ V3 toLocal(const Frame& f, V3 v) { return V3{dot(v, f.t), dot(v, f.bt), dot(v, f.n)}; }
@@ -149,6 +185,20 @@ __device__ __forceinline__ float matRoughness(const DScene& sc, const DMat& m, V
 // IMaterial::sample
 __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V2 u, float uc) {
     BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    if (m.type == AHIP_MAT_HAIR) {                                       // MarschnerHair.h:267-282
+        V3 kt, kb;
+        onb(fr.n, kt, kb);
+        float kphi = 2.f * kPi * u.x, kcos = sqrtf(u.y), ksin = sqrtf(1.f - u.y);
+        float sp, cp;
+        sincosf(kphi, &sp, &cp);
+        V3 wi = safeNormalize(kt * (ksin * cp) + kb * (ksin * sp) + fr.n * kcos);
+        float pdf;
+        V3 f = hairKK(m, fr, wo, wi, pdf);
+        s.wi = wi;
+        s.f = f * kcos;
+        s.pdf = kcos * kInvPi;
+        return s;
+    }
     if (m.type != AHIP_MAT_STANDARD) {                                   // Lambertian.h:36-51
         float r = sqrtf(u.x), phi = 2.f * kPi * u.y;
         float sp, cp;
@@ -211,6 +261,7 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V
 
 // IMaterial::evaluate: f without cosine, forward pdf.
 __device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V3 wi, float& pdf) {
+    if (m.type == AHIP_MAT_HAIR) return hairKK(m, fr, wo, wi, pdf);
     V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
     pdf = 0.f;
     if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};

@@ -28,6 +28,7 @@ namespace ahip {
 
 struct TRay {
     V3 o;
+    V3 d;       // original direction (the curve tests use it, CurveBrute.cpp:41-47, 478-498)
     V3 inv;     // 1 / safe(d)
     V3 dir;     // 1 / inv, the direction Moller-Trumbore sees (BVHBackend.cpp:267)
     float tMin, tMax;
@@ -38,6 +39,7 @@ __device__ __forceinline__ TRay makeTRay(V3 o, V3 d, float tMin, float tMax) {
 #pragma clang fp contract(off)
     TRay r;
     r.o = o;
+    r.d = d;
     r.tMin = tMin;
     r.tMax = tMax;
     float sx = fabsf(d.x) > 1e-9f ? d.x : 1e-9f;
@@ -225,6 +227,200 @@ __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, f
     tHit = closest;
     uHit = hu;
     vHit = hv;
+    return hit;
+}
+
+// ---------------------------------------------------------------- curves (CurveBrute.cpp)
+// Ray-space ribbon test and the curve BVH, with the reference's semantics: CVs projected into a
+// Frisvad frame around the ray direction; recursive de Casteljau midpoint splits to depth 6 with
+// hull pruning against [tMin, best]; a leaf accepts the closest chord point within the
+// interpolated half width, strictly nearer than the best hit; the left half is searched before
+// the right half, so the right half is pruned with the left half's result.  The recursion runs
+// on a private stack of pending right halves (at most one per level).
+struct CurveHit {
+    float tSeg;
+    V3 tang;
+    uint32_t rec;
+};
+
+// Frisvad basis (Types.h:108-114) without contraction: the ray-space frame of the curve test.
+__device__ __forceinline__ void onbNC(V3 n, V3& t, V3& bt) {
+#pragma clang fp contract(off)
+    float sign = copysignf(1.f, n.z);
+    float a = -1.f / (sign + n.z);
+    float b = n.x * n.y * a;
+    t = V3{1.f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    bt = V3{b, sign + n.y * n.y * a, -n.y};
+}
+
+struct SubFrame {
+    V3 P[4];
+    float hw0, hw3, tOff, tScale;
+    int depth;
+};
+
+__device__ __forceinline__ V3 projNC(V3 p, V3 o, V3 x, V3 y, V3 z) {
+#pragma clang fp contract(off)
+    V3 q{p.x - o.x, p.y - o.y, p.z - o.z};
+    return V3{dotNC(q, x), dotNC(q, y), dotNC(q, z)};
+}
+__device__ __forceinline__ V3 midNC(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return V3{(a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f};
+}
+
+// Returns true when a strictly nearer hit on this segment was found (best, out updated).
+__device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang) {
+#pragma clang fp contract(off)
+    SubFrame st[8];
+    int sp = 0;
+    SubFrame f0;
+    f0.P[0] = V3{S.P[0], S.P[1], S.P[2]};
+    f0.P[1] = V3{S.P[3], S.P[4], S.P[5]};
+    f0.P[2] = V3{S.P[6], S.P[7], S.P[8]};
+    f0.P[3] = V3{S.P[9], S.P[10], S.P[11]};
+    f0.hw0 = S.hw0;
+    f0.hw3 = S.hw3;
+    f0.tOff = S.tOff;
+    f0.tScale = S.tScale;
+    f0.depth = 6;
+    st[sp++] = f0;
+    bool hit = false;
+    while (sp > 0) {
+        SubFrame f = st[--sp];
+        V3 c0 = projNC(f.P[0], r.o, fx, fy, r.d), c1 = projNC(f.P[1], r.o, fx, fy, r.d);
+        V3 c2 = projNC(f.P[2], r.o, fx, fy, r.d), c3 = projNC(f.P[3], r.o, fx, fy, r.d);
+        float mh = fmaxf(f.hw0, f.hw3);
+        float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z), zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
+        if (zMin - mh > best) continue;
+        if (zMax + mh < r.tMin) continue;
+        float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
+        float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
+        float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
+        float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
+        if (cx * cx + cy * cy > mh * mh) continue;
+        if (f.depth == 0) {
+            float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
+            float l2 = dx * dx + dy * dy;
+            float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
+            float px = ax + dx * tl, py = ay + dy * tl;
+            float d2 = px * px + py * py;
+            float hw = f.hw0 * (1.f - tl) + f.hw3 * tl;
+            if (d2 > hw * hw) continue;
+            float tr = c0.z + (c3.z - c0.z) * tl;
+            if (tr < r.tMin || tr >= best) continue;
+            // bezierTangent (CurveBrute.cpp:59-64)
+            float mt = 1.f - tl;
+            float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
+            V3 d10{f.P[1].x - f.P[0].x, f.P[1].y - f.P[0].y, f.P[1].z - f.P[0].z};
+            V3 d21{f.P[2].x - f.P[1].x, f.P[2].y - f.P[1].y, f.P[2].z - f.P[1].z};
+            V3 d32{f.P[3].x - f.P[2].x, f.P[3].y - f.P[2].y, f.P[3].z - f.P[2].z};
+            V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
+                  d10.z * a0 + d21.z * a1 + d32.z * a2};
+            if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
+                tg = V3{f.P[3].x - f.P[0].x, f.P[3].y - f.P[0].y, f.P[3].z - f.P[0].z};
+            float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
+            if (l > 1e-8f) {
+                float il = 1.f / l;
+                tg = V3{tg.x * il, tg.y * il, tg.z * il};
+            } else {
+                tg = V3{0.f, 0.f, 1.f};
+            }
+            best = tr;
+            tSeg = f.tOff + tl * f.tScale;
+            tang = tg;
+            hit = true;
+            continue;
+        }
+        V3 q01 = midNC(f.P[0], f.P[1]), q12 = midNC(f.P[1], f.P[2]), q23 = midNC(f.P[2], f.P[3]);
+        V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
+        float hm = (f.hw0 + f.hw3) * 0.5f, ts = f.tScale * 0.5f;
+        SubFrame R;
+        R.P[0] = m; R.P[1] = r12; R.P[2] = q23; R.P[3] = f.P[3];
+        R.hw0 = hm; R.hw3 = f.hw3; R.tOff = f.tOff + ts; R.tScale = ts; R.depth = f.depth - 1;
+        SubFrame L;
+        L.P[0] = f.P[0]; L.P[1] = q01; L.P[2] = r01; L.P[3] = m;
+        L.hw0 = f.hw0; L.hw3 = hm; L.tOff = f.tOff; L.tScale = ts; L.depth = f.depth - 1;
+        st[sp++] = R;
+        st[sp++] = L;
+    }
+    return hit;
+}
+
+// CurveBrute::aabbHit (CurveBrute.cpp:478-498)
+__device__ __forceinline__ bool curveBox(const CNode& n, const TRay& r, float maxT) {
+#pragma clang fp contract(off)
+    float tmin = r.tMin, tmax = maxT;
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, dd[3] = {r.d.x, r.d.y, r.d.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (fabsf(dd[i]) < 1e-9f) {
+            if (o[i] < n.bmin[i] || o[i] > n.bmax[i]) return false;
+            continue;
+        }
+        float inv = 1.f / dd[i];
+        float t0 = (n.bmin[i] - o[i]) * inv, t1 = (n.bmax[i] - o[i]) * inv;
+        if (inv < 0.f) { float s = t0; t0 = t1; t1 = s; }
+        tmin = stdmax(tmin, t0);
+        tmax = stdmin(tmax, t1);
+        if (tmax < tmin) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ CNode loadCNode(const CNode* nodes, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + i);
+    float4 a = p[0], b = p[1];
+    CNode n;
+    n.bmin[0] = a.x; n.bmin[1] = a.y; n.bmin[2] = a.z;
+    n.bmax[0] = a.w; n.bmax[1] = b.x; n.bmax[2] = b.y;
+    n.lp = __float_as_uint(b.z);
+    n.rc = __float_as_uint(b.w);
+    return n;
+}
+
+// Curve part of CurveBrute::trace (CurveBrute.cpp:398-440) after the triangle BVH: `best` is the
+// triangle hit distance (or tMax).  ANY stops at the first accepted segment.
+template <bool ANY, bool COUNT>
+__device__ bool traverseCurves(const DScene& sc, const TRay& r, const TStack& S, float& best, CurveHit& ch,
+                               TraceCounters& cnt, uint32_t* overflow) {
+    if (!sc.hasCurves) return false;
+    V3 fx, fy;
+    onbNC(r.d, fx, fy);
+    bool hit = false;
+    uint32_t sp = 0;
+    spush(S, sp, 0u, 0.f);
+    while (sp > 0u) {
+        uint32_t ni;
+        float unused;
+        spop(S, sp, ni, unused);
+        CNode n = loadCNode(sc.cnodes, ni);
+        if (COUNT) ++cnt.nodes;
+        if (!curveBox(n, r, best)) continue;
+        if (n.rc & 0x80000000u) {
+            uint32_t cntSeg = n.rc & 0x7fffffffu;
+            for (uint32_t k = 0; k < cntSeg; ++k) {
+                const DSeg& Sg = sc.segs[n.lp + k];
+                if (COUNT) ++cnt.tris;
+                float tSeg;
+                V3 tg;
+                if (segmentHit(Sg, r, fx, fy, best, tSeg, tg)) {
+                    hit = true;
+                    ch.tSeg = tSeg;
+                    ch.tang = tg;
+                    ch.rec = n.lp + k;
+                    if (ANY) return true;
+                }
+            }
+        } else {
+            if (sp + 2u >= S.cap) {
+                *overflow = 1u;
+                return hit;
+            }
+            spush(S, sp, n.rc, 0.f);
+            spush(S, sp, n.lp, 0.f);
+        }
+    }
     return hit;
 }
 

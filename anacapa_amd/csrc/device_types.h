@@ -48,6 +48,25 @@ struct alignas(64) DTriShade {
 };
 static_assert(sizeof(DTriShade) == 64, "DTriShade must be 64 bytes");
 
+// Curve BVH node in the reference's own 32-byte layout (CurveNode, CurveBrute.h:27-34): the
+// curve traversal tests a child's box when it is popped, against the current best distance.
+struct CNode {
+    float bmin[3], bmax[3];
+    uint32_t lp;   // interior: left child; leaf: first segment record
+    uint32_t rc;   // interior: right child; leaf: count | 0x80000000
+};
+static_assert(sizeof(CNode) == 32, "CNode must be 32 bytes");
+
+// 64-byte cubic Bezier segment record in leaf order: 4 control points, the half widths at the
+// ends and the strand-parameter range [tOff, tOff + tScale] (CurveBrute.cpp:182-202).
+struct alignas(64) DSeg {
+    float P[12];
+    float hw0, hw3, tOff, tScale;
+};
+static_assert(sizeof(DSeg) == 64, "DSeg must be 64 bytes");
+
+constexpr uint32_t kCurveHit = 0x80000000u;   // hit code flag: low bits = segment record
+
 struct DMat {
     uint32_t type;            // AHIP_MAT_*
     uint32_t isDelta;         // StandardSurface::isDelta (StandardSurface.h:162-169)
@@ -59,6 +78,7 @@ struct DMat {
     float specColor[3];
     float coat, coatA2, f0d, coatF0;
     uint32_t hasLe;
+    float hairSigma[3];       // MarschnerHairMaterial default sigma_a
 };
 
 struct DLight {
@@ -113,6 +133,12 @@ struct DScene {
     uint32_t hasEmitters;     // any material with non-zero Le
     float envRadiance[3];
     DDome dome;
+    // curves
+    const CNode* cnodes;
+    const DSeg* segs;
+    const uint32_t* segIds;   // [2 * segments]: strand, segment index within the strand
+    const float* strandInfo;  // [4 * strands]: colour rgb, meshID (bits)
+    uint32_t hasCurves;
 };
 
 }  // namespace ahip

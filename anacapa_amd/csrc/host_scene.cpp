@@ -117,6 +117,8 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
         std::memcpy(D.albedo, m.albedo, 12);
         if (m.type == AHIP_MAT_LAMBERT || m.type == AHIP_MAT_EMISSIVE) {
             if (m.type == AHIP_MAT_EMISSIVE) std::memcpy(D.Le, m.emission, 12);
+        } else if (m.type == AHIP_MAT_HAIR) {
+            std::memcpy(D.hairSigma, m.hair_sigma_a, 12);   // MarschnerHair.h:193-199
         } else if (m.type == AHIP_MAT_STANDARD) {
             if (m.transmission > 0.001f) {
                 err = "transmissive standard_surface is not supported by this backend";

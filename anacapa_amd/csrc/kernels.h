@@ -28,7 +28,8 @@ struct WaveQueues {
     float4* ro[2];    // continuation ray origin | path id
     float4* rd[2];    // direction | sampler dimension (bit 31: specular bounce)
     float4* beta[2];  // throughput
-    float4* hit;      // t, u, v | leaf-order triangle (~0 = miss)
+    float4* hit;      // t, u, v | leaf-order triangle (~0 = miss; bit 31 = curve segment, u = strand v)
+    float4* hitAux;   // curve hits: fiber tangent
     float4* so;       // shadow ray origin | path id
     float4* sd;       // direction | tMax
     float4* sc;       // contribution if unoccluded

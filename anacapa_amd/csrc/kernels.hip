@@ -126,6 +126,17 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
         TRay r = makeTRay(v3(a), v3(b), 1e-4f, __int_as_float(0x7f800000));
         float t = 0.f, u = 0.f, v = 0.f;
         uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        if (sc.hasCurves) {
+            float best = h != ~0u ? t : r.tMax;
+            CurveHit ch;
+            if (traverseCurves<false, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) {
+                h = kCurveHit | ch.rec;
+                t = best;
+                u = ch.tSeg;
+                v = 0.f;
+                q.hitAux[i] = f4(ch.tang, 0.f);
+            }
+        }
         q.hit[i] = make_float4(t, u, v, __uint_as_float(h));
     }
 }
@@ -161,29 +172,55 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, PassDesc pd, WaveQueue
             if (tri == ~0u) {
                 if (bounce == 0u || specular) Ladd = beta * envLe(sc, dir);   // PathIntegrator.cpp:83-91
             } else {
-                // fillSurfaceInteraction (BVHBackend.cpp:285-317)
-                const DTri* T = sc.tris + tri;
-                float tt = hh.x, u = hh.y, v = hh.z, w = 1.f - u - v;
-                V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2), ng = ld3(T->n);
-                uint32_t meshID = T->meshID;
-                const DTriShade* SH = sc.shade + tri;
-                V3 p = v0 + e1 * u + e2 * v;
-                V3 n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
-                if (dot(n, ng) < 0.f) n = -n;
-                V2 uv{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v, SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
-                (void)tt;
+                V3 p, n, ng;
+                V2 uv;
+                uint32_t meshID;
+                bool isCurve = (tri & kCurveHit) != 0u;
+                Frame fr;
+                if (!isCurve) {
+                    // fillSurfaceInteraction (BVHBackend.cpp:285-317)
+                    const DTri* T = sc.tris + tri;
+                    float u = hh.y, v = hh.z, w = 1.f - u - v;
+                    V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2);
+                    ng = ld3(T->n);
+                    meshID = T->meshID;
+                    const DTriShade* SH = sc.shade + tri;
+                    p = v0 + e1 * u + e2 * v;
+                    n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
+                    if (dot(n, ng) < 0.f) n = -n;
+                    uv = V2{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v,
+                            SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
+                    fr = makeFrame(n, ng, dir);
+                } else {
+                    // curve hit record (CurveBrute.cpp:218-233)
+                    uint32_t rec = tri & ~kCurveHit;
+                    uint32_t strand = sc.segIds[2u * rec];
+                    const float* info = sc.strandInfo + 4u * (size_t)strand;
+                    V3 tang = v3(q.hitAux[i]);
+                    V3 fx, fy;
+                    onbNC(dir, fx, fy);
+                    V3 nd = -dir;
+                    V3 nrm = nd - tang * dot(nd, tang);
+                    float ln = len(nrm);
+                    nrm = ln > 1e-8f ? nrm * (1.f / ln) : fx;
+                    p = v3(ra) + dir * hh.x;
+                    n = nrm;
+                    ng = nrm;
+                    uv = V2{0.f, hh.y};
+                    meshID = __float_as_uint(info[3]);
+                    fr = makeCurveFrame(n, tang, dir, V3{info[0], info[1], info[2]});
+                }
                 int32_t mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 if (mi >= 0) {
                     const DMat& m = sc.mats[mi];
                     V3 wo = -dir;
-                    Frame fr = makeFrame(n, ng, dir);
                     if (m.hasLe && (bounce == 0u || specular)) Ladd = beta * ld3(m.Le);   // :128-132
                     if (!m.isDelta && sc.numLights > 0u) {                                // :134-144
                         float ul = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         uint32_t li = (uint32_t)(ul * nL);
                         li = li < sc.numLights - 1u ? li : sc.numLights - 1u;
                         const DLight& L = sc.lights[li];
-                        Frame fe = makeFrame(n, ng, -wo);   // estimateDirect's ShadingContext(si, -wo)
+                        const Frame& fe = fr;   // estimateDirect's ShadingContext(si, -wo): the same frame
                         // light-sampling strategy (PathIntegrator.cpp:176-198)
                         V2 u2;
                         u2.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
@@ -202,7 +239,9 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, PassDesc pd, WaveQueue
                                 if (dot(sd, n) < 0.f) off = -off;
                                 float f = ls.pdf, g = bpdf;
                                 float wgt = (f * f) / (f * f + g * g);
-                                float cosI = fabsf(dot(ls.wi, n));
+                                // hair: sin of the angle to the fiber (PathIntegrator.cpp:189-193)
+                                float dt = dot(ls.wi, fe.t);
+                                float cosI = isCurve ? sqrtf(fmaxf(0.f, 1.f - dt * dt)) : fabsf(dot(ls.wi, n));
                                 V3 Ld = bf * ls.Li * cosI * wgt / ls.pdf;
                                 shO = p + off;
                                 shD = sd;
@@ -299,6 +338,11 @@ __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t
         TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
         float t, u, v;
         uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        if (h == ~0u && sc.hasCurves) {
+            float best = r.tMax;
+            CurveHit ch;
+            if (traverseCurves<true, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) h = kCurveHit;
+        }
         if (h == ~0u) {
             uint32_t p0 = __float_as_uint(a.w);
             float4 c = q.sc[i], l = q.L[p0];
@@ -326,15 +370,28 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
         V3 Li{0.f, 0.f, 0.f};
         if (sc.hasEmitters) {
             uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            uint32_t mesh = h != ~0u ? sc.tris[h].meshID : ~0u;
+            if (sc.hasCurves) {
+                float best = h != ~0u ? t : r.tMax;
+                CurveHit ch;
+                if (traverseCurves<false, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) {
+                    h = kCurveHit | ch.rec;
+                    mesh = __float_as_uint(sc.strandInfo[4u * (size_t)sc.segIds[2u * ch.rec] + 3u]);
+                }
+            }
             if (h == ~0u) Li = envLe(sc, d);
             else {
-                uint32_t mesh = sc.tris[h].meshID;
                 int32_t mi = mesh < sc.numMeshes ? sc.meshMat[mesh] : -1;
                 if (mi >= 0 && sc.mats[mi].hasLe) Li = ld3(sc.mats[mi].Le);
             }
         } else {
             // no surface emits: only an escaping ray carries radiance, so any hit decides
             uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            if (h == ~0u && sc.hasCurves) {
+                float best = r.tMax;
+                CurveHit ch;
+                if (traverseCurves<true, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) h = kCurveHit;
+            }
             if (h == ~0u) Li = envLe(sc, d);
         }
         if (!isBlack(Li)) {
@@ -389,13 +446,28 @@ __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays
         uint32_t h;
         if (counts) h = traverse<ANY, true>(sc, r, S, t, u, v, tc, overflow);
         else h = traverse<ANY, false>(sc, r, S, t, u, v, tc, overflow);
+        CurveHit ch;
+        bool curve = false;
+        if (sc.hasCurves && !(ANY && h != ~0u)) {
+            float best = h != ~0u ? t : r.tMax;
+            if (counts) curve = traverseCurves<ANY, true>(sc, r, S, best, ch, tc, overflow);
+            else curve = traverseCurves<ANY, false>(sc, r, S, best, ch, tc, overflow);
+            if (curve) t = best;
+        }
         if (counts) { counts[2 * i] = tc.nodes; counts[2 * i + 1] = tc.tris; }
         if (ANY) {
-            occ[i] = h != ~0u ? 1 : 0;
+            occ[i] = (h != ~0u || curve) ? 1 : 0;
         } else {
             ahip_hit o;
-            if (h == ~0u) o = ahip_hit{0.f, 0.f, 0.f, ~0u, ~0u, ~0u};
-            else o = ahip_hit{t, u, v, sc.tris[h].meshID, sc.tris[h].primID, sc.tris[h].commitIndex};
+            if (curve) {
+                uint32_t strand = sc.segIds[2u * ch.rec];
+                uint32_t mesh = __float_as_uint(sc.strandInfo[4u * (size_t)strand + 3u]);
+                o = ahip_hit{t, 0.f, ch.tSeg, mesh, sc.segIds[2u * ch.rec + 1u], kCurveHit | strand};
+            } else if (h == ~0u) {
+                o = ahip_hit{0.f, 0.f, 0.f, ~0u, ~0u, ~0u};
+            } else {
+                o = ahip_hit{t, u, v, sc.tris[h].meshID, sc.tris[h].primID, sc.tris[h].commitIndex};
+            }
             hits[i] = o;
         }
     }
@@ -423,8 +495,15 @@ __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t 
             TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
             float t, u, v;
             bool any = KIND == 1 || (KIND == 2 && !sc.hasEmitters);
-            if (any) traverse<true, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-            else traverse<false, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            uint32_t h;
+            if (any) h = traverse<true, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            else h = traverse<false, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            if (sc.hasCurves && !(any && h != ~0u)) {
+                float best = h != ~0u ? t : r.tMax;
+                CurveHit ch;
+                if (any) traverseCurves<true, true>(sc, r, S, best, ch, tc, ctr(q, bounce, 3));
+                else traverseCurves<false, true>(sc, r, S, best, ch, tc, ctr(q, bounce, 3));
+            }
         }
         unsigned long long nodes = tc.nodes, trisT = tc.tris;
         for (int off = 32; off > 0; off >>= 1) {

@@ -35,6 +35,11 @@ class Scene:
     env_radiance: tuple = (0.0, 0.0, 0.0)
     env_pixels: np.ndarray | None = None  # [H,W,3] f32
     textures: list = field(default_factory=list)      # list[np.ndarray [H,W] f32]
+    strand_cv_offset: np.ndarray | None = None       # [S+1] u32
+    curve_cvs: np.ndarray | None = None               # [CV,3] f32
+    strand_width: np.ndarray | None = None            # [S,2] f32 root/tip diameter
+    strand_mesh_index: np.ndarray | None = None       # [S] u32
+    strand_color: np.ndarray | None = None            # [S,3] f32
     cam_from: tuple = (0.0, 0.0, 0.0)
     cam_at: tuple = (0.0, 0.0, 1.0)
     cam_up: tuple = (0.0, 1.0, 0.0)
@@ -110,8 +115,24 @@ class Scene:
             d.texture_width = tw.ctypes.data_as(A.U32P)
             d.texture_height = th.ctypes.data_as(A.U32P)
             d.texture_pixels = C.cast(tpa, C.POINTER(A.F32P))
+        if self.strand_cv_offset is not None and len(self.strand_cv_offset) > 1:
+            so = arr(self.strand_cv_offset, np.uint32)
+            cv = arr(self.curve_cvs.reshape(-1), np.float32)
+            sw = arr(self.strand_width.reshape(-1), np.float32)
+            sm = arr(self.strand_mesh_index, np.uint32)
+            d.num_strands = len(so) - 1
+            d.strand_cv_offset = so.ctypes.data_as(A.U32P)
+            d.curve_cvs = cv.ctypes.data_as(A.F32P)
+            d.strand_width = sw.ctypes.data_as(A.F32P)
+            d.strand_mesh_index = sm.ctypes.data_as(A.U32P)
+            if self.strand_color is not None:
+                d.strand_color = arr(self.strand_color.reshape(-1), np.float32).ctypes.data_as(A.F32P)
         keep.append(d)
         return d, keep
+
+    @property
+    def num_strands(self) -> int:
+        return 0 if self.strand_cv_offset is None else int(len(self.strand_cv_offset) - 1)
 
 
 def lambert(rgb) -> A.Material:
@@ -146,6 +167,25 @@ def standard_surface(base_color, metalness=0.0, roughness=0.5, specular=1.0, ior
     m.transmission = 0.0
     m.roughness_texture = roughness_texture
     return m
+
+
+def hair(sigma_a=(0.40, 0.65, 1.35), eta=1.55, beta_m=0.3, beta_n=0.3, alpha=2.0) -> A.Material:
+    """MarschnerHairMaterial::Params defaults (src/shading/MarschnerHair.h:193-199)."""
+    m = A.Material()
+    m.type = A.AHIP_MAT_HAIR
+    m.roughness_texture = -1
+    m.hair_sigma_a = A.F3(*sigma_a)
+    m.hair_eta = eta
+    m.hair_beta_m = beta_m
+    m.hair_beta_n = beta_n
+    m.hair_alpha = alpha
+    return m
+
+
+def eumelanin_sigma_a(ce):
+    """Absorption of eumelanin concentration ce (d'Eon et al. 2011 as used by PBRT-v4's
+    HairBxDF::SigmaAFromConcentration with no pheomelanin)."""
+    return np.outer(np.asarray(ce, np.float64), np.array([0.419, 0.697, 1.37]))
 
 
 def area_light(center, u_axis, v_axis, Le) -> A.Light:
@@ -386,3 +426,58 @@ def triangle_soup(n=2000, seed=1) -> Scene:
     mb = _MeshBuilder()
     mb.add(p, np.arange(3 * n, dtype=np.uint32), 0)
     return mb.scene("soup", materials=[lambert((0.5, 0.5, 0.5))], lights=[], env_radiance=(1.0, 1.0, 1.0))
+
+
+def hair_sphere(width=1024, height=1024, spp=32, max_depth=16, seed=0, strands=200_000, segments=4,
+                hdri_size=(1024, 512), scalp_freq=8) -> Scene:
+    """Workload 3: `strands` cubic-Bezier strands of `segments` segments (shared end points) rooted
+    at Fibonacci points of a unit sphere, grown along the normal with length U[0.3, 0.6] and a
+    seed-driven curl, width tapering 0.004 -> 0.001; per-strand eumelanin U[0.2, 1.5] encoded as
+    the strand colour the hair material turns back into sigma_a (MarschnerHair.h:430-450); one
+    1x1 area light plus a dome."""
+    rng = np.random.default_rng(seed)
+    mb = _MeshBuilder()
+    mats = [lambert((0.05, 0.04, 0.03)), emissive((0.0, 0.0, 0.0), (6.0, 6.0, 6.0)), hair()]
+    unit, tris = geodesic_sphere(scalp_freq)
+    mb.add(unit.astype(np.float32), tris.reshape(-1), 0, normals=unit.astype(np.float32))
+    y = 2.5
+    _quad(mb, (-0.5, y, -0.5), (0.5, y, -0.5), (0.5, y, 0.5), (-0.5, y, 0.5), (0, -1, 0), 1)
+    mb.add(np.zeros((0, 3), np.float32), np.zeros(0, np.uint32), 2)      # mesh slot 2: the hair material
+    n = strands
+    i = np.arange(n) + 0.5
+    polar = np.arccos(1 - 2 * i / n)
+    azim = np.pi * (1 + 5 ** 0.5) * i
+    root = np.stack([np.cos(azim) * np.sin(polar), np.cos(polar), np.sin(azim) * np.sin(polar)], 1)
+    ref = np.where(np.abs(root[:, 1:2]) < 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t1 = np.cross(root, ref)
+    t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+    t2 = np.cross(root, t1)
+    length = rng.uniform(0.3, 0.6, n)
+    freq = rng.uniform(1.0, 3.0, n)
+    amp = rng.uniform(0.02, 0.08, n)
+    ph1, ph2 = rng.uniform(0, 2 * np.pi, (2, n))
+    s = np.linspace(0.0, 1.0, 3 * segments + 1)
+    ang = 2 * np.pi * freq[:, None] * s[None, :]
+    curl = (t1[:, None, :] * (np.sin(ang + ph1[:, None]) - np.sin(ph1[:, None]))[..., None]
+            + t2[:, None, :] * (np.cos(ang + ph2[:, None]) - np.cos(ph2[:, None]))[..., None]) * amp[:, None, None]
+    cvs = root[:, None, :] * (1.0 + length[:, None, None] * s[None, :, None]) + curl * s[None, :, None]
+    ce = rng.uniform(0.2, 1.5, n)
+    sig = eumelanin_sigma_a(ce)
+    color = np.exp(-sig / 2.2)
+    lights = [area_light((0.0, y, 0.0), (0.5, 0.0, 0.0), (0.0, 0.0, 0.5), (6.0, 6.0, 6.0)), dome_light(1.0)]
+    env = procedural_hdri(*hdri_size, sun_peak=500.0, sun_sigma_deg=2.0)
+    sc = mb.scene("hair_sphere", materials=mats, lights=lights, env_light=1, env_pixels=env,
+                  cam_from=(0.0, 0.5, -3.6), cam_at=(0.0, 0.1, 0.0), cam_up=(0.0, 1.0, 0.0), vfov=40.0,
+                  width=width, height=height, spp=spp, max_depth=max_depth, seed=seed)
+    sc.strand_cv_offset = (np.arange(n + 1) * (3 * segments + 1)).astype(np.uint32)
+    sc.curve_cvs = cvs.reshape(-1, 3).astype(np.float32)
+    sc.strand_width = np.tile(np.array([[0.004, 0.001]], np.float32), (n, 1))
+    sc.strand_mesh_index = np.full(n, 2, np.uint32)
+    sc.strand_color = color.astype(np.float32)
+    return sc
+
+
+def small_hair(width=64, height=64, spp=2, max_depth=16, seed=0) -> Scene:
+    """A few-hundred-strand instance of workload 3 for oracle-speed parity tests."""
+    return hair_sphere(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, strands=400,
+                       hdri_size=(128, 64), scalp_freq=3)

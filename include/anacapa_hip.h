@@ -37,6 +37,7 @@ extern "C" {
 #define AHIP_MAT_LAMBERT   0
 #define AHIP_MAT_EMISSIVE  1
 #define AHIP_MAT_STANDARD  2
+#define AHIP_MAT_HAIR      3   /* MarschnerHairMaterial; the reference runtime evaluates Kajiya-Kay */
 
 /* light types (reference: src/shading/lights/AreaLight.h:118-226,
  * src/shading/lights/DomeLight.h:78-388) */
@@ -63,6 +64,12 @@ typedef struct ahip_material {
     float    transmission;
     int32_t  roughness_texture; /* -1 = constant roughness; otherwise index of a texture */
     uint32_t pad[2];
+    float    hair_sigma_a[3];   /* MarschnerHairMaterial::Params (MarschnerHair.h:193-199) */
+    float    hair_eta;
+    float    hair_beta_m;
+    float    hair_beta_n;
+    float    hair_alpha;        /* degrees */
+    uint32_t hair_pad;
 } ahip_material;
 
 typedef struct ahip_light {
@@ -106,6 +113,16 @@ typedef struct ahip_scene_desc {
     const uint32_t*      texture_width;
     const uint32_t*      texture_height;
     const float* const*  texture_pixels;
+
+    /* Hair strands (CurvePool / StrandDesc, include/anacapa/accel/CurvePool.h:27-77): cubic
+     * Bezier segments with shared end points, strand s owns control points
+     * [strand_cv_offset[s], strand_cv_offset[s+1]) (3*segments+1 of them). */
+    uint32_t        num_strands;
+    const uint32_t* strand_cv_offset;     /* [num_strands+1] */
+    const float*    curve_cvs;            /* [total_cvs*3] world space */
+    const float*    strand_width;         /* [num_strands*2] root and tip diameter (StrandDesc::widthAt) */
+    const uint32_t* strand_mesh_index;    /* [num_strands] StrandDesc::materialIndex, reported as meshID */
+    const float*    strand_color;         /* [num_strands*3] StrandDesc::color, may be NULL (white) */
 } ahip_scene_desc;
 
 /* Camera fields of the reference pinhole Camera (IIntegrator.h:27-74). */
@@ -193,10 +210,11 @@ int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_vi
  * rays: [n*8] = origin xyz, direction xyz, tMin, tMax (Ray, include/anacapa/core/Types.h:253-266).
  * hits: one ahip_hit per ray; mesh_id == ~0u on a miss. */
 typedef struct ahip_hit {
-    float    t, u, v;      /* ray parameter and barycentrics of the closest hit */
+    float    t, u, v;      /* ray parameter and barycentrics of the closest hit (curves: u = 0,
+                              v = strand parameter, SurfaceInteraction::uv) */
     uint32_t mesh_id;      /* SurfaceInteraction::meshID (IAccelerationStructure.h:21) */
-    uint32_t prim_id;      /* triangle index within its mesh (SurfaceInteraction::primID) */
-    uint32_t tri_index;    /* index in the flattened (commit-order) triangle array */
+    uint32_t prim_id;      /* triangle index within its mesh / segment index within its strand */
+    uint32_t tri_index;    /* commit-order triangle index, or 0x80000000 | strand index for a curve */
 } ahip_hit;
 int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits);
 int ahip_occluded_rays(ahip_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);

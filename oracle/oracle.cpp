@@ -39,6 +39,9 @@ struct SI {                                                                     
     V2 uv;
     float t = 0.f;
     uint32_t meshID = ~0u, primID = ~0u, tri = ~0u;
+    uint32_t strandID = ~0u;
+    bool isCurve = false;
+    V3 color{1.f, 1.f, 1.f};
 };
 static void fillSI(const Tri& tr, float t, float u, float v, SI& si) {               // BVHBackend.cpp:285-317
     float w = 1.f - u - v;
@@ -59,13 +62,19 @@ static void fillSI(const Tri& tr, float t, float u, float v, SI& si) {          
 struct Ctx {
     V3 p, n, ng, t, bt;
     V2 uv;
+    V3 color{1.f, 1.f, 1.f};
     bool front = true;
     Ctx(const SI& si, V3 rayDir) {
-        p = si.p; ng = si.ng; uv = si.uv;
+        p = si.p; ng = si.ng; uv = si.uv; color = si.color;
         front = dot(-rayDir, ng) > 0.f;
         n = front ? si.n : -si.n;
         ng = front ? si.ng : -si.ng;
-        onb(n, t, bt);
+        if (si.isCurve) {          // fiber tangent frame (ShadingContext.h:39-44)
+            t = si.dpdu;
+            bt = safeNormalize(cross(n, t));
+        } else {
+            onb(n, t, bt);
+        }
     }
     V3 toLocal(V3 v) const { return {dot(v, t), dot(v, bt), dot(v, n)}; }
     V3 toWorld(V3 v) const { return t * v.x + bt * v.y + n * v.z; }
@@ -221,8 +230,42 @@ struct Material {
         return f;
     }
 
+    // MarschnerHairMaterial::effectiveSigmaA (MarschnerHair.h:430-450)
+    V3 hairSigmaA(const Ctx& c) const {
+        V3 col = c.color;
+        if (col.x > 0.98f && col.y > 0.98f && col.z > 0.98f) return {m.hair_sigma_a[0], m.hair_sigma_a[1], m.hair_sigma_a[2]};
+        auto lin = [](float s) { return std::pow(fmaxs(s, 0.0f), 2.2f); };
+        float lr = lin(col.x), lg = lin(col.y), lb = lin(col.z);
+        if (lr < 0.01f && lg < 0.01f && lb < 0.01f) return {m.hair_sigma_a[0], m.hair_sigma_a[1], m.hair_sigma_a[2]};
+        return {-std::log(fmaxs(lr, 0.001f)), -std::log(fmaxs(lg, 0.001f)), -std::log(fmaxs(lb, 0.001f))};
+    }
+    // Kajiya-Kay, the hair BSDF the reference evaluates at run time (MarschnerHair.h:240, 395-422)
+    BSDFEval kkEval(const Ctx& c, V3 wo, V3 wi) const {
+        V3 sa = hairSigmaA(c);
+        V3 alb{std::exp(-sa.x), std::exp(-sa.y), std::exp(-sa.z)};
+        float dTL = dot(c.t, wi), dTV = dot(c.t, wo);
+        float sTL = std::sqrt(fmaxs(0.f, 1.f - dTL * dTL)), sTV = std::sqrt(fmaxs(0.f, 1.f - dTV * dTV));
+        BSDFEval e;
+        e.f = alb * kInvPi;
+        float spec = dTL * dTV + sTL * sTV;
+        if (spec > 0.f) e.f = e.f + V3{0.15f, 0.15f, 0.15f} * (std::pow(spec, 20.f) * kInvPi);
+        e.pdf = fmaxs(0.f, dot(wi, c.n)) * kInvPi;
+        return e;
+    }
+
     BSDFSample sample(const Ctx& c, V3 wo, V2 u, float uc) const {
         BSDFSample s;
+        if (m.type == AHIP_MAT_HAIR) {                                                // MarschnerHair.h:267-282
+            V3 kt, kb;
+            onb(c.n, kt, kb);
+            float kphi = 2.f * kPi * u.x, kcos = std::sqrt(u.y), ksin = std::sqrt(1.f - u.y);
+            V3 wi = safeNormalize(kt * (ksin * std::cos(kphi)) + kb * (ksin * std::sin(kphi)) + c.n * kcos);
+            BSDFEval e = kkEval(c, wo, wi);
+            s.wi = wi;
+            s.f = e.f * kcos;
+            s.pdf = kcos * kInvPi;
+            return s;
+        }
         if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:36-51
             V3 wl = cosineHemisphere(u);
             if (c.toLocal(wo).z < 0.f) wl.z = -wl.z;
@@ -284,6 +327,7 @@ struct Material {
     }
 
     BSDFEval evaluate(const Ctx& c, V3 wo, V3 wi) const {
+        if (m.type == AHIP_MAT_HAIR) return kkEval(c, wo, wi);
         BSDFEval e;
         V3 wil = c.toLocal(wi), wol = c.toLocal(wo);
         if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:53-66
@@ -463,8 +507,88 @@ struct Light {
 };
 
 // ---------------------------------------------------------------- scene + integrator
+// ---------------------------------------------------------------- curve intersection (CurveBrute.cpp:25-234)
+struct RayFrame {
+    V3 o, x, y, z;
+    V3 project(V3 p) const {
+        V3 q = p - o;
+        return {dot(q, x), dot(q, y), dot(q, z)};
+    }
+};
+static V3 bezierTangent(const V3 P[4], float t) {                                     // CurveBrute.cpp:59-64
+    float mt = 1.f - t;
+    return (P[1] - P[0]) * (3.f * mt * mt) + (P[2] - P[1]) * (6.f * mt * t) + (P[3] - P[2]) * (3.f * t * t);
+}
+static float min4(float a, float b, float c, float d) { return fmins(fmins(fmins(a, b), c), d); }
+static float max4(float a, float b, float c, float d) { return fmaxs(fmaxs(fmaxs(a, b), c), d); }
+
+// Recursive ribbon test (CurveBrute.cpp:85-176): de Casteljau midpoint splits to `depth` 0,
+// then the closest point of the leaf chord to the ray in ray space against the interpolated
+// half width; only strictly nearer hits are accepted.
+static bool subdivide(const RayFrame& f, const V3 P[4], float hw0, float hw3, float tOff, float tScale, float tMin,
+                      float& best, int depth, float& outT, V3& outTang) {
+    V3 c0 = f.project(P[0]), c1 = f.project(P[1]), c2 = f.project(P[2]), c3 = f.project(P[3]);
+    float mh = fmaxs(hw0, hw3);
+    float zMin = min4(c0.z, c1.z, c2.z, c3.z), zMax = max4(c0.z, c1.z, c2.z, c3.z);
+    if (zMin - mh > best) return false;
+    if (zMax + mh < tMin) return false;
+    float xMin = min4(c0.x, c1.x, c2.x, c3.x), xMax = max4(c0.x, c1.x, c2.x, c3.x);
+    float yMin = min4(c0.y, c1.y, c2.y, c3.y), yMax = max4(c0.y, c1.y, c2.y, c3.y);
+    float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
+    float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
+    if (cx * cx + cy * cy > mh * mh) return false;
+    if (depth == 0) {
+        float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
+        float l2 = dx * dx + dy * dy;
+        float tl = l2 < 1e-12f ? 0.f : fmaxs(0.f, fmins(1.f, (-ax * dx - ay * dy) / l2));
+        float px = ax + dx * tl, py = ay + dy * tl;
+        float d2 = px * px + py * py;
+        float hw = hw0 * (1.f - tl) + hw3 * tl;
+        if (d2 > hw * hw) return false;
+        float tr = c0.z + (c3.z - c0.z) * tl;
+        if (tr < tMin || tr >= best) return false;
+        V3 tg = bezierTangent(P, tl);
+        if (lengthSq(tg) < 1e-12f) tg = P[3] - P[0];
+        tg = safeNormalize(tg);
+        best = tr;
+        outT = tOff + tl * tScale;
+        outTang = tg;
+        return true;
+    }
+    V3 q01 = (P[0] + P[1]) * 0.5f, q12 = (P[1] + P[2]) * 0.5f, q23 = (P[2] + P[3]) * 0.5f;
+    V3 r01 = (q01 + q12) * 0.5f, r12 = (q12 + q23) * 0.5f, m = (r01 + r12) * 0.5f;
+    V3 L[4] = {P[0], q01, r01, m}, R[4] = {m, r12, q23, P[3]};
+    float hm = (hw0 + hw3) * 0.5f, ts = tScale * 0.5f;
+    bool hit = false;
+    float tt;
+    V3 tg;
+    if (subdivide(f, L, hw0, hm, tOff, ts, tMin, best, depth - 1, tt, tg)) { outT = tt; outTang = tg; hit = true; }
+    if (subdivide(f, R, hm, hw3, tOff + ts, ts, tMin, best, depth - 1, tt, tg)) { outT = tt; outTang = tg; hit = true; }
+    return hit;
+}
+
+// CurveBrute::aabbHit (CurveBrute.cpp:478-498)
+static bool curveBox(const CNode& n, const Ray& r, float maxT) {
+    float tmin = r.tMin, tmax = maxT;
+    for (int i = 0; i < 3; ++i) {
+        float d = r.d[i];
+        if (std::abs(d) < 1e-9f) {
+            if (r.o[i] < n.bmin[i] || r.o[i] > n.bmax[i]) return false;
+            continue;
+        }
+        float inv = 1.f / d;
+        float t0 = (n.bmin[i] - r.o[i]) * inv, t1 = (n.bmax[i] - r.o[i]) * inv;
+        if (inv < 0.f) std::swap(t0, t1);
+        tmin = fmaxs(tmin, t0);
+        tmax = fmins(tmax, t1);
+        if (tmax < tmin) return false;
+    }
+    return true;
+}
+
 struct Scene {
     Bvh bvh;
+    Curves curves;
     std::vector<Material> mats;
     std::vector<int32_t> meshMat;
     std::vector<Light> lights;
@@ -482,13 +606,60 @@ struct Scene {
         int32_t m = meshMat[mesh];
         return m >= 0 ? &mats[m] : nullptr;
     }
-    bool traceSI(const Ray& r, SI& si) const {
-        float t, u, v;
-        uint32_t h = bvh.trace(r, t, u, v);
-        if (h == ~0u) return false;
-        fillSI(bvh.tris[h], t, u, v, si);
-        si.tri = h;
-        return true;
+    // CurveBrute::trace (CurveBrute.cpp:398-440): triangles first, then the curve BVH with the
+    // triangle hit as the bound; children pushed right then left, boxes tested when popped.
+    bool traceSI(const Ray& r, SI& si, uint64_t* nodeCount = nullptr, uint64_t* primCount = nullptr) const {
+        float t = 0.f, u = 0.f, v = 0.f;
+        uint32_t h = bvh.trace(r, t, u, v, nodeCount, primCount);
+        bool hit = h != ~0u;
+        if (hit) {
+            fillSI(bvh.tris[h], t, u, v, si);
+            si.tri = h;
+        }
+        if (curves.nodes.empty()) return hit;
+        float best = hit ? si.t : r.tMax;
+        RayFrame f;
+        f.o = r.o;
+        f.z = r.d;
+        onb(f.z, f.x, f.y);
+        uint32_t st[64];
+        int top = 0;
+        st[top++] = 0;
+        while (top > 0) {
+            const CNode& n = curves.nodes[st[--top]];
+            if (nodeCount) ++*nodeCount;
+            if (!curveBox(n, r, best)) continue;
+            if (n.rc & 0x80000000u) {
+                for (uint32_t i = n.lp; i < n.lp + (n.rc & 0x7fffffffu); ++i) {
+                    const Seg& s = curves.segs[i];
+                    float tSeg = 0.f;
+                    V3 tang;
+                    if (primCount) ++*primCount;
+                    if (!subdivide(f, s.P, s.hw0, s.hw3, s.tOff, s.tScale, r.tMin, best, 6, tSeg, tang)) continue;
+                    // CurveBrute.cpp:218-233
+                    V3 nd = -r.d;
+                    V3 nrm = safeNormalize(nd - tang * dot(nd, tang), f.x);
+                    si = SI{};
+                    si.p = r.o + r.d * best;
+                    si.n = nrm;
+                    si.ng = nrm;
+                    si.dpdu = tang;
+                    si.dpdv = cross(tang, nrm);
+                    si.uv = V2{0.f, tSeg};
+                    si.t = best;
+                    si.meshID = curves.strandMesh[s.strand];
+                    si.primID = s.seg;
+                    si.strandID = s.strand;
+                    si.isCurve = true;
+                    si.color = curves.strandColor[s.strand];
+                    hit = true;
+                }
+            } else {
+                st[top++] = n.rc;
+                st[top++] = n.lp;
+            }
+        }
+        return hit;
     }
     V3 envLe(V3 d) const { return hasEnv ? dome.eval(d) * dome.intensity : envRad; }
 
@@ -520,7 +691,9 @@ struct Scene {
                     V3 Tr = shadowTr(sr);
                     if (!isBlack(Tr)) {
                         float w = power(ls.pdf, be.pdf);
-                        float cosI = absDot(ls.wi, si.n);
+                        // hair: sin of the angle to the fiber (PathIntegrator.cpp:189-193)
+                        float cosI = si.isCurve ? std::sqrt(fmaxs(0.f, 1.f - dot(ls.wi, ctx.t) * dot(ls.wi, ctx.t)))
+                                                : absDot(ls.wi, si.n);
                         Ld = Ld + be.f * ls.Li * Tr * cosI * w / ls.pdf;
                     }
                 }
@@ -703,6 +876,10 @@ int orc_generate_rays(const ahip_camera* c, uint32_t n, const uint32_t* px, cons
 void* orc_scene_create(const ahip_scene_desc* d) {
     auto s = std::make_unique<Scene>();
     s->bvh.commit(d);
+    if (d->num_strands) {
+        if (!d->strand_cv_offset || !d->curve_cvs || !d->strand_width || !d->strand_mesh_index) return nullptr;
+        s->curves.build(d);
+    }
     for (uint32_t i = 0; i < d->num_textures; ++i) {
         Texture t;
         t.w = d->texture_width[i]; t.h = d->texture_height[i];
@@ -784,11 +961,52 @@ int orc_trace(void* h, const float* rays, uint32_t n, ahip_hit* hits, uint32_t* 
         float t = 0.f, u = 0.f, v = 0.f;
         uint64_t nv = 0, tv = 0;
         uint32_t k = s->bvh.trace(r, t, u, v, &nv, &tv);
+        ahip_hit& o = hits[i];
+        if (!s->curves.nodes.empty()) {
+            SI si;
+            nv = tv = 0;
+            bool hit = s->traceSI(r, si, &nv, &tv);
+            if (hit && si.isCurve) {
+                o = {si.t, 0.f, si.uv.y, si.meshID, si.primID, 0x80000000u | si.strandID};
+                if (nodeCount) nodeCount[i] = (uint32_t)nv;
+                if (triCount) triCount[i] = (uint32_t)tv;
+                continue;
+            }
+        }
         if (nodeCount) nodeCount[i] = (uint32_t)nv;
         if (triCount) triCount[i] = (uint32_t)tv;
-        ahip_hit& o = hits[i];
         if (k == ~0u) { o = {0.f, 0.f, 0.f, ~0u, ~0u, ~0u}; continue; }
         o = {t, u, v, s->bvh.tris[k].meshID, s->bvh.tris[k].primID, k};
+    }
+    return 0;
+}
+
+// Curves only, without the curve BVH: every segment in leaf order through the same ribbon test
+// (tree-independence check of the curve BVH).  out[n*4] = t (or 0), strand v, strand, segment
+// (bits; ~0u on a miss).
+int orc_curve_brute(void* h, const float* rays, uint32_t n, float* out) {
+    auto* s = static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = rays + 8 * i;
+        Ray r;
+        r.o = {q[0], q[1], q[2]}; r.d = {q[3], q[4], q[5]}; r.tMin = q[6]; r.tMax = q[7];
+        RayFrame f;
+        f.o = r.o; f.z = r.d;
+        onb(f.z, f.x, f.y);
+        float best = r.tMax, tSeg = 0.f;
+        V3 tang;
+        uint32_t hs = ~0u, hg = ~0u;
+        for (const Seg& sg : s->curves.segs) {
+            float tt;
+            V3 tg;
+            if (subdivide(f, sg.P, sg.hw0, sg.hw3, sg.tOff, sg.tScale, r.tMin, best, 6, tt, tg)) {
+                tSeg = tt; tang = tg; hs = sg.strand; hg = sg.seg;
+            }
+        }
+        out[4 * i] = hs == ~0u ? 0.f : best;
+        out[4 * i + 1] = tSeg;
+        std::memcpy(&out[4 * i + 2], &hs, 4);
+        std::memcpy(&out[4 * i + 3], &hg, 4);
     }
     return 0;
 }
@@ -799,7 +1017,12 @@ int orc_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
         const float* q = rays + 8 * i;
         Ray r;
         r.o = {q[0], q[1], q[2]}; r.d = {q[3], q[4], q[5]}; r.tMin = q[6]; r.tMax = q[7];
-        out[i] = s->bvh.occluded(r) ? 1 : 0;
+        if (s->curves.nodes.empty()) {
+            out[i] = s->bvh.occluded(r) ? 1 : 0;
+        } else {
+            SI si;   // shadowTransmittance decides by a closest-hit trace (ShadowRay.h:26-61)
+            out[i] = s->traceSI(r, si) ? 1 : 0;
+        }
     }
     return 0;
 }

@@ -347,4 +347,23 @@ struct Bvh {
     }
 };
 
+// ---------------------------------------------------------------- curves (accel/CurveBrute.{h,cpp})
+struct CNode {                                                                        // CurveBrute.h:27-34
+    float bmin[3], bmax[3];
+    uint32_t lp, rc;   // interior: left, right child; leaf: first segment, count | 0x80000000
+};
+struct Seg {
+    V3 P[4];
+    float hw0, hw3;        // widthAt(v0) * 0.5, widthAt(v3) * 0.5 (CurveBrute.cpp:199-202)
+    float tOff, tScale;    // v0, v3 - v0
+    uint32_t strand, seg;
+};
+struct Curves {
+    std::vector<CNode> nodes;
+    std::vector<Seg> segs;           // leaf order (CurveBrute m_segRefs)
+    std::vector<uint32_t> strandMesh;
+    std::vector<V3> strandColor;
+    void build(const ahip_scene_desc* d);   // oracle_bvh.cpp (CurveBrute.cpp:247-396)
+};
+
 }  // namespace orc

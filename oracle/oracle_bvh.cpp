@@ -166,5 +166,103 @@ void Bvh::commit(const ahip_scene_desc* d) {                                    
         build(pi, 0, n);
     }
 
+// ---------------------------------------------------------------- curve BVH (CurveBrute.cpp:247-396)
+namespace {
+struct SegWork {
+    uint32_t strand, seg;
+    float bmin[3], bmax[3], c[3];
+};
+
+// StrandDesc::widthAt for the two-width (root, tip) form (CurvePool.h:45-54)
+float widthAt2(float w0, float w1, float v) { return w0 * (1.f - v) + w1 * v; }
+
+uint32_t buildCurves(std::vector<SegWork>& w, std::vector<CNode>& nodes, std::vector<uint32_t>& order, uint32_t s,
+                     uint32_t e) {
+    uint32_t idx = (uint32_t)nodes.size();
+    nodes.push_back({});
+    float bmin[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+    float bmax[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+    for (uint32_t i = s; i < e; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (w[i].bmin[k] < bmin[k]) bmin[k] = w[i].bmin[k];
+            if (w[i].bmax[k] > bmax[k]) bmax[k] = w[i].bmax[k];
+        }
+    uint32_t count = e - s;
+    if (count <= 4) {
+        uint32_t first = (uint32_t)order.size();
+        for (uint32_t i = s; i < e; ++i) order.push_back(w[i].strand);   // index into the segment list
+        for (int k = 0; k < 3; ++k) { nodes[idx].bmin[k] = bmin[k]; nodes[idx].bmax[k] = bmax[k]; }
+        nodes[idx].lp = first;
+        nodes[idx].rc = count | 0x80000000u;
+        return idx;
+    }
+    float cmin[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+    float cmax[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+    for (uint32_t i = s; i < e; ++i)
+        for (int k = 0; k < 3; ++k) {
+            if (w[i].c[k] < cmin[k]) cmin[k] = w[i].c[k];
+            if (w[i].c[k] > cmax[k]) cmax[k] = w[i].c[k];
+        }
+    int axis = 0;
+    float ext[3] = {cmax[0] - cmin[0], cmax[1] - cmin[1], cmax[2] - cmin[2]};
+    if (ext[1] > ext[0]) axis = 1;
+    if (ext[2] > ext[axis]) axis = 2;
+    uint32_t mid = (s + e) / 2;
+    std::nth_element(w.begin() + s, w.begin() + mid, w.begin() + e,
+                     [axis](const SegWork& a, const SegWork& b) { return a.c[axis] < b.c[axis]; });
+    uint32_t l = buildCurves(w, nodes, order, s, mid);
+    uint32_t r = buildCurves(w, nodes, order, mid, e);
+    for (int k = 0; k < 3; ++k) { nodes[idx].bmin[k] = bmin[k]; nodes[idx].bmax[k] = bmax[k]; }
+    nodes[idx].lp = l;
+    nodes[idx].rc = r;
+    return idx;
+}
+}  // namespace
+
+void Curves::build(const ahip_scene_desc* d) {
+    nodes.clear(); segs.clear(); strandMesh.clear(); strandColor.clear();
+    if (!d->num_strands) return;
+    std::vector<SegWork> work;
+    std::vector<Seg> all;
+    for (uint32_t s = 0; s < d->num_strands; ++s) {
+        uint32_t c0 = d->strand_cv_offset[s], c1 = d->strand_cv_offset[s + 1];
+        uint32_t ncv = c1 - c0, N = ncv < 4 ? 0 : (ncv - 1) / 3;
+        float w0 = d->strand_width[2 * s], w1 = d->strand_width[2 * s + 1];
+        strandMesh.push_back(d->strand_mesh_index[s]);
+        strandColor.push_back(d->strand_color ? V3{d->strand_color[3 * s], d->strand_color[3 * s + 1],
+                                                   d->strand_color[3 * s + 2]}
+                                              : V3{1.f, 1.f, 1.f});
+        for (uint32_t g = 0; g < N; ++g) {
+            Seg sg;
+            for (int k = 0; k < 4; ++k) {
+                const float* p = d->curve_cvs + 3 * (size_t)(c0 + 3 * g + k);
+                sg.P[k] = V3{p[0], p[1], p[2]};
+            }
+            float v0 = float(g) / float(N), v3 = float(g + 1) / float(N);
+            float wa = widthAt2(w0, w1, v0), wb = widthAt2(w0, w1, v3);
+            sg.hw0 = wa * 0.5f;
+            sg.hw3 = wb * 0.5f;
+            sg.tOff = v0;
+            sg.tScale = v3 - v0;
+            sg.strand = s;
+            sg.seg = g;
+            float mh = std::max(wa, wb) * 0.5f;
+            SegWork sw;
+            sw.strand = (uint32_t)all.size();   // index into `all`
+            sw.seg = g;
+            for (int k = 0; k < 3; ++k) {
+                sw.bmin[k] = std::min({sg.P[0][k], sg.P[1][k], sg.P[2][k], sg.P[3][k]}) - mh;
+                sw.bmax[k] = std::max({sg.P[0][k], sg.P[1][k], sg.P[2][k], sg.P[3][k]}) + mh;
+                sw.c[k] = (sw.bmin[k] + sw.bmax[k]) * 0.5f;
+            }
+            all.push_back(sg);
+            work.push_back(sw);
+        }
+    }
+    if (work.empty()) return;
+    std::vector<uint32_t> order;
+    buildCurves(work, nodes, order, 0, (uint32_t)work.size());
+    for (uint32_t i : order) segs.push_back(all[i]);
+}
 
 }  // namespace orc
