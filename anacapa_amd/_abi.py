@@ -93,7 +93,7 @@ class Stats(C.Structure):
                 ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64),
                 ("trace_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("shade_launches", C.c_uint64),
-                ("render_ms", C.c_double)]
+                ("render_ms", C.c_double), ("passes", C.c_uint64), ("graph_passes", C.c_uint64)]
 
 
 class Hit(C.Structure):

@@ -89,6 +89,10 @@ struct ahip_context {
 
     // wavefront queues
     DevBuf qRo[2], qRd[2], qBeta[2], qHit, qHitAux, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
+    DevBuf qPix, qPassIdx, dViews, dTables, dBatchAcc, dBatchLsq;
+    uint32_t tileW = 8;          // pixel tile per wavefront: tileW x (64 / tileW)
+    bool frameGraph = false;     // render_frame: per-launch timing events (the roofline figure needs them)
+    bool batchGraph = true;      // render_batch: one captured pass replayed per pass
     uint32_t qCap = 0, qCtrSlots = 0;
     uint32_t traceBlocks = 0, shadeBlocks = 0, stackCap = 0;
 
@@ -173,6 +177,8 @@ int ensureQueues(ahip_context* ctx, uint32_t cap) {
     HIPCHK(ctx->qMd.alloc(f4));
     HIPCHK(ctx->qMc.alloc(f4));
     HIPCHK(ctx->qL.alloc(f4));
+    HIPCHK(ctx->qPix.alloc((size_t)cap * sizeof(uint2)));
+    HIPCHK(ctx->qPassIdx.alloc(sizeof(uint32_t)));
     HIPCHK(ctx->qCtr.alloc(slots * sizeof(uint32_t)));
     HIPCHK(ctx->qTotals.alloc(8 * sizeof(unsigned long long)));
     ctx->qCap = cap;
@@ -207,6 +213,7 @@ WaveQueues queues(ahip_context* ctx) {
     q.md = ctx->qMd.as<float4>();
     q.mc = ctx->qMc.as<float4>();
     q.L = ctx->qL.as<float4>();
+    q.pix = ctx->qPix.as<uint2>();
     q.counters = ctx->qCtr.as<uint32_t>();
     q.ovfRef = ctx->qOvfRef.as<uint32_t>();
     q.ovfT = ctx->qOvfT.as<float>();
@@ -228,81 +235,157 @@ DCamera toDCam(const ahip_camera& c) {
     return d;
 }
 
-// Render samples [s0, s0+ns) of every pixel of region (x0, y0, w, h) and add them into
-// acc / lsq (region-sized).  count != nullptr runs the counter-enabled traversal copies too.
-int renderRegion(ahip_context* ctx, const ahip_camera& cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                 uint32_t s0, uint32_t ns, float4* acc, float* lsq, unsigned long long* countSums) {
-    if (w == 0 || h == 0 || ns == 0) return AHIP_OK;
-    const uint32_t tilesX = (w + 7u) / 8u, tilesY = (h + 7u) / 8u;
-    const uint32_t slots = tilesX * tilesY * 64u;
-    uint32_t perPass = std::max(1u, std::min(ns, ctx->maxPaths / std::max(1u, slots)));
-    uint64_t cap64 = (uint64_t)perPass * slots;
-    if (cap64 > 0x7fffffffull) return fail(ctx, AHIP_E_INVALID_ARG, "region too large for one pass");
-    int st = ensureQueues(ctx, (uint32_t)cap64);
-    if (st) return st;
-    WaveQueues q = queues(ctx);
-    const uint32_t maxDepth = ctx->params.max_depth;
-    PassDesc pd{};
-    pd.cam = toDCam(cam);
-    pd.x0 = x0; pd.y0 = y0; pd.w = w; pd.h = h;
-    pd.tilesX = tilesX;
-    pd.slotsPerSample = slots;
-    pd.seed = ctx->params.seed;
-    pd.maxDepth = maxDepth;
-    pd.minDepth = ctx->params.min_depth;
-    pd.flags = ctx->params.flags;
-    pd.accum = acc;
-    pd.accumLumSq = lsq;
-    hipStream_t s = ctx->stream;
-    for (uint32_t done = 0; done < ns; done += perPass) {
-        pd.sampleStart = s0 + done;
-        pd.samples = std::min(perPass, ns - done);
-        uint32_t paths = pd.samples * slots;
-        launchFillU32(q.counters, ctx->qCtrSlots, 0u, s);
-        {
-            EventPair& e = nextEvent(ctx, 1);
-            hipEventRecord(e.a, s);
-            uint32_t gb = std::min((paths + 255u) / 256u, ctx->shadeBlocks);
-            launchGenerate(pd, q, std::max(gb, 1u), s);
-            hipEventRecord(e.b, s);
-        }
-        uint32_t cur = 0;
-        for (uint32_t b = 0; b <= maxDepth; ++b) {
-            if (countSums) launchCount(ctx->dsc, q, b, cur, 0, ctx->traceBlocks, countSums, s);
-            {
-                EventPair& e = nextEvent(ctx, 0);
-                hipEventRecord(e.a, s);
-                launchExtend(ctx->dsc, q, b, cur, ctx->traceBlocks, s);
-                hipEventRecord(e.b, s);
-            }
-            {
-                EventPair& e = nextEvent(ctx, 1);
-                hipEventRecord(e.a, s);
-                launchShade(ctx->dsc, pd, q, b, cur, ctx->shadeBlocks, s);
-                hipEventRecord(e.b, s);
-            }
-            if (countSums) {
-                launchCount(ctx->dsc, q, b, cur, 1, ctx->traceBlocks, countSums, s);
-                launchCount(ctx->dsc, q, b, cur, 2, ctx->traceBlocks, countSums, s);
-            }
-            {
-                EventPair& e = nextEvent(ctx, 0);
-                hipEventRecord(e.a, s);
-                launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s);
-                launchMis(ctx->dsc, q, b, ctx->traceBlocks, s);
-                hipEventRecord(e.b, s);
-            }
-            cur ^= 1u;
-        }
-        {
-            EventPair& e = nextEvent(ctx, 1);
-            hipEventRecord(e.a, s);
-            launchAccumulate(pd, q, s);
-            hipEventRecord(e.b, s);
-        }
-        tallyCounters(q.counters, maxDepth, ctx->qTotals.as<unsigned long long>(), s);
-        ctx->stats.samples += (uint64_t)w * h * pd.samples;
+// One render request: a camera, a region of its film, a sample range and where to add the sums.
+struct ViewJob {
+    ahip_camera cam;
+    uint32_t x0, y0, w, h;
+    uint32_t s0, ns;
+    float4* acc;
+    float* lsq;
+};
+
+// Pack whole sample indices of the jobs, in order, into passes of at most `cap` paths (a job whose
+// single sample needs more paths than cap raises cap).  Every pass is a list of segments.
+int planPasses(ahip_context* ctx, const std::vector<ViewJob>& jobs, std::vector<ViewDesc>& views,
+               std::vector<PassTable>& tables, uint32_t& cap, uint32_t& maxSlots) {
+    const uint32_t tw = ctx->tileW, th = 64u / ctx->tileW;
+    views.clear();
+    tables.clear();
+    uint64_t cap64 = std::max(64u, ctx->maxPaths);
+    for (const ViewJob& j : jobs) {
+        ViewDesc v{};
+        v.cam = toDCam(j.cam);
+        v.x0 = j.x0; v.y0 = j.y0; v.w = j.w; v.h = j.h;
+        v.tileW = tw;
+        v.tileH = th;
+        v.tilesX = (j.w + tw - 1u) / tw;
+        uint64_t slots = (uint64_t)v.tilesX * ((j.h + th - 1u) / th) * 64u;
+        if (slots > 0x7fffffffull || j.x0 + j.w > 65536u || j.y0 + j.h > 65536u)
+            return fail(ctx, AHIP_E_INVALID_ARG, "view too large");
+        v.slots = (uint32_t)slots;
+        v.accum = j.acc;
+        v.accumLumSq = j.lsq;
+        cap64 = std::max<uint64_t>(cap64, slots);
+        views.push_back(v);
     }
+    if (cap64 > 0x7fffffffull) return fail(ctx, AHIP_E_INVALID_ARG, "pass capacity too large");
+    cap = (uint32_t)cap64;
+    maxSlots = 0;
+    PassTable cur{};
+    auto close = [&]() {
+        if (cur.numSegs) {
+            maxSlots = std::max(maxSlots, cur.totalSlots);
+            tables.push_back(cur);
+        }
+        cur = PassTable{};
+    };
+    for (uint32_t vi = 0; vi < (uint32_t)jobs.size(); ++vi) {
+        const uint32_t slots = views[vi].slots;
+        uint32_t start = jobs[vi].s0, left = jobs[vi].ns;
+        while (left > 0u) {
+            uint32_t fit = (cap - cur.totalPaths) / slots;
+            if (fit == 0u || cur.numSegs == kMaxPassSegments) {
+                close();
+                continue;
+            }
+            uint32_t k = std::min(fit, left);
+            PassSeg& sg = cur.seg[cur.numSegs++];
+            sg = PassSeg{vi, start, k, cur.totalPaths, cur.totalSlots, {0, 0, 0}};
+            cur.totalPaths += k * slots;
+            cur.totalSlots += slots;
+            start += k;
+            left -= k;
+        }
+    }
+    close();
+    return AHIP_OK;
+}
+
+// Launches of one pass.  The kernels find their pass through the device-side pass index, so the
+// same sequence (or a graph captured from it) serves every pass.
+void enqueuePass(ahip_context* ctx, const WaveQueues& q, const RenderConsts& rc, uint32_t cap, uint32_t maxSlots,
+                 bool events, unsigned long long* countSums) {
+    hipStream_t s = ctx->stream;
+    const uint32_t maxDepth = rc.maxDepth;
+    auto timed = [&](int kind, auto&& fn) {
+        if (!events) {
+            fn();
+            return;
+        }
+        EventPair& e = nextEvent(ctx, kind);
+        hipEventRecord(e.a, s);
+        fn();
+        hipEventRecord(e.b, s);
+    };
+    uint32_t gb = std::max(1u, std::min((cap + 255u) / 256u, ctx->shadeBlocks));
+    timed(1, [&] { launchGenerate(rc, q, gb, s); });
+    uint32_t cur = 0;
+    for (uint32_t b = 0; b <= maxDepth; ++b) {
+        if (countSums) launchCount(ctx->dsc, q, b, cur, 0, ctx->traceBlocks, countSums, s);
+        timed(0, [&] { launchExtend(ctx->dsc, q, b, cur, ctx->traceBlocks, s); });
+        timed(1, [&] { launchShade(ctx->dsc, rc, q, b, cur, ctx->shadeBlocks, s); });
+        if (countSums) {
+            launchCount(ctx->dsc, q, b, cur, 1, ctx->traceBlocks, countSums, s);
+            launchCount(ctx->dsc, q, b, cur, 2, ctx->traceBlocks, countSums, s);
+        }
+        timed(0, [&] {
+            launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s);
+            launchMis(ctx->dsc, q, b, ctx->traceBlocks, s);
+        });
+        cur ^= 1u;
+    }
+    uint32_t ab = std::max(1u, std::min((maxSlots + 255u) / 256u, 8192u));
+    timed(1, [&] { launchAccumulate(q, ab, s); });
+    launchPassEnd(q, maxDepth, ctx->qTotals.as<unsigned long long>(), s);
+}
+
+// Render the jobs.  graph = true captures one pass into a HIP graph and replays it per pass
+// (no host work between passes); otherwise every launch is issued with timing events.
+int renderJobs(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool graph, unsigned long long* countSums) {
+    std::vector<ViewDesc> views;
+    std::vector<PassTable> tables;
+    uint32_t cap = 0, maxSlots = 0;
+    int st = planPasses(ctx, jobs, views, tables, cap, maxSlots);
+    if (st) return st;
+    if (tables.empty()) return AHIP_OK;
+    if ((st = ensureQueues(ctx, cap))) return st;
+    HIPCHK(ctx->dViews.alloc(views.size() * sizeof(ViewDesc)));
+    HIPCHK(ctx->dTables.alloc(tables.size() * sizeof(PassTable)));
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->dViews.p, views.data(), views.size() * sizeof(ViewDesc), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ctx->dTables.p, tables.data(), tables.size() * sizeof(PassTable), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));   // the host vectors go out of scope after this call
+    WaveQueues q = queues(ctx);
+    q.views = ctx->dViews.as<ViewDesc>();
+    q.tables = ctx->dTables.as<PassTable>();
+    q.passIdx = ctx->qPassIdx.as<uint32_t>();
+    RenderConsts rc{ctx->params.seed, ctx->params.max_depth, ctx->params.min_depth, ctx->params.flags, 0};
+    launchFillU32(q.counters, ctx->qCtrSlots, 0u, s);
+    launchFillU32(q.passIdx, 1, 0u, s);
+    if (graph && !countSums) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueuePass(ctx, q, rc, cap, maxSlots, false, nullptr);
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) return fail(ctx, AHIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(ctx, AHIP_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        for (size_t p = 0; p < tables.size() && e == hipSuccess; ++p) e = hipGraphLaunch(ge, s);
+        hipError_t e2 = hipStreamSynchronize(s);
+        hipGraphExecDestroy(ge);
+        hipGraphDestroy(g);
+        if (e != hipSuccess || e2 != hipSuccess)
+            return fail(ctx, AHIP_E_HIP, std::string("graph launch: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        ctx->stats.graph_passes += tables.size();
+    } else {
+        for (size_t p = 0; p < tables.size(); ++p) enqueuePass(ctx, q, rc, cap, maxSlots, true, countSums);
+    }
+    for (const ViewJob& j : jobs) ctx->stats.samples += (uint64_t)j.w * j.h * j.ns;
+    ctx->stats.passes += tables.size();
     HIPCHK(hipGetLastError());
     return AHIP_OK;
 }
@@ -499,7 +582,10 @@ int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
 int ahip_set_launch(ahip_context* ctx, uint32_t max_paths, uint32_t tile) {
     ENTER();
     if (max_paths) ctx->maxPaths = std::max(64u, max_paths);
-    (void)tile;
+    if (tile) {
+        if (tile != 4u && tile != 8u && tile != 16u) return fail(ctx, AHIP_E_INVALID_ARG, "tile width must be 4, 8 or 16");
+        ctx->tileW = tile;
+    }
     return AHIP_OK;
 }
 
@@ -513,13 +599,14 @@ int ahip_clear_accum(ahip_context* ctx) {
     return AHIP_OK;
 }
 
-static int renderCall(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns,
-                      float4* acc, float* lsq, unsigned long long* counts) {
+static int renderCall(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool graph, unsigned long long* counts) {
     int st = checkInjected(ctx);
+    if (st) return st;
+    st = ensureQueues(ctx, 64);
     if (st) return st;
     launchFillU32(reinterpret_cast<uint32_t*>(ctx->qTotals.p), 16, 0u, ctx->stream);
     HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
-    st = renderRegion(ctx, ctx->cam, x0, y0, w, h, s0, ns, acc, lsq, counts);
+    st = renderJobs(ctx, jobs, graph, counts);
     if (st) return st;
     HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -546,8 +633,9 @@ int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32
         launchFillF32(ctx->dAccum.as<float>(), (size_t)width * height * 4, 0.f, ctx->stream);
         launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
     }
-    st = renderCall(ctx, 0, 0, width, height, sample_start, sample_count, ctx->dAccum.as<float4>(),
-                    ctx->dAccumLsq.as<float>(), nullptr);
+    std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
+                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>()}};
+    st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
     if (st) return st;
     ctx->lastRenderValid = true;
     ctx->lastW = width; ctx->lastH = height; ctx->lastS0 = sample_start; ctx->lastNs = sample_count;
@@ -591,7 +679,9 @@ int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, ui
     HIPCHK(lsq.alloc((size_t)w * h * sizeof(float)));
     launchFillF32(acc.as<float>(), (size_t)w * h * 4, 0.f, ctx->stream);
     launchFillF32(lsq.as<float>(), (size_t)w * h, 0.f, ctx->stream);
-    st = renderCall(ctx, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(), lsq.as<float>(), nullptr);
+    std::vector<ViewJob> jobs{ViewJob{ctx->cam, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(),
+                                      lsq.as<float>()}};
+    st = renderCall(ctx, jobs, false, nullptr);
     if (st) return st;
     return downloadRegion(ctx, acc, lsq, (size_t)w * h, out);
 }
@@ -600,31 +690,53 @@ int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_vi
     ENTER();
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!views && num_views) return fail(ctx, AHIP_E_INVALID_ARG, "null views");
-    ahip_camera saved = ctx->cam;
-    bool savedHas = ctx->hasCam;
-    ahip_render_params savedP = ctx->params;
-    int st = ensureQueues(ctx, 64);
-    if (st) return st;
+    // One accumulation buffer holds every view's film; views sharing a max depth share passes.
+    std::vector<size_t> off(num_views + 1, 0);
     for (uint32_t v = 0; v < num_views; ++v) {
-        const ahip_view& V = views[v];
-        uint32_t w = V.camera.width, h = V.camera.height;
-        if (!w || !h) return fail(ctx, AHIP_E_INVALID_ARG, "bad view");
-        ctx->cam = V.camera;
-        ctx->params.max_depth = V.max_depth ? V.max_depth : savedP.max_depth;
-        DevBuf acc, lsq;
-        HIPCHK(acc.alloc((size_t)w * h * sizeof(float4)));
-        HIPCHK(lsq.alloc((size_t)w * h * sizeof(float)));
-        launchFillF32(acc.as<float>(), (size_t)w * h * 4, 0.f, ctx->stream);
-        launchFillF32(lsq.as<float>(), (size_t)w * h, 0.f, ctx->stream);
-        st = ensureQueues(ctx, 64);
-        if (!st) st = renderCall(ctx, 0, 0, w, h, V.sample_start, V.sample_count, acc.as<float4>(), lsq.as<float>(), nullptr);
-        if (!st && V.accum) st = downloadRegion(ctx, acc, lsq, (size_t)w * h, V.accum);
+        const ahip_camera& c = views[v].camera;
+        if (!c.width || !c.height) return fail(ctx, AHIP_E_INVALID_ARG, "bad view");
+        off[v + 1] = off[v] + (size_t)c.width * c.height;
+    }
+    if (!num_views) return AHIP_OK;
+    HIPCHK(ctx->dBatchAcc.alloc(off[num_views] * sizeof(float4)));
+    HIPCHK(ctx->dBatchLsq.alloc(off[num_views] * sizeof(float)));
+    launchFillF32(ctx->dBatchAcc.as<float>(), off[num_views] * 4, 0.f, ctx->stream);
+    launchFillF32(ctx->dBatchLsq.as<float>(), off[num_views], 0.f, ctx->stream);
+    ahip_render_params savedP = ctx->params;
+    std::vector<uint32_t> depths;
+    for (uint32_t v = 0; v < num_views; ++v) {
+        uint32_t d = views[v].max_depth ? views[v].max_depth : savedP.max_depth;
+        if (std::find(depths.begin(), depths.end(), d) == depths.end()) depths.push_back(d);
+    }
+    int st = AHIP_OK;
+    double ms = 0.0;
+    for (uint32_t d : depths) {
+        std::vector<ViewJob> jobs;
+        for (uint32_t v = 0; v < num_views; ++v) {
+            const ahip_view& V = views[v];
+            if ((V.max_depth ? V.max_depth : savedP.max_depth) != d) continue;
+            jobs.push_back(ViewJob{V.camera, 0, 0, V.camera.width, V.camera.height, V.sample_start, V.sample_count,
+                                   ctx->dBatchAcc.as<float4>() + off[v], ctx->dBatchLsq.as<float>() + off[v]});
+        }
+        ctx->params.max_depth = d;
+        st = renderCall(ctx, jobs, ctx->batchGraph, nullptr);
+        ms += ctx->stats.render_ms;
         if (st) break;
     }
-    ctx->cam = saved;
-    ctx->hasCam = savedHas;
     ctx->params = savedP;
-    return st;
+    if (st) return st;
+    ctx->stats.render_ms = ms;
+    std::vector<float> a(off[num_views] * 4), l(off[num_views]);
+    HIPCHK(hipMemcpy(a.data(), ctx->dBatchAcc.p, a.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(l.data(), ctx->dBatchLsq.p, l.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t v = 0; v < num_views; ++v) {
+        if (!views[v].accum) continue;
+        for (size_t i = off[v]; i < off[v + 1]; ++i) {
+            float* o = views[v].accum + 5 * (i - off[v]);
+            o[0] = a[4 * i]; o[1] = a[4 * i + 1]; o[2] = a[4 * i + 2]; o[3] = a[4 * i + 3]; o[4] = l[i];
+        }
+    }
+    return AHIP_OK;
 }
 
 int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits) {
@@ -734,8 +846,9 @@ int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height, uint3
     launchFillF32(lsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
     launchFillU32(sums.as<uint32_t>(), 4, 0u, ctx->stream);
     ahip_stats saved = ctx->stats;
-    st = renderCall(ctx, 0, 0, width, height, sample_start, sample_count, acc.as<float4>(), lsq.as<float>(),
-                    sums.as<unsigned long long>());
+    std::vector<ViewJob> jobs{
+        ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count, acc.as<float4>(), lsq.as<float>()}};
+    st = renderCall(ctx, jobs, false, sums.as<unsigned long long>());
     unsigned long long s[2] = {0, 0};
     if (!st) {
         hipError_t e = hipMemcpy(s, sums.p, sizeof(s), hipMemcpyDeviceToHost);

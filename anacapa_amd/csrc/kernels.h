@@ -9,18 +9,33 @@
 
 namespace ahip {
 
-// One wavefront pass: `samples` consecutive sample indices of every pixel of a region.
-struct PassDesc {
+// One camera view (a whole film or a tile of it) with its own accumulation target.
+struct ViewDesc {
     DCamera cam;
-    uint32_t x0, y0, w, h;       // region of the film
-    uint32_t tilesX;             // 8x8 tiles across the region
-    uint32_t slotsPerSample;     // tilesX * tilesY * 64
-    uint32_t sampleStart;        // sample index of the pass's first sample
-    uint32_t samples;
-    uint64_t seed;
-    uint32_t maxDepth, minDepth, flags, pad;
+    uint32_t x0, y0, w, h;       // region of the film rendered by this view
+    uint32_t tileW, tileH;       // pixel tile that one 64-lane wavefront covers (tileW * tileH = 64)
+    uint32_t tilesX;             // tiles across the region
+    uint32_t slots;              // paths per sample index: tilesX * tilesY * 64
     float4* accum;               // region [w*h]: r, g, b sums and weight
     float* accumLumSq;           // region [w*h]
+};
+
+// A pass packs whole sample indices of one or more views into one path range.
+constexpr uint32_t kMaxPassSegments = 128;
+struct PassSeg {
+    uint32_t view, sampleStart, samples;
+    uint32_t pathBase;           // first path id of this segment
+    uint32_t slotBase;           // first accumulate slot of this segment
+    uint32_t pad[3];
+};
+struct PassTable {
+    uint32_t numSegs, totalPaths, totalSlots, pad;
+    PassSeg seg[kMaxPassSegments];
+};
+
+struct RenderConsts {
+    uint64_t seed;
+    uint32_t maxDepth, minDepth, flags, pad;
 };
 
 // Device queues of the bounce loop (all structure-of-arrays, 16-byte elements).
@@ -37,28 +52,33 @@ struct WaveQueues {
     float4* md;       // direction | tMax
     float4* mc;       // weight; times emitted radiance
     float4* L;        // per path radiance
+    uint2* pix;       // per path: px | py << 16, sample index
     uint32_t* counters;  // [(maxDepth + 2) * 4]
     uint32_t* ovfRef;    // traversal stack overflow (entries past the LDS part)
     float* ovfT;
+    const ViewDesc* views;
+    const PassTable* tables;
+    uint32_t* passIdx;   // index of the pass being run; advanced by the pass's last kernel
     uint32_t stackCap;   // total stack entries available per thread
     uint32_t cap;        // path capacity
 };
 
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
+uint32_t trace_block_size();
 void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 void launchFillF32(float* p, size_t n, float v, hipStream_t s);
 void launchHalton(uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx, const uint32_t* dim,
                   uint32_t n, float* out, hipStream_t s);
-void tallyCounters(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, hipStream_t s);
-void launchGenerate(const PassDesc& pd, const WaveQueues& q, uint32_t blocks, hipStream_t s);
+void launchPassEnd(const WaveQueues& q, uint32_t maxDepth, unsigned long long* totals, hipStream_t s);
+void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks, hipStream_t s);
 void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
                   hipStream_t s);
-void launchShade(const DScene& sc, const PassDesc& pd, const WaveQueues& q, uint32_t bounce, uint32_t cur,
+void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, uint32_t bounce, uint32_t cur,
                  uint32_t blocks, hipStream_t s);
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
-void launchAccumulate(const PassDesc& pd, const WaveQueues& q, hipStream_t s);
+void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s);
 void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
                       uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,
                       uint32_t* counts, hipStream_t s);

@@ -53,17 +53,27 @@ __device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
 
-// p0 -> pixel of this pass (8x8 pixel tiles, 64 consecutive paths = one tile).
-__device__ __forceinline__ bool pathPixel(const PassDesc& pd, uint32_t p0, uint32_t& px, uint32_t& py,
-                                          uint32_t& sample) {
-    uint32_t slot = p0 % pd.slotsPerSample, j = p0 / pd.slotsPerSample;
+// Slot of a view -> pixel: 64 consecutive slots cover one tileW x tileH pixel tile, so a 64-lane
+// wavefront traces one coherent tile at bounce 0.
+__device__ __forceinline__ bool slotPixel(const ViewDesc& v, uint32_t slot, uint32_t& lx, uint32_t& ly) {
     uint32_t tile = slot >> 6, lane = slot & 63u;
-    uint32_t tx = tile % pd.tilesX, ty = tile / pd.tilesX;
-    uint32_t lx = tx * 8u + (lane & 7u), ly = ty * 8u + (lane >> 3);
-    px = pd.x0 + lx;
-    py = pd.y0 + ly;
-    sample = pd.sampleStart + j;
-    return lx < pd.w && ly < pd.h;
+    uint32_t tx = tile % v.tilesX, ty = tile / v.tilesX;
+    lx = tx * v.tileW + lane % v.tileW;
+    ly = ty * v.tileH + lane / v.tileW;
+    return lx < v.w && ly < v.h;
+}
+
+// Segment of the pass holding `key` (largest k with base(k) <= key).
+template <bool SLOTS>
+__device__ __forceinline__ uint32_t findSeg(const PassTable& T, uint32_t key) {
+    uint32_t lo = 0, hi = T.numSegs;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        uint32_t b = SLOTS ? T.seg[mid].slotBase : T.seg[mid].pathBase;
+        if (b <= key) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 // --------------------------------------------------------------------------- utility kernels
@@ -84,22 +94,35 @@ __global__ void k_halton_eval(uint64_t seed, const uint32_t* px, const uint32_t*
 // --------------------------------------------------------------------------- generate
 // Camera rays: HaltonSampler::startPixelSample + get2D (jitter) + get2D (lens) + get1D (time)
 // and Camera::generateRay for a pinhole (PathIntegrator.cpp:34-38; IIntegrator.h:90-108).
-__global__ __launch_bounds__(256) void k_generate(PassDesc pd, WaveQueues q) {
-    const uint32_t total = pd.slotsPerSample * pd.samples;
+__global__ __launch_bounds__(256) void k_generate(RenderConsts rc, WaveQueues q) {
+    const PassTable& T = q.tables[*q.passIdx];
+    const uint32_t total = T.totalPaths;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
         uint32_t p0 = base + threadIdx.x;
         uint32_t px = 0, py = 0, s = 0;
-        bool valid = p0 < total && pathPixel(pd, p0, px, py, s);
+        bool valid = false;
+        const ViewDesc* V = nullptr;
+        if (p0 < total) {
+            const PassSeg& sg = T.seg[findSeg<false>(T, p0)];
+            V = q.views + sg.view;
+            uint32_t local = p0 - sg.pathBase, lx, ly;
+            valid = slotPixel(*V, local % V->slots, lx, ly);
+            px = V->x0 + lx;
+            py = V->y0 + ly;
+            s = sg.sampleStart + local / V->slots;
+        }
         V3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f};
         if (valid) {
-            float jx = haltonSample(c_halton, pd.seed, px, py, s, 0u);
-            float jy = haltonSample(c_halton, pd.seed, px, py, s, 1u);
-            float fs = ((float)px + jx) / (float)pd.cam.w;
-            float ft = ((float)(pd.cam.h - 1u - py) + jy) / (float)pd.cam.h;
-            V3 target = ld3(pd.cam.llc) + ld3(pd.cam.hor) * fs + ld3(pd.cam.ver) * ft;
-            o = ld3(pd.cam.origin);
+            const DCamera& cam = V->cam;
+            float jx = haltonSample(c_halton, rc.seed, px, py, s, 0u);
+            float jy = haltonSample(c_halton, rc.seed, px, py, s, 1u);
+            float fs = ((float)px + jx) / (float)cam.w;
+            float ft = ((float)(cam.h - 1u - py) + jy) / (float)cam.h;
+            V3 target = ld3(cam.llc) + ld3(cam.hor) * fs + ld3(cam.ver) * ft;
+            o = ld3(cam.origin);
             d = normalize(target - o);
             q.L[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            q.pix[p0] = make_uint2(px | (py << 16), s);
         }
         uint32_t slot = waveAppend(ctr(q, 0, 0), valid);
         if (valid) {
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- shade
-__global__ __launch_bounds__(256) void k_shade(DScene sc, PassDesc pd, WaveQueues q, uint32_t bounce,
+__global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
                                                uint32_t cur) {
     const uint32_t count = *ctr(q, bounce, 0);
     const uint32_t nxt = cur ^ 1u;
@@ -166,7 +189,10 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, PassDesc pd, WaveQueue
             beta = v3(bb);
             V3 dir = v3(rb);
             uint32_t px, py, sidx;
-            pathPixel(pd, p0, px, py, sidx);
+            uint2 pp = q.pix[p0];
+            px = pp.x & 0xffffu;
+            py = pp.x >> 16;
+            sidx = pp.y;
             uint32_t tri = __float_as_uint(hh.w);
             V3 Ladd{0.f, 0.f, 0.f};
             if (tri == ~0u) {
@@ -405,15 +431,17 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
 
 // --------------------------------------------------------------------------- accumulate
 // PathIntegrator::renderTile's per-pixel sums (PathIntegrator.cpp:27-57) for this pass.
-__global__ __launch_bounds__(256) void k_accumulate(PassDesc pd, WaveQueues q) {
-    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < pd.slotsPerSample;
-         slot += gridDim.x * blockDim.x) {
-        uint32_t px, py, s;
-        if (!pathPixel(pd, slot, px, py, s)) continue;
+__global__ __launch_bounds__(256) void k_accumulate(WaveQueues q) {
+    const PassTable& T = q.tables[*q.passIdx];
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < T.totalSlots; k += gridDim.x * blockDim.x) {
+        const PassSeg& sg = T.seg[findSeg<true>(T, k)];
+        const ViewDesc& V = q.views[sg.view];
+        uint32_t slot = k - sg.slotBase, lx, ly;
+        if (!slotPixel(V, slot, lx, ly)) continue;
         V3 acc{0.f, 0.f, 0.f};
         float lsq = 0.f;
-        for (uint32_t j = 0; j < pd.samples; ++j) {
-            float4 l = q.L[j * pd.slotsPerSample + slot];
+        for (uint32_t j = 0; j < sg.samples; ++j) {
+            float4 l = q.L[sg.pathBase + j * V.slots + slot];
             V3 L = v3(l);
             if (finite3(L)) {
                 acc = acc + L;
@@ -421,10 +449,10 @@ __global__ __launch_bounds__(256) void k_accumulate(PassDesc pd, WaveQueues q) {
                 lsq += lum * lum;
             }
         }
-        size_t o = (size_t)(py - pd.y0) * pd.w + (px - pd.x0);
-        float4 a = pd.accum[o];
-        pd.accum[o] = make_float4(a.x + acc.x, a.y + acc.y, a.z + acc.z, a.w + (float)pd.samples);
-        pd.accumLumSq[o] += lsq;
+        size_t o = (size_t)ly * V.w + lx;
+        float4 a = V.accum[o];
+        V.accum[o] = make_float4(a.x + acc.x, a.y + acc.y, a.z + acc.z, a.w + (float)sg.samples);
+        V.accumLumSq[o] += lsq;
     }
 }
 
@@ -517,12 +545,19 @@ __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t 
     }
 }
 
-// Per-pass ray totals: [0] continuation rays, [1] shadow rays, [2] MIS rays, [3] stack overflows.
-__global__ void k_tally(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals) {
-    if (blockIdx.x != 0u || threadIdx.x >= 4u) return;
-    unsigned long long s = 0ull;
-    for (uint32_t b = 0; b <= maxDepth; ++b) s += counters[b * 4u + threadIdx.x];
-    totals[threadIdx.x] += s;
+// End of a pass: ray totals ([0] continuation, [1] shadow, [2] MIS rays, [3] stack overflows) and
+// the device-side pass index advanced, so the next replay of the same launches runs the next pass.
+__global__ void k_pass_end(uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, uint32_t* passIdx) {
+    if (blockIdx.x != 0u) return;
+    if (threadIdx.x < 4u) {
+        unsigned long long s = 0ull;
+        for (uint32_t b = 0; b <= maxDepth; ++b) s += counters[b * 4u + threadIdx.x];
+        totals[threadIdx.x] += s;
+    }
+    __syncthreads();
+    // counters of the next pass start at zero (a kernel, not a memset, so it can live in a graph)
+    for (uint32_t i = threadIdx.x; i < (maxDepth + 2u) * 4u; i += blockDim.x) counters[i] = 0u;
+    if (threadIdx.x == 0u) *passIdx += 1u;
 }
 
 // --------------------------------------------------------------------------- host-side launchers
@@ -551,19 +586,20 @@ void launchHalton(uint64_t seed, const uint32_t* px, const uint32_t* py, const u
                   uint32_t n, float* out, hipStream_t s) {
     hipLaunchKernelGGL(k_halton_eval, dim3((n + 255u) / 256u), dim3(256), 0, s, seed, px, py, idx, dim, n, out);
 }
-void tallyCounters(const uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, hipStream_t s) {
-    hipLaunchKernelGGL(k_tally, dim3(1), dim3(64), 0, s, counters, maxDepth, totals);
+void launchPassEnd(const WaveQueues& q, uint32_t maxDepth, unsigned long long* totals, hipStream_t s) {
+    hipLaunchKernelGGL(k_pass_end, dim3(1), dim3(256), 0, s, q.counters, maxDepth, totals, q.passIdx);
 }
-void launchGenerate(const PassDesc& pd, const WaveQueues& q, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, pd, q);
+uint32_t trace_block_size() { return kTraceBS; }
+void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, rc, q);
 }
 void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
                   hipStream_t s) {
     hipLaunchKernelGGL((k_extend<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
 }
-void launchShade(const DScene& sc, const PassDesc& pd, const WaveQueues& q, uint32_t bounce, uint32_t cur,
+void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, uint32_t bounce, uint32_t cur,
                  uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, pd, q, bounce, cur);
+    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
@@ -571,9 +607,8 @@ void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL((k_mis<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
-void launchAccumulate(const PassDesc& pd, const WaveQueues& q, hipStream_t s) {
-    uint32_t blocks = (pd.slotsPerSample + 255u) / 256u;
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, pd, q);
+void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, q);
 }
 void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
                       uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,

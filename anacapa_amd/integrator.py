@@ -129,9 +129,17 @@ class HipPathIntegrator:
         self._check(self.lib.ahip_render_tile(self._ctx, x0, y0, w, h, sample_start, sample_count, A.ptr(out)))
         return out
 
-    def render_batch(self, views):
+    def render_batch(self, cameras, sample_starts, sample_counts, max_depth=0):
+        """Render many views of the prepared scene in one batched call (shared passes, one captured
+        HIP graph replayed per pass); returns one [h, w, 5] accumulation array per view."""
+        outs, views = [], []
+        for cam, s0, ns in zip(cameras, sample_starts, sample_counts):
+            out = np.zeros((cam.height, cam.width, 5), np.float32)
+            outs.append(out)
+            views.append(A.View(cam, s0, ns, max_depth, 0, out.ctypes.data_as(A.F32P)))
         arr = (A.View * len(views))(*views)
         self._check(self.lib.ahip_render_batch(self._ctx, arr, len(views)))
+        return outs
 
     # -------------------------------------------------------------- accel / sampler boundary
     def trace_rays(self, rays: np.ndarray) -> np.ndarray:

@@ -164,6 +164,8 @@ typedef struct ahip_stats {
     uint64_t trace_launches;
     uint64_t shade_launches;
     double   render_ms;          /* wall time of the last render call on the stream (HIP events) */
+    uint64_t passes;             /* wavefront passes run */
+    uint64_t graph_passes;       /* of which replayed from a captured HIP graph */
 } ahip_stats;
 
 typedef struct ahip_context ahip_context;
