@@ -328,10 +328,8 @@ void enqueuePass(ahip_context* ctx, const WaveQueues& q, const RenderConsts& rc,
             launchCount(ctx->dsc, q, b, cur, 1, ctx->traceBlocks, countSums, s);
             launchCount(ctx->dsc, q, b, cur, 2, ctx->traceBlocks, countSums, s);
         }
-        timed(0, [&] {
-            launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s);
-            launchMis(ctx->dsc, q, b, ctx->traceBlocks, s);
-        });
+        timed(0, [&] { launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s); });
+        timed(0, [&] { launchMis(ctx->dsc, q, b, ctx->traceBlocks, s); });
         cur ^= 1u;
     }
     uint32_t ab = std::max(1u, std::min((maxSlots + 255u) / 256u, 8192u));
@@ -432,8 +430,10 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     HIPCHK(hipSetDevice(device_ordinal));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device_ordinal));
-    ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
     ctx->numCU = (uint32_t)prop.multiProcessorCount;
+    // the marketing name can be empty (it comes from a lookup table the runtime may not find)
+    ctx->name = std::string(prop.name[0] ? prop.name : "AMD GPU") + " (" + prop.gcnArchName + ", " +
+                std::to_string(ctx->numCU) + " CUs)";
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ctx->renderBeg));
     HIPCHK(hipEventCreate(&ctx->renderEnd));

@@ -27,10 +27,13 @@ __device__ __forceinline__ uint64_t pcgSeededState(uint64_t seed) {
     return (kPcgInc + seed) * kPcgMult + kPcgInc;
 }
 
-__device__ __forceinline__ float radicalInverse(uint32_t b, float invB, uint32_t n) {
+__device__ __forceinline__ float radicalInverse(uint32_t b, float invB, uint32_t magic, uint32_t shifts,
+                                                uint32_t n) {
+    const uint32_t sh1 = shifts & 0xffu, sh2 = shifts >> 8;
     float invBN = 1.f, result = 0.f;
     while (n > 0u) {
-        uint32_t q = n / b;
+        uint32_t t = __umulhi(magic, n);
+        uint32_t q = (t + ((n - t) >> sh1)) >> sh2;       // == n / b exactly
         uint32_t d = n - q * b;
         n = q;
         invBN *= invB;
@@ -46,7 +49,7 @@ __device__ __forceinline__ float haltonSample(const HaltonTables& T, uint64_t se
         uint64_t s = seed ^ ((uint64_t)px * 2654435761ULL) ^ ((uint64_t)py * 805459861ULL) ^
                      ((uint64_t)dim * 999999893ULL);
         uint32_t scramble = pcgOutput(pcgSeededState(s));
-        return radicalInverse(T.prime[dim], T.invPrime[dim], idx ^ scramble);
+        return radicalInverse(T.prime[dim], T.invPrime[dim], T.magic[dim], T.shifts[dim], idx ^ scramble);
     }
     // Past 128 dimensions: the per-pixel PCG stream of startPixelSample (HaltonSampler.h:40, 45-48),
     // advanced once per fallback draw.

@@ -106,9 +106,14 @@ struct DDome {
 };
 
 // Halton bases and their reciprocals (1.f / (float)prime, the reference's invB), in constant memory.
+// Per-dimension base, its float reciprocal and the constants of exact division by the base
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
+// n / b == (t + ((n - t) >> sh1)) >> sh2 with t = mulhi(magic, n), for every 32-bit n.
 struct HaltonTables {
     uint32_t prime[128];
     float invPrime[128];
+    uint32_t magic[128];
+    uint32_t shifts[128];   // sh1 | sh2 << 8
 };
 
 struct DCamera {

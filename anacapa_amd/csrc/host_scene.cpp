@@ -61,7 +61,15 @@ float buildDist(const float* f, uint32_t n, float* func, float* cdf) {
 
 void fillHaltonTables(HaltonTables& t) {
     firstPrimes(t.prime, 128);
-    for (int i = 0; i < 128; ++i) t.invPrime[i] = 1.f / static_cast<float>(t.prime[i]);
+    for (int i = 0; i < 128; ++i) {
+        const uint32_t b = t.prime[i];
+        t.invPrime[i] = 1.f / static_cast<float>(b);
+        uint32_t l = 0;
+        while ((1ull << l) < b) ++l;                       // l = ceil(log2 b)
+        t.magic[i] = static_cast<uint32_t>((((1ull << l) - b) << 32) / b + 1u);
+        const uint32_t sh1 = l < 1u ? l : 1u, sh2 = l > 0u ? l - 1u : 0u;
+        t.shifts[i] = sh1 | (sh2 << 8);
+    }
 }
 
 // Camera::makePinhole (include/anacapa/integrator/IIntegrator.h:49-74)

@@ -412,6 +412,31 @@ def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, f
                     vfov=40.0, width=width, height=height, spp=spp, max_depth=max_depth, seed=seed)
 
 
+@dataclass
+class ViewSpec:
+    frm: tuple
+    at: tuple
+    width: int
+    height: int
+    spp: int
+
+
+def batch_views(sc: Scene, n=64, seed=0, resolutions=((256, 256), (512, 512), (1280, 720), (1920, 1080)),
+                spps=(1, 4, 16, 64)) -> list:
+    """`n` views of a grid scene for the batched form: resolution and spp drawn from the given sets,
+    cameras on a seeded orbit (radius U[28,40], height U[6,16]) looking at the grid centre."""
+    rng = np.random.default_rng(seed)
+    c = float(sc.cam_from[0])                      # the sphere grid is square: centre (c, 0, c)
+    out = []
+    for _ in range(n):
+        w, h = resolutions[rng.integers(len(resolutions))]
+        phi = rng.uniform(0.0, 2.0 * math.pi)
+        r = rng.uniform(28.0, 40.0)
+        frm = (c + r * math.cos(phi), rng.uniform(6.0, 16.0), c + r * math.sin(phi))
+        out.append(ViewSpec(frm, (c, 0.0, c), int(w), int(h), int(spps[rng.integers(len(spps))])))
+    return out
+
+
 def small_ggx(width=96, height=64, spp=4, max_depth=8, seed=0, grid=3, freq=3) -> Scene:
     """A few-hundred-triangle instance of workload 2 for oracle-speed parity tests."""
     return ggx_spheres(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=grid,

@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the untimed node/triangle counting pass")
+    ap.add_argument("--batch-views", type=int, default=64,
+                    help="views of the batched-form comparison (0 = skip it)")
+    ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (read + write), the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -81,8 +84,13 @@ def main():
         raise SystemExit(f"HIP backend invalid: {be.last_error()}")
     t0 = time.time()
     be.prepare(sc)
+    from anacapa_amd import autotune
+    launch = autotune.load(be.name(), autotune.plain_key(args.width, args.height, args.spp, args.depth))
     if args.max_paths:
-        be.set_launch(max_paths=args.max_paths)
+        launch = {"max_paths": args.max_paths, "tile": 8}
+    if launch:
+        be.set_launch(max_paths=launch["max_paths"], tile=launch["tile"])
+    plain_launch = launch and {"max_paths": launch["max_paths"], "tile": launch["tile"]}
     nodes, ntris, depth = be.bvh_info()
     log(f"[bench] prepare (SAH-12 build + upload) {time.time() - t0:.1f}s: {nodes} nodes, depth {depth}, "
         f"device {be.name()}")
@@ -128,6 +136,10 @@ def main():
                     "nodes_per_frame": int(ct["nodes_visited"]), "tris_per_frame": int(ct["prims_tested"]),
                     "rays_per_frame": int(ct["rays_closest"] + ct["rays_shadow"])}
 
+    batched = None
+    if args.batch_views:
+        batched = batched_form(be, sc, args, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args)
@@ -146,14 +158,68 @@ def main():
         "dtype": "f32",
         "data": "synthetic (procedural scene, seed 0)",
         "config": {"workload": f"ggx_spheres {sc.num_triangles} tris, {W}x{H}, {S} spp, depth {args.depth}, HDRI dome",
-                   "mode": "plain per-frame"},
+                   "mode": "plain per-frame", "launch": plain_launch},
         "mrays_per_s": round(mrays, 2),
         "trace_ms_per_frame": round(trace_ms / args.steps, 3),
         "shade_ms_per_frame": round(st["shade_ms"] / args.steps, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
+
+
+def batch_views(be, sc, n, seed):
+    """Cameras and spp of the batched workload: `n` seeded orbit views, resolutions from
+    {256^2, 512^2, 1280x720, 1920x1080}, spp from {1, 4, 16, 64}."""
+    from anacapa_amd import scenes
+    specs = scenes.batch_views(sc, n, seed)
+    cams = [be.pinhole(v.frm, v.at, (0.0, 1.0, 0.0), sc.vfov, v.width, v.height) for v in specs]
+    return cams, [v.spp for v in specs]
+
+
+def batched_form(be, sc, args, log):
+    """Batched persistent form: the views rendered by one ahip_render_batch call (shared passes,
+    one captured HIP graph replayed per pass, launch shape from the autotuner cache) against the
+    same views rendered one after another by the plain per-frame path.  Both produce the same
+    samples of the same paths, so their converged RMSE is equal; the largest per-pixel difference
+    (float summation order only) is reported."""
+    import numpy as np
+    from anacapa_amd import autotune
+    cams, counts = batch_views(be, sc, args.batch_views, args.batch_seed)
+    starts = [0] * len(cams)
+    samples = sum(c.width * c.height * n for c, n in zip(cams, counts))
+    tuned = autotune.load(be.name(), autotune.batch_key(args.batch_views, args.batch_seed, args.depth))
+    if tuned:
+        be.set_launch(max_paths=tuned["max_paths"], tile=tuned["tile"])
+    be.render_batch(cams[:2], starts[:2], [1, 1], args.depth)         # warm: buffers at this capacity
+    be.reset_stats()
+    t0 = time.perf_counter()
+    outs = be.render_batch(cams, starts, counts, args.depth)
+    t_batch = time.perf_counter() - t0
+    st = be.stats()
+    log(f"[bench] batched: {len(cams)} views, {samples / 1e6:.1f} Msamples in {t_batch * 1e3:.1f} ms, "
+        f"{st['passes']} passes ({st['graph_passes']} graph replays)")
+    plain = autotune.load(be.name(), autotune.plain_key(args.width, args.height, args.spp, args.depth))
+    be.set_launch(max_paths=plain["max_paths"] if plain else 1 << 22, tile=plain["tile"] if plain else 8)
+    t_seq, worst = 0.0, 0.0
+    for cam, n, out in zip(cams, counts, outs):
+        t0 = time.perf_counter()
+        be.set_camera(cam)
+        be.clear_accum()
+        be.render_frame(0, n)
+        ref = be.download_accum()
+        t_seq += time.perf_counter() - t0
+        worst = max(worst, float(np.max(np.abs(ref[..., :3] - out[..., :3]) / (np.abs(ref[..., :3]) + 1.0))))
+    be.set_camera(be.pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height))
+    log(f"[bench] sequential: {t_seq * 1e3:.1f} ms (max rel diff {worst:.2e})")
+    return {"views": len(cams), "samples": samples, "depth": args.depth,
+            "msps_batched": round(samples / t_batch / 1e6, 3),
+            "msps_sequential": round(samples / t_seq / 1e6, 3),
+            "speedup": round(t_seq / t_batch, 3), "passes": int(st["passes"]),
+            "graph_passes": int(st["graph_passes"]), "max_rel_diff_vs_sequential": worst,
+            "launch": tuned and {"max_paths": tuned["max_paths"], "tile": tuned["tile"]},
+            "timing": "host wall clock incl. download of every view's film (comparison loop excluded)"}
 
 
 def cpu_baseline(sc, args):
