@@ -70,6 +70,7 @@ class SceneDesc(C.Structure):
         ("texture_pixels", C.POINTER(F32P)),
         ("num_strands", C.c_uint32), ("strand_cv_offset", U32P), ("curve_cvs", F32P), ("strand_width", F32P),
         ("strand_mesh_index", U32P), ("strand_color", F32P),
+        ("tri_albedo", F32P),
     ]
 
 
@@ -128,6 +129,8 @@ SIGNATURES = {
     "ahip_trace_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_occluded_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_halton_eval": (C.c_int, [VP, C.c_uint64, VP, VP, VP, VP, C.c_uint32, VP]),
+    "ahip_set_eval_params": (C.c_int, [VP, VP, C.c_uint32, VP]),
+    "ahip_replay": (C.c_int, [VP, C.c_uint32, C.c_uint32, VP]),
     "ahip_adjoint": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP]),
     "ahip_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
     "ahip_reset_stats": (C.c_int, [VP]),

@@ -15,11 +15,19 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 
-PATH_CHOICES = (1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25)
+PATH_CHOICES = (1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
 TILE_CHOICES = (4, 8, 16)
 
 
+def device_key(name: str) -> str:
+    """Cache key of a device: the architecture and CU count the library reports in parentheses
+    (the marketing name in front of it depends on a lookup table the runtime may not find)."""
+    i, j = name.rfind("("), name.rfind(")")
+    return name[i + 1:j] if 0 <= i < j else name
+
+
 def load(device: str, workload: str):
+    device = device_key(device)
     try:
         with open(CACHE) as f:
             return json.load(f).get(device, {}).get(workload)
@@ -28,6 +36,7 @@ def load(device: str, workload: str):
 
 
 def save(device: str, workload: str, entry: dict):
+    device = device_key(device)
     data = {}
     try:
         with open(CACHE) as f:

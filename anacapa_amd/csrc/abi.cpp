@@ -75,13 +75,21 @@ struct ahip_context {
     std::vector<DevBuf*> texBufs;
     CurveOutput curves;
     DevBuf dCNodes, dSegs, dSegIds, dStrandInfo;
+    DevBuf dTriAlb;                      // sampling-parameter per-triangle albedo (commit order)
+    uint32_t numCommit = 0;              // triangles in commit order
+    std::vector<uint32_t> texW, texH;
     DScene dsc{};
+
+    // detached replay / adjoint
+    DevBuf dAlbE, dTexE, dLout, dRepAcc, dRepLsq, dDLdI, dGAlb, dGTex, dRec, dRepCtr;
+    bool albESet = false, texESet = false;
+    uint32_t diffTex = 0;
 
     // render settings
     ahip_camera cam{};
     bool hasCam = false;
     ahip_render_params params{8, 2, 0, 0, 0};
-    uint32_t maxPaths = 1u << 22;
+    uint32_t maxPaths = 1u << 25;
 
     // persistent accumulation
     DevBuf dAccum, dAccumLsq;
@@ -251,7 +259,7 @@ int planPasses(ahip_context* ctx, const std::vector<ViewJob>& jobs, std::vector<
     const uint32_t tw = ctx->tileW, th = 64u / ctx->tileW;
     views.clear();
     tables.clear();
-    uint64_t cap64 = std::max(64u, ctx->maxPaths);
+    uint64_t need = 0, widest = 64;
     for (const ViewJob& j : jobs) {
         ViewDesc v{};
         v.cam = toDCam(j.cam);
@@ -265,9 +273,13 @@ int planPasses(ahip_context* ctx, const std::vector<ViewJob>& jobs, std::vector<
         v.slots = (uint32_t)slots;
         v.accum = j.acc;
         v.accumLumSq = j.lsq;
-        cap64 = std::max<uint64_t>(cap64, slots);
+        widest = std::max<uint64_t>(widest, slots);
+        need += slots * j.ns;
         views.push_back(v);
     }
+    // no more paths per pass than the jobs need (queue memory scales with the capacity), never
+    // fewer than one sample of the widest view
+    uint64_t cap64 = std::max<uint64_t>(widest, std::min<uint64_t>(std::max(64u, ctx->maxPaths), need));
     if (cap64 > 0x7fffffffull) return fail(ctx, AHIP_E_INVALID_ARG, "pass capacity too large");
     cap = (uint32_t)cap64;
     maxSlots = 0;
@@ -442,6 +454,7 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     HaltonTables T;
     fillHaltonTables(T);
     setHaltonTables(T, ctx->stream);
+    setHaltonTablesReplay(T, ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->valid = true;
     return AHIP_OK;
@@ -503,6 +516,16 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         tex.push_back(DTexture{b->as<float>(), hs.texW[i], hs.texH[i]});
     }
     if ((st = upload(ctx, ctx->dTexDesc, tex.data(), tex.size()))) return st;
+    ctx->texW = hs.texW;
+    ctx->texH = hs.texH;
+    ctx->numCommit = d->num_indices / 3u;
+    ctx->dTriAlb.release();
+    if (d->tri_albedo && ctx->numCommit) {
+        if ((st = upload(ctx, ctx->dTriAlb, d->tri_albedo, 3 * (size_t)ctx->numCommit))) return st;
+        S.triAlbedo = ctx->dTriAlb.as<float>();
+    }
+    ctx->albESet = ctx->texESet = false;
+    ctx->diffTex = 0;
     S.nodes = ctx->dNodes.as<DNode>();
     S.tris = ctx->dTris.as<DTri>();
     S.shade = ctx->dShade.as<DTriShade>();
@@ -811,9 +834,127 @@ int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const
     return AHIP_OK;
 }
 
-int ahip_adjoint(ahip_context* ctx, uint32_t, uint32_t, uint32_t, uint32_t, const float*, float*, float*) {
+// ----------------------------------------------------------------- detached replay / adjoint
+int ahip_set_eval_params(ahip_context* ctx, const float* tri_albedo, uint32_t texture, const float* texels) {
     ENTER();
-    return fail(ctx, AHIP_E_UNSUPPORTED, "adjoint is not built yet");
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (texels && texture >= ctx->texW.size()) return fail(ctx, AHIP_E_INVALID_ARG, "texture index out of range");
+    ctx->diffTex = texture;
+    int st;
+    ctx->albESet = tri_albedo != nullptr;
+    if (tri_albedo && (st = upload(ctx, ctx->dAlbE, tri_albedo, 3 * (size_t)ctx->numCommit))) return st;
+    ctx->texESet = texels != nullptr;
+    if (texels && (st = upload(ctx, ctx->dTexE, texels, (size_t)ctx->texW[texture] * ctx->texH[texture]))) return st;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AHIP_OK;
+}
+
+namespace {
+
+// Shared set-up of ahip_replay / ahip_adjoint: the film of the current camera, samples
+// [s0, s0 + ns), one thread per (pixel, sample) in 8x8 pixel tiles.
+int replayArgs(ahip_context* ctx, uint32_t s0, uint32_t ns, ReplayArgs& a, uint32_t& blocks) {
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
+    if (ctx->dsc.hasCurves) return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support triangle scenes only");
+    if (ns == 0) return fail(ctx, AHIP_E_INVALID_ARG, "sample_count must be positive");
+    a = ReplayArgs{};
+    a.cam = toDCam(ctx->cam);
+    a.w = ctx->cam.width;
+    a.h = ctx->cam.height;
+    a.s0 = s0;
+    a.ns = ns;
+    a.tilesX = (a.w + 7u) / 8u;
+    uint64_t slots = (uint64_t)a.tilesX * ((a.h + 7u) / 8u) * 64u;
+    if (slots * ns > 0xffffffffull - 1u) return fail(ctx, AHIP_E_INVALID_ARG, "too many samples for one call");
+    a.slots = (uint32_t)slots;
+    a.seed = ctx->params.seed;
+    a.maxDepth = ctx->params.max_depth;
+    a.minDepth = ctx->params.min_depth;
+    a.flags = ctx->params.flags;
+    a.diffTex = ctx->diffTex < ctx->texW.size() ? ctx->diffTex : ~0u;
+    if (a.diffTex != ~0u && (ctx->texW[a.diffTex] > 65535u || ctx->texH[a.diffTex] > 65535u))
+        return fail(ctx, AHIP_E_UNSUPPORTED, "differentiable texture wider than 65535 texels");
+    a.albE = ctx->albESet ? ctx->dAlbE.as<float>() : nullptr;
+    a.texE = ctx->texESet ? ctx->dTexE.as<float>() : nullptr;
+    a.maxV = ctx->params.max_depth + 1u;
+    // blocks: a few per CU (the kernel holds ~150-180 VGPRs, 2-3 waves per SIMD), never more
+    // threads than the traversal overflow stacks were sized for
+    const uint32_t bs = replay_block_size();
+    blocks = std::max(1u, ctx->numCU) * 4u;
+    blocks = std::min(blocks, (ctx->traceBlocks * 128u) / bs);
+    a.nthreads = blocks * bs;
+    a.ovfRef = ctx->qOvfRef.as<uint32_t>();
+    a.ovfT = ctx->qOvfT.as<float>();
+    a.stackCap = ctx->stackCap;
+    HIPCHK(ctx->dRepCtr.alloc(sizeof(uint32_t)));
+    a.overflow = ctx->dRepCtr.as<uint32_t>();
+    launchFillU32(a.overflow, 1, 0u, ctx->stream);
+    return AHIP_OK;
+}
+
+}  // namespace
+
+int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count, float* out) {
+    ENTER();
+    if (!out) return fail(ctx, AHIP_E_INVALID_ARG, "null output");
+    ReplayArgs a;
+    uint32_t blocks = 0;
+    int st = replayArgs(ctx, sample_start, sample_count, a, blocks);
+    if (st) return st;
+    if ((st = checkInjected(ctx))) return st;
+    const size_t n = (size_t)a.w * a.h;
+    HIPCHK(ctx->dLout.alloc(n * a.ns * sizeof(float4)));
+    HIPCHK(ctx->dRepAcc.alloc(n * sizeof(float4)));
+    HIPCHK(ctx->dRepLsq.alloc(n * sizeof(float)));
+    a.Lout = ctx->dLout.as<float4>();
+    HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
+    launchReplay(ctx->dsc, a, false, blocks, ctx->stream);
+    launchReplayReduce(a, ctx->dRepAcc.as<float4>(), ctx->dRepLsq.as<float>(), ctx->stream);
+    HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
+    ctx->stats.render_ms = ms;
+    return downloadRegion(ctx, ctx->dRepAcc, ctx->dRepLsq, n, out);
+}
+
+int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start, uint32_t sample_count,
+                 const float* dL_dimage, float* d_albedo, float* d_roughness_tex) {
+    ENTER();
+    if (!dL_dimage) return fail(ctx, AHIP_E_INVALID_ARG, "null dL_dimage");
+    ReplayArgs a;
+    uint32_t blocks = 0;
+    int st = replayArgs(ctx, sample_start, sample_count, a, blocks);
+    if (st) return st;
+    if (width != a.w || height != a.h) return fail(ctx, AHIP_E_INVALID_ARG, "width/height must match the camera film");
+    if ((st = checkInjected(ctx))) return st;
+    const size_t n = (size_t)a.w * a.h;
+    const size_t nAlb = 3 * (size_t)ctx->numCommit;
+    const size_t nTex = a.diffTex != ~0u ? (size_t)ctx->texW[a.diffTex] * ctx->texH[a.diffTex] : 0;
+    if ((st = upload(ctx, ctx->dDLdI, dL_dimage, 3 * n))) return st;
+    HIPCHK(ctx->dGAlb.alloc(std::max<size_t>(nAlb, 1) * sizeof(float)));
+    HIPCHK(ctx->dGTex.alloc(std::max<size_t>(nTex, 1) * sizeof(float)));
+    HIPCHK(ctx->dRec.alloc((size_t)kRecFields * a.maxV * a.nthreads * sizeof(float)));
+    launchFillF32(ctx->dGAlb.as<float>(), nAlb, 0.f, ctx->stream);
+    launchFillF32(ctx->dGTex.as<float>(), nTex, 0.f, ctx->stream);
+    a.dLdI = ctx->dDLdI.as<float>();
+    a.gAlb = d_albedo && nAlb ? ctx->dGAlb.as<float>() : nullptr;
+    a.gTex = d_roughness_tex && nTex ? ctx->dGTex.as<float>() : nullptr;
+    a.rec = ctx->dRec.as<float>();
+    HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
+    launchReplay(ctx->dsc, a, true, blocks, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
+    ctx->stats.render_ms = ms;
+    if (d_albedo && nAlb) HIPCHK(hipMemcpy(d_albedo, ctx->dGAlb.p, nAlb * sizeof(float), hipMemcpyDeviceToHost));
+    if (d_roughness_tex && nTex)
+        HIPCHK(hipMemcpy(d_roughness_tex, ctx->dGTex.p, nTex * sizeof(float), hipMemcpyDeviceToHost));
+    return AHIP_OK;
 }
 
 int ahip_get_stats(ahip_context* ctx, ahip_stats* out) {

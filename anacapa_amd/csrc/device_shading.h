@@ -161,12 +161,11 @@ __device__ __forceinline__ V3 evalLobe(V3 wo, V3 wi, float a2, V3 f0, float& pf)
 }
 
 __device__ __forceinline__ V3 evalMixture(const DMat& m, V3 wo, V3 wi, float wc, float wm, float ws, float wd,
-                                          float a2, float& pdf) {                       // StandardSurface.h:608-661
+                                          float a2, V3 base, float& pdf) {              // StandardSurface.h:608-661
     V3 f{0.f, 0.f, 0.f};
     pdf = 0.f;
     if (!(wo.z * wi.z > 0.f)) return f;
     float pf;
-    V3 base = ld3(m.albedo);
     if (wc > 0.f) { V3 c = evalLobe(wo, wi, m.coatA2, V3{m.coatF0, m.coatF0, m.coatF0}, pf); f = f + c * wc; pdf += pf * wc; }
     if (wm > 0.f) { V3 c = evalLobe(wo, wi, a2, base * m.base, pf); f = f + c * wm; pdf += pf * wm; }
     if (ws > 0.f) { V3 c = evalLobe(wo, wi, a2, ld3(m.specColor) * (m.spec * m.f0d), pf); f = f + c * ws; pdf += pf * ws; }
@@ -182,8 +181,28 @@ __device__ __forceinline__ float matRoughness(const DScene& sc, const DMat& m, V
     return m.roughTex >= 0 ? texBilinear(sc.textures[m.roughTex], uv) : m.rough;
 }
 
+// What a hit contributes to its material: the base colour (the material's, or the hit
+// triangle's own when the scene carries per-triangle albedo; hair keeps its own) and the
+// roughness looked up at the hit's uv (evalTOV).
+struct MatIn {
+    V3 base;
+    float r;
+};
+__device__ __forceinline__ MatIn matInputs(const DScene& sc, const DMat& m, V2 uv, uint32_t commit) {
+    MatIn mi;
+    if (sc.triAlbedo && commit != ~0u && m.type != AHIP_MAT_HAIR) {
+        const float* a = sc.triAlbedo + 3u * (size_t)commit;
+        mi.base = V3{a[0], a[1], a[2]};
+    } else {
+        mi.base = ld3(m.albedo);
+    }
+    mi.r = m.type == AHIP_MAT_STANDARD ? matRoughness(sc, m, uv) : 0.f;
+    return mi;
+}
+
 // IMaterial::sample
-__device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V2 u, float uc) {
+__device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V2 u,
+                             float uc) {
     BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
     if (m.type == AHIP_MAT_HAIR) {                                       // MarschnerHair.h:267-282
         V3 kt, kb;
@@ -208,16 +227,16 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V
         s.wi = toWorld(fr, wl);
         float c = fabsf(wl.z);
         s.pdf = stdmax(0.f, c) * kInvPi;
-        s.f = ld3(m.albedo) * (kInvPi * c);
+        s.f = mi.base * (kInvPi * c);
         return s;
     }
     V3 wol = toLocal(fr, wo);
     if (wol.z <= 0.f) return s;
-    float r = matRoughness(sc, m, uv);
+    float r = mi.r;
     float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
     float wc, wm, ws, wd;
     if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return s;
-    V3 base = ld3(m.albedo);
+    V3 base = mi.base;
     BSample res{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
     if (uc < wc + wm + ws) {
         float a2x;
@@ -252,7 +271,7 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V
     if (!bsValid(res)) return s;
     V3 wil = toLocal(fr, res.wi);
     float pdf;
-    V3 fc = evalMixture(m, wol, wil, wc, wm, ws, wd, a2, pdf);
+    V3 fc = evalMixture(m, wol, wil, wc, wm, ws, wd, a2, base, pdf);
     res.f = fc * fabsf(wil.z);
     res.pdf = pdf;
     res.delta = m.specularFlag != 0u;
@@ -260,20 +279,74 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, V
 }
 
 // IMaterial::evaluate: f without cosine, forward pdf.
-__device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, V2 uv, V3 wo, V3 wi, float& pdf) {
+__device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V3 wi, float& pdf) {
     if (m.type == AHIP_MAT_HAIR) return hairKK(m, fr, wo, wi, pdf);
     V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
     pdf = 0.f;
     if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
     if (m.type != AHIP_MAT_STANDARD) {
         pdf = stdmax(0.f, fabsf(wil.z)) * kInvPi;
-        return ld3(m.albedo) * kInvPi;
+        return mi.base * kInvPi;
     }
-    float r = matRoughness(sc, m, uv);
+    float r = mi.r;
     float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
     float wc, wm, ws, wd;
     if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return V3{0.f, 0.f, 0.f};
-    return evalMixture(m, wol, wil, wc, wm, ws, wd, a2, pdf);
+    return evalMixture(m, wol, wil, wc, wm, ws, wd, a2, mi.base, pdf);
+}
+
+// matEval's f for a Lambert or standard_surface hit at base colour `base` and roughness `r`, with
+// its derivatives: dRho = df_c/dbase_c (the same for every channel: f is linear in the base
+// colour, channel by channel, through the diffuse lobe and the metal lobe's Schlick F0) and
+// dR = df/dr.  Only D*G of the metal and specular lobes depends on r:
+//   dD/da2 = (den - 2 a2 c^2) / (pi den^3),  den = c^2 (a2 - 1) + 1
+//   dG1/da2 = -x (1 - x^2) / (s (x + s)^2),   s = sqrt(a2 + (1 - a2) x^2)
+//   a2 = alpha^2 = r^4 while r^2 > 1e-4 (alpha = max(1e-4, r^2)), so da2/dr = 4 r^3, else 0.
+__device__ V3 matEvalDiff(const DMat& m, const Frame& fr, V3 base, float r, V3 wo, V3 wi, float& dRho, V3& dR) {
+    dRho = 0.f;
+    dR = V3{0.f, 0.f, 0.f};
+    V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
+    if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
+    if (m.type != AHIP_MAT_STANDARD) {
+        dRho = kInvPi;
+        return base * kInvPi;
+    }
+    float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
+    float wc, wm, ws, wd;
+    if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return V3{0.f, 0.f, 0.f};
+    float pdf;
+    V3 f = evalMixture(m, wol, wil, wc, wm, ws, wd, a2, base, pdf);
+    if (wd > 0.f) dRho += m.base * kInvPi * wd;
+    if ((wm > 0.f || ws > 0.f) && wol.z > 0.f && wil.z > 0.f) {
+        V3 wh = safeNormalize(wol + wil);
+        float co = fabsf(wol.z), ci = fabsf(wil.z);
+        if (wh.z > 0.f && co >= 1e-6f && ci >= 1e-6f) {
+            float dVH = stdmax(0.f, dot(wol, wh));
+            float x = 1.f - dVH, x5 = x * x * x * x * x;
+            float D = dGGX(wh.z, a2), G = g2Smith(wol.z, wil.z, a2);
+            float inv4 = 1.f / (4.f * co * ci);
+            if (wm > 0.f) dRho += wm * m.base * (1.f - x5) * D * G * inv4;
+            float c2 = wh.z * wh.z;
+            float den = c2 * (a2 - 1.f) + 1.f;
+            float dD = (den - 2.f * a2 * c2) / (kPi * den * den * den);
+            auto g1 = [&](float xx, float& dg) {
+                float s = sqrtf(a2 + (1.f - a2) * xx * xx);
+                float q = xx + s;
+                dg = -xx * (1.f - xx * xx) / (s * q * q);
+                return 2.f * xx / q;
+            };
+            float dGo, dGi;
+            float Go = g1(co, dGo), Gi = g1(ci, dGi);
+            float dDG = dD * G + D * (dGo * Gi + Go * dGi);
+            float da2 = r * r > 1e-4f ? 4.f * r * r * r : 0.f;
+            float k = dDG * da2 * inv4;
+            V3 acc{0.f, 0.f, 0.f};
+            if (wm > 0.f) acc = acc + schlickColor(dVH, base * m.base) * (wm * k);
+            if (ws > 0.f) acc = acc + schlickColor(dVH, ld3(m.specColor) * (m.spec * m.f0d)) * (ws * k);
+            dR = acc;
+        }
+    }
+    return f;
 }
 
 // ---------------------------------------------------------------- dome light

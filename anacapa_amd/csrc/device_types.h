@@ -144,6 +144,7 @@ struct DScene {
     const uint32_t* segIds;   // [2 * segments]: strand, segment index within the strand
     const float* strandInfo;  // [4 * strands]: colour rgb, meshID (bits)
     uint32_t hasCurves;
+    const float* triAlbedo;   // [3 * tris] per-triangle base colour in commit order, or null
 };
 
 }  // namespace ahip

@@ -63,6 +63,34 @@ struct WaveQueues {
     uint32_t cap;        // path capacity
 };
 
+// Detached path-replay estimator (adjoint.hip): one thread per (pixel, sample) over the film.
+constexpr uint32_t kRecFields = 18;   // floats per differentiable vertex record
+struct ReplayArgs {
+    DCamera cam;
+    uint32_t w, h, s0, ns;
+    uint32_t tilesX, slots;      // 8x8 pixel tiles: slots = tilesX * tilesY * 64 paths per sample
+    uint64_t seed;
+    uint32_t maxDepth, minDepth, flags;
+    uint32_t diffTex;            // texture whose texels are the roughness parameters (~0u: none)
+    const float* albE;           // evaluation albedo [3 * tris] commit order (null: sampling values)
+    const float* texE;           // evaluation texels of diffTex (null: sampling values)
+    float4* Lout;                // primal: radiance per (sample, pixel) [ns * w * h]
+    const float* dLdI;           // adjoint: [h * w * 3]
+    float* gAlb;                 // adjoint: [3 * tris]
+    float* gTex;                 // adjoint: [texels of diffTex]
+    float* rec;                  // adjoint: vertex records [kRecFields * maxV * nthreads]
+    uint32_t maxV, nthreads;
+    uint32_t* ovfRef;            // traversal stack overflow, stride nthreads
+    float* ovfT;
+    uint32_t stackCap;
+    uint32_t* overflow;          // stack overflow counter
+};
+
+void setHaltonTablesReplay(const HaltonTables& t, hipStream_t s);
+uint32_t replay_block_size();
+void launchReplay(const DScene& sc, const ReplayArgs& a, bool adjoint, uint32_t blocks, hipStream_t s);
+void launchReplayReduce(const ReplayArgs& a, float4* accum, float* lsq, hipStream_t s);
+
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
 uint32_t trace_block_size();

@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
             } else {
                 V3 p, n, ng;
                 V2 uv;
-                uint32_t meshID;
+                uint32_t meshID, commit = ~0u;
                 bool isCurve = (tri & kCurveHit) != 0u;
                 Frame fr;
                 if (!isCurve) {
@@ -210,6 +210,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                     V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2);
                     ng = ld3(T->n);
                     meshID = T->meshID;
+                    commit = T->commitIndex;
                     const DTriShade* SH = sc.shade + tri;
                     p = v0 + e1 * u + e2 * v;
                     n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                 int32_t mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 if (mi >= 0) {
                     const DMat& m = sc.mats[mi];
+                    const MatIn mIn = matInputs(sc, m, uv, commit);
                     V3 wo = -dir;
                     if (m.hasLe && (bounce == 0u || specular)) Ladd = beta * ld3(m.Le);   // :128-132
                     if (!m.isDelta && sc.numLights > 0u) {                                // :134-144
@@ -254,7 +256,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                         LSample ls = lightSample(sc, L, p, u2);
                         if (ls.pdf > 0.f && !isBlack(ls.Li)) {
                             float bpdf;
-                            V3 bf = matEval(sc, m, fe, uv, wo, ls.wi, bpdf);
+                            V3 bf = matEval(sc, m, fe, mIn, wo, ls.wi, bpdf);
                             if (!isBlack(bf)) {
                                 // spawnRayTo(si.p, si.n, si.p + wi*dist) (Types.h:277-284)
                                 V3 target = p + ls.wi * ls.dist;
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample(sc, m, fe, uv, wo, ub, uc);
+                        BSample bs = matSample(sc, m, fe, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             float lp = lightPdf(sc, L, p, bs.wi);
                             if (lp > 0.f) {
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample(sc, m, fr, uv, wo, ub, uc);
+                        BSample bs = matSample(sc, m, fr, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             specularNext = bs.delta;
                             beta = beta * (bs.f / bs.pdf);

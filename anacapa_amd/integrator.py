@@ -141,6 +141,30 @@ class HipPathIntegrator:
         self._check(self.lib.ahip_render_batch(self._ctx, arr, len(views)))
         return outs
 
+    # -------------------------------------------------------------- differentiation
+    def set_eval_params(self, tri_albedo=None, texture=0, texels=None):
+        """Evaluation parameters of the detached replay estimator (None = the sampling values)."""
+        a = None if tri_albedo is None else np.ascontiguousarray(tri_albedo, np.float32).reshape(-1)
+        t = None if texels is None else np.ascontiguousarray(texels, np.float32).reshape(-1)
+        self._check(self.lib.ahip_set_eval_params(self._ctx, A.ptr(a), texture, A.ptr(t)))
+
+    def replay(self, sample_start, sample_count) -> np.ndarray:
+        """Image of the detached replay estimator over the current camera's film, [h, w, 5]."""
+        out = np.zeros((self.height, self.width, 5), np.float32)
+        self._check(self.lib.ahip_replay(self._ctx, sample_start, sample_count, A.ptr(out)))
+        return out
+
+    def adjoint(self, sample_start, sample_count, dl_dimage, texture_shape=None):
+        """Gradients (d_albedo [tris, 3], d_texture [h, w] or None) of sum(dl_dimage * I), I the
+        per-pixel mean radiance of the replay estimator, at the current evaluation parameters."""
+        d = np.ascontiguousarray(dl_dimage, np.float32)
+        assert d.shape == (self.height, self.width, 3)
+        da = np.zeros((self.scene.num_triangles, 3), np.float32)
+        dt = np.zeros(texture_shape, np.float32) if texture_shape is not None else None
+        self._check(self.lib.ahip_adjoint(self._ctx, self.width, self.height, sample_start, sample_count,
+                                          A.ptr(d), A.ptr(da), A.ptr(dt)))
+        return da, dt
+
     # -------------------------------------------------------------- accel / sampler boundary
     def trace_rays(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -40,6 +40,7 @@ class Scene:
     strand_width: np.ndarray | None = None            # [S,2] f32 root/tip diameter
     strand_mesh_index: np.ndarray | None = None       # [S] u32
     strand_color: np.ndarray | None = None            # [S,3] f32
+    tri_albedo: np.ndarray | None = None              # [T,3] f32, commit order (per-triangle base colour)
     cam_from: tuple = (0.0, 0.0, 0.0)
     cam_at: tuple = (0.0, 0.0, 1.0)
     cam_up: tuple = (0.0, 1.0, 0.0)
@@ -127,6 +128,9 @@ class Scene:
             d.strand_mesh_index = sm.ctypes.data_as(A.U32P)
             if self.strand_color is not None:
                 d.strand_color = arr(self.strand_color.reshape(-1), np.float32).ctypes.data_as(A.F32P)
+        if self.tri_albedo is not None:
+            assert self.tri_albedo.size == 3 * self.num_triangles
+            d.tri_albedo = arr(self.tri_albedo.reshape(-1), np.float32).ctypes.data_as(A.F32P)
         keep.append(d)
         return d, keep
 
@@ -410,6 +414,39 @@ def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, f
     return mb.scene("ggx_spheres", materials=mats, lights=lights, env_light=0, env_pixels=env,
                     cam_from=(cx, 10.0, -18.0), cam_at=(cx, 0.0, 22.0), cam_up=(0.0, 1.0, 0.0),
                     vfov=40.0, width=width, height=height, spp=spp, max_depth=max_depth, seed=seed)
+
+
+def inverse_scene(width=512, height=512, spp=8, max_depth=4, seed=0, grid=32, freq=7, tex=1024,
+                  hdri_size=(2048, 1024)) -> Scene:
+    """Workload 5 at its ground truth: the workload-2 geometry with a per-triangle base colour
+    (U[0.05, 0.95]^3 per triangle, the albedo parameters) and one tex x tex roughness texture
+    (U[0.05, 1] per texel, sampled through each sphere's UVs) shared by every sphere material."""
+    sc = ggx_spheres(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=grid, freq=freq,
+                     hdri_size=hdri_size)
+    rng = np.random.default_rng(seed + 7919)
+    sc.name = "inverse"
+    sc.tri_albedo = rng.uniform(0.05, 0.95, size=(sc.num_triangles, 3)).astype(np.float32)
+    sc.textures = [rng.uniform(0.05, 1.0, size=(tex, tex)).astype(np.float32)]
+    for m in sc.materials[1:]:
+        m.roughness_texture = 0
+    return sc
+
+
+def perturbed_params(sc: Scene, seed=1, sigma=0.1):
+    """Initial parameters of workload 5: ground truth + N(0, sigma), clamped to [0, 1]."""
+    rng = np.random.default_rng(seed)
+    alb = np.clip(sc.tri_albedo + rng.normal(0.0, sigma, size=sc.tri_albedo.shape), 0.0, 1.0).astype(np.float32)
+    tex = np.clip(sc.textures[0] + rng.normal(0.0, sigma, size=sc.textures[0].shape), 0.0, 1.0).astype(np.float32)
+    return alb, tex
+
+
+def small_inverse(width=32, height=24, spp=2, max_depth=4, seed=0) -> Scene:
+    """A few-hundred-triangle instance of workload 5 for oracle-speed gradient tests."""
+    sc = inverse_scene(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=3, freq=3,
+                       tex=16, hdri_size=(256, 128))
+    c = 1.6                                         # centre of the 3x3 grid
+    sc.cam_from, sc.cam_at = (c, 2.2, c - 3.4), (c, 0.3, c)
+    return sc
 
 
 @dataclass

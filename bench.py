@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--batch-views", type=int, default=64,
                     help="views of the batched-form comparison (0 = skip it)")
     ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (read + write), the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -140,6 +141,10 @@ def main():
     if args.batch_views:
         batched = batched_form(be, sc, args, log)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_form(be, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args)
@@ -165,6 +170,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 
@@ -220,6 +226,51 @@ def batched_form(be, sc, args, log):
             "graph_passes": int(st["graph_passes"]), "max_rel_diff_vs_sequential": worst,
             "launch": tuned and {"max_paths": tuned["max_paths"], "tile": tuned["tile"]},
             "timing": "host wall clock incl. download of every view's film (comparison loop excluded)"}
+
+
+def adjoint_form(be, log):
+    """Workload 5: 1M-triangle spheres with per-triangle albedo and a 1024^2 roughness texture,
+    512x512, 8 spp, depth 4.  Times (HIP events) the plain wavefront forward render, the replay
+    estimator's image and the adjoint pass at the perturbed initial parameters, with the image
+    L2 loss against the ground-truth render as dL/dI."""
+    import numpy as np
+    from anacapa_amd import scenes
+    sc = scenes.inverse_scene()
+    be.prepare(sc)
+    be.set_eval_params()
+    t = be.replay(0, sc.spp)
+    target = t[..., :3] / t[..., 3:4]
+    alb0, tex0 = scenes.perturbed_params(sc)
+    sc.tri_albedo, sc.textures = alb0, [tex0]
+    be.prepare(sc)
+    be.set_eval_params()
+
+    def best(fn, reps=3):
+        ms = []
+        for _ in range(reps):
+            fn()
+            ms.append(be.stats()["render_ms"])
+        return min(ms)
+
+    def fwd():
+        be.clear_accum()
+        be.render_frame(0, sc.spp)
+
+    fwd()
+    fwd_ms = best(fwd)
+    img = be.replay(0, sc.spp)
+    I = img[..., :3] / img[..., 3:4]
+    rep_ms = best(lambda: be.replay(0, sc.spp))
+    dldi = (2.0 * (I - target)).astype(np.float32)
+    adj_ms = best(lambda: be.adjoint(0, sc.spp, dldi, tex0.shape))
+    da, dt = be.adjoint(0, sc.spp, dldi, tex0.shape)
+    log(f"[bench] adjoint: forward {fwd_ms:.2f} ms, replay {rep_ms:.2f} ms, adjoint {adj_ms:.2f} ms")
+    return {"workload": f"inverse_scene {sc.num_triangles} tris, 512x512, {sc.spp} spp, depth {sc.max_depth}, "
+                        f"1024^2 roughness texture",
+            "forward_ms": round(fwd_ms, 3), "replay_ms": round(rep_ms, 3), "adjoint_ms": round(adj_ms, 3),
+            "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
+            "nonzero_albedo_grads": int((da != 0).sum()), "nonzero_texel_grads": int((dt != 0).sum()),
+            "timing": "HIP events around the kernels of one call (best of 3)"}
 
 
 def cpu_baseline(sc, args):

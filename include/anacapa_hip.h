@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define AHIP_ABI_VERSION 1
+#define AHIP_ABI_VERSION 2
 
 /* status codes */
 #define AHIP_OK              0
@@ -123,6 +123,11 @@ typedef struct ahip_scene_desc {
     const float*    strand_width;         /* [num_strands*2] root and tip diameter (StrandDesc::widthAt) */
     const uint32_t* strand_mesh_index;    /* [num_strands] StrandDesc::materialIndex, reported as meshID */
     const float*    strand_color;         /* [num_strands*3] StrandDesc::color, may be NULL (white) */
+    /* Per-triangle base colour, [num_indices/3 * 3] in commit order (mesh by mesh, triangle by
+     * triangle), replacing the albedo / base_color of the triangle's Lambert or standard_surface
+     * material; NULL = the materials' own.  Equivalent to one material per triangle in the
+     * reference; it is the albedo parameter vector of ahip_adjoint. */
+    const float*    tri_albedo;
 } ahip_scene_desc;
 
 /* Camera fields of the reference pinhole Camera (IIntegrator.h:27-74). */
@@ -226,8 +231,28 @@ int ahip_occluded_rays(ahip_context* ctx, const float* rays, uint32_t n, uint8_t
 int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const uint32_t* py,
                      const uint32_t* idx, const uint32_t* dim, uint32_t n, float* out);
 
-/* --- differentiation: path-replay adjoint of an image L2 loss w.r.t. per-triangle albedo
- *     and the roughness texture (no reference counterpart; see header of adjoint kernels). --- */
+/* --- differentiation (no counterpart in the reference, which has no adjoint) ---
+ * The differentiated estimator is the fixed-seed path-replay estimator with detached sampling:
+ * every random decision (Halton dims, lobe choice, sampled direction, light choice, Russian
+ * roulette) and every pdf / MIS weight is taken at the *sampling* parameters (the prepared
+ * scene: its tri_albedo and the roughness texture's texels), while the BSDF values f are taken at
+ * the *evaluation* parameters set here.  Its image is smooth in the evaluation parameters, equals
+ * the ordinary render when both sets agree, and its derivative is what ahip_adjoint returns.
+ *
+ * ahip_set_eval_params: evaluation parameters.  tri_albedo [num_tris*3] commit order (NULL =
+ * the sampling values); texture = index of the differentiable roughness texture among the
+ * scene's textures; texels [w*h] of that texture (NULL = the sampling values). */
+int ahip_set_eval_params(ahip_context* ctx, const float* tri_albedo, uint32_t texture, const float* texels);
+
+/* Image of the detached estimator for the current camera, [height*width*5] like
+ * ahip_render_tile (r, g, b sums, weight, luminance-squared sum), full film. */
+int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count, float* out);
+
+/* Gradient of sum(dL_dimage * I) with I = per-pixel mean radiance of the detached estimator
+ * (full film of the current camera, samples [sample_start, +sample_count)), with respect to the
+ * evaluation parameters at their current values.  dL_dimage [height*width*3]; d_albedo
+ * [num_tris*3] commit order and d_roughness_tex [texture w*h] are overwritten (either may be NULL).
+ * Paths are regenerated from their Halton indices, not stored; per-path state is O(max_depth). */
 int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
                  uint32_t sample_count, const float* dL_dimage,
                  float* d_albedo, float* d_roughness_tex);

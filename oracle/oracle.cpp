@@ -42,6 +42,8 @@ struct SI {                                                                     
     uint32_t strandID = ~0u;
     bool isCurve = false;
     V3 color{1.f, 1.f, 1.f};
+    bool hasAlb = false;   // per-triangle base colour (ahip_scene_desc::tri_albedo)
+    V3 alb;
 };
 static void fillSI(const Tri& tr, float t, float u, float v, SI& si) {               // BVHBackend.cpp:285-317
     float w = 1.f - u - v;
@@ -64,8 +66,11 @@ struct Ctx {
     V2 uv;
     V3 color{1.f, 1.f, 1.f};
     bool front = true;
+    bool hasAlb = false;
+    V3 alb;
     Ctx(const SI& si, V3 rayDir) {
         p = si.p; ng = si.ng; uv = si.uv; color = si.color;
+        hasAlb = si.hasAlb; alb = si.alb;
         front = dot(-rayDir, ng) > 0.f;
         n = front ? si.n : -si.n;
         ng = front ? si.ng : -si.ng;
@@ -97,7 +102,31 @@ struct Texture {
         float p10 = px[(size_t)yb * w + xa], p11 = px[(size_t)yb * w + xb];
         return (1.f - ay) * ((1.f - ax) * p00 + ax * p01) + ay * ((1.f - ax) * p10 + ax * p11);
     }
+    // The four texels sample() blends and their weights (for scattering a derivative).
+    void taps(V2 uv, uint32_t idx[4], float wt[4], float& axo, float& ayo) const {
+        float s = uv.x, tt = 1.f - uv.y;
+        float fx = s * (float)w - 0.5f, fy = tt * (float)h - 0.5f;
+        float x0f = std::floor(fx), y0f = std::floor(fy);
+        float ax = fx - x0f, ay = fy - y0f;
+        int x0 = (int)x0f, y0 = (int)y0f;
+        auto wrap = [](int i, int n) { int r = i % n; return r < 0 ? r + n : r; };
+        int xa = wrap(x0, (int)w), xb = wrap(x0 + 1, (int)w), ya = wrap(y0, (int)h), yb = wrap(y0 + 1, (int)h);
+        idx[0] = ya * w + xa; idx[1] = ya * w + xb; idx[2] = yb * w + xa; idx[3] = yb * w + xb;
+        wt[0] = (1.f - ay) * (1.f - ax); wt[1] = (1.f - ay) * ax; wt[2] = ay * (1.f - ax); wt[3] = ay * ax;
+        axo = ax; ayo = ay;
+    }
 };
+
+// Forward-mode dual number (value, derivative) for the roughness derivative of the GGX lobes.
+struct Dual {
+    float v, d;
+};
+static inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+static inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+static inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+static inline Dual dsqrt(Dual a) { float r = std::sqrt(a.v); return {r, a.d / (2.f * r)}; }
+static inline Dual dconst(float c) { return {c, 0.f}; }
 
 // ---------------------------------------------------------------- materials
 struct BSDFSample { V3 f, wi; float pdf = 0.f; bool delta = false;
@@ -165,6 +194,7 @@ struct Material {
         coatF0 = F0ior(1.5f);
     }
     V3 albedo() const { return {m.albedo[0], m.albedo[1], m.albedo[2]}; }
+    V3 baseOf(const Ctx& c) const { return c.hasAlb ? c.alb : albedo(); }
     float rough(V2 uv) const { return roughTex ? roughTex->sample(uv) : m.roughness; }  // evalTOV
     bool isDelta() const {                                                            // StandardSurface.h:162-169
         if (m.type != AHIP_MAT_STANDARD) return false;
@@ -271,13 +301,13 @@ struct Material {
             if (c.toLocal(wo).z < 0.f) wl.z = -wl.z;
             s.wi = c.toWorld(wl);
             s.pdf = cosPdf(std::abs(wl.z));
-            s.f = albedo() * (kInvPi * std::abs(wl.z));
+            s.f = baseOf(c) * (kInvPi * std::abs(wl.z));
             return s;
         }
         // StandardSurface::sample, opaque branch (StandardSurface.h:231-241, 351-405)
         V3 wo_l = c.toLocal(wo);
         if (wo_l.z <= 0.f) return {};
-        V3 base = albedo();
+        V3 base = baseOf(c);
         float metal = m.metalness, spec = m.specular, r = rough(c.uv);
         float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
         float wc, wm, ws, wd;
@@ -332,13 +362,13 @@ struct Material {
         V3 wil = c.toLocal(wi), wol = c.toLocal(wo);
         if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:53-66
             if (!(wil.z * wol.z > 0.f)) return e;
-            e.f = albedo() * kInvPi;
+            e.f = baseOf(c) * kInvPi;
             e.pdf = cosPdf(std::abs(wil.z));
             return e;
         }
         // StandardSurface::evaluate opaque branch (StandardSurface.h:410-417, 448-503)
         if (!(wol.z * wil.z > 0.f)) return e;
-        V3 base = albedo();
+        V3 base = baseOf(c);
         float metal = m.metalness, spec = m.specular, r = rough(c.uv);
         float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
         float wc, wm, ws, wd;
@@ -347,6 +377,58 @@ struct Material {
         if (!ok) return e;
         e.f = evalCombined(wol, wil, wc, wm, ws, wd, base, spec, a2, e.pdf);
         return e;
+    }
+
+    // f(wo, wi) (no cosine) at base colour `base` and roughness `r`, with dRho = df_c/dbase_c
+    // (one value for every channel: f is linear in the base colour, channel by channel) and
+    // dR = df/dr.  Lambert and standard_surface only.
+    V3 evalDiff(const Ctx& c, V3 wo, V3 wi, V3 base, float r, float& dRho, V3& dR) const {
+        dRho = 0.f;
+        dR = {};
+        V3 wil = c.toLocal(wi), wol = c.toLocal(wo);
+        if (!(wil.z * wol.z > 0.f)) return {};
+        if (m.type != AHIP_MAT_STANDARD) {
+            dRho = kInvPi;
+            return base * kInvPi;
+        }
+        float metal = m.metalness, spec = m.specular;
+        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
+        float wc, wm, ws, wd;
+        bool ok;
+        weights(wol.z, metal, spec, wc, wm, ws, wd, ok);
+        if (!ok) return {};
+        float pdf;
+        V3 f = evalCombined(wol, wil, wc, wm, ws, wd, base, spec, a2, pdf);
+        V3 f1 = evalCombined(wol, wil, wc, wm, ws, wd, V3{1.f, 1.f, 1.f}, spec, a2, pdf);
+        V3 f0 = evalCombined(wol, wil, wc, wm, ws, wd, V3{}, spec, a2, pdf);
+        dRho = f1.x - f0.x;
+        // roughness: only D*G of the metal and specular lobes depends on it
+        if ((wm > 0.f || ws > 0.f) && wol.z > 0.f && wil.z > 0.f) {
+            V3 wh = safeNormalize(wol + wil);
+            float co = std::abs(wol.z), ci = std::abs(wil.z);
+            if (wh.z > 0.f && co >= 1e-6f && ci >= 1e-6f) {
+                float dVH = fmaxs(0.f, dot(wol, wh));
+                Dual rr{r, 1.f};
+                Dual al = (r * r > 1e-4f) ? rr * rr : dconst(1e-4f);
+                Dual a2d = al * al;
+                Dual ch = dconst(wh.z);
+                Dual den = ch * ch * (a2d - dconst(1.f)) + dconst(1.f);
+                Dual D = a2d / (dconst(kPi) * den * den);
+                auto g1 = [&](float cc) {
+                    Dual C = dconst(cc);
+                    return dconst(2.f) * C / (C + dsqrt(a2d + (dconst(1.f) - a2d) * C * C));
+                };
+                Dual G = g1(co) * g1(ci);
+                float dDG = (D * G).d / (4.f * co * ci);
+                V3 Fm = schlickC(dVH, base * m.base);
+                V3 Fs = schlickC(dVH, V3{m.specular_color[0], m.specular_color[1], m.specular_color[2]} * (spec * f0d));
+                V3 acc;
+                if (wm > 0.f) acc = acc + Fm * (wm * dDG);
+                if (ws > 0.f) acc = acc + Fs * (ws * dDG);
+                dR = acc;
+            }
+        }
+        return f;
     }
 };
 
@@ -600,6 +682,11 @@ struct Scene {
     uint32_t maxDepth = 8, minDepth = 2;
     uint64_t seed = 0;
     bool fixDoubleCos = false;
+    std::vector<float> triAlbedo;           // [ntris*3] commit order, or empty
+    // evaluation parameters of the detached replay estimator (empty = the sampling values)
+    std::vector<float> albE;
+    std::vector<float> texE;
+    uint32_t diffTex = 0;
 
     const Material* matOf(uint32_t mesh) const {
         if (mesh >= meshMat.size()) return nullptr;
@@ -615,6 +702,10 @@ struct Scene {
         if (hit) {
             fillSI(bvh.tris[h], t, u, v, si);
             si.tri = h;
+            if (!triAlbedo.empty()) {
+                si.hasAlb = true;
+                si.alb = V3{triAlbedo[3 * h], triAlbedo[3 * h + 1], triAlbedo[3 * h + 2]};
+            }
         }
         if (curves.nodes.empty()) return hit;
         float best = hit ? si.t : r.tMax;
@@ -771,6 +862,208 @@ struct Scene {
         return L;
     }
 
+    // ------------------------------------------------------------ detached replay estimator
+    // The bounce loop of Li with every random decision, pdf, MIS weight and Russian-roulette
+    // probability taken at the sampling parameters (the scene) and every BSDF value f at the
+    // evaluation parameters (albE / texE).  With `g` set it also returns the gradient of
+    // sum_c delta_c * L_c with respect to the evaluation parameters, by leave-one-out throughput
+    // products: for the continuation weight w_j of vertex j, dL/dw_j = sum over later emission
+    // terms E_k of (prod_{i<k, i!=j} w_i) E_k, accumulated in one forward sweep.
+    struct Grad {
+        V3 delta;                               // dLoss/dL of this path (per channel)
+        std::vector<double>* dAlb = nullptr;    // [ntris*3]
+        std::vector<double>* dTex = nullptr;    // [texels]
+    };
+    struct EvalPt {                             // evaluation-parameter view of a hit
+        bool ok = false;
+        uint32_t tri = ~0u;
+        V3 base;
+        float r = 0.f;
+        bool texR = false;
+        uint32_t taps[4] = {0, 0, 0, 0};
+        float tw[4] = {0.f, 0.f, 0.f, 0.f};
+    };
+    EvalPt evalPoint(const SI& si, const Ctx& ctx, const Material& m) const {
+        EvalPt e;
+        if (si.isCurve || m.m.type == AHIP_MAT_HAIR) return e;
+        e.ok = true;
+        e.tri = si.tri;
+        e.base = m.baseOf(ctx);
+        if (!albE.empty() && si.tri != ~0u)
+            e.base = V3{albE[3 * si.tri], albE[3 * si.tri + 1], albE[3 * si.tri + 2]};
+        e.r = m.rough(ctx.uv);
+        if (m.m.type == AHIP_MAT_STANDARD && m.m.roughness_texture >= 0 && (uint32_t)m.m.roughness_texture == diffTex) {
+            const Texture& T = textures[diffTex];
+            e.texR = true;
+            float ax, ay;
+            T.taps(ctx.uv, e.taps, e.tw, ax, ay);
+            if (!texE.empty()) {
+                float p[4];
+                for (int k = 0; k < 4; ++k) p[k] = texE[e.taps[k]];
+                // the blend of Texture::sample over the evaluation texels
+                e.r = (1.f - ay) * ((1.f - ax) * p[0] + ax * p[1]) + ay * ((1.f - ax) * p[2] + ax * p[3]);
+            }
+        }
+        return e;
+    }
+    // One differentiable vertex j of a path: leave-one-out throughput texcl = prod_{i != j} w_i
+    // over the weights so far, acc = sum of later emission terms seen through texcl (= dL/dw_j),
+    // the continuation weight's derivatives (sRho = dw_c/dbase_c, gR = dw/dr) and the summed
+    // coefficients of the vertex's own NEE terms (neeA per channel for the base colour, neeR per
+    // channel for the roughness).
+    struct Vtx {
+        EvalPt e;
+        V3 texcl, acc;
+        float sRho = 0.f;
+        V3 gR, neeA, neeR;
+    };
+    // d(sum_c delta_c L_c) of one vertex, scattered into g
+    void scatter(const Grad& g, const Vtx& v) const {
+        V3 ka = g.delta * (v.acc * v.sRho + v.neeA);
+        V3 kr = g.delta * (v.acc * v.gR + v.neeR);
+        if (g.dAlb && v.e.tri != ~0u) {
+            (*g.dAlb)[3 * v.e.tri] += ka.x;
+            (*g.dAlb)[3 * v.e.tri + 1] += ka.y;
+            (*g.dAlb)[3 * v.e.tri + 2] += ka.z;
+        }
+        if (g.dTex && v.e.texR) {
+            double dr = (double)kr.x + (double)kr.y + (double)kr.z;
+            for (int q = 0; q < 4; ++q) (*g.dTex)[v.e.taps[q]] += dr * v.e.tw[q];
+        }
+    }
+
+    V3 LiReplay(Ray r, Halton& smp, const Grad* g) const {
+        std::vector<Vtx> vs;
+        V3 L, betaS{1.f, 1.f, 1.f}, betaE{1.f, 1.f, 1.f};
+        const bool evalSet = !albE.empty() || !texE.empty();
+        auto emit = [&](V3 E, float k) {        // emission term E * k (before the throughput)
+            L = L + betaE * E * k;
+            for (Vtx& v : vs) v.acc = v.acc + v.texcl * (E * k);
+        };
+        bool specular = false;
+        for (uint32_t bounce = 0; bounce <= maxDepth; ++bounce) {
+            SI si;
+            if (!traceSI(r, si)) {
+                if (bounce == 0 || specular) emit(envLe(r.d), 1.f);
+                break;
+            }
+            const Material* mat = matOf(si.meshID);
+            if (!mat) break;
+            V3 wo = -r.d;
+            Ctx ctx(si, r.d);
+            Vtx cur;
+            cur.e = evalPoint(si, ctx, *mat);
+            cur.texcl = betaE;
+            const EvalPt& ep = cur.e;
+            V3 Le = mat->Le();
+            if (!isBlack(Le) && (bounce == 0 || specular)) emit(Le, 1.f);
+            if (!mat->isDelta() && !lights.empty()) {
+                float nL = (float)lights.size();
+                uint32_t li = (uint32_t)(smp.get1D() * nL);
+                li = std::min(li, (uint32_t)(lights.size() - 1));
+                const Light& light = lights[li];
+                V3 Ld;
+                {
+                    V2 u = smp.get2D();
+                    LightSample ls = light.sample(si.p, u);
+                    if (ls.pdf > 0.f && !isBlack(ls.Li)) {
+                        BSDFEval be = mat->evaluate(ctx, wo, ls.wi);          // sampling params: pdf
+                        if (!isBlack(be.f)) {
+                            Ray sr = spawnRayTo(si.p, si.n, si.p + ls.wi * ls.dist);
+                            V3 Tr = shadowTr(sr);
+                            if (!isBlack(Tr)) {
+                                float w = power(ls.pdf, be.pdf);
+                                float cosI = si.isCurve ? std::sqrt(fmaxs(0.f, 1.f - dot(ls.wi, ctx.t) * dot(ls.wi, ctx.t)))
+                                                        : absDot(ls.wi, si.n);
+                                V3 fe = be.f;
+                                if (ep.ok) {
+                                    float dr;
+                                    V3 gr;
+                                    V3 fd = mat->evalDiff(ctx, wo, ls.wi, ep.base, ep.r, dr, gr);
+                                    if (evalSet) fe = fd;
+                                    V3 c = betaE * (ls.Li * Tr * cosI * w / ls.pdf) * nL;
+                                    cur.neeA = cur.neeA + c * dr;
+                                    cur.neeR = cur.neeR + c * gr;
+                                }
+                                Ld = Ld + fe * ls.Li * Tr * cosI * w / ls.pdf;
+                            }
+                        }
+                    }
+                }
+                {
+                    float uc = smp.get1D();
+                    V2 u = smp.get2D();
+                    BSDFSample bs = mat->sample(ctx, wo, u, uc);
+                    if (bs.valid()) {
+                        float lp = light.pdf(si.p, bs.wi);
+                        if (lp > 0.f) {
+                            float w = bs.delta ? 1.f : power(bs.pdf, lp);
+                            Ray sr = spawnRay(si.p, si.ng, bs.wi);
+                            sr.tMax = 1e10f;
+                            V3 Tr{1.f, 1.f, 1.f};
+                            SI hs;
+                            bool hit = traceSI(sr, hs);
+                            if (hit && !matOf(hs.meshID)) Tr = {};
+                            V3 Li;
+                            if (!hit) Li = envLe(bs.wi);
+                            else if (const Material* hm = matOf(hs.meshID)) Li = hm->Le();
+                            if (!isBlack(Li) && !isBlack(Tr)) {
+                                float cosI = fixDoubleCos ? 1.f : absDot(bs.wi, si.n);
+                                V3 fe = bs.f;      // includes |cos| at the sampling parameters
+                                if (ep.ok) {
+                                    float cw = std::abs(ctx.toLocal(bs.wi).z);
+                                    float dr;
+                                    V3 gr;
+                                    V3 fd = mat->evalDiff(ctx, wo, bs.wi, ep.base, ep.r, dr, gr) * cw;
+                                    if (evalSet) fe = fd;
+                                    V3 c = betaE * (Li * Tr * cosI * w / bs.pdf) * nL;
+                                    cur.neeA = cur.neeA + c * (dr * cw);
+                                    cur.neeR = cur.neeR + c * (gr * cw);
+                                }
+                                Ld = Ld + fe * Li * Tr * cosI * w / bs.pdf;
+                            }
+                        }
+                    }
+                }
+                emit(Ld, nL);
+            }
+            float uc = smp.get1D();
+            V2 u = smp.get2D();
+            BSDFSample bs = mat->sample(ctx, wo, u, uc);
+            if (!bs.valid()) {
+                if (ep.ok) vs.push_back(cur);
+                break;
+            }
+            specular = bs.delta;
+            V3 wS = bs.f / bs.pdf, wE = wS;
+            if (ep.ok) {
+                float cw = std::abs(ctx.toLocal(bs.wi).z);
+                float dr;
+                V3 gr;
+                V3 fe = mat->evalDiff(ctx, wo, bs.wi, ep.base, ep.r, dr, gr);
+                if (evalSet) wE = fe * cw / bs.pdf;
+                cur.sRho = dr * cw / bs.pdf;
+                cur.gR = gr * (cw / bs.pdf);
+            }
+            for (Vtx& o : vs) o.texcl = o.texcl * wE;
+            if (ep.ok) vs.push_back(cur);
+            betaS = betaS * wS;
+            betaE = betaE * wE;
+            r = spawnRay(si.p, si.n, bs.wi);
+            if (bounce >= minDepth) {
+                float q = 1.f - fmins(maxComponent(betaS), 0.95f);
+                if (smp.get1D() < q) break;
+                float k = 1.f / (1.f - q);
+                betaS = betaS * k;
+                betaE = betaE * k;
+                for (Vtx& o : vs) o.texcl = o.texcl * k;
+            }
+        }
+        if (g && isFinite(L))
+            for (const Vtx& v : vs) scatter(*g, v);
+        return L;
+    }
+
     // PathIntegrator::renderTile (PathIntegrator.cpp:8-69): out[w*h*5] = r,g,b (pre-weighted), w, lumSq
     void renderTile(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float* out) const {
         Halton smp;
@@ -898,6 +1191,7 @@ void* orc_scene_create(const ahip_scene_desc* d) {
         s->mats.push_back(m);
     }
     s->meshMat.assign(d->mesh_material, d->mesh_material + d->num_meshes);
+    if (d->tri_albedo) s->triAlbedo.assign(d->tri_albedo, d->tri_albedo + 3 * (size_t)s->bvh.tris.size());
     if (d->env_light >= 0) {
         const ahip_light& l = d->lights[d->env_light];
         s->dome.W = d->env_width; s->dome.H = d->env_height;
@@ -1029,6 +1323,77 @@ int orc_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
 
 int orc_render_tile(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns, float* out) {
     static_cast<Scene*>(h)->renderTile(x0, y0, w, hh, s0, ns, out);
+    return 0;
+}
+
+// Evaluation parameters of the detached replay estimator (NULL = the sampling values).
+int orc_set_eval(void* h, const float* triAlbedo, uint32_t texture, const float* texels) {
+    auto* s = static_cast<Scene*>(h);
+    s->diffTex = texture;
+    if (triAlbedo) s->albE.assign(triAlbedo, triAlbedo + 3 * s->bvh.tris.size());
+    else s->albE.clear();
+    if (texels && texture < s->textures.size())
+        s->texE.assign(texels, texels + (size_t)s->textures[texture].w * s->textures[texture].h);
+    else s->texE.clear();
+    return 0;
+}
+
+// Image of the detached replay estimator over a tile, laid out like orc_render_tile.
+int orc_replay_tile(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns, float* out) {
+    auto* s = static_cast<Scene*>(h);
+    Halton smp;
+    smp.seed = s->seed;
+    for (uint32_t ty = 0; ty < hh; ++ty)
+        for (uint32_t tx = 0; tx < w; ++tx) {
+            uint32_t px = x0 + tx, py = y0 + ty;
+            V3 acc;
+            float lsq = 0.f;
+            for (uint32_t k = 0; k < ns; ++k) {
+                smp.start(px, py, s0 + k);
+                V2 jit = smp.get2D();
+                smp.get2D();
+                smp.get1D();
+                V3 L = s->LiReplay(generateRay(s->cam, px, py, jit.x, jit.y), smp, nullptr);
+                if (isFinite(L)) {
+                    acc = acc + L;
+                    float lum = luminance(L);
+                    lsq += lum * lum;
+                }
+            }
+            float* o = out + 5 * (ty * w + tx);
+            o[0] = acc.x; o[1] = acc.y; o[2] = acc.z; o[3] = (float)ns; o[4] = lsq;
+        }
+    return 0;
+}
+
+// Gradient of sum(dLdI * I), I = per-pixel mean of the detached estimator over samples
+// [s0, s0+ns) of the whole film, w.r.t. the evaluation parameters: dAlb[ntris*3], dTex[texels]
+// (either may be NULL), accumulated in double and returned as float.
+int orc_adjoint(void* h, uint32_t s0, uint32_t ns, const float* dLdI, float* dAlb, float* dTex) {
+    auto* s = static_cast<Scene*>(h);
+    std::vector<double> ga(dAlb ? 3 * s->bvh.tris.size() : 0), gt;
+    if (dTex && s->diffTex < s->textures.size()) gt.resize((size_t)s->textures[s->diffTex].w * s->textures[s->diffTex].h);
+    Scene::Grad g;
+    g.dAlb = dAlb ? &ga : nullptr;
+    g.dTex = gt.empty() ? nullptr : &gt;
+    Halton smp;
+    smp.seed = s->seed;
+    const uint32_t W = s->cam.w, H = s->cam.h;
+    for (uint32_t py = 0; py < H; ++py)
+        for (uint32_t px = 0; px < W; ++px) {
+            const float* d = dLdI + 3 * ((size_t)py * W + px);
+            g.delta = V3{d[0], d[1], d[2]} * (1.f / (float)ns);
+            for (uint32_t k = 0; k < ns; ++k) {
+                smp.start(px, py, s0 + k);
+                V2 jit = smp.get2D();
+                smp.get2D();
+                smp.get1D();
+                s->LiReplay(generateRay(s->cam, px, py, jit.x, jit.y), smp, &g);
+            }
+        }
+    for (size_t i = 0; i < ga.size(); ++i) dAlb[i] = (float)ga[i];
+    if (dTex)
+        for (size_t i = 0; i < gt.size(); ++i) dTex[i] = (float)gt[i];
     return 0;
 }
 

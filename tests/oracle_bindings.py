@@ -56,6 +56,9 @@ class Oracle:
         L.orc_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
         L.orc_dome_sample.argtypes = [VP, C.c_uint32, VP, VP]
         L.orc_dome_eval.argtypes = [VP, C.c_uint32, VP, VP, VP]
+        L.orc_set_eval.argtypes = [VP, VP, C.c_uint32, VP]
+        L.orc_replay_tile.argtypes = [VP] + [C.c_uint32] * 6 + [VP]
+        L.orc_adjoint.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP]
 
     # ---- stateless
     def pcg(self, seed, seq, n):
@@ -170,6 +173,26 @@ class OracleScene:
         out = np.zeros((h, w, 5), np.float32)
         self.orc.lib.orc_render_tile(self.h, x0, y0, w, h, s0, ns, P(out))
         return out
+
+    def set_eval(self, tri_albedo=None, texture=0, texels=None):
+        """Evaluation parameters of the detached replay estimator (None = sampling values)."""
+        self._eval_keep = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)
+                           for a in (tri_albedo, texels)]
+        a, t = self._eval_keep
+        self.orc.lib.orc_set_eval(self.h, P(a), texture, P(t))
+
+    def replay_tile(self, x0, y0, w, h, s0, ns):
+        out = np.zeros((h, w, 5), np.float32)
+        self.orc.lib.orc_replay_tile(self.h, x0, y0, w, h, s0, ns, P(out))
+        return out
+
+    def adjoint(self, s0, ns, dldi, texels_shape=None):
+        """Gradients (d_albedo [T,3], d_texture [h,w] or None) of sum(dldi * image)."""
+        dldi = np.ascontiguousarray(dldi, np.float32)
+        da = np.zeros((self.sc.num_triangles, 3), np.float32)
+        dt = np.zeros(texels_shape, np.float32) if texels_shape is not None else None
+        self.orc.lib.orc_adjoint(self.h, s0, ns, P(dldi), P(da), P(dt))
+        return da, dt
 
     def render_samples(self, px, py, idx):
         px, py, idx = (np.ascontiguousarray(a, np.uint32) for a in (px, py, idx))

@@ -1,0 +1,171 @@
+"""Detached path-replay estimator and its adjoint on the GPU.
+
+- replay at the sampling parameters equals the wavefront render (same paths; only the order of
+  per-pixel float sums differs);
+- replay at perturbed evaluation parameters matches the oracle's replay to the radiance tolerance;
+- the adjoint matches the oracle's adjoint (float atomics and association only);
+- the adjoint matches central differences (h = 1e-3) of the GPU replay's image L2 loss to 1e-2
+  relative: on the small scene, and at workload-5 scale (1M triangles, 1024^2 roughness texture,
+  512x512x8, depth 4) on 256 randomly chosen parameters;
+- a torch.autograd.Function over replay/adjoint gives the same gradient and descends the loss.
+"""
+import numpy as np
+import pytest
+
+from anacapa_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _radiance_close(got, ref, rel=1e-3, frac=0.99):
+    g = got[..., :3].astype(np.float64)
+    r = ref[..., :3].astype(np.float64)
+    assert np.array_equal(got[..., 3], ref[..., 3])
+    scale = max(np.abs(r).mean(), 1e-6)
+    err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
+    ok = err <= rel
+    assert ok.mean() >= frac, f"{(~ok).sum()} of {ok.size} pixels differ by more than {rel} (max {err.max()})"
+
+
+@pytest.fixture(scope="module")
+def small(gpu, oracle):
+    sc = scenes.small_inverse()
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    alb0, tex0 = scenes.perturbed_params(sc)
+    return sc, os_, alb0, tex0
+
+
+def test_replay_at_sampling_params_equals_render(gpu, small):
+    sc, _, _, _ = small
+    gpu.prepare(sc)
+    gpu.set_eval_params()
+    a = gpu.replay(0, sc.spp)
+    b = gpu.render_tile(0, 0, sc.width, sc.height, 0, sc.spp)
+    assert np.array_equal(a[..., 3], b[..., 3])
+    np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
+
+
+def test_replay_matches_oracle_at_eval_params(gpu, small):
+    sc, os_, alb0, tex0 = small
+    gpu.prepare(sc)
+    gpu.set_eval_params(alb0, 0, tex0)
+    os_.set_eval(alb0, 0, tex0)
+    got = gpu.replay(0, sc.spp)
+    ref = os_.replay_tile(0, 0, sc.width, sc.height, 0, sc.spp)
+    os_.set_eval()
+    _radiance_close(got, ref)
+
+
+def test_adjoint_matches_oracle_adjoint(gpu, small):
+    sc, os_, alb0, tex0 = small
+    gpu.prepare(sc)
+    rng = np.random.default_rng(11)
+    dldi = rng.normal(size=(sc.height, sc.width, 3)).astype(np.float32)
+    gpu.set_eval_params(alb0, 0, tex0)
+    os_.set_eval(alb0, 0, tex0)
+    ga, gt = gpu.adjoint(0, sc.spp, dldi, tex0.shape)
+    oa, ot = os_.adjoint(0, sc.spp, dldi, tex0.shape)
+    os_.set_eval()
+    for g, o in ((ga, oa), (gt, ot)):
+        assert np.isfinite(g).all()
+        err = np.linalg.norm(g.astype(np.float64) - o) / np.linalg.norm(o)
+        assert err < 2e-3, err
+        big = np.abs(o) > 1e-2 * np.abs(o).max()
+        np.testing.assert_allclose(g[big], o[big], rtol=1e-2)
+
+
+def _loss(gpu, sc, target, alb, tex):
+    gpu.set_eval_params(alb, 0, tex)
+    img = gpu.replay(0, sc.spp)
+    I = img[..., :3].astype(np.float64) / img[..., 3:4]
+    return float(((I - target) ** 2).sum()), I
+
+
+def _fd_check(gpu, sc, target, alb0, tex0, n_each, seed, h=1e-3):
+    L0, I0 = _loss(gpu, sc, target, alb0, tex0)
+    dldi = (2.0 * (I0 - target)).astype(np.float32)
+    gpu.set_eval_params(alb0, 0, tex0)
+    da, dt = gpu.adjoint(0, sc.spp, dldi, tex0.shape)
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for grad, kind in ((da, "albedo"), (dt, "texel")):
+        # parameters whose gradient is not negligible (most triangles are not seen at all)
+        cand = np.argwhere(np.abs(grad) > 1e-3 * np.abs(grad).max())
+        assert len(cand) >= n_each, (kind, len(cand))
+        for k in rng.choice(len(cand), n_each, replace=False):
+            idx = tuple(cand[k])
+            a_p, a_m, t_p, t_m = alb0, alb0, tex0, tex0
+            if kind == "albedo":
+                a_p, a_m = alb0.copy(), alb0.copy()
+                a_p[idx] += h
+                a_m[idx] -= h
+            else:
+                t_p, t_m = tex0.copy(), tex0.copy()
+                t_p[idx] += h
+                t_m[idx] -= h
+            fd = (_loss(gpu, sc, target, a_p, t_p)[0] - _loss(gpu, sc, target, a_m, t_m)[0]) / (2 * h)
+            rel = abs(grad[idx] - fd) / abs(fd)
+            worst = max(worst, rel)
+            assert rel <= 1e-2, (kind, idx, float(grad[idx]), fd)
+    # parameters with a zero gradient leave the image bit-identical
+    zero = np.argwhere(da == 0)
+    for k in rng.choice(len(zero), 4, replace=False):
+        idx = tuple(zero[k])
+        a_p = alb0.copy()
+        a_p[idx] += h
+        assert _loss(gpu, sc, target, a_p, tex0)[0] == L0
+    return worst
+
+
+def test_adjoint_matches_central_differences_small(gpu, small):
+    sc, _, alb0, tex0 = small
+    gpu.prepare(sc)
+    gpu.set_eval_params()
+    t = gpu.replay(0, sc.spp)
+    target = t[..., :3].astype(np.float64) / t[..., 3:4]
+    worst = _fd_check(gpu, sc, target, alb0, tex0, 12, 21)
+    print(f"[adjoint] small scene: worst relative FD error {worst:.2e}", flush=True)
+
+
+def test_adjoint_matches_central_differences_workload5(gpu):
+    sc = scenes.inverse_scene()
+    print(f"[adjoint] workload 5: {sc.num_triangles} triangles, {sc.width}x{sc.height}x{sc.spp}", flush=True)
+    gpu.prepare(sc)
+    gpu.set_eval_params()
+    t = gpu.replay(0, sc.spp)              # target: ground-truth parameters
+    target = t[..., :3].astype(np.float64) / t[..., 3:4]
+    alb0, tex0 = scenes.perturbed_params(sc)
+    # sampling parameters = the initial guess, as an optimiser step would hold them
+    sc.tri_albedo, sc.textures = alb0, [tex0]
+    gpu.prepare(sc)
+    worst = _fd_check(gpu, sc, target, alb0, tex0, 128, 5)
+    print(f"[adjoint] workload 5: 256 parameters, worst relative FD error {worst:.2e}", flush=True)
+
+
+def test_torch_autograd_wrapper(gpu, small):
+    torch = pytest.importorskip("torch")
+    from diff_render import render
+    sc, _, alb0, tex0 = small
+    gpu.prepare(sc)
+    gpu.set_eval_params()
+    t = gpu.replay(0, sc.spp)
+    target = torch.from_numpy(t[..., :3] / t[..., 3:4])
+    alb = torch.tensor(alb0, requires_grad=True)
+    tex = torch.tensor(tex0, requires_grad=True)
+    loss = ((render(gpu, alb, tex, 0, sc.spp) - target) ** 2).sum()
+    loss.backward()
+    img = render(gpu, alb.detach(), tex.detach(), 0, sc.spp).numpy()
+    ga, gt = gpu.adjoint(0, sc.spp, (2.0 * (img - target.numpy())).astype(np.float32), tex0.shape)
+    for g, ref in ((alb.grad.numpy(), ga), (tex.grad.numpy(), gt)):   # float atomics: order only
+        assert np.linalg.norm(g - ref) <= 1e-4 * np.linalg.norm(ref)
+    # a few plain gradient steps on the albedo lower the loss
+    losses = []
+    a = alb0.copy()
+    for _ in range(4):
+        at = torch.tensor(a, requires_grad=True)
+        lo = ((render(gpu, at, torch.tensor(tex0), 0, sc.spp) - target) ** 2).sum()
+        lo.backward()
+        losses.append(float(lo))
+        a = np.clip(a - 0.05 * at.grad.numpy() / max(1e-12, np.abs(at.grad.numpy()).max()), 0, 1)
+    assert losses[-1] < losses[0], losses
