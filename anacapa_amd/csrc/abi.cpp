@@ -449,7 +449,7 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ctx->renderBeg));
     HIPCHK(hipEventCreate(&ctx->renderEnd));
-    ctx->traceBlocks = std::max(1u, ctx->numCU) * 10u;
+    ctx->traceBlocks = std::max(1u, ctx->numCU) * 16u;   // 8 waves per SIMD of 128-thread blocks
     ctx->shadeBlocks = std::max(1u, ctx->numCU) * 8u;
     HaltonTables T;
     fillHaltonTables(T);
@@ -893,6 +893,13 @@ int replayArgs(ahip_context* ctx, uint32_t s0, uint32_t ns, ReplayArgs& a, uint3
     return AHIP_OK;
 }
 
+int replayOverflowCheck(ahip_context* ctx) {
+    uint32_t ovf = 0;
+    HIPCHK(hipMemcpy(&ovf, ctx->dRepCtr.p, sizeof(ovf), hipMemcpyDeviceToHost));
+    if (ovf) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
 }  // namespace
 
 int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count, float* out) {
@@ -917,6 +924,7 @@ int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count,
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
     ctx->stats.render_ms = ms;
+    if ((st = replayOverflowCheck(ctx))) return st;
     return downloadRegion(ctx, ctx->dRepAcc, ctx->dRepLsq, n, out);
 }
 
@@ -951,6 +959,7 @@ int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sa
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
     ctx->stats.render_ms = ms;
+    if ((st = replayOverflowCheck(ctx))) return st;
     if (d_albedo && nAlb) HIPCHK(hipMemcpy(d_albedo, ctx->dGAlb.p, nAlb * sizeof(float), hipMemcpyDeviceToHost));
     if (d_roughness_tex && nTex)
         HIPCHK(hipMemcpy(d_roughness_tex, ctx->dGTex.p, nTex * sizeof(float), hipMemcpyDeviceToHost));

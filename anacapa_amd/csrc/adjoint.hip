@@ -378,18 +378,21 @@ __global__ __launch_bounds__(kReplayBS) void k_replay(DScene sc, ReplayArgs a) {
             a.Lout[(size_t)sl * a.w * a.h + pix] = make_float4(L.x, L.y, L.z, 0.f);
         } else if (finite3(L)) {
             for (uint32_t j = 0; j < nv; ++j) {
+                // A derivative that overflowed float (a near-singular GGX lobe seen through a tiny
+                // pdf) can only give inf or inf * 0: such a vertex term is dropped, as the forward
+                // drops a non-finite radiance sample.
                 V3 acc = R.get3(F_ACC, j);
                 V3 ka = delta * acc * R.at(F_SRHO, j) + R.get3(F_KA, j);
                 uint32_t c = __float_as_uint(R.at(F_TRI, j));
-                if (a.gAlb) {
+                if (a.gAlb && finite3(ka)) {
                     atomicAdd(a.gAlb + 3u * (size_t)c, ka.x);
                     atomicAdd(a.gAlb + 3u * (size_t)c + 1u, ka.y);
                     atomicAdd(a.gAlb + 3u * (size_t)c + 2u, ka.z);
                 }
                 uint32_t tap = __float_as_uint(R.at(F_TAP, j));
-                if (a.gTex && tap != ~0u) {
-                    V3 uu = R.get3(F_U, j);
-                    float kr = uu.x * acc.x + uu.y * acc.y + uu.z * acc.z + R.at(F_KR, j);
+                V3 uu = R.get3(F_U, j);
+                float kr = uu.x * acc.x + uu.y * acc.y + uu.z * acc.z + R.at(F_KR, j);
+                if (a.gTex && tap != ~0u && isfinite(kr)) {
                     const DTexture& tx = sc.textures[a.diffTex];
                     uint32_t ti[4];
                     tapIdx(tx.w, tx.h, tap & 0xffffu, tap >> 16, ti);

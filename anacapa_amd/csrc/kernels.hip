@@ -38,17 +38,41 @@ __device__ __forceinline__ uint32_t* ctr(const WaveQueues& q, uint32_t bounce, u
     return q.counters + bounce * 4u + k;
 }
 
-__device__ __forceinline__ uint32_t waveAppend(uint32_t* counter, bool want) {
-    uint64_t mask = __ballot(want ? 1 : 0);
-    if (mask == 0ull) return 0u;
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = (uint32_t)__shfl((int)base, (int)leader);
-    uint64_t below = (lane == 0u) ? 0ull : (mask & (~0ull >> (64u - lane)));
-    return base + (uint32_t)__popcll(below);
+// Block-level append to NQ queues: ballots per wave, a prefix over the block's waves in LDS and
+// one atomic per queue per block (a wave-level atomic per queue made the shared counters the
+// bottleneck at tens of millions of paths per pass).  Every thread of the block must call it.
+template <uint32_t NQ, uint32_t WAVES>
+__device__ __forceinline__ void blockAppend(uint32_t* const* ctrs, const bool* want, uint32_t* slot,
+                                            uint32_t* lds /* [NQ * (WAVES + 1)] */) {
+    const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+    const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+    uint32_t rank[NQ];
+#pragma unroll
+    for (uint32_t k = 0; k < NQ; ++k) {
+        uint64_t mask = __ballot(want[k] ? 1 : 0);
+        rank[k] = (uint32_t)__popcll(mask & below);
+        if (lane == 0u) lds[k * WAVES + wave] = (uint32_t)__popcll(mask);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const uint32_t k = threadIdx.x;
+        uint32_t total = 0u;
+        for (uint32_t w = 0; w < WAVES; ++w) {
+            uint32_t c = lds[k * WAVES + w];
+            lds[k * WAVES + w] = total;
+            total += c;
+        }
+        lds[NQ * WAVES + k] = total ? atomicAdd(ctrs[k], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < NQ; ++k) slot[k] = lds[NQ * WAVES + k] + lds[k * WAVES + wave] + rank[k];
+    __syncthreads();   // the LDS is reused by the next call
 }
+
+// Path flag in the dimension word of a queued ray: a padding slot outside its view.
+constexpr uint32_t kDeadPath = 1u << 30;
+constexpr uint32_t kSpecularPath = 1u << 31;
 
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
@@ -121,20 +145,23 @@ __global__ __launch_bounds__(256) void k_generate(RenderConsts rc, WaveQueues q)
             V3 target = ld3(cam.llc) + ld3(cam.hor) * fs + ld3(cam.ver) * ft;
             o = ld3(cam.origin);
             d = normalize(target - o);
-            q.L[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
             q.pix[p0] = make_uint2(px | (py << 16), s);
         }
-        uint32_t slot = waveAppend(ctr(q, 0, 0), valid);
-        if (valid) {
-            q.ro[0][slot] = f4(o, __uint_as_float(p0));
-            q.rd[0][slot] = f4(d, __uint_as_float(5u));   // dims 0..4 consumed by the camera sample
-            q.beta[0][slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+        // bounce-0 queue slot = path id (no compaction: padding slots are rare and marked dead)
+        if (p0 < total) {
+            q.L[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            q.ro[0][p0] = f4(o, __uint_as_float(p0));
+            q.rd[0][p0] = f4(d, __uint_as_float(valid ? 5u : kDeadPath));   // dims 0..4: camera sample
+            q.beta[0][p0] = make_float4(1.f, 1.f, 1.f, 0.f);
         }
     }
+    if (blockIdx.x == 0u && threadIdx.x == 0u) *ctr(q, 0, 0) = total;
 }
 
 // --------------------------------------------------------------------------- extend
-template <uint32_t BS, uint32_t K>
+// CURVES = false compiles the curve traversal out (triangle-only scenes keep fewer registers and
+// more waves in flight); the host picks the instance from DScene::hasCurves.
+template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
     __shared__ uint32_t lref[K * BS];
     __shared__ float lt[K * BS];
@@ -146,10 +173,14 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
         uint32_t i = base + threadIdx.x;
         if (i >= count) break;
         float4 a = q.ro[cur][i], b = q.rd[cur][i];
+        if (__float_as_uint(b.w) & kDeadPath) {
+            q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(~0u));
+            continue;
+        }
         TRay r = makeTRay(v3(a), v3(b), 1e-4f, __int_as_float(0x7f800000));
         float t = 0.f, u = 0.f, v = 0.f;
         uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-        if (sc.hasCurves) {
+        if (CURVES && sc.hasCurves) {
             float best = h != ~0u ? t : r.tMax;
             CurveHit ch;
             if (traverseCurves<false, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) {
@@ -165,14 +196,17 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- shade
+template <bool CURVES>
 __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
                                                uint32_t cur) {
+    __shared__ uint32_t lds[3 * (4 + 1)];
+    uint32_t* const ctrs[3] = {ctr(q, bounce + 1u, 0), ctr(q, bounce, 1), ctr(q, bounce, 2)};
     const uint32_t count = *ctr(q, bounce, 0);
     const uint32_t nxt = cur ^ 1u;
     const float nL = (float)sc.numLights;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
         uint32_t i = base + threadIdx.x;
-        bool active = i < count;
+        bool active = i < count && !(__float_as_uint(q.rd[cur][i].w) & kDeadPath);
         bool cont = false, wantShadow = false, wantMis = false;
         uint32_t p0 = 0, dim = 0;
         V3 beta{0.f, 0.f, 0.f}, rayO{0.f, 0.f, 0.f}, rayD{0.f, 0.f, 0.f};
@@ -184,8 +218,8 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
             float4 ra = q.ro[cur][i], rb = q.rd[cur][i], bb = q.beta[cur][i], hh = q.hit[i];
             p0 = __float_as_uint(ra.w);
             uint32_t meta = __float_as_uint(rb.w);
-            dim = meta & 0x7fffffffu;
-            bool specular = (meta >> 31) != 0u;
+            dim = meta & ~(kDeadPath | kSpecularPath);
+            bool specular = (meta & kSpecularPath) != 0u;
             beta = v3(bb);
             V3 dir = v3(rb);
             uint32_t px, py, sidx;
@@ -201,7 +235,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                 V3 p, n, ng;
                 V2 uv;
                 uint32_t meshID, commit = ~0u;
-                bool isCurve = (tri & kCurveHit) != 0u;
+                bool isCurve = CURVES && (tri & kCurveHit) != 0u;
                 Frame fr;
                 if (!isCurve) {
                     // fillSurfaceInteraction (BVHBackend.cpp:285-317)
@@ -329,19 +363,20 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
                 q.L[p0] = make_float4(l.x + Ladd.x, l.y + Ladd.y, l.z + Ladd.z, 0.f);
             }
         }
-        uint32_t sc0 = waveAppend(ctr(q, bounce + 1u, 0), cont);
+        const bool want[3] = {cont, wantShadow, wantMis};
+        uint32_t slots[3];
+        blockAppend<3, 4>(ctrs, want, slots, lds);
+        const uint32_t sc0 = slots[0], sc1 = slots[1], sc2 = slots[2];
         if (cont) {
             q.ro[nxt][sc0] = f4(rayO, __uint_as_float(p0));
-            q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? 0x80000000u : 0u)));
+            q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? kSpecularPath : 0u)));
             q.beta[nxt][sc0] = f4(beta, 0.f);
         }
-        uint32_t sc1 = waveAppend(ctr(q, bounce, 1), wantShadow);
         if (wantShadow) {
             q.so[sc1] = f4(shO, __uint_as_float(p0));
             q.sd[sc1] = f4(shD, shTmax);
             q.sc[sc1] = f4(shC, 0.f);
         }
-        uint32_t sc2 = waveAppend(ctr(q, bounce, 2), wantMis);
         if (wantMis) {
             q.mo[sc2] = f4(miO, __uint_as_float(p0));
             q.md[sc2] = f4(miD, 1e10f);
@@ -351,7 +386,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQ
 }
 
 // --------------------------------------------------------------------------- shadow (any hit)
-template <uint32_t BS, uint32_t K>
+template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t bounce) {
     __shared__ uint32_t lref[K * BS];
     __shared__ float lt[K * BS];
@@ -366,7 +401,7 @@ __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t
         TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
         float t, u, v;
         uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-        if (h == ~0u && sc.hasCurves) {
+        if (CURVES && h == ~0u && sc.hasCurves) {
             float best = r.tMax;
             CurveHit ch;
             if (traverseCurves<true, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) h = kCurveHit;
@@ -380,7 +415,7 @@ __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- MIS (BSDF-sampled NEE)
-template <uint32_t BS, uint32_t K>
+template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bounce) {
     __shared__ uint32_t lref[K * BS];
     __shared__ float lt[K * BS];
@@ -399,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
         if (sc.hasEmitters) {
             uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
             uint32_t mesh = h != ~0u ? sc.tris[h].meshID : ~0u;
-            if (sc.hasCurves) {
+            if (CURVES && sc.hasCurves) {
                 float best = h != ~0u ? t : r.tMax;
                 CurveHit ch;
                 if (traverseCurves<false, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) {
@@ -415,7 +450,7 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
         } else {
             // no surface emits: only an escaping ray carries radiance, so any hit decides
             uint32_t h = traverse<true, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-            if (h == ~0u && sc.hasCurves) {
+            if (CURVES && h == ~0u && sc.hasCurves) {
                 float best = r.tMax;
                 CurveHit ch;
                 if (traverseCurves<true, false>(sc, r, S, best, ch, tc, ctr(q, bounce, 3))) h = kCurveHit;
@@ -506,7 +541,7 @@ __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays
 // --------------------------------------------------------------------------- traffic counting
 // Counter-enabled copies of extend / shadow / mis: per-ray node and triangle visits summed with
 // one atomic per wavefront.  Used by a separate, untimed pass.
-template <uint32_t BS, uint32_t K, int KIND>
+template <uint32_t BS, uint32_t K, int KIND, bool CURVES>
 __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur,
                                               unsigned long long* sums) {
     __shared__ uint32_t lref[K * BS];
@@ -519,7 +554,11 @@ __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t 
         TraceCounters tc{0, 0};
         if (i < count) {
             float4 a, b;
-            if (KIND == 0) { a = q.ro[cur][i]; b = q.rd[cur][i]; b.w = __int_as_float(0x7f800000); }
+            if (KIND == 0) {
+                a = q.ro[cur][i];
+                b = q.rd[cur][i];
+                b.w = (__float_as_uint(b.w) & kDeadPath) ? -1.f : __int_as_float(0x7f800000);
+            }
             else if (KIND == 1) { a = q.so[i]; b = q.sd[i]; }
             else { a = q.mo[i]; b = q.md[i]; }
             TRay r = makeTRay(v3(a), v3(b), 1e-4f, b.w);
@@ -528,7 +567,7 @@ __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t 
             uint32_t h;
             if (any) h = traverse<true, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
             else h = traverse<false, true>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-            if (sc.hasCurves && !(any && h != ~0u)) {
+            if (CURVES && sc.hasCurves && !(any && h != ~0u)) {
                 float best = h != ~0u ? t : r.tMax;
                 CurveHit ch;
                 if (any) traverseCurves<true, true>(sc, r, S, best, ch, tc, ctr(q, bounce, 3));
@@ -568,7 +607,7 @@ void setHaltonTables(const HaltonTables& t, hipStream_t s) {
 }
 
 constexpr uint32_t kTraceBS = 128;
-constexpr uint32_t kStackK = 16;
+constexpr uint32_t kStackK = 8;
 
 uint32_t lds_stack_entries() { return kStackK; }
 
@@ -597,17 +636,27 @@ void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks
 }
 void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
                   hipStream_t s) {
-    hipLaunchKernelGGL((k_extend<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    if (sc.hasCurves)
+        hipLaunchKernelGGL((k_extend<kTraceBS, kStackK, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    else
+        hipLaunchKernelGGL((k_extend<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
 }
 void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, uint32_t bounce, uint32_t cur,
                  uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    if (sc.hasCurves) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    if (sc.hasCurves)
+        hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    else
+        hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_mis<kTraceBS, kStackK>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    if (sc.hasCurves)
+        hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    else
+        hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
 void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, q);
@@ -622,14 +671,21 @@ void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit*
         hipLaunchKernelGGL((k_trace_batch<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, rays, n,
                            hits, occ, ovfRef, ovfT, cap, overflow, counts);
 }
+template <int KIND>
+static void countLaunch(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
+                        unsigned long long* sums, hipStream_t s) {
+    if (sc.hasCurves)
+        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, KIND, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce,
+                           cur, sums);
+    else
+        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, KIND, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce,
+                           cur, sums);
+}
 void launchCount(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, int kind, uint32_t blocks,
                  unsigned long long* sums, hipStream_t s) {
-    if (kind == 0)
-        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 0>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
-    else if (kind == 1)
-        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 1>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
-    else
-        hipLaunchKernelGGL((k_count<kTraceBS, kStackK, 2>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur, sums);
+    if (kind == 0) countLaunch<0>(sc, q, bounce, cur, blocks, sums, s);
+    else if (kind == 1) countLaunch<1>(sc, q, bounce, cur, blocks, sums, s);
+    else countLaunch<2>(sc, q, bounce, cur, blocks, sums, s);
 }
 
 }  // namespace ahip

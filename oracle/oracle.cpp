@@ -918,15 +918,17 @@ struct Scene {
         V3 gR, neeA, neeR;
     };
     // d(sum_c delta_c L_c) of one vertex, scattered into g
+    // A term whose float derivative overflowed (inf, or inf * 0) is dropped, as the forward drops
+    // a non-finite radiance sample.
     void scatter(const Grad& g, const Vtx& v) const {
         V3 ka = g.delta * (v.acc * v.sRho + v.neeA);
         V3 kr = g.delta * (v.acc * v.gR + v.neeR);
-        if (g.dAlb && v.e.tri != ~0u) {
+        if (g.dAlb && v.e.tri != ~0u && isFinite(ka)) {
             (*g.dAlb)[3 * v.e.tri] += ka.x;
             (*g.dAlb)[3 * v.e.tri + 1] += ka.y;
             (*g.dAlb)[3 * v.e.tri + 2] += ka.z;
         }
-        if (g.dTex && v.e.texR) {
+        if (g.dTex && v.e.texR && isFinite(kr)) {
             double dr = (double)kr.x + (double)kr.y + (double)kr.z;
             for (int q = 0; q < 4; ++q) (*g.dTex)[v.e.taps[q]] += dr * v.e.tw[q];
         }
