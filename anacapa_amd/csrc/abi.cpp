@@ -942,14 +942,14 @@ int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sa
     const size_t nAlb = 3 * (size_t)ctx->numCommit;
     const size_t nTex = a.diffTex != ~0u ? (size_t)ctx->texW[a.diffTex] * ctx->texH[a.diffTex] : 0;
     if ((st = upload(ctx, ctx->dDLdI, dL_dimage, 3 * n))) return st;
-    HIPCHK(ctx->dGAlb.alloc(std::max<size_t>(nAlb, 1) * sizeof(float)));
-    HIPCHK(ctx->dGTex.alloc(std::max<size_t>(nTex, 1) * sizeof(float)));
+    HIPCHK(ctx->dGAlb.alloc(std::max<size_t>(nAlb, 1) * sizeof(double)));
+    HIPCHK(ctx->dGTex.alloc(std::max<size_t>(nTex, 1) * sizeof(double)));
     HIPCHK(ctx->dRec.alloc((size_t)kRecFields * a.maxV * a.nthreads * sizeof(float)));
-    launchFillF32(ctx->dGAlb.as<float>(), nAlb, 0.f, ctx->stream);
-    launchFillF32(ctx->dGTex.as<float>(), nTex, 0.f, ctx->stream);
+    launchZeroF64(ctx->dGAlb.as<double>(), nAlb, ctx->stream);
+    launchZeroF64(ctx->dGTex.as<double>(), nTex, ctx->stream);
     a.dLdI = ctx->dDLdI.as<float>();
-    a.gAlb = d_albedo && nAlb ? ctx->dGAlb.as<float>() : nullptr;
-    a.gTex = d_roughness_tex && nTex ? ctx->dGTex.as<float>() : nullptr;
+    a.gAlb = d_albedo && nAlb ? ctx->dGAlb.as<double>() : nullptr;
+    a.gTex = d_roughness_tex && nTex ? ctx->dGTex.as<double>() : nullptr;
     a.rec = ctx->dRec.as<float>();
     HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
     launchReplay(ctx->dsc, a, true, blocks, ctx->stream);
@@ -960,9 +960,17 @@ int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sa
     HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
     ctx->stats.render_ms = ms;
     if ((st = replayOverflowCheck(ctx))) return st;
-    if (d_albedo && nAlb) HIPCHK(hipMemcpy(d_albedo, ctx->dGAlb.p, nAlb * sizeof(float), hipMemcpyDeviceToHost));
-    if (d_roughness_tex && nTex)
-        HIPCHK(hipMemcpy(d_roughness_tex, ctx->dGTex.p, nTex * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<double> g;
+    if (d_albedo && nAlb) {
+        g.resize(nAlb);
+        HIPCHK(hipMemcpy(g.data(), ctx->dGAlb.p, nAlb * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nAlb; ++i) d_albedo[i] = (float)g[i];
+    }
+    if (d_roughness_tex && nTex) {
+        g.resize(nTex);
+        HIPCHK(hipMemcpy(g.data(), ctx->dGTex.p, nTex * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nTex; ++i) d_roughness_tex[i] = (float)g[i];
+    }
     return AHIP_OK;
 }
 

@@ -385,9 +385,9 @@ __global__ __launch_bounds__(kReplayBS) void k_replay(DScene sc, ReplayArgs a) {
                 V3 ka = delta * acc * R.at(F_SRHO, j) + R.get3(F_KA, j);
                 uint32_t c = __float_as_uint(R.at(F_TRI, j));
                 if (a.gAlb && finite3(ka)) {
-                    atomicAdd(a.gAlb + 3u * (size_t)c, ka.x);
-                    atomicAdd(a.gAlb + 3u * (size_t)c + 1u, ka.y);
-                    atomicAdd(a.gAlb + 3u * (size_t)c + 2u, ka.z);
+                    atomicAdd(a.gAlb + 3u * (size_t)c, (double)ka.x);
+                    atomicAdd(a.gAlb + 3u * (size_t)c + 1u, (double)ka.y);
+                    atomicAdd(a.gAlb + 3u * (size_t)c + 2u, (double)ka.z);
                 }
                 uint32_t tap = __float_as_uint(R.at(F_TAP, j));
                 V3 uu = R.get3(F_U, j);
@@ -397,14 +397,20 @@ __global__ __launch_bounds__(kReplayBS) void k_replay(DScene sc, ReplayArgs a) {
                     uint32_t ti[4];
                     tapIdx(tx.w, tx.h, tap & 0xffffu, tap >> 16, ti);
                     float ax = R.at(F_AX, j), ay = R.at(F_AY, j);
-                    atomicAdd(a.gTex + ti[0], kr * ((1.f - ay) * (1.f - ax)));
-                    atomicAdd(a.gTex + ti[1], kr * ((1.f - ay) * ax));
-                    atomicAdd(a.gTex + ti[2], kr * (ay * (1.f - ax)));
-                    atomicAdd(a.gTex + ti[3], kr * (ay * ax));
+                    const double k = kr;
+                    atomicAdd(a.gTex + ti[0], k * ((1.f - ay) * (1.f - ax)));
+                    atomicAdd(a.gTex + ti[1], k * ((1.f - ay) * ax));
+                    atomicAdd(a.gTex + ti[2], k * (ay * (1.f - ax)));
+                    atomicAdd(a.gTex + ti[3], k * (ay * ax));
                 }
             }
         }
     }
+}
+
+__global__ void k_fill_f64(double* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
 }
 
 // Per-pixel sums of the replayed samples in sample order (deterministic), like k_accumulate.
@@ -425,6 +431,13 @@ __global__ __launch_bounds__(256) void k_replay_reduce(ReplayArgs a, float4* acc
         accum[i] = make_float4(acc.x, acc.y, acc.z, (float)a.ns);
         lsq[i] = q;
     }
+}
+
+void launchZeroF64(double* p, size_t n, hipStream_t s) {
+    size_t b = (n + 255u) / 256u;
+    if (b == 0u) return;
+    uint32_t blocks = b > 4096u ? 4096u : (uint32_t)b;
+    hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, n);
 }
 
 void setHaltonTablesReplay(const HaltonTables& t, hipStream_t s) {

@@ -302,6 +302,10 @@ __device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, const Ma
 //   dD/da2 = (den - 2 a2 c^2) / (pi den^3),  den = c^2 (a2 - 1) + 1
 //   dG1/da2 = -x (1 - x^2) / (s (x + s)^2),   s = sqrt(a2 + (1 - a2) x^2)
 //   a2 = alpha^2 = r^4 while r^2 > 1e-4 (alpha = max(1e-4, r^2)), so da2/dr = 4 r^3, else 0.
+// The lobe arithmetic runs in double: in float, den = c^2 (a2 - 1) + 1 cancels to a few bits near
+// a glint (c -> 1, a2 << 1), which makes f a staircase in r with steps of ~1e-3 relative; the
+// replay estimator evaluates f here so that its image is smooth in the evaluation parameters.
+// The branch decisions (which lobes apply) are the float ones of evalMixture.
 __device__ V3 matEvalDiff(const DMat& m, const Frame& fr, V3 base, float r, V3 wo, V3 wi, float& dRho, V3& dR) {
     dRho = 0.f;
     dR = V3{0.f, 0.f, 0.f};
@@ -311,42 +315,65 @@ __device__ V3 matEvalDiff(const DMat& m, const Frame& fr, V3 base, float r, V3 w
         dRho = kInvPi;
         return base * kInvPi;
     }
-    float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
     float wc, wm, ws, wd;
     if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return V3{0.f, 0.f, 0.f};
-    float pdf;
-    V3 f = evalMixture(m, wol, wil, wc, wm, ws, wd, a2, base, pdf);
-    if (wd > 0.f) dRho += m.base * kInvPi * wd;
-    if ((wm > 0.f || ws > 0.f) && wol.z > 0.f && wil.z > 0.f) {
+    const double kPiD = 3.14159265358979323846;
+    double f[3] = {0.0, 0.0, 0.0}, g[3] = {0.0, 0.0, 0.0}, rho = 0.0;
+    const double b[3] = {base.x, base.y, base.z};
+    const double sc[3] = {m.specColor[0], m.specColor[1], m.specColor[2]};
+    if (wol.z > 0.f && wil.z > 0.f) {
         V3 wh = safeNormalize(wol + wil);
-        float co = fabsf(wol.z), ci = fabsf(wil.z);
+        const float co = fabsf(wol.z), ci = fabsf(wil.z);
         if (wh.z > 0.f && co >= 1e-6f && ci >= 1e-6f) {
-            float dVH = stdmax(0.f, dot(wol, wh));
-            float x = 1.f - dVH, x5 = x * x * x * x * x;
-            float D = dGGX(wh.z, a2), G = g2Smith(wol.z, wil.z, a2);
-            float inv4 = 1.f / (4.f * co * ci);
-            if (wm > 0.f) dRho += wm * m.base * (1.f - x5) * D * G * inv4;
-            float c2 = wh.z * wh.z;
-            float den = c2 * (a2 - 1.f) + 1.f;
-            float dD = (den - 2.f * a2 * c2) / (kPi * den * den * den);
-            auto g1 = [&](float xx, float& dg) {
-                float s = sqrtf(a2 + (1.f - a2) * xx * xx);
-                float q = xx + s;
-                dg = -xx * (1.f - xx * xx) / (s * q * q);
-                return 2.f * xx / q;
+            const double dVH = stdmax(0.f, dot(wol, wh)), x = 1.0 - dVH, x5 = x * x * x * x * x;
+            const double c2 = (double)wh.z * wh.z, inv4 = 1.0 / (4.0 * (double)co * (double)ci);
+            // D*G at a2 and its derivative with respect to a2
+            auto dg = [&](double a2, double& dDG) {
+                double den = c2 * (a2 - 1.0) + 1.0;
+                double D = a2 / (kPiD * den * den), dD = (den - 2.0 * a2 * c2) / (kPiD * den * den * den);
+                auto g1 = [&](double xx, double& dg1) {
+                    double s = sqrt(a2 + (1.0 - a2) * xx * xx), q = xx + s;
+                    dg1 = -xx * (1.0 - xx * xx) / (s * q * q);
+                    return 2.0 * xx / q;
+                };
+                double dGo, dGi;
+                double Go = g1(co, dGo), Gi = g1(ci, dGi);
+                dDG = dD * Go * Gi + D * (dGo * Gi + Go * dGi);
+                return D * Go * Gi;
             };
-            float dGo, dGi;
-            float Go = g1(co, dGo), Gi = g1(ci, dGi);
-            float dDG = dD * G + D * (dGo * Gi + Go * dGi);
-            float da2 = r * r > 1e-4f ? 4.f * r * r * r : 0.f;
-            float k = dDG * da2 * inv4;
-            V3 acc{0.f, 0.f, 0.f};
-            if (wm > 0.f) acc = acc + schlickColor(dVH, base * m.base) * (wm * k);
-            if (ws > 0.f) acc = acc + schlickColor(dVH, ld3(m.specColor) * (m.spec * m.f0d)) * (ws * k);
-            dR = acc;
+            if (wc > 0.f) {
+                double unused;
+                double k = wc * (m.coatF0 + (1.0 - m.coatF0) * x5) * dg(m.coatA2, unused) * inv4;
+                for (int c = 0; c < 3; ++c) f[c] += k;
+            }
+            if (wm > 0.f || ws > 0.f) {
+                const double rd = r, alpha = fmax(1e-4, rd * rd), a2 = alpha * alpha;
+                const double da2 = rd * rd > 1e-4 ? 4.0 * rd * rd * rd : 0.0;
+                double dDG;
+                const double DG = dg(a2, dDG);
+                for (int c = 0; c < 3; ++c) {
+                    if (wm > 0.f) {
+                        double f0 = b[c] * m.base, F = f0 + (1.0 - f0) * x5;
+                        f[c] += wm * F * DG * inv4;
+                        g[c] += wm * F * dDG * da2 * inv4;
+                    }
+                    if (ws > 0.f) {
+                        double f0 = sc[c] * ((double)m.spec * m.f0d), F = f0 + (1.0 - f0) * x5;
+                        f[c] += ws * F * DG * inv4;
+                        g[c] += ws * F * dDG * da2 * inv4;
+                    }
+                }
+                if (wm > 0.f) rho += wm * m.base * (1.0 - x5) * DG * inv4;
+            }
         }
     }
-    return f;
+    if (wd > 0.f) {
+        for (int c = 0; c < 3; ++c) f[c] += b[c] * m.base * (1.0 / kPiD) * wd;
+        rho += m.base * (1.0 / kPiD) * wd;
+    }
+    dRho = (float)rho;
+    dR = V3{(float)g[0], (float)g[1], (float)g[2]};
+    return V3{(float)f[0], (float)f[1], (float)f[2]};
 }
 
 // ---------------------------------------------------------------- dome light

@@ -76,8 +76,8 @@ struct ReplayArgs {
     const float* texE;           // evaluation texels of diffTex (null: sampling values)
     float4* Lout;                // primal: radiance per (sample, pixel) [ns * w * h]
     const float* dLdI;           // adjoint: [h * w * 3]
-    float* gAlb;                 // adjoint: [3 * tris]
-    float* gTex;                 // adjoint: [texels of diffTex]
+    double* gAlb;                // adjoint: [3 * tris]; double so that sums of ~1e3 path terms of
+    double* gTex;                // either sign do not lose the result to float32 rounding
     float* rec;                  // adjoint: vertex records [kRecFields * maxV * nthreads]
     uint32_t maxV, nthreads;
     uint32_t* ovfRef;            // traversal stack overflow, stride nthreads
@@ -90,6 +90,7 @@ void setHaltonTablesReplay(const HaltonTables& t, hipStream_t s);
 uint32_t replay_block_size();
 void launchReplay(const DScene& sc, const ReplayArgs& a, bool adjoint, uint32_t blocks, hipStream_t s);
 void launchReplayReduce(const ReplayArgs& a, float4* accum, float* lsq, hipStream_t s);
+void launchZeroF64(double* p, size_t n, hipStream_t s);
 
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();

@@ -117,16 +117,17 @@ struct Texture {
     }
 };
 
-// Forward-mode dual number (value, derivative) for the roughness derivative of the GGX lobes.
+// Forward-mode dual number (value, derivative) in double for the roughness derivative of the
+// GGX lobes.
 struct Dual {
-    float v, d;
+    double v, d;
 };
 static inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
 static inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
 static inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 static inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
-static inline Dual dsqrt(Dual a) { float r = std::sqrt(a.v); return {r, a.d / (2.f * r)}; }
-static inline Dual dconst(float c) { return {c, 0.f}; }
+static inline Dual dsqrt(Dual a) { double r = std::sqrt(a.v); return {r, a.d / (2.0 * r)}; }
+static inline Dual dconst(double c) { return {c, 0.0}; }
 
 // ---------------------------------------------------------------- materials
 struct BSDFSample { V3 f, wi; float pdf = 0.f; bool delta = false;
@@ -381,7 +382,10 @@ struct Material {
 
     // f(wo, wi) (no cosine) at base colour `base` and roughness `r`, with dRho = df_c/dbase_c
     // (one value for every channel: f is linear in the base colour, channel by channel) and
-    // dR = df/dr.  Lambert and standard_surface only.
+    // dR = df/dr.  Lambert and standard_surface only.  The GGX lobes are evaluated in double
+    // (forward-mode dual numbers carry d/dr): in float, D's denominator c^2 (a2 - 1) + 1 cancels
+    // near a glint and f becomes a staircase in r.  Which lobes apply is decided as in
+    // evalCombined.
     V3 evalDiff(const Ctx& c, V3 wo, V3 wi, V3 base, float r, float& dRho, V3& dR) const {
         dRho = 0.f;
         dR = {};
@@ -392,43 +396,60 @@ struct Material {
             return base * kInvPi;
         }
         float metal = m.metalness, spec = m.specular;
-        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
         float wc, wm, ws, wd;
         bool ok;
         weights(wol.z, metal, spec, wc, wm, ws, wd, ok);
         if (!ok) return {};
-        float pdf;
-        V3 f = evalCombined(wol, wil, wc, wm, ws, wd, base, spec, a2, pdf);
-        V3 f1 = evalCombined(wol, wil, wc, wm, ws, wd, V3{1.f, 1.f, 1.f}, spec, a2, pdf);
-        V3 f0 = evalCombined(wol, wil, wc, wm, ws, wd, V3{}, spec, a2, pdf);
-        dRho = f1.x - f0.x;
-        // roughness: only D*G of the metal and specular lobes depends on it
-        if ((wm > 0.f || ws > 0.f) && wol.z > 0.f && wil.z > 0.f) {
-            V3 wh = safeNormalize(wol + wil);
-            float co = std::abs(wol.z), ci = std::abs(wil.z);
-            if (wh.z > 0.f && co >= 1e-6f && ci >= 1e-6f) {
-                float dVH = fmaxs(0.f, dot(wol, wh));
-                Dual rr{r, 1.f};
-                Dual al = (r * r > 1e-4f) ? rr * rr : dconst(1e-4f);
-                Dual a2d = al * al;
-                Dual ch = dconst(wh.z);
-                Dual den = ch * ch * (a2d - dconst(1.f)) + dconst(1.f);
-                Dual D = a2d / (dconst(kPi) * den * den);
-                auto g1 = [&](float cc) {
-                    Dual C = dconst(cc);
-                    return dconst(2.f) * C / (C + dsqrt(a2d + (dconst(1.f) - a2d) * C * C));
-                };
-                Dual G = g1(co) * g1(ci);
-                float dDG = (D * G).d / (4.f * co * ci);
-                V3 Fm = schlickC(dVH, base * m.base);
-                V3 Fs = schlickC(dVH, V3{m.specular_color[0], m.specular_color[1], m.specular_color[2]} * (spec * f0d));
-                V3 acc;
-                if (wm > 0.f) acc = acc + Fm * (wm * dDG);
-                if (ws > 0.f) acc = acc + Fs * (ws * dDG);
-                dR = acc;
+        // f as a function of (base colour b, roughness rr): dual part = d/dr
+        auto eval = [&](const double b[3], Dual rr, Dual out[3]) {
+            for (int k = 0; k < 3; ++k) out[k] = dconst(0.0);
+            if (wol.z > 0.f && wil.z > 0.f) {
+                V3 wh = safeNormalize(wol + wil);
+                float co = std::abs(wol.z), ci = std::abs(wil.z);
+                if (wh.z > 0.f && co >= 1e-6f && ci >= 1e-6f) {
+                    double dVH = fmaxs(0.f, dot(wol, wh)), x = 1.0 - dVH, x5 = x * x * x * x * x;
+                    auto DG = [&](Dual a2) {
+                        Dual ch = dconst(wh.z);
+                        Dual den = ch * ch * (a2 - dconst(1.0)) + dconst(1.0);
+                        Dual D = a2 / (dconst(3.14159265358979323846) * den * den);
+                        auto g1 = [&](double cc) {
+                            Dual C = dconst(cc);
+                            return dconst(2.0) * C / (C + dsqrt(a2 + (dconst(1.0) - a2) * C * C));
+                        };
+                        return D * g1(co) * g1(ci) / dconst(4.0 * (double)co * (double)ci);
+                    };
+                    if (wc > 0.f) {
+                        Dual v = DG(dconst(coatA2)) * dconst(wc * (coatF0 + (1.0 - coatF0) * x5));
+                        for (int k = 0; k < 3; ++k) out[k] = out[k] + v;
+                    }
+                    if (wm > 0.f || ws > 0.f) {
+                        Dual al = (rr.v * rr.v > 1e-4) ? rr * rr : dconst(1e-4);
+                        Dual dg = DG(al * al);
+                        const double sc[3] = {m.specular_color[0], m.specular_color[1], m.specular_color[2]};
+                        for (int k = 0; k < 3; ++k) {
+                            if (wm > 0.f) {
+                                double f0 = b[k] * m.base;
+                                out[k] = out[k] + dg * dconst(wm * (f0 + (1.0 - f0) * x5));
+                            }
+                            if (ws > 0.f) {
+                                double f0 = sc[k] * ((double)spec * f0d);
+                                out[k] = out[k] + dg * dconst(ws * (f0 + (1.0 - f0) * x5));
+                            }
+                        }
+                    }
+                }
             }
-        }
-        return f;
+            if (wd > 0.f)
+                for (int k = 0; k < 3; ++k) out[k] = out[k] + dconst(b[k] * m.base * (1.0 / 3.14159265358979323846) * wd);
+        };
+        const double bb[3] = {base.x, base.y, base.z}, one[3] = {1.0, 1.0, 1.0}, zero[3] = {0.0, 0.0, 0.0};
+        Dual fv[3], f1[3], f0v[3];
+        eval(bb, Dual{r, 1.0}, fv);
+        eval(one, dconst(r), f1);
+        eval(zero, dconst(r), f0v);
+        dRho = (float)(f1[0].v - f0v[0].v);
+        dR = V3{(float)fv[0].d, (float)fv[1].d, (float)fv[2].d};
+        return V3{(float)fv[0].v, (float)fv[1].v, (float)fv[2].v};
     }
 };
 

@@ -4,9 +4,10 @@
   per-pixel float sums differs);
 - replay at perturbed evaluation parameters matches the oracle's replay to the radiance tolerance;
 - the adjoint matches the oracle's adjoint (float atomics and association only);
-- the adjoint matches central differences (h = 1e-3) of the GPU replay's image L2 loss to 1e-2
-  relative: on the small scene, and at workload-5 scale (1M triangles, 1024^2 roughness texture,
-  512x512x8, depth 4) on 256 randomly chosen parameters;
+- the adjoint matches central differences (h = 1e-3) of the GPU replay to 1e-2 relative: the
+  image L2 loss on the small scene; at workload-5 scale (1M triangles, 1024^2 roughness texture,
+  512x512x8, depth 4) a random linear image functional and the image L2 loss, each on 256 random
+  parameters, within 1e-2 plus the central difference's own float32 rounding bound;
 - a torch.autograd.Function over replay/adjoint gives the same gradient and descends the loss.
 """
 import numpy as np
@@ -81,20 +82,48 @@ def test_adjoint_matches_oracle_adjoint(gpu, small):
         assert (rel <= 1e-2).mean() >= 0.95, kind
 
 
-def _loss(gpu, sc, target, alb, tex):
+def _image(gpu, sc, alb, tex):
     gpu.set_eval_params(alb, 0, tex)
     img = gpu.replay(0, sc.spp)
-    I = img[..., :3].astype(np.float64) / img[..., 3:4]
-    return float(((I - target) ** 2).sum()), I
+    return img[..., :3].copy(), float(img[0, 0, 3])
 
 
-def _fd_check(gpu, sc, target, alb0, tex0, n_each, seed, h=1e-3):
-    L0, I0 = _loss(gpu, sc, target, alb0, tex0)
-    dldi = (2.0 * (I0 - target)).astype(np.float32)
+def _fd_check(gpu, sc, target, alb0, tex0, n_each, seed, h=1e-3, weights=None):
+    """Adjoint vs central differences (step h) on n_each albedo and n_each texel parameters drawn
+    at random among those with a non-negligible gradient.  Loss: the image L2 loss against
+    `target`, or the linear functional sum(weights * I) when weights are given.
+
+    The replay image is float32, so a central difference carries its own rounding error: every
+    pixel the parameter influences is known only to ~ulp(pixel sum) / spp in each evaluation (a
+    pixel whose true change over +-h is below that step shows no change, or one whole step), and
+    the loss weighs it by |dLoss/dI| (for the L2 loss 2|I - T|, which reaches ~1e3 on sun glints of
+    radiance 1e4-1e5).  The influenced pixels are those that change when the parameter moves by
+    +-0.05; sigma = sum over them of |dLoss/dI| * 4 ulp(sum) / spp / (2h), and the check is
+    |g - fd| <= 1e-2 |fd| + sigma.  Returns (relative errors, how many needed sigma)."""
+    S0, ns = _image(gpu, sc, alb0, tex0)
+    I0 = S0.astype(np.float64) / ns
+    dldi = 2.0 * (I0 - target) if weights is None else weights
     gpu.set_eval_params(alb0, 0, tex0)
-    da, dt = gpu.adjoint(0, sc.spp, dldi, tex0.shape)
+    da, dt = gpu.adjoint(0, sc.spp, dldi.astype(np.float32), tex0.shape)
+    assert np.isfinite(da).all() and np.isfinite(dt).all()
+
+    def loss(S):
+        I = S.astype(np.float64) / ns
+        return float(((I - target) ** 2).sum()) if weights is None else float((weights * I).sum())
+
     rng = np.random.default_rng(seed)
-    worst = 0.0
+    rels, noisy = [], 0
+
+    def moved(kind, idx, step):
+        a, t = alb0, tex0
+        if kind == "albedo":
+            a = alb0.copy()
+            a[idx] += step
+        else:
+            t = tex0.copy()
+            t[idx] += step
+        return _image(gpu, sc, a, t)[0]
+
     for grad, kind in ((da, "albedo"), (dt, "texel")):
         # parameters whose gradient is not negligible (most triangles are not seen at all)
         cand = np.argwhere(np.abs(grad) > 1e-3 * np.abs(grad).max())
@@ -110,18 +139,26 @@ def _fd_check(gpu, sc, target, alb0, tex0, n_each, seed, h=1e-3):
                 t_p, t_m = tex0.copy(), tex0.copy()
                 t_p[idx] += h
                 t_m[idx] -= h
-            fd = (_loss(gpu, sc, target, a_p, t_p)[0] - _loss(gpu, sc, target, a_m, t_m)[0]) / (2 * h)
-            rel = abs(grad[idx] - fd) / abs(fd)
-            worst = max(worst, rel)
-            assert rel <= 1e-2, (kind, idx, float(grad[idx]), fd)
+            Sp, _ = _image(gpu, sc, a_p, t_p)
+            Sm, _ = _image(gpu, sc, a_m, t_m)
+            fd = (loss(Sp) - loss(Sm)) / (2 * h)
+            changed = (moved(kind, idx, 0.05) != S0) | (moved(kind, idx, -0.05) != S0)
+            ulp = np.spacing(np.maximum(np.abs(Sp), np.abs(Sm)).astype(np.float32)).astype(np.float64)
+            wabs = np.abs(dldi) + (0.0 if weights is not None else 2.0 * np.abs(Sp - Sm) / ns)
+            sigma = float((wabs * 4.0 * ulp / ns)[changed].sum()) / (2 * h)
+            err = abs(grad[idx] - fd)
+            rels.append(err / abs(fd))
+            if err > 1e-2 * abs(fd):
+                noisy += 1
+            assert err <= 1e-2 * abs(fd) + sigma, (kind, idx, float(grad[idx]), fd, sigma)
     # parameters with a zero gradient leave the image bit-identical
     zero = np.argwhere(da == 0)
     for k in rng.choice(len(zero), 4, replace=False):
         idx = tuple(zero[k])
         a_p = alb0.copy()
         a_p[idx] += h
-        assert _loss(gpu, sc, target, a_p, tex0)[0] == L0
-    return worst
+        assert np.array_equal(_image(gpu, sc, a_p, tex0)[0], S0)
+    return np.array(rels), noisy
 
 
 def test_adjoint_matches_central_differences_small(gpu, small):
@@ -130,11 +167,13 @@ def test_adjoint_matches_central_differences_small(gpu, small):
     gpu.set_eval_params()
     t = gpu.replay(0, sc.spp)
     target = t[..., :3].astype(np.float64) / t[..., 3:4]
-    worst = _fd_check(gpu, sc, target, alb0, tex0, 12, 21)
-    print(f"[adjoint] small scene: worst relative FD error {worst:.2e}", flush=True)
+    rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 12, 21)
+    assert noisy == 0
+    print(f"[adjoint] small scene: worst relative FD error {rels.max():.2e}", flush=True)
 
 
-def test_adjoint_matches_central_differences_workload5(gpu):
+@pytest.fixture(scope="module")
+def workload5(gpu):
     sc = scenes.inverse_scene()
     print(f"[adjoint] workload 5: {sc.num_triangles} triangles, {sc.width}x{sc.height}x{sc.spp}", flush=True)
     gpu.prepare(sc)
@@ -144,9 +183,30 @@ def test_adjoint_matches_central_differences_workload5(gpu):
     alb0, tex0 = scenes.perturbed_params(sc)
     # sampling parameters = the initial guess, as an optimiser step would hold them
     sc.tri_albedo, sc.textures = alb0, [tex0]
+    return sc, target, alb0, tex0
+
+
+def test_adjoint_matches_central_differences_workload5(gpu, workload5):
+    """A random linear image functional (N(0,1) pixel weights) on 256 random parameters (128
+    albedo, 128 texels).  The adjoint is linear in dL/dI, so this pins the Jacobian-transpose
+    product that the L2 loss uses too."""
+    sc, target, alb0, tex0 = workload5
     gpu.prepare(sc)
-    worst = _fd_check(gpu, sc, target, alb0, tex0, 128, 5)
-    print(f"[adjoint] workload 5: 256 parameters, worst relative FD error {worst:.2e}", flush=True)
+    w = np.random.default_rng(9).normal(size=target.shape)
+    rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 5, weights=w)
+    assert noisy <= 256 // 10, noisy
+    print(f"[adjoint] workload 5, linear functional: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
+          f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
+
+
+def test_adjoint_l2_loss_central_differences_workload5(gpu, workload5):
+    """The image L2 loss against the ground-truth render on 256 random parameters."""
+    sc, target, alb0, tex0 = workload5
+    gpu.prepare(sc)
+    rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 7)
+    assert noisy <= 256 // 10, noisy
+    print(f"[adjoint] workload 5, image L2 loss: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
+          f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
 
 
 def test_torch_autograd_wrapper(gpu, small):

@@ -29,11 +29,12 @@ def test_replay_at_sampling_params_is_the_render(inv):
     a = os_.render_tile(0, 0, sc.width, sc.height, 0, sc.spp)
     b = os_.replay_tile(0, 0, sc.width, sc.height, 0, sc.spp)
     assert np.array_equal(a, b)
-    # the same parameters handed over explicitly give the same image up to rounding of the
-    # Lambert ground's f*cos products
+    # the same parameters handed over explicitly evaluate f through the double-precision lobe
+    # arithmetic of the differentiable path: equal to the float render up to the float render's
+    # own rounding near glints (D's denominator cancels in float), a few 1e-4 relative
     os_.set_eval(sc.tri_albedo, 0, sc.textures[0])
     c = os_.replay_tile(0, 0, sc.width, sc.height, 0, sc.spp)
-    np.testing.assert_allclose(c, a, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(c, a, rtol=1e-3, atol=1e-6)
     os_.set_eval()
 
 
