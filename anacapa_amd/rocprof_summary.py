@@ -54,6 +54,11 @@ def main(argv=None):
     ap.add_argument("--write")
     ap.add_argument("--probe-mb", type=float, default=0.0)
     ap.add_argument("--family", default="k_extend,k_shadow,k_mis")
+    ap.add_argument("--sq", help="counter CSV of a run with SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU "
+                                 "SQ_WAVE_CYCLES SQ_INSTS_VALU SQ_WAVES GRBM_GUI_ACTIVE")
+    ap.add_argument("--cus", type=int, default=256)
+    ap.add_argument("--xcds", type=int, default=8,
+                    help="GRBM_GUI_ACTIVE is reported summed over the XCDs; the derived metrics take its max")
     ap.add_argument("--json")
     a = ap.parse_args(argv)
     fam = a.family.split(",")
@@ -93,6 +98,29 @@ def main(argv=None):
         traffic = (fam_kib["fetch"] + fam_kib["write"]) * 1024.0
         res["family_traffic_bytes_per_dispatch"] = round(traffic)
         print(f"\ntraversal family HBM traffic per dispatch (calibrated FETCH + WRITE): {traffic / 1e6:.2f} MB")
+    if a.sq:
+        # rocprofv3's derived-metric definitions for this agent (rocprofv3 -L), with
+        # GRBM_GUI_ACTIVE = max over XCDs (the CSV holds the sum; all XCDs run the whole kernel):
+        #   VALUBusy = 100 * SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE
+        #   VALUUtilization = 100 * SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU * 64)
+        #   OccupancyPercent = 400 * SQ_WAVE_CYCLES / GRBM_GUI_ACTIVE / CU_NUM / 32
+        per = collections.defaultdict(lambda: collections.defaultdict(list))
+        for r in csv.DictReader(open(a.sq)):
+            per[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        print("\n| kernel | dispatches | VALU busy % | VALU utilization % | occupancy % (waves/SIMD of 8) | VALU insts/wave |")
+        print("|---|---:|---:|---:|---:|---:|")
+        res["sq"] = {}
+        for n, c in sorted(per.items()):
+            if "GRBM_GUI_ACTIVE" not in c or "SQ_ACTIVE_INST_VALU" not in c:
+                continue
+            g = sum(c["GRBM_GUI_ACTIVE"]) / a.xcds
+            act = sum(c["SQ_ACTIVE_INST_VALU"])
+            busy = 100.0 * act / a.cus / max(g, 1.0)
+            util = 100.0 * sum(c.get("SQ_THREAD_CYCLES_VALU", [0.0])) / max(act * 64.0, 1.0)
+            occ = 400.0 * sum(c.get("SQ_WAVE_CYCLES", [0.0])) / max(g, 1.0) / a.cus / 32.0
+            ipw = sum(c.get("SQ_INSTS_VALU", [0.0])) / max(sum(c.get("SQ_WAVES", [1.0])), 1.0)
+            res["sq"][n] = {"valu_busy": round(busy, 1), "valu_util": round(util, 1), "occupancy": round(occ, 1)}
+            print(f"| {n} | {len(c['GRBM_GUI_ACTIVE'])} | {busy:.1f} | {util:.1f} | {occ:.1f} ({occ * 0.08:.1f}) | {ipw:.0f} |")
     if a.json:
         with open(a.json, "w") as f:
             json.dump(res, f, indent=1)

@@ -130,8 +130,14 @@ def main():
         per_launch = frame_bytes * args.steps / launches
         avg_launch_s = trace_ms / 1e3 / launches
         achieved = per_launch / avg_launch_s / 1e9
+        # measured HBM bytes per traversal dispatch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload,
+        # calibrated by a copy kernel of known size (summary and commands under profiles/)
+        traffic, prof = None, os.path.join(ROOT, "profiles", "traversal_ggx1080p16_traffic.json")
+        if (W, H, S, args.depth) == (1920, 1080, 16, 8) and os.path.exists(prof):
+            with open(prof) as f:
+                traffic = json.load(f).get("family_traffic_bytes_per_dispatch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": "traversal (extend+shadow+mis launches)",
                     "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                     "nodes_per_frame": int(ct["nodes_visited"]), "tris_per_frame": int(ct["prims_tested"]),

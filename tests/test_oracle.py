@@ -346,3 +346,23 @@ def test_fixed_double_cosine_changes_only_bsdf_branch(oracle):
     # dropping the extra cosine can only increase the BSDF-branch NEE term (cos <= 1)
     assert (b[..., :3] >= a[..., :3] - 1e-5).all()
     assert b[..., :3].sum() > a[..., :3].sum()
+
+
+def test_flat_leaf_is_never_entered_like_the_reference(oracle, reference):
+    """BVHBackend::intersectAABB rejects t1 <= t0 (BVHBackend.cpp:249-262); a leaf holding only the
+    flat ground quad has zero thickness once its 1e-7 padding is lost beside the ray origin, so
+    the quad is invisible whenever it sits alone in a leaf (8x8 spheres and up) and visible when it
+    shares a leaf with sphere triangles (2x2).  The oracle reproduces the reference build here."""
+    for grid, visible in ((2, True), (8, False)):
+        sc = scenes.ggx_spheres(width=16, height=16, grid=grid, freq=3 if grid > 2 else 2, hdri_size=(32, 16))
+        sc.materials = [scenes.lambert((0.5, 0.5, 0.5)) for _ in sc.materials]
+        sc.lights = [scenes.area_light((0, 5, 0), (1, 0, 0), (0, 0, 1), (1, 1, 1))]
+        sc.env_light, sc.env_pixels = -1, None
+        c = sc.cam_from
+        rays = np.array([[c[0], c[1], c[2], 0.0, -0.6, 0.8, 1e-4, 1e30],
+                         [c[0] + 0.3, c[1], c[2], 0.01, -0.6, 0.8, 1e-4, 1e30]], np.float32)
+        ground = sc.num_meshes - 1
+        o = oracle.scene(sc).trace(rays)
+        r = reference.scene(sc).trace(rays)
+        assert np.array_equal(o["mesh_id"], r["mesh_id"]), grid
+        assert ((o["mesh_id"] == ground).all()) == visible, (grid, o["mesh_id"])
