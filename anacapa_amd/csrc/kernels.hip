@@ -197,7 +197,7 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
 
 // --------------------------------------------------------------------------- shade
 template <bool CURVES>
-__global__ __launch_bounds__(256) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
+__global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
                                                uint32_t cur) {
     __shared__ uint32_t lds[3 * (4 + 1)];
     uint32_t* const ctrs[3] = {ctr(q, bounce + 1u, 0), ctr(q, bounce, 1), ctr(q, bounce, 2)};

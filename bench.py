@@ -46,6 +46,7 @@ def main():
                     help="views of the batched-form comparison (0 = skip it)")
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the Cornell and hair frames")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (read + write), the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -151,6 +152,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_form(be, log)
 
+    others = None
+    if not args.no_configs:
+        others = other_configs(be, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args)
@@ -177,6 +182,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "other_configs": others,
     }
     print(json.dumps(out), flush=True)
 
@@ -277,6 +283,34 @@ def adjoint_form(be, log):
             "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
             "nonzero_albedo_grads": int((da != 0).sum()), "nonzero_texel_grads": int((dt != 0).sum()),
             "timing": "HIP events around the kernels of one call (best of 3)"}
+
+
+def other_configs(be, log):
+    """Workloads 1 and 3 on the same backend: the Cornell box (36 triangles, 512x512, 64 spp, depth
+    8) and the hair sphere (200k Bezier strands = 800k segments, 1024x1024, 32 spp, depth 16), one
+    warm-up and two timed frames each (HIP events around the frame)."""
+    from anacapa_amd import scenes
+    out = {}
+    for name, sc in (("cornell", scenes.cornell()), ("hair", scenes.hair_sphere())):
+        be.prepare(sc)
+        be.clear_accum()
+        be.render_frame(0, sc.spp)
+        ms = []
+        for _ in range(2):
+            be.reset_stats()
+            be.clear_accum()
+            be.render_frame(0, sc.spp)
+            ms.append(be.stats()["render_ms"])
+        st = be.stats()
+        t = min(ms)
+        samples = sc.width * sc.height * sc.spp
+        rays = st["rays_closest"] + st["rays_shadow"]
+        out[name] = {"workload": f"{sc.name}: {sc.num_triangles} tris, {sc.num_strands} strands, "
+                                 f"{sc.width}x{sc.height}, {sc.spp} spp, depth {sc.max_depth}",
+                     "ms_per_frame": round(t, 3), "msamples_per_s": round(samples / t / 1e3, 2),
+                     "mrays_per_s": round(rays / t / 1e3, 2)}
+        log(f"[bench] {name}: {t:.1f} ms/frame, {samples / t / 1e3:.1f} Msamples/s")
+    return out
 
 
 def cpu_baseline(sc, args):
