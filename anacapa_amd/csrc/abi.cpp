@@ -417,8 +417,8 @@ int finishStats(ahip_context* ctx) {
     unsigned long long tot[8];
     HIPCHK(hipMemcpy(tot, ctx->qTotals.p, sizeof(tot), hipMemcpyDeviceToHost));
     // tot[0] continuation rays, tot[1] shadow rays, tot[2] MIS rays, tot[3] stack overflows
-    ctx->stats.rays_closest = tot[0] + (ctx->dsc.hasEmitters ? tot[2] : 0ull);
-    ctx->stats.rays_shadow = tot[1] + (ctx->dsc.hasEmitters ? 0ull : tot[2]);
+    ctx->stats.rays_closest += tot[0] + (ctx->dsc.hasEmitters ? tot[2] : 0ull);
+    ctx->stats.rays_shadow += tot[1] + (ctx->dsc.hasEmitters ? 0ull : tot[2]);
     if (tot[3]) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
     return AHIP_OK;
 }
@@ -878,10 +878,10 @@ int replayArgs(ahip_context* ctx, uint32_t s0, uint32_t ns, ReplayArgs& a, uint3
     a.albE = ctx->albESet ? ctx->dAlbE.as<float>() : nullptr;
     a.texE = ctx->texESet ? ctx->dTexE.as<float>() : nullptr;
     a.maxV = ctx->params.max_depth + 1u;
-    // blocks: a few per CU (the kernel holds ~150-180 VGPRs, 2-3 waves per SIMD), never more
+    // blocks: 6 of 2 waves per CU (the kernel is held to 168 VGPRs, 3 waves per SIMD), never more
     // threads than the traversal overflow stacks were sized for
     const uint32_t bs = replay_block_size();
-    blocks = std::max(1u, ctx->numCU) * 4u;
+    blocks = std::max(1u, ctx->numCU) * 6u;
     blocks = std::min(blocks, (ctx->traceBlocks * 128u) / bs);
     a.nthreads = blocks * bs;
     a.ovfRef = ctx->qOvfRef.as<uint32_t>();
@@ -1016,8 +1016,8 @@ int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height, uint3
     ctx->stats = saved;
     ctx->stats.nodes_visited = s[0];
     ctx->stats.prims_tested = s[1];
-    ctx->stats.rays_closest = counted.rays_closest;
-    ctx->stats.rays_shadow = counted.rays_shadow;
+    ctx->stats.rays_closest = counted.rays_closest - saved.rays_closest;   // the counted frame only
+    ctx->stats.rays_shadow = counted.rays_shadow - saved.rays_shadow;
     return st;
 }
 

@@ -96,7 +96,7 @@ struct EvalPt {
 }  // namespace
 
 template <bool ADJ>
-__global__ __launch_bounds__(kReplayBS) void k_replay(DScene sc, ReplayArgs a) {
+__global__ __launch_bounds__(kReplayBS, 3) void k_replay(DScene sc, ReplayArgs a) {
     __shared__ uint32_t lref[kReplayK * kReplayBS];
     __shared__ float lt[kReplayK * kReplayBS];
     const uint32_t gtid = blockIdx.x * kReplayBS + threadIdx.x;
