@@ -196,7 +196,9 @@ int ensureQueues(ahip_context* ctx, uint32_t cap) {
 
 int ensureStack(ahip_context* ctx) {
     uint32_t K = lds_stack_entries();
-    uint32_t capEntries = std::max(ctx->bvh.maxDepth, ctx->curves.maxDepth) + 2u;
+    // a DNode4 visit pushes at most 3 slots and every stacked slot belongs to a record on the
+    // current root-to-node path; the curve traversal pushes at most one entry per level
+    uint32_t capEntries = std::max(3u * ctx->bvh.depth4, ctx->curves.maxDepth) + 2u;
     ctx->stackCap = capEntries;
     size_t extra = capEntries > K ? capEntries - K : 1u;
     size_t threads = (size_t)ctx->traceBlocks * 128u;
@@ -500,7 +502,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.segIds = ctx->dSegIds.as<uint32_t>();
     S.strandInfo = ctx->dStrandInfo.as<float>();
     S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
-    if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes.data(), ctx->bvh.nodes.size()))) return st;
+    if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes4.data(), ctx->bvh.nodes4.size()))) return st;
     if ((st = upload(ctx, ctx->dTris, ctx->bvh.tris.data(), ctx->bvh.tris.size()))) return st;
     if ((st = upload(ctx, ctx->dShade, ctx->bvh.shade.data(), ctx->bvh.shade.size()))) return st;
     if ((st = upload(ctx, ctx->dMats, hs.mats.data(), hs.mats.size()))) return st;
@@ -526,7 +528,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     }
     ctx->albESet = ctx->texESet = false;
     ctx->diffTex = 0;
-    S.nodes = ctx->dNodes.as<DNode>();
+    S.nodes4 = ctx->dNodes.as<DNode4>();
     S.tris = ctx->dTris.as<DTri>();
     S.shade = ctx->dShade.as<DTriShade>();
     S.mats = ctx->dMats.as<DMat>();
@@ -535,7 +537,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.textures = ctx->dTexDesc.as<DTexture>();
     S.numMeshes = d->num_meshes;
     S.numLights = (uint32_t)hs.lights.size();
-    S.numNodes = (uint32_t)ctx->bvh.nodes.size();
+    S.numNodes = (uint32_t)ctx->bvh.nodes4.size();
     S.numTris = ctx->bvh.numTris;
     S.hasBVH = ctx->bvh.numTris > 0 ? 1u : 0u;
     S.rootRef = ctx->bvh.rootRef;

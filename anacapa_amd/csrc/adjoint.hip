@@ -97,10 +97,9 @@ struct EvalPt {
 
 template <bool ADJ>
 __global__ __launch_bounds__(kReplayBS, 3) void k_replay(DScene sc, ReplayArgs a) {
-    __shared__ uint32_t lref[kReplayK * kReplayBS];
-    __shared__ float lt[kReplayK * kReplayBS];
+    __shared__ uint64_t lstk[kReplayK * kReplayBS];
     const uint32_t gtid = blockIdx.x * kReplayBS + threadIdx.x;
-    TStack S{lref, lt, a.ovfRef, a.ovfT, threadIdx.x, kReplayBS, kReplayK, gtid, a.nthreads, a.stackCap};
+    TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kReplayBS, kReplayK, gtid, a.nthreads, a.stackCap};
     Rec R{a.rec, a.maxV, a.nthreads, gtid};
     const uint32_t total = a.slots * a.ns;
     const bool evalSet = a.albE != nullptr || a.texE != nullptr;

@@ -182,6 +182,68 @@ inline void xformNormalIdentity(const float n[3], float o[3]) {
     o[2] = 0.f * n[0] + 0.f * n[1] + 1.f * n[2];
 }
 
+// Collapse the flattened binary tree into DNode4 records, depth-first with each record before
+// its subtrees.  Returns the record index of binary interior node `bi`; `depth` counts records
+// on the path.  A group is valid only if its box in the parent equals the union of its two
+// children's boxes, which the traversal relies on to reproduce the group's entry distance.
+struct Collapse {
+    const std::vector<DNode>& bin;
+    std::vector<DNode4>& out;
+    uint32_t maxDepth = 0;
+    bool unionOk = true;
+
+    static void setSlot(DNode4& R, int s, const float* lo, const float* hi, uint32_t ref) {
+        for (int k = 0; k < 3; ++k) {
+            R.bmin[k][s] = lo[k];
+            R.bmax[k][s] = hi[k];
+        }
+        R.ref[s] = ref;
+    }
+    static void setEmpty(DNode4& R, int s) {
+        for (int k = 0; k < 3; ++k) {
+            R.bmin[k][s] = kInf;
+            R.bmax[k][s] = -kInf;
+        }
+        R.ref[s] = kRefEmpty;
+    }
+
+    uint32_t run(uint32_t bi, uint32_t depth) {
+        maxDepth = std::max(maxDepth, depth);
+        const uint32_t idx = static_cast<uint32_t>(out.size());
+        out.emplace_back();
+        const DNode X = bin[bi];
+        uint32_t pending[4];
+        for (int g = 0; g < 2; ++g) {
+            const uint32_t cref = g == 0 ? X.left : X.right;
+            const float* glo = g == 0 ? X.lmin : X.rmin;
+            const float* ghi = g == 0 ? X.lmax : X.rmax;
+            if (refIsLeaf(cref)) {
+                setSlot(out[idx], 2 * g, glo, ghi, cref);
+                setEmpty(out[idx], 2 * g + 1);
+                pending[2 * g] = pending[2 * g + 1] = kRefEmpty;
+                continue;
+            }
+            const DNode& C = bin[refIndex(cref)];
+            for (int k = 0; k < 3; ++k) {
+                unionOk = unionOk && glo[k] == std::min(C.lmin[k], C.rmin[k]) &&
+                          ghi[k] == std::max(C.lmax[k], C.rmax[k]);
+            }
+            setSlot(out[idx], 2 * g, C.lmin, C.lmax, C.left);
+            setSlot(out[idx], 2 * g + 1, C.rmin, C.rmax, C.right);
+            pending[2 * g] = C.left;
+            pending[2 * g + 1] = C.right;
+        }
+        out[idx].pad[0] = out[idx].pad[1] = out[idx].pad[2] = out[idx].pad[3] = 0u;
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t r = pending[s];
+            if (r == kRefEmpty || refIsLeaf(r)) continue;
+            const uint32_t child = run(refIndex(r), depth + 1);
+            out[idx].ref[s] = child;
+        }
+        return idx;
+    }
+};
+
 }  // namespace
 
 int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err) {
@@ -304,6 +366,17 @@ int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err) {
         uint32_t src = out.refOrder[n + k];
         out.tris[k] = tri[src];
         out.shade[k] = sh[src];
+    }
+    // 5. two-level records for the traversal kernels
+    if (!refIsLeaf(out.rootRef)) {
+        out.nodes4.reserve(out.nodes.size() / 2 + 2);
+        Collapse c{out.nodes, out.nodes4};
+        out.rootRef = c.run(refIndex(out.rootRef), 1);
+        out.depth4 = c.maxDepth;
+        if (!c.unionOk) {
+            err = "BVH node box differs from the union of its children's boxes";
+            return AHIP_E_INVALID_ARG;
+        }
     }
     return 0;
 }

@@ -10,6 +10,7 @@
 namespace ahip {
 
 struct DNode;
+struct DNode4;
 struct DTri;
 struct DTriShade;
 
@@ -25,12 +26,14 @@ struct BVHOutput {
     std::vector<RefNode> refNodes;     // reference binary tree, depth-first
     std::vector<uint32_t> refOrder;    // reference m_primIndices (iota prefix + leaf order)
     std::vector<DNode> nodes;          // flattened interior nodes
+    std::vector<DNode4> nodes4;        // two-level records the kernels traverse (root first)
     std::vector<DTri> tris;            // leaf-ordered triangles
     std::vector<DTriShade> shade;
     uint32_t rootRef = 0;
     float rootMin[3] = {0, 0, 0}, rootMax[3] = {0, 0, 0};
     uint32_t numTris = 0;
     uint32_t maxDepth = 0;
+    uint32_t depth4 = 0;               // most DNode4 records on one root-to-leaf path
 };
 
 int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err);

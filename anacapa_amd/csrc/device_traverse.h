@@ -1,4 +1,4 @@
-// device_traverse.h — BVH traversal over 64-byte two-box nodes with an LDS short stack.
+// device_traverse.h — BVH traversal over 128-byte two-level nodes with an LDS short stack.
 //
 // Semantics are those of BVHBackend::traceImpl / occluded (src/accel/BVHBackend.cpp:233-444):
 //  * invDir = 1 / safe(d) with |d| <= 1e-9 mapped to +1e-9, and dirNeg from the sign of d
@@ -9,9 +9,10 @@
 //    (BVHBackend.cpp:264-283, 369-372);
 //  * nearer child first, the right child first on equal entry distance, nodes culled when
 //    their entry distance exceeds the closest hit (BVHBackend.cpp:349, 376-387).
-// Both child boxes live in the parent record, so the entry distance computed when a child
-// is pushed is exactly what the reference recomputes when it pops that child; the cull test
-// is applied at push and again at pop.  Any-hit queries (shadow rays) return whether a hit
+// The kernels walk DNode4 records (two binary levels per record) in exactly the binary
+// traversal's order and with its culls (see traverse below); the entry distance stored with a
+// stacked slot is the one the reference recomputes when it pops that node, and the cull test is
+// applied at push and again at pop.  Any-hit queries (shadow rays) return whether a hit
 // exists in [tMin, tMax], which is the boolean both BVHBackend::occluded and the closest-hit
 // based shadowTransmittance (src/integrator/ShadowRay.h:26-61) produce for opaque scenes.
 //
@@ -106,38 +107,55 @@ __device__ __forceinline__ DTri loadTriGeom(const DTri* tris, uint32_t i) {
     return T;
 }
 
-struct NodeBoxes {
-    float lmin[3], lmax[3], rmin[3], rmax[3];
-    uint32_t left, right;
+// Slab test of one slot of a DNode4 given its near and far planes (already chosen by the
+// direction sign); also returns the per-axis near distances the group entry is built from.
+// The arithmetic is that of slab() above, so a slot's entry distance and verdict are the bits
+// the binary tree's test of the same box gives.
+struct SlotSlab {
+    float t0x, t0y, t0z, tmin;
+    bool hit;
 };
-__device__ __forceinline__ NodeBoxes loadNode(const DNode* nodes, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    float4 a = p[0], b = p[1], c = p[2];
-    uint4 d = reinterpret_cast<const uint4*>(nodes + i)[3];
-    NodeBoxes n;
-    n.lmin[0] = a.x; n.lmin[1] = a.y; n.lmin[2] = a.z;
-    n.lmax[0] = a.w; n.lmax[1] = b.x; n.lmax[2] = b.y;
-    n.rmin[0] = b.z; n.rmin[1] = b.w; n.rmin[2] = c.x;
-    n.rmax[0] = c.y; n.rmax[1] = c.z; n.rmax[2] = c.w;
-    n.left = d.x;
-    n.right = d.y;
-    return n;
+__device__ __forceinline__ SlotSlab slotSlab(float nx, float ny, float nz, float fx, float fy, float fz,
+                                             const TRay& r) {
+#pragma clang fp contract(off)
+    SlotSlab o;
+    o.t0x = (nx - r.o.x) * r.inv.x;
+    o.t0y = (ny - r.o.y) * r.inv.y;
+    o.t0z = (nz - r.o.z) * r.inv.z;
+    float t1x = (fx - r.o.x) * r.inv.x, t1y = (fy - r.o.y) * r.inv.y, t1z = (fz - r.o.z) * r.inv.z;
+    o.tmin = fmaxf(fmaxf(fmaxf(r.tMin, o.t0x), o.t0y), o.t0z);
+    float tmax = fminf(fminf(fminf(r.tMax, t1x), t1y), t1z);
+    o.hit = tmax > o.tmin;
+    return o;
+}
+
+// Entry distance of the union of two slot boxes.  The near plane of the union on each axis is
+// the nearer of the two slots' planes, and (p - o) * inv rounds monotonically in p, so the
+// union's per-axis distance is the smaller of the slots' distances, bit for bit what the slab
+// test of the union box itself computes.
+__device__ __forceinline__ float groupEntry(const SlotSlab& a, const SlotSlab& b, float tMin) {
+    return fmaxf(fmaxf(fmaxf(tMin, fminf(a.t0x, b.t0x)), fminf(a.t0y, b.t0y)), fminf(a.t0z, b.t0z));
 }
 
 // Short stack: K entries per thread in LDS (entry-major so a wave touches consecutive banks),
-// deeper entries in a per-thread global overflow region.
+// each entry one 8-byte (reference, entry distance) pair; deeper entries in a per-thread global
+// overflow region.  The LDS side is typed in the LDS address space so the compiler keeps the
+// LDS and global accesses apart (ds_read / global_load) instead of merging them into one
+// generic-address load.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 struct TStack {
-    uint32_t* lref;     // LDS [K * BS]
-    float* lt;          // LDS [K * BS]
+    lds_u64* l;         // LDS [K * BS]: reference | entry distance bits << 32
     uint32_t* gref;     // global [(cap-K) * gstride]
     float* gt;
     uint32_t tid, bs, K, gtid, gstride, cap;
+    __device__ TStack(uint64_t* lds, uint32_t* gr, float* g, uint32_t tid_, uint32_t bs_, uint32_t K_, uint32_t gtid_,
+                      uint32_t gs, uint32_t cap_)
+        : l((lds_u64*)lds), gref(gr), gt(g), tid(tid_), bs(bs_), K(K_), gtid(gtid_), gstride(gs), cap(cap_) {}
 };
 
 __device__ __forceinline__ void spush(const TStack& S, uint32_t& sp, uint32_t ref, float t) {
     if (sp < S.K) {
-        S.lref[sp * S.bs + S.tid] = ref;
-        S.lt[sp * S.bs + S.tid] = t;
+        S.l[sp * S.bs + S.tid] = (uint64_t)ref | ((uint64_t)__float_as_uint(t) << 32);
     } else {
         size_t o = (size_t)(sp - S.K) * S.gstride + S.gtid;
         S.gref[o] = ref;
@@ -148,8 +166,9 @@ __device__ __forceinline__ void spush(const TStack& S, uint32_t& sp, uint32_t re
 __device__ __forceinline__ void spop(const TStack& S, uint32_t& sp, uint32_t& ref, float& t) {
     --sp;
     if (sp < S.K) {
-        ref = S.lref[sp * S.bs + S.tid];
-        t = S.lt[sp * S.bs + S.tid];
+        uint64_t e = S.l[sp * S.bs + S.tid];
+        ref = (uint32_t)e;
+        t = __uint_as_float((uint32_t)(e >> 32));
     } else {
         size_t o = (size_t)(sp - S.K) * S.gstride + S.gtid;
         ref = S.gref[o];
@@ -163,6 +182,16 @@ struct TraceCounters {
 
 // Closest hit (ANY = false) or any hit (ANY = true).  Returns the leaf-order triangle index
 // of the hit, or ~0u.  *overflow is set when the stack would exceed its capacity.
+//
+// Each DNode4 visit stands for binary node X and its children A (slots 0,1) and B (slots 2,3).
+// The binary traversal enters the nearer of A and B (B on a tie), then the nearer of that
+// group's children (the second on a tie), and pops the rest later, culling each against the
+// closest hit at that moment.  Here the slots are put in that same order (group entries from
+// groupEntry, slot entries from slotSlab), the first live slot is entered and the other live
+// slots are pushed in reverse.  Culling is monotone (the closest distance only shrinks and a
+// child's entry is never below its group's), so testing each slot against the closest hit when
+// it is entered or popped skips exactly the subtrees the binary traversal skips, leaves are
+// reached in the same order, and ties between equally distant triangles resolve identically.
 template <bool ANY, bool COUNT>
 __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, float& tHit, float& uHit, float& vHit,
                              TraceCounters& cnt, uint32_t* overflow) {
@@ -171,50 +200,91 @@ __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, f
     float hu = 0.f, hv = 0.f;
     if (!sc.hasBVH) return ~0u;
     float tn;
-    if (COUNT) ++cnt.nodes;
-    if (!slab(sc.rootMin, sc.rootMax, r, tn)) return ~0u;
+    if (!slab(sc.rootMin, sc.rootMax, r, tn)) return ~0u;   // root box from the kernel argument
     uint32_t cur = sc.rootRef;
     uint32_t sp = 0;
+    const bool nx = (r.neg & 1u) != 0u, ny = (r.neg & 2u) != 0u, nz = (r.neg & 4u) != 0u;
+    uint32_t triIdx = 0u, triLeft = 0u;   // leaf being tested: next triangle, triangles left
     while (true) {
-        if (refIsLeaf(cur)) {
-            uint32_t first = refIndex(cur), n = refCount(cur);
-            for (uint32_t k = 0; k < n; ++k) {
-                DTri T = loadTriGeom(sc.tris, first + k);
-                float t, u, v;
-                if (COUNT) ++cnt.tris;
-                if (hitTri(T, r, closest, t, u, v)) {
-                    closest = t;
-                    hit = first + k;
-                    hu = u;
-                    hv = v;
-                    if (ANY) {
-                        tHit = t;
-                        return hit;
-                    }
+        // One record per iteration: a node, or ONE triangle of the current leaf.  The first three
+        // 16-byte loads are common to both, so lanes at a leaf and lanes at a node issue their
+        // loads together and the wave waits once per iteration.
+        if (triLeft == 0u && refIsLeaf(cur)) {
+            triIdx = refIndex(cur);
+            triLeft = refCount(cur);
+        }
+        const bool isTri = triLeft != 0u;
+        const float4* p = isTri ? reinterpret_cast<const float4*>(sc.tris + triIdx)
+                                : reinterpret_cast<const float4*>(sc.nodes4 + refIndex(cur));
+        const float4 mnx = p[0], mny = p[1], mnz = p[2];
+        float4 mxx, mxy, mxz;
+        uint4 rf;
+        if (!isTri) {
+            mxx = p[3];
+            mxy = p[4];
+            mxz = p[5];
+            rf = reinterpret_cast<const uint4*>(p)[6];
+        }
+        if (isTri) {
+            DTri T;
+            T.v0[0] = mnx.x; T.v0[1] = mnx.y; T.v0[2] = mnx.z;
+            T.e1[0] = mnx.w; T.e1[1] = mny.x; T.e1[2] = mny.y;
+            T.e2[0] = mny.z; T.e2[1] = mny.w; T.e2[2] = mnz.x;
+            float t, u, v;
+            if (COUNT) ++cnt.tris;
+            if (hitTri(T, r, closest, t, u, v)) {
+                closest = t;
+                hit = triIdx;
+                hu = u;
+                hv = v;
+                if (ANY) {
+                    tHit = t;
+                    return hit;
                 }
             }
+            ++triIdx;
+            if (--triLeft != 0u) continue;
         } else {
-            NodeBoxes nd = loadNode(sc.nodes, refIndex(cur));
             if (COUNT) ++cnt.nodes;
-            float tL, tR;
-            bool hL = slab(nd.lmin, nd.lmax, r, tL) && !(tL > closest);
-            bool hR = slab(nd.rmin, nd.rmax, r, tR) && !(tR > closest);
-            if (hL && hR) {
-                if (sp + 1u >= S.cap) {
+            const float4 ax = nx ? mxx : mnx, bx = nx ? mnx : mxx;
+            const float4 ay = ny ? mxy : mny, by = ny ? mny : mxy;
+            const float4 az = nz ? mxz : mnz, bz = nz ? mnz : mxz;
+            const SlotSlab s0 = slotSlab(ax.x, ay.x, az.x, bx.x, by.x, bz.x, r);
+            const SlotSlab s1 = slotSlab(ax.y, ay.y, az.y, bx.y, by.y, bz.y, r);
+            const SlotSlab s2 = slotSlab(ax.z, ay.z, az.z, bx.z, by.z, bz.z, r);
+            const SlotSlab s3 = slotSlab(ax.w, ay.w, az.w, bx.w, by.w, bz.w, r);
+            const bool l0 = s0.hit && !(s0.tmin > closest) && rf.x != kRefEmpty;
+            const bool l1 = s1.hit && !(s1.tmin > closest) && rf.y != kRefEmpty;
+            const bool l2 = s2.hit && !(s2.tmin > closest) && rf.z != kRefEmpty;
+            const bool l3 = s3.hit && !(s3.tmin > closest) && rf.w != kRefEmpty;
+            // order within each group, then of the groups (left first only when strictly nearer)
+            const bool aSwap = l0 && l1 && !(s0.tmin < s1.tmin);
+            const bool bSwap = l2 && l3 && !(s2.tmin < s3.tmin);
+            const bool gSwap = (l0 || l1) && (l2 || l3) &&
+                               !(groupEntry(s0, s1, r.tMin) < groupEntry(s2, s3, r.tMin));
+            uint32_t ra0 = aSwap ? rf.y : rf.x, ra1 = aSwap ? rf.x : rf.y;
+            float ta0 = aSwap ? s1.tmin : s0.tmin, ta1 = aSwap ? s0.tmin : s1.tmin;
+            bool la0 = aSwap ? l1 : l0, la1 = aSwap ? l0 : l1;
+            uint32_t rb0 = bSwap ? rf.w : rf.z, rb1 = bSwap ? rf.z : rf.w;
+            float tb0 = bSwap ? s3.tmin : s2.tmin, tb1 = bSwap ? s2.tmin : s3.tmin;
+            bool lb0 = bSwap ? l3 : l2, lb1 = bSwap ? l2 : l3;
+            // visiting order q0, q1, q2, q3
+            const uint32_t q0 = gSwap ? rb0 : ra0, q1 = gSwap ? rb1 : ra1, q2 = gSwap ? ra0 : rb0, q3 = gSwap ? ra1 : rb1;
+            const float u0 = gSwap ? tb0 : ta0, u1 = gSwap ? tb1 : ta1, u2 = gSwap ? ta0 : tb0, u3 = gSwap ? ta1 : tb1;
+            const bool v0 = gSwap ? lb0 : la0, v1 = gSwap ? lb1 : la1, v2 = gSwap ? la0 : lb0, v3 = gSwap ? la1 : lb1;
+            const uint32_t live = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
+            if (live != 0u) {
+                if (sp + live - 1u >= S.cap) {
                     *overflow = 1u;
                     break;
                 }
-                if (tL < tR) {
-                    spush(S, sp, nd.right, tR);
-                    cur = nd.left;
-                } else {
-                    spush(S, sp, nd.left, tL);
-                    cur = nd.right;
-                }
+                // push the live slots after the first, farthest in the order first
+                if (v3 && (v0 || v1 || v2)) spush(S, sp, q3, u3);
+                if (v2 && (v0 || v1)) spush(S, sp, q2, u2);
+                if (v1 && v0) spush(S, sp, q1, u1);
+                cur = v0 ? q0 : v1 ? q1 : v2 ? q2 : q3;
                 continue;
             }
-            if (hL) { cur = nd.left; continue; }
-            if (hR) { cur = nd.right; continue; }
         }
         bool found = false;
         while (sp > 0u) {

@@ -28,6 +28,23 @@ struct alignas(64) DNode {
 };
 static_assert(sizeof(DNode) == 64, "DNode must be 64 bytes");
 
+// Two levels of the binary BVH in one 128-byte record, the node the traversal kernels read.
+// For a binary interior node X with children A (left) and B (right), slots 0,1 hold A's two
+// children and slots 2,3 hold B's; a child that is a leaf takes its group's first slot and
+// leaves the second empty (box +inf..-inf, ref kRefEmpty).  Interior slots refer to the record
+// built for that grandchild.  Bounds are stored per axis so one 16-byte load fetches one bound
+// of all four slots.  A group's own box is the union of its slots' boxes (each node's box is
+// grown over its own primitives), so the traversal recomputes the group's slab entry distance
+// from the slots' per-axis values and visits the slots in exactly the binary tree's order.
+struct alignas(128) DNode4 {
+    float bmin[3][4];         // [axis][slot]  (bytes   0..47)
+    float bmax[3][4];         //               (bytes  48..95)
+    uint32_t ref[4];          //               (bytes  96..111)
+    uint32_t pad[4];
+};
+static_assert(sizeof(DNode4) == 128, "DNode4 must be 128 bytes");
+constexpr uint32_t kRefEmpty = 0xffffffffu;
+
 // Leaf-ordered triangle record used by intersection (BVHTriangle v0/e1/e2/n, BVHBackend.h:49-59).
 struct alignas(64) DTri {
     float v0[3];
@@ -123,7 +140,7 @@ struct DCamera {
 
 // Everything a kernel needs to see of the scene (passed by value as a kernel argument).
 struct DScene {
-    const DNode* nodes;
+    const DNode4* nodes4;     // two-level records, root at index 0 when rootRef is interior
     const DTri* tris;
     const DTriShade* shade;
     const DMat* mats;

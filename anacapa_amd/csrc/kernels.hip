@@ -163,9 +163,8 @@ __global__ __launch_bounds__(256) void k_generate(RenderConsts rc, WaveQueues q)
 // more waves in flight); the host picks the instance from DScene::hasCurves.
 template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
-    __shared__ uint32_t lref[K * BS];
-    __shared__ float lt[K * BS];
-    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
              q.stackCap};
     const uint32_t count = *ctr(q, bounce, 0);
     TraceCounters tc{0, 0};
@@ -388,9 +387,8 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
 // --------------------------------------------------------------------------- shadow (any hit)
 template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t bounce) {
-    __shared__ uint32_t lref[K * BS];
-    __shared__ float lt[K * BS];
-    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
              q.stackCap};
     const uint32_t count = *ctr(q, bounce, 1);
     TraceCounters tc{0, 0};
@@ -417,9 +415,8 @@ __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t
 // --------------------------------------------------------------------------- MIS (BSDF-sampled NEE)
 template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bounce) {
-    __shared__ uint32_t lref[K * BS];
-    __shared__ float lt[K * BS];
-    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
              q.stackCap};
     const uint32_t count = *ctr(q, bounce, 2);
     TraceCounters tc{0, 0};
@@ -498,9 +495,8 @@ template <uint32_t BS, uint32_t K, bool ANY>
 __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays, uint32_t n, ahip_hit* hits,
                                                     uint8_t* occ, uint32_t* ovfRef, float* ovfT, uint32_t cap,
                                                     uint32_t* overflow, uint32_t* counts) {
-    __shared__ uint32_t lref[K * BS];
-    __shared__ float lt[K * BS];
-    TStack S{lref, lt, ovfRef, ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS, cap};
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, ovfRef, ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS, cap};
     for (uint32_t base = blockIdx.x * BS; base < n; base += gridDim.x * BS) {
         uint32_t i = base + threadIdx.x;
         if (i >= n) break;
@@ -544,9 +540,8 @@ __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays
 template <uint32_t BS, uint32_t K, int KIND, bool CURVES>
 __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur,
                                               unsigned long long* sums) {
-    __shared__ uint32_t lref[K * BS];
-    __shared__ float lt[K * BS];
-    TStack S{lref, lt, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
              q.stackCap};
     const uint32_t count = *ctr(q, bounce, KIND);
     for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {

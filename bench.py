@@ -124,10 +124,11 @@ def main():
 
     roofline = None
     if not args.no_traffic:
-        # Algorithmic bytes of the traversal kernels for one frame: 64 B per node record read,
-        # 64 B per triangle record tested, 48 B of queue I/O per ray (32-B ray read + 16-B result).
+        # Algorithmic bytes of the traversal kernels for one frame: 112 B per two-level node
+        # visited (four slot boxes + four references), 48 B per triangle tested (v0, e1, e2),
+        # 48 B of queue I/O per ray (32-B ray read + 16-B result).
         ct = be.count_traffic(0, S)
-        frame_bytes = 64.0 * (ct["nodes_visited"] + ct["prims_tested"]) + 48.0 * (ct["rays_closest"] + ct["rays_shadow"])
+        frame_bytes = 112.0 * ct["nodes_visited"] + 48.0 * ct["prims_tested"] + 48.0 * (ct["rays_closest"] + ct["rays_shadow"])
         per_launch = frame_bytes * args.steps / launches
         avg_launch_s = trace_ms / 1e3 / launches
         achieved = per_launch / avg_launch_s / 1e9
