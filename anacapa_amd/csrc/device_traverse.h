@@ -253,25 +253,41 @@ __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, f
             const SlotSlab s1 = slotSlab(ax.y, ay.y, az.y, bx.y, by.y, bz.y, r);
             const SlotSlab s2 = slotSlab(ax.z, ay.z, az.z, bx.z, by.z, bz.z, r);
             const SlotSlab s3 = slotSlab(ax.w, ay.w, az.w, bx.w, by.w, bz.w, r);
-            const bool l0 = s0.hit && !(s0.tmin > closest) && rf.x != kRefEmpty;
-            const bool l1 = s1.hit && !(s1.tmin > closest) && rf.y != kRefEmpty;
-            const bool l2 = s2.hit && !(s2.tmin > closest) && rf.z != kRefEmpty;
-            const bool l3 = s3.hit && !(s3.tmin > closest) && rf.w != kRefEmpty;
-            // order within each group, then of the groups (left first only when strictly nearer)
-            const bool aSwap = l0 && l1 && !(s0.tmin < s1.tmin);
-            const bool bSwap = l2 && l3 && !(s2.tmin < s3.tmin);
-            const bool gSwap = (l0 || l1) && (l2 || l3) &&
-                               !(groupEntry(s0, s1, r.tMin) < groupEntry(s2, s3, r.tMin));
-            uint32_t ra0 = aSwap ? rf.y : rf.x, ra1 = aSwap ? rf.x : rf.y;
-            float ta0 = aSwap ? s1.tmin : s0.tmin, ta1 = aSwap ? s0.tmin : s1.tmin;
-            bool la0 = aSwap ? l1 : l0, la1 = aSwap ? l0 : l1;
-            uint32_t rb0 = bSwap ? rf.w : rf.z, rb1 = bSwap ? rf.z : rf.w;
-            float tb0 = bSwap ? s3.tmin : s2.tmin, tb1 = bSwap ? s2.tmin : s3.tmin;
-            bool lb0 = bSwap ? l3 : l2, lb1 = bSwap ? l2 : l3;
-            // visiting order q0, q1, q2, q3
-            const uint32_t q0 = gSwap ? rb0 : ra0, q1 = gSwap ? rb1 : ra1, q2 = gSwap ? ra0 : rb0, q3 = gSwap ? ra1 : rb1;
-            const float u0 = gSwap ? tb0 : ta0, u1 = gSwap ? tb1 : ta1, u2 = gSwap ? ta0 : tb0, u3 = gSwap ? ta1 : tb1;
-            const bool v0 = gSwap ? lb0 : la0, v1 = gSwap ? lb1 : la1, v2 = gSwap ? la0 : lb0, v3 = gSwap ? la1 : lb1;
+            uint32_t q0, q1, q2, q3;
+            float u0, u1, u2, u3;
+            bool v0, v1, v2, v3;
+            if constexpr (ANY) {
+                // An any-hit query ends at its first hit and culls nothing before it (a slot that
+                // passes the slab test enters before tMax = closest), and the answer does not
+                // depend on the order, so the live slots are taken in record order.
+                v0 = s0.hit & (rf.x != kRefEmpty);
+                v1 = s1.hit & (rf.y != kRefEmpty);
+                v2 = s2.hit & (rf.z != kRefEmpty);
+                v3 = s3.hit & (rf.w != kRefEmpty);
+                q0 = rf.x; q1 = rf.y; q2 = rf.z; q3 = rf.w;
+                u0 = s0.tmin; u1 = s1.tmin; u2 = s2.tmin; u3 = s3.tmin;
+            } else {
+                const bool l0 = s0.hit & !(s0.tmin > closest) & (rf.x != kRefEmpty);
+                const bool l1 = s1.hit & !(s1.tmin > closest) & (rf.y != kRefEmpty);
+                const bool l2 = s2.hit & !(s2.tmin > closest) & (rf.z != kRefEmpty);
+                const bool l3 = s3.hit & !(s3.tmin > closest) & (rf.w != kRefEmpty);
+                // order within each group, then of the groups (left first only when strictly
+                // nearer); the group entries are computed unconditionally (no branch)
+                const float gA = groupEntry(s0, s1, r.tMin), gB = groupEntry(s2, s3, r.tMin);
+                const bool aSwap = l0 & l1 & !(s0.tmin < s1.tmin);
+                const bool bSwap = l2 & l3 & !(s2.tmin < s3.tmin);
+                const bool gSwap = (l0 | l1) & (l2 | l3) & !(gA < gB);
+                const uint32_t ra0 = aSwap ? rf.y : rf.x, ra1 = aSwap ? rf.x : rf.y;
+                const float ta0 = aSwap ? s1.tmin : s0.tmin, ta1 = aSwap ? s0.tmin : s1.tmin;
+                const bool la0 = aSwap ? l1 : l0, la1 = aSwap ? l0 : l1;
+                const uint32_t rb0 = bSwap ? rf.w : rf.z, rb1 = bSwap ? rf.z : rf.w;
+                const float tb0 = bSwap ? s3.tmin : s2.tmin, tb1 = bSwap ? s2.tmin : s3.tmin;
+                const bool lb0 = bSwap ? l3 : l2, lb1 = bSwap ? l2 : l3;
+                // visiting order q0, q1, q2, q3
+                q0 = gSwap ? rb0 : ra0; q1 = gSwap ? rb1 : ra1; q2 = gSwap ? ra0 : rb0; q3 = gSwap ? ra1 : rb1;
+                u0 = gSwap ? tb0 : ta0; u1 = gSwap ? tb1 : ta1; u2 = gSwap ? ta0 : tb0; u3 = gSwap ? ta1 : tb1;
+                v0 = gSwap ? lb0 : la0; v1 = gSwap ? lb1 : la1; v2 = gSwap ? la0 : lb0; v3 = gSwap ? la1 : lb1;
+            }
             const uint32_t live = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
             if (live != 0u) {
                 if (sp + live - 1u >= S.cap) {
@@ -286,13 +302,20 @@ __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, f
                 continue;
             }
         }
-        bool found = false;
-        while (sp > 0u) {
+        if constexpr (ANY) {
+            // nothing is culled before the first hit: the top entry is always taken
+            if (sp == 0u) break;
             float t;
             spop(S, sp, cur, t);
-            if (!(t > closest)) { found = true; break; }
+        } else {
+            bool found = false;
+            while (sp > 0u) {
+                float t;
+                spop(S, sp, cur, t);
+                if (!(t > closest)) { found = true; break; }
+            }
+            if (!found) break;
         }
-        if (!found) break;
     }
     tHit = closest;
     uHit = hu;

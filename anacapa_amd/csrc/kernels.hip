@@ -413,7 +413,9 @@ __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- MIS (BSDF-sampled NEE)
-template <uint32_t BS, uint32_t K, bool CURVES>
+// EMIT = sc.hasEmitters, as a template parameter so the any-hit instance (no emitting surface)
+// keeps only the registers of the any-hit traversal.
+template <uint32_t BS, uint32_t K, bool CURVES, bool EMIT>
 __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bounce) {
     __shared__ uint64_t lstk[K * BS];
     TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
         TRay r = makeTRay(v3(a), d, 1e-4f, b.w);
         float t, u, v;
         V3 Li{0.f, 0.f, 0.f};
-        if (sc.hasEmitters) {
+        if constexpr (EMIT) {
             uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
             uint32_t mesh = h != ~0u ? sc.tris[h].meshID : ~0u;
             if (CURVES && sc.hasCurves) {
@@ -648,10 +650,17 @@ void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32
         hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    if (sc.hasCurves)
-        hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
-    else
-        hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    if (sc.hasCurves) {
+        if (sc.hasEmitters)
+            hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, true, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+        else
+            hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, true, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    } else {
+        if (sc.hasEmitters)
+            hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, false, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+        else
+            hipLaunchKernelGGL((k_mis<kTraceBS, kStackK, false, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    }
 }
 void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, q);
