@@ -180,6 +180,146 @@ struct TraceCounters {
     uint32_t nodes, tris;
 };
 
+// Per-ray traversal state, so a kernel can advance many rays one step at a time.
+struct TState {
+    uint32_t cur;        // node or leaf reference being visited
+    uint32_t sp;         // stack depth
+    uint32_t triIdx;     // next triangle of the current leaf
+    uint32_t triLeft;    // triangles of the current leaf still to test
+    uint32_t hit;        // leaf-order triangle index of the closest hit so far, or ~0u
+    float closest, hu, hv;
+};
+
+// Root test; false when the ray misses the scene box (or there is no BVH) and the query is over.
+__device__ __forceinline__ bool traverseBegin(const DScene& sc, const TRay& r, TState& st) {
+    st.cur = sc.rootRef;
+    st.sp = 0u;
+    st.triIdx = 0u;
+    st.triLeft = 0u;
+    st.hit = ~0u;
+    st.closest = r.tMax;
+    st.hu = 0.f;
+    st.hv = 0.f;
+    if (!sc.hasBVH) return false;
+    float tn;
+    return slab(sc.rootMin, sc.rootMax, r, tn);   // root box from the kernel argument
+}
+
+// One step: a node visit, or ONE triangle of the current leaf, then a pop when the step leaves
+// nothing to enter.  Returns true when the query is over (stack exhausted, any-hit found, or
+// stack overflow, which sets *overflow).  The first three 16-byte loads are common to a node and
+// a triangle, so lanes at a leaf and lanes at a node issue their loads together and the wave
+// waits once per step.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool traverseStep(const DScene& sc, const TRay& r, const TStack& S, TState& st,
+                                             TraceCounters& cnt, uint32_t* overflow) {
+    if (st.triLeft == 0u && refIsLeaf(st.cur)) {
+        st.triIdx = refIndex(st.cur);
+        st.triLeft = refCount(st.cur);
+    }
+    const bool isTri = st.triLeft != 0u;
+    const float4* p = isTri ? reinterpret_cast<const float4*>(sc.tris + st.triIdx)
+                            : reinterpret_cast<const float4*>(sc.nodes4 + refIndex(st.cur));
+    const float4 mnx = p[0], mny = p[1], mnz = p[2];
+    float4 mxx, mxy, mxz;
+    uint4 rf;
+    if (!isTri) {
+        mxx = p[3];
+        mxy = p[4];
+        mxz = p[5];
+        rf = reinterpret_cast<const uint4*>(p)[6];
+    }
+    if (isTri) {
+        DTri T;
+        T.v0[0] = mnx.x; T.v0[1] = mnx.y; T.v0[2] = mnx.z;
+        T.e1[0] = mnx.w; T.e1[1] = mny.x; T.e1[2] = mny.y;
+        T.e2[0] = mny.z; T.e2[1] = mny.w; T.e2[2] = mnz.x;
+        float t, u, v;
+        if (COUNT) ++cnt.tris;
+        if (hitTri(T, r, st.closest, t, u, v)) {
+            st.closest = t;
+            st.hit = st.triIdx;
+            st.hu = u;
+            st.hv = v;
+            if (ANY) return true;
+        }
+        ++st.triIdx;
+        if (--st.triLeft != 0u) return false;
+    } else {
+        if (COUNT) ++cnt.nodes;
+        const bool nx = (r.neg & 1u) != 0u, ny = (r.neg & 2u) != 0u, nz = (r.neg & 4u) != 0u;
+        const float4 ax = nx ? mxx : mnx, bx = nx ? mnx : mxx;
+        const float4 ay = ny ? mxy : mny, by = ny ? mny : mxy;
+        const float4 az = nz ? mxz : mnz, bz = nz ? mnz : mxz;
+        const SlotSlab s0 = slotSlab(ax.x, ay.x, az.x, bx.x, by.x, bz.x, r);
+        const SlotSlab s1 = slotSlab(ax.y, ay.y, az.y, bx.y, by.y, bz.y, r);
+        const SlotSlab s2 = slotSlab(ax.z, ay.z, az.z, bx.z, by.z, bz.z, r);
+        const SlotSlab s3 = slotSlab(ax.w, ay.w, az.w, bx.w, by.w, bz.w, r);
+        uint32_t q0, q1, q2, q3;
+        float u0, u1, u2, u3;
+        bool v0, v1, v2, v3;
+        if constexpr (ANY) {
+            // An any-hit query ends at its first hit and culls nothing before it (a slot that
+            // passes the slab test enters before tMax = closest), and the answer does not
+            // depend on the order, so the live slots are taken in record order.
+            v0 = s0.hit & (rf.x != kRefEmpty);
+            v1 = s1.hit & (rf.y != kRefEmpty);
+            v2 = s2.hit & (rf.z != kRefEmpty);
+            v3 = s3.hit & (rf.w != kRefEmpty);
+            q0 = rf.x; q1 = rf.y; q2 = rf.z; q3 = rf.w;
+            u0 = s0.tmin; u1 = s1.tmin; u2 = s2.tmin; u3 = s3.tmin;
+        } else {
+            const bool l0 = s0.hit & !(s0.tmin > st.closest) & (rf.x != kRefEmpty);
+            const bool l1 = s1.hit & !(s1.tmin > st.closest) & (rf.y != kRefEmpty);
+            const bool l2 = s2.hit & !(s2.tmin > st.closest) & (rf.z != kRefEmpty);
+            const bool l3 = s3.hit & !(s3.tmin > st.closest) & (rf.w != kRefEmpty);
+            // order within each group, then of the groups (left first only when strictly
+            // nearer); the group entries are computed unconditionally (no branch)
+            const float gA = groupEntry(s0, s1, r.tMin), gB = groupEntry(s2, s3, r.tMin);
+            const bool aSwap = l0 & l1 & !(s0.tmin < s1.tmin);
+            const bool bSwap = l2 & l3 & !(s2.tmin < s3.tmin);
+            const bool gSwap = (l0 | l1) & (l2 | l3) & !(gA < gB);
+            const uint32_t ra0 = aSwap ? rf.y : rf.x, ra1 = aSwap ? rf.x : rf.y;
+            const float ta0 = aSwap ? s1.tmin : s0.tmin, ta1 = aSwap ? s0.tmin : s1.tmin;
+            const bool la0 = aSwap ? l1 : l0, la1 = aSwap ? l0 : l1;
+            const uint32_t rb0 = bSwap ? rf.w : rf.z, rb1 = bSwap ? rf.z : rf.w;
+            const float tb0 = bSwap ? s3.tmin : s2.tmin, tb1 = bSwap ? s2.tmin : s3.tmin;
+            const bool lb0 = bSwap ? l3 : l2, lb1 = bSwap ? l2 : l3;
+            // visiting order q0, q1, q2, q3
+            q0 = gSwap ? rb0 : ra0; q1 = gSwap ? rb1 : ra1; q2 = gSwap ? ra0 : rb0; q3 = gSwap ? ra1 : rb1;
+            u0 = gSwap ? tb0 : ta0; u1 = gSwap ? tb1 : ta1; u2 = gSwap ? ta0 : tb0; u3 = gSwap ? ta1 : tb1;
+            v0 = gSwap ? lb0 : la0; v1 = gSwap ? lb1 : la1; v2 = gSwap ? la0 : lb0; v3 = gSwap ? la1 : lb1;
+        }
+        const uint32_t live = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
+        if (live != 0u) {
+            if (st.sp + live - 1u >= S.cap) {
+                *overflow = 1u;
+                return true;
+            }
+            // push the live slots after the first, farthest in the order first
+            if (v3 && (v0 || v1 || v2)) spush(S, st.sp, q3, u3);
+            if (v2 && (v0 || v1)) spush(S, st.sp, q2, u2);
+            if (v1 && v0) spush(S, st.sp, q1, u1);
+            st.cur = v0 ? q0 : v1 ? q1 : v2 ? q2 : q3;
+            return false;
+        }
+    }
+    if constexpr (ANY) {
+        // nothing is culled before the first hit: the top entry is always taken
+        if (st.sp == 0u) return true;
+        float t;
+        spop(S, st.sp, st.cur, t);
+        return false;
+    } else {
+        while (st.sp > 0u) {
+            float t;
+            spop(S, st.sp, st.cur, t);
+            if (!(t > st.closest)) return false;
+        }
+        return true;
+    }
+}
+
 // Closest hit (ANY = false) or any hit (ANY = true).  Returns the leaf-order triangle index
 // of the hit, or ~0u.  *overflow is set when the stack would exceed its capacity.
 //
@@ -195,132 +335,14 @@ struct TraceCounters {
 template <bool ANY, bool COUNT>
 __device__ uint32_t traverse(const DScene& sc, const TRay& r, const TStack& S, float& tHit, float& uHit, float& vHit,
                              TraceCounters& cnt, uint32_t* overflow) {
-    float closest = r.tMax;
-    uint32_t hit = ~0u;
-    float hu = 0.f, hv = 0.f;
-    if (!sc.hasBVH) return ~0u;
-    float tn;
-    if (!slab(sc.rootMin, sc.rootMax, r, tn)) return ~0u;   // root box from the kernel argument
-    uint32_t cur = sc.rootRef;
-    uint32_t sp = 0;
-    const bool nx = (r.neg & 1u) != 0u, ny = (r.neg & 2u) != 0u, nz = (r.neg & 4u) != 0u;
-    uint32_t triIdx = 0u, triLeft = 0u;   // leaf being tested: next triangle, triangles left
-    while (true) {
-        // One record per iteration: a node, or ONE triangle of the current leaf.  The first three
-        // 16-byte loads are common to both, so lanes at a leaf and lanes at a node issue their
-        // loads together and the wave waits once per iteration.
-        if (triLeft == 0u && refIsLeaf(cur)) {
-            triIdx = refIndex(cur);
-            triLeft = refCount(cur);
-        }
-        const bool isTri = triLeft != 0u;
-        const float4* p = isTri ? reinterpret_cast<const float4*>(sc.tris + triIdx)
-                                : reinterpret_cast<const float4*>(sc.nodes4 + refIndex(cur));
-        const float4 mnx = p[0], mny = p[1], mnz = p[2];
-        float4 mxx, mxy, mxz;
-        uint4 rf;
-        if (!isTri) {
-            mxx = p[3];
-            mxy = p[4];
-            mxz = p[5];
-            rf = reinterpret_cast<const uint4*>(p)[6];
-        }
-        if (isTri) {
-            DTri T;
-            T.v0[0] = mnx.x; T.v0[1] = mnx.y; T.v0[2] = mnx.z;
-            T.e1[0] = mnx.w; T.e1[1] = mny.x; T.e1[2] = mny.y;
-            T.e2[0] = mny.z; T.e2[1] = mny.w; T.e2[2] = mnz.x;
-            float t, u, v;
-            if (COUNT) ++cnt.tris;
-            if (hitTri(T, r, closest, t, u, v)) {
-                closest = t;
-                hit = triIdx;
-                hu = u;
-                hv = v;
-                if (ANY) {
-                    tHit = t;
-                    return hit;
-                }
-            }
-            ++triIdx;
-            if (--triLeft != 0u) continue;
-        } else {
-            if (COUNT) ++cnt.nodes;
-            const float4 ax = nx ? mxx : mnx, bx = nx ? mnx : mxx;
-            const float4 ay = ny ? mxy : mny, by = ny ? mny : mxy;
-            const float4 az = nz ? mxz : mnz, bz = nz ? mnz : mxz;
-            const SlotSlab s0 = slotSlab(ax.x, ay.x, az.x, bx.x, by.x, bz.x, r);
-            const SlotSlab s1 = slotSlab(ax.y, ay.y, az.y, bx.y, by.y, bz.y, r);
-            const SlotSlab s2 = slotSlab(ax.z, ay.z, az.z, bx.z, by.z, bz.z, r);
-            const SlotSlab s3 = slotSlab(ax.w, ay.w, az.w, bx.w, by.w, bz.w, r);
-            uint32_t q0, q1, q2, q3;
-            float u0, u1, u2, u3;
-            bool v0, v1, v2, v3;
-            if constexpr (ANY) {
-                // An any-hit query ends at its first hit and culls nothing before it (a slot that
-                // passes the slab test enters before tMax = closest), and the answer does not
-                // depend on the order, so the live slots are taken in record order.
-                v0 = s0.hit & (rf.x != kRefEmpty);
-                v1 = s1.hit & (rf.y != kRefEmpty);
-                v2 = s2.hit & (rf.z != kRefEmpty);
-                v3 = s3.hit & (rf.w != kRefEmpty);
-                q0 = rf.x; q1 = rf.y; q2 = rf.z; q3 = rf.w;
-                u0 = s0.tmin; u1 = s1.tmin; u2 = s2.tmin; u3 = s3.tmin;
-            } else {
-                const bool l0 = s0.hit & !(s0.tmin > closest) & (rf.x != kRefEmpty);
-                const bool l1 = s1.hit & !(s1.tmin > closest) & (rf.y != kRefEmpty);
-                const bool l2 = s2.hit & !(s2.tmin > closest) & (rf.z != kRefEmpty);
-                const bool l3 = s3.hit & !(s3.tmin > closest) & (rf.w != kRefEmpty);
-                // order within each group, then of the groups (left first only when strictly
-                // nearer); the group entries are computed unconditionally (no branch)
-                const float gA = groupEntry(s0, s1, r.tMin), gB = groupEntry(s2, s3, r.tMin);
-                const bool aSwap = l0 & l1 & !(s0.tmin < s1.tmin);
-                const bool bSwap = l2 & l3 & !(s2.tmin < s3.tmin);
-                const bool gSwap = (l0 | l1) & (l2 | l3) & !(gA < gB);
-                const uint32_t ra0 = aSwap ? rf.y : rf.x, ra1 = aSwap ? rf.x : rf.y;
-                const float ta0 = aSwap ? s1.tmin : s0.tmin, ta1 = aSwap ? s0.tmin : s1.tmin;
-                const bool la0 = aSwap ? l1 : l0, la1 = aSwap ? l0 : l1;
-                const uint32_t rb0 = bSwap ? rf.w : rf.z, rb1 = bSwap ? rf.z : rf.w;
-                const float tb0 = bSwap ? s3.tmin : s2.tmin, tb1 = bSwap ? s2.tmin : s3.tmin;
-                const bool lb0 = bSwap ? l3 : l2, lb1 = bSwap ? l2 : l3;
-                // visiting order q0, q1, q2, q3
-                q0 = gSwap ? rb0 : ra0; q1 = gSwap ? rb1 : ra1; q2 = gSwap ? ra0 : rb0; q3 = gSwap ? ra1 : rb1;
-                u0 = gSwap ? tb0 : ta0; u1 = gSwap ? tb1 : ta1; u2 = gSwap ? ta0 : tb0; u3 = gSwap ? ta1 : tb1;
-                v0 = gSwap ? lb0 : la0; v1 = gSwap ? lb1 : la1; v2 = gSwap ? la0 : lb0; v3 = gSwap ? la1 : lb1;
-            }
-            const uint32_t live = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
-            if (live != 0u) {
-                if (sp + live - 1u >= S.cap) {
-                    *overflow = 1u;
-                    break;
-                }
-                // push the live slots after the first, farthest in the order first
-                if (v3 && (v0 || v1 || v2)) spush(S, sp, q3, u3);
-                if (v2 && (v0 || v1)) spush(S, sp, q2, u2);
-                if (v1 && v0) spush(S, sp, q1, u1);
-                cur = v0 ? q0 : v1 ? q1 : v2 ? q2 : q3;
-                continue;
-            }
-        }
-        if constexpr (ANY) {
-            // nothing is culled before the first hit: the top entry is always taken
-            if (sp == 0u) break;
-            float t;
-            spop(S, sp, cur, t);
-        } else {
-            bool found = false;
-            while (sp > 0u) {
-                float t;
-                spop(S, sp, cur, t);
-                if (!(t > closest)) { found = true; break; }
-            }
-            if (!found) break;
-        }
+    TState st;
+    if (!traverseBegin(sc, r, st)) return ~0u;
+    while (!traverseStep<ANY, COUNT>(sc, r, S, st, cnt, overflow)) {
     }
-    tHit = closest;
-    uHit = hu;
-    vHit = hv;
-    return hit;
+    tHit = st.closest;
+    uHit = st.hu;
+    vHit = st.hv;
+    return st.hit;
 }
 
 // ---------------------------------------------------------------- curves (CurveBrute.cpp)
