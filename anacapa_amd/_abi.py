@@ -126,6 +126,10 @@ SIGNATURES = {
     "ahip_render_tile": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, VP]),
     "ahip_render_batch": (C.c_int, [VP, C.POINTER(View), C.c_uint32]),
+    "ahip_set_aovs": (C.c_int, [VP, C.c_int]),
+    "ahip_download_aovs": (C.c_int, [VP, VP]),
+    "ahip_render_tile_aovs": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, VP, VP, VP]),
     "ahip_trace_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_occluded_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_halton_eval": (C.c_int, [VP, C.c_uint64, VP, VP, VP, VP, C.c_uint32, VP]),

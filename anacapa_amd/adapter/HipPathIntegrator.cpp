@@ -221,16 +221,22 @@ void HipPathIntegrator::renderTile(const SceneView& scene, const TileRequest& ti
                                    ISampler&, TileBuffer& out) {
     std::lock_guard<std::mutex> lk(m_mutex);
     if (!m_prepared || !ensureCamera(scene, W, H)) return;
-    std::vector<float> acc((size_t)tile.width * tile.height * 5);
-    if (ahip_render_tile(m_ctx, tile.x0, tile.y0, tile.width, tile.height, tile.sampleStart, tile.sampleCount,
-                         acc.data()) != AHIP_OK)
+    const size_t n = (size_t)tile.width * tile.height;
+    std::vector<float> acc(n * 5), alb(n * 3), nrm(n * 3);
+    if (ahip_render_tile_aovs(m_ctx, tile.x0, tile.y0, tile.width, tile.height, tile.sampleStart, tile.sampleCount,
+                              acc.data(), alb.data(), nrm.data()) != AHIP_OK)
         return;
     for (uint32_t y = 0; y < tile.height; ++y)
         for (uint32_t x = 0; x < tile.width; ++x) {
-            const float* p = &acc[5 * ((size_t)y * tile.width + x)];
+            const size_t i = (size_t)y * tile.width + x;
+            const float* p = &acc[5 * i];
             float w = p[3] > 0.f ? p[3] : 1.f;
             out.add(x, y, p[0] / w, p[1] / w, p[2] / w, w);
             out.addLumSq(x, y, p[4]);
+            if (tile.sampleCount > 0) {   // the denoising AOVs PathIntegrator::renderTile records (:58-66)
+                out.addAlbedo(x, y, makeSpectrum(alb[3 * i], alb[3 * i + 1], alb[3 * i + 2]));
+                out.addNormal(x, y, Vec3f{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]});
+            }
         }
 }
 

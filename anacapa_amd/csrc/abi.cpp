@@ -92,7 +92,7 @@ struct ahip_context {
     uint32_t maxPaths = 1u << 25;
 
     // persistent accumulation
-    DevBuf dAccum, dAccumLsq;
+    DevBuf dAccum, dAccumLsq, dAccumAov;   // persistent frame sums (AOVs when aovOn)
     uint32_t accW = 0, accH = 0;
 
     // wavefront queues
@@ -110,6 +110,9 @@ struct ahip_context {
     size_t eventsUsed = 0;
     hipEvent_t renderBeg = nullptr, renderEnd = nullptr;
     bool lastRenderValid = false;
+    bool aovOn = false;          // ahip_set_aovs
+    DevBuf qAovA, qAovN;         // per-path first-hit AOVs (allocated when a render records them)
+    uint32_t qAovCap = 0;
     uint32_t lastW = 0, lastH = 0, lastS0 = 0, lastNs = 0;
 
     ~ahip_context() {
@@ -252,6 +255,7 @@ struct ViewJob {
     uint32_t s0, ns;
     float4* acc;
     float* lsq;
+    float* aov = nullptr;        // [w*h*6] AOV sums, or null
 };
 
 // Pack whole sample indices of the jobs, in order, into passes of at most `cap` paths (a job whose
@@ -275,6 +279,7 @@ int planPasses(ahip_context* ctx, const std::vector<ViewJob>& jobs, std::vector<
         v.slots = (uint32_t)slots;
         v.accum = j.acc;
         v.accumLumSq = j.lsq;
+        v.accumAov = j.aov;
         widest = std::max<uint64_t>(widest, slots);
         need += slots * j.ns;
         views.push_back(v);
@@ -368,6 +373,17 @@ int renderJobs(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool graph, 
     HIPCHK(hipMemcpyAsync(ctx->dTables.p, tables.data(), tables.size() * sizeof(PassTable), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));   // the host vectors go out of scope after this call
     WaveQueues q = queues(ctx);
+    bool aov = false;
+    for (const ViewJob& j : jobs) aov = aov || j.aov != nullptr;
+    if (aov) {
+        if (ctx->qAovCap < cap || !ctx->qAovA.p) {
+            HIPCHK(ctx->qAovA.alloc((size_t)cap * sizeof(float4)));
+            HIPCHK(ctx->qAovN.alloc((size_t)cap * sizeof(float4)));
+            ctx->qAovCap = cap;
+        }
+        q.aovA = ctx->qAovA.as<float4>();
+        q.aovN = ctx->qAovN.as<float4>();
+    }
     q.views = ctx->dViews.as<ViewDesc>();
     q.tables = ctx->dTables.as<PassTable>();
     q.passIdx = ctx->qPassIdx.as<uint32_t>();
@@ -619,6 +635,7 @@ int ahip_clear_accum(ahip_context* ctx) {
     if (ctx->dAccum.p) {
         launchFillF32(ctx->dAccum.as<float>(), (size_t)ctx->accW * ctx->accH * 4, 0.f, ctx->stream);
         launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)ctx->accW * ctx->accH, 0.f, ctx->stream);
+        if (ctx->dAccumAov.p) launchFillF32(ctx->dAccumAov.as<float>(), (size_t)ctx->accW * ctx->accH * 6, 0.f, ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     return AHIP_OK;
@@ -653,13 +670,20 @@ int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32
     if (ctx->accW != width || ctx->accH != height || !ctx->dAccum.p) {
         HIPCHK(ctx->dAccum.alloc((size_t)width * height * sizeof(float4)));
         HIPCHK(ctx->dAccumLsq.alloc((size_t)width * height * sizeof(float)));
+        ctx->dAccumAov.release();
         ctx->accW = width;
         ctx->accH = height;
         launchFillF32(ctx->dAccum.as<float>(), (size_t)width * height * 4, 0.f, ctx->stream);
         launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
     }
+    if (ctx->aovOn && !ctx->dAccumAov.p) {
+        // AOV sums start at zero when turned on; the radiance sums already hold earlier samples
+        HIPCHK(ctx->dAccumAov.alloc((size_t)width * height * 6 * sizeof(float)));
+        launchFillF32(ctx->dAccumAov.as<float>(), (size_t)width * height * 6, 0.f, ctx->stream);
+    }
     std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
-                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>()}};
+                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(),
+                                      ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
     st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
     if (st) return st;
     ctx->lastRenderValid = true;
@@ -688,27 +712,81 @@ int ahip_download_accum(ahip_context* ctx, float* out) {
     return downloadRegion(ctx, ctx->dAccum, ctx->dAccumLsq, (size_t)ctx->accW * ctx->accH, out);
 }
 
-int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
-                     uint32_t sample_count, float* out) {
-    ENTER();
+static int renderTileImpl(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
+                          uint32_t sample_count, float* out, float* albedo, float* normal) {
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
     if (!out || w == 0 || h == 0) return fail(ctx, AHIP_E_INVALID_ARG, "bad tile");
+    const bool aov = albedo != nullptr || normal != nullptr;
+    if (aov && (!albedo || !normal)) return fail(ctx, AHIP_E_INVALID_ARG, "albedo and normal both needed");
     w = std::min(w, ctx->cam.width > x0 ? ctx->cam.width - x0 : 0u);
     h = std::min(h, ctx->cam.height > y0 ? ctx->cam.height - y0 : 0u);
     if (w == 0 || h == 0) return fail(ctx, AHIP_E_INVALID_ARG, "tile outside the film");
     int st = ensureQueues(ctx, 64);
     if (st) return st;
-    DevBuf acc, lsq;
-    HIPCHK(acc.alloc((size_t)w * h * sizeof(float4)));
-    HIPCHK(lsq.alloc((size_t)w * h * sizeof(float)));
-    launchFillF32(acc.as<float>(), (size_t)w * h * 4, 0.f, ctx->stream);
-    launchFillF32(lsq.as<float>(), (size_t)w * h, 0.f, ctx->stream);
+    const size_t n = (size_t)w * h;
+    DevBuf acc, lsq, aovBuf;
+    HIPCHK(acc.alloc(n * sizeof(float4)));
+    HIPCHK(lsq.alloc(n * sizeof(float)));
+    launchFillF32(acc.as<float>(), n * 4, 0.f, ctx->stream);
+    launchFillF32(lsq.as<float>(), n, 0.f, ctx->stream);
+    if (aov) {
+        HIPCHK(aovBuf.alloc(n * 6 * sizeof(float)));
+        launchFillF32(aovBuf.as<float>(), n * 6, 0.f, ctx->stream);
+    }
     std::vector<ViewJob> jobs{ViewJob{ctx->cam, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(),
-                                      lsq.as<float>()}};
+                                      lsq.as<float>(), aov ? aovBuf.as<float>() : nullptr}};
     st = renderCall(ctx, jobs, false, nullptr);
     if (st) return st;
-    return downloadRegion(ctx, acc, lsq, (size_t)w * h, out);
+    if ((st = downloadRegion(ctx, acc, lsq, n, out))) return st;
+    if (!aov) return AHIP_OK;
+    // PathIntegrator::renderTile (PathIntegrator.cpp:58-66): mean over the tile's samples; the mean
+    // normal scaled to unit length when longer than 1e-6
+    std::vector<float> sums(n * 6);
+    HIPCHK(hipMemcpy(sums.data(), aovBuf.p, n * 6 * sizeof(float), hipMemcpyDeviceToHost));
+    const float invN = sample_count ? 1.f / (float)sample_count : 0.f;
+    for (size_t i = 0; i < n; ++i) {
+        const float* a = sums.data() + 6 * i;
+        for (int c = 0; c < 3; ++c) albedo[3 * i + c] = a[c] * invN;
+        float nx = a[3] * invN, ny = a[4] * invN, nz = a[5] * invN;
+        float len = std::sqrt(nx * nx + ny * ny + nz * nz);   // Vec3f::length (Types.h)
+        if (len > 1e-6f) {
+            float il = 1.f / len;
+            nx *= il; ny *= il; nz *= il;
+        }
+        normal[3 * i + 0] = nx;
+        normal[3 * i + 1] = ny;
+        normal[3 * i + 2] = nz;
+    }
+    return AHIP_OK;
+}
+
+int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t sample_start,
+                     uint32_t sample_count, float* out) {
+    ENTER();
+    return renderTileImpl(ctx, x0, y0, w, h, sample_start, sample_count, out, nullptr, nullptr);
+}
+
+int ahip_render_tile_aovs(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                          uint32_t sample_start, uint32_t sample_count, float* out, float* albedo, float* normal) {
+    ENTER();
+    if (!albedo || !normal) return fail(ctx, AHIP_E_INVALID_ARG, "null AOV output");
+    return renderTileImpl(ctx, x0, y0, w, h, sample_start, sample_count, out, albedo, normal);
+}
+
+int ahip_set_aovs(ahip_context* ctx, int enable) {
+    ENTER();
+    ctx->aovOn = enable != 0;
+    if (!ctx->aovOn) ctx->dAccumAov.release();
+    return AHIP_OK;
+}
+
+int ahip_download_aovs(ahip_context* ctx, float* out) {
+    ENTER();
+    if (!out) return fail(ctx, AHIP_E_INVALID_ARG, "null output");
+    if (!ctx->dAccumAov.p) return fail(ctx, AHIP_E_NOT_PREPARED, "no AOVs recorded (ahip_set_aovs, then render)");
+    HIPCHK(hipMemcpy(out, ctx->dAccumAov.p, (size_t)ctx->accW * ctx->accH * 6 * sizeof(float), hipMemcpyDeviceToHost));
+    return AHIP_OK;
 }
 
 int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views) {

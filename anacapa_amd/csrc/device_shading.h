@@ -200,6 +200,22 @@ __device__ __forceinline__ MatIn matInputs(const DScene& sc, const DMat& m, V2 u
     return mi;
 }
 
+// IMaterial::reflectance, the denoising albedo: Lambertian.h:76-78 (also EmissiveMaterial, which
+// inherits it), StandardSurface.h:189-195, MarschnerHair.h:224-232.
+__device__ __forceinline__ V3 matReflectance(const DMat& m, const Frame& fr, const MatIn& mi) {
+#pragma clang fp contract(off)
+    if (m.type == AHIP_MAT_STANDARD) {
+        float diff = (1.f - m.metal) * (1.f - m.spec * 0.5f);
+        float k = diff + m.metal;
+        return V3{mi.base.x * m.base * k, mi.base.y * m.base * k, mi.base.z * m.base * k};
+    }
+    if (m.type == AHIP_MAT_HAIR) {
+        V3 sa = hairSigmaA(m, fr.color);
+        return V3{expf(-sa.x), expf(-sa.y), expf(-sa.z)};
+    }
+    return mi.base;
+}
+
 // IMaterial::sample
 __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V2 u,
                              float uc) {

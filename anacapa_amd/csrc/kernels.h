@@ -18,6 +18,7 @@ struct ViewDesc {
     uint32_t slots;              // paths per sample index: tilesX * tilesY * 64
     float4* accum;               // region [w*h]: r, g, b sums and weight
     float* accumLumSq;           // region [w*h]
+    float* accumAov;             // region [w*h*6]: first-hit reflectance and normal sums, or null
 };
 
 // A pass packs whole sample indices of one or more views into one path range.
@@ -52,6 +53,8 @@ struct WaveQueues {
     float4* md;       // direction | tMax
     float4* mc;       // weight; times emitted radiance
     float4* L;        // per path radiance
+    float4* aovA;     // per path first-hit reflectance (null: no view of the pass records AOVs)
+    float4* aovN;     // per path first-hit shading normal
     uint2* pix;       // per path: px | py << 16, sample index
     uint32_t* counters;  // [(maxDepth + 2) * 4]
     uint32_t* ovfRef;    // traversal stack overflow (entries past the LDS part)

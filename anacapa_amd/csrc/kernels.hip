@@ -153,6 +153,10 @@ __global__ __launch_bounds__(256) void k_generate(RenderConsts rc, WaveQueues q)
             q.ro[0][p0] = f4(o, __uint_as_float(p0));
             q.rd[0][p0] = f4(d, __uint_as_float(valid ? 5u : kDeadPath));   // dims 0..4: camera sample
             q.beta[0][p0] = make_float4(1.f, 1.f, 1.f, 0.f);
+            if (q.aovA) {   // a camera ray that hits nothing reports zero AOVs
+                q.aovA[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
+                q.aovN[p0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
     }
     if (blockIdx.x == 0u && threadIdx.x == 0u) *ctr(q, 0, 0) = total;
@@ -275,6 +279,10 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     const DMat& m = sc.mats[mi];
                     const MatIn mIn = matInputs(sc, m, uv, commit);
                     V3 wo = -dir;
+                    if (bounce == 0u && q.aovA) {   // first-hit AOVs (PathIntegrator.cpp:121-126)
+                        q.aovA[p0] = f4(matReflectance(m, fr, mIn), 0.f);
+                        q.aovN[p0] = f4(n, 0.f);
+                    }
                     if (m.hasLe && (bounce == 0u || specular)) Ladd = beta * ld3(m.Le);   // :128-132
                     if (!m.isDelta && sc.numLights > 0u) {                                // :134-144
                         float ul = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
@@ -489,6 +497,18 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveQueues q) {
         float4 a = V.accum[o];
         V.accum[o] = make_float4(a.x + acc.x, a.y + acc.y, a.z + acc.z, a.w + (float)sg.samples);
         V.accumLumSq[o] += lsq;
+        if (V.accumAov) {
+            // first-hit reflectance and normal of every sample (PathIntegrator.cpp:45-47), in
+            // sample order
+            V3 sa{0.f, 0.f, 0.f}, sn{0.f, 0.f, 0.f};
+            for (uint32_t j = 0; j < sg.samples; ++j) {
+                sa = sa + v3(q.aovA[sg.pathBase + j * V.slots + slot]);
+                sn = sn + v3(q.aovN[sg.pathBase + j * V.slots + slot]);
+            }
+            float* A = V.accumAov + 6u * o;
+            A[0] += sa.x; A[1] += sa.y; A[2] += sa.z;
+            A[3] += sn.x; A[4] += sn.y; A[5] += sn.z;
+        }
     }
 }
 

@@ -129,6 +129,29 @@ class HipPathIntegrator:
         self._check(self.lib.ahip_render_tile(self._ctx, x0, y0, w, h, sample_start, sample_count, A.ptr(out)))
         return out
 
+    def render_tile_aovs(self, x0, y0, w, h, sample_start, sample_count):
+        """render_tile plus the tile's denoising AOVs as PathIntegrator::renderTile forms them:
+        (accum [h, w, 5], albedo [h, w, 3] mean first-hit reflectance, normal [h, w, 3] mean
+        first-hit normal scaled to unit length)."""
+        w = min(w, self.width - x0)
+        h = min(h, self.height - y0)
+        out = np.zeros((h, w, 5), np.float32)
+        alb = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        self._check(self.lib.ahip_render_tile_aovs(self._ctx, x0, y0, w, h, sample_start, sample_count,
+                                                   A.ptr(out), A.ptr(alb), A.ptr(nrm)))
+        return out, alb, nrm
+
+    def set_aovs(self, enable: bool = True):
+        """Accumulate first-hit reflectance and normal sums in render_frame (cleared by clear_accum)."""
+        self._check(self.lib.ahip_set_aovs(self._ctx, 1 if enable else 0))
+
+    def download_aovs(self) -> np.ndarray:
+        """[h, w, 6]: sums of first-hit reflectance rgb and normal xyz over the accumulated samples."""
+        out = np.zeros((self.height, self.width, 6), np.float32)
+        self._check(self.lib.ahip_download_aovs(self._ctx, A.ptr(out)))
+        return out
+
     def render_batch(self, cameras, sample_starts, sample_counts, max_depth=0):
         """Render many views of the prepared scene in one batched call (shared passes, one captured
         HIP graph replayed per pass); returns one [h, w, 5] accumulation array per view."""

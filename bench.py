@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+DEFAULT_MAX_PATHS = 1 << 25   # ahip_set_launch default: one 1080p x 16 spp frame per pass
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -220,7 +223,8 @@ def batched_form(be, sc, args, log):
     log(f"[bench] batched: {len(cams)} views, {samples / 1e6:.1f} Msamples in {t_batch * 1e3:.1f} ms, "
         f"{st['passes']} passes ({st['graph_passes']} graph replays)")
     plain = autotune.load(be.name(), autotune.plain_key(args.width, args.height, args.spp, args.depth))
-    be.set_launch(max_paths=plain["max_paths"] if plain else 1 << 22, tile=plain["tile"] if plain else 8)
+    # sequential plain renders at the plain workload's tuned launch, else the library default
+    be.set_launch(max_paths=plain["max_paths"] if plain else DEFAULT_MAX_PATHS, tile=plain["tile"] if plain else 8)
     t_seq, worst = 0.0, 0.0
     for cam, n, out in zip(cams, counts, outs):
         t0 = time.perf_counter()
@@ -292,6 +296,7 @@ def other_configs(be, log):
     warm-up and two timed frames each (HIP events around the frame)."""
     from anacapa_amd import scenes
     out = {}
+    be.set_launch(max_paths=DEFAULT_MAX_PATHS, tile=8)   # the library default, whatever ran before
     for name, sc in (("cornell", scenes.cornell()), ("hair", scenes.hair_sphere())):
         be.prepare(sc)
         be.clear_accum()

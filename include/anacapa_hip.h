@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define AHIP_ABI_VERSION 2
+#define AHIP_ABI_VERSION 3
 
 /* status codes */
 #define AHIP_OK              0
@@ -211,6 +211,27 @@ int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, ui
                      uint32_t sample_start, uint32_t sample_count, float* out);
 /* Batched persistent form: many views share one BVH; one graph-captured bounce loop. */
 int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views);
+
+/* --- denoising AOVs: first-hit reflectance (IMaterial::reflectance) and shading normal (si.n),
+ *     captured where PathIntegrator::Li captures them (PathIntegrator.cpp:121-126) and recorded per
+ *     pixel like PathIntegrator::renderTile records them in TileBuffer (PathIntegrator.cpp:40-66,
+ *     Film.h:108-118).  A camera ray that misses, or hits a mesh without material, reports zeros.
+ *     The CUDA backend computes no AOVs; these calls extend the GPU path to the CPU path's
+ *     outputs. --- */
+/* Turn AOV accumulation by ahip_render_frame on (1) or off (0); the sums live beside the radiance
+ * accumulation buffer and are cleared by ahip_clear_accum. */
+int ahip_set_aovs(ahip_context* ctx, int enable);
+/* [width*height*6]: per pixel, sums over the accumulated samples of the first-hit reflectance
+ * (r, g, b) and of the first-hit normal (x, y, z); the sample count is ahip_download_accum's
+ * weight. */
+int ahip_download_aovs(ahip_context* ctx, float* out);
+/* ahip_render_tile plus the tile's AOVs as PathIntegrator::renderTile forms them
+ * (PathIntegrator.cpp:58-66): albedo[w*h*3] = the sum of the samples' reflectance times
+ * 1/sample_count; normal[w*h*3] = the sum of the samples' normals times 1/sample_count, scaled to
+ * unit length when that length exceeds 1e-6. */
+int ahip_render_tile_aovs(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                          uint32_t sample_start, uint32_t sample_count, float* out, float* albedo,
+                          float* normal);
 
 /* --- acceleration boundary (IAccelerationStructure::traceBatch / occludedBatch,
  *     include/anacapa/accel/IAccelerationStructure.h:75-87) ---

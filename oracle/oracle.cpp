@@ -204,6 +204,10 @@ struct Material {
         return r < 0.001f && (mm > 0.999f || s > 0.f);
     }
     bool flagsSpecular() const { return m.type == AHIP_MAT_STANDARD && m.roughness < 0.001f; }
+    // IMaterial::reflectance, the first-hit denoising albedo: LambertianMaterial::m_albedo
+    // (Lambertian.h:76-78, inherited by EmissiveMaterial), StandardSurface.h:189-195,
+    // MarschnerHair.h:224-232 (exp(-effectiveSigmaA)).
+    V3 reflectance(const Ctx& c) const;
     V3 Le() const {
         if (m.type == AHIP_MAT_EMISSIVE) return {m.emission[0], m.emission[1], m.emission[2]};  // Lambertian.h:93-95
         if (m.type == AHIP_MAT_STANDARD && m.emission_weight > 0.f)                         // StandardSurface.h:183-187
@@ -454,6 +458,18 @@ struct Material {
 };
 
 // ---------------------------------------------------------------- lights
+V3 Material::reflectance(const Ctx& c) const {
+    if (m.type == AHIP_MAT_STANDARD) {
+        float diff = (1.f - m.metalness) * (1.f - m.specular * 0.5f);
+        return baseOf(c) * m.base * (diff + m.metalness);
+    }
+    if (m.type == AHIP_MAT_HAIR) {
+        V3 sa = hairSigmaA(c);
+        return {std::exp(-sa.x), std::exp(-sa.y), std::exp(-sa.z)};
+    }
+    return baseOf(c);
+}
+
 struct LightSample { V3 Li, wi; float pdf = 0.f, dist = 0.f; };
 
 struct Dist1D {                                                                       // DomeLight.h:19-55
@@ -845,7 +861,9 @@ struct Scene {
     }
 
     // PathIntegrator::Li (PathIntegrator.cpp:71-164)
-    V3 Li(Ray r, Halton& smp, uint32_t* rays = nullptr) const {
+    // outAlb / outN (optional): first-hit reflectance and shading normal, left untouched when the
+    // camera ray misses or hits a mesh without material (PathIntegrator.cpp:121-126).
+    V3 Li(Ray r, Halton& smp, uint32_t* rays = nullptr, V3* outAlb = nullptr, V3* outN = nullptr) const {
         V3 L, beta{1.f, 1.f, 1.f};
         bool specular = false;
         for (uint32_t bounce = 0; bounce <= maxDepth; ++bounce) {
@@ -859,6 +877,10 @@ struct Scene {
             if (!mat) break;
             V3 wo = -r.d;
             Ctx ctx(si, r.d);
+            if (bounce == 0 && outAlb) {
+                *outAlb = mat->reflectance(ctx);
+                *outN = si.n;
+            }
             V3 Le = mat->Le();
             if (!isBlack(Le) && (bounce == 0 || specular)) L = L + beta * Le;
             if (!mat->isDelta() && !lights.empty()) {
@@ -1088,13 +1110,17 @@ struct Scene {
     }
 
     // PathIntegrator::renderTile (PathIntegrator.cpp:8-69): out[w*h*5] = r,g,b (pre-weighted), w, lumSq
-    void renderTile(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float* out) const {
+    // With albedo / normal [w*h*3] set, also the tile's AOVs (PathIntegrator.cpp:28-31, 40-47,
+    // 58-66): mean first-hit reflectance, mean first-hit normal scaled to unit length.
+    void renderTile(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float* out,
+                    float* albedo = nullptr, float* normal = nullptr) const {
         Halton smp;
         smp.seed = seed;
         for (uint32_t ty = 0; ty < h; ++ty)
             for (uint32_t tx = 0; tx < w; ++tx) {
                 uint32_t px = x0 + tx, py = y0 + ty;
-                V3 acc;
+                V3 acc, accA, accN;
+                uint32_t aovCount = 0;
                 float lsq = 0.f;
                 for (uint32_t s = 0; s < ns; ++s) {
                     smp.start(px, py, s0 + s);
@@ -1102,12 +1128,26 @@ struct Scene {
                     smp.get2D();   // lens
                     smp.get1D();   // time
                     Ray ray = generateRay(cam, px, py, jit.x, jit.y);
-                    V3 L = Li(ray, smp);
+                    V3 a, nrm;
+                    V3 L = Li(ray, smp, nullptr, albedo ? &a : nullptr, &nrm);
                     if (isFinite(L)) {
                         acc = acc + L;
                         float lum = luminance(L);
                         lsq += lum * lum;
                     }
+                    accA = accA + a;
+                    accN = accN + nrm;
+                    ++aovCount;
+                }
+                if (albedo && aovCount > 0) {
+                    float invN = 1.f / (float)aovCount;
+                    V3 ma = accA * invN, mn = accN * invN;
+                    float len = length(mn);
+                    if (len > 1e-6f) mn = mn * (1.f / len);
+                    float* A = albedo + 3 * (ty * w + tx);
+                    float* N = normal + 3 * (ty * w + tx);
+                    A[0] = ma.x; A[1] = ma.y; A[2] = ma.z;
+                    N[0] = mn.x; N[1] = mn.y; N[2] = mn.z;
                 }
                 float inv = 1.f / (float)ns, wt = (float)ns;
                 float* o = out + 5 * (ty * w + tx);
@@ -1341,6 +1381,12 @@ int orc_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
             out[i] = s->traceSI(r, si) ? 1 : 0;
         }
     }
+    return 0;
+}
+
+int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
+                         float* out, float* albedo, float* normal) {
+    static_cast<Scene*>(h)->renderTile(x0, y0, w, hh, s0, ns, out, albedo, normal);
     return 0;
 }
 

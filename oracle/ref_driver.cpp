@@ -261,6 +261,33 @@ int ref_render_tile(void* h, const ahip_camera* cam, uint32_t x0, uint32_t y0, u
     return 0;
 }
 
+// The same with the tile's denoising AOVs as TileBuffer holds them after one renderTile call
+// (PathIntegrator.cpp:58-66, Film.h:108-118): albedo / normal [w*h*3] = the recorded sums (one
+// record per pixel per call).
+int ref_render_tile_aovs(void* h, const ahip_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh,
+                         uint32_t sampleStart, uint32_t sampleCount, uint32_t maxDepth, uint64_t seed, float* out,
+                         float* albedo, float* normal) {
+    auto* s = static_cast<RefScene*>(h);
+    s->view.camera = toCamera(cam);
+    PathIntegrator integ(maxDepth);
+    integ.prepare(s->view);
+    HaltonSampler base(sampleCount, seed);
+    auto sampler = base.clone();
+    TileBuffer tb(x0, y0, w, hh);
+    TileRequest tile{x0, y0, w, hh, sampleStart, sampleCount};
+    integ.renderTile(s->view, tile, cam->width, cam->height, *sampler, tb);
+    writeTile(tb, out);
+    for (uint32_t i = 0; i < w * hh; ++i) {
+        albedo[3 * i + 0] = tb.albedo[i].r;
+        albedo[3 * i + 1] = tb.albedo[i].g;
+        albedo[3 * i + 2] = tb.albedo[i].b;
+        normal[3 * i + 0] = tb.normals[i].r;
+        normal[3 * i + 1] = tb.normals[i].g;
+        normal[3 * i + 2] = tb.normals[i].b;
+    }
+    return 0;
+}
+
 // RenderSession::render non-adaptive CPU branch (RenderSession.cpp:465-547): 64x64 tiles
 // (scheduleTiles, RenderSession.cpp:380-397), ThreadPool(nthreads) (0 = hardware
 // concurrency), HaltonSampler(spp, seed) cloned per tile, PathIntegrator(maxDepth) ->

@@ -60,6 +60,12 @@ def main():
     out["cornell_occ"] = rs.occluded(rays)
     out["cornell_tile_200_220_48x40_s0_n8"] = rs.render_tile(200, 220, 48, 40, 0, 8)
     out["cornell_tile_0_0_32x32_s3_n5_d3"] = rs.render_tile(0, 0, 32, 32, 3, 5, max_depth=3)
+    # denoising AOVs of the same tiles (TileBuffer albedo / normals after one renderTile)
+    for key, args, kw in (("cornell_tile_200_220_48x40_s0_n8", (200, 220, 48, 40, 0, 8), {}),
+                          ("cornell_tile_0_0_32x32_s3_n5_d3", (0, 0, 32, 32, 3, 5), {"max_depth": 3})):
+        acc, alb, nrm = rs.render_tile_aovs(*args, **kw)
+        assert np.array_equal(acc, out[key])
+        out[key + "_albedo"], out[key + "_normal"] = alb, nrm
     # triangle soup: topology and hits under heavy overlap
     soup = scenes.triangle_soup()
     out["soup_pos_sum"] = np.float64(soup.positions.astype(np.float64).sum())

@@ -48,6 +48,7 @@ class Oracle:
         L.orc_trace.argtypes = [VP, VP, C.c_uint32, VP, VP, VP]
         L.orc_occluded.argtypes = [VP, VP, C.c_uint32, VP]
         L.orc_render_tile.argtypes = [VP] + [C.c_uint32] * 6 + [VP]
+        L.orc_render_tile_aovs.argtypes = [VP] + [C.c_uint32] * 6 + [VP, VP, VP]
         L.orc_render_samples.argtypes = [VP, C.c_uint32, VP, VP, VP, VP, VP]
         L.orc_render_frame.argtypes = [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP]
         L.orc_material_probe.argtypes = [C.POINTER(A.Material), C.c_uint32, VP, VP]
@@ -174,6 +175,13 @@ class OracleScene:
         self.orc.lib.orc_render_tile(self.h, x0, y0, w, h, s0, ns, P(out))
         return out
 
+    def render_tile_aovs(self, x0, y0, w, h, s0, ns):
+        out = np.zeros((h, w, 5), np.float32)
+        alb = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        self.orc.lib.orc_render_tile_aovs(self.h, x0, y0, w, h, s0, ns, P(out), P(alb), P(nrm))
+        return out, alb, nrm
+
     def set_eval(self, tri_albedo=None, texture=0, texels=None):
         """Evaluation parameters of the detached replay estimator (None = sampling values)."""
         self._eval_keep = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)
@@ -244,6 +252,7 @@ class Reference:
         L.ref_trace.argtypes = [VP, VP, C.c_uint32, VP]
         L.ref_occluded.argtypes = [VP, VP, C.c_uint32, VP]
         L.ref_render_tile.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP]
+        L.ref_render_tile_aovs.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP, VP, VP]
         L.ref_render_frame.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
                                        C.c_uint32, C.c_uint32, VP, VP, VP]
         L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
@@ -342,6 +351,16 @@ class RefScene:
                                      sc.max_depth if max_depth is None else max_depth,
                                      sc.seed if seed is None else seed, P(out))
         return out
+
+    def render_tile_aovs(self, x0, y0, w, h, s0, ns, max_depth=None, seed=None, cam=None):
+        out = np.zeros((h, w, 5), np.float32)
+        alb = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        sc = self.sc
+        self.ref.lib.ref_render_tile_aovs(self.h, C.byref(cam or self.cam), x0, y0, w, h, s0, ns,
+                                          sc.max_depth if max_depth is None else max_depth,
+                                          sc.seed if seed is None else seed, P(out), P(alb), P(nrm))
+        return out, alb, nrm
 
     def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None):
         cam = cam or self.cam

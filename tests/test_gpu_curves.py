@@ -51,3 +51,16 @@ def test_hair_radiance_vs_oracle(gpu, oracle):
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(r.mean(), 1e-6))
     assert (err <= 1e-3).mean() >= 0.99, (err > 1e-3).sum()
     assert abs(g.mean() / r.mean() - 1) < 1e-3
+
+
+def test_hair_aovs_vs_oracle(gpu, oracle):
+    """Hair AOVs: reflectance exp(-effectiveSigmaA) of the strand colour (MarschnerHair.h:224-232)
+    and the curve normal, against the oracle (expf differs from std::exp by an ulp)."""
+    sc = scenes.small_hair(width=64, height=64, spp=2)
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    _, alb, nrm = gpu.render_tile_aovs(0, 0, 64, 64, 0, 2)
+    _, oalb, onrm = os_.render_tile_aovs(0, 0, 64, 64, 0, 2)
+    assert (alb.sum(-1) > 0).mean() > 0.2
+    np.testing.assert_allclose(alb, oalb, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(nrm, onrm, rtol=0, atol=1e-5)

@@ -329,6 +329,35 @@ def test_cornell_radiance_matches_reference_vectors(oracle, golden):
     np.testing.assert_allclose(got, golden["cornell_tile_0_0_32x32_s3_n5_d3"], rtol=1e-4, atol=1e-5)
 
 
+def test_cornell_aovs_match_reference_vectors(oracle, golden):
+    """First-hit reflectance and normal AOVs (PathIntegrator.cpp:40-66, 121-126) against the
+    reference build's TileBuffer: albedo bit-exact (Lambert / Emissive copies, same float sums),
+    normals to 1 ulp (the reference build contracts the length's sum of squares)."""
+    sc = scenes.cornell()
+    os_ = oracle.scene(sc)
+    for key, args, depth in (("cornell_tile_200_220_48x40_s0_n8", (200, 220, 48, 40, 0, 8), 8),
+                             ("cornell_tile_0_0_32x32_s3_n5_d3", (0, 0, 32, 32, 3, 5), 3)):
+        os_.set_render(max_depth=depth)
+        _, alb, nrm = os_.render_tile_aovs(*args)
+        assert np.array_equal(alb, golden[key + "_albedo"])
+        np.testing.assert_allclose(nrm, golden[key + "_normal"], rtol=0, atol=2.5e-7)
+        # missed camera rays report zeros; every other pixel has a unit normal
+        hitany = np.abs(golden[key + "_normal"]).sum(-1) > 0
+        np.testing.assert_allclose(np.linalg.norm(nrm[hitany], axis=-1), 1.0, atol=1e-6)
+
+
+def test_aovs_zero_where_camera_rays_miss(oracle):
+    """A pixel whose camera rays all escape reports zero albedo and normal (Li never reaches
+    the first-hit capture); the radiance there is the dome's."""
+    sc = scenes.small_ggx()
+    os_ = oracle.scene(sc)
+    acc, alb, nrm = os_.render_tile_aovs(0, 0, sc.width, sc.height, 0, 2)
+    miss = (alb.sum(-1) == 0) & (np.abs(nrm).sum(-1) == 0)
+    assert miss.any() and (~miss).any()
+    assert (acc[miss][:, :3].sum(-1) > 0).all()
+    np.testing.assert_allclose(np.linalg.norm(nrm[~miss], axis=-1), 1.0, atol=1e-5)
+
+
 def test_cornell_radiance_matches_live_reference(oracle, reference):
     sc = scenes.cornell()
     os_, rs = oracle.scene(sc), reference.scene(sc)
