@@ -387,77 +387,83 @@ __device__ __forceinline__ V3 midNC(V3 a, V3 b) {
 // Returns true when a strictly nearer hit on this segment was found (best, out updated).
 __device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang) {
 #pragma clang fp contract(off)
-    SubFrame st[8];
+    // the frame being tested stays in registers; only pending right halves go to the private
+    // stack (at most one per level), in the order the recursion would visit them
+    SubFrame st[7];
     int sp = 0;
-    SubFrame f0;
-    f0.P[0] = V3{S.P[0], S.P[1], S.P[2]};
-    f0.P[1] = V3{S.P[3], S.P[4], S.P[5]};
-    f0.P[2] = V3{S.P[6], S.P[7], S.P[8]};
-    f0.P[3] = V3{S.P[9], S.P[10], S.P[11]};
-    f0.hw0 = S.hw0;
-    f0.hw3 = S.hw3;
-    f0.tOff = S.tOff;
-    f0.tScale = S.tScale;
-    f0.depth = 6;
-    st[sp++] = f0;
+    SubFrame f;
+    f.P[0] = V3{S.P[0], S.P[1], S.P[2]};
+    f.P[1] = V3{S.P[3], S.P[4], S.P[5]};
+    f.P[2] = V3{S.P[6], S.P[7], S.P[8]};
+    f.P[3] = V3{S.P[9], S.P[10], S.P[11]};
+    f.hw0 = S.hw0;
+    f.hw3 = S.hw3;
+    f.tOff = S.tOff;
+    f.tScale = S.tScale;
+    f.depth = 6;
     bool hit = false;
-    while (sp > 0) {
-        SubFrame f = st[--sp];
-        V3 c0 = projNC(f.P[0], r.o, fx, fy, r.d), c1 = projNC(f.P[1], r.o, fx, fy, r.d);
-        V3 c2 = projNC(f.P[2], r.o, fx, fy, r.d), c3 = projNC(f.P[3], r.o, fx, fy, r.d);
-        float mh = fmaxf(f.hw0, f.hw3);
-        float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z), zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
-        if (zMin - mh > best) continue;
-        if (zMax + mh < r.tMin) continue;
-        float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
-        float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
-        float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
-        float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
-        if (cx * cx + cy * cy > mh * mh) continue;
-        if (f.depth == 0) {
-            float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
-            float l2 = dx * dx + dy * dy;
-            float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
-            float px = ax + dx * tl, py = ay + dy * tl;
-            float d2 = px * px + py * py;
-            float hw = f.hw0 * (1.f - tl) + f.hw3 * tl;
-            if (d2 > hw * hw) continue;
-            float tr = c0.z + (c3.z - c0.z) * tl;
-            if (tr < r.tMin || tr >= best) continue;
-            // bezierTangent (CurveBrute.cpp:59-64)
-            float mt = 1.f - tl;
-            float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
-            V3 d10{f.P[1].x - f.P[0].x, f.P[1].y - f.P[0].y, f.P[1].z - f.P[0].z};
-            V3 d21{f.P[2].x - f.P[1].x, f.P[2].y - f.P[1].y, f.P[2].z - f.P[1].z};
-            V3 d32{f.P[3].x - f.P[2].x, f.P[3].y - f.P[2].y, f.P[3].z - f.P[2].z};
-            V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
-                  d10.z * a0 + d21.z * a1 + d32.z * a2};
-            if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
-                tg = V3{f.P[3].x - f.P[0].x, f.P[3].y - f.P[0].y, f.P[3].z - f.P[0].z};
-            float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
-            if (l > 1e-8f) {
-                float il = 1.f / l;
-                tg = V3{tg.x * il, tg.y * il, tg.z * il};
-            } else {
-                tg = V3{0.f, 0.f, 1.f};
+    while (true) {
+        {   // one frame; every exit to `next` pops the next pending half
+            V3 c0 = projNC(f.P[0], r.o, fx, fy, r.d), c1 = projNC(f.P[1], r.o, fx, fy, r.d);
+            V3 c2 = projNC(f.P[2], r.o, fx, fy, r.d), c3 = projNC(f.P[3], r.o, fx, fy, r.d);
+            float mh = fmaxf(f.hw0, f.hw3);
+            float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z), zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
+            if (zMin - mh > best) goto next;
+            if (zMax + mh < r.tMin) goto next;
+            float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
+            float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
+            float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
+            float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
+            if (cx * cx + cy * cy > mh * mh) goto next;
+            if (f.depth == 0) {
+                float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
+                float l2 = dx * dx + dy * dy;
+                float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
+                float px = ax + dx * tl, py = ay + dy * tl;
+                float d2 = px * px + py * py;
+                float hw = f.hw0 * (1.f - tl) + f.hw3 * tl;
+                if (d2 > hw * hw) goto next;
+                float tr = c0.z + (c3.z - c0.z) * tl;
+                if (tr < r.tMin || tr >= best) goto next;
+                // bezierTangent (CurveBrute.cpp:59-64)
+                float mt = 1.f - tl;
+                float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
+                V3 d10{f.P[1].x - f.P[0].x, f.P[1].y - f.P[0].y, f.P[1].z - f.P[0].z};
+                V3 d21{f.P[2].x - f.P[1].x, f.P[2].y - f.P[1].y, f.P[2].z - f.P[1].z};
+                V3 d32{f.P[3].x - f.P[2].x, f.P[3].y - f.P[2].y, f.P[3].z - f.P[2].z};
+                V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
+                      d10.z * a0 + d21.z * a1 + d32.z * a2};
+                if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
+                    tg = V3{f.P[3].x - f.P[0].x, f.P[3].y - f.P[0].y, f.P[3].z - f.P[0].z};
+                float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
+                if (l > 1e-8f) {
+                    float il = 1.f / l;
+                    tg = V3{tg.x * il, tg.y * il, tg.z * il};
+                } else {
+                    tg = V3{0.f, 0.f, 1.f};
+                }
+                best = tr;
+                tSeg = f.tOff + tl * f.tScale;
+                tang = tg;
+                hit = true;
+                goto next;
             }
-            best = tr;
-            tSeg = f.tOff + tl * f.tScale;
-            tang = tg;
-            hit = true;
+            V3 q01 = midNC(f.P[0], f.P[1]), q12 = midNC(f.P[1], f.P[2]), q23 = midNC(f.P[2], f.P[3]);
+            V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
+            float hm = (f.hw0 + f.hw3) * 0.5f, ts = f.tScale * 0.5f;
+            SubFrame R;
+            R.P[0] = m; R.P[1] = r12; R.P[2] = q23; R.P[3] = f.P[3];
+            R.hw0 = hm; R.hw3 = f.hw3; R.tOff = f.tOff + ts; R.tScale = ts; R.depth = f.depth - 1;
+            SubFrame L;
+            L.P[0] = f.P[0]; L.P[1] = q01; L.P[2] = r01; L.P[3] = m;
+            L.hw0 = f.hw0; L.hw3 = hm; L.tOff = f.tOff; L.tScale = ts; L.depth = f.depth - 1;
+            st[sp++] = R;
+            f = L;
             continue;
         }
-        V3 q01 = midNC(f.P[0], f.P[1]), q12 = midNC(f.P[1], f.P[2]), q23 = midNC(f.P[2], f.P[3]);
-        V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
-        float hm = (f.hw0 + f.hw3) * 0.5f, ts = f.tScale * 0.5f;
-        SubFrame R;
-        R.P[0] = m; R.P[1] = r12; R.P[2] = q23; R.P[3] = f.P[3];
-        R.hw0 = hm; R.hw3 = f.hw3; R.tOff = f.tOff + ts; R.tScale = ts; R.depth = f.depth - 1;
-        SubFrame L;
-        L.P[0] = f.P[0]; L.P[1] = q01; L.P[2] = r01; L.P[3] = m;
-        L.hw0 = f.hw0; L.hw3 = hm; L.tOff = f.tOff; L.tScale = ts; L.depth = f.depth - 1;
-        st[sp++] = R;
-        st[sp++] = L;
+    next:
+        if (sp == 0) break;
+        f = st[--sp];
     }
     return hit;
 }
