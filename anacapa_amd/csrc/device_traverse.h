@@ -469,17 +469,18 @@ __device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& be
 }
 
 // CurveBrute::aabbHit (CurveBrute.cpp:478-498)
-__device__ __forceinline__ bool curveBox(const CNode& n, const TRay& r, float maxT) {
+// invD = 1 / r.d per axis, computed once per ray (the same quotient the box test divides out).
+__device__ __forceinline__ bool curveBox(const CNode& n, const TRay& r, const V3& invD, float maxT) {
 #pragma clang fp contract(off)
     float tmin = r.tMin, tmax = maxT;
-    const float o[3] = {r.o.x, r.o.y, r.o.z}, dd[3] = {r.d.x, r.d.y, r.d.z};
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, dd[3] = {r.d.x, r.d.y, r.d.z}, id[3] = {invD.x, invD.y, invD.z};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if (fabsf(dd[i]) < 1e-9f) {
             if (o[i] < n.bmin[i] || o[i] > n.bmax[i]) return false;
             continue;
         }
-        float inv = 1.f / dd[i];
+        float inv = id[i];
         float t0 = (n.bmin[i] - o[i]) * inv, t1 = (n.bmax[i] - o[i]) * inv;
         if (inv < 0.f) { float s = t0; t0 = t1; t1 = s; }
         tmin = stdmax(tmin, t0);
@@ -508,6 +509,7 @@ __device__ bool traverseCurves(const DScene& sc, const TRay& r, const TStack& S,
     if (!sc.hasCurves) return false;
     V3 fx, fy;
     onbNC(r.d, fx, fy);
+    const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
     bool hit = false;
     uint32_t sp = 0;
     spush(S, sp, 0u, 0.f);
@@ -517,7 +519,7 @@ __device__ bool traverseCurves(const DScene& sc, const TRay& r, const TStack& S,
         spop(S, sp, ni, unused);
         CNode n = loadCNode(sc.cnodes, ni);
         if (COUNT) ++cnt.nodes;
-        if (!curveBox(n, r, best)) continue;
+        if (!curveBox(n, r, invD, best)) continue;
         if (n.rc & 0x80000000u) {
             uint32_t cntSeg = n.rc & 0x7fffffffu;
             for (uint32_t k = 0; k < cntSeg; ++k) {
