@@ -70,7 +70,7 @@ struct ahip_context {
     // scene
     bool prepared = false;
     BVHOutput bvh;
-    DevBuf dNodes, dTris, dShade, dMats, dMeshMat, dLights, dTexDesc, dDomePx, dDomeMF, dDomeMC, dDomeCF, dDomeCC,
+    DevBuf dNodes, dTris, dShade, dMats, dMeshMat, dLights, dTexDesc, dDomePx, dDomeMF, dDomeMC, dDomeCF, dDomeCC, dDomeMG, dDomeCG,
         dDomeCI;
     std::vector<DevBuf*> texBufs;
     CurveOutput curves;
@@ -569,6 +569,8 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         if ((st = upload(ctx, ctx->dDomeCF, D.condFunc.data(), D.condFunc.size()))) return st;
         if ((st = upload(ctx, ctx->dDomeCC, D.condCdf.data(), D.condCdf.size()))) return st;
         if ((st = upload(ctx, ctx->dDomeCI, D.condInt.data(), D.condInt.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeMG, D.margGuide.data(), D.margGuide.size()))) return st;
+        if ((st = upload(ctx, ctx->dDomeCG, D.condGuide.data(), D.condGuide.size()))) return st;
         S.hasEnv = 1u;
         S.dome.px = ctx->dDomePx.as<float>();
         S.dome.margFunc = ctx->dDomeMF.as<float>();
@@ -576,6 +578,10 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         S.dome.condFunc = ctx->dDomeCF.as<float>();
         S.dome.condCdf = ctx->dDomeCC.as<float>();
         S.dome.condInt = ctx->dDomeCI.as<float>();
+        S.dome.margGuide = ctx->dDomeMG.as<uint32_t>();
+        S.dome.condGuide = ctx->dDomeCG.as<uint32_t>();
+        S.dome.margM = D.margM;
+        S.dome.condM = D.condM;
         S.dome.margInt = D.margInt;
         S.dome.W = D.W;
         S.dome.H = D.H;

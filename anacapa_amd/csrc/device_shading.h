@@ -393,8 +393,15 @@ __device__ V3 matEvalDiff(const DMat& m, const Frame& fr, V3 base, float r, V3 w
 }
 
 // ---------------------------------------------------------------- dome light
-__device__ __forceinline__ uint32_t lowerBound(const float* cdf, uint32_t n, float u) {
-    uint32_t lo = 0, count = n;
+// First index i of cdf[0..n] with cdf[i] >= u (the lower-bound search of Distribution1D::sample),
+// through a guide table: guide[k] = that index for u = k / M (M a power of two).  The search is
+// monotone in u, so for u in [k/M, (k+1)/M) its result lies in [guide[k], guide[k+1]] and the
+// restricted search returns the same index in one or two steps instead of log2(n).  u * M is
+// exact and u < 1.
+__device__ __forceinline__ uint32_t lowerBoundGuided(const float* cdf, const uint32_t* guide, uint32_t M, float u) {
+    uint32_t k = (uint32_t)(u * (float)M);
+    k = k < M ? k : M - 1u;
+    uint32_t lo = guide[k], count = guide[k + 1u] - lo;
     while (count > 0u) {
         uint32_t step = count >> 1;
         if (cdf[lo + step] < u) { lo += step + 1u; count -= step + 1u; }
@@ -404,9 +411,10 @@ __device__ __forceinline__ uint32_t lowerBound(const float* cdf, uint32_t n, flo
 }
 
 // Distribution1D::sample (DomeLight.h:40-54) over func[n], cdf[n+1].
-__device__ __forceinline__ uint32_t dist1dSample(const float* func, const float* cdf, float integral, uint32_t n,
-                                                 float u, float& pdf, float& ur) {
-    int idx = (int)lowerBound(cdf, n + 1u, u) - 1;
+__device__ __forceinline__ uint32_t dist1dSample(const float* func, const float* cdf, const uint32_t* guide,
+                                                 uint32_t M, float integral, uint32_t n, float u, float& pdf,
+                                                 float& ur) {
+    int idx = (int)lowerBoundGuided(cdf, guide, M, u) - 1;
     idx = idx < 0 ? 0 : (idx > (int)n - 1 ? (int)n - 1 : idx);
     float bw = cdf[idx + 1] - cdf[idx];
     ur = (bw > 1e-12f) ? (u - cdf[idx]) / bw : 0.f;
@@ -469,9 +477,10 @@ __device__ float domePdf(const DDome& d, V3 w) {
 // DomeLight::sampleDirection (DomeLight.h:264-295)
 __device__ V3 domeSampleDir(const DDome& d, V2 u, float& pdf) {
     float pr, pc, ur, uc;
-    uint32_t row = dist1dSample(d.margFunc, d.margCdf, d.margInt, d.H, u.y, pr, ur);
-    uint32_t col = dist1dSample(d.condFunc + (size_t)row * d.W, d.condCdf + (size_t)row * (d.W + 1u), d.condInt[row],
-                                d.W, u.x, pc, uc);
+    uint32_t row = dist1dSample(d.margFunc, d.margCdf, d.margGuide, d.margM, d.margInt, d.H, u.y, pr, ur);
+    uint32_t col = dist1dSample(d.condFunc + (size_t)row * d.W, d.condCdf + (size_t)row * (d.W + 1u),
+                                d.condGuide + (size_t)row * (d.condM + 1u), d.condM, d.condInt[row], d.W, u.x, pc,
+                                uc);
     float v = ((float)row + ur) / (float)d.H;
     float uu = ((float)col + uc) / (float)d.W;
     float theta = v * kPi, phi = uu * k2Pi;

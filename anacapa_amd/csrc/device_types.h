@@ -116,6 +116,9 @@ struct DDome {
     const float* condFunc;    // [H*W]
     const float* condCdf;     // [H*(W+1)]
     const float* condInt;     // [H]
+    const uint32_t* margGuide;  // [margM+1]: first marginal CDF index >= k / margM
+    const uint32_t* condGuide;  // [H*(condM+1)]: the same per row of the conditional CDF
+    uint32_t margM, condM;      // powers of two
     float margInt;
     uint32_t W, H;
     float intensity;

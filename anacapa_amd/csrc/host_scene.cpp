@@ -45,6 +45,32 @@ V ld(const float* p) { return {p[0], p[1], p[2]}; }
 void st(float* p, V v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 
 // Distribution1D::build (src/shading/lights/DomeLight.h:25-37)
+// First index i of cdf[0..n) with cdf[i] >= u (n if none), the device search's predicate.
+uint32_t lowerBoundHost(const float* cdf, uint32_t n, float u) {
+    uint32_t lo = 0, count = n;
+    while (count > 0u) {
+        uint32_t step = count >> 1;
+        if (cdf[lo + step] < u) { lo += step + 1u; count -= step + 1u; }
+        else count = step;
+    }
+    return lo;
+}
+
+// Guide table size: a power of two of at least twice the bins, so k / M and u * M are exact.
+uint32_t guideSize(uint32_t bins) {
+    uint32_t m = 1;
+    while (m < 2u * bins && m < (1u << 20)) m <<= 1;
+    return m;
+}
+
+// guide[k] = lowerBound(cdf, k / M) for k = 0..M.  The search is monotone in u, so for u in
+// [k/M, (k+1)/M) its result lies in [guide[k], guide[k+1]] and a search restricted to that range
+// returns the same index.
+void buildGuide(const float* cdf, uint32_t n, uint32_t M, uint32_t* guide) {
+    const float inv = 1.f / (float)M;
+    for (uint32_t k = 0; k <= M; ++k) guide[k] = lowerBoundHost(cdf, n, (float)k * inv);
+}
+
 float buildDist(const float* f, uint32_t n, float* func, float* cdf) {
     std::memcpy(func, f, n * sizeof(float));
     cdf[0] = 0.f;
@@ -242,6 +268,13 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
         D.margFunc.resize(D.H);
         D.margCdf.resize(D.H + 1);
         D.margInt = buildDist(rowSums.data(), D.H, D.margFunc.data(), D.margCdf.data());
+        D.margM = guideSize(D.H);
+        D.condM = guideSize(D.W);
+        D.margGuide.resize(D.margM + 1);
+        buildGuide(D.margCdf.data(), D.H + 1, D.margM, D.margGuide.data());
+        D.condGuide.resize((size_t)(D.condM + 1) * D.H);
+        for (uint32_t r = 0; r < D.H; ++r)
+            buildGuide(&D.condCdf[(size_t)r * (D.W + 1)], D.W + 1, D.condM, &D.condGuide[(size_t)r * (D.condM + 1)]);
         out.hasDome = true;
         (void)k2Pi;
     }

@@ -16,6 +16,10 @@ struct HostDome {
     uint32_t W = 0, H = 0;
     std::vector<float> px4;          // W*H*4 (rgb + pad)
     std::vector<float> margFunc, margCdf, condFunc, condCdf, condInt;
+    // guide tables of the CDF searches: guide[k] = first index i with cdf[i] >= k / M (M a power
+    // of two), so a sample u in [k/M, (k+1)/M) is found between guide[k] and guide[k+1]
+    std::vector<uint32_t> margGuide, condGuide;
+    uint32_t margM = 0, condM = 0;
     float margInt = 0.f;
     float intensity = 1.f;
     float rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
