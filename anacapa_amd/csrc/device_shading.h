@@ -127,11 +127,6 @@ __device__ __forceinline__ float texBilinear(const DTexture& t, V2 uv) {
     return (1.f - ay) * ((1.f - ax) * p00 + ax * p01) + ay * ((1.f - ax) * p10 + ax * p11);
 }
 
-struct MatEval {
-    const DMat* m;
-    float rough;   // evalTOV(roughness, uv)
-};
-
 // Layer weights at wo (StandardSurface.h:355-371); false when they sum to zero.
 __device__ __forceinline__ bool layerWeights(const DMat& m, float woz, float& wc, float& wm, float& ws, float& wd) {
     float coatF = schlickScalar(woz, m.coatF0), specF = schlickScalar(woz, m.f0d);
