@@ -29,8 +29,19 @@ __device__ __forceinline__ uint64_t pcgSeededState(uint64_t seed) {
 
 __device__ __forceinline__ float radicalInverse(uint32_t b, float invB, uint32_t magic, uint32_t shifts,
                                                 uint32_t n) {
-    const uint32_t sh1 = shifts & 0xffu, sh2 = shifts >> 8;
     float invBN = 1.f, result = 0.f;
+    if (shifts & 0x10000u) {   // n / b == mulhi(magic, n) >> sh for every 32-bit n
+        const uint32_t sh = shifts & 0xffu;
+        while (n > 0u) {
+            uint32_t q = __umulhi(magic, n) >> sh;
+            uint32_t d = n - q * b;
+            n = q;
+            invBN *= invB;
+            result = fmaf((float)d, invBN, result);
+        }
+        return result;
+    }
+    const uint32_t sh1 = shifts & 0xffu, sh2 = shifts >> 8;
     while (n > 0u) {
         uint32_t t = __umulhi(magic, n);
         uint32_t q = (t + ((n - t) >> sh1)) >> sh2;       // == n / b exactly

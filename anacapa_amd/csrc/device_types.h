@@ -127,13 +127,15 @@ struct DDome {
 
 // Halton bases and their reciprocals (1.f / (float)prime, the reference's invB), in constant memory.
 // Per-dimension base, its float reciprocal and the constants of exact division by the base
-// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
-// n / b == (t + ((n - t) >> sh1)) >> sh2 with t = mulhi(magic, n), for every 32-bit n.
+// (Granlund & Montgomery, "Division by invariant integers using multiplication", PLDI 1994): for
+// 94 of the 128 bases a 32-bit multiplier exists and n / b == mulhi(magic, n) >> sh (shifts =
+// sh | 1 << 16, Theorem 4.2); for the others n / b == (t + ((n - t) >> sh1)) >> sh2 with
+// t = mulhi(magic, n) (figure 4.1; shifts = sh1 | sh2 << 8), for every 32-bit n.
 struct HaltonTables {
     uint32_t prime[128];
     float invPrime[128];
     uint32_t magic[128];
-    uint32_t shifts[128];   // sh1 | sh2 << 8
+    uint32_t shifts[128];
 };
 
 struct DCamera {

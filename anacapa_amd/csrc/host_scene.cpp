@@ -90,6 +90,20 @@ void fillHaltonTables(HaltonTables& t) {
     for (int i = 0; i < 128; ++i) {
         const uint32_t b = t.prime[i];
         t.invPrime[i] = 1.f / static_cast<float>(b);
+        // A 32-bit multiplier when one exists (Granlund & Montgomery, Theorem 4.2 with N = 32):
+        // m = ceil(2^(32+s) / b) and m * b - 2^(32+s) <= 2^s give n / b == mulhi(m, n) >> s for
+        // every 32-bit n (flag bit 16).  Otherwise the 33-bit multiplier of their figure 4.1.
+        bool simple = false;
+        for (uint32_t sh = 0; sh < 32u && !simple; ++sh) {
+            const uint64_t p2 = 1ull << (32u + sh);
+            const uint64_t m = (p2 + b - 1u) / b;
+            if (m < (1ull << 32) && m * b - p2 <= (1ull << sh)) {
+                t.magic[i] = static_cast<uint32_t>(m);
+                t.shifts[i] = sh | 0x10000u;
+                simple = true;
+            }
+        }
+        if (simple) continue;
         uint32_t l = 0;
         while ((1ull << l) < b) ++l;                       // l = ceil(log2 b)
         t.magic[i] = static_cast<uint32_t>((((1ull << l) - b) << 32) / b + 1u);
