@@ -523,6 +523,8 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.strandInfo = ctx->dStrandInfo.as<float>();
     S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
     if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes4.data(), ctx->bvh.nodes4.size()))) return st;
+    for (DTri& t : ctx->bvh.tris)   // the material of each triangle, so shading skips the mesh table
+        t.matIndex = t.meshID < hs.meshMat.size() ? hs.meshMat[t.meshID] : -1;
     if ((st = upload(ctx, ctx->dTris, ctx->bvh.tris.data(), ctx->bvh.tris.size()))) return st;
     if ((st = upload(ctx, ctx->dShade, ctx->bvh.shade.data(), ctx->bvh.shade.size()))) return st;
     if ((st = upload(ctx, ctx->dMats, hs.mats.data(), hs.mats.size()))) return st;

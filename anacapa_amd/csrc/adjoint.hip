@@ -152,14 +152,14 @@ __global__ __launch_bounds__(kReplayBS, 3) void k_replay(DScene sc, ReplayArgs a
             float u = tu, v = tv, w = 1.f - u - v;
             V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2);
             V3 ng = ld3(T->n);
-            const uint32_t meshID = T->meshID, commit = T->commitIndex;
+            const uint32_t commit = T->commitIndex;
             const DTriShade* SH = sc.shade + tri;
             V3 p = v0 + e1 * u + e2 * v;
             V3 n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
             if (dot(n, ng) < 0.f) n = -n;
             V2 uv{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v, SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
             Frame fr = makeFrame(n, ng, dir);
-            int32_t mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
+            const int32_t mi = T->matIndex;   // sc.meshMat[meshID], resolved at prepare time
             if (mi < 0) break;
             const DMat& m = sc.mats[mi];
             const MatIn mIn = matInputs(sc, m, uv, commit);

@@ -284,7 +284,7 @@ int buildBVH(const ahip_scene_desc* d, BVHOutput& out, std::string& err) {
             T.meshID = m;
             T.primID = k;
             T.commitIndex = t;
-            T.pad = 0;
+            T.matIndex = -1;   // resolved against the material table at prepare time
             DTriShade& S = sh[t];
             float* sn[3] = {S.sn0, S.sn1, S.sn2};
             float* uv[3] = {S.uv0, S.uv1, S.uv2};

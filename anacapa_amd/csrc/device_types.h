@@ -53,7 +53,7 @@ struct alignas(64) DTri {
     float n[3];               // geometric normal (normalised cross(e1, e2))
     uint32_t meshID, primID;  // SurfaceInteraction ids
     uint32_t commitIndex;     // index in commit order (reference m_tris)
-    uint32_t pad;
+    int32_t matIndex;         // scene.materials[meshID] resolved at prepare time, -1 = none
 };
 static_assert(sizeof(DTri) == 64, "DTri must be 64 bytes");
 

@@ -238,6 +238,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                 V3 p, n, ng;
                 V2 uv;
                 uint32_t meshID, commit = ~0u;
+                int32_t mi = -1;
                 bool isCurve = CURVES && (tri & kCurveHit) != 0u;
                 Frame fr;
                 if (!isCurve) {
@@ -248,6 +249,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     ng = ld3(T->n);
                     meshID = T->meshID;
                     commit = T->commitIndex;
+                    mi = T->matIndex;   // sc.meshMat[meshID], resolved at prepare time
                     const DTriShade* SH = sc.shade + tri;
                     p = v0 + e1 * u + e2 * v;
                     n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
@@ -273,8 +275,8 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     uv = V2{0.f, hh.y};
                     meshID = __float_as_uint(info[3]);
                     fr = makeCurveFrame(n, tang, dir, V3{info[0], info[1], info[2]});
+                    mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 }
-                int32_t mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 if (mi >= 0) {
                     const DMat& m = sc.mats[mi];
                     const MatIn mIn = matInputs(sc, m, uv, commit);
