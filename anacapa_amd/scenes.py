@@ -290,7 +290,7 @@ def _block(mb, lo, hi, deg, mat):
 
 
 def cornell(width=512, height=512, spp=64, max_depth=8, seed=0) -> Scene:
-    """Workload 1: unit-box Cornell scene, 36 triangles, one 0.25x0.25 quad light (17,12,4)."""
+    """The Cornell box: unit-box scene, 36 triangles, one 0.25x0.25 quad light (17,12,4)."""
     mats = [lambert((0.73, 0.73, 0.73)), lambert((0.63, 0.065, 0.05)), lambert((0.14, 0.45, 0.09)),
             emissive((0.0, 0.0, 0.0), (17.0, 12.0, 4.0))]
     mb = _MeshBuilder()
@@ -377,7 +377,7 @@ def procedural_hdri(width=2048, height=1024, sun_peak=5e4, sun_sigma_deg=0.5,
 
 def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, freq=7,
                 hdri_size=(2048, 1024)) -> Scene:
-    """Workload 2: grid x grid displaced geodesic spheres (20*freq^2 triangles each; freq=7 gives
+    """The GGX-spheres scene: grid x grid displaced geodesic spheres (20*freq^2 triangles each; freq=7 gives
     1,003,520 sphere triangles for a 32x32 grid) with per-sphere standard_surface on a 2-triangle
     ground, lit by a procedural HDRI dome with importance sampling."""
     rng = np.random.default_rng(seed)
@@ -418,7 +418,7 @@ def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, f
 
 def inverse_scene(width=512, height=512, spp=8, max_depth=4, seed=0, grid=32, freq=7, tex=1024,
                   hdri_size=(2048, 1024)) -> Scene:
-    """Workload 5 at its ground truth: the workload-2 geometry with a per-triangle base colour
+    """The inverse-rendering scene at its ground truth: the GGX-spheres geometry with a per-triangle base colour
     (U[0.05, 0.95]^3 per triangle, the albedo parameters) and one tex x tex roughness texture
     (U[0.05, 1] per texel, sampled through each sphere's UVs) shared by every sphere material."""
     sc = ggx_spheres(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=grid, freq=freq,
@@ -433,7 +433,7 @@ def inverse_scene(width=512, height=512, spp=8, max_depth=4, seed=0, grid=32, fr
 
 
 def perturbed_params(sc: Scene, seed=1, sigma=0.1):
-    """Initial parameters of workload 5: ground truth + N(0, sigma), clamped to [0, 1]."""
+    """Initial parameters of the inverse-rendering scene: ground truth + N(0, sigma), clamped to [0, 1]."""
     rng = np.random.default_rng(seed)
     alb = np.clip(sc.tri_albedo + rng.normal(0.0, sigma, size=sc.tri_albedo.shape), 0.0, 1.0).astype(np.float32)
     tex = np.clip(sc.textures[0] + rng.normal(0.0, sigma, size=sc.textures[0].shape), 0.0, 1.0).astype(np.float32)
@@ -441,7 +441,7 @@ def perturbed_params(sc: Scene, seed=1, sigma=0.1):
 
 
 def small_inverse(width=32, height=24, spp=2, max_depth=4, seed=0) -> Scene:
-    """A few-hundred-triangle instance of workload 5 for oracle-speed gradient tests."""
+    """A few-hundred-triangle instance of the inverse-rendering scene for oracle-speed gradient tests."""
     sc = inverse_scene(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=3, freq=3,
                        tex=16, hdri_size=(256, 128))
     c = 1.6                                         # centre of the 3x3 grid
@@ -475,7 +475,7 @@ def batch_views(sc: Scene, n=64, seed=0, resolutions=((256, 256), (512, 512), (1
 
 
 def small_ggx(width=96, height=64, spp=4, max_depth=8, seed=0, grid=3, freq=3) -> Scene:
-    """A few-hundred-triangle instance of workload 2 for oracle-speed parity tests."""
+    """A few-hundred-triangle instance of the GGX-spheres scene for oracle-speed parity tests."""
     return ggx_spheres(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, grid=grid,
                        freq=freq, hdri_size=(256, 128))
 
@@ -492,7 +492,7 @@ def triangle_soup(n=2000, seed=1) -> Scene:
 
 def hair_sphere(width=1024, height=1024, spp=32, max_depth=16, seed=0, strands=200_000, segments=4,
                 hdri_size=(1024, 512), scalp_freq=8) -> Scene:
-    """Workload 3: `strands` cubic-Bezier strands of `segments` segments (shared end points) rooted
+    """The hair sphere: `strands` cubic-Bezier strands of `segments` segments (shared end points) rooted
     at Fibonacci points of a unit sphere, grown along the normal with length U[0.3, 0.6] and a
     seed-driven curl, width tapering 0.004 -> 0.001; per-strand eumelanin U[0.2, 1.5] encoded as
     the strand colour the hair material turns back into sigma_a (MarschnerHair.h:430-450); one
@@ -540,6 +540,6 @@ def hair_sphere(width=1024, height=1024, spp=32, max_depth=16, seed=0, strands=2
 
 
 def small_hair(width=64, height=64, spp=2, max_depth=16, seed=0) -> Scene:
-    """A few-hundred-strand instance of workload 3 for oracle-speed parity tests."""
+    """A few-hundred-strand instance of the hair sphere for oracle-speed parity tests."""
     return hair_sphere(width=width, height=height, spp=spp, max_depth=max_depth, seed=seed, strands=400,
                        hdri_size=(128, 64), scalp_freq=3)

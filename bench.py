@@ -246,7 +246,7 @@ def batched_form(be, sc, args, log):
 
 
 def adjoint_form(be, log):
-    """Workload 5: 1M-triangle spheres with per-triangle albedo and a 1024^2 roughness texture,
+    """The inverse-rendering workload: 1M-triangle spheres with per-triangle albedo and a 1024^2 roughness texture,
     512x512, 8 spp, depth 4.  Times (HIP events) the plain wavefront forward render, the replay
     estimator's image and the adjoint pass at the perturbed initial parameters, with the image
     L2 loss against the ground-truth render as dL/dI."""
@@ -291,7 +291,7 @@ def adjoint_form(be, log):
 
 
 def other_configs(be, log):
-    """Workloads 1 and 3 on the same backend: the Cornell box (36 triangles, 512x512, 64 spp, depth
+    """The Cornell and hair workloads on the same backend: the Cornell box (36 triangles, 512x512, 64 spp, depth
     8) and the hair sphere (200k Bezier strands = 800k segments, 1024x1024, 32 spp, depth 16), one
     warm-up and two timed frames each (HIP events around the frame)."""
     from anacapa_amd import scenes

@@ -5,7 +5,7 @@
 - replay at perturbed evaluation parameters matches the oracle's replay to the radiance tolerance;
 - the adjoint matches the oracle's adjoint (float atomics and association only);
 - the adjoint matches central differences (h = 1e-3) of the GPU replay to 1e-2 relative: the
-  image L2 loss on the small scene; at workload-5 scale (1M triangles, 1024^2 roughness texture,
+  image L2 loss on the small scene; at full scale (1M triangles, 1024^2 roughness texture,
   512x512x8, depth 4) a random linear image functional and the image L2 loss, each on 256 random
   parameters, within 1e-2 plus the central difference's own float32 rounding bound;
 - a torch.autograd.Function over replay/adjoint gives the same gradient and descends the loss.
@@ -173,9 +173,9 @@ def test_adjoint_matches_central_differences_small(gpu, small):
 
 
 @pytest.fixture(scope="module")
-def workload5(gpu):
+def inverse512(gpu):
     sc = scenes.inverse_scene()
-    print(f"[adjoint] workload 5: {sc.num_triangles} triangles, {sc.width}x{sc.height}x{sc.spp}", flush=True)
+    print(f"[adjoint] inverse-rendering scene: {sc.num_triangles} triangles, {sc.width}x{sc.height}x{sc.spp}", flush=True)
     gpu.prepare(sc)
     gpu.set_eval_params()
     t = gpu.replay(0, sc.spp)              # target: ground-truth parameters
@@ -186,26 +186,26 @@ def workload5(gpu):
     return sc, target, alb0, tex0
 
 
-def test_adjoint_matches_central_differences_workload5(gpu, workload5):
+def test_adjoint_matches_central_differences_inverse512(gpu, inverse512):
     """A random linear image functional (N(0,1) pixel weights) on 256 random parameters (128
     albedo, 128 texels).  The adjoint is linear in dL/dI, so this pins the Jacobian-transpose
     product that the L2 loss uses too."""
-    sc, target, alb0, tex0 = workload5
+    sc, target, alb0, tex0 = inverse512
     gpu.prepare(sc)
     w = np.random.default_rng(9).normal(size=target.shape)
     rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 5, weights=w)
     assert noisy <= 256 // 10, noisy
-    print(f"[adjoint] workload 5, linear functional: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
+    print(f"[adjoint] inverse512, linear functional: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
           f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
 
 
-def test_adjoint_l2_loss_central_differences_workload5(gpu, workload5):
+def test_adjoint_l2_loss_central_differences_inverse512(gpu, inverse512):
     """The image L2 loss against the ground-truth render on 256 random parameters."""
-    sc, target, alb0, tex0 = workload5
+    sc, target, alb0, tex0 = inverse512
     gpu.prepare(sc)
     rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 7)
     assert noisy <= 256 // 10, noisy
-    print(f"[adjoint] workload 5, image L2 loss: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
+    print(f"[adjoint] inverse512, image L2 loss: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
           f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
 
 
