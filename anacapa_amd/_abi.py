@@ -143,6 +143,14 @@ SIGNATURES = {
     "ahip_fail_next_launch": (C.c_int, [VP]),
     "ahip_camera_pinhole": (C.c_int, [F32P, F32P, F32P, C.c_float, C.c_uint32, C.c_uint32,
                                       C.POINTER(Camera)]),
+    "ahip_usd_load": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_char_p, C.POINTER(VP),
+                                C.c_char_p, C.c_size_t]),
+    "ahip_usd_load_text": (C.c_int, [C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint32, C.c_char_p,
+                                     C.POINTER(VP), C.c_char_p, C.c_size_t]),
+    "ahip_usd_free": (None, [VP]),
+    "ahip_usd_scene_desc": (C.POINTER(SceneDesc), [VP]),
+    "ahip_usd_camera": (C.c_int, [VP, F32P, F32P, F32P, F32P, C.POINTER(Camera)]),
+    "ahip_usd_mesh_name": (C.c_char_p, [VP, C.c_uint32]),
 }
 
 _lib = None

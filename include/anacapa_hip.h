@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define AHIP_ABI_VERSION 3
+#define AHIP_ABI_VERSION 4
 
 /* status codes */
 #define AHIP_OK              0
@@ -295,6 +295,34 @@ int ahip_fail_next_launch(ahip_context* ctx);
 /* --- host helper: reference pinhole camera (Camera::makePinhole, IIntegrator.h:49-74) --- */
 int ahip_camera_pinhole(const float from[3], const float at[3], const float up[3],
                         float vfov_degrees, uint32_t width, uint32_t height, ahip_camera* out);
+
+/* --- USD-lite scene ingest (host only; no device context needed) ---------------------------
+ * Reads one text USD layer (.usda) without the USD SDK into a flat scene for ahip_prepare,
+ * restating the reference's loadUSD (src/scene/usd/USDLoader.h:26-29, USDLoader.cpp:1166-1833,
+ * returning LoadedScene, include/anacapa/scene/SceneLoader.h:20-33): meshes fan-triangulated
+ * into world space with three vertices per triangle, GeomSubset and displayColor materials,
+ * UsdPreviewSurface constants as standard_surface, RectLight / SphereLight / DiskLight and
+ * emissive meshes as area lights, the pinhole camera (explicit path, RenderSettings camera, or
+ * the first camera), upAxis Z turned to Y-up.  A layer that needs composition (references,
+ * payloads, sublayers, variants), animation, image textures, DistantLight / DomeLight or a
+ * thin-lens camera is refused with AHIP_E_UNSUPPORTED; malformed text with AHIP_E_INVALID_ARG.
+ * The message goes to err (err_len bytes, may be NULL).  The scene owns every array the
+ * descriptor points to; both stay valid until ahip_usd_free. */
+typedef struct ahip_usd_scene ahip_usd_scene;
+int ahip_usd_load(const char* path, uint32_t film_width, uint32_t film_height, const char* camera_path,
+                  ahip_usd_scene** out, char* err, size_t err_len);
+int ahip_usd_load_text(const char* text, size_t len, uint32_t film_width, uint32_t film_height,
+                       const char* camera_path, ahip_usd_scene** out, char* err, size_t err_len);
+void ahip_usd_free(ahip_usd_scene* scene);
+/* LoadedScene::geomPool + sceneView as the descriptor ahip_prepare takes. */
+const ahip_scene_desc* ahip_usd_scene_desc(const ahip_usd_scene* scene);
+/* LoadedScene::camera (buildCamera, USDLoader.cpp:1084-1161): eye, target, up, vertical field of
+ * view for the film size given at load, and the pinhole built from them; any output may be NULL.
+ * AHIP_E_NOT_PREPARED when the layer has no camera (the caller picks its default). */
+int ahip_usd_camera(const ahip_usd_scene* scene, float from[3], float at[3], float up[3],
+                    float* vfov_degrees, ahip_camera* camera);
+/* MeshDesc::name of mesh i (the prim name), NULL past the end. */
+const char* ahip_usd_mesh_name(const ahip_usd_scene* scene, uint32_t mesh);
 
 #ifdef __cplusplus
 }

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     lib = A.load_library()
     for name in _declared():
         assert hasattr(lib, name), name
-    assert lib.ahip_abi_version() == 3
+    assert lib.ahip_abi_version() == 4
     assert lib.ahip_backend_type() == b"HIP"
 
 
