@@ -71,6 +71,7 @@ class SceneDesc(C.Structure):
         ("num_strands", C.c_uint32), ("strand_cv_offset", U32P), ("curve_cvs", F32P), ("strand_width", F32P),
         ("strand_mesh_index", U32P), ("strand_color", F32P),
         ("tri_albedo", F32P),
+        ("strand_width_offset", U32P), ("strand_widths", F32P),
     ]
 
 
@@ -95,6 +96,21 @@ class Stats(C.Structure):
                 ("trace_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("shade_launches", C.c_uint64),
                 ("render_ms", C.c_double), ("passes", C.c_uint64), ("graph_passes", C.c_uint64)]
+
+
+AHIP_CURVE_LINEAR, AHIP_CURVE_BEZIER, AHIP_CURVE_BSPLINE, AHIP_CURVE_CATMULL_ROM = 0, 1, 2, 3
+
+
+class CurveOptions(C.Structure):
+    _fields_ = [("default_width", C.c_float), ("width_scale", C.c_float), ("material_index", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class Strands(C.Structure):
+    _fields_ = [("num_strands", C.c_uint32), ("num_cvs", C.c_uint32), ("num_skipped", C.c_uint32),
+                ("pad", C.c_uint32), ("strand_cv_offset", U32P), ("curve_cvs", F32P),
+                ("strand_width_offset", U32P), ("strand_widths", F32P), ("strand_mesh_index", U32P),
+                ("strand_color", F32P)]
 
 
 class Hit(C.Structure):
@@ -151,6 +167,10 @@ SIGNATURES = {
     "ahip_usd_scene_desc": (C.POINTER(SceneDesc), [VP]),
     "ahip_usd_camera": (C.c_int, [VP, F32P, F32P, F32P, F32P, C.POINTER(Camera)]),
     "ahip_usd_mesh_name": (C.c_char_p, [VP, C.c_uint32]),
+    "ahip_curves_to_bezier": (C.c_int, [C.c_uint32, C.c_uint32, VP, VP, VP, C.c_uint32, VP, VP,
+                                        C.POINTER(CurveOptions), C.POINTER(VP), C.c_char_p, C.c_size_t]),
+    "ahip_curve_set_strands": (C.POINTER(Strands), [VP]),
+    "ahip_curve_set_free": (None, [VP]),
 }
 
 _lib = None

@@ -443,14 +443,25 @@ struct CurveBuilder {
     }
 };
 
-float widthAt(float root, float tip, float v) { return root * (1.f - v) + tip * v; }   // CurvePool.h:45-54
+// StrandDesc::widthAt (CurvePool.h:45-54) over a strand's diameters w[0, n)
+float widthAt(const float* w, uint32_t n, float v) {
+    if (n == 0) return 0.001f;
+    if (n == 1) return w[0];
+    if (n == 2) return w[0] * (1.f - v) + w[1] * v;
+    const float x = v * float(int(n) - 1);
+    const int lo = std::max(0, int(x));
+    const int hi = std::min(int(n) - 1, lo + 1);
+    const float f = x - float(lo);
+    return w[lo] * (1.f - f) + w[hi] * f;
+}
 
 }  // namespace
 
 int buildCurves(const ahip_scene_desc* d, CurveOutput& out, std::string& err) {
     out = CurveOutput{};
     if (!d->num_strands) return 0;
-    if (!d->strand_cv_offset || !d->curve_cvs || !d->strand_width || !d->strand_mesh_index) {
+    if (!d->strand_cv_offset || !d->curve_cvs || !(d->strand_width || d->strand_widths) || !d->strand_mesh_index ||
+        (d->strand_widths && !d->strand_width_offset)) {
         err = "strand arrays missing";
         return 1;
     }
@@ -465,7 +476,15 @@ int buildCurves(const ahip_scene_desc* d, CurveOutput& out, std::string& err) {
             return 1;
         }
         uint32_t ncv = c1 - c0, N = ncv < 4 ? 0u : (ncv - 1) / 3;
-        float root = d->strand_width[2 * s], tip = d->strand_width[2 * s + 1];
+        const float* sw = d->strand_widths ? d->strand_widths + d->strand_width_offset[s] : d->strand_width + 2 * s;
+        uint32_t nw = 2;
+        if (d->strand_widths) {
+            if (d->strand_width_offset[s + 1] < d->strand_width_offset[s]) {
+                err = "strand_width_offset not increasing";
+                return 1;
+            }
+            nw = d->strand_width_offset[s + 1] - d->strand_width_offset[s];
+        }
         float* info = &out.strandInfo[4 * (size_t)s];
         for (int k = 0; k < 3; ++k) info[k] = d->strand_color ? d->strand_color[3 * (size_t)s + k] : 1.f;
         uint32_t mesh = d->strand_mesh_index[s];
@@ -474,7 +493,7 @@ int buildCurves(const ahip_scene_desc* d, CurveOutput& out, std::string& err) {
             DSeg S;
             for (int k = 0; k < 4; ++k) std::memcpy(&S.P[3 * k], d->curve_cvs + 3 * (size_t)(c0 + 3 * g + k), 12);
             float v0 = float(g) / float(N), v3 = float(g + 1) / float(N);
-            float wa = widthAt(root, tip, v0), wb = widthAt(root, tip, v3);
+            float wa = widthAt(sw, nw, v0), wb = widthAt(sw, nw, v3);
             S.hw0 = wa * 0.5f;
             S.hw3 = wb * 0.5f;
             S.tOff = v0;

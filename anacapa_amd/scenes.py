@@ -41,6 +41,8 @@ class Scene:
     strand_mesh_index: np.ndarray | None = None       # [S] u32
     strand_color: np.ndarray | None = None            # [S,3] f32
     tri_albedo: np.ndarray | None = None              # [T,3] f32, commit order (per-triangle base colour)
+    strand_width_offset: np.ndarray | None = None     # [S+1] u32 ranges of strand_widths
+    strand_widths: np.ndarray | None = None           # per-CV diameters (replace strand_width)
     cam_from: tuple = (0.0, 0.0, 0.0)
     cam_at: tuple = (0.0, 0.0, 1.0)
     cam_up: tuple = (0.0, 1.0, 0.0)
@@ -119,12 +121,15 @@ class Scene:
         if self.strand_cv_offset is not None and len(self.strand_cv_offset) > 1:
             so = arr(self.strand_cv_offset, np.uint32)
             cv = arr(self.curve_cvs.reshape(-1), np.float32)
-            sw = arr(self.strand_width.reshape(-1), np.float32)
             sm = arr(self.strand_mesh_index, np.uint32)
             d.num_strands = len(so) - 1
             d.strand_cv_offset = so.ctypes.data_as(A.U32P)
             d.curve_cvs = cv.ctypes.data_as(A.F32P)
-            d.strand_width = sw.ctypes.data_as(A.F32P)
+            if self.strand_width is not None:
+                d.strand_width = arr(self.strand_width.reshape(-1), np.float32).ctypes.data_as(A.F32P)
+            if self.strand_widths is not None:
+                d.strand_width_offset = arr(self.strand_width_offset, np.uint32).ctypes.data_as(A.U32P)
+                d.strand_widths = arr(self.strand_widths, np.float32).ctypes.data_as(A.F32P)
             d.strand_mesh_index = sm.ctypes.data_as(A.U32P)
             if self.strand_color is not None:
                 d.strand_color = arr(self.strand_color.reshape(-1), np.float32).ctypes.data_as(A.F32P)

@@ -128,6 +128,12 @@ typedef struct ahip_scene_desc {
      * material; NULL = the materials' own.  Equivalent to one material per triangle in the
      * reference; it is the albedo parameter vector of ahip_adjoint. */
     const float*    tri_albedo;
+    /* Per-CV strand diameters (StrandDesc::widths, CurvePool.h:30), used instead of strand_width
+     * when non-NULL: strand s owns strand_widths[strand_width_offset[s], strand_width_offset[s+1]).
+     * StrandDesc::widthAt (CurvePool.h:45-54): no value gives 0.001, one a constant, two a
+     * root-to-tip ramp, more a piecewise-linear ramp over their index. */
+    const uint32_t* strand_width_offset;  /* [num_strands+1] */
+    const float*    strand_widths;
 } ahip_scene_desc;
 
 /* Camera fields of the reference pinhole Camera (IIntegrator.h:27-74). */
@@ -323,6 +329,49 @@ int ahip_usd_camera(const ahip_usd_scene* scene, float from[3], float at[3], flo
                     float* vfov_degrees, ahip_camera* camera);
 /* MeshDesc::name of mesh i (the prim name), NULL past the end. */
 const char* ahip_usd_mesh_name(const ahip_usd_scene* scene, uint32_t mesh);
+
+/* --- hair curves in interchange bases (host only) ------------------------------------------
+ * The curve handling of the reference's Alembic reader without the Alembic SDK
+ * (loadAlembicCurves, src/scene/alembic/AlembicLoader.h:42-45; processICurves and the basis
+ * conversions, AlembicLoader.cpp:58-306): one ICurves sample (per-curve vertex counts, points,
+ * widths, per-curve colours, the object's world matrix) becomes endpoint-sharing cubic Bezier
+ * strands with per-CV diameters, ready for ahip_scene_desc's strand arrays. */
+#define AHIP_CURVE_LINEAR      0   /* kLinear */
+#define AHIP_CURVE_BEZIER      1   /* kCubic, kBezierBasis (3n+1 CVs, used as is) */
+#define AHIP_CURVE_BSPLINE     2   /* kCubic, kBsplineBasis (uniform) */
+#define AHIP_CURVE_CATMULL_ROM 3   /* kCubic, kCatmullromBasis */
+
+/* AlembicCurveOptions (AlembicLoader.h:12-20). */
+typedef struct ahip_curve_options {
+    float    default_width;   /* diameter when the curves carry no widths (0.005 in the reference) */
+    float    width_scale;     /* multiplies authored widths (1.0) */
+    uint32_t material_index;  /* StrandDesc::materialIndex of every strand (baseMaterialIndex) */
+    uint32_t pad;
+} ahip_curve_options;
+
+/* The converted strands, in ahip_scene_desc's strand layout; owned by the curve set. */
+typedef struct ahip_strands {
+    uint32_t        num_strands;
+    uint32_t        num_cvs;
+    uint32_t        num_skipped;          /* curves dropped for too few or mis-counted CVs */
+    uint32_t        pad;
+    const uint32_t* strand_cv_offset;     /* [num_strands+1] */
+    const float*    curve_cvs;            /* [num_cvs*3] world space */
+    const uint32_t* strand_width_offset;  /* [num_strands+1] */
+    const float*    strand_widths;        /* [num_cvs] per-CV diameters */
+    const uint32_t* strand_mesh_index;    /* [num_strands] */
+    const float*    strand_color;         /* [num_strands*3], white where no colours were given */
+} ahip_strands;
+
+typedef struct ahip_curve_set ahip_curve_set;
+/* widths: num_widths == total points (per vertex), == num_curves (per curve), == 1 (constant),
+ * 0 (none: default_width); colors: [num_curves*3] or NULL; world: row-major 4x4 acting on column
+ * vectors (Mat4f, Types.h:144-150) or NULL for identity. */
+int ahip_curves_to_bezier(uint32_t basis, uint32_t num_curves, const uint32_t* counts, const float* points,
+                          const float* widths, uint32_t num_widths, const float* colors, const float* world,
+                          const ahip_curve_options* opts, ahip_curve_set** out, char* err, size_t err_len);
+const ahip_strands* ahip_curve_set_strands(const ahip_curve_set* set);
+void ahip_curve_set_free(ahip_curve_set* set);
 
 #ifdef __cplusplus
 }

@@ -1233,7 +1233,9 @@ void* orc_scene_create(const ahip_scene_desc* d) {
     auto s = std::make_unique<Scene>();
     s->bvh.commit(d);
     if (d->num_strands) {
-        if (!d->strand_cv_offset || !d->curve_cvs || !d->strand_width || !d->strand_mesh_index) return nullptr;
+        if (!d->strand_cv_offset || !d->curve_cvs || !(d->strand_width || d->strand_widths) || !d->strand_mesh_index ||
+            (d->strand_widths && !d->strand_width_offset))
+            return nullptr;
         s->curves.build(d);
     }
     for (uint32_t i = 0; i < d->num_textures; ++i) {

@@ -173,8 +173,18 @@ struct SegWork {
     float bmin[3], bmax[3], c[3];
 };
 
-// StrandDesc::widthAt for the two-width (root, tip) form (CurvePool.h:45-54)
-float widthAt2(float w0, float w1, float v) { return w0 * (1.f - v) + w1 * v; }
+// StrandDesc::widthAt (CurvePool.h:45-54): none -> 0.001, one -> constant, two -> root/tip ramp,
+// more -> linear interpolation between the two diameters around v * (n - 1)
+float widthAtN(const float* w, uint32_t n, float v) {
+    if (n == 0) return 0.001f;
+    if (n == 1) return w[0];
+    if (n == 2) return w[0] * (1.f - v) + w[1] * v;
+    float idx = v * float(int(n) - 1);
+    int lo = std::max(0, int(idx));
+    int hi = std::min(int(n) - 1, lo + 1);
+    float frac = idx - float(lo);
+    return w[lo] * (1.f - frac) + w[hi] * frac;
+}
 
 uint32_t buildCurves(std::vector<SegWork>& w, std::vector<CNode>& nodes, std::vector<uint32_t>& order, uint32_t s,
                      uint32_t e) {
@@ -219,6 +229,9 @@ uint32_t buildCurves(std::vector<SegWork>& w, std::vector<CNode>& nodes, std::ve
 }
 }  // namespace
 
+// exported for the tests: the oracle's StrandDesc::widthAt
+extern "C" float orc_width_at(const float* w, uint32_t n, float v) { return widthAtN(w, n, v); }
+
 void Curves::build(const ahip_scene_desc* d) {
     nodes.clear(); segs.clear(); strandMesh.clear(); strandColor.clear();
     if (!d->num_strands) return;
@@ -227,7 +240,8 @@ void Curves::build(const ahip_scene_desc* d) {
     for (uint32_t s = 0; s < d->num_strands; ++s) {
         uint32_t c0 = d->strand_cv_offset[s], c1 = d->strand_cv_offset[s + 1];
         uint32_t ncv = c1 - c0, N = ncv < 4 ? 0 : (ncv - 1) / 3;
-        float w0 = d->strand_width[2 * s], w1 = d->strand_width[2 * s + 1];
+        const float* wv = d->strand_widths ? d->strand_widths + d->strand_width_offset[s] : d->strand_width + 2 * s;
+        uint32_t nw = d->strand_widths ? d->strand_width_offset[s + 1] - d->strand_width_offset[s] : 2u;
         strandMesh.push_back(d->strand_mesh_index[s]);
         strandColor.push_back(d->strand_color ? V3{d->strand_color[3 * s], d->strand_color[3 * s + 1],
                                                    d->strand_color[3 * s + 2]}
@@ -239,7 +253,7 @@ void Curves::build(const ahip_scene_desc* d) {
                 sg.P[k] = V3{p[0], p[1], p[2]};
             }
             float v0 = float(g) / float(N), v3 = float(g + 1) / float(N);
-            float wa = widthAt2(w0, w1, v0), wb = widthAt2(w0, w1, v3);
+            float wa = widthAtN(wv, nw, v0), wb = widthAtN(wv, nw, v3);
             sg.hw0 = wa * 0.5f;
             sg.hw3 = wb * 0.5f;
             sg.tOff = v0;

@@ -36,6 +36,8 @@ class Oracle:
         L.orc_scene_destroy.argtypes = [VP]
         L.orc_radical_inverse.restype = C.c_float
         L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint64]
+        L.orc_width_at.restype = C.c_float
+        L.orc_width_at.argtypes = [VP, C.c_uint32, C.c_float]
         for n in ("orc_pcg_u32",):
             getattr(L, n).argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, VP]
         L.orc_halton.argtypes = [C.c_uint64, C.c_uint32, VP, VP, VP, VP, VP]
@@ -122,6 +124,10 @@ class Oracle:
         return out[:, :3], out[:, 3]
 
     # ---- scenes
+    def width_at(self, widths, v):
+        w = np.ascontiguousarray(widths, np.float32)
+        return self.lib.orc_width_at(P(w) if len(w) else None, len(w), v)
+
     def scene(self, sc):
         return OracleScene(self, sc)
 

@@ -64,3 +64,42 @@ def test_hair_aovs_vs_oracle(gpu, oracle):
     assert (alb.sum(-1) > 0).mean() > 0.2
     np.testing.assert_allclose(alb, oalb, rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(nrm, onrm, rtol=0, atol=1e-5)
+
+
+def _bspline_hair(**kw):
+    """The small hair sphere with its control points read as uniform B-spline CVs and per-vertex
+    diameters bulging mid-strand, converted by ahip_curves_to_bezier: 31 Bezier CVs and 31
+    diameters per strand, so widthAt interpolates between CV diameters."""
+    from anacapa_amd import curves
+    sc = scenes.small_hair(**kw)
+    n = sc.num_strands
+    per = len(sc.curve_cvs) // n
+    s = np.linspace(0.0, 1.0, per, dtype=np.float32)
+    w = (0.001 + 0.024 * s * (1 - s)).astype(np.float32)
+    st = curves.to_bezier("bspline", np.full(n, per), sc.curve_cvs, widths=np.tile(w, n), colors=sc.strand_color,
+                          material_index=2)
+    assert st["num_skipped"] == 0 and len(st["strand_widths"]) == n * (3 * (per - 3) + 1)
+    return curves.attach(sc, st)
+
+
+def test_converted_strands_with_per_cv_widths_vs_oracle(gpu, oracle):
+    sc = _bspline_hair(width=48, height=48, spp=2)
+    print(f"[curves] {sc.num_strands} B-spline strands, {len(sc.strand_widths)} CV widths", flush=True)
+    gpu.prepare(sc)
+    os_ = oracle.scene(sc)
+    r = _hair_rays(6000, 3)
+    g, o = gpu.trace_rays(r), os_.trace(r)
+    assert ((o["tri_index"] & 0x80000000) != 0).sum() > 200
+    for k in ("mesh_id", "prim_id", "tri_index"):
+        assert np.array_equal(g[k], o[k]), k
+    hit = o["mesh_id"] != 0xFFFFFFFF
+    for k in ("t", "u", "v"):
+        assert np.array_equal(g[k][hit].view(np.uint32), o[k][hit].view(np.uint32)), k
+    assert np.array_equal(gpu.occluded_rays(r), os_.occluded(r))
+    got = gpu.render_tile(0, 0, 48, 48, 0, 2)
+    ref = os_.render_tile(0, 0, 48, 48, 0, 2)
+    assert np.array_equal(got[..., 3], ref[..., 3])
+    gg, rr = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    err = np.abs(gg - rr).max(-1) / np.maximum(np.abs(rr).max(-1), 1e-3 * max(rr.mean(), 1e-6))
+    assert (err <= 1e-3).mean() >= 0.99, (err > 1e-3).sum()
+    assert abs(gg.mean() / rr.mean() - 1) < 1e-3
