@@ -142,6 +142,7 @@ SIGNATURES = {
     "ahip_render_tile": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, VP]),
     "ahip_render_batch": (C.c_int, [VP, C.POINTER(View), C.c_uint32]),
+    "ahip_render_adaptive": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP]),
     "ahip_set_aovs": (C.c_int, [VP, C.c_int]),
     "ahip_download_aovs": (C.c_int, [VP, VP]),
     "ahip_render_tile_aovs": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,

@@ -256,6 +256,7 @@ struct ViewJob {
     float4* acc;
     float* lsq;
     float* aov = nullptr;        // [w*h*6] AOV sums, or null
+    uint32_t stride = 0;         // pixels per row of acc / lsq / aov; 0 = w
 };
 
 // Pack whole sample indices of the jobs, in order, into passes of at most `cap` paths (a job whose
@@ -277,6 +278,7 @@ int planPasses(ahip_context* ctx, const std::vector<ViewJob>& jobs, std::vector<
         if (slots > 0x7fffffffull || j.x0 + j.w > 65536u || j.y0 + j.h > 65536u)
             return fail(ctx, AHIP_E_INVALID_ARG, "view too large");
         v.slots = (uint32_t)slots;
+        v.stride = j.stride ? j.stride : j.w;
         v.accum = j.acc;
         v.accumLumSq = j.lsq;
         v.accumAov = j.aov;
@@ -670,13 +672,34 @@ static int renderCall(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool 
     return finishStats(ctx);
 }
 
-int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
-                      uint32_t sample_count) {
-    ENTER();
+// Make sure the persistent film-sized accumulation (and, when on, the AOV sums) exists.
+static int ensureFrameAccum(ahip_context* ctx, uint32_t width, uint32_t height);
+
+static int renderFrameImpl(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
+                           uint32_t sample_count) {
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
     if (width != ctx->cam.width || height != ctx->cam.height)
         return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");
+    int st = ensureFrameAccum(ctx, width, height);
+    if (st) return st;
+    std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
+                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(),
+                                      ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
+    st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
+    if (st) return st;
+    ctx->lastRenderValid = true;
+    ctx->lastW = width; ctx->lastH = height; ctx->lastS0 = sample_start; ctx->lastNs = sample_count;
+    return AHIP_OK;
+}
+
+int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
+                      uint32_t sample_count) {
+    ENTER();
+    return renderFrameImpl(ctx, width, height, sample_start, sample_count);
+}
+
+static int ensureFrameAccum(ahip_context* ctx, uint32_t width, uint32_t height) {
     int st = ensureQueues(ctx, 64);
     if (st) return st;
     if (ctx->accW != width || ctx->accH != height || !ctx->dAccum.p) {
@@ -693,13 +716,60 @@ int ahip_render_frame(ahip_context* ctx, uint32_t width, uint32_t height, uint32
         HIPCHK(ctx->dAccumAov.alloc((size_t)width * height * 6 * sizeof(float)));
         launchFillF32(ctx->dAccumAov.as<float>(), (size_t)width * height * 6, 0.f, ctx->stream);
     }
-    std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
-                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(),
-                                      ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
-    st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
+    return AHIP_OK;
+}
+
+// RenderSession's adaptive two-pass render (RenderSession.cpp:549-653) on the device: the base
+// pass over the whole frame, then the CPU tile path's allocation of the remaining samples by tile
+// variance, rendered as one list of tile jobs into the same accumulation.
+int ahip_render_adaptive(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t tile_size, uint32_t total_spp,
+                         uint32_t base_spp, uint32_t* extra_per_tile, float* tile_variance) {
+    ENTER();
+    if (tile_size == 0 || total_spp == 0) return fail(ctx, AHIP_E_INVALID_ARG, "tile size and sample count must be > 0");
+    uint32_t base = base_spp ? base_spp : std::min(32u, std::max(16u, total_spp / 16u));
+    base = std::min(base, total_spp);
+    const uint32_t extra = total_spp - base;
+    int st = renderFrameImpl(ctx, width, height, 0, base);
     if (st) return st;
-    ctx->lastRenderValid = true;
-    ctx->lastW = width; ctx->lastH = height; ctx->lastS0 = sample_start; ctx->lastNs = sample_count;
+    const uint32_t tilesX = (width + tile_size - 1u) / tile_size, tilesY = (height + tile_size - 1u) / tile_size;
+    const uint32_t nt = tilesX * tilesY;
+    std::vector<float> var(nt, 0.f);
+    std::vector<uint32_t> per(nt, 0u);
+    if (extra > 0) {
+        DevBuf dVar;
+        HIPCHK(dVar.alloc(nt * sizeof(float)));
+        launchTileVariance(ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(), width, height, tile_size,
+                           dVar.as<float>(), ctx->stream);
+        HIPCHK(hipMemcpyAsync(var.data(), dVar.p, nt * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        // half the extra samples everywhere, the other half by log(1 + variance) share
+        std::vector<float> logW(nt);
+        float totalW = 0.f;
+        for (uint32_t i = 0; i < nt; ++i) logW[i] = std::log(1.f + var[i]);
+        for (uint32_t i = 0; i < nt; ++i) totalW = totalW + logW[i];
+        const uint32_t uniformHalf = extra / 2u, bonusHalf = extra - uniformHalf;
+        std::vector<ViewJob> jobs;
+        for (uint32_t i = 0; i < nt; ++i) {
+            uint32_t bonus = 0;
+            if (totalW > 1e-6f)
+                bonus = (uint32_t)std::round(logW[i] / totalW * (float)(bonusHalf * nt));
+            per[i] = std::min(uniformHalf + bonus, extra);
+            if (per[i] == 0) continue;
+            const uint32_t x0 = (i % tilesX) * tile_size, y0 = (i / tilesX) * tile_size;
+            const size_t o = (size_t)y0 * width + x0;
+            ViewJob j{ctx->cam, x0, y0, std::min(tile_size, width - x0), std::min(tile_size, height - y0), base, per[i],
+                      ctx->dAccum.as<float4>() + o, ctx->dAccumLsq.as<float>() + o,
+                      ctx->aovOn ? ctx->dAccumAov.as<float>() + 6 * o : nullptr};
+            j.stride = width;
+            jobs.push_back(j);
+        }
+        if (!jobs.empty()) {
+            st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
+            if (st) return st;
+        }
+    }
+    if (extra_per_tile) std::copy(per.begin(), per.end(), extra_per_tile);
+    if (tile_variance) std::copy(var.begin(), var.end(), tile_variance);
     return AHIP_OK;
 }
 

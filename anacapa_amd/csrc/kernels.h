@@ -16,7 +16,9 @@ struct ViewDesc {
     uint32_t tileW, tileH;       // pixel tile that one 64-lane wavefront covers (tileW * tileH = 64)
     uint32_t tilesX;             // tiles across the region
     uint32_t slots;              // paths per sample index: tilesX * tilesY * 64
-    float4* accum;               // region [w*h]: r, g, b sums and weight
+    uint32_t stride;             // pixels per row of the accumulation arrays (w, or the film width
+                                 // when the region is a tile of a frame buffer)
+    float4* accum;               // region [h rows of stride]: r, g, b sums and weight
     float* accumLumSq;           // region [w*h]
     float* accumAov;             // region [w*h*6]: first-hit reflectance and normal sums, or null
 };
@@ -111,6 +113,8 @@ void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, 
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
 void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s);
+void launchTileVariance(const float4* acc, const float* lsq, uint32_t W, uint32_t H, uint32_t tile, float* out,
+                        hipStream_t s);
 void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
                       uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,
                       uint32_t* counts, hipStream_t s);

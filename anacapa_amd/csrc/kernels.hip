@@ -495,7 +495,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveQueues q) {
                 lsq += lum * lum;
             }
         }
-        size_t o = (size_t)ly * V.w + lx;
+        size_t o = (size_t)ly * V.stride + lx;
         float4 a = V.accum[o];
         V.accum[o] = make_float4(a.x + acc.x, a.y + acc.y, a.z + acc.z, a.w + (float)sg.samples);
         V.accumLumSq[o] += lsq;
@@ -512,6 +512,38 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveQueues q) {
             A[3] += sn.x; A[4] += sn.y; A[5] += sn.z;
         }
     }
+}
+
+// --------------------------------------------------------------------------- adaptive sampling
+// Per-tile score of the adaptive pass (RenderSession.cpp:592-600): Film::varianceAt (Film.cpp:80-90,
+// Rec.709 weights on the mean, the ACEScg luminance-squared sum) summed over the tile's pixels in
+// row-major order and divided by the pixel count.  One thread per tile keeps that sequential float
+// order, and every product and sum is rounded on its own (no contraction), so the host can restate
+// the score exactly.
+__global__ __launch_bounds__(64) void k_tile_variance(const float4* acc, const float* lsq, uint32_t W, uint32_t H,
+                                                      uint32_t ts, uint32_t tilesX, uint32_t nTiles, float* out) {
+#pragma clang fp contract(off)
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nTiles) return;
+    const uint32_t x0 = (t % tilesX) * ts, y0 = (t / tilesX) * ts;
+    const uint32_t w = min(ts, W - x0), h = min(ts, H - y0);
+    float sum = 0.f;
+    for (uint32_t y = y0; y < y0 + h; ++y) {
+        const float4* row = acc + (size_t)y * W;
+        const float* lrow = lsq + (size_t)y * W;
+        for (uint32_t x = x0; x < x0 + w; ++x) {
+            const float4 a = row[x];
+            float v = 0.f;
+            if (a.w > 0.f) {
+                const float invW = 1.f / a.w;
+                const float meanL = (a.x * 0.2126f + a.y * 0.7152f + a.z * 0.0722f) * invW;
+                const float eLsq = lrow[x] * invW;
+                v = fmaxf(0.f, eLsq - meanL * meanL);
+            }
+            sum = sum + v;
+        }
+    }
+    out[t] = sum / (float)(w * h);
 }
 
 // --------------------------------------------------------------------------- ray-batch entry points
@@ -638,6 +670,12 @@ void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
     if (blocks > 1024u) blocks = 1024u;
     if (blocks == 0u) return;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launchTileVariance(const float4* acc, const float* lsq, uint32_t W, uint32_t H, uint32_t tile, float* out,
+                        hipStream_t s) {
+    const uint32_t tilesX = (W + tile - 1u) / tile, tilesY = (H + tile - 1u) / tile, n = tilesX * tilesY;
+    if (n == 0u) return;
+    hipLaunchKernelGGL(k_tile_variance, dim3((n + 63u) / 64u), dim3(64), 0, s, acc, lsq, W, H, tile, tilesX, n, out);
 }
 void launchFillF32(float* p, size_t n, float v, hipStream_t s) {
     size_t blocks = (n + 255u) / 256u;

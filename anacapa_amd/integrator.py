@@ -117,6 +117,17 @@ class HipPathIntegrator:
         self._check(self.lib.ahip_render_frame(self._ctx, width or self.width, height or self.height,
                                                sample_start, sample_count))
 
+    def render_adaptive(self, total_spp: int, tile: int = 64, base_spp: int = 0):
+        """RenderSession's adaptive two-pass render on the device (base pass, then the remaining
+        samples by tile variance) into the accumulation buffer; returns (extra spp per tile,
+        tile scores), both [tilesY, tilesX]."""
+        ty, tx = -(-self.height // tile), -(-self.width // tile)
+        extra = np.zeros((ty, tx), np.uint32)
+        var = np.zeros((ty, tx), np.float32)
+        self._check(self.lib.ahip_render_adaptive(self._ctx, self.width, self.height, tile, total_spp, base_spp,
+                                                  A.ptr(extra), A.ptr(var)))
+        return extra, var
+
     def download_accum(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 5), np.float32)
         self._check(self.lib.ahip_download_accum(self._ctx, A.ptr(out)))

@@ -217,6 +217,17 @@ int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, ui
                      uint32_t sample_start, uint32_t sample_count, float* out);
 /* Batched persistent form: many views share one BVH; one graph-captured bounce loop. */
 int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views);
+/* Adaptive two-pass render into the accumulation buffer (the caller clears it first, as
+ * RenderSession does before the base pass).  The reference's GPU path refines uniformly
+ * (RenderSession.cpp:577-586); this restates its CPU tile path instead (RenderSession.cpp:549-653):
+ * base pass [0, base) over the frame (base_spp 0 = min(32, max(16, total/16)), capped at total),
+ * then each tile_size tile (row-major, RenderSession::scheduleTiles) scored by the mean of
+ * Film::varianceAt over its pixels (Film.cpp:80-90), and the remaining samples handed out as
+ * half uniformly plus half by log(1 + score) share, capped at the remainder; the tiles are
+ * rendered as one job list over [base, base + extra_i).  extra_per_tile and tile_variance
+ * ([tiles], may be NULL) receive the allocation and the scores. */
+int ahip_render_adaptive(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t tile_size,
+                         uint32_t total_spp, uint32_t base_spp, uint32_t* extra_per_tile, float* tile_variance);
 
 /* --- denoising AOVs: first-hit reflectance (IMaterial::reflectance) and shading normal (si.n),
  *     captured where PathIntegrator::Li captures them (PathIntegrator.cpp:121-126) and recorded per
