@@ -217,6 +217,26 @@ bool HipPathIntegrator::renderFrame(const SceneView& scene, uint32_t W, uint32_t
     return true;
 }
 
+bool HipPathIntegrator::renderAdaptive(const SceneView& scene, uint32_t W, uint32_t H, uint32_t tileSize,
+                                       uint32_t totalSpp, uint32_t baseSpp, Film& film) {
+    std::lock_guard<std::mutex> lk(m_mutex);
+    if (!m_prepared || !ensureCamera(scene, W, H)) return false;
+    if (ahip_clear_accum(m_ctx) != AHIP_OK) return false;
+    if (ahip_render_adaptive(m_ctx, W, H, tileSize, totalSpp, baseSpp, nullptr, nullptr) != AHIP_OK) return false;
+    std::vector<float> acc((size_t)W * H * 5);
+    if (ahip_download_accum(m_ctx, acc.data()) != AHIP_OK) return false;
+    TileBuffer tb(0, 0, W, H);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const float* p = &acc[5 * ((size_t)y * W + x)];
+            float w = p[3] > 0.f ? p[3] : 1.f;
+            tb.add(x, y, p[0] / w, p[1] / w, p[2] / w, w);
+            tb.addLumSq(x, y, p[4]);
+        }
+    film.mergeTile(tb);
+    return true;
+}
+
 void HipPathIntegrator::renderTile(const SceneView& scene, const TileRequest& tile, uint32_t W, uint32_t H,
                                    ISampler&, TileBuffer& out) {
     std::lock_guard<std::mutex> lk(m_mutex);

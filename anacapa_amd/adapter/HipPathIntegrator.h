@@ -26,6 +26,12 @@ public:
     void renderTile(const SceneView& scene, const TileRequest& tile, uint32_t filmWidth, uint32_t filmHeight,
                     ISampler& sampler, TileBuffer& out) override;
 
+    // RenderSession's adaptive two-pass render (RenderSession.cpp:549-653) in one device call:
+    // base pass, per-tile variance allocation of the remaining samples, refinement of the tiles;
+    // the result is merged into `film`.  Returns false (RenderSession then renders tiles) on failure.
+    bool renderAdaptive(const SceneView& scene, uint32_t filmWidth, uint32_t filmHeight, uint32_t tileSize,
+                        uint32_t totalSpp, uint32_t baseSpp, Film& film);
+
     bool isValid() const;
     void clearAccum();
     std::string name() const;
