@@ -142,6 +142,10 @@ SIGNATURES = {
     "ahip_render_tile": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                    C.c_uint32, C.c_uint32, VP]),
     "ahip_render_batch": (C.c_int, [VP, C.POINTER(View), C.c_uint32]),
+    "ahip_resolve": (C.c_int, [VP, C.c_float, VP, VP, VP, VP, VP]),
+    "ahip_write_png": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, VP]),
+    "ahip_write_exr": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_char_p),
+                                 C.POINTER(VP)]),
     "ahip_render_adaptive": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP]),
     "ahip_set_aovs": (C.c_int, [VP, C.c_int]),
     "ahip_download_aovs": (C.c_int, [VP, VP]),

@@ -773,6 +773,35 @@ int ahip_render_adaptive(ahip_context* ctx, uint32_t width, uint32_t height, uin
     return AHIP_OK;
 }
 
+int ahip_resolve(ahip_context* ctx, float exposure, uint8_t* srgb8, float* rgb, float* variance, float* albedo,
+                 float* normal) {
+    ENTER();
+    if (!ctx->dAccum.p) return fail(ctx, AHIP_E_NOT_PREPARED, "nothing rendered");
+    if ((albedo || normal) && !ctx->dAccumAov.p)
+        return fail(ctx, AHIP_E_INVALID_ARG, "AOVs were not accumulated (ahip_set_aovs before rendering)");
+    const size_t n = (size_t)ctx->accW * ctx->accH;
+    DevBuf dS, dR, dV, dA, dN;
+    if (srgb8) HIPCHK(dS.alloc(n * 3));
+    if (rgb) HIPCHK(dR.alloc(n * 3 * sizeof(float)));
+    if (variance) HIPCHK(dV.alloc(n * sizeof(float)));
+    if (albedo) HIPCHK(dA.alloc(n * 3 * sizeof(float)));
+    if (normal) HIPCHK(dN.alloc(n * 3 * sizeof(float)));
+    const float evScale = std::pow(2.f, exposure);
+    launchResolve(ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(), ctx->dAccumAov.as<float>(), n, evScale,
+                  dS.as<uint8_t>(), dR.as<float>(), dV.as<float>(), dA.as<float>(), dN.as<float>(), ctx->stream);
+    HIPCHK(hipGetLastError());
+    auto get = [&](const DevBuf& d, void* host, size_t bytes) -> hipError_t {
+        return host ? hipMemcpyAsync(host, d.p, bytes, hipMemcpyDeviceToHost, ctx->stream) : hipSuccess;
+    };
+    HIPCHK(get(dS, srgb8, n * 3));
+    HIPCHK(get(dR, rgb, n * 3 * sizeof(float)));
+    HIPCHK(get(dV, variance, n * sizeof(float)));
+    HIPCHK(get(dA, albedo, n * 3 * sizeof(float)));
+    HIPCHK(get(dN, normal, n * 3 * sizeof(float)));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AHIP_OK;
+}
+
 static int downloadRegion(ahip_context* ctx, const DevBuf& acc, const DevBuf& lsq, size_t n, float* out) {
     std::vector<float> a(n * 4), l(n);
     HIPCHK(hipMemcpy(a.data(), acc.p, n * sizeof(float4), hipMemcpyDeviceToHost));

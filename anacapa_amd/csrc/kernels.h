@@ -113,6 +113,8 @@ void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, 
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s);
 void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s);
+void launchResolve(const float4* acc, const float* lsq, const float* aov, size_t n, float evScale, uint8_t* srgb,
+                   float* rgb, float* var, float* alb, float* nrm, hipStream_t s);
 void launchTileVariance(const float4* acc, const float* lsq, uint32_t W, uint32_t H, uint32_t tile, float* out,
                         hipStream_t s);
 void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,

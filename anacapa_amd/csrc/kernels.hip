@@ -546,6 +546,53 @@ __global__ __launch_bounds__(64) void k_tile_variance(const float4* acc, const f
     out[t] = sum / (float)(w * h);
 }
 
+// Film resolve of the accumulation (PixelAccumulator::resolve, Film.h:46-55; Film::varianceAt,
+// Film.cpp:80-90; writePNG's exposure, clamp and sRGB encoding, Film.cpp:233-256; the AOV means of
+// PathIntegrator::renderTile, PathIntegrator.cpp:58-66) in one pass over the buffers, each output
+// optional.  Contraction is off: every product and sum rounds as written.
+__global__ __launch_bounds__(256) void k_resolve(const float4* acc, const float* lsq, const float* aov, size_t n,
+                                                 float evScale, uint8_t* srgb, float* rgb, float* var, float* alb,
+                                                 float* nrm) {
+#pragma clang fp contract(off)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 a = acc[i];
+        const bool has = a.w > 0.f;
+        const float invW = has ? 1.f / a.w : 0.f;
+        const float c[3] = {a.x * invW, a.y * invW, a.z * invW};
+        if (rgb)
+            for (int k = 0; k < 3; ++k) rgb[3 * i + k] = c[k];
+        if (var) {
+            float v = 0.f;
+            if (has) {
+                const float meanL = (a.x * 0.2126f + a.y * 0.7152f + a.z * 0.0722f) * invW;
+                v = fmaxf(0.f, lsq[i] * invW - meanL * meanL);
+            }
+            var[i] = v;
+        }
+        if (srgb)
+            for (int k = 0; k < 3; ++k) {
+                const float x = fmaxf(0.f, fminf(1.f, c[k] * evScale));
+                const float e = x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+                srgb[3 * i + k] = (uint8_t)(e * 255.f + 0.5f);
+            }
+        if (aov && (alb || nrm)) {
+            const float* s = aov + 6 * i;
+            float m[6];
+            for (int k = 0; k < 6; ++k) m[k] = s[k] * invW;
+            if (alb)
+                for (int k = 0; k < 3; ++k) alb[3 * i + k] = m[k];
+            if (nrm) {
+                const float len = sqrtf(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+                if (len > 1e-6f) {
+                    const float il = 1.f / len;
+                    m[3] = m[3] * il; m[4] = m[4] * il; m[5] = m[5] * il;
+                }
+                for (int k = 0; k < 3; ++k) nrm[3 * i + k] = m[3 + k];
+            }
+        }
+    }
+}
+
 // --------------------------------------------------------------------------- ray-batch entry points
 template <uint32_t BS, uint32_t K, bool ANY>
 __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays, uint32_t n, ahip_hit* hits,
@@ -676,6 +723,13 @@ void launchTileVariance(const float4* acc, const float* lsq, uint32_t W, uint32_
     const uint32_t tilesX = (W + tile - 1u) / tile, tilesY = (H + tile - 1u) / tile, n = tilesX * tilesY;
     if (n == 0u) return;
     hipLaunchKernelGGL(k_tile_variance, dim3((n + 63u) / 64u), dim3(64), 0, s, acc, lsq, W, H, tile, tilesX, n, out);
+}
+void launchResolve(const float4* acc, const float* lsq, const float* aov, size_t n, float evScale, uint8_t* srgb,
+                   float* rgb, float* var, float* alb, float* nrm, hipStream_t s) {
+    size_t blocks = std::min<size_t>((n + 255u) / 256u, 8192u);
+    if (blocks == 0u) return;
+    hipLaunchKernelGGL(k_resolve, dim3((uint32_t)blocks), dim3(256), 0, s, acc, lsq, aov, n, evScale, srgb, rgb, var,
+                       alb, nrm);
 }
 void launchFillF32(float* p, size_t n, float v, hipStream_t s) {
     size_t blocks = (n + 255u) / 256u;

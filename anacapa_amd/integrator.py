@@ -128,6 +128,25 @@ class HipPathIntegrator:
                                                   A.ptr(extra), A.ptr(var)))
         return extra, var
 
+    def resolve(self, exposure=0.0, srgb8=True, rgb=True, variance=False, aovs=False):
+        """Device resolve of the accumulation: dict of the requested outputs (srgb8 [h,w,3] u8,
+        rgb [h,w,3], variance [h,w], albedo / normal [h,w,3])."""
+        h, w = self.height, self.width
+        out = {}
+        if srgb8:
+            out["srgb8"] = np.zeros((h, w, 3), np.uint8)
+        if rgb:
+            out["rgb"] = np.zeros((h, w, 3), np.float32)
+        if variance:
+            out["variance"] = np.zeros((h, w), np.float32)
+        if aovs:
+            out["albedo"] = np.zeros((h, w, 3), np.float32)
+            out["normal"] = np.zeros((h, w, 3), np.float32)
+        self._check(self.lib.ahip_resolve(self._ctx, exposure, A.ptr(out.get("srgb8")), A.ptr(out.get("rgb")),
+                                          A.ptr(out.get("variance")), A.ptr(out.get("albedo")),
+                                          A.ptr(out.get("normal"))))
+        return out
+
     def download_accum(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 5), np.float32)
         self._check(self.lib.ahip_download_accum(self._ctx, A.ptr(out)))
@@ -240,3 +259,26 @@ def resolve(accum: np.ndarray) -> np.ndarray:
     """Film pixel value = pre-weighted sum / weight (PixelAccumulator::resolve, Film.h:46-55)."""
     w = accum[..., 3:4]
     return np.where(w > 0, accum[..., :3] / np.maximum(w, 1e-30), 0.0)
+
+
+def write_png(path: str, srgb8: np.ndarray, lib: C.CDLL | None = None):
+    """8-bit RGB PNG of a resolved film ([h, w, 3] uint8), renamed into place when complete."""
+    lib = lib or A.load_library()
+    a = np.ascontiguousarray(srgb8, np.uint8)
+    st = lib.ahip_write_png(str(path).encode(), a.shape[1], a.shape[0], A.ptr(a))
+    if st != A.AHIP_OK:
+        raise BackendError(st, f"cannot write {path}")
+
+
+def write_exr(path: str, layers: dict, lib: C.CDLL | None = None):
+    """Float OpenEXR of RGB layers {prefix: [h, w, 3]} ('' = beauty, 'albedo.', 'normals.')."""
+    lib = lib or A.load_library()
+    names = list(layers)
+    arrs = [np.ascontiguousarray(layers[k], np.float32) for k in names]
+    h, w = arrs[0].shape[:2]
+    assert all(a.shape == (h, w, 3) for a in arrs)
+    pre = (C.c_char_p * len(names))(*[k.encode() for k in names])
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    st = lib.ahip_write_exr(str(path).encode(), w, h, len(names), pre, ptrs)
+    if st != A.AHIP_OK:
+        raise BackendError(st, f"cannot write {path}")

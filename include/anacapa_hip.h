@@ -215,6 +215,20 @@ int ahip_download_accum(ahip_context* ctx, float* out);
 /* Render one tile into out[w*h*5] with the same layout (CudaPathIntegrator::renderTile, .cu:449-489). */
 int ahip_render_tile(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                      uint32_t sample_start, uint32_t sample_count, float* out);
+/* Resolve the accumulation buffer on the device, each output optional ([w*h*3] or [w*h]):
+ * srgb8 — Film::writePNG's pixels (exposure 2^exposure, clamp, sRGB curve, 8 bits; Film.cpp:233-256);
+ * rgb — PixelAccumulator::resolve (sums / weight, Film.h:46-55); variance — Film::varianceAt
+ * (Film.cpp:80-90); albedo, normal — the first-hit AOV means (normal scaled to unit length), which
+ * need ahip_set_aovs before rendering.  Only the requested outputs cross the bus. */
+int ahip_resolve(ahip_context* ctx, float exposure, uint8_t* srgb8, float* rgb, float* variance, float* albedo,
+                 float* normal);
+/* Image writers for resolved buffers (host only), in place of the reference's OpenImageIO output:
+ * ahip_write_png — 8-bit RGB, written beside the target and renamed into place (Film.cpp:257-275);
+ * ahip_write_exr — float RGB layers, channels "<prefix>R/G/B" (the beauty with prefix "", AOVs as
+ * "albedo." / "normals.", Film.cpp:158-228), uncompressed scanlines. */
+int ahip_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* rgb);
+int ahip_write_exr(const char* path, uint32_t width, uint32_t height, uint32_t num_layers,
+                   const char* const* layer_prefixes, const float* const* layers);
 /* Batched persistent form: many views share one BVH; one graph-captured bounce loop. */
 int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views);
 /* Adaptive two-pass render into the accumulation buffer (the caller clears it first, as
