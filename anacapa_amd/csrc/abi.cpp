@@ -111,6 +111,8 @@ struct ahip_context {
     hipEvent_t renderBeg = nullptr, renderEnd = nullptr;
     bool lastRenderValid = false;
     bool aovOn = false;          // ahip_set_aovs
+    bool accumHasSamples = false; // the frame accumulation holds samples since the last clear
+    bool aovPartial = false;     // AOV sums started after samples were already accumulated
     DevBuf qAovA, qAovN;         // per-path first-hit AOVs (allocated when a render records them)
     uint32_t qAovCap = 0;
     uint32_t lastW = 0, lastH = 0, lastS0 = 0, lastNs = 0;
@@ -652,6 +654,8 @@ int ahip_clear_accum(ahip_context* ctx) {
         if (ctx->dAccumAov.p) launchFillF32(ctx->dAccumAov.as<float>(), (size_t)ctx->accW * ctx->accH * 6, 0.f, ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
+    ctx->accumHasSamples = false;
+    ctx->aovPartial = false;
     return AHIP_OK;
 }
 
@@ -688,6 +692,7 @@ static int renderFrameImpl(ahip_context* ctx, uint32_t width, uint32_t height, u
                                       ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
     st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
     if (st) return st;
+    ctx->accumHasSamples = ctx->accumHasSamples || sample_count > 0;
     ctx->lastRenderValid = true;
     ctx->lastW = width; ctx->lastH = height; ctx->lastS0 = sample_start; ctx->lastNs = sample_count;
     return AHIP_OK;
@@ -710,9 +715,13 @@ static int ensureFrameAccum(ahip_context* ctx, uint32_t width, uint32_t height) 
         ctx->accH = height;
         launchFillF32(ctx->dAccum.as<float>(), (size_t)width * height * 4, 0.f, ctx->stream);
         launchFillF32(ctx->dAccumLsq.as<float>(), (size_t)width * height, 0.f, ctx->stream);
+        ctx->accumHasSamples = false;
+        ctx->aovPartial = false;
     }
     if (ctx->aovOn && !ctx->dAccumAov.p) {
-        // AOV sums start at zero when turned on; the radiance sums already hold earlier samples
+        // AOV sums start at zero when turned on; if the radiance sums already hold earlier samples,
+        // the AOV sums cover fewer samples than the weight until the next clear
+        ctx->aovPartial = ctx->accumHasSamples;
         HIPCHK(ctx->dAccumAov.alloc((size_t)width * height * 6 * sizeof(float)));
         launchFillF32(ctx->dAccumAov.as<float>(), (size_t)width * height * 6, 0.f, ctx->stream);
     }
@@ -779,6 +788,8 @@ int ahip_resolve(ahip_context* ctx, float exposure, uint8_t* srgb8, float* rgb, 
     if (!ctx->dAccum.p) return fail(ctx, AHIP_E_NOT_PREPARED, "nothing rendered");
     if ((albedo || normal) && !ctx->dAccumAov.p)
         return fail(ctx, AHIP_E_INVALID_ARG, "AOVs were not accumulated (ahip_set_aovs before rendering)");
+    if ((albedo || normal) && ctx->aovPartial)
+        return fail(ctx, AHIP_E_INVALID_ARG, "AOVs were turned on after samples were accumulated (clear first)");
     const size_t n = (size_t)ctx->accW * ctx->accH;
     DevBuf dS, dR, dV, dA, dN;
     if (srgb8) HIPCHK(dS.alloc(n * 3));

@@ -51,3 +51,25 @@ def test_resolve_matches_film_formulas(gpu, tmp_path):
         assert np.array_equal(np.asarray(im), res["srgb8"])
     write_exr(tmp_path / "f.exr", {"": res["rgb"], "albedo.": res["albedo"], "normals.": res["normal"]})
     assert (tmp_path / "f.exr").stat().st_size > 64 * 48 * 9 * 4
+
+
+def test_resolve_refuses_aovs_started_late(gpu):
+    """AOV sums begun after radiance samples were accumulated cover fewer samples than the weight:
+    the AOV means are refused until the next clear."""
+    from anacapa_amd.integrator import BackendError
+    sc = scenes.cornell(width=32, height=32)
+    gpu.prepare(sc)
+    gpu.clear_accum()
+    gpu.render_frame(0, 2)
+    gpu.set_aovs(True)
+    try:
+        gpu.render_frame(2, 2)
+        with pytest.raises(BackendError):
+            gpu.resolve(aovs=True)
+        assert "rgb" in gpu.resolve(aovs=False)
+        gpu.clear_accum()
+        gpu.render_frame(0, 2)
+        out = gpu.resolve(aovs=True)
+        assert out["albedo"].max() > 0
+    finally:
+        gpu.set_aovs(False)
