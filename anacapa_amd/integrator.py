@@ -121,11 +121,12 @@ class HipPathIntegrator:
         """RenderSession's adaptive two-pass render on the device (base pass, then the remaining
         samples by tile variance) into the accumulation buffer; returns (extra spp per tile,
         tile scores), both [tilesY, tilesX]."""
-        ty, tx = -(-self.height // tile), -(-self.width // tile)
+        ty, tx = (-(-self.height // tile), -(-self.width // tile)) if tile > 0 else (0, 0)
         extra = np.zeros((ty, tx), np.uint32)
         var = np.zeros((ty, tx), np.float32)
-        self._check(self.lib.ahip_render_adaptive(self._ctx, self.width, self.height, tile, total_spp, base_spp,
-                                                  A.ptr(extra), A.ptr(var)))
+        self._check(self.lib.ahip_render_adaptive(self._ctx, self.width, self.height, max(tile, 0), total_spp,
+                                                  base_spp, A.ptr(extra) if extra.size else None,
+                                                  A.ptr(var) if var.size else None))
         return extra, var
 
     def resolve(self, exposure=0.0, srgb8=True, rgb=True, variance=False, aovs=False):

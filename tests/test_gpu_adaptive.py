@@ -114,3 +114,22 @@ def test_adaptive_with_aovs_and_explicit_base(gpu):
     extra, var = gpu.render_adaptive(4, tile=16, base_spp=4)
     assert not extra.any() and not var.any()
     assert np.all(gpu.download_accum()[..., 3] == 4)
+
+
+def test_adaptive_and_resolve_argument_errors(gpu):
+    from anacapa_amd.integrator import BackendError
+    from anacapa_amd import _abi as A
+    sc = scenes.cornell(width=32, height=32)
+    gpu.prepare(sc)
+    with pytest.raises(BackendError) as e:
+        gpu.render_adaptive(8, tile=0)
+    assert e.value.status == A.AHIP_E_INVALID_ARG
+    with pytest.raises(BackendError) as e:
+        gpu.render_adaptive(0, tile=16)
+    assert e.value.status == A.AHIP_E_INVALID_ARG
+    gpu.set_aovs(False)
+    gpu.clear_accum()
+    gpu.render_frame(0, 1)
+    with pytest.raises(BackendError) as e:                      # AOV means need ahip_set_aovs first
+        gpu.resolve(aovs=True)
+    assert e.value.status == A.AHIP_E_INVALID_ARG
