@@ -93,6 +93,7 @@ struct ahip_context {
 
     // persistent accumulation
     DevBuf dAccum, dAccumLsq, dAccumAov;   // persistent frame sums (AOVs when aovOn)
+    DevBuf dResolve[5];                    // ahip_resolve outputs, kept for the next call (preview loops)
     uint32_t accW = 0, accH = 0;
 
     // wavefront queues
@@ -791,7 +792,8 @@ int ahip_resolve(ahip_context* ctx, float exposure, uint8_t* srgb8, float* rgb, 
     if ((albedo || normal) && ctx->aovPartial)
         return fail(ctx, AHIP_E_INVALID_ARG, "AOVs were turned on after samples were accumulated (clear first)");
     const size_t n = (size_t)ctx->accW * ctx->accH;
-    DevBuf dS, dR, dV, dA, dN;
+    DevBuf &dS = ctx->dResolve[0], &dR = ctx->dResolve[1], &dV = ctx->dResolve[2], &dA = ctx->dResolve[3],
+           &dN = ctx->dResolve[4];
     if (srgb8) HIPCHK(dS.alloc(n * 3));
     if (rgb) HIPCHK(dR.alloc(n * 3 * sizeof(float)));
     if (variance) HIPCHK(dV.alloc(n * sizeof(float)));
@@ -799,7 +801,9 @@ int ahip_resolve(ahip_context* ctx, float exposure, uint8_t* srgb8, float* rgb, 
     if (normal) HIPCHK(dN.alloc(n * 3 * sizeof(float)));
     const float evScale = std::pow(2.f, exposure);
     launchResolve(ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(), ctx->dAccumAov.as<float>(), n, evScale,
-                  dS.as<uint8_t>(), dR.as<float>(), dV.as<float>(), dA.as<float>(), dN.as<float>(), ctx->stream);
+                  srgb8 ? dS.as<uint8_t>() : nullptr, rgb ? dR.as<float>() : nullptr,
+                  variance ? dV.as<float>() : nullptr, albedo ? dA.as<float>() : nullptr,
+                  normal ? dN.as<float>() : nullptr, ctx->stream);
     HIPCHK(hipGetLastError());
     auto get = [&](const DevBuf& d, void* host, size_t bytes) -> hipError_t {
         return host ? hipMemcpyAsync(host, d.p, bytes, hipMemcpyDeviceToHost, ctx->stream) : hipSuccess;
