@@ -98,6 +98,8 @@ struct ahip_context {
 
     // wavefront queues
     DevBuf qRo[2], qRd[2], qBeta[2], qHit, qHitAux, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
+    DevBuf qCList, qCState, qCStack;   // curve scenes: the resumable curve-tree walk
+    uint32_t qCurveCap = 0, qCurveStack = 0;
     DevBuf qPix, qPassIdx, dViews, dTables, dBatchAcc, dBatchLsq;
     uint32_t tileW = 8;          // pixel tile per wavefront: tileW x (64 / tileW)
     bool frameGraph = false;     // render_frame: per-launch timing events (the roofline figure needs them)
@@ -174,6 +176,18 @@ EventPair& nextEvent(ahip_context* ctx, int kind) {
 
 int ensureQueues(ahip_context* ctx, uint32_t cap) {
     uint32_t slots = (ctx->params.max_depth + 2u) * 4u;
+    if (!ctx->curves.nodes.empty()) {
+        // a walk holds at most one pending right child per level below the root, plus the child
+        // being entered
+        const uint32_t pc = std::max(cap, ctx->qCap), sc = ctx->curves.maxDepth + 2u;
+        if (ctx->qCurveCap != pc || ctx->qCurveStack != sc || !ctx->qCList.p) {
+            HIPCHK(ctx->qCList.alloc((size_t)pc * kCurveList * sizeof(uint32_t)));
+            HIPCHK(ctx->qCState.alloc((size_t)pc * sizeof(uint32_t)));
+            HIPCHK(ctx->qCStack.alloc((size_t)pc * sc * sizeof(uint32_t)));
+            ctx->qCurveCap = pc;
+            ctx->qCurveStack = sc;
+        }
+    }
     if (cap <= ctx->qCap && slots <= ctx->qCtrSlots && ctx->qOvfRef.p) return AHIP_OK;
     cap = std::max(cap, ctx->qCap);
     size_t f4 = (size_t)cap * sizeof(float4);
@@ -235,6 +249,12 @@ WaveQueues queues(ahip_context* ctx) {
     q.ovfT = ctx->qOvfT.as<float>();
     q.stackCap = ctx->stackCap;
     q.cap = ctx->qCap;
+    if (!ctx->curves.nodes.empty() && ctx->qCurveCap == ctx->qCap) {
+        q.cList = ctx->qCList.as<uint32_t>();
+        q.cState = ctx->qCState.as<uint32_t>();
+        q.cStack = ctx->qCStack.as<uint32_t>();
+        q.cStackCap = ctx->qCurveStack;
+    }
     return q;
 }
 

@@ -66,7 +66,17 @@ struct WaveQueues {
     uint32_t* passIdx;   // index of the pass being run; advanced by the pass's last kernel
     uint32_t stackCap;   // total stack entries available per thread
     uint32_t cap;        // path capacity
+    // curve scenes (null otherwise): the resumable curve-tree walk of the closest-hit query
+    uint32_t* cList;     // [kCurveList][cap] records listed for the next wave test, entry-major
+    uint32_t* cState;    // [cap] listed count | saved stack depth << 8
+    uint32_t* cStack;    // [cStackCap][cap] saved walk stack (node indices), entry-major
+    uint32_t cStackCap;
 };
+
+// Curve records a walk lists before a wave tests them; walk-and-test rounds before the rest of the
+// walk runs per ray.
+constexpr uint32_t kCurveList = 64;
+constexpr uint32_t kCurveRounds = 8;
 
 // Detached path-replay estimator (adjoint.hip): one thread per (pixel, sample) over the film.
 constexpr uint32_t kRecFields = 18;   // floats per differentiable vertex record

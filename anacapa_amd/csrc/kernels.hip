@@ -18,6 +18,7 @@
 // count comes from a device counter, so launch shapes are static and the host never reads a
 // queue length (the same launches can be replayed from a HIP graph).
 #include <hip/hip_runtime.h>
+#include <cstdlib>
 #include <stdint.h>
 
 #include "../../include/anacapa_hip.h"
@@ -394,6 +395,268 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
     }
 }
 
+// --------------------------------------------------------------------------- queries, curve scenes
+// Through dense hair almost all of a query is spent in the segment tests (the subdivision of
+// CurveBrute::subdivide), and one ray's tests can outnumber its neighbours' many times over, so a
+// wave walking 64 rays keeps few lanes busy.  In curve scenes the three queries of a bounce (the
+// paths' closest hit, the shadow rays' any hit, the BSDF-sampled rays' closest or any hit) run their
+// curve part in rounds: k_curve_walk walks a ray's curve tree (after its triangle tree, on the first
+// round) with the node test bounded by the nearest hit so far and lists up to kCurveList leaf
+// records in traversal order, saving its stack; k_curve_test spreads one ray's list over a wave,
+// one record per lane.  After kCurveRounds rounds k_curve_finish gives each unfinished ray a wave of
+// its own that repeats list-and-test until its walk ends.  Results are left in q.hit for the
+// queue's own resolve kernel.
+// The result is the reference's (CurveBrute.cpp:398-440).  Any hit: a record's test under tMax
+// does not depend on the order of tests, and the walk culls with tMax only until the first hit.
+// Closest hit: the reference's walk visits records in increasing record order (the builder lays
+// leaves out depth-first, left before right) and keeps a hit only if strictly nearer, so it returns
+// the least (t, record) over all accepted chords; its box and hull culls are conservative (a culled
+// box or sub-curve holds no chord nearer than the bound it was culled with), so testing a list
+// under the bound of its round, larger than the running one, finds the same records at the same t,
+// and the wave keeps the least (t, record).
+enum : uint32_t { kQPath = 0u, kQShadow = 1u, kQMis = 2u };   // counter slot of each queue
+
+template <uint32_t Q>
+__device__ __forceinline__ TRay curveQueryRay(const WaveQueues& q, uint32_t cur, uint32_t i, bool& dead) {
+    float4 a, b;
+    float tMax;
+    if (Q == kQPath) {
+        a = q.ro[cur][i];
+        b = q.rd[cur][i];
+        tMax = __int_as_float(0x7f800000);
+        dead = (__float_as_uint(b.w) & kDeadPath) != 0u;
+    } else if (Q == kQShadow) {
+        a = q.so[i];
+        b = q.sd[i];
+        tMax = b.w;
+        dead = false;
+    } else {
+        a = q.mo[i];
+        b = q.md[i];
+        tMax = b.w;
+        dead = false;
+    }
+    return makeTRay(v3(a), v3(b), 1e-4f, tMax);
+}
+__device__ __forceinline__ float hitBound(const float4& h, float tMax) {
+    return __float_as_uint(h.w) != ~0u ? h.x : tMax;
+}
+__device__ __forceinline__ uint64_t waveMin(uint64_t m) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        m = o < m ? o : m;
+    }
+    return m;
+}
+
+template <uint32_t BS, uint32_t K, uint32_t Q, bool ANY, bool FIRST>
+__global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, Q);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        const uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        bool dead;
+        const TRay r = curveQueryRay<Q>(q, cur, i, dead);
+        uint32_t sp;
+        float bound;
+        if (FIRST) {
+            if (dead) {
+                q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(~0u));
+                q.cState[i] = 0u;
+                continue;
+            }
+            float t = 0.f, u = 0.f, v = 0.f;
+            const uint32_t h = traverse<ANY, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+            q.hit[i] = make_float4(t, u, v, __uint_as_float(h));
+            if (ANY && h != ~0u) {   // occluded by a triangle: no curve walk
+                q.cState[i] = 0u;
+                continue;
+            }
+            bound = h != ~0u ? t : r.tMax;
+            q.cStack[i] = 0u;   // the curve root
+            sp = 1u;
+        } else {
+            sp = q.cState[i] >> 8;
+            if (sp == 0u) {
+                q.cState[i] = 0u;
+                continue;
+            }
+            bound = hitBound(q.hit[i], r.tMax);
+        }
+        const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
+        uint32_t n = 0u;
+        while (sp > 0u) {
+            const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)(sp - 1u) * q.cap + i]);
+            const bool leaf = (nd.rc & 0x80000000u) != 0u;
+            if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;   // the list is full: resume here
+            --sp;
+            if (!curveBox(nd, r, invD, bound)) continue;
+            if (leaf) {
+                const uint32_t cnt = nd.rc & 0x7fffffffu;
+                for (uint32_t k = 0; k < cnt; ++k, ++n) q.cList[(size_t)n * q.cap + i] = nd.lp + k;
+            } else {
+                if (sp + 2u > q.cStackCap) {
+                    *ctr(q, bounce, 3) = 1u;
+                    sp = 0u;
+                    break;
+                }
+                q.cStack[(size_t)sp * q.cap + i] = nd.rc;
+                q.cStack[(size_t)(sp + 1u) * q.cap + i] = nd.lp;
+                sp += 2u;
+            }
+        }
+        q.cState[i] = n | (sp << 8);
+    }
+}
+
+template <uint32_t BS, uint32_t Q, bool ANY>
+__global__ __launch_bounds__(BS) void k_curve_test(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+    const uint32_t count = *ctr(q, bounce, Q);
+    const uint32_t lane = __lane_id();
+    const uint32_t waves = gridDim.x * (BS / 64u);
+    for (uint32_t i = blockIdx.x * (BS / 64u) + threadIdx.x / 64u; i < count; i += waves) {
+        const uint32_t st = q.cState[i], n = st & 0xffu;
+        if (n == 0u) continue;
+        bool dead;
+        const TRay r = curveQueryRay<Q>(q, cur, i, dead);
+        V3 fx, fy;
+        onbNC(r.d, fx, fy);
+        uint64_t key = ~0ull;
+        float tSeg = 0.f;
+        V3 tang{0.f, 0.f, 0.f};
+        if (lane < n) {
+            const uint32_t rec = q.cList[(size_t)lane * q.cap + i];
+            float best = hitBound(q.hit[i], r.tMax), ts;
+            V3 tg;
+            if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                key = ((uint64_t)__float_as_uint(best) << 32) | rec;   // t > 0: bits order as floats
+                tSeg = ts;
+                tang = tg;
+            }
+        }
+        const uint64_t m = waveMin(key);
+        if (m == ~0ull) continue;
+        if (ANY) {
+            if (lane == 0u) {
+                q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
+                q.cState[i] = 0u;   // occluded: the walk is over
+            }
+        } else if (key == m) {   // records are listed once: one lane holds the least key
+            q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
+                                   __uint_as_float(kCurveHit | (uint32_t)m));
+            q.hitAux[i] = f4(tang, 0.f);
+        }
+    }
+}
+
+// One wave per unfinished ray: the walk runs in every lane alike (same loads, same branches) and
+// lane k takes the k-th listed record.
+template <uint32_t BS, uint32_t Q, bool ANY>
+__global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+    const uint32_t count = *ctr(q, bounce, Q);
+    const uint32_t lane = __lane_id();
+    const uint32_t waves = gridDim.x * (BS / 64u);
+    for (uint32_t i = blockIdx.x * (BS / 64u) + threadIdx.x / 64u; i < count; i += waves) {
+        uint32_t sp = q.cState[i] >> 8;
+        if (sp == 0u) continue;
+        bool dead;
+        const TRay r = curveQueryRay<Q>(q, cur, i, dead);
+        float bound = hitBound(q.hit[i], r.tMax);
+        V3 fx, fy;
+        onbNC(r.d, fx, fy);
+        const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};
+        uint64_t key = ~0ull, m = ~0ull;
+        float tSeg = 0.f;
+        V3 tang{0.f, 0.f, 0.f};
+        while (sp > 0u) {
+            uint32_t n = 0u, rec = ~0u;
+            while (sp > 0u) {
+                const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)(sp - 1u) * q.cap + i]);
+                const bool leaf = (nd.rc & 0x80000000u) != 0u;
+                if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;
+                --sp;
+                if (!curveBox(nd, r, invD, bound)) continue;
+                if (leaf) {
+                    const uint32_t cnt = nd.rc & 0x7fffffffu;
+                    if (lane - n < cnt) rec = nd.lp + (lane - n);   // lane in [n, n + cnt)
+                    n += cnt;
+                } else {
+                    if (sp + 2u > q.cStackCap) {
+                        *ctr(q, bounce, 3) = 1u;
+                        sp = 0u;
+                        break;
+                    }
+                    q.cStack[(size_t)sp * q.cap + i] = nd.rc;   // every lane stores the same entry
+                    q.cStack[(size_t)(sp + 1u) * q.cap + i] = nd.lp;
+                    sp += 2u;
+                }
+            }
+            if (n == 0u) continue;
+            if (rec != ~0u) {
+                float best = bound, ts;
+                V3 tg;
+                if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                    key = ((uint64_t)__float_as_uint(best) << 32) | rec;
+                    tSeg = ts;
+                    tang = tg;
+                }
+            }
+            m = waveMin(key);
+            if (m != ~0ull) {
+                if (ANY) break;
+                bound = __uint_as_float((uint32_t)(m >> 32));
+            }
+        }
+        if (m == ~0ull) continue;
+        if (ANY) {
+            if (lane == 0u) q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
+        } else if (key == m) {
+            q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
+                                   __uint_as_float(kCurveHit | (uint32_t)m));
+            q.hitAux[i] = f4(tang, 0.f);
+        }
+    }
+}
+
+// The shadow and BSDF-sampled queues' results, as k_shadow and k_mis apply them.
+__global__ __launch_bounds__(256) void k_shadow_resolve(WaveQueues q, uint32_t bounce) {
+    const uint32_t count = *ctr(q, bounce, kQShadow);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        if (__float_as_uint(q.hit[i].w) != ~0u) continue;
+        const uint32_t p0 = __float_as_uint(q.so[i].w);
+        const float4 c = q.sc[i], l = q.L[p0];
+        q.L[p0] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+    }
+}
+template <bool EMIT>
+__global__ __launch_bounds__(256) void k_mis_resolve(DScene sc, WaveQueues q, uint32_t bounce) {
+    const uint32_t count = *ctr(q, bounce, kQMis);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t h = __float_as_uint(q.hit[i].w);
+        const V3 d = v3(q.md[i]);
+        V3 Li{0.f, 0.f, 0.f};
+        if (h == ~0u) Li = envLe(sc, d);
+        else if (EMIT) {
+            const uint32_t mesh = (h & kCurveHit)
+                ? __float_as_uint(sc.strandInfo[4u * (size_t)sc.segIds[2u * (h & ~kCurveHit)] + 3u])
+                : sc.tris[h].meshID;
+            const int32_t mi = mesh < sc.numMeshes ? sc.meshMat[mesh] : -1;
+            if (mi >= 0 && sc.mats[mi].hasLe) Li = ld3(sc.mats[mi].Le);
+        }
+        if (!isBlack(Li)) {
+            const uint32_t p0 = __float_as_uint(q.mo[i].w);
+            const float4 c = q.mc[i], l = q.L[p0];
+            const V3 add = v3(c) * Li;
+            q.L[p0] = make_float4(l.x + add.x, l.y + add.y, l.z + add.z, 0.f);
+        }
+    }
+}
+
 // --------------------------------------------------------------------------- shadow (any hit)
 template <uint32_t BS, uint32_t K, bool CURVES>
 __global__ __launch_bounds__(BS) void k_shadow(DScene sc, WaveQueues q, uint32_t bounce) {
@@ -748,9 +1011,29 @@ uint32_t trace_block_size() { return kTraceBS; }
 void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, rc, q);
 }
+template <uint32_t Q, bool ANY>
+void launchCurveQuery(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
+                      hipStream_t s) {
+    hipLaunchKernelGGL((k_curve_walk<kTraceBS, kStackK, Q, ANY, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    hipLaunchKernelGGL((k_curve_test<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    for (uint32_t k = 1; k < kCurveRounds; ++k) {
+        hipLaunchKernelGGL((k_curve_walk<kTraceBS, kStackK, Q, ANY, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+        hipLaunchKernelGGL((k_curve_test<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    }
+    hipLaunchKernelGGL((k_curve_finish<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+}
+// The round-based curve queries need their buffers (q.cList); AHIP_DEBUG_WHOLE_RAY=1 selects the
+// whole-ray kernels instead, for comparisons.
+bool curveRounds(const DScene& sc, const WaveQueues& q) {
+    const char* dbg = getenv("AHIP_DEBUG_WHOLE_RAY");
+    return sc.hasCurves && q.cList && !(dbg && dbg[0] == '1');
+}
+
 void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
                   hipStream_t s) {
-    if (sc.hasCurves)
+    if (curveRounds(sc, q))
+        launchCurveQuery<kQPath, false>(sc, q, bounce, cur, blocks, s);
+    else if (sc.hasCurves)
         hipLaunchKernelGGL((k_extend<kTraceBS, kStackKCurves, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
     else
         hipLaunchKernelGGL((k_extend<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
@@ -761,13 +1044,24 @@ void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, 
     else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    if (sc.hasCurves)
+    if (curveRounds(sc, q)) {
+        launchCurveQuery<kQShadow, true>(sc, q, bounce, 0u, blocks, s);
+        hipLaunchKernelGGL(k_shadow_resolve, dim3(blocks), dim3(256), 0, s, q, bounce);
+    } else if (sc.hasCurves)
         hipLaunchKernelGGL((k_shadow<kTraceBS, kStackKCurves, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
     else
         hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    if (sc.hasCurves) {
+    if (curveRounds(sc, q)) {
+        if (sc.hasEmitters) {
+            launchCurveQuery<kQMis, false>(sc, q, bounce, 0u, blocks, s);
+            hipLaunchKernelGGL((k_mis_resolve<true>), dim3(blocks), dim3(256), 0, s, sc, q, bounce);
+        } else {
+            launchCurveQuery<kQMis, true>(sc, q, bounce, 0u, blocks, s);
+            hipLaunchKernelGGL((k_mis_resolve<false>), dim3(blocks), dim3(256), 0, s, sc, q, bounce);
+        }
+    } else if (sc.hasCurves) {
         if (sc.hasEmitters)
             hipLaunchKernelGGL((k_mis<kTraceBS, kStackKCurves, true, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
         else

@@ -103,3 +103,25 @@ def test_converted_strands_with_per_cv_widths_vs_oracle(gpu, oracle):
     err = np.abs(gg - rr).max(-1) / np.maximum(np.abs(rr).max(-1), 1e-3 * max(rr.mean(), 1e-6))
     assert (err <= 1e-3).mean() >= 0.99, (err > 1e-3).sum()
     assert abs(gg.mean() / rr.mean() - 1) < 1e-3
+
+
+@pytest.mark.parametrize("emitting_quad", [True, False])
+def test_round_queries_equal_whole_ray_queries(gpu, monkeypatch, emitting_quad):
+    """In curve scenes the closest-hit, shadow and BSDF-sampled queries run in list-and-test rounds
+    (a wave tests one ray's listed segments, the rest of a long walk finishes one ray per wave);
+    AHIP_DEBUG_WHOLE_RAY=1 selects the one-ray-per-lane kernels the oracle tests above also pin.
+    Both must give the same film bit for bit: 20k strands, so that many rays outlast the rounds.
+    Without an emitting surface the BSDF-sampled query is an any-hit query."""
+    from anacapa_amd import _abi as A
+    sc = scenes.hair_sphere(width=96, height=96, spp=4, strands=20_000, hdri_size=(128, 64))
+    if not emitting_quad:
+        sc.materials[1].emission = A.F3(0.0, 0.0, 0.0)
+    gpu.prepare(sc)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("AHIP_DEBUG_WHOLE_RAY", mode)
+        print(f"[curves] whole-ray={mode} emitting_quad={emitting_quad}", flush=True)
+        out[mode] = gpu.render_tile_aovs(0, 0, 96, 96, 0, 4)
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a, b)
+    assert out["0"][0][..., 3].min() == 4 and out["0"][0][..., :3].mean() > 0.01
