@@ -67,9 +67,9 @@ struct WaveQueues {
     uint32_t stackCap;   // total stack entries available per thread
     uint32_t cap;        // path capacity
     // curve scenes (null otherwise): the resumable curve-tree walk of the closest-hit query
-    uint32_t* cList;     // [kCurveList][cap] records listed for the next wave test, entry-major
+    uint32_t* cList;     // [cap][kCurveList] records listed for the next wave test, a row per ray
     uint32_t* cState;    // [cap] listed count | saved stack depth << 8
-    uint32_t* cStack;    // [cStackCap][cap] saved walk stack (node indices), entry-major
+    uint32_t* cStack;    // [cap][cStackCap] saved walk stack (node indices), a row per ray
     uint32_t cStackCap;
 };
 

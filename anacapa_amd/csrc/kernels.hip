@@ -478,7 +478,7 @@ __global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint
                 continue;
             }
             bound = h != ~0u ? t : r.tMax;
-            q.cStack[i] = 0u;   // the curve root
+            q.cStack[(size_t)i * q.cStackCap] = 0u;   // the curve root
             sp = 1u;
         } else {
             sp = q.cState[i] >> 8;
@@ -491,22 +491,22 @@ __global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint
         const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
         uint32_t n = 0u;
         while (sp > 0u) {
-            const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)(sp - 1u) * q.cap + i]);
+            const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)i * q.cStackCap + sp - 1u]);
             const bool leaf = (nd.rc & 0x80000000u) != 0u;
             if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;   // the list is full: resume here
             --sp;
             if (!curveBox(nd, r, invD, bound)) continue;
             if (leaf) {
                 const uint32_t cnt = nd.rc & 0x7fffffffu;
-                for (uint32_t k = 0; k < cnt; ++k, ++n) q.cList[(size_t)n * q.cap + i] = nd.lp + k;
+                for (uint32_t k = 0; k < cnt; ++k, ++n) q.cList[(size_t)i * kCurveList + n] = nd.lp + k;
             } else {
                 if (sp + 2u > q.cStackCap) {
                     *ctr(q, bounce, 3) = 1u;
                     sp = 0u;
                     break;
                 }
-                q.cStack[(size_t)sp * q.cap + i] = nd.rc;
-                q.cStack[(size_t)(sp + 1u) * q.cap + i] = nd.lp;
+                q.cStack[(size_t)i * q.cStackCap + sp] = nd.rc;
+                q.cStack[(size_t)i * q.cStackCap + sp + 1u] = nd.lp;
                 sp += 2u;
             }
         }
@@ -514,42 +514,75 @@ __global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint
     }
 }
 
+// Lanes in groups of kCurveGroup, one ray per group: the group tests the ray's list kCurveGroup
+// records at a time in list order, each chunk bounded by the nearest hit of the chunks before (a
+// later record with an equal t never wins, as in the reference's walk).
+constexpr uint32_t kCurveGroup = 16;
+__device__ __forceinline__ uint64_t groupMin(uint64_t m) {
+    for (int off = kCurveGroup / 2; off >= 1; off >>= 1) {   // xor below the group size stays in the group
+        const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        m = o < m ? o : m;
+    }
+    return m;
+}
+
 template <uint32_t BS, uint32_t Q, bool ANY>
 __global__ __launch_bounds__(BS) void k_curve_test(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
     const uint32_t count = *ctr(q, bounce, Q);
-    const uint32_t lane = __lane_id();
+    const uint32_t lane = __lane_id(), grp = lane / kCurveGroup, gl = lane % kCurveGroup;
     const uint32_t waves = gridDim.x * (BS / 64u);
-    for (uint32_t i = blockIdx.x * (BS / 64u) + threadIdx.x / 64u; i < count; i += waves) {
-        const uint32_t st = q.cState[i], n = st & 0xffu;
-        if (n == 0u) continue;
-        bool dead;
-        const TRay r = curveQueryRay<Q>(q, cur, i, dead);
-        V3 fx, fy;
-        onbNC(r.d, fx, fy);
-        uint64_t key = ~0ull;
-        float tSeg = 0.f;
-        V3 tang{0.f, 0.f, 0.f};
-        if (lane < n) {
-            const uint32_t rec = q.cList[(size_t)lane * q.cap + i];
-            float best = hitBound(q.hit[i], r.tMax), ts;
-            V3 tg;
-            if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
-                key = ((uint64_t)__float_as_uint(best) << 32) | rec;   // t > 0: bits order as floats
-                tSeg = ts;
-                tang = tg;
+    // wave w takes rays w, w + waves, w + 2 waves ... (neighbouring rays cost alike, so they are
+    // spread over the waves), reading the states of 64 of them at once
+    const uint32_t w = blockIdx.x * (BS / 64u) + threadIdx.x / 64u;
+    for (uint32_t k0 = 0; w + waves * k0 < count; k0 += 64u) {
+        const uint32_t mine = w + waves * (k0 + lane);
+        uint64_t todo = __ballot(mine < count && (q.cState[mine] & 0xffu) != 0u);
+        while (todo) {
+            // the g-th ray still to do goes to group g
+            uint32_t i = ~0u;
+            for (uint32_t g = 0; g < 64u / kCurveGroup && todo; ++g) {
+                if (g == grp) i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
+                todo &= todo - 1ull;
             }
-        }
-        const uint64_t m = waveMin(key);
-        if (m == ~0ull) continue;
-        if (ANY) {
-            if (lane == 0u) {
-                q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
-                q.cState[i] = 0u;   // occluded: the walk is over
+            if (i == ~0u) continue;   // (the loop test reads the wave's `todo`, alike in every lane)
+            const uint32_t n = q.cState[i] & 0xffu;
+            bool dead;
+            const TRay r = curveQueryRay<Q>(q, cur, i, dead);
+            V3 fx, fy;
+            onbNC(r.d, fx, fy);
+            float bnd = hitBound(q.hit[i], r.tMax);
+            uint64_t key = ~0ull, m = ~0ull;
+            float tSeg = 0.f;
+            V3 tang{0.f, 0.f, 0.f};
+            for (uint32_t c = 0; c < n; c += kCurveGroup) {   // alike in the group
+                if (c + gl < n) {
+                    const uint32_t rec = q.cList[(size_t)i * kCurveList + c + gl];
+                    float best = bnd, ts;
+                    V3 tg;
+                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                        key = ((uint64_t)__float_as_uint(best) << 32) | rec;   // t > 0: bits order as floats
+                        tSeg = ts;
+                        tang = tg;
+                    }
+                }
+                m = groupMin(key);
+                if (m != ~0ull) {
+                    if (ANY) break;
+                    bnd = __uint_as_float((uint32_t)(m >> 32));
+                }
             }
-        } else if (key == m) {   // records are listed once: one lane holds the least key
-            q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
-                                   __uint_as_float(kCurveHit | (uint32_t)m));
-            q.hitAux[i] = f4(tang, 0.f);
+            if (m == ~0ull) continue;
+            if (ANY) {
+                if (gl == 0u) {
+                    q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
+                    q.cState[i] = 0u;   // occluded: the walk is over
+                }
+            } else if (key == m) {   // records are listed once: one lane holds the least key
+                q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
+                                       __uint_as_float(kCurveHit | (uint32_t)m));
+                q.hitAux[i] = f4(tang, 0.f);
+            }
         }
     }
 }
@@ -561,9 +594,14 @@ __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, ui
     const uint32_t count = *ctr(q, bounce, Q);
     const uint32_t lane = __lane_id();
     const uint32_t waves = gridDim.x * (BS / 64u);
-    for (uint32_t i = blockIdx.x * (BS / 64u) + threadIdx.x / 64u; i < count; i += waves) {
+    const uint32_t w = blockIdx.x * (BS / 64u) + threadIdx.x / 64u;
+    for (uint32_t k0 = 0; w + waves * k0 < count; k0 += 64u) {
+    const uint32_t mine = w + waves * (k0 + lane);
+    uint64_t todo = __ballot(mine < count && (q.cState[mine] >> 8) != 0u);
+    while (todo) {
+        const uint32_t i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
+        todo &= todo - 1ull;
         uint32_t sp = q.cState[i] >> 8;
-        if (sp == 0u) continue;
         bool dead;
         const TRay r = curveQueryRay<Q>(q, cur, i, dead);
         float bound = hitBound(q.hit[i], r.tMax);
@@ -576,7 +614,7 @@ __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, ui
         while (sp > 0u) {
             uint32_t n = 0u, rec = ~0u;
             while (sp > 0u) {
-                const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)(sp - 1u) * q.cap + i]);
+                const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)i * q.cStackCap + sp - 1u]);
                 const bool leaf = (nd.rc & 0x80000000u) != 0u;
                 if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;
                 --sp;
@@ -591,8 +629,8 @@ __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, ui
                         sp = 0u;
                         break;
                     }
-                    q.cStack[(size_t)sp * q.cap + i] = nd.rc;   // every lane stores the same entry
-                    q.cStack[(size_t)(sp + 1u) * q.cap + i] = nd.lp;
+                    q.cStack[(size_t)i * q.cStackCap + sp] = nd.rc;   // every lane stores the same entry
+                    q.cStack[(size_t)i * q.cStackCap + sp + 1u] = nd.lp;
                     sp += 2u;
                 }
             }
@@ -620,6 +658,7 @@ __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, ui
                                    __uint_as_float(kCurveHit | (uint32_t)m));
             q.hitAux[i] = f4(tang, 0.f);
         }
+    }
     }
 }
 
