@@ -587,78 +587,83 @@ __global__ __launch_bounds__(BS) void k_curve_test(DScene sc, WaveQueues q, uint
     }
 }
 
-// One wave per unfinished ray: the walk runs in every lane alike (same loads, same branches) and
-// lane k takes the k-th listed record.
+// The unfinished rays, one per lane group: the walk runs in every lane of the group alike (same
+// loads, same branches), lists up to kCurveGroup records and lane k of the group tests the k-th.
 template <uint32_t BS, uint32_t Q, bool ANY>
 __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
     const uint32_t count = *ctr(q, bounce, Q);
-    const uint32_t lane = __lane_id();
+    const uint32_t lane = __lane_id(), grp = lane / kCurveGroup, gl = lane % kCurveGroup;
     const uint32_t waves = gridDim.x * (BS / 64u);
     const uint32_t w = blockIdx.x * (BS / 64u) + threadIdx.x / 64u;
     for (uint32_t k0 = 0; w + waves * k0 < count; k0 += 64u) {
-    const uint32_t mine = w + waves * (k0 + lane);
-    uint64_t todo = __ballot(mine < count && (q.cState[mine] >> 8) != 0u);
-    while (todo) {
-        const uint32_t i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
-        todo &= todo - 1ull;
-        uint32_t sp = q.cState[i] >> 8;
-        bool dead;
-        const TRay r = curveQueryRay<Q>(q, cur, i, dead);
-        float bound = hitBound(q.hit[i], r.tMax);
-        V3 fx, fy;
-        onbNC(r.d, fx, fy);
-        const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};
-        uint64_t key = ~0ull, m = ~0ull;
-        float tSeg = 0.f;
-        V3 tang{0.f, 0.f, 0.f};
-        while (sp > 0u) {
-            uint32_t n = 0u, rec = ~0u;
+        const uint32_t mine = w + waves * (k0 + lane);
+        uint64_t todo = __ballot(mine < count && (q.cState[mine] >> 8) != 0u);
+        while (todo) {
+            uint32_t i = ~0u;
+            for (uint32_t g = 0; g < 64u / kCurveGroup && todo; ++g) {
+                if (g == grp) i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
+                todo &= todo - 1ull;
+            }
+            if (i == ~0u) continue;
+            uint32_t sp = q.cState[i] >> 8;
+            bool dead;
+            const TRay r = curveQueryRay<Q>(q, cur, i, dead);
+            float bound = hitBound(q.hit[i], r.tMax);
+            V3 fx, fy;
+            onbNC(r.d, fx, fy);
+            const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};
+            uint64_t key = ~0ull, m = ~0ull;
+            float tSeg = 0.f;
+            V3 tang{0.f, 0.f, 0.f};
+            uint32_t* const stk = q.cStack + (size_t)i * q.cStackCap;
             while (sp > 0u) {
-                const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)i * q.cStackCap + sp - 1u]);
-                const bool leaf = (nd.rc & 0x80000000u) != 0u;
-                if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;
-                --sp;
-                if (!curveBox(nd, r, invD, bound)) continue;
-                if (leaf) {
-                    const uint32_t cnt = nd.rc & 0x7fffffffu;
-                    if (lane - n < cnt) rec = nd.lp + (lane - n);   // lane in [n, n + cnt)
-                    n += cnt;
-                } else {
-                    if (sp + 2u > q.cStackCap) {
-                        *ctr(q, bounce, 3) = 1u;
-                        sp = 0u;
-                        break;
+                uint32_t n = 0u, rec = ~0u;
+                while (sp > 0u) {
+                    const CNode nd = loadCNode(sc.cnodes, stk[sp - 1u]);
+                    const bool leaf = (nd.rc & 0x80000000u) != 0u;
+                    if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveGroup) break;   // a leaf holds at most 4
+                    --sp;
+                    if (!curveBox(nd, r, invD, bound)) continue;
+                    if (leaf) {
+                        const uint32_t cnt = nd.rc & 0x7fffffffu;
+                        if (gl - n < cnt) rec = nd.lp + (gl - n);   // gl in [n, n + cnt)
+                        n += cnt;
+                    } else {
+                        if (sp + 2u > q.cStackCap) {
+                            *ctr(q, bounce, 3) = 1u;
+                            sp = 0u;
+                            break;
+                        }
+                        stk[sp] = nd.rc;   // every lane of the group stores the same entry
+                        stk[sp + 1u] = nd.lp;
+                        sp += 2u;
                     }
-                    q.cStack[(size_t)i * q.cStackCap + sp] = nd.rc;   // every lane stores the same entry
-                    q.cStack[(size_t)i * q.cStackCap + sp + 1u] = nd.lp;
-                    sp += 2u;
+                }
+                if (n == 0u) continue;
+                if (rec != ~0u) {
+                    float best = bound, ts;
+                    V3 tg;
+                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                        key = ((uint64_t)__float_as_uint(best) << 32) | rec;
+                        tSeg = ts;
+                        tang = tg;
+                    }
+                }
+                m = groupMin(key);
+                if (m != ~0ull) {
+                    if (ANY) break;
+                    bound = __uint_as_float((uint32_t)(m >> 32));
                 }
             }
-            if (n == 0u) continue;
-            if (rec != ~0u) {
-                float best = bound, ts;
-                V3 tg;
-                if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
-                    key = ((uint64_t)__float_as_uint(best) << 32) | rec;
-                    tSeg = ts;
-                    tang = tg;
-                }
-            }
-            m = waveMin(key);
-            if (m != ~0ull) {
-                if (ANY) break;
-                bound = __uint_as_float((uint32_t)(m >> 32));
+            if (m == ~0ull) continue;
+            if (ANY) {
+                if (gl == 0u) q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
+            } else if (key == m) {
+                q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
+                                       __uint_as_float(kCurveHit | (uint32_t)m));
+                q.hitAux[i] = f4(tang, 0.f);
             }
         }
-        if (m == ~0ull) continue;
-        if (ANY) {
-            if (lane == 0u) q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
-        } else if (key == m) {
-            q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
-                                   __uint_as_float(kCurveHit | (uint32_t)m));
-            q.hitAux[i] = f4(tang, 0.f);
-        }
-    }
     }
 }
 
