@@ -153,6 +153,7 @@ SIGNATURES = {
                                         C.c_uint32, C.c_uint32, VP, VP, VP]),
     "ahip_trace_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_occluded_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
+    "ahip_trace_rays_noalloc": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_halton_eval": (C.c_int, [VP, C.c_uint64, VP, VP, VP, VP, C.c_uint32, VP]),
     "ahip_set_eval_params": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_replay": (C.c_int, [VP, C.c_uint32, C.c_uint32, VP]),

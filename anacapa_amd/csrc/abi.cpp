@@ -55,6 +55,38 @@ struct EventPair {
     int kind = 0;   // 0 trace, 1 shade
 };
 
+// ------------------------------------------------------------ no-allocation traversal scratch
+// ahip_trace_rays_noalloc runs on this fixed device array instead of allocating: a 256-byte header
+// (word 0: the stack-overflow flag), then the traversal stack entries past the LDS part as
+// [AHIP_NOALLOC_MAX_STACK][kNoAllocThreads] child references followed by as many entry distances
+// (the layout TStack indexes with the grid's thread count as stride).  Every GPU holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in, not device memory:
+// host code only ever uses the per-device address hipGetSymbolAddress returns for it.
+constexpr uint32_t kNoAllocThreads = 262144;   // 2048 blocks of 128 threads, grid-stride over the rays
+constexpr size_t kNoAllocHeader = 256;
+constexpr size_t kNoAllocStackWords = (size_t)AHIP_NOALLOC_MAX_STACK * kNoAllocThreads;
+constexpr size_t kNoAllocBytes = kNoAllocHeader + 2u * kNoAllocStackWords * sizeof(uint32_t);
+static_assert(kNoAllocBytes == (64u << 20) + 256u, "no-allocation scratch: 64 MiB of stack + header");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratchMu;
+std::vector<void*> g_scratchAddr;   // per device ordinal, resolved once
+
+// The device address of `scratch` on the current device (ordinal dev), resolved on first use.
+hipError_t scratchAddress(int dev, void** out) {
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if ((int)g_scratchAddr.size() <= dev) g_scratchAddr.resize(dev + 1, nullptr);
+    if (!g_scratchAddr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratchAddr[dev] = p;
+    }
+    *out = g_scratchAddr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct ahip_context {
@@ -112,6 +144,8 @@ struct ahip_context {
     std::vector<EventPair> events;
     size_t eventsUsed = 0;
     hipEvent_t renderBeg = nullptr, renderEnd = nullptr;
+    void* scratch = nullptr;     // this device's no-allocation scratch (ahip_trace_rays_noalloc)
+    std::string scratchErr;      // why it could not be resolved at create
     bool lastRenderValid = false;
     bool aovOn = false;          // ahip_set_aovs
     bool accumHasSamples = false; // the frame accumulation holds samples since the last clear
@@ -503,6 +537,13 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     setHaltonTables(T, ctx->stream);
     setHaltonTablesReplay(T, ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    // the no-allocation scratch of this device (the device is current); a failure here only
+    // disables ahip_trace_rays_noalloc, which reports it
+    hipError_t se = scratchAddress(device_ordinal, &ctx->scratch);
+    if (se != hipSuccess) {
+        ctx->scratch = nullptr;
+        ctx->scratchErr = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+    }
     ctx->valid = true;
     return AHIP_OK;
 }
@@ -1010,6 +1051,47 @@ int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* 
     HIPCHK(hipMemcpyAsync(&flag, dflag.p, 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (flag) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
+// ahip_trace_rays on caller-owned device arrays, with the traversal's overflow stack and flag in
+// this device's fixed scratch array: no allocation.  The same kernel and tree as ahip_trace_rays,
+// so the hits are identical (the stack's grid stride differs, which only moves where entries live).
+int ahip_trace_rays_noalloc(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits) {
+    ENTER();
+    if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (n == 0) return AHIP_OK;
+    if (!rays || !hits) return fail(ctx, AHIP_E_INVALID_ARG, "null arrays");
+    if (n > AHIP_NOALLOC_MAX_RAYS)
+        return fail(ctx, AHIP_E_INVALID_ARG, "ahip_trace_rays_noalloc: " + std::to_string(n) +
+                                                 " rays, more than AHIP_NOALLOC_MAX_RAYS");
+    if (!ctx->scratch) return fail(ctx, AHIP_E_HIP, "ahip_trace_rays_noalloc: " + ctx->scratchErr);
+    const uint32_t K = lds_stack_entries();
+    const uint32_t extra = ctx->stackCap > K ? ctx->stackCap - K : 0u;
+    if (extra > AHIP_NOALLOC_MAX_STACK)
+        return fail(ctx, AHIP_E_UNSUPPORTED,
+                    "ahip_trace_rays_noalloc: the scene's BVH needs " + std::to_string(extra) +
+                        " traversal stack entries beyond the LDS ones; the fixed scratch holds " +
+                        std::to_string(AHIP_NOALLOC_MAX_STACK));
+    const uint32_t blocks = kNoAllocThreads / trace_block_size();
+    if (blocks * trace_block_size() != kNoAllocThreads)
+        return fail(ctx, AHIP_E_HIP, "ahip_trace_rays_noalloc: block size does not divide the scratch grid");
+    int st = checkInjected(ctx);
+    if (st) return st;
+    unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(base);
+    uint32_t* ovfRef = reinterpret_cast<uint32_t*>(base + kNoAllocHeader);
+    float* ovfT = reinterpret_cast<float*>(base + kNoAllocHeader + kNoAllocStackWords * sizeof(uint32_t));
+    // the array keeps the previous call's contents: the flag must start at zero (a fill kernel, so
+    // the call stays valid inside a captured graph); stack entries are written before they are read
+    launchFillU32(flag, 1, 0u, ctx->stream);
+    launchTraceBatch(ctx->dsc, rays, n, hits, nullptr, false, ovfRef, ovfT, ctx->stackCap, blocks, flag, nullptr,
+                     ctx->stream);
+    HIPCHK(hipGetLastError());
+    uint32_t f = 0;
+    HIPCHK(hipMemcpyAsync(&f, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (f) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
     return AHIP_OK;
 }
 

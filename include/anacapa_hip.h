@@ -277,6 +277,19 @@ typedef struct ahip_hit {
 } ahip_hit;
 int ahip_trace_rays(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits);
 int ahip_occluded_rays(ahip_context* ctx, const float* rays, uint32_t n, uint8_t* occluded);
+/* ahip_trace_rays_noalloc — no counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the closest hits of ahip_trace_rays, bit for bit, with no allocation of any kind.
+ * `rays` and `hits` are DEVICE pointers (the layouts above) that the caller owns.  The traversal
+ * stack past its LDS part and the overflow flag live in one fixed device array per GPU, 64 MiB,
+ * held until the process ends: a scene whose BVH needs more than AHIP_NOALLOC_MAX_STACK stack
+ * entries beyond the LDS ones (a tree deeper than 12 two-level records), or n above
+ * AHIP_NOALLOC_MAX_RAYS, is refused with AHIP_E_UNSUPPORTED / AHIP_E_INVALID_ARG.  Calls share
+ * that array, so calls that overlap on one GPU (from different contexts or threads) race; calls
+ * through one context are serialised.  Synchronous on the context's stream. */
+#define AHIP_NOALLOC_MAX_STACK 32u
+#define AHIP_NOALLOC_MAX_RAYS  (1u << 30)
+int ahip_trace_rays_noalloc(ahip_context* ctx, const float* rays, uint32_t n, ahip_hit* hits);
 
 /* --- sampler (HaltonSampler::get1D, src/sampling/HaltonSampler.h:43-50): dimension dim of
  *     sample idx at pixel (px,py); dims >= 128 come from the per-pixel PCG stream. --- */
