@@ -30,6 +30,7 @@ AHIP_MAT_HAIR = 3
 AHIP_LIGHT_AREA = 0
 AHIP_LIGHT_DOME = 1
 AHIP_FLAG_FIX_DOUBLE_COSINE = 1
+AHIP_FLAG_HAIR_MARSCHNER = 2
 
 F3 = C.c_float * 3
 U32P = C.POINTER(C.c_uint32)

@@ -47,6 +47,11 @@ extern "C" {
 /* ahip_render_params.flags */
 #define AHIP_FLAG_FIX_DOUBLE_COSINE 1u  /* opt out of the reference's extra cos in NEE's BSDF branch
                                           (src/integrator/PathIntegrator.cpp:257-258) */
+#define AHIP_FLAG_HAIR_MARSCHNER    2u  /* hair materials evaluate and sample the Marschner R/TT/TRT
+                                          model MarschnerHairMaterial carries below its Kajiya-Kay
+                                          returns (src/shading/MarschnerHair.h:243-257, 284-375,
+                                          490-556) instead of the Kajiya-Kay BSDF it runs (:240,
+                                          266-282); off = the reference's runtime behaviour */
 
 typedef struct ahip_material {
     uint32_t type;              /* AHIP_MAT_* */

@@ -11,11 +11,16 @@
 // StandardSurface materials, quad AreaLight and equirect DomeLight (constant textures plus
 // a single-channel roughness texture with this build's own bilinear lookup).
 //
+// Scope also covers curves (the ray-space ribbon test) and the hair material in both of its
+// forms: the Kajiya-Kay BSDF the reference runs, and the Marschner R/TT/TRT model it carries
+// below its Kajiya-Kay returns (render flag AHIP_FLAG_HAIR_MARSCHNER).
+//
 // Parity pins (see tests/): PCG vs the published pcg32 known answer and vs the reference
 // build; Halton, camera rays, BVH topology, closest hits and Cornell-box radiance vs the
 // reference build in oracle/_ref; GGX helpers and Distribution1D vs the reference's
-// header-inline functions; StandardSurface mixture and DomeLight sampling by closed-form
-// invariants (pdf normalisation, sample/evaluate consistency, constant-dome pdf).
+// header-inline functions; the StandardSurface mixture (sample / evaluate), DomeLight sampling,
+// pdf and lookup, and the hair material in both forms vs the reference's own code in oracle/_ref
+// (tests/golden/shading_vectors.npz); curve subdivision against the analytic Bezier.
 // Parity unpinned: the roughness-texture lookup (reference uses OIIO's TextureSystem,
 // src/shading/Texture.cpp:35-83, which is not available here).
 
@@ -132,7 +137,7 @@ static inline Dual dconst(double c) { return {c, 0.0}; }
 // ---------------------------------------------------------------- materials
 struct BSDFSample { V3 f, wi; float pdf = 0.f; bool delta = false;
                     bool valid() const { return pdf > 0.f && !isBlack(f); } };        // IMaterial.h:26-40
-struct BSDFEval { V3 f; float pdf = 0.f; };
+struct BSDFEval { V3 f; float pdf = 0.f, pdfRev = 0.f; };
 
 static inline V3 cosineHemisphere(V2 u) {                                             // Lambertian.h:12-20
     float r = std::sqrt(u.x), phi = 2.f * kPi * u.y;
@@ -183,16 +188,166 @@ static inline float fresnelDielectric(float cosI, float eta) {                  
     return 0.5f * (rs * rs + rp * rp);
 }
 
+// ---------------------------------------------------------------- Marschner hair (MarschnerHair.h)
+// The R / TT / TRT model the reference's MarschnerHairMaterial carries but does not run (its
+// evaluate and sample return Kajiya-Kay first): longitudinal M_p, azimuthal N_p and attenuation
+// A_p with the PBRT-v4 curve fits, as the reference states them.  Evaluated when the render flags
+// ask for AHIP_FLAG_HAIR_MARSCHNER.
+namespace mh {
+static float I0(float x) {                                                            // :62-71
+    float sum = 0.f, x2i = 1.f, denom = 1.f;
+    for (int i = 0; i < 12; ++i) {
+        if (i > 0) {
+            x2i *= x * x;
+            denom *= (float)i * (float)i * 4.f;
+        }
+        sum += x2i / denom;
+    }
+    return sum;
+}
+static float logI0(float x) {                                                         // :73-77
+    if (x > 12.f) return x + 0.5f * (-std::log(k2Pi) + std::log(1.f / x) + 1.f / (8.f * x));
+    return std::log(I0(x));
+}
+static float Mp(float cosI, float sinI, float cosR, float sinR, float v) {            // :88-98
+    if (v < 1e-5f) v = 1e-5f;
+    float a = cosI * cosR / v, b = sinI * sinR / v;
+    if (v <= 0.1f) return std::exp(logI0(a) - b - 1.f / v + 0.6931472f + std::log(0.5f / v));
+    return std::exp(-b) * I0(a) / (2.f * v * std::sinh(1.f / v));
+}
+static float logistic(float x, float s) {                                             // :103-106
+    float e = std::exp(-std::abs(x) / s);
+    return e / (s * (1.f + e) * (1.f + e));
+}
+static float logisticCDF(float x, float s) { return 1.f / (1.f + std::exp(-x / s)); }  // :108-110
+static float trimmedLogistic(float x, float s) {                                      // :113-116
+    return logistic(x, s) / (logisticCDF(kPi, s) - logisticCDF(-kPi, s));
+}
+static float sampleTrimmedLogistic(float u, float s) {                                // :119-124
+    float a = logisticCDF(-kPi, s), b = logisticCDF(kPi, s);
+    float x = -s * std::log(1.f / (a + u * (b - a)) - 1.f);
+    return std::clamp(x, -kPi, kPi);
+}
+static float Phi(int p, float gO, float gT) {                                         // :129-131
+    return 2.f * (float)p * gT - 2.f * gO + (float)p * kPi;
+}
+static float wrapPhi(float x) {                                                       // :133-137
+    while (x > kPi) x -= k2Pi;
+    while (x < -kPi) x += k2Pi;
+    return x;
+}
+static float Np(float phi, int p, float s, float gO, float gT) {                      // :140-142
+    return trimmedLogistic(wrapPhi(phi - Phi(p, gO, gT)), s);
+}
+static float clampH(float h) { return std::clamp(h, -1.f + 1e-5f, 1.f - 1e-5f); }
+// sin of the refracted azimuth gamma_t for impact parameter h at the modified index of cos(theta_o)
+static float sinGammaT(float cosO, float eta, float h) {
+    float etaP = std::sqrt(fmaxs(0.f, eta * eta - (1.f - cosO * cosO))) / cosO;
+    return clampH(h / (etaP > 1e-5f ? etaP : 1e-5f));
+}
+static void Ap(float cosO, float eta, float h, V3 sa, V3 ap[3]) {                      // :146-176
+    float s2 = fmaxs(0.f, 1.f - cosO * cosO);
+    float etaP = std::sqrt(fmaxs(0.f, eta * eta - s2)) / cosO;
+    float sgt = clampH(h / (etaP > 1e-5f ? etaP : 1e-5f));
+    float cgt = std::sqrt(fmaxs(0.f, 1.f - sgt * sgt));
+    V3 T{std::exp(-sa.x * 2.f * cgt), std::exp(-sa.y * 2.f * cgt), std::exp(-sa.z * 2.f * cgt)};
+    float cgo = std::sqrt(fmaxs(0.f, 1.f - h * h));
+    float fr = fresnelDielectric(fmaxs(0.f, cosO * cgo), eta);
+    ap[0] = V3{fr, fr, fr};
+    ap[1] = T * ((1.f - fr) * (1.f - fr));
+    ap[2] = ap[1] * T * fr;
+}
+static float sampleGaussian(float u1, float u2) {                                     // :181-184
+    u1 = fmaxs(u1, 1e-6f);
+    return std::sqrt(-2.f * std::log(u1)) * std::cos(k2Pi * u2);
+}
+static void longitudinal(V3 d, V3 t, float& s, float& c) {                            // :456-460
+    s = dot(d, t);
+    c = std::sqrt(fmaxs(0.f, 1.f - s * s));
+}
+static float azimuth(V3 wo, V3 wi, V3 t, float sO, float sI) {                        // :463-475
+    V3 po = wo - t * sO, pi = wi - t * sI;
+    float lo = length(po), li = length(pi);
+    if (lo < 1e-5f || li < 1e-5f) return 0.f;
+    po = po * (1.f / lo);
+    pi = pi * (1.f / li);
+    float c = std::clamp(dot(po, pi), -1.f, 1.f);
+    float s = dot(cross(pi, po), t);
+    return std::atan2(s, c);
+}
+static float impact(V3 wo, V3 t, V3 n, float sO) {                                    // :478-483
+    V3 po = wo - t * sO;
+    float l = length(po);
+    if (l < 1e-5f) return 0.f;
+    return clampH(dot(po * (1.f / l), n));
+}
+// Per-material constants of the constructor (:200-216): variances v[p], logistic scale s and the
+// cuticle tilts alpha[p] of the R, TT and TRT lobes.
+struct Consts {
+    float v[3], s, alpha[3];
+    void init(float betaM, float betaN, float alphaDeg) {
+        float bm = std::clamp(betaM, 1e-3f, 1.f);
+        float v0 = 0.726f * bm + 0.812f * bm * bm + 3.7f * std::pow(bm, 20.f);
+        v[0] = v0 * v0;
+        v[1] = v[0] * 0.25f;
+        v[2] = v[0] * 4.0f;
+        float bn = std::clamp(betaN, 1e-3f, 1.f);
+        s = 0.626657069f * (0.265f * bn + 1.194f * bn * bn + 5.372f * std::pow(bn, 22.f));
+        alpha[0] = -alphaDeg * kPi / 180.f;
+        alpha[1] = alphaDeg * kPi / 180.f * 0.5f;
+        alpha[2] = -alphaDeg * kPi / 180.f * 1.5f;
+    }
+    // sum over p of M_p A_p N_p / max(1/4, cos^2 theta_d) (:490-522)
+    V3 lobes(float eta, float sO, float cO, float sI, float cI, float phi, float h, V3 sa) const {
+        float cd = std::sqrt(fmaxs(0.f, 0.5f * (1.f + cO * cI + sO * sI)));
+        float denom = fmaxs(0.25f, cd * cd);
+        float gO = std::asin(clampH(h)), gT = std::asin(sinGammaT(cO, eta, h));
+        V3 ap[3];
+        Ap(cO, eta, h, sa, ap);
+        V3 f;
+        for (int p = 0; p < 3; ++p) {
+            float so = sO * std::cos(2.f * alpha[p]) + cO * std::sin(2.f * alpha[p]);
+            float co = std::sqrt(fmaxs(0.f, 1.f - so * so));
+            float m = Mp(cI, sI, co, so, v[p]);
+            float nn = Np(phi, p, s, gO, gT);
+            f = f + ap[p] * (m * nn);
+        }
+        return f * (1.f / denom);
+    }
+    // lobe-energy mixture of M_p cos(theta_i) N_p (:528-556)
+    float pdf(float eta, float sO, float cO, float sI, float cI, float phi, float h, V3 sa) const {
+        float gO = std::asin(clampH(h)), gT = std::asin(sinGammaT(cO, eta, h));
+        V3 ap[3];
+        Ap(cO, eta, h, sa, ap);
+        float w[3] = {luminance(ap[0]), luminance(ap[1]), luminance(ap[2])};
+        float total = w[0] + w[1] + w[2];
+        if (total < 1e-8f) return 0.f;
+        float pd = 0.f;
+        for (int p = 0; p < 3; ++p) {
+            float so = sO * std::cos(2.f * alpha[p]) + cO * std::sin(2.f * alpha[p]);
+            float co = std::sqrt(fmaxs(0.f, 1.f - so * so));
+            float m = Mp(cI, sI, co, so, v[p]) * cI;
+            float nn = Np(phi, p, s, gO, gT);
+            pd += (w[p] / total) * m * nn;
+        }
+        return fmaxs(0.f, pd);
+    }
+};
+}  // namespace mh
+
 struct Material {
     ahip_material m;
     const Texture* roughTex = nullptr;
     // derived (StandardSurface.h:152-160)
     float coatA2 = 0.f, f0d = 0.f, coatF0 = 0.f;
+    bool marschner = false;   // hair: R/TT/TRT instead of the reference's runtime Kajiya-Kay
+    mh::Consts hc{};
     void init() {
         float ca = fmaxs(1e-4f, m.coat_roughness * m.coat_roughness);
         coatA2 = ca * ca;
         f0d = F0ior(m.specular_ior);
         coatF0 = F0ior(1.5f);
+        hc.init(m.hair_beta_m, m.hair_beta_n, m.hair_alpha);
     }
     V3 albedo() const { return {m.albedo[0], m.albedo[1], m.albedo[2]}; }
     V3 baseOf(const Ctx& c) const { return c.hasAlb ? c.alb : albedo(); }
@@ -288,8 +443,78 @@ struct Material {
         return e;
     }
 
+    // MarschnerHairMaterial::evaluate below its Kajiya-Kay return (MarschnerHair.h:243-257)
+    BSDFEval marschnerEval(const Ctx& c, V3 wo, V3 wi) const {
+        float sO, cO, sI, cI;
+        mh::longitudinal(wo, c.t, sO, cO);
+        mh::longitudinal(wi, c.t, sI, cI);
+        float phi = mh::azimuth(wo, wi, c.t, sO, sI);
+        float h = mh::impact(wo, c.t, c.n, sO);
+        V3 sa = hairSigmaA(c);
+        BSDFEval e;
+        e.f = hc.lobes(m.hair_eta, sO, cO, sI, cI, phi, h, sa);
+        e.pdf = hc.pdf(m.hair_eta, sO, cO, sI, cI, phi, h, sa);
+        e.pdfRev = hc.pdf(m.hair_eta, sI, cI, sO, cO, -phi, h, sa);
+        return e;
+    }
+    // MarschnerHairMaterial::sample below its Kajiya-Kay return (MarschnerHair.h:284-375): pick a
+    // lobe by the luminance of A_p, theta_i from a Gaussian around the tilted mirror angle, phi from
+    // the trimmed logistic around the lobe's specular azimuth; f and pdf are the lobe sums at the
+    // resulting direction.
+    BSDFSample marschnerSample(const Ctx& c, V3 wo, V2 u, float uc) const {
+        BSDFSample bs;
+        float sO, cO;
+        mh::longitudinal(wo, c.t, sO, cO);
+        float h = mh::impact(wo, c.t, c.n, sO);
+        V3 sa = hairSigmaA(c);
+        V3 ap[3];
+        mh::Ap(cO, m.hair_eta, h, sa, ap);
+        float w[3] = {luminance(ap[0]), luminance(ap[1]), luminance(ap[2])};
+        float total = w[0] + w[1] + w[2];
+        if (total < 1e-8f) return bs;
+        float inv = 1.f / total;
+        float cdf0 = w[0] * inv, cdf1 = cdf0 + w[1] * inv, cdf2 = 1.f;
+        int lobe = 0;
+        float ur = uc;
+        if (uc < cdf0) {
+            ur = uc / cdf0;
+        } else if (uc < cdf1) {
+            lobe = 1;
+            ur = (uc - cdf0) / (cdf1 - cdf0);
+        } else {
+            lobe = 2;
+            ur = (uc - cdf1) / (cdf2 - cdf1);
+        }
+        float thetaO = std::asin(mh::clampH(sO));
+        float z = mh::sampleGaussian(u.x, u.y);
+        float thetaI = thetaO + hc.alpha[lobe] + z * std::sqrt(hc.v[lobe]);
+        thetaI = std::clamp(thetaI, -kPi * 0.5f, kPi * 0.5f);
+        float sI = std::sin(thetaI), cI = std::cos(thetaI);
+        float gO = std::asin(mh::clampH(h)), gT = std::asin(mh::sinGammaT(cO, m.hair_eta, h));
+        float phiS = mh::sampleTrimmedLogistic(ur, hc.s) + mh::Phi(lobe, gO, gT);
+        V3 woPerp = safeNormalize(wo - c.t * sO);
+        if (lengthSq(woPerp) < 1e-8f) {
+            V3 unused;
+            onb(c.t, woPerp, unused);
+        }
+        V3 ctPerp = cross(c.t, woPerp);
+        V3 wi = c.t * sI + woPerp * (cI * std::cos(phiS)) + ctPerp * (cI * std::sin(phiS));
+        wi = safeNormalize(wi);
+        float sIa, cIa;
+        mh::longitudinal(wi, c.t, sIa, cIa);
+        float phiA = mh::azimuth(wo, wi, c.t, sO, sIa);
+        V3 f = hc.lobes(m.hair_eta, sO, cO, sIa, cIa, phiA, h, sa);
+        float pd = hc.pdf(m.hair_eta, sO, cO, sIa, cIa, phiA, h, sa);
+        if (pd < 1e-8f) return bs;
+        bs.wi = wi;
+        bs.f = f * std::abs(dot(wi, c.n));
+        bs.pdf = pd;
+        return bs;
+    }
+
     BSDFSample sample(const Ctx& c, V3 wo, V2 u, float uc) const {
         BSDFSample s;
+        if (m.type == AHIP_MAT_HAIR && marschner) return marschnerSample(c, wo, u, uc);
         if (m.type == AHIP_MAT_HAIR) {                                                // MarschnerHair.h:267-282
             V3 kt, kb;
             onb(c.n, kt, kb);
@@ -362,7 +587,7 @@ struct Material {
     }
 
     BSDFEval evaluate(const Ctx& c, V3 wo, V3 wi) const {
-        if (m.type == AHIP_MAT_HAIR) return kkEval(c, wo, wi);
+        if (m.type == AHIP_MAT_HAIR) return marschner ? marschnerEval(c, wo, wi) : kkEval(c, wo, wi);
         BSDFEval e;
         V3 wil = c.toLocal(wi), wol = c.toLocal(wo);
         if (m.type != AHIP_MAT_STANDARD) {                                            // Lambertian.h:53-66
@@ -1294,6 +1519,7 @@ int orc_set_render(void* h, const ahip_camera* cam, uint32_t maxDepth, uint32_t 
     if (cam) s->cam = toCam(cam);
     s->maxDepth = maxDepth; s->minDepth = minDepth; s->seed = seed;
     s->fixDoubleCos = (flags & AHIP_FLAG_FIX_DOUBLE_COSINE) != 0;
+    for (Material& m : s->mats) m.marschner = (flags & AHIP_FLAG_HAIR_MARSCHNER) != 0;
     return 0;
 }
 
@@ -1610,6 +1836,156 @@ int orc_dome_eval(void* h, uint32_t n, const float* dirs, float* rgb, float* pdf
         V3 c = s->dome.eval(d) * s->dome.intensity;
         rgb[3 * i] = c.x; rgb[3 * i + 1] = c.y; rgb[3 * i + 2] = c.z;
         if (pdf) pdf[i] = s->dome.evalPdf(d);
+    }
+    return 0;
+}
+
+// Material probe, the layout every probe of this repository shares:
+// in[n*19]  = normal n, tangent t, strand colour, wo, wi, u.x, u.y, uComponent, curve (0 / 1)
+// out[n*16] = evaluate(wo, wi): f rgb, pdf, pdfRev | sample(wo, u, uc): wi, f rgb, pdf, valid |
+//             effectiveSigmaA rgb (hair; zero otherwise)
+// The shading context is the integrator's for a hit seen along -wo (a curve hit takes t as its
+// fiber tangent).  flags: AHIP_FLAG_HAIR_MARSCHNER selects the R/TT/TRT hair model.
+int orc_probe(const ahip_material* mat, uint32_t flags, uint32_t n, const float* in, float* out) {
+    Material m;
+    m.m = *mat;
+    m.init();
+    m.marschner = (flags & AHIP_FLAG_HAIR_MARSCHNER) != 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 19 * i;
+        SI si;
+        si.n = V3{q[0], q[1], q[2]};
+        si.ng = si.n;
+        si.dpdu = V3{q[3], q[4], q[5]};
+        si.color = V3{q[6], q[7], q[8]};
+        si.isCurve = q[18] != 0.f;
+        V3 wo{q[9], q[10], q[11]}, wi{q[12], q[13], q[14]};
+        Ctx c(si, -wo);
+        BSDFEval e = m.evaluate(c, wo, wi);
+        BSDFSample bs = m.sample(c, wo, V2{q[15], q[16]}, q[17]);
+        V3 sa = m.m.type == AHIP_MAT_HAIR ? m.hairSigmaA(c) : V3{};
+        float* o = out + 16 * i;
+        o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf; o[4] = e.pdfRev;
+        o[5] = bs.wi.x; o[6] = bs.wi.y; o[7] = bs.wi.z;
+        o[8] = bs.f.x; o[9] = bs.f.y; o[10] = bs.f.z; o[11] = bs.pdf; o[12] = bs.valid() ? 1.f : 0.f;
+        o[13] = sa.x; o[14] = sa.y; o[15] = sa.z;
+    }
+    return 0;
+}
+
+// The Marschner pieces one by one, in the layouts of the reference driver's probes:
+// constants v[3], s, alpha[3]; the free helpers by index (see orc_mh_fn); A_p; the lobe sums and
+// pdf; the fiber geometry.
+int orc_hair_consts(const ahip_material* mat, float* out) {
+    mh::Consts c;
+    c.init(mat->hair_beta_m, mat->hair_beta_n, mat->hair_alpha);
+    for (int p = 0; p < 3; ++p) out[p] = c.v[p];
+    out[3] = c.s;
+    for (int p = 0; p < 3; ++p) out[4 + p] = c.alpha[p];
+    return 0;
+}
+// 0 Mp(a,b,c,d,e)  1 Np(a,(int)b,c,d,e)  2 trimmedLogistic(a,b)  3 sampleTrimmedLogistic(a,b)
+// 4 sampleGaussian(a,b)  5 logI0(a)  6 I0(a)  7 fresnelDielectric(a,b)  8 Phi((int)a,b,c)
+int orc_mh_fn(int which, uint32_t n, const float* a, const float* b, const float* c, const float* d, const float* e,
+              float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = mh::Mp(a[i], b[i], c[i], d[i], e[i]); break;
+            case 1: out[i] = mh::Np(a[i], (int)b[i], c[i], d[i], e[i]); break;
+            case 2: out[i] = mh::trimmedLogistic(a[i], b[i]); break;
+            case 3: out[i] = mh::sampleTrimmedLogistic(a[i], b[i]); break;
+            case 4: out[i] = mh::sampleGaussian(a[i], b[i]); break;
+            case 5: out[i] = mh::logI0(a[i]); break;
+            case 6: out[i] = mh::I0(a[i]); break;
+            case 7: out[i] = fresnelDielectric(a[i], b[i]); break;
+            case 8: out[i] = mh::Phi((int)a[i], b[i], c[i]); break;
+            default: return 1;
+        }
+    }
+    return 0;
+}
+int orc_mh_ap(uint32_t n, const float* in, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 6 * i;
+        V3 ap[3];
+        mh::Ap(q[0], q[1], q[2], V3{q[3], q[4], q[5]}, ap);
+        for (int p = 0; p < 3; ++p)
+            for (int k = 0; k < 3; ++k) out[9 * i + 3 * p + k] = ap[p][k];
+    }
+    return 0;
+}
+int orc_hair_lobes(const ahip_material* mat, uint32_t n, const float* in, float* out) {
+    mh::Consts c;
+    c.init(mat->hair_beta_m, mat->hair_beta_n, mat->hair_alpha);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 9 * i;
+        V3 sa{q[6], q[7], q[8]};
+        V3 f = c.lobes(mat->hair_eta, q[0], q[1], q[2], q[3], q[4], q[5], sa);
+        out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
+        out[4 * i + 3] = c.pdf(mat->hair_eta, q[0], q[1], q[2], q[3], q[4], q[5], sa);
+    }
+    return 0;
+}
+int orc_hair_geom(uint32_t n, const float* in, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 12 * i;
+        V3 wo{q[0], q[1], q[2]}, wi{q[3], q[4], q[5]}, t{q[6], q[7], q[8]}, nn{q[9], q[10], q[11]};
+        float sO, cO, sI, cI;
+        mh::longitudinal(wo, t, sO, cO);
+        mh::longitudinal(wi, t, sI, cI);
+        float* o = out + 6 * i;
+        o[0] = sO; o[1] = cO; o[2] = sI; o[3] = cI;
+        o[4] = mh::azimuth(wo, wi, t, sO, sI);
+        o[5] = mh::impact(wo, t, nn, sO);
+    }
+    return 0;
+}
+
+// A dome over an RGB f32 equirect image, outside any scene (the reference driver's layout):
+// create / destroy; sample u[n*2] -> out[n*7] = wi, pdf, Li rgb; eval dirs[n*3] -> out[n*4] =
+// Le rgb, pdf.
+void* orc_dome_create(uint32_t w, uint32_t h, const float* rgb, float intensity, const float* rot) {
+    auto* d = new Dome();
+    d->W = w;
+    d->H = h;
+    d->px.assign(rgb, rgb + (size_t)w * h * 3);
+    d->intensity = intensity;
+    for (int r = 0; r < 3; ++r) d->rot[r] = V3{rot[3 * r], rot[3 * r + 1], rot[3 * r + 2]};
+    d->build();
+    return d;
+}
+void orc_dome_destroy(void* p) { delete static_cast<Dome*>(p); }
+int orc_dome_sample_raw(void* p, uint32_t n, const float* u, float* out) {
+    const Dome& d = *static_cast<Dome*>(p);
+    for (uint32_t i = 0; i < n; ++i) {
+        float pdf;
+        V3 wi = d.sampleDir(V2{u[2 * i], u[2 * i + 1]}, pdf);
+        V3 li = pdf > 0.f ? d.eval(wi) * d.intensity : V3{};
+        if (!(pdf > 0.f)) wi = V3{};
+        float* o = out + 7 * i;
+        o[0] = wi.x; o[1] = wi.y; o[2] = wi.z; o[3] = pdf; o[4] = li.x; o[5] = li.y; o[6] = li.z;
+    }
+    return 0;
+}
+int orc_dome_eval_raw(void* p, uint32_t n, const float* dirs, float* out) {
+    const Dome& d = *static_cast<Dome*>(p);
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 w{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+        V3 le = d.eval(w) * d.intensity;
+        out[4 * i] = le.x; out[4 * i + 1] = le.y; out[4 * i + 2] = le.z;
+        out[4 * i + 3] = d.evalPdf(w);
+    }
+    return 0;
+}
+
+// One GGX lobe (StandardSurface.h:578-603), the reference driver's layout:
+// in[n*10] = woLocal, wiLocal, alpha2, f0 rgb; out[n*5] = f rgb, pdfFwd, 0.
+int orc_eval_ggx(uint32_t n, const float* in, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 10 * i;
+        float pf;
+        V3 f = Material::evalGGX(V3{q[0], q[1], q[2]}, V3{q[3], q[4], q[5]}, q[6], V3{q[7], q[8], q[9]}, pf);
+        out[5 * i] = f.x; out[5 * i + 1] = f.y; out[5 * i + 2] = f.z; out[5 * i + 3] = pf; out[5 * i + 4] = 0.f;
     }
     return 0;
 }

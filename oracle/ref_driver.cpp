@@ -8,14 +8,24 @@
 //
 // What it can drive: PCGRng, HaltonSampler, Camera::makePinhole, BVHBackend (commit/trace),
 // PathIntegrator::renderTile with LambertianMaterial / EmissiveMaterial / AreaLight and a
-// constant background, and the RenderSession non-adaptive tile loop on the reference
-// ThreadPool (RenderSession.cpp:511-547).  StandardSurface (needs evalTOV from the
-// OIIO-only Texture.cpp), DomeLight (loadImage lives in the OIIO-only DomeLight.cpp) and
-// CurveBrute (needs spdlog) are not linkable here without stand-ins, so they are not built.
+// constant background, the RenderSession non-adaptive tile loop on the reference ThreadPool
+// (RenderSession.cpp:511-547), the Kajiya-Kay hair material (MarschnerHairMaterial), and the parts
+// of StandardSurfaceMaterial and DomeLight that do not need OpenImageIO:
+//   * StandardSurfaceMaterial's virtual sample/evaluate call evalTOV, defined only in the
+//     OIIO-only Texture.cpp, and constructing the class would emit its vtable and so need that
+//     symbol.  Its private lobe helpers (sampleGGX, sampleDiffuse, evalGGX, evalCombined,
+//     StandardSurface.h:511-661) read only member data, so they run here on an object whose
+//     constructor is not called: the members are set as the constructor sets them (:152-160) and
+//     the opaque branch of sample / evaluate above the helpers (:231-241, 351-405, 446-503) is
+//     restated for constant parameters.
+//   * DomeLight's constructor calls loadImage, defined in the OIIO-only DomeLight.cpp.  Its
+//     distribution build, sampling and lookup (DomeLight.h:117-133, 168-171, 228-364) are header
+//     code; they run here on an object whose constructor is not called, with the pixels the loader
+//     would have produced set directly.
+// CurveBrute.cpp includes spdlog, which this image lacks: it is not built.
 //
-// The reference BVH keeps its nodes private; to dump the built tree for topology checks
-// this one translation unit reads them through an access macro applied only to the
-// reference headers it includes (no reference code is changed or replaced).
+// Private members are read through an access macro applied only to the reference headers this
+// translation unit includes (no reference code is changed or replaced).
 
 #include <algorithm>
 #include <array>
@@ -38,14 +48,15 @@
 
 #define private public
 #include "accel/BVHBackend.h"
+#include "shading/StandardSurface.h"    // free GGX helpers; the material's private lobe helpers
+#include "shading/MarschnerHair.h"      // the runtime (Kajiya-Kay) hair material
+#include "shading/lights/DomeLight.h"   // Distribution1D; the dome's private sampling / lookup
 #undef private
 #include "integrator/PathIntegrator.h"
 #include "sampling/HaltonSampler.h"
 #include "sampling/PCGRng.h"
 #include "shading/Lambertian.h"
 #include "shading/lights/AreaLight.h"
-#include "shading/lights/DomeLight.h"   // Distribution1D only (header-inline)
-#include "shading/StandardSurface.h"    // free GGX helpers only (header-inline)
 #include "render/ThreadPool.h"
 #include <anacapa/film/Film.h>
 
@@ -385,6 +396,281 @@ int ref_lambert(const float* albedo, uint32_t n, const float* in, float* out) {
         o[6] = bs.isValid() ? 1.f : 0.f;
         o[7] = be.pdf;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------- hair (Kajiya-Kay at run time)
+// Material probe on a curve hit, the layout every probe of this repository shares:
+// in[n*19]  = normal n, tangent t, strand colour, wo, wi, u.x, u.y, uComponent, curve (1 here)
+// out[n*16] = evaluate(wo, wi): f rgb, pdf, pdfRev | sample(wo, u, uc): wi, f rgb, pdf, valid |
+//             effectiveSigmaA rgb
+int ref_hair_kk_probe(const ahip_material* mat, uint32_t n, const float* in, float* out) {
+    MarschnerHairMaterial::Params p;
+    p.eta = mat->hair_eta;
+    p.sigma_a = v3(mat->hair_sigma_a);
+    p.beta_m = mat->hair_beta_m;
+    p.beta_n = mat->hair_beta_n;
+    p.alpha = mat->hair_alpha;
+    MarschnerHairMaterial m(p);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 19 * i;
+        SurfaceInteraction si;
+        si.n = v3(q);
+        si.ng = si.n;
+        si.dpdu = v3(q + 3);
+        si.color = v3(q + 6);
+        si.isCurve = true;
+        Vec3f wo = v3(q + 9), wi = v3(q + 12);
+        ShadingContext ctx(si, -wo);
+        BSDFEval e = m.evaluate(ctx, wo, wi);
+        BSDFSample s = m.sample(ctx, wo, Vec2f{q[15], q[16]}, q[17]);
+        Spectrum sa = m.effectiveSigmaA(ctx);
+        float* o = out + 16 * i;
+        o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf; o[4] = e.pdfRev;
+        o[5] = s.wi.x; o[6] = s.wi.y; o[7] = s.wi.z;
+        o[8] = s.f.x; o[9] = s.f.y; o[10] = s.f.z; o[11] = s.pdf; o[12] = s.isValid() ? 1.f : 0.f;
+        o[13] = sa.x; o[14] = sa.y; o[15] = sa.z;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- StandardSurface (opaque, constant)
+namespace {
+// Storage whose StandardSurfaceMaterial member is never constructed (see the header comment).
+union SSStorage {
+    SSStorage() {}
+    ~SSStorage() {}
+    StandardSurfaceMaterial m;
+};
+
+struct SSHolder {
+    SSStorage s;
+    explicit SSHolder(const ahip_material& a) {
+        StandardSurfaceMaterial& m = s.m;
+        new (&m.m_p) StandardSurfaceMaterial::Params();
+        m.m_p.base_color = Spectrum{a.albedo[0], a.albedo[1], a.albedo[2]};
+        m.m_p.base = a.base;
+        m.m_p.metalness = a.metalness;
+        m.m_p.roughness = a.roughness;
+        m.m_p.specular = a.specular;
+        m.m_p.specular_color = v3(a.specular_color);
+        m.m_p.specular_IOR = a.specular_ior;
+        m.m_p.coat = a.coat;
+        m.m_p.coat_roughness = a.coat_roughness;
+        m.m_p.transmission = a.transmission;
+        // StandardSurfaceMaterial(const Params&), StandardSurface.h:152-160
+        float r = std::max(1e-4f, m.m_p.roughness.value);
+        m.m_alpha = r * r;
+        m.m_alpha2 = m.m_alpha * m.m_alpha;
+        m.m_coatAlpha = std::max(1e-4f, m.m_p.coat_roughness * m.m_p.coat_roughness);
+        m.m_coatAlpha2 = m.m_coatAlpha * m.m_coatAlpha;
+        m.m_f0Dielectric = F0_fromIOR(m.m_p.specular_IOR);
+        m.m_coatF0 = F0_fromIOR(1.5f);
+    }
+    ~SSHolder() { s.m.m_p.~Params(); }
+    SSHolder(const SSHolder&) = delete;
+    SSHolder& operator=(const SSHolder&) = delete;
+
+    // Layer weights at wo (StandardSurface.h:355-371 and 478-489, identical in sample and evaluate).
+    bool weights(float woz, float metal, float spec, float& wCoat, float& wMetal, float& wSpec,
+                 float& wDiff) const {
+        const StandardSurfaceMaterial& m = s.m;
+        float coatF = schlickDielectric(woz, m.m_coatF0);
+        float specF = schlickDielectric(woz, m.m_f0Dielectric);
+        wCoat = m.m_p.coat * coatF;
+        wMetal = metal * (1.f - wCoat);
+        wSpec = spec * (1.f - metal) * specF * (1.f - wCoat);
+        wDiff = m.m_p.base * (1.f - metal) * (1.f - spec * specF) * (1.f - wCoat);
+        float wSum = wCoat + wMetal + wSpec + wDiff;
+        if (wSum <= 0.f) return false;
+        float invW = 1.f / wSum;
+        wCoat *= invW;
+        wMetal *= invW;
+        wSpec *= invW;
+        wDiff *= invW;
+        return true;
+    }
+
+    // evaluate, opaque branch with constant parameters (StandardSurface.h:410-417, 446-503)
+    BSDFEval evaluate(const ShadingContext& ctx, Vec3f wo, Vec3f wi) const {
+        const StandardSurfaceMaterial& m = s.m;
+        Vec3f woLocal = ctx.toLocal(wo), wiLocal = ctx.toLocal(wi);
+        if (!sameHemisphere(woLocal, wiLocal)) return {};
+        Spectrum base_color = m.m_p.base_color.value;
+        float metal = m.m_p.metalness.value, spec = m.m_p.specular.value, rough = m.m_p.roughness.value;
+        float alpha = std::max(1e-4f, rough * rough);
+        float alpha2 = alpha * alpha;
+        float wCoat, wMetal, wSpec, wDiff;
+        if (!weights(woLocal.z, metal, spec, wCoat, wMetal, wSpec, wDiff)) return {};
+        float pdfFwd, pdfRev;
+        Spectrum f = m.evalCombined(ctx, woLocal, wiLocal, wCoat, wMetal, wSpec, wDiff, base_color, spec, alpha2,
+                                    pdfFwd, pdfRev);
+        BSDFEval e;
+        e.f = f;
+        e.pdf = pdfFwd;
+        e.pdfRev = pdfRev;
+        return e;
+    }
+
+    // sample, opaque branch with constant parameters and no normal map (StandardSurface.h:231-241,
+    // 351-405)
+    BSDFSample sample(const ShadingContext& ctx, Vec3f wo, Vec2f u, float uComponent) const {
+        const StandardSurfaceMaterial& m = s.m;
+        const ShadingContext& sctx = ctx;
+        Vec3f woLocal = sctx.toLocal(wo);
+        if (woLocal.z <= 0.f) return {};
+        Spectrum base_color = m.m_p.base_color.value;
+        float metal = m.m_p.metalness.value, spec = m.m_p.specular.value, rough = m.m_p.roughness.value;
+        float alpha = std::max(1e-4f, rough * rough);
+        float alpha2 = alpha * alpha;
+        float wCoat, wMetal, wSpec, wDiff;
+        if (!weights(woLocal.z, metal, spec, wCoat, wMetal, wSpec, wDiff)) return {};
+        BSDFSample result;
+        if (uComponent < wCoat) {
+            result = m.sampleGGX(sctx, woLocal, u, m.m_coatAlpha, m.m_coatAlpha2,
+                                 Spectrum{m.m_coatF0, m.m_coatF0, m.m_coatF0}, false);
+        } else if (uComponent < wCoat + wMetal) {
+            result = m.sampleGGX(sctx, woLocal, u, alpha, alpha2, base_color * m.m_p.base, false);
+        } else if (uComponent < wCoat + wMetal + wSpec) {
+            Spectrum f0 = m.m_p.specular_color * (spec * m.m_f0Dielectric);
+            result = m.sampleGGX(sctx, woLocal, u, alpha, alpha2, f0, false);
+        } else {
+            result = m.sampleDiffuse(sctx, woLocal, u, base_color);
+        }
+        if (!result.isValid()) return {};
+        float pdfFwd = 0.f, pdfRev = 0.f;
+        Spectrum fCombined = m.evalCombined(ctx, woLocal, sctx.toLocal(result.wi), wCoat, wMetal, wSpec, wDiff,
+                                            base_color, spec, alpha2, pdfFwd, pdfRev);
+        result.f = fCombined * std::abs(sctx.toLocal(result.wi).z);
+        result.pdf = pdfFwd;
+        result.pdfRev = pdfRev;
+        return result;
+    }
+};
+}  // namespace
+
+// The probe layout above, on a surface hit (ShadingContext from n alone: t, bt by
+// buildOrthonormalBasis, ShadingContext.h:45-47); the sigma_a slots are zero.
+int ref_ss_probe(const ahip_material* mat, uint32_t n, const float* in, float* out) {
+    if (mat->type != AHIP_MAT_STANDARD || mat->transmission > 0.001f) return 1;
+    SSHolder h(*mat);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 19 * i;
+        SurfaceInteraction si;
+        si.n = v3(q);
+        si.ng = si.n;
+        Vec3f wo = v3(q + 9), wi = v3(q + 12);
+        ShadingContext ctx(si, -wo);
+        BSDFEval e = h.evaluate(ctx, wo, wi);
+        BSDFSample s = h.sample(ctx, wo, Vec2f{q[15], q[16]}, q[17]);
+        float* o = out + 16 * i;
+        o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf; o[4] = e.pdfRev;
+        o[5] = s.wi.x; o[6] = s.wi.y; o[7] = s.wi.z;
+        o[8] = s.f.x; o[9] = s.f.y; o[10] = s.f.z; o[11] = s.pdf; o[12] = s.isValid() ? 1.f : 0.f;
+        o[13] = o[14] = o[15] = 0.f;
+    }
+    return 0;
+}
+
+// One GGX lobe through the material's own evalGGX (StandardSurface.h:578-603):
+// in[n*10] = woLocal, wiLocal, alpha2, f0 rgb; out[n*5] = f rgb, pdfFwd, pdfRev.
+int ref_ss_eval_ggx(uint32_t n, const float* in, float* out) {
+    ahip_material a{};
+    a.type = AHIP_MAT_STANDARD;
+    a.base = 1.f;
+    a.roughness = 0.5f;
+    a.specular_ior = 1.5f;
+    SSHolder h(a);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* q = in + 10 * i;
+        float pf, pr;
+        Spectrum f = h.s.m.evalGGX(v3(q), v3(q + 3), q[6], v3(q + 7), pf, pr);
+        out[5 * i] = f.x; out[5 * i + 1] = f.y; out[5 * i + 2] = f.z; out[5 * i + 3] = pf; out[5 * i + 4] = pr;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- DomeLight (pixels set directly)
+namespace {
+union DomeStorage {
+    DomeStorage() {}
+    ~DomeStorage() {}
+    DomeLight d;
+};
+struct DomeHolder {
+    DomeStorage s;
+    DomeHolder(uint32_t w, uint32_t h, const float* rgb, float intensity, const float* rot) {
+        DomeLight& d = s.d;
+        new (&d.m_pixels) std::vector<float>(rgb, rgb + (size_t)w * h * 3);
+        new (&d.m_conditional) std::vector<Distribution1D>();
+        new (&d.m_marginal) Distribution1D();
+        d.m_width = w;
+        d.m_height = h;
+        d.m_intensity = intensity;
+        d.m_sceneRadius = 10.f;
+        d.m_sceneCenter = Vec3f{0.f, 0.f, 0.f};
+        d.m_envIntegral = 0.f;
+        // DomeLight(...) identity rotation then setRotation (DomeLight.h:95, 105-112)
+        d.m_rot[0] = v3(rot);
+        d.m_rot[1] = v3(rot + 3);
+        d.m_rot[2] = v3(rot + 6);
+        d.buildDistribution();
+    }
+    ~DomeHolder() {
+        DomeLight& d = s.d;
+        using PixVec = std::vector<float>;
+        using CondVec = std::vector<Distribution1D>;
+        d.m_pixels.~PixVec();
+        d.m_conditional.~CondVec();
+        d.m_marginal.~Distribution1D();
+    }
+    DomeHolder(const DomeHolder&) = delete;
+    DomeHolder& operator=(const DomeHolder&) = delete;
+};
+}  // namespace
+
+// A dome over an RGB f32 equirect image (row 0 = zenith) and a world->envmap rotation (3 rows).
+void* ref_dome_create(uint32_t w, uint32_t h, const float* rgb, float intensity, const float* rot) {
+    return new DomeHolder(w, h, rgb, intensity, rot);
+}
+void ref_dome_destroy(void* p) { delete static_cast<DomeHolder*>(p); }
+
+// DomeLight::sample (DomeLight.h:117-128): u[n*2] -> out[n*7] = wi, pdf, Li rgb.
+int ref_dome_sample(void* p, uint32_t n, const float* u, float* out) {
+    const DomeLight& d = static_cast<DomeHolder*>(p)->s.d;
+    for (uint32_t i = 0; i < n; ++i) {
+        LightSample ls = d.DomeLight::sample(Vec3f{0.f, 0.f, 0.f}, Vec3f{0.f, 1.f, 0.f}, Vec2f{u[2 * i], u[2 * i + 1]});
+        float* o = out + 7 * i;
+        o[0] = ls.wi.x; o[1] = ls.wi.y; o[2] = ls.wi.z; o[3] = ls.pdf;
+        o[4] = ls.Li.x; o[5] = ls.Li.y; o[6] = ls.Li.z;
+    }
+    return 0;
+}
+
+// DomeLight::Le and DomeLight::pdf (DomeLight.h:130-133, 168-171): dirs[n*3] -> out[n*4] = Le rgb, pdf.
+int ref_dome_eval(void* p, uint32_t n, const float* dirs, float* out) {
+    const DomeLight& d = static_cast<DomeHolder*>(p)->s.d;
+    for (uint32_t i = 0; i < n; ++i) {
+        Vec3f w = v3(dirs + 3 * i);
+        Spectrum le = d.DomeLight::Le(Vec3f{}, Vec3f{}, w);
+        out[4 * i] = le.x; out[4 * i + 1] = le.y; out[4 * i + 2] = le.z;
+        out[4 * i + 3] = d.DomeLight::pdf(Vec3f{}, w);
+    }
+    return 0;
+}
+
+// The built distribution (DomeLight.h:228-259): marginal func[H] and cdf[H+1], conditional
+// func[H*W] and cdf[H*(W+1)], the marginal integral.
+int ref_dome_tables(void* p, float* margFunc, float* margCdf, float* condFunc, float* condCdf, float* margInt) {
+    const DomeLight& d = static_cast<DomeHolder*>(p)->s.d;
+    std::memcpy(margFunc, d.m_marginal.func.data(), sizeof(float) * d.m_height);
+    std::memcpy(margCdf, d.m_marginal.cdf.data(), sizeof(float) * (d.m_height + 1));
+    for (uint32_t r = 0; r < d.m_height; ++r) {
+        std::memcpy(condFunc + (size_t)r * d.m_width, d.m_conditional[r].func.data(), sizeof(float) * d.m_width);
+        std::memcpy(condCdf + (size_t)r * (d.m_width + 1), d.m_conditional[r].cdf.data(),
+                    sizeof(float) * (d.m_width + 1));
+    }
+    *margInt = d.m_marginal.integral;
     return 0;
 }
 

@@ -23,7 +23,100 @@ def _load(path):
     return C.CDLL(path)
 
 
-class Oracle:
+PROBE_IN, PROBE_OUT = 19, 16   # material probe layout (oracle.cpp / ref_driver.cpp / ahip_material_probe)
+
+
+def probe_inputs(n_, t, color, wo, wi, u, uc, curve):
+    """Pack the shared material-probe input rows: normal, tangent, colour, wo, wi, u(2), uc, curve."""
+    m = len(wo)
+    out = np.zeros((m, PROBE_IN), np.float32)
+    out[:, 0:3], out[:, 3:6], out[:, 6:9] = n_, t, color
+    out[:, 9:12], out[:, 12:15], out[:, 15:17] = wo, wi, u
+    out[:, 17], out[:, 18] = uc, curve
+    return out
+
+
+class _ShadingProbes:
+    """Shading probes both CPU libraries export under their own prefix (orc_ / ref_)."""
+
+    prefix = ""
+
+    def _bind_probes(self):
+        L, p = self.lib, self.prefix
+        getattr(L, p + "hair_consts").argtypes = [C.POINTER(A.Material), VP]
+        getattr(L, p + "mh_fn").argtypes = [C.c_int, C.c_uint32, VP, VP, VP, VP, VP, VP]
+        getattr(L, p + "mh_ap").argtypes = [C.c_uint32, VP, VP]
+        getattr(L, p + "hair_lobes").argtypes = [C.POINTER(A.Material), C.c_uint32, VP, VP]
+        getattr(L, p + "hair_geom").argtypes = [C.c_uint32, VP, VP]
+        getattr(L, p + "dome_create").restype = VP
+        getattr(L, p + "dome_create").argtypes = [C.c_uint32, C.c_uint32, VP, C.c_float, VP]
+        getattr(L, p + "dome_destroy").argtypes = [VP]
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def hair_consts(self, mat):
+        out = np.zeros(7, np.float32)
+        self._fn("hair_consts")(C.byref(mat), P(out))
+        return out
+
+    def mh_fn(self, which, a, b=None, c=None, d=None, e=None):
+        a = np.ascontiguousarray(a, np.float32)
+        args = [np.ascontiguousarray(a if x is None else x, np.float32) for x in (b, c, d, e)]
+        out = np.zeros(len(a), np.float32)
+        self._fn("mh_fn")(which, len(a), P(a), *[P(x) for x in args], P(out))
+        return out
+
+    def mh_ap(self, inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), 9), np.float32)
+        self._fn("mh_ap")(len(inp), P(inp), P(out))
+        return out
+
+    def hair_lobes(self, mat, inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), 4), np.float32)
+        self._fn("hair_lobes")(C.byref(mat), len(inp), P(inp), P(out))
+        return out
+
+    def hair_geom(self, inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), 6), np.float32)
+        self._fn("hair_geom")(len(inp), P(inp), P(out))
+        return out
+
+    def dome(self, rgb, intensity=1.0, rot=None):
+        return _Dome(self, rgb, intensity, rot)
+
+
+class _Dome:
+    def __init__(self, owner, rgb, intensity, rot):
+        self.o = owner
+        self.rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w = self.rgb.shape[:2]
+        self.w, self.h = w, h
+        r = np.ascontiguousarray(np.eye(3) if rot is None else rot, np.float32).reshape(9)
+        self.p = owner._fn("dome_create")(w, h, P(self.rgb), float(intensity), P(r))
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            self.o._fn("dome_destroy")(self.p)
+            self.p = None
+
+    def sample(self, u):
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        out = np.zeros((len(u), 7), np.float32)
+        self.o._fn(self.o.dome_sample_name)(self.p, len(u), P(u), P(out))
+        return out
+
+    def eval(self, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((len(dirs), 4), np.float32)
+        self.o._fn(self.o.dome_eval_name)(self.p, len(dirs), P(dirs), P(out))
+        return out
+
+
+class Oracle(_ShadingProbes):
     """This repository's C++ restatement of the reference CPU path (oracle/oracle.cpp)."""
 
     def __init__(self):
@@ -62,6 +155,26 @@ class Oracle:
         L.orc_set_eval.argtypes = [VP, VP, C.c_uint32, VP]
         L.orc_replay_tile.argtypes = [VP] + [C.c_uint32] * 6 + [VP]
         L.orc_adjoint.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP]
+        L.orc_probe.argtypes = [C.POINTER(A.Material), C.c_uint32, C.c_uint32, VP, VP]
+        L.orc_eval_ggx.argtypes = [C.c_uint32, VP, VP]
+        L.orc_dome_sample_raw.argtypes = [VP, C.c_uint32, VP, VP]
+        L.orc_dome_eval_raw.argtypes = [VP, C.c_uint32, VP, VP]
+        self._bind_probes()
+
+    prefix = "orc_"
+    dome_sample_name, dome_eval_name = "dome_sample_raw", "dome_eval_raw"
+
+    def probe(self, mat, inp, flags=0):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), PROBE_OUT), np.float32)
+        self.lib.orc_probe(C.byref(mat), flags, len(inp), P(inp), P(out))
+        return out
+
+    def eval_ggx(self, inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), 5), np.float32)
+        self.lib.orc_eval_ggx(len(inp), P(inp), P(out))
+        return out
 
     # ---- stateless
     def pcg(self, seed, seq, n):
@@ -236,7 +349,7 @@ class OracleScene:
         return rgb, pdf
 
 
-class Reference:
+class Reference(_ShadingProbes):
     """The reference's own BVHBackend / PathIntegrator / Halton, built by oracle/Makefile."""
 
     def __init__(self):
@@ -264,6 +377,45 @@ class Reference:
         L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
         L.ref_ggx_helper.argtypes = [C.c_int, C.c_uint32, VP, VP, VP, VP]
         L.ref_lambert.argtypes = [VP, C.c_uint32, VP, VP]
+        for n in ("ref_hair_kk_probe", "ref_hair_live_probe", "ref_ss_probe"):
+            getattr(L, n).argtypes = [C.POINTER(A.Material), C.c_uint32, VP, VP]
+        L.ref_ss_eval_ggx.argtypes = [C.c_uint32, VP, VP]
+        L.ref_dome_sample.argtypes = [VP, C.c_uint32, VP, VP]
+        L.ref_dome_eval.argtypes = [VP, C.c_uint32, VP, VP]
+        L.ref_dome_tables.argtypes = [VP, VP, VP, VP, VP, VP]
+        self._bind_probes()
+
+    prefix = "ref_"
+    dome_sample_name, dome_eval_name = "dome_sample", "dome_eval"
+
+    def probe(self, mat, inp, flags=0):
+        """The reference's material on probe rows: StandardSurface (opaque, constant parameters),
+        or hair as it runs (Kajiya-Kay) / with its Marschner code reached (AHIP_FLAG_HAIR_MARSCHNER)."""
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), PROBE_OUT), np.float32)
+        if mat.type == A.AHIP_MAT_STANDARD:
+            fn = self.lib.ref_ss_probe
+        elif mat.type == A.AHIP_MAT_HAIR:
+            fn = self.lib.ref_hair_live_probe if flags & A.AHIP_FLAG_HAIR_MARSCHNER else self.lib.ref_hair_kk_probe
+        else:
+            raise ValueError("no reference probe for this material type")
+        if fn(C.byref(mat), len(inp), P(inp), P(out)) != 0:
+            raise ValueError("reference probe refused the material")
+        return out
+
+    def ss_eval_ggx(self, inp):
+        inp = np.ascontiguousarray(inp, np.float32)
+        out = np.zeros((len(inp), 5), np.float32)
+        self.lib.ref_ss_eval_ggx(len(inp), P(inp), P(out))
+        return out
+
+    def dome_tables(self, dome):
+        w, h = dome.w, dome.h
+        mf, mc = np.zeros(h, np.float32), np.zeros(h + 1, np.float32)
+        cf, cc = np.zeros((h, w), np.float32), np.zeros((h, w + 1), np.float32)
+        mi = np.zeros(1, np.float32)
+        self.lib.ref_dome_tables(dome.p, P(mf), P(mc), P(cf), P(cc), P(mi))
+        return mf, mc, cf, cc, float(mi[0])
 
     def pcg(self, seed, seq, n):
         out = np.zeros(n, np.uint32)

@@ -109,3 +109,67 @@ def test_hair_render_is_finite(oracle):
     os_ = oracle.scene(sc)
     img = os_.render_tile(0, 0, 24, 24, 0, 1)
     assert np.isfinite(img).all() and img[..., :3].sum() > 0
+
+
+def _bezier(P, t):
+    t = np.asarray(t, np.float64)[..., None]
+    mt = 1 - t
+    return mt ** 3 * P[0] + 3 * mt * mt * t * P[1] + 3 * mt * t * t * P[2] + t ** 3 * P[3]
+
+
+def test_curved_segment_hits_lie_on_the_analytic_bezier(oracle):
+    """Curved cubic segments against the analytic curve.  The ribbon test subdivides a segment to
+    depth 6 and intersects each leaf's chord (CurveBrute.cpp:85-176); the chord at leaf parameter
+    s is the linear interpolation of the curve between the leaf's ends, which differs from the
+    curve at the same parameter by at most E = h^2 / 8 max|B''| (h = 1/64, B'' = 6 (1 - t) D0 +
+    6 t D1 with D0 = P0 - 2 P1 + P2, D1 = P1 - 2 P2 + P3).  So every hit's point B(v) lies within
+    hw(v) + E of the ray in the plane across it and its depth within E of t; rays passing the
+    curve at less than hw - 2 E must hit and rays farther than hw + 2 E from all of it must miss.
+    A dropped de Casteljau term, a wrong v mapping or a wrong depth interpolation fails these."""
+    segs = [
+        np.array([[0, 0, 0], [0, 1.2, 0], [1.0, 1.2, 0.3], [1.0, 0, 0.5]]),      # a tight arch
+        np.array([[0, 0, 0], [0.8, 0.9, -0.4], [-0.3, 0.6, 0.6], [0.6, 0.1, 0.2]]),  # an S with twist
+    ]
+    width = 0.05
+    rng = np.random.default_rng(11)
+    for P in segs:
+        sc = _one_strand(P.tolist(), [width, width])
+        os_ = oracle.scene(sc)
+        d0, d1 = P[0] - 2 * P[1] + P[2], P[1] - 2 * P[2] + P[3]
+        E = (1 / 64) ** 2 / 8 * 6 * max(np.linalg.norm(d0), np.linalg.norm(d1))
+        hw = width / 2
+        assert E < hw / 4                               # the bound is informative at this width
+        n = 1500
+        ts = rng.uniform(0.02, 0.98, n)
+        c = _bezier(P, ts)
+        dirs = rng.normal(size=(n, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        # offset across the ray, inside or outside the ribbon
+        side = np.cross(dirs, rng.normal(size=(n, 3)))
+        side /= np.linalg.norm(side, axis=1, keepdims=True)
+        off = rng.uniform(0, 2.0 * hw, n)
+        org = c + side * off[:, None] - dirs * 4.0
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7] = org, dirs, 1e-4, np.inf
+        h = os_.trace(rays)
+        hit = (h["mesh_id"] != 0xFFFFFFFF) & ((h["tri_index"] & 0x80000000) != 0)
+        # distance across each ray to the whole curve, densely sampled
+        grid = _bezier(P, np.linspace(0, 1, 4001))                       # (4001, 3)
+        o64, d64 = rays[:, :3].astype(np.float64), rays[:, 3:6].astype(np.float64)
+        rel = grid[None, :, :] - o64[:, None, :]
+        along = (rel * d64[:, None, :]).sum(-1)
+        perp = np.linalg.norm(rel - along[..., None] * d64[:, None, :], axis=-1)
+        dmin = perp.min(1)
+        inside = dmin < hw - 2 * E
+        outside = dmin > hw + 2 * E
+        assert hit[inside].all(), f"{(~hit[inside]).sum()} rays through the ribbon missed"
+        assert not hit[outside].any(), f"{hit[outside].sum()} rays beside the ribbon hit"
+        assert inside.sum() > 300 and outside.sum() > 300
+        # every hit: B(v) within hw + E across the ray, depth within E of t (plus float32 slack)
+        b = _bezier(P, h["v"][hit].astype(np.float64))
+        rb = b - o64[hit]
+        a = (rb * d64[hit]).sum(1)
+        across = np.linalg.norm(rb - a[:, None] * d64[hit], axis=1)
+        slack = 2e-5 * 4.0
+        assert (across <= hw + E + slack).all(), across.max() - hw
+        assert (np.abs(a - h["t"][hit]) <= E + slack).all(), np.abs(a - h["t"][hit]).max()
