@@ -156,6 +156,7 @@ SIGNATURES = {
     "ahip_occluded_rays": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_trace_rays_noalloc": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_halton_eval": (C.c_int, [VP, C.c_uint64, VP, VP, VP, VP, C.c_uint32, VP]),
+    "ahip_material_probe": (C.c_int, [VP, C.POINTER(Material), C.c_uint32, C.c_uint32, VP, VP]),
     "ahip_set_eval_params": (C.c_int, [VP, VP, C.c_uint32, VP]),
     "ahip_replay": (C.c_int, [VP, C.c_uint32, C.c_uint32, VP]),
     "ahip_adjoint": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP]),

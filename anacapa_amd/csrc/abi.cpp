@@ -588,6 +588,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.segIds = ctx->dSegIds.as<uint32_t>();
     S.strandInfo = ctx->dStrandInfo.as<float>();
     S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
+    S.hasHair = hs.hasHair ? 1u : 0u;
     if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes4.data(), ctx->bvh.nodes4.size()))) return st;
     for (DTri& t : ctx->bvh.tris)   // the material of each triangle, so shading skips the mesh table
         t.matIndex = t.meshID < hs.meshMat.size() ? hs.meshMat[t.meshID] : -1;
@@ -1138,6 +1139,32 @@ int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const
                  ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, d + 4 * (size_t)n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return AHIP_OK;
+}
+
+// IMaterial::evaluate / sample of one material on probe rows, on the device (the per-lobe parity
+// checks of the shading code against the reference's material classes).
+int ahip_material_probe(ahip_context* ctx, const ahip_material* mat, uint32_t flags, uint32_t n, const float* in,
+                        float* out) {
+    ENTER();
+    if (!mat) return fail(ctx, AHIP_E_INVALID_ARG, "null material");
+    if (n == 0) return AHIP_OK;
+    if (!in || !out) return fail(ctx, AHIP_E_INVALID_ARG, "null arrays");
+    if (n > (1u << 24)) return fail(ctx, AHIP_E_INVALID_ARG, "more than 2^24 probe rows");
+    DMat D;
+    std::string err;
+    int st = convertMaterial(*mat, 0u, D, err);
+    if (st) return fail(ctx, st, err);
+    DevBuf b;
+    const size_t inB = (size_t)n * 19u * sizeof(float), outB = (size_t)n * 16u * sizeof(float);
+    HIPCHK(b.alloc(inB + outB));
+    float* din = b.as<float>();
+    float* dout = din + (size_t)n * 19u;
+    HIPCHK(hipMemcpyAsync(din, in, inB, hipMemcpyHostToDevice, ctx->stream));
+    launchMaterialProbe(D, (flags & AHIP_FLAG_HAIR_MARSCHNER) != 0u, n, din, dout, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout, outB, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return AHIP_OK;
 }

@@ -65,6 +65,144 @@ __device__ __forceinline__ V3 hairKK(const DMat& m, const struct Frame& fr, V3 w
     pdf = fmaxf(0.f, dot(wi, fr.n)) * kInvPi;
     return f;
 }
+// ---- Marschner R / TT / TRT, the model MarschnerHairMaterial carries below its Kajiya-Kay returns
+// (MarschnerHair.h:49-185 helpers, :243-257 evaluate, :284-375 sample, :490-556 lobes and pdf),
+// selected by AHIP_FLAG_HAIR_MARSCHNER.  Per-material constants (variances, logistic scale, tilts)
+// are computed on the host as the constructor computes them.
+namespace mh {
+__device__ __forceinline__ float I0(float x) {                                  // :62-71
+    float sum = 0.f, x2i = 1.f, denom = 1.f;
+    const float xx = x * x;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        if (i > 0) {
+            x2i *= xx;
+            denom *= (float)i * (float)i * 4.f;
+        }
+        sum += x2i / denom;
+    }
+    return sum;
+}
+__device__ __forceinline__ float logI0(float x) {                               // :73-77
+    if (x > 12.f) return x + 0.5f * (-logf(k2Pi) + logf(1.f / x) + 1.f / (8.f * x));
+    return logf(I0(x));
+}
+__device__ __forceinline__ float Mp(float cosI, float sinI, float cosR, float sinR, float v) {   // :88-98
+    if (v < 1e-5f) v = 1e-5f;
+    float a = cosI * cosR / v, b = sinI * sinR / v;
+    if (v <= 0.1f) return expf(logI0(a) - b - 1.f / v + 0.6931472f + logf(0.5f / v));
+    return expf(-b) * I0(a) / (2.f * v * sinhf(1.f / v));
+}
+__device__ __forceinline__ float logisticCDF(float x, float s) { return 1.f / (1.f + expf(-x / s)); }
+__device__ __forceinline__ float trimmedLogistic(float x, float s) {            // :103-116
+    float e = expf(-fabsf(x) / s);
+    return (e / (s * (1.f + e) * (1.f + e))) / (logisticCDF(kPi, s) - logisticCDF(-kPi, s));
+}
+__device__ __forceinline__ float sampleTrimmedLogistic(float u, float s) {      // :119-124
+    float a = logisticCDF(-kPi, s), b = logisticCDF(kPi, s);
+    float x = -s * logf(1.f / (a + u * (b - a)) - 1.f);
+    return fminf(fmaxf(x, -kPi), kPi);
+}
+__device__ __forceinline__ float Phi(int p, float gO, float gT) {               // :129-131
+    return 2.f * (float)p * gT - 2.f * gO + (float)p * kPi;
+}
+__device__ __forceinline__ float Np(float phi, int p, float s, float gO, float gT) {   // :133-142
+    float x = phi - Phi(p, gO, gT);
+    while (x > kPi) x -= k2Pi;
+    while (x < -kPi) x += k2Pi;
+    return trimmedLogistic(x, s);
+}
+__device__ __forceinline__ float clampH(float h) { return fminf(fmaxf(h, -1.f + 1e-5f), 1.f - 1e-5f); }
+__device__ __forceinline__ float fresnel(float cosI, float eta) {              // :46-55
+    float s2i = fmaxf(0.f, 1.f - cosI * cosI);
+    float s2t = s2i / (eta * eta);
+    if (s2t >= 1.f) return 1.f;
+    float ct = sqrtf(1.f - s2t);
+    float rs = (cosI - eta * ct) / (cosI + eta * ct);
+    float rp = (eta * cosI - ct) / (eta * cosI + ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+// A_p of R, TT, TRT (:146-176)
+__device__ __forceinline__ void Ap(float cosO, float eta, float h, V3 sa, V3 ap[3]) {
+    float s2 = fmaxf(0.f, 1.f - cosO * cosO);
+    float etaP = sqrtf(fmaxf(0.f, eta * eta - s2)) / cosO;
+    float sgt = clampH(h / (etaP > 1e-5f ? etaP : 1e-5f));
+    float cgt = sqrtf(fmaxf(0.f, 1.f - sgt * sgt));
+    V3 T{expf(-sa.x * 2.f * cgt), expf(-sa.y * 2.f * cgt), expf(-sa.z * 2.f * cgt)};
+    float cgo = sqrtf(fmaxf(0.f, 1.f - h * h));
+    float fr = fresnel(fmaxf(0.f, cosO * cgo), eta);
+    ap[0] = V3{fr, fr, fr};
+    ap[1] = T * ((1.f - fr) * (1.f - fr));
+    ap[2] = ap[1] * T * fr;
+}
+__device__ __forceinline__ float lum(V3 s) { return 0.2722287f * s.x + 0.6740818f * s.y + 0.0536895f * s.z; }
+// azimuths of the entry point and of the refracted ray for impact parameter h (:498-502)
+__device__ __forceinline__ void gammas(float cosO, float eta, float h, float& gO, float& gT) {
+    float etaP = sqrtf(fmaxf(0.f, eta * eta - (1.f - cosO * cosO))) / cosO;
+    gO = asinf(clampH(h));
+    gT = asinf(clampH(h / (etaP > 1e-5f ? etaP : 1e-5f)));
+}
+// sum over p of M_p A_p N_p / max(1/4, cos^2 theta_d) (:490-522) and the lobe mixture pdf
+// (:528-556), sharing A_p and the azimuths
+__device__ __forceinline__ V3 lobes(const DMat& m, float sO, float cO, float sI, float cI, float phi, float h, V3 sa,
+                                    float& pdf) {
+    float cd = sqrtf(fmaxf(0.f, 0.5f * (1.f + cO * cI + sO * sI)));
+    float denom = fmaxf(0.25f, cd * cd);
+    float gO, gT;
+    gammas(cO, m.hairEta, h, gO, gT);
+    V3 ap[3];
+    Ap(cO, m.hairEta, h, sa, ap);
+    float w0 = lum(ap[0]), w1 = lum(ap[1]), w2 = lum(ap[2]);
+    float total = w0 + w1 + w2;
+    V3 f{0.f, 0.f, 0.f};
+    float pd = 0.f;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        float s2a, c2a;
+        sincosf(2.f * m.hairAlpha[p], &s2a, &c2a);
+        float so = sO * c2a + cO * s2a;
+        float co = sqrtf(fmaxf(0.f, 1.f - so * so));
+        float mp = Mp(cI, sI, co, so, m.hairV[p]);
+        float np = Np(phi, p, m.hairS, gO, gT);
+        f = f + ap[p] * (mp * np);
+        float wp = p == 0 ? w0 : (p == 1 ? w1 : w2);
+        pd += (wp / total) * (mp * cI) * np;
+    }
+    pdf = total < 1e-8f ? 0.f : fmaxf(0.f, pd);
+    return f * (1.f / denom);
+}
+__device__ __forceinline__ void longitudinal(V3 d, V3 t, float& s, float& c) {   // :456-460
+    s = dot(d, t);
+    c = sqrtf(fmaxf(0.f, 1.f - s * s));
+}
+__device__ __forceinline__ float azimuth(V3 wo, V3 wi, V3 t, float sO, float sI) {   // :463-475
+    V3 po = wo - t * sO, pi = wi - t * sI;
+    float lo = len(po), li = len(pi);
+    if (lo < 1e-5f || li < 1e-5f) return 0.f;
+    po = po * (1.f / lo);
+    pi = pi * (1.f / li);
+    float c = fminf(fmaxf(dot(po, pi), -1.f), 1.f);
+    float s = dot(cross(pi, po), t);
+    return atan2f(s, c);
+}
+__device__ __forceinline__ float impact(V3 wo, V3 t, V3 n, float sO) {          // :478-483
+    V3 po = wo - t * sO;
+    float l = len(po);
+    if (l < 1e-5f) return 0.f;
+    return clampH(dot(po * (1.f / l), n));
+}
+}  // namespace mh
+
+// evaluate (:243-257): f without cosine and the forward pdf
+__device__ __forceinline__ V3 hairMarschner(const DMat& m, const Frame& fr, V3 wo, V3 wi, float& pdf) {
+    float sO, cO, sI, cI;
+    mh::longitudinal(wo, fr.t, sO, cO);
+    mh::longitudinal(wi, fr.t, sI, cI);
+    float phi = mh::azimuth(wo, wi, fr.t, sO, sI);
+    float h = mh::impact(wo, fr.t, fr.n, sO);
+    return mh::lobes(m, sO, cO, sI, cI, phi, h, hairSigmaA(m, fr.color), pdf);
+}
+
 __device__ __forceinline__ V3 toLocal(const Frame& f, V3 v) { return V3{dot(v, f.t), dot(v, f.bt), dot(v, f.n)}; }
 __device__ __forceinline__ V3 toWorld(const Frame& f, V3 v) { return f.t * v.x + f.bt * v.y + f.n * v.z; }
 
@@ -211,10 +349,59 @@ __device__ __forceinline__ V3 matReflectance(const DMat& m, const Frame& fr, con
     return mi.base;
 }
 
-// IMaterial::sample
+// sample below the Kajiya-Kay return (MarschnerHair.h:284-375): a lobe by the luminance of A_p,
+// theta_i from a Gaussian around the tilted mirror angle, phi from the trimmed logistic around the
+// lobe's specular azimuth; f and pdf are the lobe sums at the resulting direction (f * |cos|).
+__device__ __noinline__ BSample hairMarschnerSample(const DMat& m, const Frame& fr, V3 wo, V2 u, float uc) {
+    BSample bs{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    float sO, cO;
+    mh::longitudinal(wo, fr.t, sO, cO);
+    float h = mh::impact(wo, fr.t, fr.n, sO);
+    V3 sa = hairSigmaA(m, fr.color);
+    V3 ap[3];
+    mh::Ap(cO, m.hairEta, h, sa, ap);
+    float w0 = mh::lum(ap[0]), w1 = mh::lum(ap[1]), w2 = mh::lum(ap[2]);
+    float total = w0 + w1 + w2;
+    if (total < 1e-8f) return bs;
+    float inv = 1.f / total;
+    float cdf0 = w0 * inv, cdf1 = cdf0 + w1 * inv, cdf2 = 1.f;
+    int lobe;
+    float ur;
+    if (uc < cdf0) { lobe = 0; ur = uc / cdf0; }
+    else if (uc < cdf1) { lobe = 1; ur = (uc - cdf0) / (cdf1 - cdf0); }
+    else { lobe = 2; ur = (uc - cdf1) / (cdf2 - cdf1); }
+    float thetaO = asinf(mh::clampH(sO));
+    float z = sqrtf(-2.f * logf(fmaxf(u.x, 1e-6f))) * cosf(k2Pi * u.y);      // Box-Muller (:181-184)
+    float thetaI = thetaO + m.hairAlpha[lobe] + z * sqrtf(m.hairV[lobe]);
+    thetaI = fminf(fmaxf(thetaI, -kPi * 0.5f), kPi * 0.5f);
+    float sI, cI;
+    sincosf(thetaI, &sI, &cI);
+    float gO, gT;
+    mh::gammas(cO, m.hairEta, h, gO, gT);
+    float phiS = mh::sampleTrimmedLogistic(ur, m.hairS) + mh::Phi(lobe, gO, gT);
+    V3 woPerp = safeNormalize(wo - fr.t * sO);
+    V3 ctPerp = cross(fr.t, woPerp);
+    float sp, cp;
+    sincosf(phiS, &sp, &cp);
+    V3 wi = safeNormalize(fr.t * sI + woPerp * (cI * cp) + ctPerp * (cI * sp));
+    float sIa, cIa;
+    mh::longitudinal(wi, fr.t, sIa, cIa);
+    float phiA = mh::azimuth(wo, wi, fr.t, sO, sIa);
+    float pd;
+    V3 f = mh::lobes(m, sO, cO, sIa, cIa, phiA, h, sa, pd);
+    if (pd < 1e-8f) return bs;
+    bs.wi = wi;
+    bs.f = f * fabsf(dot(wi, fr.n));
+    bs.pdf = pd;
+    return bs;
+}
+
+// IMaterial::sample.  HAIRM: hair evaluates the Marschner model instead of Kajiya-Kay.
+template <bool HAIRM = false>
 __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V2 u,
                              float uc) {
     BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    if (HAIRM && m.type == AHIP_MAT_HAIR) return hairMarschnerSample(m, fr, wo, u, uc);
     if (m.type == AHIP_MAT_HAIR) {                                       // MarschnerHair.h:267-282
         V3 kt, kb;
         onb(fr.n, kt, kb);
@@ -289,9 +476,10 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, c
     return res;
 }
 
-// IMaterial::evaluate: f without cosine, forward pdf.
+// IMaterial::evaluate: f without cosine, forward pdf.  HAIRM as for matSample.
+template <bool HAIRM = false>
 __device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V3 wi, float& pdf) {
-    if (m.type == AHIP_MAT_HAIR) return hairKK(m, fr, wo, wi, pdf);
+    if (m.type == AHIP_MAT_HAIR) return HAIRM ? hairMarschner(m, fr, wo, wi, pdf) : hairKK(m, fr, wo, wi, pdf);
     V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
     pdf = 0.f;
     if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};

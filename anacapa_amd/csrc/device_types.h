@@ -96,6 +96,10 @@ struct DMat {
     float coat, coatA2, f0d, coatF0;
     uint32_t hasLe;
     float hairSigma[3];       // MarschnerHairMaterial default sigma_a
+    float hairEta;            // Marschner R/TT/TRT: index of the cortex,
+    float hairV[3];           //   longitudinal variance of R, TT, TRT,
+    float hairS;              //   azimuthal logistic scale,
+    float hairAlpha[3];       //   cuticle tilts (radians) (MarschnerHair.h:200-216)
 };
 
 struct DLight {
@@ -167,6 +171,7 @@ struct DScene {
     const float* strandInfo;  // [4 * strands]: colour rgb, meshID (bits)
     uint32_t hasCurves;
     const float* triAlbedo;   // [3 * tris] per-triangle base colour in commit order, or null
+    uint32_t hasHair;         // any hair material
 };
 
 }  // namespace ahip

@@ -133,6 +133,66 @@ int makePinholeCamera(const float from[3], const float at[3], const float up[3],
     return AHIP_OK;
 }
 
+int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& D, std::string& err) {
+    D = DMat{};
+    D.type = m.type;
+    D.roughTex = -1;
+    std::memcpy(D.albedo, m.albedo, 12);
+    if (m.type == AHIP_MAT_LAMBERT || m.type == AHIP_MAT_EMISSIVE) {
+        if (m.type == AHIP_MAT_EMISSIVE) std::memcpy(D.Le, m.emission, 12);
+    } else if (m.type == AHIP_MAT_HAIR) {
+        std::memcpy(D.hairSigma, m.hair_sigma_a, 12);   // MarschnerHair.h:193-199
+        // the constructor's Marschner constants (MarschnerHair.h:200-216): longitudinal variances
+        // of R, TT, TRT, the azimuthal logistic scale and the cuticle tilts
+        D.hairEta = m.hair_eta;
+        float bm = std::clamp(m.hair_beta_m, 1e-3f, 1.f);
+        float v0 = 0.726f * bm + 0.812f * bm * bm + 3.7f * std::pow(bm, 20.f);
+        D.hairV[0] = v0 * v0;
+        D.hairV[1] = D.hairV[0] * 0.25f;
+        D.hairV[2] = D.hairV[0] * 4.0f;
+        float bn = std::clamp(m.hair_beta_n, 1e-3f, 1.f);
+        D.hairS = 0.626657069f * (0.265f * bn + 1.194f * bn * bn + 5.372f * std::pow(bn, 22.f));
+        const float kPiF = 3.14159265358979323846f;
+        D.hairAlpha[0] = -m.hair_alpha * kPiF / 180.f;
+        D.hairAlpha[1] = m.hair_alpha * kPiF / 180.f * 0.5f;
+        D.hairAlpha[2] = -m.hair_alpha * kPiF / 180.f * 1.5f;
+    } else if (m.type == AHIP_MAT_STANDARD) {
+        if (m.transmission > 0.001f) {
+            err = "transmissive standard_surface is not supported by this backend";
+            return AHIP_E_UNSUPPORTED;
+        }
+        D.base = m.base;
+        D.metal = m.metalness;
+        D.spec = m.specular;
+        D.rough = m.roughness;
+        std::memcpy(D.specColor, m.specular_color, 12);
+        D.coat = m.coat;
+        float ca = std::max(1e-4f, m.coat_roughness * m.coat_roughness);
+        D.coatA2 = ca * ca;
+        float r = (1.f - m.specular_ior) / (1.f + m.specular_ior);
+        D.f0d = r * r;
+        float rc = (1.f - 1.5f) / (1.f + 1.5f);
+        D.coatF0 = rc * rc;
+        float rr = m.roughness, mm = m.metalness, ss = m.specular;
+        D.isDelta = (rr < 0.001f && (mm > 0.999f || ss > 0.f)) ? 1u : 0u;
+        D.specularFlag = rr < 0.001f ? 1u : 0u;
+        if (m.emission_weight > 0.f)
+            for (int k = 0; k < 3; ++k) D.Le[k] = m.emission[k] * m.emission_weight;
+        if (m.roughness_texture >= 0) {
+            if ((uint32_t)m.roughness_texture >= numTextures) {
+                err = "roughness_texture out of range";
+                return AHIP_E_INVALID_ARG;
+            }
+            D.roughTex = m.roughness_texture;
+        }
+    } else {
+        err = "unknown material type";
+        return AHIP_E_INVALID_ARG;
+    }
+    D.hasLe = (D.Le[0] != 0.f || D.Le[1] != 0.f || D.Le[2] != 0.f) ? 1u : 0u;
+    return AHIP_OK;
+}
+
 int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
     out = HostScene{};
     if (d->num_meshes && (!d->mesh_vertex_offset || !d->mesh_index_offset || !d->mesh_material || !d->positions ||
@@ -155,53 +215,14 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
         out.texW.push_back(w);
         out.texH.push_back(h);
     }
-    // materials (Lambertian.h:29-99, StandardSurface.h:152-187)
+    // materials (Lambertian.h:29-99, StandardSurface.h:152-187, MarschnerHair.h:190-216)
     std::vector<bool> emits(d->num_materials, false);
     for (uint32_t i = 0; i < d->num_materials; ++i) {
-        const ahip_material& m = d->materials[i];
         DMat D{};
-        D.type = m.type;
-        D.roughTex = -1;
-        std::memcpy(D.albedo, m.albedo, 12);
-        if (m.type == AHIP_MAT_LAMBERT || m.type == AHIP_MAT_EMISSIVE) {
-            if (m.type == AHIP_MAT_EMISSIVE) std::memcpy(D.Le, m.emission, 12);
-        } else if (m.type == AHIP_MAT_HAIR) {
-            std::memcpy(D.hairSigma, m.hair_sigma_a, 12);   // MarschnerHair.h:193-199
-        } else if (m.type == AHIP_MAT_STANDARD) {
-            if (m.transmission > 0.001f) {
-                err = "transmissive standard_surface is not supported by this backend";
-                return AHIP_E_UNSUPPORTED;
-            }
-            D.base = m.base;
-            D.metal = m.metalness;
-            D.spec = m.specular;
-            D.rough = m.roughness;
-            std::memcpy(D.specColor, m.specular_color, 12);
-            D.coat = m.coat;
-            float ca = std::max(1e-4f, m.coat_roughness * m.coat_roughness);
-            D.coatA2 = ca * ca;
-            float r = (1.f - m.specular_ior) / (1.f + m.specular_ior);
-            D.f0d = r * r;
-            float rc = (1.f - 1.5f) / (1.f + 1.5f);
-            D.coatF0 = rc * rc;
-            float rr = m.roughness, mm = m.metalness, ss = m.specular;
-            D.isDelta = (rr < 0.001f && (mm > 0.999f || ss > 0.f)) ? 1u : 0u;
-            D.specularFlag = rr < 0.001f ? 1u : 0u;
-            if (m.emission_weight > 0.f)
-                for (int k = 0; k < 3; ++k) D.Le[k] = m.emission[k] * m.emission_weight;
-            if (m.roughness_texture >= 0) {
-                if ((uint32_t)m.roughness_texture >= d->num_textures) {
-                    err = "roughness_texture out of range";
-                    return AHIP_E_INVALID_ARG;
-                }
-                D.roughTex = m.roughness_texture;
-            }
-        } else {
-            err = "unknown material type";
-            return AHIP_E_INVALID_ARG;
-        }
-        D.hasLe = (D.Le[0] != 0.f || D.Le[1] != 0.f || D.Le[2] != 0.f) ? 1u : 0u;
+        int st = convertMaterial(d->materials[i], d->num_textures, D, err);
+        if (st) return st;
         emits[i] = D.hasLe != 0u;
+        if (D.type == AHIP_MAT_HAIR) out.hasHair = true;
         out.mats.push_back(D);
     }
     out.meshMat.assign(d->mesh_material, d->mesh_material + d->num_meshes);

@@ -34,7 +34,12 @@ struct HostScene {
     HostDome dome;
     bool hasDome = false;
     bool hasEmitters = false;
+    bool hasHair = false;
 };
+
+// One material's device record (Lambert / Emissive / opaque StandardSurface / hair); refuses
+// transmissive standard_surface.  Returns an AHIP_E_* status.
+int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& out, std::string& err);
 
 // Validates the description and builds materials, lights, textures and the dome's sampling
 // distributions.  Returns an AHIP_E_* status.

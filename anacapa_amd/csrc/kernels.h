@@ -111,6 +111,7 @@ void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
 uint32_t trace_block_size();
 void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+void launchMaterialProbe(const DMat& m, bool marschner, uint32_t n, const float* in, float* out, hipStream_t s);
 void launchFillF32(float* p, size_t n, float v, hipStream_t s);
 void launchHalton(uint64_t seed, const uint32_t* px, const uint32_t* py, const uint32_t* idx, const uint32_t* dim,
                   uint32_t n, float* out, hipStream_t s);

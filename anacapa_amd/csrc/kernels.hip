@@ -200,7 +200,9 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- shade
-template <bool CURVES>
+// HAIRM: hair materials evaluate the Marschner model (AHIP_FLAG_HAIR_MARSCHNER); the instances
+// without it carry none of that code.
+template <bool CURVES, bool HAIRM>
 __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
                                                uint32_t cur) {
     __shared__ uint32_t lds[3 * (4 + 1)];
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         LSample ls = lightSample(sc, L, p, u2);
                         if (ls.pdf > 0.f && !isBlack(ls.Li)) {
                             float bpdf;
-                            V3 bf = matEval(sc, m, fe, mIn, wo, ls.wi, bpdf);
+                            V3 bf = matEval<HAIRM>(sc, m, fe, mIn, wo, ls.wi, bpdf);
                             if (!isBlack(bf)) {
                                 // spawnRayTo(si.p, si.n, si.p + wi*dist) (Types.h:277-284)
                                 V3 target = p + ls.wi * ls.dist;
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample(sc, m, fe, mIn, wo, ub, uc);
+                        BSample bs = matSample<HAIRM>(sc, m, fe, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             float lp = lightPdf(sc, L, p, bs.wi);
                             if (lp > 0.f) {
@@ -350,7 +352,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample(sc, m, fr, mIn, wo, ub, uc);
+                        BSample bs = matSample<HAIRM>(sc, m, fr, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             specularNext = bs.delta;
                             beta = beta * (bs.f / bs.pdf);
@@ -900,6 +902,35 @@ __global__ __launch_bounds__(256) void k_resolve(const float4* acc, const float*
     }
 }
 
+// --------------------------------------------------------------------------- material probe
+// IMaterial::evaluate and sample of one material on independent rows (the layout of
+// ahip_material_probe): the frame is the integrator's for a hit seen along -wo, a curve row takes
+// its tangent as the fiber direction.  out: f rgb, pdf, 0 (pdfRev is BDPT's), sampled wi, f rgb,
+// pdf, valid, hair effectiveSigmaA rgb.
+template <bool HAIRM>
+__global__ __launch_bounds__(256) void k_material_probe(DMat m, uint32_t n, const float* in, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = in + 19u * (size_t)i;
+    V3 nn{q[0], q[1], q[2]}, t{q[3], q[4], q[5]}, col{q[6], q[7], q[8]};
+    V3 wo{q[9], q[10], q[11]}, wi{q[12], q[13], q[14]};
+    V2 u{q[15], q[16]};
+    const float uc = q[17];
+    Frame fr = q[18] != 0.f ? makeCurveFrame(nn, t, -wo, col) : makeFrame(nn, nn, -wo);
+    DScene sc{};
+    const MatIn mi = matInputs(sc, m, V2{0.f, 0.f}, ~0u);
+    float pdf;
+    V3 f = matEval<HAIRM>(sc, m, fr, mi, wo, wi, pdf);
+    BSample bs = matSample<HAIRM>(sc, m, fr, mi, wo, u, uc);
+    V3 sa = m.type == AHIP_MAT_HAIR ? hairSigmaA(m, fr.color) : V3{0.f, 0.f, 0.f};
+    float* o = out + 16u * (size_t)i;
+    const bool valid = bsValid(bs);
+    o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = pdf; o[4] = 0.f;
+    o[5] = bs.wi.x; o[6] = bs.wi.y; o[7] = bs.wi.z;
+    o[8] = bs.f.x; o[9] = bs.f.y; o[10] = bs.f.z; o[11] = bs.pdf; o[12] = valid ? 1.f : 0.f;
+    o[13] = sa.x; o[14] = sa.y; o[15] = sa.z;
+}
+
 // --------------------------------------------------------------------------- ray-batch entry points
 template <uint32_t BS, uint32_t K, bool ANY>
 __global__ __launch_bounds__(BS) void k_trace_batch(DScene sc, const float* rays, uint32_t n, ahip_hit* hits,
@@ -1084,8 +1115,14 @@ void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32
 }
 void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, uint32_t bounce, uint32_t cur,
                  uint32_t blocks, hipStream_t s) {
-    if (sc.hasCurves) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    const bool hairm = sc.hasHair && (rc.flags & AHIP_FLAG_HAIR_MARSCHNER) != 0u;
+    if (sc.hasCurves) {
+        if (hairm) hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+        else hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    } else {
+        if (hairm) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+        else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    }
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
     if (curveRounds(sc, q)) {
@@ -1119,6 +1156,11 @@ void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t 
 }
 void launchAccumulate(const WaveQueues& q, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, q);
+}
+void launchMaterialProbe(const DMat& m, bool marschner, uint32_t n, const float* in, float* out, hipStream_t s) {
+    const uint32_t blocks = (n + 255u) / 256u;
+    if (marschner) hipLaunchKernelGGL(k_material_probe<true>, dim3(blocks), dim3(256), 0, s, m, n, in, out);
+    else hipLaunchKernelGGL(k_material_probe<false>, dim3(blocks), dim3(256), 0, s, m, n, in, out);
 }
 void launchTraceBatch(const DScene& sc, const float* rays, uint32_t n, ahip_hit* hits, uint8_t* occ, bool any,
                       uint32_t* ovfRef, float* ovfT, uint32_t cap, uint32_t blocks, uint32_t* overflow,

@@ -239,6 +239,14 @@ class HipPathIntegrator:
                                               len(px), A.ptr(out)))
         return out
 
+    def material_probe(self, mat: A.Material, rows: np.ndarray, flags: int = 0) -> np.ndarray:
+        """IMaterial::evaluate / sample of one material on the device (ahip_material_probe)."""
+        rows = np.ascontiguousarray(rows, np.float32).reshape(-1, 19)
+        out = np.zeros((len(rows), 16), np.float32)
+        self._check(self.lib.ahip_material_probe(self._ctx, C.byref(mat), flags, len(rows), A.ptr(rows),
+                                                 A.ptr(out)))
+        return out
+
     # -------------------------------------------------------------- measurement / testing
     def stats(self) -> dict:
         s = A.Stats()

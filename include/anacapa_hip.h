@@ -301,6 +301,16 @@ int ahip_trace_rays_noalloc(ahip_context* ctx, const float* rays, uint32_t n, ah
 int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const uint32_t* py,
                      const uint32_t* idx, const uint32_t* dim, uint32_t n, float* out);
 
+/* --- material probe: IMaterial::evaluate and IMaterial::sample (include/anacapa/shading/IMaterial.h:
+ *     52-100) of one material on the device, for the per-lobe parity checks.  flags:
+ *     AHIP_FLAG_HAIR_MARSCHNER as for rendering.  in[n*19] = normal, tangent, strand colour, wo, wi,
+ *     u.x, u.y, uComponent, curve (0/1: a curve row takes the tangent as its fiber, ShadingContext.h:
+ *     39-44); out[n*16] = evaluate(wo, wi): f rgb, pdf, 0 | sample(wo, u, uComponent): wi, f rgb, pdf,
+ *     valid | hair effectiveSigmaA rgb (zero otherwise).  The frame is the integrator's for a hit
+ *     seen along -wo. --- */
+int ahip_material_probe(ahip_context* ctx, const ahip_material* mat, uint32_t flags, uint32_t n, const float* in,
+                        float* out);
+
 /* --- differentiation (no counterpart in the reference, which has no adjoint) ---
  * The differentiated estimator is the fixed-seed path-replay estimator with detached sampling:
  * every random decision (Halton dims, lobe choice, sampled direction, light choice, Russian
