@@ -86,7 +86,8 @@ def test_adaptive_scores_allocation_and_image(gpu, oracle):
             assert np.all(got[..., 3] == 16 + extra[j, i])
             g, r = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
             err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(r.mean(), 1e-6))
-            assert (err <= 1e-3).mean() >= 0.99, (j, i, (err > 1e-3).sum())
+            print(f"[radiance] adaptive tile {j},{i}: {(err > 1e-3).sum()} of {err.size} pixels beyond 1e-3", flush=True)
+            assert (err <= 1e-3).all(), (j, i, (err > 1e-3).sum())     # measured: 0 beyond 1e-3
             assert abs(g.mean() / r.mean() - 1) < 1e-3
 
 

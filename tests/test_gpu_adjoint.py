@@ -18,13 +18,14 @@ from anacapa_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def _radiance_close(got, ref, rel=1e-3, frac=0.99):
+def _radiance_close(got, ref, rel=1e-3, frac=0.99, what=""):
     g = got[..., :3].astype(np.float64)
     r = ref[..., :3].astype(np.float64)
     assert np.array_equal(got[..., 3], ref[..., 3])
     scale = max(np.abs(r).mean(), 1e-6)
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
     ok = err <= rel
+    print(f"[radiance] {what}: {(~ok).sum()} of {ok.size} pixels beyond {rel} (max {err.max():.3g})", flush=True)
     assert ok.mean() >= frac, f"{(~ok).sum()} of {ok.size} pixels differ by more than {rel} (max {err.max()})"
 
 
@@ -55,7 +56,10 @@ def test_replay_matches_oracle_at_eval_params(gpu, small):
     got = gpu.replay(0, sc.spp)
     ref = os_.replay_tile(0, 0, sc.width, sc.height, 0, sc.spp)
     os_.set_eval()
-    _radiance_close(got, ref)
+    # measured: 2 of 768 pixels beyond 1e-3 — paths that took another branch where the device's
+    # fused multiply-adds round a sampled direction differently (the per-sample classes of
+    # test_gpu_workload.py); every other pixel within 1e-3
+    _radiance_close(got, ref, frac=0.99, what="replay vs oracle replay")
 
 
 def test_adjoint_matches_oracle_adjoint(gpu, small):
@@ -194,7 +198,7 @@ def test_adjoint_matches_central_differences_inverse512(gpu, inverse512):
     gpu.prepare(sc)
     w = np.random.default_rng(9).normal(size=target.shape)
     rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 5, weights=w)
-    assert noisy <= 256 // 10, noisy
+    assert noisy == 0, noisy   # every parameter within 1e-2 of its central difference
     print(f"[adjoint] inverse512, linear functional: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
           f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
 
@@ -204,7 +208,7 @@ def test_adjoint_l2_loss_central_differences_inverse512(gpu, inverse512):
     sc, target, alb0, tex0 = inverse512
     gpu.prepare(sc)
     rels, noisy = _fd_check(gpu, sc, target, alb0, tex0, 128, 7)
-    assert noisy <= 256 // 10, noisy
+    assert noisy == 0, noisy   # every parameter within 1e-2 of its central difference
     print(f"[adjoint] inverse512, image L2 loss: 256 parameters, within 1e-2: {1 - noisy / 256:.3f} "
           f"(the rest within the difference's float32 bound), median {np.median(rels):.2e}", flush=True)
 

@@ -49,7 +49,8 @@ def test_hair_radiance_vs_oracle(gpu, oracle):
     assert np.array_equal(got[..., 3], ref[..., 3])
     g, r = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(r.mean(), 1e-6))
-    assert (err <= 1e-3).mean() >= 0.99, (err > 1e-3).sum()
+    print(f"[radiance] hair: {(err > 1e-3).sum()} of {err.size} pixels beyond 1e-3 (max {err.max():.3g})", flush=True)
+    assert (err <= 1e-3).all(), (err > 1e-3).sum()      # measured: 0 pixels beyond 1e-3
     assert abs(g.mean() / r.mean() - 1) < 1e-3
 
 
@@ -101,7 +102,8 @@ def test_converted_strands_with_per_cv_widths_vs_oracle(gpu, oracle):
     assert np.array_equal(got[..., 3], ref[..., 3])
     gg, rr = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     err = np.abs(gg - rr).max(-1) / np.maximum(np.abs(rr).max(-1), 1e-3 * max(rr.mean(), 1e-6))
-    assert (err <= 1e-3).mean() >= 0.99, (err > 1e-3).sum()
+    print(f"[radiance] hair: {(err > 1e-3).sum()} of {err.size} pixels beyond 1e-3 (max {err.max():.3g})", flush=True)
+    assert (err <= 1e-3).all(), (err > 1e-3).sum()      # measured: 0 pixels beyond 1e-3
     assert abs(gg.mean() / rr.mean() - 1) < 1e-3
 
 

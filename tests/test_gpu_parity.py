@@ -3,7 +3,7 @@
 Bars: bit-exact for the sampler, BVH topology and closest-hit records (ids, t, u, v) against the
 oracle, whose traversal is a fixed IEEE program; hit ids equal to the reference's except on
 recorded ties (|dt|/t < 1e-6); per-pixel radiance at identical sample indices within 1e-3
-relative (FP32) on at least 99% of pixels, with the image means within 1e-3.
+relative (FP32) on every pixel, with the image means within 1e-3.
 """
 import numpy as np
 import pytest
@@ -23,13 +23,18 @@ def _hits_equal(a, b):
         assert np.array_equal(a[k][m].view(np.uint32), b[k][m].view(np.uint32)), k
 
 
-def _radiance_close(got, ref, rel=1e-3, frac=0.99):
+def _radiance_close(got, ref, rel=1e-3, frac=1.0, what=""):
+    """Per-pixel radiance within `rel` on at least `frac` of the pixels (every pixel by default:
+    each parity scene here measures 0 pixels beyond 1e-3), image means within `rel`; prints how
+    many pixels are beyond `rel`."""
     g = got[..., :3].astype(np.float64)
     r = ref[..., :3].astype(np.float64)
     assert np.array_equal(got[..., 3], ref[..., 3])          # sample counts
     scale = max(np.abs(r).mean(), 1e-6)
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
     ok = err <= rel
+    print(f"[radiance] {what}: {(~ok).sum()} of {ok.size} pixels beyond {rel} (max {err.max():.3g}), "
+          f"means {g.mean():.6g} / {r.mean():.6g}", flush=True)
     assert ok.mean() >= frac, f"{(~ok).sum()} of {ok.size} pixels differ by more than {rel} (max {err.max()})"
     assert abs(g.mean() / r.mean() - 1) < rel
 
@@ -104,11 +109,11 @@ def test_cornell_radiance_vs_oracle_and_reference(gpu, oracle, golden):
     gpu.prepare(sc)
     os_ = oracle.scene(sc)
     got = gpu.render_tile(200, 220, 48, 40, 0, 8)
-    _radiance_close(got, os_.render_tile(200, 220, 48, 40, 0, 8))
-    _radiance_close(got, golden["cornell_tile_200_220_48x40_s0_n8"])
+    _radiance_close(got, os_.render_tile(200, 220, 48, 40, 0, 8), what="cornell vs oracle")
+    _radiance_close(got, golden["cornell_tile_200_220_48x40_s0_n8"], what="cornell vs reference")
     gpu.set_params(max_depth=3, min_depth=2, seed=0)
     got = gpu.render_tile(0, 0, 32, 32, 3, 5)
-    _radiance_close(got, golden["cornell_tile_0_0_32x32_s3_n5_d3"])
+    _radiance_close(got, golden["cornell_tile_0_0_32x32_s3_n5_d3"], what="cornell depth 3 vs reference")
 
 
 def test_small_ggx_radiance_vs_oracle(gpu, oracle):
@@ -117,7 +122,7 @@ def test_small_ggx_radiance_vs_oracle(gpu, oracle):
     os_ = oracle.scene(sc)
     got = gpu.render_tile(0, 0, sc.width, sc.height, 0, 4)
     ref = os_.render_tile(0, 0, sc.width, sc.height, 0, 4)
-    _radiance_close(got, ref)
+    _radiance_close(got, ref, what="small_ggx vs oracle")
 
 
 def test_aovs_vs_oracle_and_reference(gpu, oracle, golden):
@@ -129,7 +134,7 @@ def test_aovs_vs_oracle_and_reference(gpu, oracle, golden):
     os_ = oracle.scene(sc)
     key = "cornell_tile_200_220_48x40_s0_n8"
     acc, alb, nrm = gpu.render_tile_aovs(200, 220, 48, 40, 0, 8)
-    _radiance_close(acc, golden[key])
+    _radiance_close(acc, golden[key], what=key)
     assert np.array_equal(alb, golden[key + "_albedo"])
     np.testing.assert_allclose(nrm, golden[key + "_normal"], rtol=0, atol=1e-6)
     _, oalb, onrm = os_.render_tile_aovs(200, 220, 48, 40, 0, 8)
