@@ -83,7 +83,7 @@ class Camera(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("min_depth", C.c_uint32), ("seed", C.c_uint64),
-                ("flags", C.c_uint32), ("pad", C.c_uint32)]
+                ("flags", C.c_uint32), ("curve_depth", C.c_uint32)]
 
 
 class View(C.Structure):

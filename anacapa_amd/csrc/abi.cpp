@@ -589,6 +589,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.strandInfo = ctx->dStrandInfo.as<float>();
     S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
     S.hasHair = hs.hasHair ? 1u : 0u;
+    S.curveDepth = ctx->params.curve_depth ? ctx->params.curve_depth : 6u;
     if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes4.data(), ctx->bvh.nodes4.size()))) return st;
     for (DTri& t : ctx->bvh.tris)   // the material of each triangle, so shading skips the mesh table
         t.matIndex = t.meshID < hs.meshMat.size() ? hs.meshMat[t.meshID] : -1;
@@ -695,7 +696,9 @@ int ahip_set_camera(ahip_context* ctx, const ahip_camera* cam) {
 int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
     ENTER();
     if (!p || p->max_depth > 1000u) return fail(ctx, AHIP_E_INVALID_ARG, "bad render params");
+    if (p->curve_depth > 6u) return fail(ctx, AHIP_E_INVALID_ARG, "curve_depth must be 0..6");
     ctx->params = *p;
+    ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : 6u;
     return AHIP_OK;
 }
 

@@ -385,7 +385,10 @@ __device__ __forceinline__ V3 midNC(V3 a, V3 b) {
 }
 
 // Returns true when a strictly nearer hit on this segment was found (best, out updated).
-__device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang) {
+// depth: de Casteljau levels before the chord test (the reference's MAX_DEPTH = 6, CurveBrute.cpp:194;
+// 1..6, which the 7-entry stack of pending halves holds).
+__device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang,
+                           int depth) {
 #pragma clang fp contract(off)
     // the frame being tested stays in registers; only pending right halves go to the private
     // stack (at most one per level), in the order the recursion would visit them
@@ -400,7 +403,7 @@ __device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& be
     f.hw3 = S.hw3;
     f.tOff = S.tOff;
     f.tScale = S.tScale;
-    f.depth = 6;
+    f.depth = depth;
     bool hit = false;
     while (true) {
         {   // one frame; every exit to `next` pops the next pending half
@@ -527,7 +530,7 @@ __device__ bool traverseCurves(const DScene& sc, const TRay& r, const TStack& S,
                 if (COUNT) ++cnt.tris;
                 float tSeg;
                 V3 tg;
-                if (segmentHit(Sg, r, fx, fy, best, tSeg, tg)) {
+                if (segmentHit(Sg, r, fx, fy, best, tSeg, tg, (int)sc.curveDepth)) {
                     hit = true;
                     ch.tSeg = tSeg;
                     ch.tang = tg;

@@ -172,6 +172,7 @@ struct DScene {
     uint32_t hasCurves;
     const float* triAlbedo;   // [3 * tris] per-triangle base colour in commit order, or null
     uint32_t hasHair;         // any hair material
+    uint32_t curveDepth;      // de Casteljau depth of the ribbon test (the reference's MAX_DEPTH is 6)
 };
 
 }  // namespace ahip

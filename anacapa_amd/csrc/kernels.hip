@@ -562,7 +562,7 @@ __global__ __launch_bounds__(BS) void k_curve_test(DScene sc, WaveQueues q, uint
                     const uint32_t rec = q.cList[(size_t)i * kCurveList + c + gl];
                     float best = bnd, ts;
                     V3 tg;
-                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg, (int)sc.curveDepth)) {
                         key = ((uint64_t)__float_as_uint(best) << 32) | rec;   // t > 0: bits order as floats
                         tSeg = ts;
                         tang = tg;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, ui
                 if (rec != ~0u) {
                     float best = bound, ts;
                     V3 tg;
-                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg)) {
+                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg, (int)sc.curveDepth)) {
                         key = ((uint64_t)__float_as_uint(best) << 32) | rec;
                         tSeg = ts;
                         tang = tg;

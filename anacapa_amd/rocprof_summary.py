@@ -72,7 +72,9 @@ def main(argv=None):
     fcalls = sum(c for n, c, *_ in st if in_fam(n))
     ftot = sum(t for n, _, t, *_ in st if in_fam(n))
     res = {"family": fam, "family_dispatches": fcalls,
-           "family_avg_dispatch_us": round(ftot / max(1, fcalls) / 1e3, 2)}
+           "family_avg_dispatch_us": round(ftot / max(1, fcalls) / 1e3, 2),
+           "kernels": {n: {"calls": c, "total_ms": round(tot / 1e6, 3), "avg_us": round(avg / 1e3, 2)}
+                       for n, c, tot, avg, _ in st}}
     print(f"\ntraversal family ({', '.join(fam)}): {fcalls} dispatches, mean {res['family_avg_dispatch_us']} us")
 
     if a.fetch and a.write and a.probe_mb:

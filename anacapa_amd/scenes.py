@@ -381,10 +381,15 @@ def procedural_hdri(width=2048, height=1024, sun_peak=5e4, sun_sigma_deg=0.5,
 
 
 def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, freq=7,
-                hdri_size=(2048, 1024)) -> Scene:
+                hdri_size=(2048, 1024), ground_tilt_deg=0.0) -> Scene:
     """The GGX-spheres scene: grid x grid displaced geodesic spheres (20*freq^2 triangles each; freq=7 gives
     1,003,520 sphere triangles for a 32x32 grid) with per-sphere standard_surface on a 2-triangle
-    ground, lit by a procedural HDRI dome with importance sampling."""
+    ground, lit by a procedural HDRI dome with importance sampling.
+
+    The flat ground at y = 0 sits alone in a BVH leaf whose box has no height, and the reference's
+    slab test rejects such a box (t1 <= t0, BVHBackend.cpp:249-262): no ray hits the ground.
+    ground_tilt_deg > 0 turns the ground about the grid's centre line (x axis) by that angle, so its
+    box has height and the ground is seen, as the workload describes it."""
     rng = np.random.default_rng(seed)
     unit, tris = geodesic_sphere(freq)
     mb = _MeshBuilder()
@@ -410,8 +415,10 @@ def ggx_spheres(width=1920, height=1080, spp=16, max_depth=8, seed=0, grid=32, f
     g = 200.0
     ext = spacing * (grid - 1)
     c0 = ext * 0.5
-    mb.add([(c0 - g, 0, c0 - g), (c0 - g, 0, c0 + g), (c0 + g, 0, c0 + g), (c0 + g, 0, c0 - g)],
-           [0, 1, 2, 0, 2, 3], 0, normals=[(0, 1, 0)] * 4, uvs=[(0, 0), (0, 1), (1, 1), (1, 0)])
+    ty = math.tan(math.radians(ground_tilt_deg)) * g       # height of the ground's z edges
+    gn = (0.0, math.cos(math.radians(ground_tilt_deg)), -math.sin(math.radians(ground_tilt_deg)))
+    mb.add([(c0 - g, -ty, c0 - g), (c0 - g, ty, c0 + g), (c0 + g, ty, c0 + g), (c0 + g, -ty, c0 - g)],
+           [0, 1, 2, 0, 2, 3], 0, normals=[gn] * 4, uvs=[(0, 0), (0, 1), (1, 1), (1, 0)])
     # the ground is the last mesh; material 0 is the ground's
     env = procedural_hdri(*hdri_size)
     lights = [dome_light(1.0)]

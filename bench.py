@@ -127,26 +127,7 @@ def main():
 
     roofline = None
     if not args.no_traffic:
-        # Algorithmic bytes of the traversal kernels for one frame: 112 B per two-level node
-        # visited (four slot boxes + four references), 48 B per triangle tested (v0, e1, e2),
-        # 48 B of queue I/O per ray (32-B ray read + 16-B result).
-        ct = be.count_traffic(0, S)
-        frame_bytes = 112.0 * ct["nodes_visited"] + 48.0 * ct["prims_tested"] + 48.0 * (ct["rays_closest"] + ct["rays_shadow"])
-        per_launch = frame_bytes * args.steps / launches
-        avg_launch_s = trace_ms / 1e3 / launches
-        achieved = per_launch / avg_launch_s / 1e9
-        # measured HBM bytes per traversal dispatch: rocprofv3 FETCH_SIZE + WRITE_SIZE of this workload,
-        # calibrated by a copy kernel of known size (summary and commands under profiles/)
-        traffic, prof = None, os.path.join(ROOT, "profiles", "traversal_ggx1080p16_traffic.json")
-        if (W, H, S, args.depth) == (1920, 1080, 16, 8) and os.path.exists(prof):
-            with open(prof) as f:
-                traffic = json.load(f).get("family_traffic_bytes_per_dispatch")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "traversal (extend+shadow+mis launches)",
-                    "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-                    "nodes_per_frame": int(ct["nodes_visited"]), "tris_per_frame": int(ct["prims_tested"]),
-                    "rays_per_frame": int(ct["rays_closest"] + ct["rays_shadow"])}
+        roofline = traversal_roofline(be, args, S, trace_ms, launches, W, H)
 
     batched = None
     if args.batch_views:
@@ -189,6 +170,74 @@ def main():
         "other_configs": others,
     }
     print(json.dumps(out), flush=True)
+
+
+def _profile(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        return json.load(f), "profiles/" + name
+
+
+def traversal_roofline(be, args, S, trace_ms, launches, W, H):
+    """Roofline of the dominant kernel family, BVH traversal (the extend, shadow and MIS launches).
+
+    achieved: algorithmic bytes per launch / mean launch time (HIP events on the library's stream
+    over the timed frames).  Per unit: 64 B per binary BVH node visited, 64 B per triangle tested,
+    176 B of queue I/O per ray (a 32-B ray read, a 16-B hit write, 64-B path state read and
+    written).  The node and triangle counts come from ahip_count_traffic, an untimed re-run of the
+    same deterministic frame with counter-enabled traversal kernels; the kernels walk two-level
+    records that each stand for a binary node's two children, so binary visits = 2 x record
+    visits.  The node and triangle set (~85 MB) stays in the 256 MB Infinity Cache, so this rate
+    can pass the HBM peak: what HBM itself carries is `traffic` (rocprofv3 FETCH_SIZE + WRITE_SIZE
+    of this workload, copy-kernel calibrated), and the binding resource is VALU issue (`valu`:
+    busy x lane utilization from the SQ counters, time-weighted over the three kernels); both come
+    from profiles of this same workload committed under profiles/, named in `source`."""
+    ct = be.count_traffic(0, S)
+    nodes, tris = ct["nodes_visited"], ct["prims_tested"]
+    rays = ct["rays_closest"] + ct["rays_shadow"]
+    frame_bytes = 64.0 * 2 * nodes + 64.0 * tris + 176.0 * rays
+    record_bytes = 112.0 * nodes + 48.0 * tris + 48.0 * rays   # what the kernels' records take
+    per_launch = frame_bytes * args.steps / launches
+    avg_launch_s = trace_ms / 1e3 / launches
+    achieved = per_launch / avg_launch_s / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "kernel": "traversal (extend+shadow+mis launches)",
+           "per_unit": "64 B per binary node visited, 64 B per triangle tested, 176 B queue I/O per ray",
+           "bytes_per_launch": round(per_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+           "record_bytes_per_launch": round(record_bytes * args.steps / launches),
+           "record_gbs": round(record_bytes * args.steps / launches / avg_launch_s / 1e9, 1),
+           "node_records_per_frame": int(nodes), "binary_nodes_per_frame": int(2 * nodes),
+           "tris_per_frame": int(tris), "rays_per_frame": int(rays)}
+    full = (W, H, S, args.depth) == (1920, 1080, 16, 8)
+    prof, src = _profile("traversal_ggx1080p16_traffic.json")
+    if full and prof and prof.get("family_traffic_bytes_per_dispatch"):
+        traffic = float(prof["family_traffic_bytes_per_dispatch"])
+        out["traffic"] = round(traffic)
+        out["hbm_measured"] = {"gbs": round(traffic / avg_launch_s / 1e9, 1),
+                               "frac": round(traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS, 4),
+                               "traffic_over_algorithmic": round(traffic / per_launch, 3), "source": src}
+    prof, src = _profile("traversal_ggx1080p16_valu.json")
+    if full and prof and prof.get("sq") and prof.get("kernels"):
+        per, wsum, acc = {}, 0.0, 0.0
+        for name, m in prof["sq"].items():
+            base = name.split("::")[-1]
+            if not any(base.startswith(k) for k in ("k_extend", "k_shadow", "k_mis")):
+                continue
+            t = prof["kernels"].get(name, {}).get("total_ms", 0.0)
+            f = min(m["valu_busy"] / 100.0, 1.0) * m["valu_util"] / 100.0
+            per[base.split("<")[0]] = {"busy": round(m["valu_busy"] / 100.0, 3),
+                                       "lane_utilization": round(m["valu_util"] / 100.0, 3), "frac": round(f, 3)}
+            wsum += t
+            acc += t * f
+        if wsum > 0:
+            out["valu"] = {"frac": round(acc / wsum, 3), "per_kernel": per,
+                           "definition": "VALU busy (issue share of SIMD cycles, capped at 1) x lanes active per "
+                                         "VALU instruction, weighted by kernel time",
+                           "source": src}
+    return out
 
 
 def batch_views(be, sc, n, seed):
@@ -291,14 +340,27 @@ def adjoint_form(be, log):
 
 
 def other_configs(be, log):
-    """The Cornell and hair workloads on the same backend: the Cornell box (36 triangles, 512x512, 64 spp, depth
-    8) and the hair sphere (200k Bezier strands = 800k segments, 1024x1024, 32 spp, depth 16), one
-    warm-up and two timed frames each (HIP events around the frame)."""
+    """The other workloads on the same backend, one warm-up and two timed frames each (HIP events
+    around the frame):
+      cornell: the Cornell box (36 triangles, 512x512, 64 spp, depth 8);
+      ggx_tilted_ground: the 1080p x 16 spp GGX workload with its ground turned by 0.05 degrees,
+        so the ground is hit (flat, it sits alone in a box of no height that the reference's slab
+        test never enters, and the timed workload is the spheres over the sky);
+      hair: the hair sphere (200k Bezier strands = 800k segments, 1024x1024, 32 spp, depth 16) as the
+        reference renders it: Kajiya-Kay hair, ribbon depth 6 (the parity mode);
+      hair_config3: the same frame as the hair workload specifies it: Marschner R/TT/TRT hair
+        (beta_m = beta_n = 0.3, alpha = 2 degrees, IOR 1.55, eumelanin sigma_a), ribbon depth 5."""
+    from anacapa_amd import _abi as A
     from anacapa_amd import scenes
     out = {}
     be.set_launch(max_paths=DEFAULT_MAX_PATHS, tile=8)   # the library default, whatever ran before
-    for name, sc in (("cornell", scenes.cornell()), ("hair", scenes.hair_sphere())):
-        be.prepare(sc)
+    hair = scenes.hair_sphere()
+    runs = (("cornell", scenes.cornell(), 0, 0, ""),
+            ("ggx_tilted_ground", scenes.ggx_spheres(ground_tilt_deg=0.05), 0, 0, ", ground tilted 0.05 deg"),
+            ("hair", hair, 0, 0, ", Kajiya-Kay, ribbon depth 6"),
+            ("hair_config3", hair, A.AHIP_FLAG_HAIR_MARSCHNER, 5, ", Marschner R/TT/TRT, ribbon depth 5"))
+    for name, sc, flags, cdepth, note in runs:
+        be.prepare(sc, flags=flags, curve_depth=cdepth)
         be.clear_accum()
         be.render_frame(0, sc.spp)
         ms = []
@@ -312,24 +374,48 @@ def other_configs(be, log):
         samples = sc.width * sc.height * sc.spp
         rays = st["rays_closest"] + st["rays_shadow"]
         out[name] = {"workload": f"{sc.name}: {sc.num_triangles} tris, {sc.num_strands} strands, "
-                                 f"{sc.width}x{sc.height}, {sc.spp} spp, depth {sc.max_depth}",
+                                 f"{sc.width}x{sc.height}, {sc.spp} spp, depth {sc.max_depth}{note}",
                      "ms_per_frame": round(t, 3), "msamples_per_s": round(samples / t / 1e3, 2),
                      "mrays_per_s": round(rays / t / 1e3, 2)}
         log(f"[bench] {name}: {t:.1f} ms/frame, {samples / t / 1e3:.1f} Msamples/s")
     return out
 
 
+def host_info():
+    """CPU model and hardware thread count of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "hardware_concurrency": os.cpu_count() or 1}
+
+
 def cpu_baseline(sc, args):
-    """The oracle (this repository's port of the reference CPU PathIntegrator, tile-parallel over
-    host threads like RenderSession's ThreadPool) on rows [0, k) of the same frame, full spp."""
+    """The CPU baseline on the same box's host cores.
+
+    value: the oracle — this repository's port of the reference CPU PathIntegrator, tile-parallel
+    over host threads like RenderSession's ThreadPool, built at the reference's Release flags
+    (oracle/liboracle_release.so: -O3 -mavx2 -mfma) — on evenly spaced 8-row bands of the same
+    1080p frame at full spp.  The reference's own code cannot render this workload here: its
+    StandardSurface and DomeLight need the OpenImageIO-only Texture.cpp and DomeLight.cpp.
+    reference_check: the reference's own BVHBackend + PathIntegrator + ThreadPool (oracle/_ref,
+    RenderSession's non-adaptive tile loop) and the port on the Cornell box, which both can render,
+    on the same threads: how the port's speed compares with the reference code's."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     try:
         from oracle_bindings import Oracle
-        orc = Oracle()
+        orc = Oracle(release=True)
     except Exception as e:  # noqa: BLE001
         log(f"[bench] cpu baseline unavailable: {e}")
         return None
-    threads = min(os.cpu_count() or 1, 16)
+    host = host_info()
+    # the GPU box grants a 16-thread CPU share per GPU whatever the host reports
+    threads = min(host["hardware_concurrency"], 16)
     os_ = orc.scene(sc)
     # 8-row bands spread evenly over the frame (sky at the top is cheaper than the spheres);
     # calibrate with 4 bands, then size the sample to ~cpu_seconds.
@@ -347,10 +433,36 @@ def cpu_baseline(sc, args):
     nb = int(max(4, min(nb_total, 4 * args.cpu_seconds / max(sec4, 1e-3))))
     sec, nb = run(nb)
     samples = args.width * 8 * nb * args.spp
-    log(f"[bench] cpu baseline: {nb} bands of 8 rows x {args.width} x {args.spp} spp in {sec:.1f}s on {threads} threads")
-    return {"value": round(samples / sec / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{nb} evenly spaced 8-row bands of the same {args.width}x{args.height} frame at "
-                      f"{args.spp} spp ({samples} samples, {sec:.1f}s)"}
+    log(f"[bench] cpu baseline: {nb} bands of 8 rows x {args.width} x {args.spp} spp in {sec:.1f}s on {threads} "
+        f"threads ({host['cpu_model']}, {host['hardware_concurrency']} hardware threads)")
+    out = {"value": round(samples / sec / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+           "sample": f"{nb} evenly spaced 8-row bands of the same {args.width}x{args.height} frame at "
+                     f"{args.spp} spp ({samples} samples, {sec:.1f}s)",
+           "build": "oracle/liboracle_release.so (the reference's Release flags)", "host": host}
+    out["reference_check"] = reference_check(orc, threads)
+    return out
+
+
+def reference_check(orc, threads):
+    """The Cornell box (36 triangles, 512x512, depth 8) at 16 spp — a quarter of its 64-spp
+    workload, to keep the check to seconds — rendered by the reference's own code (oracle/_ref,
+    RenderSession.cpp:511-547: 64x64 tiles on its ThreadPool) and by the port, both timed around
+    the tile loop only, median of 3."""
+    from anacapa_amd import scenes
+    from oracle_bindings import Reference, reference_available
+    if not reference_available():
+        return None
+    sc = scenes.cornell()
+    rs = Reference().scene(sc)
+    os_ = orc.scene(sc)
+    spp = 16
+    ref_s = sorted(rs.render_frame(spp, threads=threads)[1] for _ in range(3))[1]
+    port_s = sorted(os_.render_frame(spp, threads=threads)[1] for _ in range(3))[1]
+    samples = sc.width * sc.height * spp
+    log(f"[bench] cpu reference check (cornell): reference {ref_s:.2f}s, port {port_s:.2f}s on {threads} threads")
+    return {"workload": "cornell 36 tris, 512x512, 16 spp, depth 8", "threads": threads,
+            "reference_msps": round(samples / ref_s / 1e6, 3), "port_msps": round(samples / port_s / 1e6, 3),
+            "port_over_reference": round(ref_s / port_s, 3)}
 
 
 if __name__ == "__main__":

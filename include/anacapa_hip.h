@@ -156,7 +156,9 @@ typedef struct ahip_render_params {
     uint32_t min_depth;   /* Russian roulette from bounce >= min_depth (PathIntegrator.cpp:156) */
     uint64_t seed;        /* HaltonSampler seed (HaltonSampler.h:24) */
     uint32_t flags;       /* AHIP_FLAG_* */
-    uint32_t pad;
+    uint32_t curve_depth; /* de Casteljau depth of the ribbon test, 1..6; 0 = the reference's 6
+                             (MAX_DEPTH, src/accel/CurveBrute.cpp:194); other depths are not pinned by
+                             the reference, which fixes 6 */
 } ahip_render_params;
 
 /* One camera view of the batched persistent form. */

@@ -11,6 +11,7 @@ from anacapa_amd import _abi as A
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_RELEASE_SO = os.path.join(ROOT, "oracle", "liboracle_release.so")   # same source, Release flags
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libanacapa_ref.so")
 
 VP = C.c_void_p
@@ -119,10 +120,12 @@ class _Dome:
 class Oracle(_ShadingProbes):
     """This repository's C++ restatement of the reference CPU path (oracle/oracle.cpp)."""
 
-    def __init__(self):
-        self.lib = _load(ORACLE_SO)
+    def __init__(self, release: bool = False):
+        """release: the build at the reference's Release flags (timing only; tests use the default)."""
+        path = ORACLE_RELEASE_SO if release else ORACLE_SO
+        self.lib = _load(path)
         if self.lib is None:
-            raise RuntimeError("oracle/liboracle.so not built (make -C oracle)")
+            raise RuntimeError(f"{path} not built (make -C oracle)")
         L = self.lib
         L.orc_scene_create.restype = VP
         L.orc_scene_create.argtypes = [C.POINTER(A.SceneDesc)]

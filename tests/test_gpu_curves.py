@@ -127,3 +127,47 @@ def test_round_queries_equal_whole_ray_queries(gpu, monkeypatch, emitting_quad):
     for a, b in zip(out["1"], out["0"]):
         assert np.array_equal(a, b)
     assert out["0"][0][..., 3].min() == 4 and out["0"][0][..., :3].mean() > 0.01
+
+
+@pytest.mark.parametrize("depth", [5, 6])
+def test_ribbon_depth_hits_lie_on_the_analytic_bezier(gpu, depth):
+    """The ribbon test at de Casteljau depth 5 (the hair workload's) and 6 (the reference's) on the
+    GPU, against the analytic curve: a depth-d leaf chord differs from the curve at the same
+    parameter by at most E = h^2 / 8 max|B''| with h = 2^-d, so every hit's B(v) lies within
+    hw + E across the ray and within E in depth, rays through the ribbon (closer than hw - 2E)
+    hit and rays beside it (farther than hw + 2E) miss.  Depth 6 is also bit-exact against the
+    oracle above; depth 5 is pinned by these bounds only (the reference fixes 6)."""
+    from test_oracle_curves import _bezier, _one_strand
+    P = np.array([[0, 0, 0], [0, 1.2, 0], [1.0, 1.2, 0.3], [1.0, 0, 0.5]])
+    width, hw = 0.05, 0.025
+    sc = _one_strand(P.tolist(), [width, width])
+    gpu.prepare(sc, curve_depth=depth)
+    d0, d1 = P[0] - 2 * P[1] + P[2], P[1] - 2 * P[2] + P[3]
+    E = (2.0 ** -depth) ** 2 / 8 * 6 * max(np.linalg.norm(d0), np.linalg.norm(d1))
+    rng = np.random.default_rng(12)
+    n = 3000
+    c = _bezier(P, rng.uniform(0.02, 0.98, n))
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    side = np.cross(dirs, rng.normal(size=(n, 3)))
+    side /= np.linalg.norm(side, axis=1, keepdims=True)
+    org = c + side * rng.uniform(0, 2.0 * hw, n)[:, None] - dirs * 4.0
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7] = org, dirs, 1e-4, np.inf
+    h = gpu.trace_rays(rays)
+    hit = (h["mesh_id"] != 0xFFFFFFFF) & ((h["tri_index"] & 0x80000000) != 0)
+    grid = _bezier(P, np.linspace(0, 1, 4001))
+    o64, d64 = rays[:, :3].astype(np.float64), rays[:, 3:6].astype(np.float64)
+    rel = grid[None] - o64[:, None]
+    along = (rel * d64[:, None]).sum(-1)
+    dmin = np.linalg.norm(rel - along[..., None] * d64[:, None], axis=-1).min(1)
+    inside, outside = dmin < hw - 2 * E, dmin > hw + 2 * E
+    print(f"[curves] depth {depth}: E = {E:.2e}, {hit.sum()} hits, {inside.sum()} inside, {outside.sum()} outside",
+          flush=True)
+    assert hit[inside].all() and not hit[outside].any()
+    b = _bezier(P, h["v"][hit].astype(np.float64))
+    rb = b - o64[hit]
+    a = (rb * d64[hit]).sum(1)
+    across = np.linalg.norm(rb - a[:, None] * d64[hit], axis=1)
+    assert (across <= hw + E + 8e-5).all() and (np.abs(a - h["t"][hit]) <= E + 8e-5).all()
+    gpu.set_params(sc.max_depth, sc.min_depth, sc.seed, 0, 0)
