@@ -1,11 +1,10 @@
 // HipPathIntegrator.cpp — flattens SceneView / GeometryPool / materials / lights into the
 // ahip_scene_desc of include/anacapa_hip.h and maps ahip status codes onto the IIntegrator
 // contract (renderFrame returns false on failure so RenderSession renders tiles instead,
-// RenderSession.cpp:546-547).
+// RenderSession.cpp:546-547).  The merge into a Film lives in HipPathIntegratorFilm.cpp.
 #include "HipPathIntegrator.h"
 
 #include <anacapa/accel/IAccelerationStructure.h>
-#include <anacapa/film/Film.h>
 #include <anacapa/shading/ILight.h>
 #include <anacapa/shading/IMaterial.h>
 #include <anacapa/shading/ShadingContext.h>
@@ -71,7 +70,7 @@ bool toAhip(const IMaterial* mat, ahip_material& m) {
 
 }  // namespace
 
-HipPathIntegrator::HipPathIntegrator(int device) : m_maxDepth(8) {
+HipPathIntegrator::HipPathIntegrator(int device, uint32_t maxDepth) : m_maxDepth(maxDepth) {
     if (ahip_create(device, &m_ctx) != AHIP_OK)
         std::fprintf(stderr, "[warn] HipPathIntegrator: %s\n", ahip_last_error(m_ctx));
 }
@@ -79,6 +78,10 @@ HipPathIntegrator::HipPathIntegrator(int device) : m_maxDepth(8) {
 HipPathIntegrator::~HipPathIntegrator() { ahip_destroy(m_ctx); }
 
 bool HipPathIntegrator::isValid() const { return ahip_is_valid(m_ctx) != 0; }
+bool HipPathIntegrator::prepared() const {
+    std::lock_guard<std::mutex> lk(m_mutex);
+    return m_prepared;
+}
 std::string HipPathIntegrator::name() const { return ahip_device_name(m_ctx); }
 std::string HipPathIntegrator::lastError() const { return ahip_last_error(m_ctx); }
 
@@ -204,16 +207,8 @@ bool HipPathIntegrator::renderFrame(const SceneView& scene, uint32_t W, uint32_t
     std::lock_guard<std::mutex> lk(m_mutex);
     if (!m_prepared || !ensureCamera(scene, W, H)) return false;
     std::vector<float> acc((size_t)W * H * 5);
-    TileBuffer tb(0, 0, W, H);
     if (ahip_render_tile(m_ctx, 0, 0, W, H, sampleStart, sampleCount, acc.data()) != AHIP_OK) return false;
-    for (uint32_t y = 0; y < H; ++y)
-        for (uint32_t x = 0; x < W; ++x) {
-            const float* p = &acc[5 * ((size_t)y * W + x)];
-            float w = p[3] > 0.f ? p[3] : 1.f;
-            tb.add(x, y, p[0] / w, p[1] / w, p[2] / w, w);
-            tb.addLumSq(x, y, p[4]);
-        }
-    film.mergeTile(tb);
+    mergeAccumIntoFilm(film, acc.data(), 0, 0, W, H);
     return true;
 }
 
@@ -225,15 +220,7 @@ bool HipPathIntegrator::renderAdaptive(const SceneView& scene, uint32_t W, uint3
     if (ahip_render_adaptive(m_ctx, W, H, tileSize, totalSpp, baseSpp, nullptr, nullptr) != AHIP_OK) return false;
     std::vector<float> acc((size_t)W * H * 5);
     if (ahip_download_accum(m_ctx, acc.data()) != AHIP_OK) return false;
-    TileBuffer tb(0, 0, W, H);
-    for (uint32_t y = 0; y < H; ++y)
-        for (uint32_t x = 0; x < W; ++x) {
-            const float* p = &acc[5 * ((size_t)y * W + x)];
-            float w = p[3] > 0.f ? p[3] : 1.f;
-            tb.add(x, y, p[0] / w, p[1] / w, p[2] / w, w);
-            tb.addLumSq(x, y, p[4]);
-        }
-    film.mergeTile(tb);
+    mergeAccumIntoFilm(film, acc.data(), 0, 0, W, H);
     return true;
 }
 

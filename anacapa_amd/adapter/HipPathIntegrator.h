@@ -6,6 +6,7 @@
 #pragma once
 
 #include <anacapa/integrator/IIntegrator.h>
+#include <anacapa/film/Film.h>
 
 #include <memory>
 #include <mutex>
@@ -15,9 +16,16 @@ struct ahip_context;
 
 namespace anacapa {
 
+// Adds a device accumulation (w*h pixels of r, g, b sums, sample count, luminance-squared sum, as
+// ahip_download_accum / ahip_render_tile return it) into `film` at (x0, y0) through a TileBuffer
+// and Film::mergeTile (the CudaPathIntegrator merge, CudaPathIntegrator.cu:404-417).  Defined in
+// HipPathIntegratorFilm.cpp, the adapter's only use of Film.cpp.
+void mergeAccumIntoFilm(Film& film, const float* acc, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h);
+
 class HipPathIntegrator : public IIntegrator {
 public:
-    explicit HipPathIntegrator(int device = 0);
+    // maxDepth: PathIntegrator's (RenderSession passes m_settings.maxDepth, RenderSession.cpp:463).
+    explicit HipPathIntegrator(int device = 0, uint32_t maxDepth = 8);
     ~HipPathIntegrator() override;
 
     void prepare(const SceneView& scene) override;
@@ -33,6 +41,7 @@ public:
                         uint32_t totalSpp, uint32_t baseSpp, Film& film);
 
     bool isValid() const;
+    bool prepared() const;        // the last prepare() took the scene (else RenderSession uses the CPU)
     void clearAccum();
     std::string name() const;
     std::string lastError() const;

@@ -1,5 +1,7 @@
-"""The reference-side adapter (anacapa_amd/adapter/HipPathIntegrator.cpp) compiles against the
-reference's own headers, i.e. it is a drop-in IIntegrator for RenderSession."""
+"""The reference-side adapter (anacapa_amd/adapter/HipPathIntegrator*.cpp) compiles against the
+reference's own headers, i.e. it is a drop-in IIntegrator for RenderSession, and linked into the
+reference's render loop without a device it leaves the render to the CPU integrator.  The run on
+the GPU is tests/test_gpu_adapter.py."""
 import os
 import shutil
 import subprocess
@@ -15,6 +17,33 @@ def test_adapter_compiles_against_reference_headers():
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("no g++")
-    subprocess.check_call([cxx, "-std=c++20", "-fsyntax-only", "-Wall", "-I", os.path.join(REF, "include"),
-                           "-I", os.path.join(REF, "src"), "-I", os.path.join(ROOT, "include"),
-                           os.path.join(ROOT, "anacapa_amd", "adapter", "HipPathIntegrator.cpp")])
+    for src in ("HipPathIntegrator.cpp", "HipPathIntegratorFilm.cpp"):
+        subprocess.check_call([cxx, "-std=c++20", "-fsyntax-only", "-Wall", "-I", os.path.join(REF, "include"),
+                               "-I", os.path.join(REF, "src"), "-I", os.path.join(ROOT, "include"),
+                               os.path.join(ROOT, "anacapa_amd", "adapter", src)])
+
+
+def _harness_scene():
+    from anacapa_amd import scenes
+    from oracle_bindings import Oracle
+    sc = scenes.cornell(width=96, height=80)
+    cam = Oracle().pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height)
+    return sc, cam
+
+
+def test_adapter_falls_back_without_a_device():
+    """Linked into the reference's render loop with no HIP device present, the adapter reports
+    isValid() == false and does not take the scene, so RenderSession keeps the CPU integrator
+    (RenderSession.cpp:441-464); the reference integrator renders through the same harness."""
+    from adapter_harness import AdapterHarness, available
+    if not available():
+        pytest.skip("adapter harness (oracle/_ref) not built")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: the device path is tested in test_gpu_adapter.py")
+    sc, cam = _harness_scene()
+    h = AdapterHarness(sc, cam, sc.max_depth)
+    st = h.status()
+    assert not st["valid"] and not st["prepared"], st
+    out, _, _ = h.render("reference", 2, threads=4)
+    assert (out[..., 3] == 2).all() and out[..., :3].sum() > 0
