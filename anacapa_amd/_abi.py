@@ -96,7 +96,8 @@ class Stats(C.Structure):
                 ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64),
                 ("trace_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("shade_launches", C.c_uint64),
-                ("render_ms", C.c_double), ("passes", C.c_uint64), ("graph_passes", C.c_uint64)]
+                ("render_ms", C.c_double), ("passes", C.c_uint64), ("graph_passes", C.c_uint64),
+                ("graph_captures", C.c_uint64)]
 
 
 AHIP_CURVE_LINEAR, AHIP_CURVE_BEZIER, AHIP_CURVE_BSPLINE, AHIP_CURVE_CATMULL_ROM = 0, 1, 2, 3
@@ -162,6 +163,7 @@ SIGNATURES = {
     "ahip_adjoint": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP]),
     "ahip_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
     "ahip_reset_stats": (C.c_int, [VP]),
+    "ahip_get_bounce_stats": (C.c_int, [VP, C.c_uint32, VP, VP, C.POINTER(C.c_uint32)]),
     "ahip_count_traffic": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "ahip_set_launch": (C.c_int, [VP, C.c_uint32, C.c_uint32]),
     "ahip_fail_next_launch": (C.c_int, [VP]),

@@ -52,7 +52,8 @@ struct DevBuf {
 
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
-    int kind = 0;   // 0 trace, 1 shade
+    int kind = 0;     // 0 trace, 1 shade
+    int bounce = -1;  // the bounce a traversal / shade launch belongs to (-1: generate, accumulate)
 };
 
 // ------------------------------------------------------------ no-allocation traversal scratch
@@ -141,9 +142,18 @@ struct ahip_context {
 
     // statistics
     ahip_stats stats{};
+    std::vector<unsigned long long> bounceRays;   // per bounce: continuation, shadow, MIS rays
+    std::vector<double> bounceMs;                 // per bounce: timed traversal + shade launches
     std::vector<EventPair> events;
     size_t eventsUsed = 0;
     hipEvent_t renderBeg = nullptr, renderEnd = nullptr;
+    hipGraphExec_t passGraph = nullptr;          // the last captured pass (render_batch)
+    std::vector<unsigned char> passGraphKey;     // the kernel arguments it was captured with
+    void dropPassGraph() {
+        if (passGraph) hipGraphExecDestroy(passGraph);
+        passGraph = nullptr;
+        passGraphKey.clear();
+    }
     void* scratch = nullptr;     // this device's no-allocation scratch (ahip_trace_rays_noalloc)
     std::string scratchErr;      // why it could not be resolved at create
     bool lastRenderValid = false;
@@ -155,6 +165,7 @@ struct ahip_context {
     uint32_t lastW = 0, lastH = 0, lastS0 = 0, lastNs = 0;
 
     ~ahip_context() {
+        dropPassGraph();
         for (auto* b : texBufs) delete b;
         for (auto& e : events) {
             if (e.a) hipEventDestroy(e.a);
@@ -196,7 +207,7 @@ int checkInjected(ahip_context* ctx) {
     return AHIP_OK;
 }
 
-EventPair& nextEvent(ahip_context* ctx, int kind) {
+EventPair& nextEvent(ahip_context* ctx, int kind, int bounce) {
     if (ctx->eventsUsed == ctx->events.size()) {
         EventPair e;
         hipEventCreate(&e.a);
@@ -205,6 +216,7 @@ EventPair& nextEvent(ahip_context* ctx, int kind) {
     }
     EventPair& e = ctx->events[ctx->eventsUsed++];
     e.kind = kind;
+    e.bounce = bounce;
     return e;
 }
 
@@ -242,7 +254,7 @@ int ensureQueues(ahip_context* ctx, uint32_t cap) {
     HIPCHK(ctx->qPix.alloc((size_t)cap * sizeof(uint2)));
     HIPCHK(ctx->qPassIdx.alloc(sizeof(uint32_t)));
     HIPCHK(ctx->qCtr.alloc(slots * sizeof(uint32_t)));
-    HIPCHK(ctx->qTotals.alloc(8 * sizeof(unsigned long long)));
+    HIPCHK(ctx->qTotals.alloc((8 + (size_t)slots) * sizeof(unsigned long long)));
     ctx->qCap = cap;
     ctx->qCtrSlots = slots;
     return AHIP_OK;
@@ -385,33 +397,33 @@ void enqueuePass(ahip_context* ctx, const WaveQueues& q, const RenderConsts& rc,
                  bool events, unsigned long long* countSums) {
     hipStream_t s = ctx->stream;
     const uint32_t maxDepth = rc.maxDepth;
-    auto timed = [&](int kind, auto&& fn) {
+    auto timed = [&](int kind, int bounce, auto&& fn) {
         if (!events) {
             fn();
             return;
         }
-        EventPair& e = nextEvent(ctx, kind);
+        EventPair& e = nextEvent(ctx, kind, bounce);
         hipEventRecord(e.a, s);
         fn();
         hipEventRecord(e.b, s);
     };
     uint32_t gb = std::max(1u, std::min((cap + 255u) / 256u, ctx->shadeBlocks));
-    timed(1, [&] { launchGenerate(rc, q, gb, s); });
+    timed(1, -1, [&] { launchGenerate(rc, q, gb, s); });
     uint32_t cur = 0;
     for (uint32_t b = 0; b <= maxDepth; ++b) {
         if (countSums) launchCount(ctx->dsc, q, b, cur, 0, ctx->traceBlocks, countSums, s);
-        timed(0, [&] { launchExtend(ctx->dsc, q, b, cur, ctx->traceBlocks, s); });
-        timed(1, [&] { launchShade(ctx->dsc, rc, q, b, cur, ctx->shadeBlocks, s); });
+        timed(0, (int)b, [&] { launchExtend(ctx->dsc, q, b, cur, ctx->traceBlocks, s); });
+        timed(1, (int)b, [&] { launchShade(ctx->dsc, rc, q, b, cur, ctx->shadeBlocks, s); });
         if (countSums) {
             launchCount(ctx->dsc, q, b, cur, 1, ctx->traceBlocks, countSums, s);
             launchCount(ctx->dsc, q, b, cur, 2, ctx->traceBlocks, countSums, s);
         }
-        timed(0, [&] { launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s); });
-        timed(0, [&] { launchMis(ctx->dsc, q, b, ctx->traceBlocks, s); });
+        timed(0, (int)b, [&] { launchShadow(ctx->dsc, q, b, ctx->traceBlocks, s); });
+        timed(0, (int)b, [&] { launchMis(ctx->dsc, q, b, ctx->traceBlocks, s); });
         cur ^= 1u;
     }
     uint32_t ab = std::max(1u, std::min((maxSlots + 255u) / 256u, 8192u));
-    timed(1, [&] { launchAccumulate(q, ab, s); });
+    timed(1, -1, [&] { launchAccumulate(q, ab, s); });
     launchPassEnd(q, maxDepth, ctx->qTotals.as<unsigned long long>(), s);
 }
 
@@ -450,23 +462,41 @@ int renderJobs(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool graph, 
     launchFillU32(q.counters, ctx->qCtrSlots, 0u, s);
     launchFillU32(q.passIdx, 1, 0u, s);
     if (graph && !countSums) {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        enqueuePass(ctx, q, rc, cap, maxSlots, false, nullptr);
-        hipError_t e = hipStreamEndCapture(s, &g);
-        if (e != hipSuccess) return fail(ctx, AHIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
+        // One pass captured into a graph and replayed per pass.  The graph bakes in the kernel
+        // arguments (queue and scene pointers, render constants, pass capacity), so an executable
+        // graph whose arguments are byte-identical is reused across calls instead of re-captured.
+        std::vector<unsigned char> key;
+        auto put = [&](const void* p, size_t n) {
+            const unsigned char* b = static_cast<const unsigned char*>(p);
+            key.insert(key.end(), b, b + n);
+        };
+        put(&q, sizeof(q));
+        put(&rc, sizeof(rc));
+        put(&ctx->dsc, sizeof(ctx->dsc));
+        const uint32_t dims[4] = {cap, maxSlots, ctx->traceBlocks, ctx->shadeBlocks};
+        put(dims, sizeof(dims));
+        if (!ctx->passGraph || key != ctx->passGraphKey) {
+            ctx->dropPassGraph();
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueuePass(ctx, q, rc, cap, maxSlots, false, nullptr);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess) return fail(ctx, AHIP_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            hipGraphExec_t ge = nullptr;
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
-            return fail(ctx, AHIP_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            if (e != hipSuccess) return fail(ctx, AHIP_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            ctx->passGraph = ge;
+            ctx->passGraphKey = std::move(key);
+            ctx->stats.graph_captures += 1;
         }
-        for (size_t p = 0; p < tables.size() && e == hipSuccess; ++p) e = hipGraphLaunch(ge, s);
+        hipError_t e = hipSuccess;
+        for (size_t p = 0; p < tables.size() && e == hipSuccess; ++p) e = hipGraphLaunch(ctx->passGraph, s);
         hipError_t e2 = hipStreamSynchronize(s);
-        hipGraphExecDestroy(ge);
-        hipGraphDestroy(g);
-        if (e != hipSuccess || e2 != hipSuccess)
+        if (e != hipSuccess || e2 != hipSuccess) {
+            ctx->dropPassGraph();
             return fail(ctx, AHIP_E_HIP, std::string("graph launch: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
         ctx->stats.graph_passes += tables.size();
     } else {
         for (size_t p = 0; p < tables.size(); ++p) enqueuePass(ctx, q, rc, cap, maxSlots, true, countSums);
@@ -489,10 +519,20 @@ int finishStats(ahip_context* ctx) {
             ctx->stats.shade_ms += ms;
             ctx->stats.shade_launches += 1;
         }
+        const int b = ctx->events[i].bounce;
+        if (b >= 0) {
+            if (ctx->bounceMs.size() <= (size_t)b) ctx->bounceMs.resize(b + 1, 0.0);
+            ctx->bounceMs[b] += ms;
+        }
     }
     ctx->eventsUsed = 0;
-    unsigned long long tot[8];
-    HIPCHK(hipMemcpy(tot, ctx->qTotals.p, sizeof(tot), hipMemcpyDeviceToHost));
+    const uint32_t nb = ctx->params.max_depth + 1u;
+    std::vector<unsigned long long> all(8 + 4 * (size_t)nb);
+    HIPCHK(hipMemcpy(all.data(), ctx->qTotals.p, all.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (ctx->bounceRays.size() < 3 * (size_t)nb) ctx->bounceRays.resize(3 * (size_t)nb, 0ull);
+    for (uint32_t b = 0; b < nb; ++b)
+        for (uint32_t k = 0; k < 3; ++k) ctx->bounceRays[3 * b + k] += all[8 + 4 * b + k];
+    const unsigned long long* tot = all.data();
     // tot[0] continuation rays, tot[1] shadow rays, tot[2] MIS rays, tot[3] stack overflows
     ctx->stats.rays_closest += tot[0] + (ctx->dsc.hasEmitters ? tot[2] : 0ull);
     ctx->stats.rays_shadow += tot[1] + (ctx->dsc.hasEmitters ? 0ull : tot[2]);
@@ -730,7 +770,7 @@ static int renderCall(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool 
     if (st) return st;
     st = ensureQueues(ctx, 64);
     if (st) return st;
-    launchFillU32(reinterpret_cast<uint32_t*>(ctx->qTotals.p), 16, 0u, ctx->stream);
+    launchFillU32(reinterpret_cast<uint32_t*>(ctx->qTotals.p), (uint32_t)(ctx->qTotals.bytes / 4), 0u, ctx->stream);
     HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
     st = renderJobs(ctx, jobs, graph, counts);
     if (st) return st;
@@ -1325,6 +1365,21 @@ int ahip_reset_stats(ahip_context* ctx) {
     if (!ctx) return AHIP_E_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->stats = ahip_stats{};
+    ctx->bounceRays.clear();
+    ctx->bounceMs.clear();
+    return AHIP_OK;
+}
+
+int ahip_get_bounce_stats(ahip_context* ctx, uint32_t capacity, uint64_t* rays, double* ms, uint32_t* num_bounces) {
+    if (!ctx || !num_bounces) return AHIP_E_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const uint32_t nb = (uint32_t)std::max(ctx->bounceRays.size() / 3, ctx->bounceMs.size());
+    *num_bounces = nb;
+    for (uint32_t b = 0; b < std::min(nb, capacity); ++b) {
+        for (uint32_t k = 0; k < 3; ++k)
+            if (rays) rays[3 * b + k] = 3 * (size_t)b + k < ctx->bounceRays.size() ? ctx->bounceRays[3 * b + k] : 0ull;
+        if (ms) ms[b] = b < ctx->bounceMs.size() ? ctx->bounceMs[b] : 0.0;
+    }
     return AHIP_OK;
 }
 

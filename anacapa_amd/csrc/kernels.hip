@@ -1024,6 +1024,8 @@ __global__ __launch_bounds__(BS) void k_count(DScene sc, WaveQueues q, uint32_t 
 
 // End of a pass: ray totals ([0] continuation, [1] shadow, [2] MIS rays, [3] stack overflows) and
 // the device-side pass index advanced, so the next replay of the same launches runs the next pass.
+// totals[0..3]: the pass's continuation, shadow and MIS rays and stack overflows summed over the
+// bounces; totals[8 + 4 b + k]: the same per bounce b (the occupancy of each bounce's launches).
 __global__ void k_pass_end(uint32_t* counters, uint32_t maxDepth, unsigned long long* totals, uint32_t* passIdx) {
     if (blockIdx.x != 0u) return;
     if (threadIdx.x < 4u) {
@@ -1031,6 +1033,7 @@ __global__ void k_pass_end(uint32_t* counters, uint32_t maxDepth, unsigned long 
         for (uint32_t b = 0; b <= maxDepth; ++b) s += counters[b * 4u + threadIdx.x];
         totals[threadIdx.x] += s;
     }
+    for (uint32_t i = threadIdx.x; i < (maxDepth + 1u) * 4u; i += blockDim.x) totals[8u + i] += counters[i];
     __syncthreads();
     // counters of the next pass start at zero (a kernel, not a memset, so it can live in a graph)
     for (uint32_t i = threadIdx.x; i < (maxDepth + 2u) * 4u; i += blockDim.x) counters[i] = 0u;

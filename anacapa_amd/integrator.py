@@ -253,6 +253,15 @@ class HipPathIntegrator:
         self._check(self.lib.ahip_get_stats(self._ctx, C.byref(s)))
         return {k: getattr(s, k) for k, _ in A.Stats._fields_}
 
+    def bounce_stats(self):
+        """Per bounce since reset_stats: rays [n, 3] (continuation, shadow, MIS) and ms [n]."""
+        n = C.c_uint32()
+        self._check(self.lib.ahip_get_bounce_stats(self._ctx, 0, None, None, C.byref(n)))
+        rays = np.zeros((n.value, 3), np.uint64)
+        ms = np.zeros(n.value, np.float64)
+        self._check(self.lib.ahip_get_bounce_stats(self._ctx, n.value, A.ptr(rays), A.ptr(ms), C.byref(n)))
+        return rays, ms
+
     def reset_stats(self):
         self._check(self.lib.ahip_reset_stats(self._ctx))
 

@@ -275,21 +275,62 @@ def batched_form(be, sc, args, log):
     # sequential plain renders at the plain workload's tuned launch, else the library default
     be.set_launch(max_paths=plain["max_paths"] if plain else DEFAULT_MAX_PATHS, tile=plain["tile"] if plain else 8)
     t_seq, worst = 0.0, 0.0
+    # occupancy of the sequential renders: every bounce's launches, timed with HIP events, against the
+    # paths they carry; a launch with fewer continuation paths than the GPU holds resident threads
+    # (CUs x 2048) leaves lanes idle, the only time packing several views into one pass can recover
+    resident = 2048 * int(be.name().split(" CUs")[0].split()[-1]) if " CUs" in be.name() else 524288
+    t_bounce = t_under = 0.0
+    under_launches = launches = 0
+    per_bounce = {}
     for cam, n, out in zip(cams, counts, outs):
         t0 = time.perf_counter()
         be.set_camera(cam)
         be.clear_accum()
+        be.reset_stats()
         be.render_frame(0, n)
+        rays, ms = be.bounce_stats()
         ref = be.download_accum()
         t_seq += time.perf_counter() - t0
+        passes = max(1, int(be.stats()["passes"]))
+        for b in range(len(ms)):
+            launches += passes
+            t_bounce += ms[b]
+            pb = per_bounce.setdefault(b, {"bounce": b, "launches": 0, "paths": 0, "ms": 0.0,
+                                           "underfilled_launches": 0, "underfilled_ms": 0.0})
+            pb["launches"] += passes
+            pb["paths"] += int(rays[b, 0])
+            pb["ms"] += float(ms[b])
+            if rays[b, 0] / passes < resident:
+                t_under += ms[b]
+                under_launches += passes
+                pb["underfilled_launches"] += passes
+                pb["underfilled_ms"] += float(ms[b])
         worst = max(worst, float(np.max(np.abs(ref[..., :3] - out[..., :3]) / (np.abs(ref[..., :3]) + 1.0))))
     be.set_camera(be.pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height))
-    log(f"[bench] sequential: {t_seq * 1e3:.1f} ms (max rel diff {worst:.2e})")
+    log(f"[bench] sequential: {t_seq * 1e3:.1f} ms (max rel diff {worst:.2e}); bounce launches under "
+        f"{resident} paths: {under_launches} of {launches}, {t_under:.1f} of {t_bounce:.1f} ms")
+    # the same call again: the captured pass graph is reused (no capture, no instantiation)
+    be.reset_stats()
+    t0 = time.perf_counter()
+    be.render_batch(cams, starts, counts, args.depth)
+    t_again = time.perf_counter() - t0
+    captures = int(be.stats()["graph_captures"])
+    log(f"[bench] batched again: {t_again * 1e3:.1f} ms, {captures} graph captures")
     return {"views": len(cams), "samples": samples, "depth": args.depth,
             "msps_batched": round(samples / t_batch / 1e6, 3),
             "msps_sequential": round(samples / t_seq / 1e6, 3),
             "speedup": round(t_seq / t_batch, 3), "passes": int(st["passes"]),
             "graph_passes": int(st["graph_passes"]), "max_rel_diff_vs_sequential": worst,
+            "repeat_call_ms": round(t_again * 1e3, 1), "repeat_call_graph_captures": captures,
+            "occupancy": {"resident_threads": resident, "sequential_bounce_launches": launches,
+                          "underfilled_launches": under_launches, "sequential_bounce_ms": round(t_bounce, 2),
+                          "underfilled_ms": round(t_under, 2),
+                          "underfilled_share": round(t_under / max(t_bounce, 1e-9), 4),
+                          "bound_on_speedup": round(t_seq * 1e3 / max(t_seq * 1e3 - t_under, 1e-9), 3),
+                          "per_bounce": [dict(v, ms=round(v["ms"], 2), underfilled_ms=round(v["underfilled_ms"], 2))
+                                         for _, v in sorted(per_bounce.items())],
+                          "definition": "bounce launches (traversal + shade) of the 64 sequential renders whose "
+                                        "continuation paths per pass are fewer than the GPU's resident threads"},
             "launch": tuned and {"max_paths": tuned["max_paths"], "tile": tuned["tile"]},
             "timing": "host wall clock incl. download of every view's film (comparison loop excluded)"}
 

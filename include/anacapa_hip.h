@@ -184,6 +184,7 @@ typedef struct ahip_stats {
     double   render_ms;          /* wall time of the last render call on the stream (HIP events) */
     uint64_t passes;             /* wavefront passes run */
     uint64_t graph_passes;       /* of which replayed from a captured HIP graph */
+    uint64_t graph_captures;     /* pass graphs captured (a graph is reused while its launch arguments hold) */
 } ahip_stats;
 
 typedef struct ahip_context ahip_context;
@@ -341,6 +342,11 @@ int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sa
 
 /* --- measurement --- */
 int ahip_get_stats(ahip_context* ctx, ahip_stats* out);
+/* Per-bounce occupancy since the last ahip_reset_stats: rays[3*b + k] = continuation (k = 0),
+ * shadow (1) and BSDF-sampled MIS (2) rays of bounce b summed over passes; ms[b] = HIP-event time of
+ * bounce b's traversal and shade launches (launches outside graphs only).  Either array may be
+ * NULL; *num_bounces = bounces recorded (entries beyond `capacity` are not written). */
+int ahip_get_bounce_stats(ahip_context* ctx, uint32_t capacity, uint64_t* rays, double* ms, uint32_t* num_bounces);
 int ahip_reset_stats(ahip_context* ctx);
 /* Re-run the last render_frame workload with per-ray node/primitive counters
  * (a separate, untimed pass) and fill nodes_visited/prims_tested. */

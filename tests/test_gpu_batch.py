@@ -46,3 +46,25 @@ def test_tile_shape_does_not_change_the_image(gpu, tile):
     b = gpu.render_tile(0, 0, 48, 40, 0, 3)
     gpu.set_launch(tile=8)
     np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-7)
+
+
+def test_batch_graph_is_reused_across_calls(gpu):
+    """A second ahip_render_batch call with the same launch arguments replays the pass graph the
+    first one captured (no new capture) and produces the same films bit for bit; changing the
+    depth changes the captured arguments and captures again."""
+    sc = scenes.small_ggx()
+    gpu.prepare(sc)
+    cams = _cams(gpu, sc, [(32, 32), (48, 24), (64, 40)])
+    gpu.set_launch(max_paths=4096)
+    gpu.reset_stats()
+    a = gpu.render_batch(cams, [0, 0, 0], [2, 3, 1])
+    first = gpu.stats()["graph_captures"]
+    gpu.reset_stats()
+    b = gpu.render_batch(cams, [0, 0, 0], [2, 3, 1])
+    assert first >= 1 and gpu.stats()["graph_captures"] == 0
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    gpu.reset_stats()
+    gpu.render_batch(cams, [0, 0, 0], [2, 3, 1], max_depth=3)
+    assert gpu.stats()["graph_captures"] >= 1
+    gpu.set_launch(max_paths=1 << 22)
