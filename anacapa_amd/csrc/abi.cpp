@@ -131,8 +131,9 @@ struct ahip_context {
 
     // wavefront queues
     DevBuf qRo[2], qRd[2], qBeta[2], qHit, qHitAux, qSo, qSd, qSc, qMo, qMd, qMc, qL, qCtr, qOvfRef, qOvfT, qTotals;
-    DevBuf qCList, qCState, qCStack;   // curve scenes: the resumable curve-tree walk
+    DevBuf qCState, qCwRef, qCwT;      // curve scenes: the pooled curve queries
     uint32_t qCurveCap = 0, qCurveStack = 0;
+    uint32_t poolBlocks = 0;           // k_curve_pool grid (one wave per block)
     DevBuf qPix, qPassIdx, dViews, dTables, dBatchAcc, dBatchLsq;
     uint32_t tileW = 8;          // pixel tile per wavefront: tileW x (64 / tileW)
     bool frameGraph = false;     // render_frame: per-launch timing events (the roofline figure needs them)
@@ -223,13 +224,14 @@ EventPair& nextEvent(ahip_context* ctx, int kind, int bounce) {
 int ensureQueues(ahip_context* ctx, uint32_t cap) {
     uint32_t slots = (ctx->params.max_depth + 2u) * 4u;
     if (!ctx->curves.nodes.empty()) {
-        // a walk holds at most one pending right child per level below the root, plus the child
-        // being entered
+        // a walk that pushes a node's children after testing them holds at most one pending child
+        // per level below the root, plus the two just pushed
         const uint32_t pc = std::max(cap, ctx->qCap), sc = ctx->curves.maxDepth + 2u;
-        if (ctx->qCurveCap != pc || ctx->qCurveStack != sc || !ctx->qCList.p) {
-            HIPCHK(ctx->qCList.alloc((size_t)pc * kCurveList * sizeof(uint32_t)));
+        if (ctx->qCurveCap != pc || ctx->qCurveStack != sc || !ctx->qCState.p) {
             HIPCHK(ctx->qCState.alloc((size_t)pc * sizeof(uint32_t)));
-            HIPCHK(ctx->qCStack.alloc((size_t)pc * sc * sizeof(uint32_t)));
+            const size_t deep = (size_t)(sc > kPoolStackK ? sc - kPoolStackK : 1u) * ctx->poolBlocks * 64u;
+            HIPCHK(ctx->qCwRef.alloc(deep * sizeof(uint32_t)));
+            HIPCHK(ctx->qCwT.alloc(deep * sizeof(float)));
             ctx->qCurveCap = pc;
             ctx->qCurveStack = sc;
         }
@@ -295,11 +297,14 @@ WaveQueues queues(ahip_context* ctx) {
     q.ovfT = ctx->qOvfT.as<float>();
     q.stackCap = ctx->stackCap;
     q.cap = ctx->qCap;
-    if (!ctx->curves.nodes.empty() && ctx->qCurveCap == ctx->qCap) {
-        q.cList = ctx->qCList.as<uint32_t>();
+    // the pool entry holds a record in kPoolSlotShift bits; larger curve sets use the whole-ray kernels
+    if (!ctx->curves.nodes.empty() && ctx->qCurveCap == ctx->qCap &&
+        ctx->curves.segs.size() < (size_t(1) << kPoolSlotShift)) {
         q.cState = ctx->qCState.as<uint32_t>();
-        q.cStack = ctx->qCStack.as<uint32_t>();
-        q.cStackCap = ctx->qCurveStack;
+        q.cwRef = ctx->qCwRef.as<uint32_t>();
+        q.cwT = ctx->qCwT.as<float>();
+        q.cwCap = ctx->qCurveStack;
+        q.poolBlocks = ctx->poolBlocks;
     }
     return q;
 }
@@ -571,6 +576,9 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     // ones keep the CUs evenly loaded to the end of a launch (measured on the 1080p frame:
     // traversal 18.6 ms at 16 blocks per CU, 16.1 ms at 128; shade 5.9 ms at 8, 5.45 ms at 32).
     ctx->traceBlocks = std::max(1u, ctx->numCU) * 128u;
+    // the pooled curve queries: 16 one-wave blocks per CU, the waves that stay resident at the
+    // kernel's register use (each loops over its share of the rays)
+    ctx->poolBlocks = std::max(1u, ctx->numCU) * 16u;
     ctx->shadeBlocks = std::max(1u, ctx->numCU) * 32u;
     HaltonTables T;
     fillHaltonTables(T);

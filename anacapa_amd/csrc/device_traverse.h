@@ -384,16 +384,10 @@ __device__ __forceinline__ V3 midNC(V3 a, V3 b) {
     return V3{(a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f};
 }
 
-// Returns true when a strictly nearer hit on this segment was found (best, out updated).
-// depth: de Casteljau levels before the chord test (the reference's MAX_DEPTH = 6, CurveBrute.cpp:194;
-// 1..6, which the 7-entry stack of pending halves holds).
-__device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang,
-                           int depth) {
-#pragma clang fp contract(off)
-    // the frame being tested stays in registers; only pending right halves go to the private
-    // stack (at most one per level), in the order the recursion would visit them
-    SubFrame st[7];
-    int sp = 0;
+// The root frame of a segment record: its control points, end half widths and parameter range,
+// `depth` de Casteljau levels above the chord test (the reference's MAX_DEPTH = 6,
+// CurveBrute.cpp:194; 1..6, which the 7-entry stack of pending halves holds).
+__device__ __forceinline__ SubFrame rootFrame(const DSeg& S, int depth) {
     SubFrame f;
     f.P[0] = V3{S.P[0], S.P[1], S.P[2]};
     f.P[1] = V3{S.P[3], S.P[4], S.P[5]};
@@ -404,69 +398,100 @@ __device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& be
     f.tOff = S.tOff;
     f.tScale = S.tScale;
     f.depth = depth;
+    return f;
+}
+
+// One frame of the reference's recursion (CurveBrute.cpp:85-176), as an explicit depth-first
+// walk: the frame `f` is tested in ray space against [tMin, best]; a frame that passes and is not
+// a leaf is split, its left half becomes `f` and its right half waits on `st` (at most one per
+// level, in the order the recursion visits them); otherwise the next pending half is taken.
+// Returns true when no frame is left.  `acc` is set when the leaf chord of `f` was accepted,
+// strictly nearer than `best`, with its distance `tr`, strand parameter `tSeg` and unit tangent.
+__device__ __forceinline__ bool segmentStep(SubFrame& f, SubFrame* st, int& sp, V3 o, V3 d, float tMin, V3 fx,
+                                            V3 fy, float best, bool& acc, float& tr, float& tSeg, V3& tang) {
+#pragma clang fp contract(off)
+    acc = false;
+    {
+        V3 c0 = projNC(f.P[0], o, fx, fy, d), c1 = projNC(f.P[1], o, fx, fy, d);
+        V3 c2 = projNC(f.P[2], o, fx, fy, d), c3 = projNC(f.P[3], o, fx, fy, d);
+        float mh = fmaxf(f.hw0, f.hw3);
+        float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z), zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
+        if (zMin - mh > best) goto next;
+        if (zMax + mh < tMin) goto next;
+        float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
+        float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
+        float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
+        float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
+        if (cx * cx + cy * cy > mh * mh) goto next;
+        if (f.depth == 0) {
+            float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
+            float l2 = dx * dx + dy * dy;
+            float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
+            float px = ax + dx * tl, py = ay + dy * tl;
+            float d2 = px * px + py * py;
+            float hw = f.hw0 * (1.f - tl) + f.hw3 * tl;
+            if (d2 > hw * hw) goto next;
+            float t = c0.z + (c3.z - c0.z) * tl;
+            if (t < tMin || t >= best) goto next;
+            // bezierTangent (CurveBrute.cpp:59-64)
+            float mt = 1.f - tl;
+            float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
+            V3 d10{f.P[1].x - f.P[0].x, f.P[1].y - f.P[0].y, f.P[1].z - f.P[0].z};
+            V3 d21{f.P[2].x - f.P[1].x, f.P[2].y - f.P[1].y, f.P[2].z - f.P[1].z};
+            V3 d32{f.P[3].x - f.P[2].x, f.P[3].y - f.P[2].y, f.P[3].z - f.P[2].z};
+            V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
+                  d10.z * a0 + d21.z * a1 + d32.z * a2};
+            if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
+                tg = V3{f.P[3].x - f.P[0].x, f.P[3].y - f.P[0].y, f.P[3].z - f.P[0].z};
+            float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
+            if (l > 1e-8f) {
+                float il = 1.f / l;
+                tg = V3{tg.x * il, tg.y * il, tg.z * il};
+            } else {
+                tg = V3{0.f, 0.f, 1.f};
+            }
+            acc = true;
+            tr = t;
+            tSeg = f.tOff + tl * f.tScale;
+            tang = tg;
+            goto next;
+        }
+        V3 q01 = midNC(f.P[0], f.P[1]), q12 = midNC(f.P[1], f.P[2]), q23 = midNC(f.P[2], f.P[3]);
+        V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
+        float hm = (f.hw0 + f.hw3) * 0.5f, ts = f.tScale * 0.5f;
+        SubFrame R;
+        R.P[0] = m; R.P[1] = r12; R.P[2] = q23; R.P[3] = f.P[3];
+        R.hw0 = hm; R.hw3 = f.hw3; R.tOff = f.tOff + ts; R.tScale = ts; R.depth = f.depth - 1;
+        st[sp++] = R;
+        f.P[1] = q01; f.P[2] = r01; f.P[3] = m;
+        f.hw3 = hm; f.tScale = ts; f.depth = f.depth - 1;
+        return false;
+    }
+next:
+    if (sp == 0) return true;
+    f = st[--sp];
+    return false;
+}
+
+// Returns true when a strictly nearer hit on this segment was found (best, out updated).
+__device__ bool segmentHit(const DSeg& S, const TRay& r, V3 fx, V3 fy, float& best, float& tSeg, V3& tang,
+                           int depth) {
+    SubFrame st[7];
+    int sp = 0;
+    SubFrame f = rootFrame(S, depth);
     bool hit = false;
     while (true) {
-        {   // one frame; every exit to `next` pops the next pending half
-            V3 c0 = projNC(f.P[0], r.o, fx, fy, r.d), c1 = projNC(f.P[1], r.o, fx, fy, r.d);
-            V3 c2 = projNC(f.P[2], r.o, fx, fy, r.d), c3 = projNC(f.P[3], r.o, fx, fy, r.d);
-            float mh = fmaxf(f.hw0, f.hw3);
-            float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z), zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
-            if (zMin - mh > best) goto next;
-            if (zMax + mh < r.tMin) goto next;
-            float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
-            float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
-            float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
-            float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
-            if (cx * cx + cy * cy > mh * mh) goto next;
-            if (f.depth == 0) {
-                float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
-                float l2 = dx * dx + dy * dy;
-                float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
-                float px = ax + dx * tl, py = ay + dy * tl;
-                float d2 = px * px + py * py;
-                float hw = f.hw0 * (1.f - tl) + f.hw3 * tl;
-                if (d2 > hw * hw) goto next;
-                float tr = c0.z + (c3.z - c0.z) * tl;
-                if (tr < r.tMin || tr >= best) goto next;
-                // bezierTangent (CurveBrute.cpp:59-64)
-                float mt = 1.f - tl;
-                float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
-                V3 d10{f.P[1].x - f.P[0].x, f.P[1].y - f.P[0].y, f.P[1].z - f.P[0].z};
-                V3 d21{f.P[2].x - f.P[1].x, f.P[2].y - f.P[1].y, f.P[2].z - f.P[1].z};
-                V3 d32{f.P[3].x - f.P[2].x, f.P[3].y - f.P[2].y, f.P[3].z - f.P[2].z};
-                V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
-                      d10.z * a0 + d21.z * a1 + d32.z * a2};
-                if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
-                    tg = V3{f.P[3].x - f.P[0].x, f.P[3].y - f.P[0].y, f.P[3].z - f.P[0].z};
-                float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
-                if (l > 1e-8f) {
-                    float il = 1.f / l;
-                    tg = V3{tg.x * il, tg.y * il, tg.z * il};
-                } else {
-                    tg = V3{0.f, 0.f, 1.f};
-                }
-                best = tr;
-                tSeg = f.tOff + tl * f.tScale;
-                tang = tg;
-                hit = true;
-                goto next;
-            }
-            V3 q01 = midNC(f.P[0], f.P[1]), q12 = midNC(f.P[1], f.P[2]), q23 = midNC(f.P[2], f.P[3]);
-            V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
-            float hm = (f.hw0 + f.hw3) * 0.5f, ts = f.tScale * 0.5f;
-            SubFrame R;
-            R.P[0] = m; R.P[1] = r12; R.P[2] = q23; R.P[3] = f.P[3];
-            R.hw0 = hm; R.hw3 = f.hw3; R.tOff = f.tOff + ts; R.tScale = ts; R.depth = f.depth - 1;
-            SubFrame L;
-            L.P[0] = f.P[0]; L.P[1] = q01; L.P[2] = r01; L.P[3] = m;
-            L.hw0 = f.hw0; L.hw3 = hm; L.tOff = f.tOff; L.tScale = ts; L.depth = f.depth - 1;
-            st[sp++] = R;
-            f = L;
-            continue;
+        bool acc;
+        float t, ts;
+        V3 tg;
+        const bool done = segmentStep(f, st, sp, r.o, r.d, r.tMin, fx, fy, best, acc, t, ts, tg);
+        if (acc) {
+            best = t;
+            tSeg = ts;
+            tang = tg;
+            hit = true;
         }
-    next:
-        if (sp == 0) break;
-        f = st[--sp];
+        if (done) break;
     }
     return hit;
 }
@@ -490,6 +515,30 @@ __device__ __forceinline__ bool curveBox(const CNode& n, const TRay& r, const V3
         tmax = stdmin(tmax, t1);
         if (tmax < tmin) return false;
     }
+    return true;
+}
+
+// curveBox that also gives the entry distance (the larger of tMin and the near slab planes), by
+// which a walk orders two children.
+__device__ __forceinline__ bool curveBoxT(const CNode& n, V3 o, V3 d, const V3& invD, float tMin, float maxT,
+                                          float& entry) {
+#pragma clang fp contract(off)
+    float tmin = tMin, tmax = maxT;
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, id[3] = {invD.x, invD.y, invD.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (fabsf(dd[i]) < 1e-9f) {
+            if (oo[i] < n.bmin[i] || oo[i] > n.bmax[i]) return false;
+            continue;
+        }
+        float inv = id[i];
+        float t0 = (n.bmin[i] - oo[i]) * inv, t1 = (n.bmax[i] - oo[i]) * inv;
+        if (inv < 0.f) { float s = t0; t0 = t1; t1 = s; }
+        tmin = stdmax(tmin, t0);
+        tmax = stdmin(tmax, t1);
+        if (tmax < tmin) return false;
+    }
+    entry = tmin;
     return true;
 }
 

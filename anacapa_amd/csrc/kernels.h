@@ -66,17 +66,25 @@ struct WaveQueues {
     uint32_t* passIdx;   // index of the pass being run; advanced by the pass's last kernel
     uint32_t stackCap;   // total stack entries available per thread
     uint32_t cap;        // path capacity
-    // curve scenes (null otherwise): the resumable curve-tree walk of the closest-hit query
-    uint32_t* cList;     // [cap][kCurveList] records listed for the next wave test, a row per ray
-    uint32_t* cState;    // [cap] listed count | saved stack depth << 8
-    uint32_t* cStack;    // [cap][cStackCap] saved walk stack (node indices), a row per ray
-    uint32_t cStackCap;
+    // curve scenes (null otherwise): the pooled curve queries
+    uint32_t* cState;    // [cap] 1 when the ray's curve walk is to run (after the triangle part)
+    uint32_t* cwRef;     // curve walk stack entries past the LDS part, [(cwCap - kPoolStackK) * threads]
+    float* cwT;
+    uint32_t cwCap;      // curve walk stack entries per lane
+    uint32_t poolBlocks; // grid of k_curve_pool (one wave per block)
 };
 
-// Curve records a walk lists before a wave tests them; walk-and-test rounds before the rest of the
-// walk runs per ray.
-constexpr uint32_t kCurveList = 64;
-constexpr uint32_t kCurveRounds = 8;
+// The curve queries' ray interval starts here (CurveBrute.cpp:398-440 tests from the ray's tMin,
+// which every queued ray sets to 1e-4).
+constexpr float kCurveTMin = 1e-4f;
+// k_curve_pool: records waiting per wave (a power of two); the walkers step while fewer than
+// kPoolLow wait, so the pool never holds more than kPoolLow + 64 * 7 (a leaf holds at most 7)
+constexpr uint32_t kPoolCap = 512;
+constexpr uint32_t kPoolLow = 64;
+constexpr uint32_t kPoolSlotShift = 26;          // pool entry: slot << 26 | record (records < 2^26)
+constexpr uint32_t kPoolIterCap = 1u << 24;      // steps of one wave before it gives up (flagged)
+constexpr uint32_t kPoolStackK = 8;              // walk stack entries per lane in LDS
+static_assert(kPoolLow + 64u * 7u <= kPoolCap, "the curve pool must hold a full walk step");
 
 // Detached path-replay estimator (adjoint.hip): one thread per (pixel, sample) over the film.
 constexpr uint32_t kRecFields = 18;   // floats per differentiable vertex record

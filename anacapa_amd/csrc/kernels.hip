@@ -398,24 +398,31 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
 }
 
 // --------------------------------------------------------------------------- queries, curve scenes
-// Through dense hair almost all of a query is spent in the segment tests (the subdivision of
-// CurveBrute::subdivide), and one ray's tests can outnumber its neighbours' many times over, so a
-// wave walking 64 rays keeps few lanes busy.  In curve scenes the three queries of a bounce (the
-// paths' closest hit, the shadow rays' any hit, the BSDF-sampled rays' closest or any hit) run their
-// curve part in rounds: k_curve_walk walks a ray's curve tree (after its triangle tree, on the first
-// round) with the node test bounded by the nearest hit so far and lists up to kCurveList leaf
-// records in traversal order, saving its stack; k_curve_test spreads one ray's list over a wave,
-// one record per lane.  After kCurveRounds rounds k_curve_finish gives each unfinished ray a wave of
-// its own that repeats list-and-test until its walk ends.  Results are left in q.hit for the
-// queue's own resolve kernel.
-// The result is the reference's (CurveBrute.cpp:398-440).  Any hit: a record's test under tMax
-// does not depend on the order of tests, and the walk culls with tMax only until the first hit.
-// Closest hit: the reference's walk visits records in increasing record order (the builder lays
-// leaves out depth-first, left before right) and keeps a hit only if strictly nearer, so it returns
-// the least (t, record) over all accepted chords; its box and hull culls are conservative (a culled
-// box or sub-curve holds no chord nearer than the bound it was culled with), so testing a list
-// under the bound of its round, larger than the running one, finds the same records at the same t,
-// and the wave keeps the least (t, record).
+// Through dense hair a ray's curve-tree walk reaches a hundred or more segment records whose boxes
+// it crosses before its hit, most of them rejected by the first ray-space hull test and a few
+// subdivided ten or more frames deep, so one ray per lane leaves most lanes idle.  In curve scenes
+// the three queries of a bounce (the paths' closest hit, the shadow rays' any hit, the
+// BSDF-sampled rays' closest or any hit) run in two kernels:
+//   k_curve_tri  — the triangle tree, one ray per lane, as CurveBrute::trace does first; the hit
+//                  (or tMax) bounds the curve part, and rays whose query is settled are marked;
+//   k_curve_pool — one wave holds up to 64 rays, one per lane (the ray's "slot").  Each lane
+//                  walks its ray's curve tree, nearer child first, and puts the records of the
+//                  leaves it reaches into the wave's pool; every lane takes records from the pool,
+//                  whichever ray they belong to, and advances one subdivision frame per step, taking
+//                  the next record when its frame walk ends.  A ray's best hit so far is a 64-bit
+//                  key (t, record) in LDS that every frame test and node test is bounded by; when a
+//                  ray's walk is over and none of its records is pending, its lane writes the result
+//                  and takes the next ray.
+// The result is the reference's (CurveBrute.cpp:398-440).  Closest hit: the reference's walk
+// visits records in increasing record order (the builder lays leaves out depth-first, left before
+// right) and keeps a hit only if strictly nearer, so it returns the least (t, record) over the
+// accepted chords, and within that record the first chord (in its recursion order) at that t.
+// Box, hull and pop culls are conservative (a culled box or sub-curve holds no chord nearer than
+// the bound it was culled with), so testing records in another order under any bound no smaller
+// than the final one finds the same least key: a record below the key's record is tested against
+// the next float above the key's t (an equal t still wins), any other record strictly below it.
+// Any hit: a record's test under tMax does not depend on the order of tests, and no cull uses
+// anything but tMax before the first hit.
 enum : uint32_t { kQPath = 0u, kQShadow = 1u, kQMis = 2u };   // counter slot of each queue
 
 template <uint32_t Q>
@@ -438,22 +445,15 @@ __device__ __forceinline__ TRay curveQueryRay(const WaveQueues& q, uint32_t cur,
         tMax = b.w;
         dead = false;
     }
-    return makeTRay(v3(a), v3(b), 1e-4f, tMax);
+    return makeTRay(v3(a), v3(b), kCurveTMin, tMax);
 }
 __device__ __forceinline__ float hitBound(const float4& h, float tMax) {
     return __float_as_uint(h.w) != ~0u ? h.x : tMax;
 }
-__device__ __forceinline__ uint64_t waveMin(uint64_t m) {
-    for (int off = 32; off >= 1; off >>= 1) {
-        const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
-        const uint64_t o = ((uint64_t)hi << 32) | lo;
-        m = o < m ? o : m;
-    }
-    return m;
-}
 
-template <uint32_t BS, uint32_t K, uint32_t Q, bool ANY, bool FIRST>
-__global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+// The triangle part of the three queries; q.cState[i] = 1 when the ray's curve walk is to run.
+template <uint32_t BS, uint32_t K, uint32_t Q, bool ANY>
+__global__ __launch_bounds__(BS) void k_curve_tri(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
     __shared__ uint64_t lstk[K * BS];
     TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
              q.stackCap};
@@ -464,207 +464,212 @@ __global__ __launch_bounds__(BS) void k_curve_walk(DScene sc, WaveQueues q, uint
         if (i >= count) break;
         bool dead;
         const TRay r = curveQueryRay<Q>(q, cur, i, dead);
-        uint32_t sp;
-        float bound;
-        if (FIRST) {
-            if (dead) {
-                q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(~0u));
-                q.cState[i] = 0u;
-                continue;
-            }
-            float t = 0.f, u = 0.f, v = 0.f;
-            const uint32_t h = traverse<ANY, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
-            q.hit[i] = make_float4(t, u, v, __uint_as_float(h));
-            if (ANY && h != ~0u) {   // occluded by a triangle: no curve walk
-                q.cState[i] = 0u;
-                continue;
-            }
-            bound = h != ~0u ? t : r.tMax;
-            q.cStack[(size_t)i * q.cStackCap] = 0u;   // the curve root
-            sp = 1u;
-        } else {
-            sp = q.cState[i] >> 8;
-            if (sp == 0u) {
-                q.cState[i] = 0u;
-                continue;
-            }
-            bound = hitBound(q.hit[i], r.tMax);
+        if (dead) {
+            q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(~0u));
+            q.cState[i] = 0u;
+            continue;
         }
-        const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
-        uint32_t n = 0u;
-        while (sp > 0u) {
-            const CNode nd = loadCNode(sc.cnodes, q.cStack[(size_t)i * q.cStackCap + sp - 1u]);
-            const bool leaf = (nd.rc & 0x80000000u) != 0u;
-            if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveList) break;   // the list is full: resume here
-            --sp;
-            if (!curveBox(nd, r, invD, bound)) continue;
-            if (leaf) {
-                const uint32_t cnt = nd.rc & 0x7fffffffu;
-                for (uint32_t k = 0; k < cnt; ++k, ++n) q.cList[(size_t)i * kCurveList + n] = nd.lp + k;
-            } else {
-                if (sp + 2u > q.cStackCap) {
-                    *ctr(q, bounce, 3) = 1u;
-                    sp = 0u;
-                    break;
-                }
-                q.cStack[(size_t)i * q.cStackCap + sp] = nd.rc;
-                q.cStack[(size_t)i * q.cStackCap + sp + 1u] = nd.lp;
-                sp += 2u;
-            }
-        }
-        q.cState[i] = n | (sp << 8);
+        float t = 0.f, u = 0.f, v = 0.f;
+        const uint32_t h = traverse<ANY, false>(sc, r, S, t, u, v, tc, ctr(q, bounce, 3));
+        q.hit[i] = make_float4(t, u, v, __uint_as_float(h));
+        q.cState[i] = (ANY && h != ~0u) ? 0u : 1u;   // occluded by a triangle: no curve walk
     }
 }
 
-// Lanes in groups of kCurveGroup, one ray per group: the group tests the ray's list kCurveGroup
-// records at a time in list order, each chunk bounded by the nearest hit of the chunks before (a
-// later record with an equal t never wins, as in the reference's walk).
-constexpr uint32_t kCurveGroup = 16;
-__device__ __forceinline__ uint64_t groupMin(uint64_t m) {
-    for (int off = kCurveGroup / 2; off >= 1; off >>= 1) {   // xor below the group size stays in the group
-        const uint32_t lo = __shfl_xor((uint32_t)m, off), hi = __shfl_xor((uint32_t)(m >> 32), off);
-        const uint64_t o = ((uint64_t)hi << 32) | lo;
-        m = o < m ? o : m;
-    }
-    return m;
+__device__ __forceinline__ uint32_t laneRank(uint64_t mask) {   // set bits of `mask` below this lane
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+template <typename T>
+__device__ __forceinline__ T ldsLoad(T* p) {
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
 }
 
-template <uint32_t BS, uint32_t Q, bool ANY>
-__global__ __launch_bounds__(BS) void k_curve_test(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+// One wave per block (the pool, the slots and the keys are the wave's own, so no barrier is
+// needed: a wave's LDS accesses take effect in program order).  K: LDS entries of each lane's walk
+// stack, deeper entries in q.cwRef / q.cwT.
+template <uint32_t K, uint32_t Q, bool ANY>
+__global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
+    __shared__ uint64_t lstk[K * 64];
+    __shared__ float4 sO[64], sD[64];          // slot rays
+    __shared__ float4 sRes[64];                // tangent, strand parameter of the slot's key
+    __shared__ unsigned long long sKey[64];    // t bits << 32 | record; any hit: 0 once occluded
+    __shared__ uint32_t sOut[64];              // the slot's records in the pool or being tested
+    __shared__ uint32_t sPool[kPoolCap];       // slot << kPoolSlotShift | record
+    const uint32_t lane = threadIdx.x;
+    const TStack S{lstk, q.cwRef, q.cwT, lane, 64u, K, blockIdx.x * 64u + lane, gridDim.x * 64u, q.cwCap};
     const uint32_t count = *ctr(q, bounce, Q);
-    const uint32_t lane = __lane_id(), grp = lane / kCurveGroup, gl = lane % kCurveGroup;
-    const uint32_t waves = gridDim.x * (BS / 64u);
-    // wave w takes rays w, w + waves, w + 2 waves ... (neighbouring rays cost alike, so they are
-    // spread over the waves), reading the states of 64 of them at once
-    const uint32_t w = blockIdx.x * (BS / 64u) + threadIdx.x / 64u;
-    for (uint32_t k0 = 0; w + waves * k0 < count; k0 += 64u) {
-        const uint32_t mine = w + waves * (k0 + lane);
-        uint64_t todo = __ballot(mine < count && (q.cState[mine] & 0xffu) != 0u);
-        while (todo) {
-            // the g-th ray still to do goes to group g
-            uint32_t i = ~0u;
-            for (uint32_t g = 0; g < 64u / kCurveGroup && todo; ++g) {
-                if (g == grp) i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
-                todo &= todo - 1ull;
-            }
-            if (i == ~0u) continue;   // (the loop test reads the wave's `todo`, alike in every lane)
-            const uint32_t n = q.cState[i] & 0xffu;
-            bool dead;
-            const TRay r = curveQueryRay<Q>(q, cur, i, dead);
-            V3 fx, fy;
-            onbNC(r.d, fx, fy);
-            float bnd = hitBound(q.hit[i], r.tMax);
-            uint64_t key = ~0ull, m = ~0ull;
-            float tSeg = 0.f;
-            V3 tang{0.f, 0.f, 0.f};
-            for (uint32_t c = 0; c < n; c += kCurveGroup) {   // alike in the group
-                if (c + gl < n) {
-                    const uint32_t rec = q.cList[(size_t)i * kCurveList + c + gl];
-                    float best = bnd, ts;
-                    V3 tg;
-                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg, (int)sc.curveDepth)) {
-                        key = ((uint64_t)__float_as_uint(best) << 32) | rec;   // t > 0: bits order as floats
-                        tSeg = ts;
-                        tang = tg;
-                    }
-                }
-                m = groupMin(key);
-                if (m != ~0ull) {
-                    if (ANY) break;
-                    bnd = __uint_as_float((uint32_t)(m >> 32));
-                }
-            }
-            if (m == ~0ull) continue;
-            if (ANY) {
-                if (gl == 0u) {
-                    q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
-                    q.cState[i] = 0u;   // occluded: the walk is over
-                }
-            } else if (key == m) {   // records are listed once: one lane holds the least key
-                q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
-                                       __uint_as_float(kCurveHit | (uint32_t)m));
-                q.hitAux[i] = f4(tang, 0.f);
-            }
-        }
-    }
-}
+    uint32_t* const overflow = ctr(q, bounce, 3);
+    const float inf = __int_as_float(0x7f800000);
+    const int depth = (int)sc.curveDepth;
 
-// The unfinished rays, one per lane group: the walk runs in every lane of the group alike (same
-// loads, same branches), lists up to kCurveGroup records and lane k of the group tests the k-th.
-template <uint32_t BS, uint32_t Q, bool ANY>
-__global__ __launch_bounds__(BS) void k_curve_finish(DScene sc, WaveQueues q, uint32_t bounce, uint32_t cur) {
-    const uint32_t count = *ctr(q, bounce, Q);
-    const uint32_t lane = __lane_id(), grp = lane / kCurveGroup, gl = lane % kCurveGroup;
-    const uint32_t waves = gridDim.x * (BS / 64u);
-    const uint32_t w = blockIdx.x * (BS / 64u) + threadIdx.x / 64u;
-    for (uint32_t k0 = 0; w + waves * k0 < count; k0 += 64u) {
-        const uint32_t mine = w + waves * (k0 + lane);
-        uint64_t todo = __ballot(mine < count && (q.cState[mine] >> 8) != 0u);
-        while (todo) {
-            uint32_t i = ~0u;
-            for (uint32_t g = 0; g < 64u / kCurveGroup && todo; ++g) {
-                if (g == grp) i = w + waves * (k0 + (uint32_t)__builtin_ctzll(todo));
-                todo &= todo - 1ull;
+    // this lane's slot
+    uint32_t ray = ~0u, wsp = 0u;
+    bool walking = false;
+    V3 wo{0.f, 0.f, 0.f}, wd{0.f, 0.f, 1.f}, winv{0.f, 0.f, 0.f};
+    unsigned long long initKey = 0ull;
+    // the record this lane tests
+    bool busy = false;
+    uint32_t tSlot = 0u, tRec = 0u;
+    V3 to{0.f, 0.f, 0.f}, td{0.f, 0.f, 1.f}, fx{0.f, 0.f, 0.f}, fy{0.f, 0.f, 0.f};
+    SubFrame f;
+    SubFrame st[7];
+    int sp = 0;
+    float own = inf;
+    // wave-uniform: rays blockIdx.x + gridDim.x * k for k < nextK have been taken; the pool holds
+    // `avail` records from `head` on
+    uint32_t nextK = 0u, head = 0u, avail = 0u;
+
+    for (uint32_t it = 0;; ++it) {
+        // 1. a slot whose walk is over and whose records are all tested hands back its result
+        if (ray != ~0u && !walking && ldsLoad(&sOut[lane]) == 0u) {
+            const unsigned long long key = ldsLoad(&sKey[lane]);
+            if (key != initKey) {
+                if (ANY) {
+                    q.hit[ray] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
+                } else {
+                    const float4 res = sRes[lane];
+                    q.hit[ray] = make_float4(__uint_as_float((uint32_t)(key >> 32)), res.w, 0.f,
+                                             __uint_as_float(kCurveHit | (uint32_t)key));
+                    q.hitAux[ray] = make_float4(res.x, res.y, res.z, 0.f);
+                }
             }
-            if (i == ~0u) continue;
-            uint32_t sp = q.cState[i] >> 8;
-            bool dead;
-            const TRay r = curveQueryRay<Q>(q, cur, i, dead);
-            float bound = hitBound(q.hit[i], r.tMax);
-            V3 fx, fy;
-            onbNC(r.d, fx, fy);
-            const V3 invD{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};
-            uint64_t key = ~0ull, m = ~0ull;
-            float tSeg = 0.f;
-            V3 tang{0.f, 0.f, 0.f};
-            uint32_t* const stk = q.cStack + (size_t)i * q.cStackCap;
-            while (sp > 0u) {
-                uint32_t n = 0u, rec = ~0u;
-                while (sp > 0u) {
-                    const CNode nd = loadCNode(sc.cnodes, stk[sp - 1u]);
-                    const bool leaf = (nd.rc & 0x80000000u) != 0u;
-                    if (leaf && n + (nd.rc & 0x7fffffffu) > kCurveGroup) break;   // a leaf holds at most 4
-                    --sp;
-                    if (!curveBox(nd, r, invD, bound)) continue;
-                    if (leaf) {
-                        const uint32_t cnt = nd.rc & 0x7fffffffu;
-                        if (gl - n < cnt) rec = nd.lp + (gl - n);   // gl in [n, n + cnt)
-                        n += cnt;
-                    } else {
-                        if (sp + 2u > q.cStackCap) {
-                            *ctr(q, bounce, 3) = 1u;
-                            sp = 0u;
-                            break;
+            ray = ~0u;
+        }
+        // empty slots take the next rays whose curve walk is to run
+        while (true) {
+            const uint64_t want = __ballot(ray == ~0u);
+            const uint64_t first = (uint64_t)blockIdx.x + (uint64_t)gridDim.x * nextK;
+            if (want == 0ull || first >= count) break;
+            if (ray == ~0u) {
+                const uint64_t i = first + (uint64_t)gridDim.x * laneRank(want);
+                if (i < count && q.cState[i] != 0u) {
+                    bool dead;
+                    const TRay r = curveQueryRay<Q>(q, cur, (uint32_t)i, dead);
+                    const float b0 = hitBound(q.hit[i], r.tMax);
+                    ray = (uint32_t)i;
+                    initKey = (unsigned long long)__float_as_uint(b0) << 32;
+                    sO[lane] = f4(r.o, 0.f);
+                    sD[lane] = f4(r.d, 0.f);
+                    sRes[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    sKey[lane] = initKey;
+                    sOut[lane] = 0u;
+                    wo = r.o;
+                    wd = r.d;
+                    winv = V3{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
+                    wsp = 0u;
+                    float te;
+                    if (curveBoxT(loadCNode(sc.cnodes, 0u), wo, wd, winv, kCurveTMin, b0, te)) spush(S, wsp, 0u, te);
+                    walking = wsp > 0u;
+                }
+            }
+            nextK += (uint32_t)__popcll(want);
+        }
+        // 2. when the pool runs low, every walking lane takes one step of its walk
+        if (avail < kPoolLow) {
+            uint32_t n = 0u, lp = 0u;
+            if (walking) {
+                const unsigned long long key = ldsLoad(&sKey[lane]);
+                if (ANY && key != initKey) {   // occluded: the walk is over
+                    walking = false;
+                } else {
+                    const float bound = __uint_as_float((uint32_t)(key >> 32));
+                    uint32_t node;
+                    float te;
+                    spop(S, wsp, node, te);
+                    if (!(te > bound)) {
+                        const CNode nd = loadCNode(sc.cnodes, node);
+                        if (nd.rc & 0x80000000u) {
+                            n = nd.rc & 0x7fffffffu;
+                            lp = nd.lp;
+                        } else if (wsp + 2u > S.cap) {
+                            *overflow = 1u;
+                            wsp = 0u;
+                        } else {
+                            const CNode a = loadCNode(sc.cnodes, nd.lp), b = loadCNode(sc.cnodes, nd.rc);
+                            float ta = 0.f, tb = 0.f;
+                            const bool ha = curveBoxT(a, wo, wd, winv, kCurveTMin, bound, ta);
+                            const bool hb = curveBoxT(b, wo, wd, winv, kCurveTMin, bound, tb);
+                            if (ha && hb) {   // the nearer child is taken next (the left one on a tie)
+                                if (tb < ta) {
+                                    spush(S, wsp, nd.lp, ta);
+                                    spush(S, wsp, nd.rc, tb);
+                                } else {
+                                    spush(S, wsp, nd.rc, tb);
+                                    spush(S, wsp, nd.lp, ta);
+                                }
+                            } else if (ha) {
+                                spush(S, wsp, nd.lp, ta);
+                            } else if (hb) {
+                                spush(S, wsp, nd.rc, tb);
+                            }
                         }
-                        stk[sp] = nd.rc;   // every lane of the group stores the same entry
-                        stk[sp + 1u] = nd.lp;
-                        sp += 2u;
                     }
-                }
-                if (n == 0u) continue;
-                if (rec != ~0u) {
-                    float best = bound, ts;
-                    V3 tg;
-                    if (segmentHit(sc.segs[rec], r, fx, fy, best, ts, tg, (int)sc.curveDepth)) {
-                        key = ((uint64_t)__float_as_uint(best) << 32) | rec;
-                        tSeg = ts;
-                        tang = tg;
-                    }
-                }
-                m = groupMin(key);
-                if (m != ~0ull) {
-                    if (ANY) break;
-                    bound = __uint_as_float((uint32_t)(m >> 32));
+                    walking = wsp > 0u;
                 }
             }
-            if (m == ~0ull) continue;
-            if (ANY) {
-                if (gl == 0u) q.hit[i] = make_float4(0.f, 0.f, 0.f, __uint_as_float(kCurveHit));
-            } else if (key == m) {
-                q.hit[i] = make_float4(__uint_as_float((uint32_t)(m >> 32)), tSeg, 0.f,
-                                       __uint_as_float(kCurveHit | (uint32_t)m));
-                q.hitAux[i] = f4(tang, 0.f);
+            // the leaf's records go to the pool at a wave prefix of the counts (at most 7 each)
+            const uint64_t b0 = __ballot((n & 1u) != 0u), b1 = __ballot((n & 2u) != 0u), b2 = __ballot((n & 4u) != 0u);
+            const uint32_t pre = laneRank(b0) + 2u * laneRank(b1) + 4u * laneRank(b2);
+            const uint32_t tot = (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+            for (uint32_t k = 0; k < n; ++k)
+                sPool[(head + avail + pre + k) & (kPoolCap - 1u)] = (lane << kPoolSlotShift) | (lp + k);
+            if (n) atomicAdd(&sOut[lane], n);
+            avail += tot;
+        }
+        // 3. lanes without a record take the next ones of the pool
+        {
+            const uint64_t idle = __ballot(!busy);
+            const uint32_t take = min((uint32_t)__popcll(idle), avail);
+            if (!busy) {
+                const uint32_t rank = laneRank(idle);
+                if (rank < take) {
+                    const uint32_t item = ldsLoad(&sPool[(head + rank) & (kPoolCap - 1u)]);
+                    tSlot = item >> kPoolSlotShift;
+                    tRec = item & ((1u << kPoolSlotShift) - 1u);
+                    to = v3(sO[tSlot]);
+                    td = v3(sD[tSlot]);
+                    onbNC(td, fx, fy);
+                    f = rootFrame(sc.segs[tRec], depth);
+                    sp = 0;
+                    own = inf;
+                    busy = true;
+                }
             }
+            head += take;
+            avail -= take;
+        }
+        // 4. one subdivision frame per busy lane
+        if (busy) {
+            const unsigned long long key = ldsLoad(&sKey[tSlot]);
+            bool stop;
+            if (ANY && key == 0ull) {
+                stop = true;
+            } else {
+                const float kt = __uint_as_float((uint32_t)(key >> 32));
+                const float thr = (tRec < (uint32_t)key && kt < inf) ? __uint_as_float(__float_as_uint(kt) + 1u) : kt;
+                bool acc;
+                float t, ts;
+                V3 tg;
+                stop = segmentStep(f, st, sp, to, td, kCurveTMin, fx, fy, fminf(own, thr), acc, t, ts, tg);
+                if (acc) {
+                    own = t;
+                    const unsigned long long mine = ANY ? 0ull : (((unsigned long long)__float_as_uint(t) << 32) | tRec);
+                    atomicMin(&sKey[tSlot], mine);
+                    // the lanes of this step that hold the slot's least key write its tangent
+                    if (!ANY && ldsLoad(&sKey[tSlot]) == mine) sRes[tSlot] = f4(tg, ts);
+                    if (ANY) stop = true;
+                }
+            }
+            if (stop) {
+                busy = false;
+                atomicSub(&sOut[tSlot], 1u);
+            }
+        }
+        const bool rest = (uint64_t)blockIdx.x + (uint64_t)gridDim.x * nextK < count;
+        if (__ballot(ray != ~0u || busy) == 0ull && avail == 0u && !rest) break;
+        if (it >= kPoolIterCap) {   // never reached by a scene the tree and the pool can hold
+            *overflow = 1u;
+            break;
         }
     }
 }
@@ -1092,19 +1097,14 @@ void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks
 template <uint32_t Q, bool ANY>
 void launchCurveQuery(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_curve_walk<kTraceBS, kStackK, Q, ANY, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
-    hipLaunchKernelGGL((k_curve_test<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
-    for (uint32_t k = 1; k < kCurveRounds; ++k) {
-        hipLaunchKernelGGL((k_curve_walk<kTraceBS, kStackK, Q, ANY, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
-        hipLaunchKernelGGL((k_curve_test<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
-    }
-    hipLaunchKernelGGL((k_curve_finish<kTraceBS, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    hipLaunchKernelGGL((k_curve_tri<kTraceBS, kStackK, Q, ANY>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce, cur);
+    hipLaunchKernelGGL((k_curve_pool<kPoolStackK, Q, ANY>), dim3(q.poolBlocks), dim3(64), 0, s, sc, q, bounce, cur);
 }
-// The round-based curve queries need their buffers (q.cList); AHIP_DEBUG_WHOLE_RAY=1 selects the
+// The pooled curve queries need their buffers (q.cState); AHIP_DEBUG_WHOLE_RAY=1 selects the
 // whole-ray kernels instead, for comparisons.
 bool curveRounds(const DScene& sc, const WaveQueues& q) {
     const char* dbg = getenv("AHIP_DEBUG_WHOLE_RAY");
-    return sc.hasCurves && q.cList && !(dbg && dbg[0] == '1');
+    return sc.hasCurves && q.cState && !(dbg && dbg[0] == '1');
 }
 
 void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t cur, uint32_t blocks,

@@ -108,12 +108,13 @@ def test_converted_strands_with_per_cv_widths_vs_oracle(gpu, oracle):
 
 
 @pytest.mark.parametrize("emitting_quad", [True, False])
-def test_round_queries_equal_whole_ray_queries(gpu, monkeypatch, emitting_quad):
-    """In curve scenes the closest-hit, shadow and BSDF-sampled queries run in list-and-test rounds
-    (a wave tests one ray's listed segments, the rest of a long walk finishes one ray per wave);
+def test_pooled_queries_equal_whole_ray_queries(gpu, monkeypatch, emitting_quad):
+    """In curve scenes the closest-hit, shadow and BSDF-sampled queries run pooled (k_curve_pool:
+    a wave walks 64 rays' curve trees nearer child first and its lanes test any ray's segment
+    records one subdivision frame at a time, bounded by each ray's best (t, record) key);
     AHIP_DEBUG_WHOLE_RAY=1 selects the one-ray-per-lane kernels the oracle tests above also pin.
-    Both must give the same film bit for bit: 20k strands, so that many rays outlast the rounds.
-    Without an emitting surface the BSDF-sampled query is an any-hit query."""
+    Both must give the same film and AOVs bit for bit (20k strands).  Without an emitting surface
+    the BSDF-sampled query is an any-hit query."""
     from anacapa_amd import _abi as A
     sc = scenes.hair_sphere(width=96, height=96, spp=4, strands=20_000, hdri_size=(128, 64))
     if not emitting_quad:
