@@ -576,9 +576,10 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     // ones keep the CUs evenly loaded to the end of a launch (measured on the 1080p frame:
     // traversal 18.6 ms at 16 blocks per CU, 16.1 ms at 128; shade 5.9 ms at 8, 5.45 ms at 32).
     ctx->traceBlocks = std::max(1u, ctx->numCU) * 128u;
-    // the pooled curve queries: 16 one-wave blocks per CU, the waves that stay resident at the
-    // kernel's register use (each loops over its share of the rays)
-    ctx->poolBlocks = std::max(1u, ctx->numCU) * 16u;
+    // the pooled curve queries: exactly the one-wave blocks that stay resident together (each
+    // loops over its share of the rays, so a block queued behind them would run alone at the end;
+    // measured on the hair frame at 8 spp: 773 ms at the 6 resident blocks per CU, 1181 ms at 7)
+    ctx->poolBlocks = std::max(1u, ctx->numCU) * curve_pool_blocks_per_cu();
     ctx->shadeBlocks = std::max(1u, ctx->numCU) * 32u;
     HaltonTables T;
     fillHaltonTables(T);
@@ -637,7 +638,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.strandInfo = ctx->dStrandInfo.as<float>();
     S.hasCurves = ctx->curves.nodes.empty() ? 0u : 1u;
     S.hasHair = hs.hasHair ? 1u : 0u;
-    S.curveDepth = ctx->params.curve_depth ? ctx->params.curve_depth : 6u;
+    S.curveDepth = ctx->params.curve_depth ? ctx->params.curve_depth : kMaxCurveDepth;
     if ((st = upload(ctx, ctx->dNodes, ctx->bvh.nodes4.data(), ctx->bvh.nodes4.size()))) return st;
     for (DTri& t : ctx->bvh.tris)   // the material of each triangle, so shading skips the mesh table
         t.matIndex = t.meshID < hs.meshMat.size() ? hs.meshMat[t.meshID] : -1;
@@ -744,9 +745,9 @@ int ahip_set_camera(ahip_context* ctx, const ahip_camera* cam) {
 int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
     ENTER();
     if (!p || p->max_depth > 1000u) return fail(ctx, AHIP_E_INVALID_ARG, "bad render params");
-    if (p->curve_depth > 6u) return fail(ctx, AHIP_E_INVALID_ARG, "curve_depth must be 0..6");
+    if (p->curve_depth > kMaxCurveDepth) return fail(ctx, AHIP_E_INVALID_ARG, "curve_depth must be 0..6");
     ctx->params = *p;
-    ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : 6u;
+    ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : kMaxCurveDepth;
     return AHIP_OK;
 }
 

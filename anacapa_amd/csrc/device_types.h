@@ -83,6 +83,7 @@ struct alignas(64) DSeg {
 static_assert(sizeof(DSeg) == 64, "DSeg must be 64 bytes");
 
 constexpr uint32_t kCurveHit = 0x80000000u;   // hit code flag: low bits = segment record
+constexpr uint32_t kMaxCurveDepth = 6;        // ribbon subdivision levels (CurveBrute.cpp:194 fixes 6)
 
 struct DMat {
     uint32_t type;            // AHIP_MAT_*

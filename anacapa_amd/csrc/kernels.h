@@ -83,7 +83,7 @@ constexpr uint32_t kPoolCap = 512;
 constexpr uint32_t kPoolLow = 64;
 constexpr uint32_t kPoolSlotShift = 26;          // pool entry: slot << 26 | record (records < 2^26)
 constexpr uint32_t kPoolIterCap = 1u << 24;      // steps of one wave before it gives up (flagged)
-constexpr uint32_t kPoolStackK = 8;              // walk stack entries per lane in LDS
+constexpr uint32_t kPoolStackK = 2;              // walk stack entries per lane in LDS
 static_assert(kPoolLow + 64u * 7u <= kPoolCap, "the curve pool must hold a full walk step");
 
 // Detached path-replay estimator (adjoint.hip): one thread per (pixel, sample) over the film.
@@ -118,6 +118,8 @@ void launchZeroF64(double* p, size_t n, hipStream_t s);
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
 uint32_t trace_block_size();
+// Blocks of k_curve_pool that stay resident on one CU together (its LDS use bounds them).
+uint32_t curve_pool_blocks_per_cu();
 void launchFillU32(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
 void launchMaterialProbe(const DMat& m, bool marschner, uint32_t n, const float* in, float* out, hipStream_t s);
 void launchFillF32(float* p, size_t n, float v, hipStream_t s);

@@ -484,6 +484,155 @@ __device__ __forceinline__ T ldsLoad(T* p) {
     return __atomic_load_n(p, __ATOMIC_RELAXED);
 }
 
+// The subdivision of a pooled record without a private stack.  Pending right halves wait in LDS,
+// one slot per level (the recursion keeps at most one per level; the leaf level's, taken right
+// after its left sibling, in registers), each as its first three control points and its first
+// half width: a pending half's last control point and last half width are
+// the first ones of the next shallower pending half (or the segment's own), because a half
+// shares its end with its parent and descending right keeps the parent's end.  Strand parameters
+// are formed only for an accepted chord, by the reference's sequence of halvings and additions
+// along the path of left / right turns (CurveBrute.cpp:150-172).
+constexpr uint32_t kSubComps = 10;   // P0, P1, P2, hw0 of a pending half
+typedef __attribute__((address_space(3))) float lds_f32;
+struct PoolSeg {
+    V3 P[4];          // the frame being tested
+    float hw0, hw3;
+    uint32_t level;   // splits from the segment
+    uint32_t path;    // bit i - 1 set: the right half was taken at level i
+    uint32_t pend;    // bit i set: a right half of level i waits in LDS
+    bool leaf;        // a right half of the leaf level waits in registers (lP*, lHw0)
+    V3 lP0, lP1, lP2;
+    float lHw0;
+};
+struct SegRoot {
+    V3 P3;
+    float hw3, tOff, tScale;
+};
+__device__ __forceinline__ uint32_t subIdx(uint32_t level, uint32_t comp, uint32_t lane) {
+    return ((level - 1u) * kSubComps + comp) * 64u + lane;
+}
+
+// One frame of the pooled subdivision: the test of segmentStep; a frame that passes and is not a
+// leaf leaves its right half in LDS and becomes its left half; otherwise the deepest pending half
+// is taken.  Returns true when no frame is left.
+__device__ __forceinline__ bool poolSegStep(PoolSeg& s, const SegRoot& rt, lds_f32* stk, uint32_t lane, V3 o, V3 d,
+                                            V3 fx, V3 fy, float best, uint32_t depth, bool& acc, float& tr,
+                                            float& tSeg, V3& tang) {
+#pragma clang fp contract(off)
+    acc = false;
+    {
+        const V3 c0 = projNC(s.P[0], o, fx, fy, d), c1 = projNC(s.P[1], o, fx, fy, d);
+        const V3 c2 = projNC(s.P[2], o, fx, fy, d), c3 = projNC(s.P[3], o, fx, fy, d);
+        const float mh = fmaxf(s.hw0, s.hw3);
+        const float zMin = fminf(fminf(fminf(c0.z, c1.z), c2.z), c3.z);
+        const float zMax = fmaxf(fmaxf(fmaxf(c0.z, c1.z), c2.z), c3.z);
+        if (zMin - mh > best) goto next;
+        if (zMax + mh < kCurveTMin) goto next;
+        const float xMin = fminf(fminf(fminf(c0.x, c1.x), c2.x), c3.x), xMax = fmaxf(fmaxf(fmaxf(c0.x, c1.x), c2.x), c3.x);
+        const float yMin = fminf(fminf(fminf(c0.y, c1.y), c2.y), c3.y), yMax = fmaxf(fmaxf(fmaxf(c0.y, c1.y), c2.y), c3.y);
+        const float cx = (0.f < xMin) ? xMin : (0.f > xMax ? xMax : 0.f);
+        const float cy = (0.f < yMin) ? yMin : (0.f > yMax ? yMax : 0.f);
+        if (cx * cx + cy * cy > mh * mh) goto next;
+        if (s.level == depth) {
+            const float ax = c0.x, ay = c0.y, dx = c3.x - ax, dy = c3.y - ay;
+            const float l2 = dx * dx + dy * dy;
+            const float tl = l2 < 1e-12f ? 0.f : stdmax(0.f, stdmin(1.f, (-ax * dx - ay * dy) / l2));
+            const float px = ax + dx * tl, py = ay + dy * tl;
+            const float hw = s.hw0 * (1.f - tl) + s.hw3 * tl;
+            if (px * px + py * py > hw * hw) goto next;
+            const float t = c0.z + (c3.z - c0.z) * tl;
+            if (t < kCurveTMin || t >= best) goto next;
+            // bezierTangent (CurveBrute.cpp:59-64)
+            const float mt = 1.f - tl;
+            const float a0 = 3.f * mt * mt, a1 = 6.f * mt * tl, a2 = 3.f * tl * tl;
+            const V3 d10{s.P[1].x - s.P[0].x, s.P[1].y - s.P[0].y, s.P[1].z - s.P[0].z};
+            const V3 d21{s.P[2].x - s.P[1].x, s.P[2].y - s.P[1].y, s.P[2].z - s.P[1].z};
+            const V3 d32{s.P[3].x - s.P[2].x, s.P[3].y - s.P[2].y, s.P[3].z - s.P[2].z};
+            V3 tg{d10.x * a0 + d21.x * a1 + d32.x * a2, d10.y * a0 + d21.y * a1 + d32.y * a2,
+                  d10.z * a0 + d21.z * a1 + d32.z * a2};
+            if (tg.x * tg.x + tg.y * tg.y + tg.z * tg.z < 1e-12f)
+                tg = V3{s.P[3].x - s.P[0].x, s.P[3].y - s.P[0].y, s.P[3].z - s.P[0].z};
+            const float l = sqrtf(tg.x * tg.x + tg.y * tg.y + tg.z * tg.z);
+            if (l > 1e-8f) {
+                const float il = 1.f / l;
+                tg = V3{tg.x * il, tg.y * il, tg.z * il};
+            } else {
+                tg = V3{0.f, 0.f, 1.f};
+            }
+            // the leaf's parameter offset and width: tOff + tScale/2 at each right turn, in order
+            float off = rt.tOff, sc = rt.tScale;
+            for (uint32_t i = 0; i < s.level; ++i) {
+                sc = sc * 0.5f;
+                if ((s.path >> i) & 1u) off = off + sc;
+            }
+            acc = true;
+            tr = t;
+            tSeg = off + tl * sc;
+            tang = tg;
+            goto next;
+        }
+        const V3 q01 = midNC(s.P[0], s.P[1]), q12 = midNC(s.P[1], s.P[2]), q23 = midNC(s.P[2], s.P[3]);
+        const V3 r01 = midNC(q01, q12), r12 = midNC(q12, q23), m = midNC(r01, r12);
+        const float hm = (s.hw0 + s.hw3) * 0.5f;
+        const uint32_t lv = s.level + 1u;
+        if (lv == depth) {   // its left half is a leaf: the right half is taken next
+            s.leaf = true;
+            s.lP0 = m;
+            s.lP1 = r12;
+            s.lP2 = q23;
+            s.lHw0 = hm;
+        } else {
+        stk[subIdx(lv, 0, lane)] = m.x;
+        stk[subIdx(lv, 1, lane)] = m.y;
+        stk[subIdx(lv, 2, lane)] = m.z;
+        stk[subIdx(lv, 3, lane)] = r12.x;
+        stk[subIdx(lv, 4, lane)] = r12.y;
+        stk[subIdx(lv, 5, lane)] = r12.z;
+        stk[subIdx(lv, 6, lane)] = q23.x;
+        stk[subIdx(lv, 7, lane)] = q23.y;
+        stk[subIdx(lv, 8, lane)] = q23.z;
+        stk[subIdx(lv, 9, lane)] = hm;
+        s.pend |= 1u << lv;
+        }
+        s.P[1] = q01;
+        s.P[2] = r01;
+        s.P[3] = m;
+        s.hw3 = hm;
+        s.level = lv;
+        s.path &= ~(1u << (lv - 1u));
+        return false;
+    }
+next:
+    uint32_t k;
+    if (s.leaf) {
+        k = depth;
+        s.P[0] = s.lP0;
+        s.P[1] = s.lP1;
+        s.P[2] = s.lP2;
+        s.hw0 = s.lHw0;
+        s.leaf = false;
+    } else {
+        if (s.pend == 0u) return true;
+        k = 31u - (uint32_t)__clz(s.pend);
+        s.P[0] = V3{stk[subIdx(k, 0, lane)], stk[subIdx(k, 1, lane)], stk[subIdx(k, 2, lane)]};
+        s.P[1] = V3{stk[subIdx(k, 3, lane)], stk[subIdx(k, 4, lane)], stk[subIdx(k, 5, lane)]};
+        s.P[2] = V3{stk[subIdx(k, 6, lane)], stk[subIdx(k, 7, lane)], stk[subIdx(k, 8, lane)]};
+        s.hw0 = stk[subIdx(k, 9, lane)];
+        s.pend &= ~(1u << k);
+    }
+    if (s.pend != 0u) {
+        const uint32_t k2 = 31u - (uint32_t)__clz(s.pend);
+        s.P[3] = V3{stk[subIdx(k2, 0, lane)], stk[subIdx(k2, 1, lane)], stk[subIdx(k2, 2, lane)]};
+        s.hw3 = stk[subIdx(k2, 9, lane)];
+    } else {
+        s.P[3] = rt.P3;
+        s.hw3 = rt.hw3;
+    }
+    s.level = k;
+    s.path = (s.path & ((1u << (k - 1u)) - 1u)) | (1u << (k - 1u));
+    return false;
+}
+
 // One wave per block (the pool, the slots and the keys are the wave's own, so no barrier is
 // needed: a wave's LDS accesses take effect in program order).  K: LDS entries of each lane's walk
 // stack, deeper entries in q.cwRef / q.cwT.
@@ -495,12 +644,14 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
     __shared__ unsigned long long sKey[64];    // t bits << 32 | record; any hit: 0 once occluded
     __shared__ uint32_t sOut[64];              // the slot's records in the pool or being tested
     __shared__ uint32_t sPool[kPoolCap];       // slot << kPoolSlotShift | record
+    __shared__ float sSub[(kMaxCurveDepth - 1u) * kSubComps * 64];   // pending right halves above the leaf level
+    lds_f32* const stk = (lds_f32*)sSub;
     const uint32_t lane = threadIdx.x;
     const TStack S{lstk, q.cwRef, q.cwT, lane, 64u, K, blockIdx.x * 64u + lane, gridDim.x * 64u, q.cwCap};
     const uint32_t count = *ctr(q, bounce, Q);
     uint32_t* const overflow = ctr(q, bounce, 3);
     const float inf = __int_as_float(0x7f800000);
-    const int depth = (int)sc.curveDepth;
+    const uint32_t depth = sc.curveDepth;
 
     // this lane's slot
     uint32_t ray = ~0u, wsp = 0u;
@@ -511,9 +662,8 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
     bool busy = false;
     uint32_t tSlot = 0u, tRec = 0u;
     V3 to{0.f, 0.f, 0.f}, td{0.f, 0.f, 1.f}, fx{0.f, 0.f, 0.f}, fy{0.f, 0.f, 0.f};
-    SubFrame f;
-    SubFrame st[7];
-    int sp = 0;
+    PoolSeg ps;
+    SegRoot rt;
     float own = inf;
     // wave-uniform: rays blockIdx.x + gridDim.x * k for k < nextK have been taken; the pool holds
     // `avail` records from `head` on
@@ -629,8 +779,16 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
                     to = v3(sO[tSlot]);
                     td = v3(sD[tSlot]);
                     onbNC(td, fx, fy);
-                    f = rootFrame(sc.segs[tRec], depth);
-                    sp = 0;
+                    const DSeg& g = sc.segs[tRec];
+                    ps.P[0] = V3{g.P[0], g.P[1], g.P[2]};
+                    ps.P[1] = V3{g.P[3], g.P[4], g.P[5]};
+                    ps.P[2] = V3{g.P[6], g.P[7], g.P[8]};
+                    ps.P[3] = V3{g.P[9], g.P[10], g.P[11]};
+                    ps.hw0 = g.hw0;
+                    ps.hw3 = g.hw3;
+                    ps.level = ps.path = ps.pend = 0u;
+                    ps.leaf = false;
+                    rt = SegRoot{ps.P[3], g.hw3, g.tOff, g.tScale};
                     own = inf;
                     busy = true;
                 }
@@ -650,7 +808,7 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
                 bool acc;
                 float t, ts;
                 V3 tg;
-                stop = segmentStep(f, st, sp, to, td, kCurveTMin, fx, fy, fminf(own, thr), acc, t, ts, tg);
+                stop = poolSegStep(ps, rt, stk, lane, to, td, fx, fy, fminf(own, thr), depth, acc, t, ts, tg);
                 if (acc) {
                     own = t;
                     const unsigned long long mine = ANY ? 0ull : (((unsigned long long)__float_as_uint(t) << 32) | tRec);
@@ -1091,6 +1249,13 @@ void launchPassEnd(const WaveQueues& q, uint32_t maxDepth, unsigned long long* t
     hipLaunchKernelGGL(k_pass_end, dim3(1), dim3(256), 0, s, q.counters, maxDepth, totals, q.passIdx);
 }
 uint32_t trace_block_size() { return kTraceBS; }
+uint32_t curve_pool_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_curve_pool<kPoolStackK, kQPath, false>, 64, 0) != hipSuccess ||
+        n <= 0)
+        return 1u;
+    return (uint32_t)n;
+}
 void launchGenerate(const RenderConsts& rc, const WaveQueues& q, uint32_t blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, rc, q);
 }
