@@ -107,7 +107,7 @@ struct ahip_context {
         dDomeCI;
     std::vector<DevBuf*> texBufs;
     CurveOutput curves;
-    DevBuf dCNodes, dSegs, dSegIds, dStrandInfo;
+    DevBuf dCNodes, dCWide, dSegs, dSegIds, dStrandInfo;
     DevBuf dTriAlb;                      // sampling-parameter per-triangle albedo (commit order)
     uint32_t numCommit = 0;              // triangles in commit order
     std::vector<uint32_t> texW, texH;
@@ -298,7 +298,7 @@ WaveQueues queues(ahip_context* ctx) {
     q.stackCap = ctx->stackCap;
     q.cap = ctx->qCap;
     // the pool entry holds a record in kPoolSlotShift bits; larger curve sets use the whole-ray kernels
-    if (!ctx->curves.nodes.empty() && ctx->qCurveCap == ctx->qCap &&
+    if (!ctx->curves.nodes.empty() && ctx->qCurveCap == ctx->qCap && ctx->dsc.cwide &&
         ctx->curves.segs.size() < (size_t(1) << kPoolSlotShift)) {
         q.cState = ctx->qCState.as<uint32_t>();
         q.cwRef = ctx->qCwRef.as<uint32_t>();
@@ -633,6 +633,40 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     if ((st = upload(ctx, ctx->dSegIds, ctx->curves.segIds.data(), ctx->curves.segIds.size()))) return st;
     if ((st = upload(ctx, ctx->dStrandInfo, ctx->curves.strandInfo.data(), ctx->curves.strandInfo.size()))) return st;
     S.cnodes = ctx->dCNodes.as<CNode>();
+    // child pairs for the pooled walk; not built (the whole-ray kernels run) when a leaf holds
+    // more than 4 records or starts past 2^29 (the reference's builder makes leaves of at most 4)
+    S.cwide = nullptr;
+    if (!ctx->curves.nodes.empty()) {
+        const std::vector<CNode>& cn = ctx->curves.nodes;
+        std::vector<CWide> wide(cn.size());
+        bool ok = true;
+        auto ref = [&](uint32_t c) -> uint32_t {
+            const CNode& n = cn[c];
+            if (!(n.rc & 0x80000000u)) return c;
+            const uint32_t cnt = n.rc & 0x7fffffffu;
+            if (cnt == 0u || cnt > 4u || n.lp >= (1u << 29)) ok = false;
+            return kCWLeaf | ((cnt - 1u) & 3u) << 29 | (n.lp & ((1u << 29) - 1u));
+        };
+        for (size_t i = 0; i < cn.size() && ok; ++i) {
+            if (cn[i].rc & 0x80000000u) continue;
+            CWide& w = wide[i];
+            const CNode &a = cn[cn[i].lp], &b = cn[cn[i].rc];
+            for (int k = 0; k < 3; ++k) {
+                w.a[k] = a.bmin[k];
+                w.a[3 + k] = a.bmax[k];
+                w.b[k] = b.bmin[k];
+                w.b[3 + k] = b.bmax[k];
+            }
+            w.refA = ref(cn[i].lp);
+            w.refB = ref(cn[i].rc);
+            w.pad0 = w.pad1 = 0u;
+        }
+        if (ok && (cn[0].rc & 0x80000000u)) ref(0u);   // a one-leaf tree: its root is pushed as a leaf
+        if (ok) {
+            if ((st = upload(ctx, ctx->dCWide, wide.data(), wide.size()))) return st;
+            S.cwide = ctx->dCWide.as<CWide>();
+        }
+    }
     S.segs = ctx->dSegs.as<DSeg>();
     S.segIds = ctx->dSegIds.as<uint32_t>();
     S.strandInfo = ctx->dStrandInfo.as<float>();

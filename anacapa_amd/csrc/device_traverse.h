@@ -520,19 +520,19 @@ __device__ __forceinline__ bool curveBox(const CNode& n, const TRay& r, const V3
 
 // curveBox that also gives the entry distance (the larger of tMin and the near slab planes), by
 // which a walk orders two children.
-__device__ __forceinline__ bool curveBoxT(const CNode& n, V3 o, V3 d, const V3& invD, float tMin, float maxT,
-                                          float& entry) {
+__device__ __forceinline__ bool curveBoxT(const float* bmin, const float* bmax, V3 o, V3 d, const V3& invD, float tMin,
+                                          float maxT, float& entry) {
 #pragma clang fp contract(off)
     float tmin = tMin, tmax = maxT;
     const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, id[3] = {invD.x, invD.y, invD.z};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         if (fabsf(dd[i]) < 1e-9f) {
-            if (oo[i] < n.bmin[i] || oo[i] > n.bmax[i]) return false;
+            if (oo[i] < bmin[i] || oo[i] > bmax[i]) return false;
             continue;
         }
         float inv = id[i];
-        float t0 = (n.bmin[i] - oo[i]) * inv, t1 = (n.bmax[i] - oo[i]) * inv;
+        float t0 = (bmin[i] - oo[i]) * inv, t1 = (bmax[i] - oo[i]) * inv;
         if (inv < 0.f) { float s = t0; t0 = t1; t1 = s; }
         tmin = stdmax(tmin, t0);
         tmax = stdmin(tmax, t1);
@@ -540,6 +540,19 @@ __device__ __forceinline__ bool curveBoxT(const CNode& n, V3 o, V3 d, const V3& 
     }
     entry = tmin;
     return true;
+}
+
+// A child-pair record as four 16-byte loads issued together (the box tests then run on registers).
+__device__ __forceinline__ CWide loadCWide(const CWide* w, uint32_t i) {
+    const float4* p = reinterpret_cast<const float4*>(w + i);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    CWide r;
+    r.a[0] = a.x; r.a[1] = a.y; r.a[2] = a.z; r.a[3] = a.w; r.a[4] = b.x; r.a[5] = b.y;
+    r.b[0] = b.z; r.b[1] = b.w; r.b[2] = c.x; r.b[3] = c.y; r.b[4] = c.z; r.b[5] = c.w;
+    r.refA = __float_as_uint(d.x);
+    r.refB = __float_as_uint(d.y);
+    r.pad0 = r.pad1 = 0u;
+    return r;
 }
 
 __device__ __forceinline__ CNode loadCNode(const CNode* nodes, uint32_t i) {

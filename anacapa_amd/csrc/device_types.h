@@ -83,6 +83,16 @@ struct alignas(64) DSeg {
 static_assert(sizeof(DSeg) == 64, "DSeg must be 64 bytes");
 
 constexpr uint32_t kCurveHit = 0x80000000u;   // hit code flag: low bits = segment record
+
+// Both children of an interior curve-tree node in one 64-byte record, for a walk that orders the
+// children before it descends: each child's box (bmin, bmax) and reference, the child's node index
+// if it is interior, else kCWLeaf | (record count - 1) << 29 | first record.
+struct alignas(64) CWide {
+    float a[6], b[6];
+    uint32_t refA, refB, pad0, pad1;
+};
+static_assert(sizeof(CWide) == 64, "CWide must be 64 bytes");
+constexpr uint32_t kCWLeaf = 0x80000000u;
 constexpr uint32_t kMaxCurveDepth = 6;        // ribbon subdivision levels (CurveBrute.cpp:194 fixes 6)
 
 struct DMat {
@@ -167,6 +177,7 @@ struct DScene {
     DDome dome;
     // curves
     const CNode* cnodes;
+    const CWide* cwide;       // [nodes] child pairs of the interior nodes (null: not built)
     const DSeg* segs;
     const uint32_t* segIds;   // [2 * segments]: strand, segment index within the strand
     const float* strandInfo;  // [4 * strands]: colour rgb, meshID (bits)

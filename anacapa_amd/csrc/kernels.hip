@@ -708,7 +708,9 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
                     winv = V3{1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z};   // used only where |d| >= 1e-9
                     wsp = 0u;
                     float te;
-                    if (curveBoxT(loadCNode(sc.cnodes, 0u), wo, wd, winv, kCurveTMin, b0, te)) spush(S, wsp, 0u, te);
+                    const CNode root = loadCNode(sc.cnodes, 0u);
+                    if (curveBoxT(root.bmin, root.bmax, wo, wd, winv, kCurveTMin, b0, te))
+                        spush(S, wsp, (root.rc & 0x80000000u) ? kCWLeaf | ((root.rc - 1u) & 3u) << 29 | root.lp : 0u, te);
                     walking = wsp > 0u;
                 }
             }
@@ -723,35 +725,34 @@ __global__ __launch_bounds__(64) void k_curve_pool(DScene sc, WaveQueues q, uint
                     walking = false;
                 } else {
                     const float bound = __uint_as_float((uint32_t)(key >> 32));
-                    uint32_t node;
+                    uint32_t ref;
                     float te;
-                    spop(S, wsp, node, te);
-                    if (!(te > bound)) {
-                        const CNode nd = loadCNode(sc.cnodes, node);
-                        if (nd.rc & 0x80000000u) {
-                            n = nd.rc & 0x7fffffffu;
-                            lp = nd.lp;
-                        } else if (wsp + 2u > S.cap) {
-                            *overflow = 1u;
-                            wsp = 0u;
-                        } else {
-                            const CNode a = loadCNode(sc.cnodes, nd.lp), b = loadCNode(sc.cnodes, nd.rc);
-                            float ta = 0.f, tb = 0.f;
-                            const bool ha = curveBoxT(a, wo, wd, winv, kCurveTMin, bound, ta);
-                            const bool hb = curveBoxT(b, wo, wd, winv, kCurveTMin, bound, tb);
-                            if (ha && hb) {   // the nearer child is taken next (the left one on a tie)
-                                if (tb < ta) {
-                                    spush(S, wsp, nd.lp, ta);
-                                    spush(S, wsp, nd.rc, tb);
-                                } else {
-                                    spush(S, wsp, nd.rc, tb);
-                                    spush(S, wsp, nd.lp, ta);
-                                }
-                            } else if (ha) {
-                                spush(S, wsp, nd.lp, ta);
-                            } else if (hb) {
-                                spush(S, wsp, nd.rc, tb);
+                    spop(S, wsp, ref, te);
+                    if (te > bound) {
+                        // nearer hits were found since the entry was pushed
+                    } else if (ref & kCWLeaf) {
+                        n = ((ref >> 29) & 3u) + 1u;
+                        lp = ref & ((1u << 29) - 1u);
+                    } else if (wsp + 2u > S.cap) {
+                        *overflow = 1u;
+                        wsp = 0u;
+                    } else {
+                        const CWide w = loadCWide(sc.cwide, ref);
+                        float ta = 0.f, tb = 0.f;
+                        const bool ha = curveBoxT(w.a, w.a + 3, wo, wd, winv, kCurveTMin, bound, ta);
+                        const bool hb = curveBoxT(w.b, w.b + 3, wo, wd, winv, kCurveTMin, bound, tb);
+                        if (ha && hb) {   // the nearer child is taken next (the left one on a tie)
+                            if (tb < ta) {
+                                spush(S, wsp, w.refA, ta);
+                                spush(S, wsp, w.refB, tb);
+                            } else {
+                                spush(S, wsp, w.refB, tb);
+                                spush(S, wsp, w.refA, ta);
                             }
+                        } else if (ha) {
+                            spush(S, wsp, w.refA, ta);
+                        } else if (hb) {
+                            spush(S, wsp, w.refB, tb);
                         }
                     }
                     walking = wsp > 0u;
