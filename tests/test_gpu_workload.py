@@ -5,7 +5,8 @@
     kernels, one pass), compared band by band with the oracle's render of the same rows at the
     same sample indices.
   * Hair, 200,000 strands x 4 Bezier segments: the curve tree and 20,000 rays (closest and any
-    hit), bit for bit.
+    hit), bit for bit; the whole 1024x1024 frame at 1 spp in both hair modes, the pooled curve
+    queries against the one-ray-per-lane kernels, bit for bit.
   * The batched form: the 64 seeded views of the GGX scene (256^2 .. 1080p, 1 .. 64 spp) in one
     ahip_render_batch call, every view equal to its own plain render, and a tile of every view
     against the oracle.
@@ -120,6 +121,26 @@ def test_hair_workload_tree_and_hits_vs_oracle(gpu, oracle):
     for k in ("t", "u", "v"):
         assert np.array_equal(g[k][hit].view(np.uint32), o[k][hit].view(np.uint32)), k
     assert np.array_equal(gpu.occluded_rays(rays), os_.occluded(rays))
+
+
+def test_hair_frame_pooled_equals_whole_ray(gpu, monkeypatch):
+    """The pooled curve queries on the full hair frame (200k strands, 1024x1024, one sample per
+    pixel, depth 16, both hair modes): film and AOVs bit for bit equal to the one-ray-per-lane
+    kernels (AHIP_DEBUG_WHOLE_RAY=1), which the curve tests pin to the oracle.  At this size every
+    walk stack reaches its global part, pools refill many times per wave and the any-hit queries
+    stop rays with records still pending."""
+    from anacapa_amd import _abi as A
+    sc = scenes.hair_sphere(spp=1)
+    for flags, depth in ((0, 0), (A.AHIP_FLAG_HAIR_MARSCHNER, 5)):
+        gpu.prepare(sc, flags=flags, curve_depth=depth)
+        out = {}
+        for mode in ("1", "0"):
+            monkeypatch.setenv("AHIP_DEBUG_WHOLE_RAY", mode)
+            print(f"[workload] hair 1 spp, flags {flags}, depth {depth or 6}, whole-ray={mode}", flush=True)
+            out[mode] = gpu.render_tile_aovs(0, 0, sc.width, sc.height, 0, 1)
+        for a, b in zip(out["1"], out["0"]):
+            assert np.array_equal(a, b)
+        assert out["0"][0][..., 3].min() == 1 and out["0"][0][..., :3].mean() > 0.01
 
 
 def test_batched_workload_views(gpu, ggx):
