@@ -31,6 +31,8 @@ AHIP_LIGHT_AREA = 0
 AHIP_LIGHT_DOME = 1
 AHIP_FLAG_FIX_DOUBLE_COSINE = 1
 AHIP_FLAG_HAIR_MARSCHNER = 2
+AHIP_INTEGRATOR_PATH = 0
+AHIP_INTEGRATOR_BDPT = 1
 
 F3 = C.c_float * 3
 U32P = C.POINTER(C.c_uint32)
@@ -83,7 +85,8 @@ class Camera(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("min_depth", C.c_uint32), ("seed", C.c_uint64),
-                ("flags", C.c_uint32), ("curve_depth", C.c_uint32)]
+                ("flags", C.c_uint32), ("curve_depth", C.c_uint32), ("integrator", C.c_uint32),
+                ("firefly_clamp", C.c_float)]
 
 
 class View(C.Structure):

@@ -198,7 +198,7 @@ bool HipPathIntegrator::ensureCamera(const SceneView& scene, uint32_t w, uint32_
     put3(c.vertical, cam.vertical);
     c.width = w;
     c.height = h;
-    ahip_render_params p{m_maxDepth, 2, 0, 0, 0};
+    ahip_render_params p{m_maxDepth, 2, 0, 0, 0, AHIP_INTEGRATOR_PATH, 10.f};
     return ahip_set_camera(m_ctx, &c) == AHIP_OK && ahip_set_params(m_ctx, &p) == AHIP_OK;
 }
 

@@ -121,7 +121,7 @@ struct ahip_context {
     // render settings
     ahip_camera cam{};
     bool hasCam = false;
-    ahip_render_params params{8, 2, 0, 0, 0};
+    ahip_render_params params{8, 2, 0, 0, 0, AHIP_INTEGRATOR_PATH, 10.f};
     uint32_t maxPaths = 1u << 25;
 
     // persistent accumulation
@@ -134,6 +134,11 @@ struct ahip_context {
     DevBuf qCState, qCwRef, qCwT;      // curve scenes: the pooled curve queries
     uint32_t qCurveCap = 0, qCurveStack = 0;
     uint32_t poolBlocks = 0;           // k_curve_pool grid (one wave per block)
+    // BDPT (bdpt.hip): why the prepared scene is refused (empty: accepted), the light sampler's
+    // alias table and the vertex stores of a chunk of slots
+    std::string bdptRefusal = "no scene";
+    DevBuf dAliasProb, dAlias, dLightW, dBdptVerts, dBdptCounts, dBdptAovA, dBdptAovN, dBdptL, dBdptCtr;
+    size_t bdptCap = 0, bdptMaxV = 0;
     DevBuf qPix, qPassIdx, dViews, dTables, dBatchAcc, dBatchLsq;
     uint32_t tileW = 8;          // pixel tile per wavefront: tileW x (64 / tileW)
     bool frameGraph = false;     // render_frame: per-launch timing events (the roofline figure needs them)
@@ -585,6 +590,7 @@ int ahip_create(int device_ordinal, ahip_context** out) {
     fillHaltonTables(T);
     setHaltonTables(T, ctx->stream);
     setHaltonTablesReplay(T, ctx->stream);
+    setHaltonTablesBdpt(T, ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // the no-allocation scratch of this device (the device is current); a failure here only
     // disables ahip_trace_rays_noalloc, which reports it
@@ -745,6 +751,53 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         S.dome.intensity = D.intensity;
         std::memcpy(S.dome.rot, D.rot, sizeof(D.rot));
     }
+    // BDPT takes triangles with Lambert / Emissive materials lit by quad area lights, no environment
+    ctx->bdptRefusal.clear();
+    if (!ctx->curves.nodes.empty()) ctx->bdptRefusal = "curves";
+    else if (hs.hasDome || d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
+        ctx->bdptRefusal = "an environment";
+    for (const DMat& m : hs.mats)
+        if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE)
+            ctx->bdptRefusal = "a material other than Lambert / Emissive";
+    for (const DLight& L : hs.lights)
+        if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA) ctx->bdptRefusal = "a light other than an area light";
+    if (ctx->bdptRefusal.empty() && !hs.lights.empty()) {
+        // LightSampler::build (LightSampler.h:28-69): Vose's alias table over the lights' power,
+        // AreaLight::power (AreaLight.h:114-117)
+        const uint32_t n = (uint32_t)hs.lights.size();
+        std::vector<float> w(n), prob(n);
+        std::vector<uint32_t> alias(n, 0u), small, large;
+        float total = 0.f;
+        for (uint32_t i = 0; i < n; ++i) {
+            const DLight& L = hs.lights[i];
+            const float lum = 0.2722287f * L.Le[0] + 0.6740818f * L.Le[1] + 0.0536895f * L.Le[2];
+            w[i] = std::max(0.f, lum * L.area * 3.14159265358979323846f);
+            total += w[i];
+        }
+        if (total <= 0.f) {
+            for (float& x : w) x = 1.f;
+            total = (float)n;
+        }
+        const float inv = 1.f / total;
+        for (float& x : w) x *= inv;
+        for (uint32_t i = 0; i < n; ++i) {
+            prob[i] = w[i] * (float)n;
+            (prob[i] < 1.f ? small : large).push_back(i);
+        }
+        while (!small.empty() && !large.empty()) {
+            const uint32_t sm = small.back(), lg = large.back();
+            small.pop_back();
+            large.pop_back();
+            alias[sm] = lg;
+            prob[lg] = prob[lg] + prob[sm] - 1.f;
+            (prob[lg] < 1.f ? small : large).push_back(lg);
+        }
+        for (uint32_t i : small) prob[i] = 1.f;
+        for (uint32_t i : large) prob[i] = 1.f;
+        if ((st = upload(ctx, ctx->dAliasProb, prob.data(), n))) return st;
+        if ((st = upload(ctx, ctx->dAlias, alias.data(), n))) return st;
+        if ((st = upload(ctx, ctx->dLightW, w.data(), n))) return st;
+    }
     if ((st = ensureStack(ctx))) return st;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->prepared = true;
@@ -780,6 +833,9 @@ int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
     ENTER();
     if (!p || p->max_depth > 1000u) return fail(ctx, AHIP_E_INVALID_ARG, "bad render params");
     if (p->curve_depth > kMaxCurveDepth) return fail(ctx, AHIP_E_INVALID_ARG, "curve_depth must be 0..6");
+    if (p->integrator > AHIP_INTEGRATOR_BDPT) return fail(ctx, AHIP_E_INVALID_ARG, "unknown integrator");
+    if (!(p->firefly_clamp >= 0.f) || !std::isfinite(p->firefly_clamp))
+        return fail(ctx, AHIP_E_INVALID_ARG, "firefly_clamp must be finite and >= 0");
     ctx->params = *p;
     ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : kMaxCurveDepth;
     return AHIP_OK;
@@ -828,6 +884,75 @@ static int renderCall(ahip_context* ctx, const std::vector<ViewJob>& jobs, bool 
 // Make sure the persistent film-sized accumulation (and, when on, the AOV sums) exists.
 static int ensureFrameAccum(ahip_context* ctx, uint32_t width, uint32_t height);
 
+// BDPTIntegrator::renderTile over the tile (x0, y0, w, h) x [s0, s0 + ns): chunks of whole pixels,
+// each (pixel, sample) slot on one lane (bdpt.hip), added into outAcc / outLsq / outAov at
+// (outX0 + x, outY0 + y) with row stride outStride.
+static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns,
+                      float4* outAcc, float* outLsq, float* outAov, uint32_t outStride, uint32_t outX0,
+                      uint32_t outY0) {
+    if (!ctx->bdptRefusal.empty())
+        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive materials and area lights; "
+                                             "the scene has " + ctx->bdptRefusal);
+    if (ns == 0u) return AHIP_OK;
+    int st = ensureQueues(ctx, 64);
+    if (st) return st;
+    // about 1M slots per chunk: 2 x (maxDepth + 2) vertices of 80 B each per slot
+    const size_t maxV = (size_t)ctx->params.max_depth + 2u;
+    const size_t pixPerChunk = std::max<size_t>(1u, (size_t)(1u << 20) / ns);
+    const size_t cap = pixPerChunk * ns;
+    if (cap * 2u * maxV * 5u > (size_t)1 << 31) return fail(ctx, AHIP_E_INVALID_ARG, "BDPT chunk too large");
+    if (ctx->bdptCap < cap || ctx->bdptMaxV < maxV || !ctx->dBdptVerts.p) {
+        HIPCHK(ctx->dBdptVerts.alloc(cap * 2u * maxV * 5u * sizeof(float4)));
+        HIPCHK(ctx->dBdptCounts.alloc(cap * sizeof(uint32_t)));
+        HIPCHK(ctx->dBdptAovA.alloc(cap * sizeof(float4)));
+        HIPCHK(ctx->dBdptAovN.alloc(cap * sizeof(float4)));
+        HIPCHK(ctx->dBdptL.alloc(cap * sizeof(float4)));
+        HIPCHK(ctx->dBdptCtr.alloc(sizeof(uint32_t)));
+        ctx->bdptCap = cap;
+        ctx->bdptMaxV = maxV;
+    }
+    BdptArgs a{};
+    a.cam = toDCam(ctx->cam);
+    a.x0 = x0; a.y0 = y0; a.w = w; a.h = h; a.s0 = s0; a.ns = ns;
+    a.seed = ctx->params.seed;
+    a.maxDepth = ctx->params.max_depth;
+    a.maxV = (uint32_t)maxV;
+    a.clamp = ctx->params.firefly_clamp;
+    a.cap = (uint32_t)ctx->bdptCap;
+    a.verts = ctx->dBdptVerts.as<float4>();
+    a.counts = ctx->dBdptCounts.as<uint32_t>();
+    a.aovA = ctx->dBdptAovA.as<float4>();
+    a.aovN = ctx->dBdptAovN.as<float4>();
+    a.sampleL = ctx->dBdptL.as<float4>();
+    a.aliasProb = ctx->dAliasProb.as<float>();
+    a.alias = ctx->dAlias.as<uint32_t>();
+    a.lightW = ctx->dLightW.as<float>();
+    a.ovfRef = ctx->qOvfRef.as<uint32_t>();
+    a.ovfT = ctx->qOvfT.as<float>();
+    a.stackCap = ctx->stackCap;
+    a.overflow = ctx->dBdptCtr.as<uint32_t>();
+    a.outAcc = outAcc;
+    a.outLsq = outLsq;
+    a.outAov = outAov;
+    a.outStride = outStride;
+    a.outX0 = outX0;
+    a.outY0 = outY0;
+    launchFillU32(a.overflow, 1, 0u, ctx->stream);
+    const size_t npix = (size_t)w * h;
+    for (size_t p0 = 0; p0 < npix; p0 += pixPerChunk) {
+        a.pix0 = (uint32_t)p0;
+        a.slots = (uint32_t)(std::min(pixPerChunk, npix - p0) * ns);
+        // every slot on its own lane; never more threads than the traversal overflow stacks hold
+        const uint32_t blocks = std::min<uint32_t>(ctx->traceBlocks, (a.slots + 127u) / 128u);
+        launchBdpt(ctx->dsc, a, blocks, ctx->stream);
+    }
+    uint32_t ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, a.overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ovf) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    return AHIP_OK;
+}
+
 static int renderFrameImpl(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
                            uint32_t sample_count) {
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
@@ -836,10 +961,16 @@ static int renderFrameImpl(ahip_context* ctx, uint32_t width, uint32_t height, u
         return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");
     int st = ensureFrameAccum(ctx, width, height);
     if (st) return st;
-    std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
-                                      ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(),
-                                      ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
-    st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
+    if (ctx->params.integrator == AHIP_INTEGRATOR_BDPT) {
+        if (ctx->aovOn) return fail(ctx, AHIP_E_UNSUPPORTED, "AOVs of BDPT frames (render tiles with AOVs instead)");
+        st = bdptRender(ctx, 0, 0, width, height, sample_start, sample_count, ctx->dAccum.as<float4>(),
+                        ctx->dAccumLsq.as<float>(), nullptr, width, 0, 0);
+    } else {
+        std::vector<ViewJob> jobs{ViewJob{ctx->cam, 0, 0, width, height, sample_start, sample_count,
+                                          ctx->dAccum.as<float4>(), ctx->dAccumLsq.as<float>(),
+                                          ctx->aovOn ? ctx->dAccumAov.as<float>() : nullptr}};
+        st = renderCall(ctx, jobs, ctx->frameGraph, nullptr);
+    }
     if (st) return st;
     ctx->accumHasSamples = ctx->accumHasSamples || sample_count > 0;
     ctx->lastRenderValid = true;
@@ -883,6 +1014,8 @@ static int ensureFrameAccum(ahip_context* ctx, uint32_t width, uint32_t height) 
 int ahip_render_adaptive(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t tile_size, uint32_t total_spp,
                          uint32_t base_spp, uint32_t* extra_per_tile, float* tile_variance) {
     ENTER();
+    if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_render_adaptive runs the path tracer only");
     if (tile_size == 0 || total_spp == 0) return fail(ctx, AHIP_E_INVALID_ARG, "tile size and sample count must be > 0");
     uint32_t base = base_spp ? base_spp : std::min(32u, std::max(16u, total_spp / 16u));
     base = std::min(base, total_spp);
@@ -1008,12 +1141,28 @@ static int renderTileImpl(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t 
         HIPCHK(aovBuf.alloc(n * 6 * sizeof(float)));
         launchFillF32(aovBuf.as<float>(), n * 6, 0.f, ctx->stream);
     }
-    std::vector<ViewJob> jobs{ViewJob{ctx->cam, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(),
-                                      lsq.as<float>(), aov ? aovBuf.as<float>() : nullptr}};
-    st = renderCall(ctx, jobs, false, nullptr);
+    const bool bdpt = ctx->params.integrator == AHIP_INTEGRATOR_BDPT;
+    if (bdpt) {
+        st = bdptRender(ctx, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(), lsq.as<float>(),
+                        aov ? aovBuf.as<float>() : nullptr, w, 0, 0);
+    } else {
+        std::vector<ViewJob> jobs{ViewJob{ctx->cam, x0, y0, w, h, sample_start, sample_count, acc.as<float4>(),
+                                          lsq.as<float>(), aov ? aovBuf.as<float>() : nullptr}};
+        st = renderCall(ctx, jobs, false, nullptr);
+    }
     if (st) return st;
     if ((st = downloadRegion(ctx, acc, lsq, n, out))) return st;
     if (!aov) return AHIP_OK;
+    if (bdpt) {   // BDPTIntegrator::renderTile records each pixel's means once (BDPTIntegrator.cpp:567-580)
+        std::vector<float> sums(n * 6);
+        HIPCHK(hipMemcpy(sums.data(), aovBuf.p, n * 6 * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                albedo[3 * i + c] = sums[6 * i + c];
+                normal[3 * i + c] = sums[6 * i + 3 + c];
+            }
+        return AHIP_OK;
+    }
     // PathIntegrator::renderTile (PathIntegrator.cpp:58-66): mean over the tile's samples; the mean
     // normal scaled to unit length when longer than 1e-6
     std::vector<float> sums(n * 6);
@@ -1065,6 +1214,8 @@ int ahip_download_aovs(ahip_context* ctx, float* out) {
 
 int ahip_render_batch(ahip_context* ctx, const ahip_view* views, uint32_t num_views) {
     ENTER();
+    if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_render_batch runs the path tracer only");
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!views && num_views) return fail(ctx, AHIP_E_INVALID_ARG, "null views");
     // One accumulation buffer holds every view's film; views sharing a max depth share passes.
@@ -1327,6 +1478,8 @@ int replayOverflowCheck(ahip_context* ctx) {
 
 int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count, float* out) {
     ENTER();
+    if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_replay runs the path tracer only");
     if (!out) return fail(ctx, AHIP_E_INVALID_ARG, "null output");
     ReplayArgs a;
     uint32_t blocks = 0;
@@ -1354,6 +1507,8 @@ int ahip_replay(ahip_context* ctx, uint32_t sample_start, uint32_t sample_count,
 int ahip_adjoint(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start, uint32_t sample_count,
                  const float* dL_dimage, float* d_albedo, float* d_roughness_tex) {
     ENTER();
+    if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_adjoint runs the path tracer only");
     if (!dL_dimage) return fail(ctx, AHIP_E_INVALID_ARG, "null dL_dimage");
     ReplayArgs a;
     uint32_t blocks = 0;
@@ -1429,6 +1584,8 @@ int ahip_get_bounce_stats(ahip_context* ctx, uint32_t capacity, uint64_t* rays, 
 int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height, uint32_t sample_start,
                        uint32_t sample_count) {
     ENTER();
+    if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_count_traffic runs the path tracer only");
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!ctx->hasCam || width != ctx->cam.width || height != ctx->cam.height)
         return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");

@@ -1,0 +1,385 @@
+// bdpt.hip — the bidirectional path tracer (BDPTIntegrator, src/integrator/BDPTIntegrator.cpp and
+// MISWeight.cpp) on gfx950, for scenes of triangles with Lambert / Emissive materials lit by quad
+// area lights (the reference's BDPT configuration; anything else is refused on the host).
+//
+// One sample per lane in three kernels over a chunk of (pixel, sample) slots:
+//   k_bdpt_paths   — camera subpath (BDPTIntegrator.cpp:70-194) and light subpath (:199-303),
+//                    vertices written to a slot-major vertex store;
+//   k_bdpt_connect — every strategy (s, t) of the sample (:308-459, :521-557): the reference's
+//                    connections (s = 0 emission, s = 1 light sampling, s, t >= 2 vertex pairs;
+//                    t = 1 is not a strategy there), any-hit visibility, bdptMISWeight in double
+//                    (MISWeight.cpp:14-72), the firefly clamp of s, t >= 2;
+//   k_bdpt_accum   — per pixel, its samples in order: the radiance sum, luminance squares and the
+//                    first-hit AOVs, added to the output as renderTile's TileBuffer::add does
+//                    (:559-580).
+// Sampler dimensions follow the reference's call order (g++ evaluates sample(ctx, wo, get2D(),
+// get1D()) and sampleLe(get2D(), get2D()) right to left).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/anacapa_hip.h"
+#include "device_math.h"
+#include "device_sampling.h"
+#include "device_shading.h"
+#include "device_traverse.h"
+#include "device_types.h"
+#include "kernels.h"
+
+namespace ahip {
+
+__constant__ HaltonTables c_haltonB;
+
+namespace {
+
+constexpr uint32_t kVCamera = 0u, kVSurface = 1u, kVLight = 2u, kVDelta = 4u;   // PathVertex.h:24-26
+constexpr uint32_t kBdptBS = 128u, kBdptK = 8u;
+
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+
+// vertex store: field f of vertex v of subpath `sub` (0 camera, 1 light) of slot i
+//   f0: position | pdfFwd   f1: normal | pdfRev   f2: wo | flags   f3: beta | min roughness
+//   f4: Le | material index (surface) or light index (light endpoint)
+__device__ __forceinline__ size_t vAt(const BdptArgs& a, uint32_t sub, uint32_t v, uint32_t f, uint32_t i) {
+    return ((size_t)((sub * a.maxV + v) * 5u + f)) * a.cap + i;
+}
+
+__device__ __forceinline__ float toArea(float pdfSA, V3 prev, V3 next, V3 nextN) {   // PathVertex.h:147-156
+#pragma clang fp contract(off)
+    V3 d = next - prev;
+    float d2 = dot(d, d);
+    if (d2 < 1e-10f) return 0.f;
+    return pdfSA * fabsf(dot(safeNormalize(d), nextN)) / d2;
+}
+
+// BDPTIntegrator::evalVertex (BDPTIntegrator.cpp:40-63) at a surface vertex of material m, normal
+// n, outgoing wo: f(wo, wi) |cos wi| and the forward / reverse pdfs of LambertianMaterial::evaluate
+// (Lambertian.h:53-66).
+__device__ __forceinline__ V3 evalVertex(const DScene& sc, int32_t mi, V3 n, V3 wo, V3 wi, float& fwd, float& rev) {
+    fwd = rev = 0.f;
+    if (mi < 0) return V3{0.f, 0.f, 0.f};
+    const DMat& m = sc.mats[mi];
+    const Frame fr = makeFrame(n, n, -wo);
+    const V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
+    if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
+    fwd = stdmax(0.f, fabsf(wil.z)) * kInvPi;
+    rev = stdmax(0.f, fabsf(wol.z)) * kInvPi;
+    return ld3(m.albedo) * kInvPi * fabsf(dot(wi, n));
+}
+
+__device__ __forceinline__ float hs(const BdptArgs& a, uint32_t px, uint32_t py, uint32_t s, uint32_t& dim) {
+    return haltonSample(c_haltonB, a.seed, px, py, s, dim++);
+}
+
+// The bounce loop shared by the two subpaths (BDPTIntegrator.cpp:97-190 / 240-300): from vertex
+// `count - 1` at prevP along `r`, recording surface vertices until a miss, roulette or an invalid
+// sample.  `light`: the light subpath (no emission recorded).  Returns the vertex count.
+__device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, uint32_t i, uint32_t sub,
+                         uint32_t count, V3 prevP, TRay r, V3 beta, float pdfFwd, uint32_t px, uint32_t py,
+                         uint32_t sidx, uint32_t& dim, bool light, TraceCounters& tc) {
+    for (uint32_t depth = 0; depth < a.maxDepth && count < a.maxV; ++depth) {
+        float t = 0.f, u = 0.f, v = 0.f;
+        const uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, a.overflow);
+        if (h == ~0u) break;
+        // fillSurfaceInteraction (BVHBackend.cpp:285-317)
+        const DTri* T = sc.tris + h;
+        const float w = 1.f - u - v;
+        const V3 v0 = ld3(T->v0), e1 = ld3(T->e1), e2 = ld3(T->e2), ng = ld3(T->n);
+        const DTriShade* SH = sc.shade + h;
+        const V3 p = v0 + e1 * u + e2 * v;
+        V3 n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
+        if (dot(n, ng) < 0.f) n = -n;
+        const V2 uv{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v, SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
+        const int32_t mi = T->matIndex;
+        if (mi < 0) break;
+        const DMat& m = sc.mats[mi];
+        const V3 dir = r.d, wo = -dir;
+        const Frame fr = makeFrame(n, ng, dir);
+        const float pdfFwdArea = toArea(pdfFwd, prevP, p, n);
+        const uint32_t vi = count++;
+        a.verts[vAt(a, sub, vi, 0, i)] = f4(p, pdfFwdArea);
+        a.verts[vAt(a, sub, vi, 1, i)] = f4(n, 0.f);
+        a.verts[vAt(a, sub, vi, 2, i)] = f4(wo, __uint_as_float(kVSurface));
+        a.verts[vAt(a, sub, vi, 3, i)] = f4(beta, 1.f);   // Lambert: roughness() = 1 (IMaterial.h:112)
+        a.verts[vAt(a, sub, vi, 4, i)] = f4((!light && m.hasLe) ? ld3(m.Le) : V3{0.f, 0.f, 0.f}, __int_as_float(mi));
+        if (vi > 0u) {   // the previous vertex's reverse pdf, from this one
+            float dummy, revSA;
+            evalVertex(sc, mi, n, wo, -wo, dummy, revSA);
+            const float4 pv = a.verts[vAt(a, sub, vi - 1u, 0, i)], pn = a.verts[vAt(a, sub, vi - 1u, 1, i)];
+            a.verts[vAt(a, sub, vi - 1u, 1, i)] = make_float4(pn.x, pn.y, pn.z, toArea(revSA, p, v3(pv), v3(pn)));
+        }
+        if (depth >= 2u) {
+            const float q = 1.f - stdmin(maxc(beta), 0.95f);
+            if (hs(a, px, py, sidx, dim) < q) break;
+            beta = beta * (1.f / (1.f - q));
+        }
+        const float uc = hs(a, px, py, sidx, dim);
+        V2 ub;
+        ub.x = hs(a, px, py, sidx, dim);
+        ub.y = hs(a, px, py, sidx, dim);
+        const MatIn mIn = matInputs(sc, m, uv, T->commitIndex);
+        const BSample bs = matSample(sc, m, fr, mIn, wo, ub, uc);
+        if (!bsValid(bs)) break;
+        beta = beta * (bs.f / bs.pdf);
+        pdfFwd = bs.pdf;
+        // spawnRay (Types.h:269-275)
+        V3 off = n * 1e-4f;
+        if (dot(bs.wi, n) < 0.f) off = -off;
+        r = makeTRay(p + off, bs.wi, 1e-4f, __int_as_float(0x7f800000));
+        prevP = p;
+    }
+    return count;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
+    __shared__ uint64_t lstk[kBdptK * kBdptBS];
+    TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
+             gridDim.x * kBdptBS, a.stackCap};
+    TraceCounters tc{0, 0};
+    for (uint32_t i = blockIdx.x * kBdptBS + threadIdx.x; i < a.slots; i += gridDim.x * kBdptBS) {
+        const uint32_t pix = a.pix0 + i / a.ns, sidx = a.s0 + i % a.ns;
+        const uint32_t px = a.x0 + pix % a.w, py = a.y0 + pix / a.w;
+        uint32_t dim = 0u;
+        // camera sample: jitter, lens, time (BDPTIntegrator.cpp:493-497); Camera::generateRay
+        const float jx = hs(a, px, py, sidx, dim), jy = hs(a, px, py, sidx, dim);
+        dim += 3u;
+        const DCamera& cam = a.cam;
+        const float fs = ((float)px + jx) / (float)cam.w;
+        const float ft = ((float)(cam.h - 1u - py) + jy) / (float)cam.h;
+        const V3 o = ld3(cam.origin), d = normalize(ld3(cam.llc) + ld3(cam.hor) * fs + ld3(cam.ver) * ft - o);
+        // camera endpoint (BDPTIntegrator.cpp:80-92), pinhole area pdf 1
+        a.verts[vAt(a, 0, 0, 0, i)] = f4(o, 1.f);
+        a.verts[vAt(a, 0, 0, 1, i)] = f4(-d, 0.f);
+        a.verts[vAt(a, 0, 0, 2, i)] = f4(-d, __uint_as_float(kVCamera));
+        a.verts[vAt(a, 0, 0, 3, i)] = make_float4(1.f, 1.f, 1.f, 1.f);
+        a.verts[vAt(a, 0, 0, 4, i)] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        const uint32_t nc = walk(sc, a, S, i, 0u, 1u, o, makeTRay(o, d, 1e-4f, __int_as_float(0x7f800000)),
+                                 V3{1.f, 1.f, 1.f}, 1.f, px, py, sidx, dim, false, tc);
+        // first-hit AOVs (BDPTIntegrator.cpp:502-512): LambertianMaterial::reflectance = albedo
+        float4 alb = make_float4(0.f, 0.f, 0.f, 0.f), nrm = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nc >= 2u) {
+            const int32_t mi = __float_as_int(a.verts[vAt(a, 0, 1, 4, i)].w);
+            alb = f4(ld3(sc.mats[mi].albedo), 1.f);
+            nrm = f4(v3(a.verts[vAt(a, 0, 1, 1, i)]), 0.f);
+        }
+        a.aovA[i] = alb;
+        a.aovN[i] = nrm;
+        // light subpath (BDPTIntegrator.cpp:514-519, 199-303)
+        uint32_t nl = 0u;
+        if (sc.numLights > 0u) {
+            // LightSampler::sample (LightSampler.h:72-80)
+            const float ul = hs(a, px, py, sidx, dim);
+            const uint32_t n = sc.numLights;
+            const uint32_t k = min((uint32_t)(ul * (float)n), n - 1u);
+            const float uf = ul * (float)n - (float)k;
+            const uint32_t li = uf < a.aliasProb[k] ? k : a.alias[k];
+            const float lpdf = a.lightW[li];
+            const DLight& L = sc.lights[li];
+            const V2 uDir{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
+            const V2 uPos{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
+            if (lpdf > 0.f) {
+                // AreaLight::sampleLe (AreaLight.h:81-105)
+                const V3 ln = ld3(L.normal), Le = ld3(L.Le);
+                const V3 pos = ld3(L.center) + ld3(L.uA) * (2.f * uPos.x - 1.f) + ld3(L.vA) * (2.f * uPos.y - 1.f);
+                const float phi = 2.f * kPi * uDir.x;
+                const float cosT = sqrtf(uDir.y), sinT = sqrtf(1.f - uDir.y);
+                float sp, cp;
+                sincosf(phi, &sp, &cp);
+                V3 t, bt;
+                onb(ln, t, bt);
+                const V3 dirL = t * (sinT * cp) + bt * (sinT * sp) + ln * cosT;
+                const float pdfPos = 1.f / L.area, pdfDir = cosT * kInvPi;
+                if (pdfPos > 0.f && pdfDir > 0.f) {
+                    const V3 beta = Le * fabsf(dot(dirL, ln)) / (lpdf * pdfPos * pdfDir);
+                    a.verts[vAt(a, 1, 0, 0, i)] = f4(pos, lpdf * pdfPos);
+                    a.verts[vAt(a, 1, 0, 1, i)] = f4(ln, 0.f);
+                    a.verts[vAt(a, 1, 0, 2, i)] = f4(dirL, __uint_as_float(kVLight));
+                    a.verts[vAt(a, 1, 0, 3, i)] = f4(Le / (lpdf * pdfPos), 1.f);
+                    a.verts[vAt(a, 1, 0, 4, i)] = f4(Le, __int_as_float((int32_t)li));
+                    nl = walk(sc, a, S, i, 1u, 1u, pos, makeTRay(pos, dirL, 1e-4f, __int_as_float(0x7f800000)), beta,
+                              pdfDir, px, py, sidx, dim, true, tc);
+                }
+            }
+        }
+        a.counts[i] = nc | (nl << 16);
+    }
+}
+
+namespace {
+
+// bdptMISWeight (MISWeight.cpp:14-72): pdfs as stored on the vertices, ratios in double
+__device__ float misWeight(const BdptArgs& a, uint32_t i, uint32_t nl, uint32_t nc, uint32_t s, uint32_t t) {
+    if (s + t == 2u) return 1.f;
+    double sum = 0.0;
+    for (uint32_t side = 0; side < 2u; ++side) {
+        const uint32_t sub = side == 0u ? 1u : 0u, cnt = side == 0u ? nl : nc, k = side == 0u ? s : t;
+        double ri = 1.0;
+        for (int j = (int)k - 1; j >= 0; --j) {
+            const bool in = j < (int)cnt;
+            const float num = in ? a.verts[vAt(a, sub, (uint32_t)j, 1, i)].w : 0.f;
+            const float den = in ? a.verts[vAt(a, sub, (uint32_t)j, 0, i)].w : 0.f;
+            if (den == 0.f) break;
+            ri *= (double)num / (double)den;
+            const bool prevDelta = j > 0 && (__float_as_uint(a.verts[vAt(a, sub, (uint32_t)j - 1u, 2, i)].w) & kVDelta);
+            const bool curDelta = in && (__float_as_uint(a.verts[vAt(a, sub, (uint32_t)j, 2, i)].w) & kVDelta);
+            if (!prevDelta && !curDelta) sum += ri * ri;
+        }
+    }
+    return (float)(1.0 / (sum + 1.0));
+}
+
+// any hit in [1e-4, tMax) of spawnRayTo(from, n, to) (Types.h:277-284): shadowTransmittance of an
+// opaque scene (ShadowRay.h:26-61)
+__device__ __forceinline__ bool visible(const DScene& sc, const BdptArgs& a, const TStack& S, V3 from, V3 n, V3 to,
+                                        TraceCounters& tc) {
+    V3 dir = to - from;
+    const float dist = len(dir);
+    dir = dir * (1.f / dist);
+    V3 off = n * 1e-4f;
+    if (dot(dir, n) < 0.f) off = -off;
+    float t, u, v;
+    return traverse<true, false>(sc, makeTRay(from + off, dir, 1e-4f, dist * (1.f - 1e-4f)), S, t, u, v, tc,
+                                 a.overflow) == ~0u;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a) {
+    __shared__ uint64_t lstk[kBdptK * kBdptBS];
+    TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
+             gridDim.x * kBdptBS, a.stackCap};
+    TraceCounters tc{0, 0};
+    for (uint32_t i = blockIdx.x * kBdptBS + threadIdx.x; i < a.slots; i += gridDim.x * kBdptBS) {
+        const uint32_t cnt = a.counts[i], nc = cnt & 0xffffu, nl = cnt >> 16;
+        V3 sum{0.f, 0.f, 0.f};
+        for (uint32_t t = 1; t <= nc; ++t) {
+            const uint32_t ct = t - 1u;
+            const float4 c0 = a.verts[vAt(a, 0, ct, 0, i)], c1 = a.verts[vAt(a, 0, ct, 1, i)];
+            const float4 c2 = a.verts[vAt(a, 0, ct, 2, i)], c3 = a.verts[vAt(a, 0, ct, 3, i)];
+            const float4 c4 = a.verts[vAt(a, 0, ct, 4, i)];
+            const uint32_t cflags = __float_as_uint(c2.w);
+            const bool cSurf = (cflags & 3u) == kVSurface;
+            const int32_t cm = cSurf ? __float_as_int(c4.w) : -1;
+            const V3 cp = v3(c0), cn = v3(c1), cwo = v3(c2), cbeta = v3(c3);
+            for (uint32_t s = 0; s <= nl; ++s) {
+                if (s + t < 2u) continue;
+                V3 C{0.f, 0.f, 0.f};
+                if (s == 0u) {
+                    // (0, t): emission at the last camera vertex (BDPTIntegrator.cpp:318-343)
+                    if (t >= 2u && cSurf) C = cbeta * v3(c4);
+                } else if (s == 1u) {
+                    // (1, t): the light endpoint toward camera vertex t-1 (:345-410)
+                    if (!(cflags & kVDelta) && cm >= 0) {
+                        const float4 l0 = a.verts[vAt(a, 1, 0, 0, i)], l1 = a.verts[vAt(a, 1, 0, 1, i)];
+                        const float4 l4 = a.verts[vAt(a, 1, 0, 4, i)];
+                        const V3 toL = v3(l0) - cp;
+                        const float dist = len(toL);
+                        if (dist >= 1e-6f) {
+                            const V3 wi = toL * (1.f / dist);
+                            if (visible(sc, a, S, cp, cn, cp + wi * dist, tc)) {
+                                float pdf;
+                                const DMat& m = sc.mats[cm];
+                                const Frame fr = makeFrame(cn, cn, -cwo);
+                                const V3 f = matEval(sc, m, fr, MatIn{ld3(m.albedo), 1.f}, cwo, wi, pdf);
+                                const V3 ln = v3(l1);
+                                const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
+                                if (!isBlack(f) && !isBlack(Le)) {
+                                    const float cosI = fabsf(dot(wi, cn)), cosL = fabsf(dot(-wi, ln));
+                                    C = cbeta * f * cosI * Le * cosL / (dist * dist * l0.w);
+                                }
+                            }
+                        }
+                    }
+                } else if (t >= 2u) {
+                    // (s, t >= 2): connect light vertex s-1 and camera vertex t-1 (:419-459)
+                    const uint32_t ls = s - 1u;
+                    const float4 l0 = a.verts[vAt(a, 1, ls, 0, i)], l1 = a.verts[vAt(a, 1, ls, 1, i)];
+                    const float4 l2 = a.verts[vAt(a, 1, ls, 2, i)], l3 = a.verts[vAt(a, 1, ls, 3, i)];
+                    const float4 l4 = a.verts[vAt(a, 1, ls, 4, i)];
+                    const uint32_t lflags = __float_as_uint(l2.w);
+                    const bool lSurf = (lflags & 3u) == kVSurface;
+                    const int32_t lm = lSurf ? __float_as_int(l4.w) : -1;
+                    if (!(lflags & kVDelta) && !(cflags & kVDelta) && lm >= 0 && cm >= 0 && l3.w >= 0.08f &&
+                        c3.w >= 0.08f && visible(sc, a, S, v3(l0), v3(l1), cp, tc)) {
+                        const V3 d = cp - v3(l0);
+                        const float dist = len(d);
+                        if (dist >= 1e-6f) {
+                            const V3 wi = d * (1.f / dist);
+                            float f1, r1, f2, r2;
+                            const V3 fL = evalVertex(sc, lm, v3(l1), v3(l2), wi, f1, r1);
+                            const V3 fC = isBlack(fL) ? V3{0.f, 0.f, 0.f} : evalVertex(sc, cm, cn, cwo, -wi, f2, r2);
+                            if (!isBlack(fC)) C = v3(l3) * fL * (1.f / (dist * dist)) * fC * cbeta;
+                        }
+                    }
+                }
+                if (isBlack(C) || !finite3(C)) continue;
+                V3 c = C * misWeight(a, i, nl, nc, s, t);
+                if (!finite3(c)) continue;
+                if (a.clamp > 0.f && s >= 2u && t >= 2u) {
+                    const float l = luminance(c);
+                    if (l > a.clamp) c = c * (a.clamp / l);
+                }
+                sum = sum + c;
+            }
+        }
+        a.sampleL[i] = f4(sum, 0.f);
+    }
+}
+
+// Per pixel of the chunk: its samples in order (BDPTIntegrator.cpp:521-580); the tile output gets
+// pixelL / spp * spp, the weight spp, the luminance squares and the AOV sums.
+__global__ __launch_bounds__(256) void k_bdpt_accum(BdptArgs a) {
+    const uint32_t npix = a.slots / a.ns;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < npix; k += gridDim.x * blockDim.x) {
+        const uint32_t pix = a.pix0 + k;
+        const uint32_t lx = pix % a.w, ly = pix / a.w;
+        V3 L{0.f, 0.f, 0.f}, A{0.f, 0.f, 0.f}, N{0.f, 0.f, 0.f};
+        float lsq = 0.f, na = 0.f;
+        for (uint32_t s = 0; s < a.ns; ++s) {
+            const uint32_t i = k * a.ns + s;
+            const V3 x = v3(a.sampleL[i]);
+            L = L + x;
+            const float l = luminance(x);
+            lsq += l * l;
+            const float4 al = a.aovA[i];
+            if (al.w > 0.f) {
+                A = A + v3(al);
+                N = N + v3(a.aovN[i]);
+                na += 1.f;
+            }
+        }
+        const float inv = 1.f / (float)a.ns, wt = (float)a.ns;
+        const size_t o = (size_t)(a.outY0 + ly) * a.outStride + (a.outX0 + lx);
+        const float4 prev = a.outAcc[o];
+        a.outAcc[o] = make_float4(prev.x + L.x * inv * wt, prev.y + L.y * inv * wt, prev.z + L.z * inv * wt,
+                                  prev.w + wt);
+        a.outLsq[o] += lsq;
+        if (a.outAov) {   // mean reflectance and mean normal of the samples that hit a surface
+            float* q = a.outAov + 6u * o;
+            if (na > 0.f) {
+                const float invN = 1.f / na;
+                const V3 ma = A * invN;
+                V3 mn = N * invN;
+                const float l = len(mn);
+                if (l > 1e-6f) mn = mn * (1.f / l);
+                q[0] += ma.x; q[1] += ma.y; q[2] += ma.z;
+                q[3] += mn.x; q[4] += mn.y; q[5] += mn.z;
+            }
+        }
+    }
+}
+
+void setHaltonTablesBdpt(const HaltonTables& t, hipStream_t s) {
+    hipMemcpyToSymbolAsync(HIP_SYMBOL(c_haltonB), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
+}
+
+void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_bdpt_paths, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    hipLaunchKernelGGL(k_bdpt_connect, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    const uint32_t npix = a.slots / a.ns;
+    hipLaunchKernelGGL(k_bdpt_accum, dim3((npix + 255u) / 256u), dim3(256), 0, s, a);
+}
+
+}  // namespace ahip

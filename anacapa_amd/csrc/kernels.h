@@ -115,6 +115,37 @@ void launchReplay(const DScene& sc, const ReplayArgs& a, bool adjoint, uint32_t 
 void launchReplayReduce(const ReplayArgs& a, float4* accum, float* lsq, hipStream_t s);
 void launchZeroF64(double* p, size_t n, hipStream_t s);
 
+// The bidirectional path tracer (bdpt.hip): a chunk of `slots` (pixel, sample) slots of a tile,
+// pixel-major (slot = pixel * ns + sample), pixels pix0.. of the tile in row-major order.
+struct BdptArgs {
+    DCamera cam;
+    uint32_t x0, y0, w, h;       // tile on the film
+    uint32_t s0, ns;             // sample range of every pixel
+    uint32_t pix0, slots;        // first pixel of the chunk, slots of the chunk (a multiple of ns)
+    uint64_t seed;
+    uint32_t maxDepth, maxV;     // BDPTIntegrator maxDepth; vertices per subpath (maxDepth + 2)
+    float clamp;                 // firefly clamp of the s, t >= 2 strategies (0 = off)
+    uint32_t cap;                // slot capacity of the stores below
+    float4* verts;               // [2 subpaths][maxV][5 fields][cap]
+    uint32_t* counts;            // [cap] camera vertices | light vertices << 16
+    float4* aovA;                // [cap] first-hit reflectance | 1 when the camera ray hit a surface
+    float4* aovN;                // [cap] first-hit normal
+    float4* sampleL;             // [cap] the sample's radiance
+    const float* aliasProb;      // LightSampler alias table (LightSampler.h:28-69)
+    const uint32_t* alias;
+    const float* lightW;         // light selection pdfs
+    uint32_t* ovfRef;            // traversal stack overflow (stride: grid threads)
+    float* ovfT;
+    uint32_t stackCap;
+    uint32_t* overflow;
+    float4* outAcc;              // output sums r, g, b (pre-weighted), weight; [rows * outStride]
+    float* outLsq;               // luminance-squared sums
+    float* outAov;               // null or [rows * outStride * 6]: mean albedo, mean normal per call
+    uint32_t outStride, outX0, outY0;
+};
+void setHaltonTablesBdpt(const HaltonTables& t, hipStream_t s);
+void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, hipStream_t s);
+
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
 uint32_t trace_block_size();

@@ -63,7 +63,7 @@ class HipPathIntegrator:
 
     # -------------------------------------------------------------- scene / settings
     def prepare(self, scene, camera: A.Camera | None = None, max_depth=None, min_depth=None, seed=None,
-                flags: int = 0, curve_depth: int = 0):
+                flags: int = 0, curve_depth: int = 0, integrator: int = 0, firefly_clamp: float = 10.0):
         desc, keep = scene.desc()
         self._check(self.lib.ahip_prepare(self._ctx, C.byref(desc)))
         self._keep = keep
@@ -73,7 +73,7 @@ class HipPathIntegrator:
         self.set_camera(camera)
         self.set_params(scene.max_depth if max_depth is None else max_depth,
                         scene.min_depth if min_depth is None else min_depth,
-                        scene.seed if seed is None else seed, flags, curve_depth)
+                        scene.seed if seed is None else seed, flags, curve_depth, integrator, firefly_clamp)
 
     def pinhole(self, frm, at, up, vfov, w, h) -> A.Camera:
         cam = A.Camera()
@@ -87,8 +87,10 @@ class HipPathIntegrator:
         self.camera = cam
         self.width, self.height = cam.width, cam.height
 
-    def set_params(self, max_depth=8, min_depth=2, seed=0, flags=0, curve_depth=0):
-        p = A.RenderParams(max_depth, min_depth, seed, flags, curve_depth)
+    def set_params(self, max_depth=8, min_depth=2, seed=0, flags=0, curve_depth=0, integrator=0, firefly_clamp=10.0):
+        """integrator: A.AHIP_INTEGRATOR_PATH or A.AHIP_INTEGRATOR_BDPT; firefly_clamp: BDPT's largest
+        luminance of one s, t >= 2 contribution (0 = off; the reference's RenderSettings default 10)."""
+        p = A.RenderParams(max_depth, min_depth, seed, flags, curve_depth, integrator, firefly_clamp)
         self._check(self.lib.ahip_set_params(self._ctx, C.byref(p)))
 
     def set_launch(self, max_paths=0, tile=0):

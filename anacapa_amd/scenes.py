@@ -294,8 +294,10 @@ def _block(mb, lo, hi, deg, mat):
     mb.add(pos, idx, mat, normals=nrm)
 
 
-def cornell(width=512, height=512, spp=64, max_depth=8, seed=0) -> Scene:
-    """The Cornell box: unit-box scene, 36 triangles, one 0.25x0.25 quad light (17,12,4)."""
+def cornell(width=512, height=512, spp=64, max_depth=8, seed=0, second_light=False) -> Scene:
+    """The Cornell box: unit-box scene, 36 triangles, one 0.25x0.25 quad light (17,12,4).
+    second_light: a dimmer 0.1x0.1 quad (4,6,9) near the back of the ceiling as well (two lights of
+    unequal power, for light selection)."""
     mats = [lambert((0.73, 0.73, 0.73)), lambert((0.63, 0.065, 0.05)), lambert((0.14, 0.45, 0.09)),
             emissive((0.0, 0.0, 0.0), (17.0, 12.0, 4.0))]
     mb = _MeshBuilder()
@@ -309,6 +311,10 @@ def cornell(width=512, height=512, spp=64, max_depth=8, seed=0) -> Scene:
     y = 0.999
     _quad(mb, (0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625), (0, -1, 0), 3)
     lights = [area_light((0.5, y, 0.5), (0.125, 0.0, 0.0), (0.0, 0.0, 0.125), (17.0, 12.0, 4.0))]
+    if second_light:
+        mats.append(emissive((0.0, 0.0, 0.0), (4.0, 6.0, 9.0)))
+        _quad(mb, (0.2, y, 0.8), (0.3, y, 0.8), (0.3, y, 0.9), (0.2, y, 0.9), (0, -1, 0), 4)
+        lights.append(area_light((0.25, y, 0.85), (0.05, 0.0, 0.0), (0.0, 0.0, 0.05), (4.0, 6.0, 9.0)))
     return mb.scene("cornell", materials=mats, lights=lights, cam_from=(0.5, 0.5, -1.4),
                     cam_at=(0.5, 0.5, 0.0), cam_up=(0.0, 1.0, 0.0), vfov=40.0, width=width,
                     height=height, spp=spp, max_depth=max_depth, seed=seed)

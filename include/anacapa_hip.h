@@ -151,14 +151,25 @@ typedef struct ahip_camera {
     uint32_t height;
 } ahip_camera;
 
+/* ahip_render_params.integrator (RenderSettings::integrator, src/render/RenderSession.h:45-47) */
+#define AHIP_INTEGRATOR_PATH 0u   /* PathIntegrator (src/integrator/PathIntegrator.cpp) */
+#define AHIP_INTEGRATOR_BDPT 1u   /* BDPTIntegrator (src/integrator/BDPTIntegrator.cpp): scenes of
+                                     triangles with Lambert / Emissive materials and area lights, no
+                                     environment; ahip_render_tile, ahip_render_tile_aovs and
+                                     ahip_render_frame (without AOVs) */
+
 typedef struct ahip_render_params {
-    uint32_t max_depth;   /* PathIntegrator maxDepth, inclusive (PathIntegrator.cpp:80) */
+    uint32_t max_depth;   /* PathIntegrator maxDepth, inclusive (PathIntegrator.cpp:80); BDPT: bounces
+                             per subpath (BDPTIntegrator.cpp:97, 240) */
     uint32_t min_depth;   /* Russian roulette from bounce >= min_depth (PathIntegrator.cpp:156) */
     uint64_t seed;        /* HaltonSampler seed (HaltonSampler.h:24) */
     uint32_t flags;       /* AHIP_FLAG_* */
     uint32_t curve_depth; /* de Casteljau depth of the ribbon test, 1..6; 0 = the reference's 6
                              (MAX_DEPTH, src/accel/CurveBrute.cpp:194); other depths are not pinned by
                              the reference, which fixes 6 */
+    uint32_t integrator;  /* AHIP_INTEGRATOR_*; 0 = the path tracer */
+    float firefly_clamp;  /* BDPT: largest luminance of one s, t >= 2 strategy's contribution, 0 = off
+                             (RenderSettings::fireflyClamp, RenderSession.h:47: the reference's default 10) */
 } ahip_render_params;
 
 /* One camera view of the batched persistent form. */

@@ -594,6 +594,7 @@ struct Material {
             if (!(wil.z * wol.z > 0.f)) return e;
             e.f = baseOf(c) * kInvPi;
             e.pdf = cosPdf(std::abs(wil.z));
+            e.pdfRev = cosPdf(std::abs(wol.z));
             return e;
         }
         // StandardSurface::evaluate opaque branch (StandardSurface.h:410-417, 448-503)
@@ -1383,6 +1384,351 @@ struct Scene {
                 o[4] = lsq;
             }
     }
+
+    // ------------------------------------------------------------ BDPT (BDPTIntegrator.cpp, MISWeight.cpp)
+    // Scenes of triangles with Lambert / Emissive materials and quad area lights, no environment:
+    // the scope the device implements.
+    struct PVert {                                                                    // PathVertex.h:28-60
+        V3 p, n, wo, beta, Le;
+        float pdfFwd = 0.f, pdfRev = 0.f, minRough = 1.f;
+        uint32_t flags = 0;
+        const Material* mat = nullptr;
+        const Light* light = nullptr;
+    };
+    static constexpr uint32_t kVCamera = 0, kVSurface = 1, kVLight = 2, kVDelta = 4, kVInf = 8;
+    struct PPath {
+        std::vector<PVert> v;
+        uint32_t count = 0;
+        float lightPdf = 1.f;
+        uint32_t type(uint32_t i) const { return v[i].flags & 3u; }
+        bool delta(uint32_t i) const { return (v[i].flags & kVDelta) != 0u; }
+        bool full() const { return count >= v.size(); }
+    };
+    // LightSampler::build / sample / pdf (LightSampler.h:28-91): Vose's alias table over power
+    struct AliasTable {
+        std::vector<float> prob, w;
+        std::vector<uint32_t> alias;
+        void build(const std::vector<Light>& L) {
+            uint32_t n = (uint32_t)L.size();
+            if (n == 0) return;
+            std::vector<float> wt(n);
+            float total = 0.f;
+            for (uint32_t i = 0; i < n; ++i) {
+                wt[i] = fmaxs(0.f, luminance(L[i].Le) * L[i].area * kPi);          // AreaLight.h:114-117
+                total += wt[i];
+            }
+            if (total <= 0.f) {
+                for (auto& x : wt) x = 1.f;
+                total = (float)n;
+            }
+            float inv = 1.f / total;
+            for (auto& x : wt) x *= inv;
+            prob.resize(n);
+            alias.assign(n, 0u);
+            std::vector<uint32_t> small, large;
+            for (uint32_t i = 0; i < n; ++i) {
+                prob[i] = wt[i] * (float)n;
+                (prob[i] < 1.f ? small : large).push_back(i);
+            }
+            while (!small.empty() && !large.empty()) {
+                uint32_t s = small.back(); small.pop_back();
+                uint32_t l = large.back(); large.pop_back();
+                alias[s] = l;
+                prob[l] = prob[l] + prob[s] - 1.f;
+                (prob[l] < 1.f ? small : large).push_back(l);
+            }
+            for (uint32_t i : small) prob[i] = 1.f;
+            for (uint32_t i : large) prob[i] = 1.f;
+            w = wt;
+        }
+        uint32_t sample(float u) const {
+            uint32_t n = (uint32_t)prob.size();
+            uint32_t i = std::min((uint32_t)(u * (float)n), n - 1u);
+            float uf = u * (float)n - (float)i;
+            return uf < prob[i] ? i : alias[i];
+        }
+    };
+    static float toArea(float pdfSA, V3 prev, V3 next, V3 nextN) {                 // PathVertex.h:147-156
+        V3 d = next - prev;
+        float d2 = lengthSq(d);
+        if (d2 < 1e-10f) return 0.f;
+        return pdfSA * std::abs(dot(safeNormalize(d), nextN)) / d2;
+    }
+    // BDPTIntegrator::evalVertex (BDPTIntegrator.cpp:40-63)
+    static V3 evalVertex(const PPath& P, uint32_t i, V3 toNext, float& fwd, float& rev) {
+        fwd = rev = 0.f;
+        if (P.type(i) != kVSurface || !P.v[i].mat) return {};
+        SI si;
+        si.p = P.v[i].p;
+        si.n = si.ng = P.v[i].n;
+        Ctx ctx(si, -P.v[i].wo);
+        BSDFEval be = P.v[i].mat->evaluate(ctx, P.v[i].wo, toNext);
+        fwd = be.pdf;
+        rev = be.pdfRev;
+        return be.f * std::abs(dot(toNext, P.v[i].n));
+    }
+    // the shared bounce loop of traceCameraSubpath / traceLightSubpath (BDPTIntegrator.cpp:97-190,
+    // 240-300); `light` selects the light subpath's differences (no emission recorded, no
+    // environment vertex)
+    void walk(Ray ray, V3 beta, float pdfFwd, Halton& smp, PPath& P, bool light) const {
+        for (uint32_t depth = 0; depth < maxDepth && !P.full(); ++depth) {
+            SI si;
+            if (!traceSI(ray, si)) break;   // no environment in the supported scenes
+            const Material* mat = matOf(si.meshID);
+            if (!mat) break;
+            V3 wo = -ray.d;
+            Ctx ctx(si, ray.d);
+            float pdfFwdArea = toArea(pdfFwd, P.v[P.count - 1].p, si.p, si.n);
+            uint32_t vi = P.count++;
+            PVert& V = P.v[vi];
+            V = PVert{};
+            V.p = si.p;
+            V.n = si.n;
+            V.wo = wo;
+            V.beta = beta;
+            V.Le = light ? V3{} : mat->Le();
+            V.pdfFwd = pdfFwdArea;
+            V.flags = kVSurface | (mat->isDelta() ? kVDelta : 0u);
+            V.mat = mat;
+            V.minRough = fmins(vi > 0 ? P.v[vi - 1].minRough : 1.f, 1.f);   // Lambert: roughness() = 1
+            if (vi > 0) {
+                float dummy, revSA;
+                evalVertex(P, vi, -wo, dummy, revSA);
+                P.v[vi - 1].pdfRev = toArea(revSA, si.p, P.v[vi - 1].p, P.v[vi - 1].n);
+            }
+            if (depth >= 2) {
+                float q = 1.f - fmins(maxComponent(beta), 0.95f);
+                if (smp.get1D() < q) break;
+                beta = beta * (1.f / (1.f - q));
+            }
+            // mat->sample(ctx, wo, get2D(), get1D()): g++ evaluates the arguments right to left
+            float uc = smp.get1D();
+            V2 u = smp.get2D();
+            BSDFSample bs = mat->sample(ctx, wo, u, uc);
+            if (!bs.valid()) break;
+            beta = beta * (bs.f / bs.pdf);
+            pdfFwd = bs.pdf;
+            ray = spawnRay(si.p, si.n, bs.wi);
+        }
+    }
+    // traceCameraSubpath (BDPTIntegrator.cpp:70-194) with the pinhole's area pdf 1
+    void cameraSubpath(const Ray& primary, Halton& smp, PPath& P) const {
+        P.count = 0;
+        PVert& V = P.v[P.count++];
+        V = PVert{};
+        V.p = primary.o;
+        V.n = -primary.d;
+        V.wo = -primary.d;
+        V.beta = {1.f, 1.f, 1.f};
+        V.pdfFwd = 1.f;
+        V.flags = kVCamera;
+        walk(primary, {1.f, 1.f, 1.f}, 1.f, smp, P, false);
+    }
+    // traceLightSubpath (BDPTIntegrator.cpp:199-303); AreaLight::sampleLe (AreaLight.h:81-105)
+    void lightSubpath(uint32_t li, float lpdf, Halton& smp, PPath& P) const {
+        P.count = 0;
+        P.lightPdf = lpdf;
+        if (lpdf <= 0.f) return;
+        const Light& L = lights[li];
+        // light->sampleLe(get2D(), get2D()): g++ evaluates the arguments right to left
+        V2 uDir = smp.get2D();
+        V2 uPos = smp.get2D();
+        V3 pos = L.center + L.uA * (2.f * uPos.x - 1.f) + L.vA * (2.f * uPos.y - 1.f);
+        float phi = 2.f * kPi * uDir.x;
+        float cosT = std::sqrt(uDir.y), sinT = std::sqrt(1.f - uDir.y);
+        V3 t, bt;
+        onb(L.normal, t, bt);
+        V3 dir = t * (sinT * std::cos(phi)) + bt * (sinT * std::sin(phi)) + L.normal * cosT;
+        float pdfPos = 1.f / L.area, pdfDir = cosT * kInvPi;
+        if (pdfPos <= 0.f || pdfDir <= 0.f) return;
+        V3 beta = L.Le * std::abs(dot(dir, L.normal)) / (lpdf * pdfPos * pdfDir);
+        PVert& V = P.v[P.count++];
+        V = PVert{};
+        V.p = pos;
+        V.n = L.normal;
+        V.wo = dir;
+        V.beta = L.Le / (lpdf * pdfPos);
+        V.Le = L.Le;
+        V.pdfFwd = lpdf * pdfPos;
+        V.flags = kVLight;
+        V.light = &L;
+        Ray ray{pos, dir};
+        ray.tMin = 1e-4f;
+        walk(ray, beta, pdfDir, smp, P, true);
+    }
+    // bdptMISWeight (MISWeight.cpp:14-72), in double as there
+    static float misWeight(const PPath& lp, const PPath& cp, uint32_t s, uint32_t t) {
+        if (s + t == 2) return 1.f;
+        double sum = 0.0, ri = 1.0;
+        for (int i = (int)s - 1; i >= 0; --i) {
+            float num = i < (int)lp.count ? lp.v[i].pdfRev : 0.f;
+            float den = i < (int)lp.count ? lp.v[i].pdfFwd : 0.f;
+            if (den == 0.f) break;
+            ri *= (double)num / (double)den;
+            bool prevDelta = i > 0 && lp.delta(i - 1);
+            bool curDelta = i < (int)lp.count && lp.delta(i);
+            if (!prevDelta && !curDelta) sum += ri * ri;
+        }
+        ri = 1.0;
+        for (int i = (int)t - 1; i >= 0; --i) {
+            float num = i < (int)cp.count ? cp.v[i].pdfRev : 0.f;
+            float den = i < (int)cp.count ? cp.v[i].pdfFwd : 0.f;
+            if (den == 0.f) break;
+            ri *= (double)num / (double)den;
+            bool prevDelta = i > 0 && cp.delta(i - 1);
+            bool curDelta = i < (int)cp.count && cp.delta(i);
+            if (!prevDelta && !curDelta) sum += ri * ri;
+        }
+        return (float)(1.0 / (sum + 1.0));
+    }
+    // shadowTransmittance of a connection (opaque scenes: visible or not).  A connection to a light
+    // point ends 1e-4 short of a surface that is itself geometry (the emitter's own mesh), so its
+    // verdict can hang on the last bits of the hit distance; `border` counts the verdicts that change
+    // between tMax * (1 - 4e-7) and tMax * (1 + 4e-7), where builds that round differently can
+    // disagree.
+    mutable uint32_t border = 0;
+    bool connVisible(const Ray& r) const {
+        bool vis = !isBlack(shadowTr(r));
+        Ray a = r, b = r;
+        a.tMax = r.tMax * (1.f - 4e-7f);
+        b.tMax = r.tMax * (1.f + 4e-7f);
+        if (isBlack(shadowTr(a)) != isBlack(shadowTr(b))) ++border;
+        return vis;
+    }
+    // BDPTIntegrator::connect (BDPTIntegrator.cpp:308-459) without film splats (t = 1 is unused there)
+    V3 connect(const PPath& lp, uint32_t s, const PPath& cp, uint32_t t) const {
+        if (t == 0) return {};
+        if (s == 0) {
+            if (t < 2) return {};
+            uint32_t last = t - 1;
+            if (cp.type(last) == kVSurface) return cp.v[last].beta * cp.v[last].Le;
+            return {};   // light-type camera vertices are environment vertices: none here
+        }
+        if (s == 1) {
+            uint32_t ct = t - 1;
+            if (cp.delta(ct) || !cp.v[ct].mat) return {};
+            const Light* light = lp.v[0].light;
+            if (!light) return {};
+            V3 toL = lp.v[0].p - cp.v[ct].p;
+            float dist = length(toL);
+            if (dist < 1e-6f) return {};
+            V3 wi = toL * (1.f / dist);
+            Ray sr = spawnRayTo(cp.v[ct].p, cp.v[ct].n, cp.v[ct].p + wi * dist);
+            if (!connVisible(sr)) return {};
+            SI si;
+            si.p = cp.v[ct].p;
+            si.n = si.ng = cp.v[ct].n;
+            Ctx ctx(si, -cp.v[ct].wo);
+            BSDFEval be = cp.v[ct].mat->evaluate(ctx, cp.v[ct].wo, wi);
+            if (isBlack(be.f)) return {};
+            V3 Le = dot(wi, lp.v[0].n) < 0.f ? light->Le : V3{};                     // AreaLight.h:110-112
+            if (isBlack(Le)) return {};
+            float cosI = std::abs(dot(wi, cp.v[ct].n));
+            float cosL = std::abs(dot(-wi, lp.v[0].n));
+            float d2 = dist * dist;
+            return cp.v[ct].beta * be.f * cosI * Le * cosL * V3{1.f, 1.f, 1.f} / (d2 * lp.v[0].pdfFwd);
+        }
+        if (t == 1) return {};
+        uint32_t ls = s - 1, ct = t - 1;
+        if (lp.delta(ls) || cp.delta(ct)) return {};
+        if (!lp.v[ls].mat || !cp.v[ct].mat) return {};
+        if (lp.v[ls].minRough < 0.08f || cp.v[ct].minRough < 0.08f) return {};
+        Ray sr = spawnRayTo(lp.v[ls].p, lp.v[ls].n, cp.v[ct].p);
+        if (!connVisible(sr)) return {};
+        V3 d = cp.v[ct].p - lp.v[ls].p;
+        float dist = length(d);
+        if (dist < 1e-6f) return {};
+        V3 wi = d * (1.f / dist);
+        float fL1, rL1, fC1, rC1;
+        V3 fL = evalVertex(lp, ls, wi, fL1, rL1);
+        if (isBlack(fL)) return {};
+        V3 fC = evalVertex(cp, ct, -wi, fC1, rC1);
+        if (isBlack(fC)) return {};
+        float invD2 = 1.f / (dist * dist);
+        return lp.v[ls].beta * fL * invD2 * fC * cp.v[ct].beta * V3{1.f, 1.f, 1.f};
+    }
+    // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581): out[w*h*5] as renderTile above
+    // border (optional, [w*h*ns]): 1 where a connection's visibility of the sample hangs on the last
+    // bits of its hit distance (connVisible).
+    void renderTileBDPT(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float clamp,
+                        float* out, float* albedo = nullptr, float* normal = nullptr,
+                        uint8_t* borderOut = nullptr) const {
+        AliasTable ls;
+        ls.build(lights);
+        Halton smp;
+        smp.seed = seed;
+        PPath cp, lp;
+        cp.v.resize(maxDepth + 2);
+        lp.v.resize(maxDepth + 2);
+        for (uint32_t ty = 0; ty < h; ++ty)
+            for (uint32_t tx = 0; tx < w; ++tx) {
+                uint32_t px = x0 + tx, py = y0 + ty;
+                V3 pixL, accA, accN;
+                uint32_t aovCount = 0;
+                float lsq = 0.f;
+                for (uint32_t s = 0; s < ns; ++s) {
+                    smp.start(px, py, s0 + s);
+                    V2 jit = smp.get2D();
+                    smp.get2D();   // lens
+                    smp.get1D();   // time
+                    Ray primary = generateRay(cam, px, py, jit.x, jit.y);
+                    border = 0;
+                    cameraSubpath(primary, smp, cp);
+                    if (cp.count >= 2 && cp.type(1) == kVSurface && cp.v[1].mat) {
+                        SI si;
+                        si.p = cp.v[1].p;
+                        si.n = si.ng = cp.v[1].n;
+                        Ctx ctx(si, -cp.v[1].wo);
+                        accA = accA + cp.v[1].mat->reflectance(ctx);
+                        accN = accN + cp.v[1].n;
+                        ++aovCount;
+                    }
+                    uint32_t li = 0;
+                    if (!lights.empty()) li = ls.sample(smp.get1D());
+                    lightSubpath(li, lights.empty() ? 0.f : ls.w[li], smp, lp);
+                    V3 sL;
+                    for (uint32_t t = 1; t <= cp.count; ++t)
+                        for (uint32_t s2 = 0; s2 <= lp.count; ++s2) {
+                            if (s2 + t < 2) continue;
+                            V3 C = connect(lp, s2, cp, t);
+                            if (isBlack(C) || !isFinite(C)) continue;
+                            float wgt = misWeight(lp, cp, s2, t);
+                            V3 c = C * wgt;
+                            if (!isFinite(c)) continue;
+                            if (clamp > 0.f && s2 >= 2 && t >= 2) {
+                                float lum = luminance(c);
+                                if (lum > clamp) c = c * (clamp / lum);
+                            }
+                            sL = sL + c;
+                        }
+                    pixL = pixL + sL;
+                    float lum = luminance(sL);
+                    lsq += lum * lum;
+                    if (borderOut) borderOut[(ty * w + tx) * ns + s] = border ? 1u : 0u;
+                }
+                float inv = 1.f / (float)ns, wt = (float)ns;
+                float* o = out + 5 * (ty * w + tx);
+                o[0] = pixL.x * inv * wt;
+                o[1] = pixL.y * inv * wt;
+                o[2] = pixL.z * inv * wt;
+                o[3] = wt;
+                o[4] = lsq;
+                if (albedo) {
+                    V3 ma, mn;
+                    if (aovCount > 0) {
+                        float invN = 1.f / (float)aovCount;
+                        ma = accA * invN;
+                        mn = accN * invN;
+                        float len = length(mn);
+                        if (len > 1e-6f) mn = mn * (1.f / len);
+                    }
+                    float* A = albedo + 3 * (ty * w + tx);
+                    float* N = normal + 3 * (ty * w + tx);
+                    A[0] = ma.x; A[1] = ma.y; A[2] = ma.z;
+                    N[0] = mn.x; N[1] = mn.y; N[2] = mn.z;
+                }
+            }
+    }
 };
 
 }  // namespace orc
@@ -1615,6 +1961,22 @@ int orc_occluded(void* h, const float* rays, uint32_t n, uint8_t* out) {
 int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
                          float* out, float* albedo, float* normal) {
     static_cast<Scene*>(h)->renderTile(x0, y0, w, hh, s0, ns, out, albedo, normal);
+    return 0;
+}
+
+// BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581) with firefly clamp `clamp` (0 = off):
+// out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and border [w*h*ns] (Scene::connVisible)
+// may be null.  Scenes of triangles with
+// Lambert / Emissive materials and area lights only (returns 1 otherwise).
+int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
+                         float clamp, float* out, float* albedo, float* normal, uint8_t* border) {
+    const Scene* s = static_cast<Scene*>(h);
+    if (!s->curves.nodes.empty() || s->hasEnv || !isBlack(s->envRad)) return 1;
+    for (const Light& L : s->lights)
+        if (L.type != AHIP_LIGHT_AREA) return 1;
+    for (const Material& m : s->mats)
+        if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE) return 1;
+    s->renderTileBDPT(x0, y0, w, hh, s0, ns, clamp, out, albedo, normal, border);
     return 0;
 }
 

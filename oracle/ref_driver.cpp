@@ -53,6 +53,7 @@
 #include "shading/lights/DomeLight.h"   // Distribution1D; the dome's private sampling / lookup
 #undef private
 #include "integrator/PathIntegrator.h"
+#include "integrator/BDPTIntegrator.h"
 #include "sampling/HaltonSampler.h"
 #include "sampling/PCGRng.h"
 #include "shading/Lambertian.h"
@@ -206,6 +207,32 @@ int ref_render_tile(void* h, const ahip_camera* cam, uint32_t x0, uint32_t y0, u
     TileRequest tile{x0, y0, w, hh, sampleStart, sampleCount};
     integ.renderTile(s->view, tile, cam->width, cam->height, *sampler, tb);
     writeTile(tb, out);
+    return 0;
+}
+
+// BDPTIntegrator(maxDepth, fireflyClamp)::renderTile on the same scene (RenderSession.cpp:459-461
+// constructs it so), out[w*h*5] as ref_render_tile; albedo / normal [w*h*3] the TileBuffer AOV sums.
+int ref_render_tile_bdpt(void* h, const ahip_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh,
+                         uint32_t sampleStart, uint32_t sampleCount, uint32_t maxDepth, uint64_t seed, float clamp,
+                         float* out, float* albedo, float* normal) {
+    auto* s = static_cast<RefScene*>(h);
+    s->view.camera = toCamera(cam);
+    BDPTIntegrator integ(maxDepth, clamp);
+    integ.prepare(s->view);
+    HaltonSampler base(sampleCount, seed);
+    auto sampler = base.clone();
+    TileBuffer tb(x0, y0, w, hh);
+    TileRequest tile{x0, y0, w, hh, sampleStart, sampleCount};
+    integ.renderTile(s->view, tile, cam->width, cam->height, *sampler, tb);
+    writeTile(tb, out);
+    for (uint32_t i = 0; albedo && i < w * hh; ++i) {
+        albedo[3 * i + 0] = tb.albedo[i].r;
+        albedo[3 * i + 1] = tb.albedo[i].g;
+        albedo[3 * i + 2] = tb.albedo[i].b;
+        normal[3 * i + 0] = tb.normals[i].r;
+        normal[3 * i + 1] = tb.normals[i].g;
+        normal[3 * i + 2] = tb.normals[i].b;
+    }
     return 0;
 }
 

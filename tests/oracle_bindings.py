@@ -146,6 +146,7 @@ class Oracle(_ShadingProbes):
         L.orc_trace.argtypes = [VP, VP, C.c_uint32, VP, VP, VP]
         L.orc_occluded.argtypes = [VP, VP, C.c_uint32, VP]
         L.orc_render_tile.argtypes = [VP] + [C.c_uint32] * 6 + [VP]
+        L.orc_render_tile_bdpt.argtypes = [VP] + [C.c_uint32] * 6 + [C.c_float, VP, VP, VP, VP]
         L.orc_render_tile_aovs.argtypes = [VP] + [C.c_uint32] * 6 + [VP, VP, VP]
         L.orc_render_samples.argtypes = [VP, C.c_uint32, VP, VP, VP, VP, VP]
         L.orc_render_frame.argtypes = [VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP]
@@ -304,6 +305,19 @@ class OracleScene:
         self.orc.lib.orc_render_tile_aovs(self.h, x0, y0, w, h, s0, ns, P(out), P(alb), P(nrm))
         return out, alb, nrm
 
+    def render_tile_bdpt(self, x0, y0, w, h, s0, ns, clamp=10.0, border=False):
+        """BDPTIntegrator::renderTile: (accum [h, w, 5], albedo [h, w, 3], normal [h, w, 3]) and with
+        border=True also [h, w, ns] flags of the samples whose connection visibility hangs on the
+        last bits of a hit distance (Scene::connVisible)."""
+        out = np.zeros((h, w, 5), np.float32)
+        alb = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        bd = np.zeros((h, w, ns), np.uint8)
+        st = self.orc.lib.orc_render_tile_bdpt(self.h, x0, y0, w, h, s0, ns, clamp, P(out), P(alb), P(nrm), P(bd))
+        if st:
+            raise ValueError("the oracle's BDPT takes Lambert / Emissive triangles and area lights only")
+        return (out, alb, nrm, bd) if border else (out, alb, nrm)
+
     def set_eval(self, tri_albedo=None, texture=0, texels=None):
         """Evaluation parameters of the detached replay estimator (None = sampling values)."""
         self._eval_keep = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)
@@ -375,6 +389,8 @@ class Reference(_ShadingProbes):
         L.ref_occluded.argtypes = [VP, VP, C.c_uint32, VP]
         L.ref_render_tile.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP]
         L.ref_render_tile_aovs.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP, VP, VP]
+        L.ref_render_tile_bdpt.argtypes = ([VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 +
+                                           [C.c_uint64, C.c_float, VP, VP, VP])
         L.ref_render_frame.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
                                        C.c_uint32, C.c_uint32, VP, VP, VP]
         L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
@@ -521,6 +537,17 @@ class RefScene:
         self.ref.lib.ref_render_tile_aovs(self.h, C.byref(cam or self.cam), x0, y0, w, h, s0, ns,
                                           sc.max_depth if max_depth is None else max_depth,
                                           sc.seed if seed is None else seed, P(out), P(alb), P(nrm))
+        return out, alb, nrm
+
+    def render_tile_bdpt(self, x0, y0, w, h, s0, ns, clamp=10.0, max_depth=None, seed=None, cam=None):
+        """The reference's BDPTIntegrator(maxDepth, clamp)::renderTile: (accum, albedo sums, normal sums)."""
+        out = np.zeros((h, w, 5), np.float32)
+        alb = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        sc = self.sc
+        self.ref.lib.ref_render_tile_bdpt(self.h, C.byref(cam or self.cam), x0, y0, w, h, s0, ns,
+                                          sc.max_depth if max_depth is None else max_depth,
+                                          sc.seed if seed is None else seed, clamp, P(out), P(alb), P(nrm))
         return out, alb, nrm
 
     def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None):

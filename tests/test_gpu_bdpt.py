@@ -1,0 +1,100 @@
+"""The device BDPT (bdpt.hip) against the oracle's restatement of BDPTIntegrator (itself held to the
+reference's own build by tests/test_oracle_bdpt.py), on the Cornell box with one and with two lights.
+
+Per sample (each sample index rendered on its own on both sides): every sample within 2e-4
+relative except samples whose connection visibility hangs on the last bits of a hit distance (the
+oracle flags them: Scene::connVisible); measured on the MI355X: 20 of 16384 samples beyond 1e-2, all
+flagged, the largest error of an unflagged sample 1.3e-4.  Sample weights exact, AOVs as the
+reference records them (first-vertex albedo exact, normals within 1e-6)."""
+import numpy as np
+import pytest
+
+from anacapa_amd import _abi as A
+from anacapa_amd import scenes
+
+pytestmark = pytest.mark.gpu
+NS = 4
+BDPT = A.AHIP_INTEGRATOR_BDPT
+
+
+@pytest.mark.parametrize("two_lights", [False, True])
+def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights):
+    sc = scenes.cornell(width=64, height=64, second_light=two_lights)
+    os_ = oracle.scene(sc)
+    gpu.prepare(sc, integrator=BDPT)
+    print(f"[bdpt] cornell 64x64 x {NS} spp, two lights {two_lights}", flush=True)
+    got, alb, nrm = gpu.render_tile_aovs(0, 0, 64, 64, 0, NS)
+    ref, oalb, onrm, bd = os_.render_tile_bdpt(0, 0, 64, 64, 0, NS, border=True)
+    assert np.array_equal(got[..., 3], ref[..., 3])
+    assert abs(got[..., :3].astype(np.float64).mean() / ref[..., :3].astype(np.float64).mean() - 1) < 2e-4
+    np.testing.assert_array_equal(alb, oalb)
+    np.testing.assert_allclose(nrm, onrm, rtol=0, atol=1e-6)
+    g = np.stack([gpu.render_tile(0, 0, 64, 64, s, 1)[..., :3] for s in range(NS)]).astype(np.float64)
+    r = np.stack([os_.render_tile_bdpt(0, 0, 64, 64, s, 1)[0][..., :3] for s in range(NS)]).astype(np.float64)
+    np.testing.assert_allclose(g.sum(0), got[..., :3], rtol=1e-5, atol=1e-6)   # the samples add up to the tile
+    border = np.moveaxis(bd, -1, 0).astype(bool)
+    scale = max(np.abs(r).mean(), 1e-6)
+    e = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
+    off = e > 2e-4
+    print(f"[bdpt] {int(off.sum())} of {e.size} samples beyond 2e-4, {int((off & border).sum())} flagged; "
+          f"largest unflagged {e[~border].max():.3g}", flush=True)
+    assert not (off & ~border).any()
+    assert off.sum() <= 3e-3 * e.size
+
+
+def test_bdpt_frame_equals_tile_and_clamp(gpu, oracle):
+    """ahip_render_frame with the BDPT integrator adds the same sums as one tile over the film, and the
+    firefly clamp parameter reaches the kernels: a clamp of 0.05 (low enough to cap many s, t >= 2
+    strategies) and 0 (off), each against the oracle's."""
+    sc = scenes.cornell(width=96, height=80)
+    os_ = oracle.scene(sc)
+    for clamp in (0.05, 0.0):
+        gpu.prepare(sc, integrator=BDPT, firefly_clamp=clamp)
+        tile = gpu.render_tile(0, 0, 96, 80, 0, 2)
+        gpu.clear_accum()
+        gpu.render_frame(0, 2)
+        frame = gpu.download_accum()
+        assert np.array_equal(frame, tile), clamp
+        ref = os_.render_tile_bdpt(0, 0, 96, 80, 0, 2, clamp)[0]
+        assert abs(tile[..., :3].astype(np.float64).mean() / ref[..., :3].astype(np.float64).mean() - 1) < 1e-3, clamp
+        if clamp:
+            clamped = tile
+    assert clamped[..., :3].mean() < 0.99 * tile[..., :3].mean()
+
+
+def test_bdpt_refusals(gpu):
+    sc = scenes.ggx_spheres(width=16, height=16, spp=1, grid=2, freq=2, hdri_size=(16, 8))
+    gpu.prepare(sc, integrator=BDPT)
+    with pytest.raises(Exception):
+        gpu.render_tile(0, 0, 8, 8, 0, 1)
+    sc = scenes.cornell(width=32, height=32)
+    gpu.prepare(sc, integrator=BDPT)
+    gpu.render_tile(0, 0, 8, 8, 0, 1)
+    cam = gpu.camera
+    with pytest.raises(Exception):
+        gpu.render_batch([cam], [0], [1], sc.max_depth)
+    with pytest.raises(Exception):
+        gpu.set_params(sc.max_depth, sc.min_depth, sc.seed, integrator=7)
+    with pytest.raises(Exception):
+        gpu.set_params(sc.max_depth, sc.min_depth, sc.seed, integrator=BDPT, firefly_clamp=-1.0)
+    gpu.set_params(sc.max_depth, sc.min_depth, sc.seed)   # back to the path tracer
+    gpu.render_batch([cam], [0], [1], sc.max_depth)
+
+
+def test_bdpt_cornell_frame_bands_vs_oracle(gpu, oracle):
+    """The Cornell frame at its configured size (512x512 x 64 spp, depth 8): two 4-row bands against
+    the oracle's render of the same rows, sample weights exact and band means within 1e-3."""
+    sc = scenes.cornell()
+    os_ = oracle.scene(sc)
+    gpu.prepare(sc, integrator=BDPT)
+    gpu.clear_accum()
+    gpu.render_frame(0, sc.spp)
+    frame = gpu.download_accum()
+    for y0 in (128, 384):
+        print(f"[bdpt] cornell 512x512 x 64 spp, rows {y0}..{y0 + 3}", flush=True)
+        ref = os_.render_tile_bdpt(0, y0, 512, 4, 0, sc.spp)[0]
+        got = frame[y0:y0 + 4]
+        assert np.array_equal(got[..., 3], ref[..., 3])
+        m, mr = got[..., :3].astype(np.float64).mean(), ref[..., :3].astype(np.float64).mean()
+        print(f"[bdpt] band means {m:.6f} / {mr:.6f}", flush=True)
+        assert abs(m / mr - 1) < 1e-3
