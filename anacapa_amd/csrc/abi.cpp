@@ -894,7 +894,9 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
         return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive materials and area lights; "
                                              "the scene has " + ctx->bdptRefusal);
     if (ns == 0u) return AHIP_OK;
-    int st = ensureQueues(ctx, 64);
+    int st = checkInjected(ctx);
+    if (st) return st;
+    st = ensureQueues(ctx, 64);
     if (st) return st;
     // about 1M slots per chunk: 2 x (maxDepth + 2) vertices of 80 B each per slot
     const size_t maxV = (size_t)ctx->params.max_depth + 2u;
@@ -938,6 +940,7 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
     a.outX0 = outX0;
     a.outY0 = outY0;
     launchFillU32(a.overflow, 1, 0u, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->renderBeg, ctx->stream));
     const size_t npix = (size_t)w * h;
     for (size_t p0 = 0; p0 < npix; p0 += pixPerChunk) {
         a.pix0 = (uint32_t)p0;
@@ -946,10 +949,15 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
         const uint32_t blocks = std::min<uint32_t>(ctx->traceBlocks, (a.slots + 127u) / 128u);
         launchBdpt(ctx->dsc, a, blocks, ctx->stream);
     }
+    HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
     uint32_t ovf = 0;
     HIPCHK(hipMemcpyAsync(&ovf, a.overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ovf) return fail(ctx, AHIP_E_HIP, "traversal stack capacity exceeded");
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->renderBeg, ctx->renderEnd));
+    ctx->stats.render_ms = ms;
+    ctx->stats.samples += (uint64_t)npix * ns;
     return AHIP_OK;
 }
 

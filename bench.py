@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--no-configs", action="store_true", help="skip the Cornell and hair frames")
+    ap.add_argument("--no-bdpt", action="store_true", help="skip the BDPT Cornell frame")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one elementwise copy of this many MiB (read + write), the known-size kernel "
                          "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -141,6 +142,10 @@ def main():
     if not args.no_configs:
         others = other_configs(be, log)
 
+    bdpt = None
+    if not args.no_bdpt:
+        bdpt = bdpt_form(be, args, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc, args)
@@ -168,6 +173,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "other_configs": others,
+        "bdpt": bdpt,
     }
     print(json.dumps(out), flush=True)
 
@@ -419,6 +425,54 @@ def other_configs(be, log):
                      "ms_per_frame": round(t, 3), "msamples_per_s": round(samples / t / 1e3, 2),
                      "mrays_per_s": round(rays / t / 1e3, 2)}
         log(f"[bench] {name}: {t:.1f} ms/frame, {samples / t / 1e3:.1f} Msamples/s")
+    return out
+
+
+def bdpt_form(be, args, log):
+    """The bidirectional path tracer on the Cornell box at the configuration of the reference's published
+    BDPT timing (400x400, 64 spp, depth 8: ~1168 ms on an Apple M3 Pro with 11 threads, README.md:444-448),
+    best of two timed frames after a warm-up, and the reference's own BDPTIntegrator (oracle/_ref) on
+    the host threads through RenderSession's tile loop for the same frame, when that build is present."""
+    from anacapa_amd import _abi as A
+    from anacapa_amd import scenes
+    sc = scenes.cornell(width=400, height=400, spp=64)
+    be.prepare(sc, integrator=A.AHIP_INTEGRATOR_BDPT, firefly_clamp=10.0)
+    be.clear_accum()
+    be.render_frame(0, sc.spp)
+    ms = []
+    for _ in range(2):
+        be.reset_stats()
+        be.clear_accum()
+        be.render_frame(0, sc.spp)
+        ms.append(be.stats()["render_ms"])
+    t = min(ms)
+    samples = sc.width * sc.height * sc.spp
+    out = {"workload": "cornell: 36 tris, 400x400, 64 spp, depth 8, firefly clamp 10 (BDPTIntegrator)",
+           "ms_per_frame": round(t, 3), "msamples_per_s": round(samples / t / 1e3, 2),
+           "published_cpu": {"ms_per_frame": 1168, "host": "Apple M3 Pro, 11 threads", "source": "README.md:444-448"}}
+    log(f"[bench] bdpt: {t:.1f} ms/frame, {samples / t / 1e3:.1f} Msamples/s")
+    be.set_params(sc.max_depth, sc.min_depth, sc.seed)   # back to the path tracer
+    if args.no_cpu_baseline:
+        return out
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        from oracle_bindings import Reference, reference_available
+        if not reference_available():
+            return out
+        rs = Reference().scene(sc)
+    except Exception as e:  # noqa: BLE001
+        log(f"[bench] bdpt cpu reference unavailable: {e}")
+        return out
+    threads = min(host_info()["hardware_concurrency"], 16)
+    cspp = 16   # a quarter of the frame's samples keeps the CPU leg to seconds
+    _, sec, used = rs.render_frame_bdpt(cspp, 10.0, threads=threads)
+    csamples = sc.width * sc.height * cspp
+    out["cpu_reference"] = {"msamples_per_s": round(csamples / sec / 1e6, 3),
+                            "ms_per_frame": round(sec * 1e3 * sc.spp / cspp, 1), "threads": used, "kind": "reference",
+                            "sample": f"the same frame at {cspp} spp ({csamples} samples, {sec:.2f}s), ms_per_frame "
+                                      f"scaled to {sc.spp} spp",
+                            "build": "oracle/_ref: BDPTIntegrator.cpp + MISWeight.cpp at the reference's Release flags"}
+    log(f"[bench] bdpt cpu reference: {csamples / sec / 1e6:.2f} Msamples/s on {used} threads")
     return out
 
 

@@ -307,6 +307,45 @@ int ref_render_frame(void* h, const ahip_camera* cam, uint32_t spp, uint32_t max
     return 0;
 }
 
+// ref_render_frame with BDPTIntegrator(maxDepth, clamp) (RenderSession.cpp:459-461) as the integrator.
+int ref_render_frame_bdpt(void* h, const ahip_camera* cam, uint32_t spp, uint32_t maxDepth, uint64_t seed, float clamp,
+                          uint32_t nthreads, uint32_t tileSize, float* out, double* seconds, uint32_t* threadsUsed) {
+    auto* s = static_cast<RefScene*>(h);
+    s->view.camera = toCamera(cam);
+    const uint32_t W = cam->width, H = cam->height;
+    BDPTIntegrator integ(maxDepth, clamp);
+    integ.prepare(s->view);
+    HaltonSampler base(spp, seed);
+    ThreadPool pool(nthreads);
+    if (threadsUsed) *threadsUsed = pool.numThreads();
+    std::vector<TileRequest> tiles;
+    for (uint32_t y = 0; y < H; y += tileSize)
+        for (uint32_t x = 0; x < W; x += tileSize)
+            tiles.push_back({x, y, std::min(tileSize, W - x), std::min(tileSize, H - y), 0, spp});
+    std::memset(out, 0, sizeof(float) * W * H * 5);
+    std::mutex mergeMutex;
+    auto t0 = std::chrono::steady_clock::now();
+    pool.parallelFor((uint32_t)tiles.size(), [&](uint32_t ti) {
+        const TileRequest& t = tiles[ti];
+        auto sampler = base.clone();
+        TileBuffer tb(t.x0, t.y0, t.width, t.height);
+        integ.renderTile(s->view, t, W, H, *sampler, tb);
+        std::lock_guard<std::mutex> lk(mergeMutex);
+        for (uint32_t ty = 0; ty < t.height; ++ty)
+            for (uint32_t tx = 0; tx < t.width; ++tx) {
+                uint32_t fi = (t.y0 + ty) * W + (t.x0 + tx), li = ty * t.width + tx;
+                out[fi * 5 + 0] += tb.pixels[li].r;
+                out[fi * 5 + 1] += tb.pixels[li].g;
+                out[fi * 5 + 2] += tb.pixels[li].b;
+                out[fi * 5 + 3] += tb.pixels[li].weight;
+                out[fi * 5 + 4] += tb.sumLumSq[li];
+            }
+    });
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
 // Distribution1D (DomeLight.h:19-55): build over values, then sample at us[k].
 int ref_distribution1d(const float* values, uint32_t count, uint32_t nsamples, const float* us,
                        uint32_t* idx, float* pdf, float* uRemapped, float* cdfOut, float* integral) {

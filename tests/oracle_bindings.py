@@ -391,6 +391,8 @@ class Reference(_ShadingProbes):
         L.ref_render_tile_aovs.argtypes = [VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 + [C.c_uint64, VP, VP, VP]
         L.ref_render_tile_bdpt.argtypes = ([VP, C.POINTER(A.Camera)] + [C.c_uint32] * 7 +
                                            [C.c_uint64, C.c_float, VP, VP, VP])
+        L.ref_render_frame_bdpt.argtypes = ([VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64, C.c_float,
+                                             C.c_uint32, C.c_uint32, VP, VP, VP])
         L.ref_render_frame.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
                                        C.c_uint32, C.c_uint32, VP, VP, VP]
         L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
@@ -549,6 +551,16 @@ class RefScene:
                                           sc.max_depth if max_depth is None else max_depth,
                                           sc.seed if seed is None else seed, clamp, P(out), P(alb), P(nrm))
         return out, alb, nrm
+
+    def render_frame_bdpt(self, spp, clamp=10.0, threads=0, tile=64, max_depth=None, cam=None):
+        """The frame through RenderSession's tile loop with BDPTIntegrator: (accum, seconds, threads)."""
+        cam = cam or self.cam
+        out = np.zeros((cam.height, cam.width, 5), np.float32)
+        sec = np.zeros(1, np.float64)
+        used = np.zeros(1, np.uint32)
+        self.ref.lib.ref_render_frame_bdpt(self.h, C.byref(cam), spp, self.sc.max_depth if max_depth is None else max_depth,
+                                           self.sc.seed, clamp, threads, tile, P(out), P(sec), P(used))
+        return out, float(sec[0]), int(used[0])
 
     def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None):
         cam = cam or self.cam
