@@ -222,6 +222,9 @@ __device__ float misWeight(const BdptArgs& a, uint32_t i, uint32_t nl, uint32_t 
             const float den = in ? a.verts[vAt(a, sub, (uint32_t)j, 0, i)].w : 0.f;
             if (den == 0.f) break;
             ri *= (double)num / (double)den;
+            // num / den of two floats with den != 0 is finite in double, so once the ratio product is
+            // 0 every later term is 0 (a NaN product is kept to the end, as the reference's loop does)
+            if (ri == 0.0) break;
             const bool prevDelta = j > 0 && (__float_as_uint(a.verts[vAt(a, sub, (uint32_t)j - 1u, 2, i)].w) & kVDelta);
             const bool curDelta = in && (__float_as_uint(a.verts[vAt(a, sub, (uint32_t)j, 2, i)].w) & kVDelta);
             if (!prevDelta && !curDelta) sum += ri * ri;
