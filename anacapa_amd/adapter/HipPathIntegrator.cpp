@@ -70,7 +70,8 @@ bool toAhip(const IMaterial* mat, ahip_material& m) {
 
 }  // namespace
 
-HipPathIntegrator::HipPathIntegrator(int device, uint32_t maxDepth) : m_maxDepth(maxDepth) {
+HipPathIntegrator::HipPathIntegrator(int device, uint32_t maxDepth, uint32_t integrator, float fireflyClamp)
+    : m_maxDepth(maxDepth), m_integrator(integrator), m_fireflyClamp(fireflyClamp) {
     if (ahip_create(device, &m_ctx) != AHIP_OK)
         std::fprintf(stderr, "[warn] HipPathIntegrator: %s\n", ahip_last_error(m_ctx));
 }
@@ -184,6 +185,13 @@ void HipPathIntegrator::prepare(const SceneView& scene) {
         std::fprintf(stderr, "[warn] HipPathIntegrator::prepare: %s\n", ahip_last_error(m_ctx));
         return;
     }
+    // the integrator is checked against the scene now (ahip_set_params refuses BDPT for scenes it
+    // does not take), so that RenderSession can keep its CPU integrator
+    const ahip_render_params p{m_maxDepth, 2, 0, 0, 0, m_integrator, m_fireflyClamp};
+    if (ahip_set_params(m_ctx, &p) != AHIP_OK) {
+        std::fprintf(stderr, "[warn] HipPathIntegrator::prepare: %s\n", ahip_last_error(m_ctx));
+        return;
+    }
     m_prepared = true;
 }
 
@@ -198,7 +206,7 @@ bool HipPathIntegrator::ensureCamera(const SceneView& scene, uint32_t w, uint32_
     put3(c.vertical, cam.vertical);
     c.width = w;
     c.height = h;
-    ahip_render_params p{m_maxDepth, 2, 0, 0, 0, AHIP_INTEGRATOR_PATH, 10.f};
+    ahip_render_params p{m_maxDepth, 2, 0, 0, 0, m_integrator, m_fireflyClamp};
     return ahip_set_camera(m_ctx, &c) == AHIP_OK && ahip_set_params(m_ctx, &p) == AHIP_OK;
 }
 
@@ -240,7 +248,13 @@ void HipPathIntegrator::renderTile(const SceneView& scene, const TileRequest& ti
             float w = p[3] > 0.f ? p[3] : 1.f;
             out.add(x, y, p[0] / w, p[1] / w, p[2] / w, w);
             out.addLumSq(x, y, p[4]);
-            if (tile.sampleCount > 0) {   // the denoising AOVs PathIntegrator::renderTile records (:58-66)
+            // the denoising AOVs PathIntegrator::renderTile records (:58-66); BDPTIntegrator::renderTile
+            // records them only for pixels whose camera subpaths reached a surface (:567-580), which
+            // the device reports as zero albedo and zero normal (a surface hit leaves one of them
+            // nonzero unless every hit surface is black and their normals cancel exactly)
+            const bool hitSurface = alb[3 * i] != 0.f || alb[3 * i + 1] != 0.f || alb[3 * i + 2] != 0.f ||
+                                    nrm[3 * i] != 0.f || nrm[3 * i + 1] != 0.f || nrm[3 * i + 2] != 0.f;
+            if (tile.sampleCount > 0 && (m_integrator == AHIP_INTEGRATOR_PATH || hitSurface)) {
                 out.addAlbedo(x, y, makeSpectrum(alb[3 * i], alb[3 * i + 1], alb[3 * i + 2]));
                 out.addNormal(x, y, Vec3f{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]});
             }

@@ -25,7 +25,11 @@ void mergeAccumIntoFilm(Film& film, const float* acc, uint32_t x0, uint32_t y0, 
 class HipPathIntegrator : public IIntegrator {
 public:
     // maxDepth: PathIntegrator's (RenderSession passes m_settings.maxDepth, RenderSession.cpp:463).
-    explicit HipPathIntegrator(int device = 0, uint32_t maxDepth = 8);
+    // integrator: 0 = the path tracer, 1 = BDPTIntegrator(maxDepth, fireflyClamp) (RenderSession.cpp:
+    // 459-461; AHIP_INTEGRATOR_* of anacapa_hip.h); prepare() leaves a BDPT adapter unprepared for
+    // scenes the device BDPT does not take, so RenderSession keeps the CPU integrator.
+    explicit HipPathIntegrator(int device = 0, uint32_t maxDepth = 8, uint32_t integrator = 0,
+                               float fireflyClamp = 10.f);
     ~HipPathIntegrator() override;
 
     void prepare(const SceneView& scene) override;
@@ -53,6 +57,8 @@ private:
     mutable std::mutex m_mutex;      // renderTile is called concurrently (RenderSession.cpp:515-521)
     bool m_prepared = false;
     uint32_t m_maxDepth;
+    uint32_t m_integrator;
+    float m_fireflyClamp;
     uint32_t m_camW = 0, m_camH = 0;
 };
 

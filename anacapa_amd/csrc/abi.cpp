@@ -836,6 +836,9 @@ int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
     if (p->integrator > AHIP_INTEGRATOR_BDPT) return fail(ctx, AHIP_E_INVALID_ARG, "unknown integrator");
     if (!(p->firefly_clamp >= 0.f) || !std::isfinite(p->firefly_clamp))
         return fail(ctx, AHIP_E_INVALID_ARG, "firefly_clamp must be finite and >= 0");
+    if (p->integrator == AHIP_INTEGRATOR_BDPT && ctx->prepared && !ctx->bdptRefusal.empty())
+        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive materials and area lights; "
+                                             "the scene has " + ctx->bdptRefusal);
     ctx->params = *p;
     ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : kMaxCurveDepth;
     return AHIP_OK;

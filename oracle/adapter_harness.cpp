@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "accel/BVHBackend.h"
+#include "integrator/BDPTIntegrator.h"
 #include "integrator/PathIntegrator.h"
 #include "render/ThreadPool.h"
 #include "sampling/HaltonSampler.h"
@@ -99,12 +100,13 @@ extern "C" {
 
 // The reference SceneView of the scene (Lambert / Emissive / quad area lights) and the adapter on
 // HIP device 0, prepared with it.  Returns null when the reference cannot represent the scene.
-void* ah_create(const ahip_scene_desc* d, const ahip_camera* cam, uint32_t maxDepth) {
+// integrator / clamp: the adapter's (0 = path tracer, 1 = BDPT with that firefly clamp).
+void* ah_create(const ahip_scene_desc* d, const ahip_camera* cam, uint32_t maxDepth, uint32_t integrator, float clamp) {
     auto h = std::make_unique<Harness>();
     h->scene = buildRefScene(d);
     if (!h->scene) return nullptr;
     h->scene->view.camera = toCamera(cam);
-    h->hip = std::make_unique<HipPathIntegrator>(0, maxDepth);
+    h->hip = std::make_unique<HipPathIntegrator>(0, maxDepth, integrator, clamp);
     h->hip->prepare(h->scene->view);
     return h.release();
 }
@@ -121,14 +123,20 @@ int ah_status(void* p, int* valid, int* prepared, char* name, size_t nameLen, ch
     return 0;
 }
 
-// The frame through RenderSession's tile loop with the adapter (which = 0) or the reference
-// PathIntegrator(maxDepth) (which = 1) as the IIntegrator.
+// The frame through RenderSession's tile loop with the adapter (which = 0), the reference
+// PathIntegrator(maxDepth) (which = 1) or the reference BDPTIntegrator(maxDepth, clamp) (which = 2)
+// as the IIntegrator.
 int ah_render(void* p, int which, uint32_t spp, uint32_t maxDepth, uint64_t seed, uint32_t nthreads, uint32_t tileSize,
-              float* out, float* albedo, float* normal) {
+              float clamp, float* out, float* albedo, float* normal) {
     auto* h = static_cast<Harness*>(p);
     const Camera& cam = *h->scene->view.camera;
     if (which == 0) {
         renderTiles(*h->hip, h->scene->view, cam.imageWidth, cam.imageHeight, spp, seed, nthreads, tileSize, out,
+                    albedo, normal);
+    } else if (which == 2) {
+        BDPTIntegrator integ(maxDepth, clamp);
+        integ.prepare(h->scene->view);
+        renderTiles(integ, h->scene->view, cam.imageWidth, cam.imageHeight, spp, seed, nthreads, tileSize, out,
                     albedo, normal);
     } else {
         PathIntegrator integ(maxDepth);

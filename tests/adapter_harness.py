@@ -20,19 +20,19 @@ def available() -> bool:
 
 
 class AdapterHarness:
-    def __init__(self, scene, cam: A.Camera, max_depth: int):
+    def __init__(self, scene, cam: A.Camera, max_depth: int, integrator: int = 0, clamp: float = 10.0):
         L = C.CDLL(HARNESS_SO)
         L.ah_create.restype = C.c_void_p
-        L.ah_create.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(A.Camera), C.c_uint32]
+        L.ah_create.argtypes = [C.POINTER(A.SceneDesc), C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_float]
         L.ah_destroy.argtypes = [C.c_void_p]
         L.ah_status.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_size_t,
                                 C.c_char_p, C.c_size_t]
         L.ah_render.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
-                                C.c_void_p, C.c_void_p, C.c_void_p]
+                                C.c_float, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ah_film_merges.restype = C.c_int
-        self.lib, self.cam, self.max_depth = L, cam, max_depth
+        self.lib, self.cam, self.max_depth, self.clamp = L, cam, max_depth, clamp
         self.desc, self._keep = scene.desc()
-        self.h = L.ah_create(C.byref(self.desc), C.byref(cam), max_depth)
+        self.h = L.ah_create(C.byref(self.desc), C.byref(cam), max_depth, integrator, clamp)
         if not self.h:
             raise RuntimeError("the reference cannot represent this scene")
 
@@ -49,12 +49,14 @@ class AdapterHarness:
                 "error": err.value.decode()}
 
     def render(self, which: str, spp: int, threads: int = 8, tile: int = 64, seed: int = 0):
-        """The frame through the reference tile loop with the adapter ("adapter") or the reference
-        PathIntegrator ("reference"): (accum [h, w, 5], albedo sums [h, w, 3], normal sums [h, w, 3])."""
+        """The frame through the reference tile loop with the adapter ("adapter"), the reference
+        PathIntegrator ("reference") or the reference BDPTIntegrator ("reference_bdpt", with the
+        harness's clamp): (accum [h, w, 5], albedo sums [h, w, 3], normal sums [h, w, 3])."""
         h, w = self.cam.height, self.cam.width
         out = np.zeros((h, w, 5), np.float32)
         alb, nrm = np.zeros((h, w, 3), np.float32), np.zeros((h, w, 3), np.float32)
-        self.lib.ah_render(self.h, 0 if which == "adapter" else 1, spp, self.max_depth, seed, threads, tile,
+        which_i = {"adapter": 0, "reference": 1, "reference_bdpt": 2}[which]
+        self.lib.ah_render(self.h, which_i, spp, self.max_depth, seed, threads, tile, self.clamp,
                            A.ptr(out), A.ptr(alb), A.ptr(nrm))
         return out, alb, nrm
 
