@@ -138,6 +138,7 @@ struct ahip_context {
     // alias table and the vertex stores of a chunk of slots
     std::string bdptRefusal = "no scene";
     DevBuf dAliasProb, dAlias, dLightW, dBdptVerts, dBdptCounts, dBdptAovA, dBdptAovN, dBdptL, dBdptCtr;
+    DevBuf dBdptTerms, dBdptNTerms, dBdptConnO, dBdptConnD, dBdptStageO, dBdptStageD;
     size_t bdptCap = 0, bdptMaxV = 0;
     DevBuf qPix, qPassIdx, dViews, dTables, dBatchAcc, dBatchLsq;
     uint32_t tileW = 8;          // pixel tile per wavefront: tileW x (64 / tileW)
@@ -901,18 +902,26 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
     if (st) return st;
     st = ensureQueues(ctx, 64);
     if (st) return st;
-    // about 1M slots per chunk: 2 x (maxDepth + 2) vertices of 80 B each per slot
-    const size_t maxV = (size_t)ctx->params.max_depth + 2u;
-    const size_t pixPerChunk = std::max<size_t>(1u, (size_t)(1u << 20) / ns);
+    // about 512K slots per chunk; per slot 2 x (maxDepth + 2) vertices of 80 B, up to
+    // (maxDepth + 2)(maxDepth + 3) strategy terms of 16 B and as many staged and queued shadow rays
+    // of 32 B each (10.4 KB at depth 8: 5.5 GB per chunk)
+    const size_t maxV = (size_t)ctx->params.max_depth + 2u, R = maxV * (maxV + 1u);
+    const size_t pixPerChunk = std::max<size_t>(1u, (size_t)(1u << 19) / ns);
     const size_t cap = pixPerChunk * ns;
-    if (cap * 2u * maxV * 5u > (size_t)1 << 31) return fail(ctx, AHIP_E_INVALID_ARG, "BDPT chunk too large");
+    if (cap * R > (size_t)1 << 31) return fail(ctx, AHIP_E_INVALID_ARG, "BDPT chunk too large");
     if (ctx->bdptCap < cap || ctx->bdptMaxV < maxV || !ctx->dBdptVerts.p) {
         HIPCHK(ctx->dBdptVerts.alloc(cap * 2u * maxV * 5u * sizeof(float4)));
         HIPCHK(ctx->dBdptCounts.alloc(cap * sizeof(uint32_t)));
         HIPCHK(ctx->dBdptAovA.alloc(cap * sizeof(float4)));
         HIPCHK(ctx->dBdptAovN.alloc(cap * sizeof(float4)));
         HIPCHK(ctx->dBdptL.alloc(cap * sizeof(float4)));
-        HIPCHK(ctx->dBdptCtr.alloc(sizeof(uint32_t)));
+        HIPCHK(ctx->dBdptTerms.alloc(cap * R * sizeof(float4)));
+        HIPCHK(ctx->dBdptNTerms.alloc(cap * sizeof(uint32_t)));
+        HIPCHK(ctx->dBdptConnO.alloc(cap * R * sizeof(float4)));
+        HIPCHK(ctx->dBdptConnD.alloc(cap * R * sizeof(float4)));
+        HIPCHK(ctx->dBdptStageO.alloc(cap * R * sizeof(float4)));
+        HIPCHK(ctx->dBdptStageD.alloc(cap * R * sizeof(float4)));
+        HIPCHK(ctx->dBdptCtr.alloc(2 * sizeof(uint32_t)));
         ctx->bdptCap = cap;
         ctx->bdptMaxV = maxV;
     }
@@ -929,6 +938,13 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
     a.aovA = ctx->dBdptAovA.as<float4>();
     a.aovN = ctx->dBdptAovN.as<float4>();
     a.sampleL = ctx->dBdptL.as<float4>();
+    a.terms = ctx->dBdptTerms.as<float4>();
+    a.nTerms = ctx->dBdptNTerms.as<uint32_t>();
+    a.connO = ctx->dBdptConnO.as<float4>();
+    a.connD = ctx->dBdptConnD.as<float4>();
+    a.stageO = ctx->dBdptStageO.as<float4>();
+    a.stageD = ctx->dBdptStageD.as<float4>();
+    a.connCount = ctx->dBdptCtr.as<uint32_t>() + 1;
     a.aliasProb = ctx->dAliasProb.as<float>();
     a.alias = ctx->dAlias.as<uint32_t>();
     a.lightW = ctx->dLightW.as<float>();
@@ -950,7 +966,7 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
         a.slots = (uint32_t)(std::min(pixPerChunk, npix - p0) * ns);
         // every slot on its own lane; never more threads than the traversal overflow stacks hold
         const uint32_t blocks = std::min<uint32_t>(ctx->traceBlocks, (a.slots + 127u) / 128u);
-        launchBdpt(ctx->dsc, a, blocks, ctx->stream);
+        launchBdpt(ctx->dsc, a, blocks, ctx->traceBlocks, ctx->stream);
     }
     HIPCHK(hipEventRecord(ctx->renderEnd, ctx->stream));
     uint32_t ovf = 0;

@@ -2,16 +2,21 @@
 // MISWeight.cpp) on gfx950, for scenes of triangles with Lambert / Emissive materials lit by quad
 // area lights (the reference's BDPT configuration; anything else is refused on the host).
 //
-// One sample per lane in three kernels over a chunk of (pixel, sample) slots:
-//   k_bdpt_paths   — camera subpath (BDPTIntegrator.cpp:70-194) and light subpath (:199-303),
-//                    vertices written to a slot-major vertex store;
-//   k_bdpt_connect — every strategy (s, t) of the sample (:308-459, :521-557): the reference's
-//                    connections (s = 0 emission, s = 1 light sampling, s, t >= 2 vertex pairs;
-//                    t = 1 is not a strategy there), any-hit visibility, bdptMISWeight in double
-//                    (MISWeight.cpp:14-72), the firefly clamp of s, t >= 2;
-//   k_bdpt_accum   — per pixel, its samples in order: the radiance sum, luminance squares and the
-//                    first-hit AOVs, added to the output as renderTile's TileBuffer::add does
-//                    (:559-580).
+// Kernels over a chunk of (pixel, sample) slots:
+//   k_bdpt_paths        — one slot per lane: camera subpath (BDPTIntegrator.cpp:70-194) and light
+//                         subpath (:199-303), vertices written to a slot-major vertex store;
+//   k_bdpt_conn_gen     — one slot per lane: every strategy (s, t) of the sample (:308-459,
+//                         :521-557), the reference's connections (s = 0 emission, s = 1 light
+//                         sampling, s, t >= 2 vertex pairs; t = 1 is not a strategy there) with
+//                         bdptMISWeight in double (MISWeight.cpp:14-72) and the firefly clamp of
+//                         s, t >= 2, into the slot's term list; the shadow rays into one queue;
+//   k_bdpt_conn_trace   — one shadow ray per lane: any-hit visibility of the queue;
+//   k_bdpt_conn_resolve — one slot per lane: the visible terms summed in the reference's order;
+//   k_bdpt_accum        — one pixel per lane, its samples in order: the radiance sum, luminance
+//                         squares and the first-hit AOVs, added to the output as renderTile's
+//                         TileBuffer::add does (:559-580).
+// Tracing the shadow rays from a queue keeps the lanes of a wave on rays, whatever the number of
+// strategies of each slot (a slot's strategies vary from 1 to (maxDepth + 2)^2).
 // Sampler dimensions follow the reference's call order (g++ evaluates sample(ctx, wo, get2D(),
 // get1D()) and sampleLe(get2D(), get2D()) right to left).
 #include <hip/hip_runtime.h>
@@ -233,30 +238,19 @@ __device__ float misWeight(const BdptArgs& a, uint32_t i, uint32_t nl, uint32_t 
     return (float)(1.0 / (sum + 1.0));
 }
 
-// any hit in [1e-4, tMax) of spawnRayTo(from, n, to) (Types.h:277-284): shadowTransmittance of an
-// opaque scene (ShadowRay.h:26-61)
-__device__ __forceinline__ bool visible(const DScene& sc, const BdptArgs& a, const TStack& S, V3 from, V3 n, V3 to,
-                                        TraceCounters& tc) {
-    V3 dir = to - from;
-    const float dist = len(dir);
-    dir = dir * (1.f / dist);
-    V3 off = n * 1e-4f;
-    if (dot(dir, n) < 0.f) off = -off;
-    float t, u, v;
-    return traverse<true, false>(sc, makeTRay(from + off, dir, 1e-4f, dist * (1.f - 1e-4f)), S, t, u, v, tc,
-                                 a.overflow) == ~0u;
-}
-
 }  // namespace
 
-__global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a) {
-    __shared__ uint64_t lstk[kBdptK * kBdptBS];
-    TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
-             gridDim.x * kBdptBS, a.stackCap};
-    TraceCounters tc{0, 0};
-    for (uint32_t i = blockIdx.x * kBdptBS + threadIdx.x; i < a.slots; i += gridDim.x * kBdptBS) {
-        const uint32_t cnt = a.counts[i], nc = cnt & 0xffffu, nl = cnt >> 16;
-        V3 sum{0.f, 0.f, 0.f};
+// Every strategy (s, t) of the slot in the reference's order (t outer, s inner), without its
+// visibility: the weighted, clamped contribution goes to the slot's term list (term k of slot i at
+// terms[k * cap + i]); a strategy that needs a visibility test leaves its term at weight 0 and
+// appends its shadow ray (spawnRayTo, Types.h:277-284) to the connection queue.
+__global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t base0 = blockIdx.x * kBdptBS; base0 < a.slots; base0 += gridDim.x * kBdptBS) {
+        const uint32_t i = base0 + threadIdx.x;
+        const bool valid = i < a.slots;
+        const uint32_t cnt = valid ? a.counts[i] : 0u, nc = cnt & 0xffffu, nl = cnt >> 16;
+        uint32_t k = 0u, nr = 0u;
         for (uint32_t t = 1; t <= nc; ++t) {
             const uint32_t ct = t - 1u;
             const float4 c0 = a.verts[vAt(a, 0, ct, 0, i)], c1 = a.verts[vAt(a, 0, ct, 1, i)];
@@ -269,6 +263,8 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a)
             for (uint32_t s = 0; s <= nl; ++s) {
                 if (s + t < 2u) continue;
                 V3 C{0.f, 0.f, 0.f};
+                bool ray = false;
+                V3 from{0.f, 0.f, 0.f}, fromN{0.f, 0.f, 0.f}, to{0.f, 0.f, 0.f};
                 if (s == 0u) {
                     // (0, t): emission at the last camera vertex (BDPTIntegrator.cpp:318-343)
                     if (t >= 2u && cSurf) C = cbeta * v3(c4);
@@ -281,17 +277,19 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a)
                         const float dist = len(toL);
                         if (dist >= 1e-6f) {
                             const V3 wi = toL * (1.f / dist);
-                            if (visible(sc, a, S, cp, cn, cp + wi * dist, tc)) {
-                                float pdf;
-                                const DMat& m = sc.mats[cm];
-                                const Frame fr = makeFrame(cn, cn, -cwo);
-                                const V3 f = matEval(sc, m, fr, MatIn{ld3(m.albedo), 1.f}, cwo, wi, pdf);
-                                const V3 ln = v3(l1);
-                                const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
-                                if (!isBlack(f) && !isBlack(Le)) {
-                                    const float cosI = fabsf(dot(wi, cn)), cosL = fabsf(dot(-wi, ln));
-                                    C = cbeta * f * cosI * Le * cosL / (dist * dist * l0.w);
-                                }
+                            float pdf;
+                            const DMat& m = sc.mats[cm];
+                            const Frame fr = makeFrame(cn, cn, -cwo);
+                            const V3 f = matEval(sc, m, fr, MatIn{ld3(m.albedo), 1.f}, cwo, wi, pdf);
+                            const V3 ln = v3(l1);
+                            const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
+                            if (!isBlack(f) && !isBlack(Le)) {
+                                const float cosI = fabsf(dot(wi, cn)), cosL = fabsf(dot(-wi, ln));
+                                C = cbeta * f * cosI * Le * cosL / (dist * dist * l0.w);
+                                ray = true;
+                                from = cp;
+                                fromN = cn;
+                                to = cp + wi * dist;
                             }
                         }
                     }
@@ -305,7 +303,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a)
                     const bool lSurf = (lflags & 3u) == kVSurface;
                     const int32_t lm = lSurf ? __float_as_int(l4.w) : -1;
                     if (!(lflags & kVDelta) && !(cflags & kVDelta) && lm >= 0 && cm >= 0 && l3.w >= 0.08f &&
-                        c3.w >= 0.08f && visible(sc, a, S, v3(l0), v3(l1), cp, tc)) {
+                        c3.w >= 0.08f) {
                         const V3 d = cp - v3(l0);
                         const float dist = len(d);
                         if (dist >= 1e-6f) {
@@ -313,19 +311,87 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_connect(DScene sc, BdptArgs a)
                             float f1, r1, f2, r2;
                             const V3 fL = evalVertex(sc, lm, v3(l1), v3(l2), wi, f1, r1);
                             const V3 fC = isBlack(fL) ? V3{0.f, 0.f, 0.f} : evalVertex(sc, cm, cn, cwo, -wi, f2, r2);
-                            if (!isBlack(fC)) C = v3(l3) * fL * (1.f / (dist * dist)) * fC * cbeta;
+                            if (!isBlack(fC)) {
+                                C = v3(l3) * fL * (1.f / (dist * dist)) * fC * cbeta;
+                                ray = true;
+                                from = v3(l0);
+                                fromN = v3(l1);
+                                to = cp;
+                            }
                         }
                     }
                 }
-                if (isBlack(C) || !finite3(C)) continue;
-                V3 c = C * misWeight(a, i, nl, nc, s, t);
-                if (!finite3(c)) continue;
-                if (a.clamp > 0.f && s >= 2u && t >= 2u) {
-                    const float l = luminance(c);
-                    if (l > a.clamp) c = c * (a.clamp / l);
+                bool keep = !isBlack(C) && finite3(C);
+                V3 c{0.f, 0.f, 0.f};
+                if (keep) {
+                    c = C * misWeight(a, i, nl, nc, s, t);
+                    keep = finite3(c);
+                    if (keep && a.clamp > 0.f && s >= 2u && t >= 2u) {
+                        const float l = luminance(c);
+                        if (l > a.clamp) c = c * (a.clamp / l);
+                    }
                 }
-                sum = sum + c;
+                if (keep) {
+                    const uint32_t idx = k * a.cap + i;
+                    a.terms[idx] = f4(c, ray ? 0.f : 1.f);
+                    if (ray) {   // staged in the slot's own rows until the wave's queue space is known
+                        // spawnRayTo(from, fromN, to) (Types.h:277-284)
+                        V3 dir = to - from;
+                        const float dist = len(dir);
+                        dir = dir * (1.f / dist);
+                        V3 off = fromN * 1e-4f;
+                        if (dot(dir, fromN) < 0.f) off = -off;
+                        a.stageO[nr * a.cap + i] = f4(from + off, __uint_as_float(idx));
+                        a.stageD[nr * a.cap + i] = f4(dir, dist * (1.f - 1e-4f));
+                        ++nr;
+                    }
+                    ++k;
+                }
             }
+        }
+        if (valid) a.nTerms[i] = k;
+        // the wave's shadow rays go to the queue at a wave prefix of the slots' counts: one atomic
+        // per wave
+        uint32_t incl = nr;
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        uint32_t qbase = 0u;
+        if (lane == 0u && total) qbase = atomicAdd(a.connCount, total);
+        qbase = __shfl(qbase, 0) + incl - nr;
+        for (uint32_t r = 0; r < nr; ++r) {
+            a.connO[qbase + r] = a.stageO[r * a.cap + i];
+            a.connD[qbase + r] = a.stageD[r * a.cap + i];
+        }
+    }
+}
+
+// The connection queue's visibility: any hit in [1e-4, tMax) (shadowTransmittance of an opaque scene,
+// ShadowRay.h:26-61); an unoccluded term gets weight 1.
+__global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_trace(DScene sc, BdptArgs a) {
+    __shared__ uint64_t lstk[kBdptK * kBdptBS];
+    TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
+             gridDim.x * kBdptBS, a.stackCap};
+    TraceCounters tc{0, 0};
+    const uint32_t count = *a.connCount;
+    for (uint32_t j = blockIdx.x * kBdptBS + threadIdx.x; j < count; j += gridDim.x * kBdptBS) {
+        const float4 o = a.connO[j], d = a.connD[j];
+        float t, u, v;
+        if (traverse<true, false>(sc, makeTRay(v3(o), v3(d), 1e-4f, d.w), S, t, u, v, tc, a.overflow) == ~0u)
+            a.terms[__float_as_uint(o.w)].w = 1.f;
+    }
+}
+
+// The slot's radiance: its visible terms in the reference's order (BDPTIntegrator.cpp:523-557).
+__global__ __launch_bounds__(256) void k_bdpt_conn_resolve(BdptArgs a) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.slots; i += gridDim.x * blockDim.x) {
+        const uint32_t n = a.nTerms[i];
+        V3 sum{0.f, 0.f, 0.f};
+        for (uint32_t k = 0; k < n; ++k) {
+            const float4 x = a.terms[k * a.cap + i];
+            if (x.w != 0.f) sum = sum + v3(x);
         }
         a.sampleL[i] = f4(sum, 0.f);
     }
@@ -378,9 +444,14 @@ void setHaltonTablesBdpt(const HaltonTables& t, hipStream_t s) {
     hipMemcpyToSymbolAsync(HIP_SYMBOL(c_haltonB), &t, sizeof(t), 0, hipMemcpyHostToDevice, s);
 }
 
-void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, hipStream_t s) {
+__global__ void k_bdpt_zero(uint32_t* p) { *p = 0u; }
+
+void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, uint32_t traceBlocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_bdpt_zero, dim3(1), dim3(1), 0, s, a.connCount);
     hipLaunchKernelGGL(k_bdpt_paths, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
-    hipLaunchKernelGGL(k_bdpt_connect, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    hipLaunchKernelGGL(k_bdpt_conn_gen, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    hipLaunchKernelGGL(k_bdpt_conn_trace, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
+    hipLaunchKernelGGL(k_bdpt_conn_resolve, dim3((a.slots + 255u) / 256u), dim3(256), 0, s, a);
     const uint32_t npix = a.slots / a.ns;
     hipLaunchKernelGGL(k_bdpt_accum, dim3((npix + 255u) / 256u), dim3(256), 0, s, a);
 }

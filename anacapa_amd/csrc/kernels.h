@@ -131,6 +131,13 @@ struct BdptArgs {
     float4* aovA;                // [cap] first-hit reflectance | 1 when the camera ray hit a surface
     float4* aovN;                // [cap] first-hit normal
     float4* sampleL;             // [cap] the sample's radiance
+    float4* terms;               // [R][cap] weighted strategy contributions | visibility (0 / 1)
+    uint32_t* nTerms;            // [cap] terms of the slot
+    float4* stageO;              // [R][cap] a slot's shadow rays before they are queued
+    float4* stageD;
+    float4* connO;               // connection queue: shadow ray origin | term index
+    float4* connD;               // direction | tMax
+    uint32_t* connCount;         // connection queue length
     const float* aliasProb;      // LightSampler alias table (LightSampler.h:28-69)
     const uint32_t* alias;
     const float* lightW;         // light selection pdfs
@@ -144,7 +151,8 @@ struct BdptArgs {
     uint32_t outStride, outX0, outY0;
 };
 void setHaltonTablesBdpt(const HaltonTables& t, hipStream_t s);
-void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, hipStream_t s);
+// blocks: the slot kernels' grid; traceBlocks: the connection queue's (grid-stride over its length)
+void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, uint32_t traceBlocks, hipStream_t s);
 
 void setHaltonTables(const HaltonTables& t, hipStream_t s);
 uint32_t lds_stack_entries();
