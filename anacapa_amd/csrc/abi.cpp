@@ -752,14 +752,17 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         S.dome.intensity = D.intensity;
         std::memcpy(S.dome.rot, D.rot, sizeof(D.rot));
     }
-    // BDPT takes triangles with Lambert / Emissive materials lit by quad area lights, no environment
+    // BDPT takes triangles with Lambert / Emissive / untextured StandardSurface materials lit by quad
+    // area lights, no environment (the device forms no per-vertex uv, which textures would need)
     ctx->bdptRefusal.clear();
     if (!ctx->curves.nodes.empty()) ctx->bdptRefusal = "curves";
     else if (hs.hasDome || d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
         ctx->bdptRefusal = "an environment";
+    if (ctx->bdptRefusal.empty() && d->tri_albedo) ctx->bdptRefusal = "per-triangle albedo";
     for (const DMat& m : hs.mats)
-        if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE)
-            ctx->bdptRefusal = "a material other than Lambert / Emissive";
+        if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE &&
+            !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0))
+            ctx->bdptRefusal = "a material other than Lambert / Emissive / untextured StandardSurface";
     for (const DLight& L : hs.lights)
         if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA) ctx->bdptRefusal = "a light other than an area light";
     if (ctx->bdptRefusal.empty() && !hs.lights.empty()) {
@@ -838,8 +841,8 @@ int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
     if (!(p->firefly_clamp >= 0.f) || !std::isfinite(p->firefly_clamp))
         return fail(ctx, AHIP_E_INVALID_ARG, "firefly_clamp must be finite and >= 0");
     if (p->integrator == AHIP_INTEGRATOR_BDPT && ctx->prepared && !ctx->bdptRefusal.empty())
-        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive materials and area lights; "
-                                             "the scene has " + ctx->bdptRefusal);
+        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive / StandardSurface materials "
+                                             "and area lights; the scene has " + ctx->bdptRefusal);
     ctx->params = *p;
     ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : kMaxCurveDepth;
     return AHIP_OK;
@@ -895,8 +898,8 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
                       float4* outAcc, float* outLsq, float* outAov, uint32_t outStride, uint32_t outX0,
                       uint32_t outY0) {
     if (!ctx->bdptRefusal.empty())
-        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive materials and area lights; "
-                                             "the scene has " + ctx->bdptRefusal);
+        return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive / StandardSurface materials "
+                                             "and area lights; the scene has " + ctx->bdptRefusal);
     if (ns == 0u) return AHIP_OK;
     int st = checkInjected(ctx);
     if (st) return st;

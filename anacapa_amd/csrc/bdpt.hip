@@ -1,6 +1,6 @@
 // bdpt.hip — the bidirectional path tracer (BDPTIntegrator, src/integrator/BDPTIntegrator.cpp and
-// MISWeight.cpp) on gfx950, for scenes of triangles with Lambert / Emissive materials lit by quad
-// area lights (the reference's BDPT configuration; anything else is refused on the host).
+// MISWeight.cpp) on gfx950, for scenes of triangles with Lambert / Emissive / untextured opaque
+// StandardSurface materials lit by quad area lights (anything else is refused on the host).
 //
 // Kernels over a chunk of (pixel, sample) slots:
 //   k_bdpt_paths        — one slot per lane: camera subpath (BDPTIntegrator.cpp:70-194) and light
@@ -57,19 +57,24 @@ __device__ __forceinline__ float toArea(float pdfSA, V3 prev, V3 next, V3 nextN)
     return pdfSA * fabsf(dot(safeNormalize(d), nextN)) / d2;
 }
 
+// The untextured material inputs of a vertex (the vertex keeps no uv: BDPTIntegrator::evalVertex
+// rebuilds its ShadingContext from the position and normal only).
+__device__ __forceinline__ MatIn vertexInputs(const DMat& m) { return MatIn{ld3(m.albedo), m.rough}; }
+
 // BDPTIntegrator::evalVertex (BDPTIntegrator.cpp:40-63) at a surface vertex of material m, normal
-// n, outgoing wo: f(wo, wi) |cos wi| and the forward / reverse pdfs of LambertianMaterial::evaluate
-// (Lambertian.h:53-66).
+// n, outgoing wo: f(wo, wi) |cos wi| and the forward pdf of IMaterial::evaluate.  The reverse pdf is
+// read by the subpaths at wi = -wo only, where both LambertianMaterial::evaluate (Lambertian.h:53-66)
+// and the opaque StandardSurface (StandardSurface.h:410-417) reject the pair as opposite hemispheres
+// and give 0; at other pairs it is formed for Lambert only.
 __device__ __forceinline__ V3 evalVertex(const DScene& sc, int32_t mi, V3 n, V3 wo, V3 wi, float& fwd, float& rev) {
     fwd = rev = 0.f;
     if (mi < 0) return V3{0.f, 0.f, 0.f};
     const DMat& m = sc.mats[mi];
     const Frame fr = makeFrame(n, n, -wo);
-    const V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
-    if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
-    fwd = stdmax(0.f, fabsf(wil.z)) * kInvPi;
-    rev = stdmax(0.f, fabsf(wol.z)) * kInvPi;
-    return ld3(m.albedo) * kInvPi * fabsf(dot(wi, n));
+    const V3 f = matEval(sc, m, fr, vertexInputs(m), wo, wi, fwd);
+    if (isBlack(f)) return f;
+    if (m.type != AHIP_MAT_STANDARD) rev = stdmax(0.f, fabsf(toLocal(fr, wo).z)) * kInvPi;
+    return f * fabsf(dot(wi, n));
 }
 
 __device__ __forceinline__ float hs(const BdptArgs& a, uint32_t px, uint32_t py, uint32_t s, uint32_t& dim) {
@@ -104,8 +109,12 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         const uint32_t vi = count++;
         a.verts[vAt(a, sub, vi, 0, i)] = f4(p, pdfFwdArea);
         a.verts[vAt(a, sub, vi, 1, i)] = f4(n, 0.f);
-        a.verts[vAt(a, sub, vi, 2, i)] = f4(wo, __uint_as_float(kVSurface));
-        a.verts[vAt(a, sub, vi, 3, i)] = f4(beta, 1.f);   // Lambert: roughness() = 1 (IMaterial.h:112)
+        // IMaterial::roughness() (IMaterial.h:112: 1; StandardSurface.h:171: the roughness value), a
+        // delta vertex counting as 1 (BDPTIntegrator.cpp:163-172)
+        const float rough = m.isDelta ? 1.f : (m.type == AHIP_MAT_STANDARD ? m.rough : 1.f);
+        const float minRough = vi > 0u ? stdmin(a.verts[vAt(a, sub, vi - 1u, 3, i)].w, rough) : rough;
+        a.verts[vAt(a, sub, vi, 2, i)] = f4(wo, __uint_as_float(kVSurface | (m.isDelta ? kVDelta : 0u)));
+        a.verts[vAt(a, sub, vi, 3, i)] = f4(beta, minRough);
         a.verts[vAt(a, sub, vi, 4, i)] = f4((!light && m.hasLe) ? ld3(m.Le) : V3{0.f, 0.f, 0.f}, __int_as_float(mi));
         if (vi > 0u) {   // the previous vertex's reverse pdf, from this one
             float dummy, revSA;
@@ -162,12 +171,15 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
         a.verts[vAt(a, 0, 0, 4, i)] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         const uint32_t nc = walk(sc, a, S, i, 0u, 1u, o, makeTRay(o, d, 1e-4f, __int_as_float(0x7f800000)),
                                  V3{1.f, 1.f, 1.f}, 1.f, px, py, sidx, dim, false, tc);
-        // first-hit AOVs (BDPTIntegrator.cpp:502-512): LambertianMaterial::reflectance = albedo
+        // first-hit AOVs (BDPTIntegrator.cpp:502-512): IMaterial::reflectance at the first vertex
         float4 alb = make_float4(0.f, 0.f, 0.f, 0.f), nrm = make_float4(0.f, 0.f, 0.f, 0.f);
         if (nc >= 2u) {
             const int32_t mi = __float_as_int(a.verts[vAt(a, 0, 1, 4, i)].w);
-            alb = f4(ld3(sc.mats[mi].albedo), 1.f);
-            nrm = f4(v3(a.verts[vAt(a, 0, 1, 1, i)]), 0.f);
+            const DMat& m = sc.mats[mi];
+            const V3 n1 = v3(a.verts[vAt(a, 0, 1, 1, i)]);
+            const Frame fr = makeFrame(n1, n1, -v3(a.verts[vAt(a, 0, 1, 2, i)]));
+            alb = f4(matReflectance(m, fr, vertexInputs(m)), 1.f);
+            nrm = f4(n1, 0.f);
         }
         a.aovA[i] = alb;
         a.aovN[i] = nrm;
@@ -280,7 +292,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                             float pdf;
                             const DMat& m = sc.mats[cm];
                             const Frame fr = makeFrame(cn, cn, -cwo);
-                            const V3 f = matEval(sc, m, fr, MatIn{ld3(m.albedo), 1.f}, cwo, wi, pdf);
+                            const V3 f = matEval(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
                             const V3 ln = v3(l1);
                             const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
                             if (!isBlack(f) && !isBlack(Le)) {

@@ -294,26 +294,36 @@ def _block(mb, lo, hi, deg, mat):
     mb.add(pos, idx, mat, normals=nrm)
 
 
-def cornell(width=512, height=512, spp=64, max_depth=8, seed=0, second_light=False) -> Scene:
+def cornell(width=512, height=512, spp=64, max_depth=8, seed=0, second_light=False, glossy=False) -> Scene:
     """The Cornell box: unit-box scene, 36 triangles, one 0.25x0.25 quad light (17,12,4).
     second_light: a dimmer 0.1x0.1 quad (4,6,9) near the back of the ceiling as well (two lights of
-    unequal power, for light selection)."""
+    unequal power, for light selection).  glossy: the tall block rough gold metal, the short block a
+    coated blue dielectric and a small mirror panel on the back wall (standard_surface, the mirror a
+    delta BSDF)."""
     mats = [lambert((0.73, 0.73, 0.73)), lambert((0.63, 0.065, 0.05)), lambert((0.14, 0.45, 0.09)),
             emissive((0.0, 0.0, 0.0), (17.0, 12.0, 4.0))]
+    tall, short = 0, 0
+    if glossy:
+        mats += [standard_surface((0.9, 0.75, 0.45), metalness=1.0, roughness=0.3),
+                 standard_surface((0.2, 0.4, 0.8), roughness=0.25, coat=0.5, coat_roughness=0.1),
+                 standard_surface((0.95, 0.95, 0.95), metalness=1.0, roughness=0.0)]
+        tall, short = 4, 5
     mb = _MeshBuilder()
     _quad(mb, (0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), 0)      # floor
     _quad(mb, (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, 1, 0), (0, -1, 0), 0)     # ceiling
     _quad(mb, (0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1), 0)     # back
     _quad(mb, (0, 0, 1), (0, 1, 1), (0, 1, 0), (0, 0, 0), (1, 0, 0), 1)      # left (red)
     _quad(mb, (1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 2)     # right (green)
-    _block(mb, (0.13, 0.0, 0.12), (0.43, 0.3, 0.42), -18.0, 0)              # short block
-    _block(mb, (0.53, 0.0, 0.45), (0.83, 0.6, 0.75), 18.0, 0)               # tall block
+    _block(mb, (0.13, 0.0, 0.12), (0.43, 0.3, 0.42), -18.0, short)          # short block
+    _block(mb, (0.53, 0.0, 0.45), (0.83, 0.6, 0.75), 18.0, tall)            # tall block
+    if glossy:   # mirror panel just in front of the back wall
+        _quad(mb, (0.1, 0.35, 0.998), (0.1, 0.75, 0.998), (0.45, 0.75, 0.998), (0.45, 0.35, 0.998), (0, 0, -1), 6)
     y = 0.999
     _quad(mb, (0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625), (0, -1, 0), 3)
     lights = [area_light((0.5, y, 0.5), (0.125, 0.0, 0.0), (0.0, 0.0, 0.125), (17.0, 12.0, 4.0))]
     if second_light:
         mats.append(emissive((0.0, 0.0, 0.0), (4.0, 6.0, 9.0)))
-        _quad(mb, (0.2, y, 0.8), (0.3, y, 0.8), (0.3, y, 0.9), (0.2, y, 0.9), (0, -1, 0), 4)
+        _quad(mb, (0.2, y, 0.8), (0.3, y, 0.8), (0.3, y, 0.9), (0.2, y, 0.9), (0, -1, 0), len(mats) - 1)
         lights.append(area_light((0.25, y, 0.85), (0.05, 0.0, 0.0), (0.0, 0.0, 0.05), (4.0, 6.0, 9.0)))
     return mb.scene("cornell", materials=mats, lights=lights, cam_from=(0.5, 0.5, -1.4),
                     cam_at=(0.5, 0.5, 0.0), cam_up=(0.0, 1.0, 0.0), vfov=40.0, width=width,

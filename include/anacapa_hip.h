@@ -154,8 +154,9 @@ typedef struct ahip_camera {
 /* ahip_render_params.integrator (RenderSettings::integrator, src/render/RenderSession.h:45-47) */
 #define AHIP_INTEGRATOR_PATH 0u   /* PathIntegrator (src/integrator/PathIntegrator.cpp) */
 #define AHIP_INTEGRATOR_BDPT 1u   /* BDPTIntegrator (src/integrator/BDPTIntegrator.cpp): scenes of
-                                     triangles with Lambert / Emissive materials and area lights, no
-                                     environment; ahip_render_tile, ahip_render_tile_aovs and
+                                     triangles with Lambert / Emissive / untextured StandardSurface
+                                     materials and area lights, no environment, no per-triangle
+                                     albedo; ahip_render_tile, ahip_render_tile_aovs and
                                      ahip_render_frame (without AOVs) */
 
 typedef struct ahip_render_params {

@@ -1386,8 +1386,8 @@ struct Scene {
     }
 
     // ------------------------------------------------------------ BDPT (BDPTIntegrator.cpp, MISWeight.cpp)
-    // Scenes of triangles with Lambert / Emissive materials and quad area lights, no environment:
-    // the scope the device implements.
+    // Scenes of triangles with Lambert / Emissive / untextured opaque StandardSurface materials and
+    // quad area lights, no environment: the scope the device implements.
     struct PVert {                                                                    // PathVertex.h:28-60
         V3 p, n, wo, beta, Le;
         float pdfFwd = 0.f, pdfRev = 0.f, minRough = 1.f;
@@ -1490,7 +1490,10 @@ struct Scene {
             V.pdfFwd = pdfFwdArea;
             V.flags = kVSurface | (mat->isDelta() ? kVDelta : 0u);
             V.mat = mat;
-            V.minRough = fmins(vi > 0 ? P.v[vi - 1].minRough : 1.f, 1.f);   // Lambert: roughness() = 1
+            // IMaterial::roughness() (IMaterial.h:112: 1; StandardSurface.h:171: the roughness value),
+            // a delta vertex counting as 1 (BDPTIntegrator.cpp:163-172)
+            const float rough = mat->m.type == AHIP_MAT_STANDARD ? mat->m.roughness : 1.f;
+            V.minRough = fmins(vi > 0 ? P.v[vi - 1].minRough : 1.f, mat->isDelta() ? 1.f : rough);
             if (vi > 0) {
                 float dummy, revSA;
                 evalVertex(P, vi, -wo, dummy, revSA);
@@ -1505,6 +1508,7 @@ struct Scene {
             float uc = smp.get1D();
             V2 u = smp.get2D();
             BSDFSample bs = mat->sample(ctx, wo, u, uc);
+            if (mat->isDelta()) ++smooth;
             if (!bs.valid()) break;
             beta = beta * (bs.f / bs.pdf);
             pdfFwd = bs.pdf;
@@ -1595,6 +1599,15 @@ struct Scene {
         if (isBlack(shadowTr(a)) != isBlack(shadowTr(b))) ++border;
         return vis;
     }
+    // A smooth StandardSurface (roughness < 0.001: isDelta) is sampled as GGX at alpha = 1e-4,
+    // alpha^2 = 1e-8, where alpha^2 - 1 rounds to -1 in float: the sampled half-vector is the normal
+    // itself, and evalCombined's D_GGX of the half-vector rebuilt from (wo, wi) is infinite when
+    // that vector's z rounds to exactly 1 (cos^2 * (alpha^2 - 1) + 1 = 0) and finite when it rounds
+    // to 1 - 2^-24.  An infinite D makes f / pdf = inf / inf = NaN, and the rest of the subpath
+    // then adds nothing (the non-finite contributions are dropped).  Which way the last bit falls
+    // depends on how the shading-frame transforms of wo and wi were rounded, so builds that
+    // contract differently disagree on those samples; `smooth` counts such sampled vertices.
+    mutable uint32_t smooth = 0;
     // BDPTIntegrator::connect (BDPTIntegrator.cpp:308-459) without film splats (t = 1 is unused there)
     V3 connect(const PPath& lp, uint32_t s, const PPath& cp, uint32_t t) const {
         if (t == 0) return {};
@@ -1648,8 +1661,9 @@ struct Scene {
         return lp.v[ls].beta * fL * invD2 * fC * cp.v[ct].beta * V3{1.f, 1.f, 1.f};
     }
     // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581): out[w*h*5] as renderTile above
-    // border (optional, [w*h*ns]): 1 where a connection's visibility of the sample hangs on the last
-    // bits of its hit distance (connVisible).
+    // border (optional, [w*h*ns]): bit 0 where a connection's visibility of the sample hangs on the
+    // last bits of its hit distance (connVisible), bit 1 where a subpath sampled a smooth
+    // StandardSurface (`smooth`).
     void renderTileBDPT(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float clamp,
                         float* out, float* albedo = nullptr, float* normal = nullptr,
                         uint8_t* borderOut = nullptr) const {
@@ -1672,7 +1686,7 @@ struct Scene {
                     smp.get2D();   // lens
                     smp.get1D();   // time
                     Ray primary = generateRay(cam, px, py, jit.x, jit.y);
-                    border = 0;
+                    border = smooth = 0;
                     cameraSubpath(primary, smp, cp);
                     if (cp.count >= 2 && cp.type(1) == kVSurface && cp.v[1].mat) {
                         SI si;
@@ -1704,7 +1718,7 @@ struct Scene {
                     pixL = pixL + sL;
                     float lum = luminance(sL);
                     lsq += lum * lum;
-                    if (borderOut) borderOut[(ty * w + tx) * ns + s] = border ? 1u : 0u;
+                    if (borderOut) borderOut[(ty * w + tx) * ns + s] = (border ? 1u : 0u) | (smooth ? 2u : 0u);
                 }
                 float inv = 1.f / (float)ns, wt = (float)ns;
                 float* o = out + 5 * (ty * w + tx);
@@ -1965,17 +1979,20 @@ int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
 }
 
 // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581) with firefly clamp `clamp` (0 = off):
-// out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and border [w*h*ns] (Scene::connVisible)
-// may be null.  Scenes of triangles with
-// Lambert / Emissive materials and area lights only (returns 1 otherwise).
+// out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and the per-sample flags border [w*h*ns]
+// (bit 0: Scene::connVisible's borderline visibility, bit 1: Scene::smooth) may be null.  Scenes of triangles with
+// Lambert / Emissive / untextured StandardSurface materials and area lights only (returns 1 otherwise).
 int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
                          float clamp, float* out, float* albedo, float* normal, uint8_t* border) {
     const Scene* s = static_cast<Scene*>(h);
     if (!s->curves.nodes.empty() || s->hasEnv || !isBlack(s->envRad)) return 1;
     for (const Light& L : s->lights)
         if (L.type != AHIP_LIGHT_AREA) return 1;
+    if (!s->triAlbedo.empty()) return 1;
     for (const Material& m : s->mats)
-        if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE) return 1;
+        if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE &&
+            !(m.m.type == AHIP_MAT_STANDARD && !m.roughTex))
+            return 1;
     s->renderTileBDPT(x0, y0, w, hh, s0, ns, clamp, out, albedo, normal, border);
     return 0;
 }

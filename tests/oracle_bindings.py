@@ -307,8 +307,9 @@ class OracleScene:
 
     def render_tile_bdpt(self, x0, y0, w, h, s0, ns, clamp=10.0, border=False):
         """BDPTIntegrator::renderTile: (accum [h, w, 5], albedo [h, w, 3], normal [h, w, 3]) and with
-        border=True also [h, w, ns] flags of the samples whose connection visibility hangs on the
-        last bits of a hit distance (Scene::connVisible)."""
+        border=True also [h, w, ns] per-sample flags: bit 0 where a connection's visibility hangs on
+        the last bits of a hit distance (Scene::connVisible), bit 1 where a subpath sampled a smooth
+        StandardSurface, whose throughput hangs on the last bit of a half-vector (Scene::smooth)."""
         out = np.zeros((h, w, 5), np.float32)
         alb = np.zeros((h, w, 3), np.float32)
         nrm = np.zeros((h, w, 3), np.float32)

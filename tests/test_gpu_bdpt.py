@@ -1,11 +1,14 @@
 """The device BDPT (bdpt.hip) against the oracle's restatement of BDPTIntegrator (itself held to the
-reference's own build by tests/test_oracle_bdpt.py), on the Cornell box with one and with two lights.
+reference's own build by tests/test_oracle_bdpt.py), on the Cornell box with one and with two lights,
+and with StandardSurface blocks and a smooth mirror.
 
 Per sample (each sample index rendered on its own on both sides): every sample within 2e-4
-relative except samples whose connection visibility hangs on the last bits of a hit distance (the
-oracle flags them: Scene::connVisible); measured on the MI355X: 20 of 16384 samples beyond 1e-2, all
+relative except samples the oracle flags as hanging on the last bits of a float: a connection whose
+visibility changes within an ulp of the hit distance (Scene::connVisible), or a subpath through a
+smooth StandardSurface, whose throughput is NaN or finite as the rebuilt half-vector's z rounds to 1
+or not (Scene::smooth).  Measured on the MI355X (Lambert): 20 of 16384 samples beyond 1e-2, all
 flagged, the largest error of an unflagged sample 1.3e-4.  Sample weights exact, AOVs as the
-reference records them (first-vertex albedo exact, normals within 1e-6)."""
+reference records them (first-vertex albedo within 1e-6, normals within 1e-6)."""
 import numpy as np
 import pytest
 
@@ -17,29 +20,31 @@ NS = 4
 BDPT = A.AHIP_INTEGRATOR_BDPT
 
 
-@pytest.mark.parametrize("two_lights", [False, True])
-def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights):
-    sc = scenes.cornell(width=64, height=64, second_light=two_lights)
+@pytest.mark.parametrize("two_lights,glossy", [(False, False), (True, False), (True, True)])
+def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights, glossy):
+    """glossy: standard_surface gold metal and coated dielectric blocks and a mirror panel (a delta
+    BSDF, whose connections the reference skips; measured: 64 samples beyond 2e-4, all flagged)."""
+    sc = scenes.cornell(width=64, height=64, second_light=two_lights, glossy=glossy)
     os_ = oracle.scene(sc)
     gpu.prepare(sc, integrator=BDPT)
-    print(f"[bdpt] cornell 64x64 x {NS} spp, two lights {two_lights}", flush=True)
+    print(f"[bdpt] cornell 64x64 x {NS} spp, two lights {two_lights}, glossy {glossy}", flush=True)
     got, alb, nrm = gpu.render_tile_aovs(0, 0, 64, 64, 0, NS)
     ref, oalb, onrm, bd = os_.render_tile_bdpt(0, 0, 64, 64, 0, NS, border=True)
     assert np.array_equal(got[..., 3], ref[..., 3])
     assert abs(got[..., :3].astype(np.float64).mean() / ref[..., :3].astype(np.float64).mean() - 1) < 2e-4
-    np.testing.assert_array_equal(alb, oalb)
+    np.testing.assert_allclose(alb, oalb, rtol=1e-6, atol=0)
     np.testing.assert_allclose(nrm, onrm, rtol=0, atol=1e-6)
     g = np.stack([gpu.render_tile(0, 0, 64, 64, s, 1)[..., :3] for s in range(NS)]).astype(np.float64)
     r = np.stack([os_.render_tile_bdpt(0, 0, 64, 64, s, 1)[0][..., :3] for s in range(NS)]).astype(np.float64)
     np.testing.assert_allclose(g.sum(0), got[..., :3], rtol=1e-5, atol=1e-6)   # the samples add up to the tile
-    border = np.moveaxis(bd, -1, 0).astype(bool)
+    border = np.moveaxis(bd, -1, 0) > 0
     scale = max(np.abs(r).mean(), 1e-6)
     e = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * scale)
     off = e > 2e-4
-    print(f"[bdpt] {int(off.sum())} of {e.size} samples beyond 2e-4, {int((off & border).sum())} flagged; "
-          f"largest unflagged {e[~border].max():.3g}", flush=True)
+    print(f"[bdpt] {int(off.sum())} of {e.size} samples beyond 2e-4, {int((off & border).sum())} flagged "
+          f"({int(border.sum())} flagged in all); largest unflagged {e[~border].max():.3g}", flush=True)
     assert not (off & ~border).any()
-    assert off.sum() <= 3e-3 * e.size
+    assert off.sum() <= (5e-3 if glossy else 3e-3) * e.size
 
 
 def test_bdpt_frame_equals_tile_and_clamp(gpu, oracle):
@@ -65,9 +70,14 @@ def test_bdpt_frame_equals_tile_and_clamp(gpu, oracle):
 def test_bdpt_refusals(gpu):
     """BDPT is refused for scenes it does not take (at ahip_set_params once a scene is prepared, and
     at render time), and by the path-tracer-only entry points."""
-    sc = scenes.ggx_spheres(width=16, height=16, spp=1, grid=2, freq=2, hdri_size=(16, 8))
+    sc = scenes.ggx_spheres(width=16, height=16, spp=1, grid=2, freq=2, hdri_size=(16, 8))   # a dome
     with pytest.raises(Exception):
         gpu.prepare(sc, integrator=BDPT)
+    textured = scenes.cornell(width=16, height=16, glossy=True)
+    textured.materials[4].roughness_texture = 0
+    textured.textures = [np.full((4, 4), 0.3, np.float32)]
+    with pytest.raises(Exception):
+        gpu.prepare(textured, integrator=BDPT)
     gpu.prepare(sc)
     with pytest.raises(Exception):
         gpu.set_params(sc.max_depth, sc.min_depth, sc.seed, integrator=BDPT)
@@ -136,10 +146,9 @@ def test_bdpt_adapter_in_reference_render_loop():
     assert abs(g.mean() / r.mean() - 1) < 2e-4
     np.testing.assert_array_equal(ga, ra)
     np.testing.assert_allclose(gn, rn, rtol=0, atol=1e-6)
-    ggx = scenes.ggx_spheres(width=16, height=16, spp=1, grid=2, freq=2, hdri_size=(16, 8))
-    cam2 = Oracle().pinhole(ggx.cam_from, ggx.cam_at, ggx.cam_up, ggx.vfov, ggx.width, ggx.height)
-    try:
-        h2 = AdapterHarness(ggx, cam2, ggx.max_depth, integrator=BDPT)
-    except RuntimeError:
-        return   # the reference cannot represent it either
-    assert not h2.status()["prepared"]
+    lit = scenes.cornell(width=16, height=16)
+    lit.env_radiance = (0.2, 0.2, 0.2)   # a constant background: an environment, which BDPT refuses
+    h2 = AdapterHarness(lit, Oracle().pinhole(lit.cam_from, lit.cam_at, lit.cam_up, lit.vfov, 16, 16),
+                        lit.max_depth, integrator=BDPT)
+    st2 = h2.status()
+    assert st2["valid"] and not st2["prepared"], st2
