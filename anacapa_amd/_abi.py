@@ -46,9 +46,9 @@ class Material(C.Structure):
         ("base", C.c_float), ("metalness", C.c_float), ("roughness", C.c_float),
         ("specular", C.c_float), ("specular_color", F3), ("specular_ior", C.c_float),
         ("coat", C.c_float), ("coat_roughness", C.c_float), ("transmission", C.c_float),
-        ("roughness_texture", C.c_int32), ("pad", C.c_uint32 * 2),
+        ("roughness_texture", C.c_int32), ("opacity", C.c_float), ("opacity_texture", C.c_int32),
         ("hair_sigma_a", F3), ("hair_eta", C.c_float), ("hair_beta_m", C.c_float), ("hair_beta_n", C.c_float),
-        ("hair_alpha", C.c_float), ("hair_pad", C.c_uint32),
+        ("hair_alpha", C.c_float), ("alpha_mask", C.c_uint32),
     ]
 
 

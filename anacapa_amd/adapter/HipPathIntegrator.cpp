@@ -44,8 +44,8 @@ bool toAhip(const IMaterial* mat, ahip_material& m) {
     if (auto* ss = dynamic_cast<const StandardSurfaceMaterial*>(mat)) {
         const auto& p = ss->params();
         if (p.base_color.hasTexture() || p.metalness.hasTexture() || p.roughness.hasTexture() ||
-            p.specular.hasTexture() || p.has_normal_map || p.alphaMask || p.transmission > 0.001f)
-            return false;   // textures / glass are served by the CPU integrator
+            p.specular.hasTexture() || p.has_normal_map || p.opacity.hasTexture())
+            return false;   // textures are served by the CPU integrator
         m.type = AHIP_MAT_STANDARD;
         put3(m.albedo, p.base_color.value);
         put3(m.emission, p.emission_color.value);
@@ -58,6 +58,10 @@ bool toAhip(const IMaterial* mat, ahip_material& m) {
         m.specular_ior = p.specular_IOR;
         m.coat = p.coat;
         m.coat_roughness = p.coat_roughness;
+        m.transmission = p.transmission;           // glass (StandardSurface.h:248-347)
+        m.alpha_mask = p.alphaMask ? 1u : 0u;      // constant opacity (StandardSurface.h:197-217)
+        m.opacity = p.opacity.value;
+        m.opacity_texture = -1;
         return true;
     }
     if (dynamic_cast<const LambertianMaterial*>(mat)) {

@@ -724,6 +724,8 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     std::memcpy(S.rootMin, ctx->bvh.rootMin, 12);
     std::memcpy(S.rootMax, ctx->bvh.rootMax, 12);
     S.hasEmitters = hs.hasEmitters ? 1u : 0u;
+    S.hasTransparent = hs.hasTransparent ? 1u : 0u;
+    S.hasAlpha = hs.hasAlpha ? 1u : 0u;
     std::memcpy(S.envRadiance, d->env_radiance, 12);
     if (hs.hasDome) {
         const HostDome& D = hs.dome;
@@ -759,6 +761,7 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     else if (hs.hasDome || d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
         ctx->bdptRefusal = "an environment";
     if (ctx->bdptRefusal.empty() && d->tri_albedo) ctx->bdptRefusal = "per-triangle albedo";
+    if (ctx->bdptRefusal.empty() && hs.hasTransparent) ctx->bdptRefusal = "a glass or alpha-masked material";
     for (const DMat& m : hs.mats)
         if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE &&
             !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0))
@@ -1459,6 +1462,8 @@ int replayArgs(ahip_context* ctx, uint32_t s0, uint32_t ns, ReplayArgs& a, uint3
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
     if (!ctx->hasCam) return fail(ctx, AHIP_E_INVALID_ARG, "set_camera first");
     if (ctx->dsc.hasCurves) return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support triangle scenes only");
+    if (ctx->dsc.hasTransparent)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support opaque scenes only (no glass or alpha masks)");
     if (ns == 0) return fail(ctx, AHIP_E_INVALID_ARG, "sample_count must be positive");
     a = ReplayArgs{};
     a.cam = toDCam(ctx->cam);
@@ -1617,6 +1622,8 @@ int ahip_count_traffic(ahip_context* ctx, uint32_t width, uint32_t height, uint3
     if (ctx->params.integrator != AHIP_INTEGRATOR_PATH)
         return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_count_traffic runs the path tracer only");
     if (!ctx->prepared) return fail(ctx, AHIP_E_NOT_PREPARED, "prepare first");
+    if (ctx->dsc.hasTransparent)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "ahip_count_traffic counts opaque scenes only (no glass or alpha masks)");
     if (!ctx->hasCam || width != ctx->cam.width || height != ctx->cam.height)
         return fail(ctx, AHIP_E_INVALID_ARG, "film size differs from the camera's");
     int st = ensureQueues(ctx, 64);

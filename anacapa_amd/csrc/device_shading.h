@@ -249,7 +249,11 @@ __device__ __forceinline__ V3 schlickColor(float c, V3 f0) {
     return f0 + (V3{1.f, 1.f, 1.f} - f0) * x5;
 }
 
+// Bilinear lookup with periodic wrap, v flipped as Texture.cpp:67-68 does.  Uncontracted, so the
+// value is the oracle's bit for bit: an opacity decides the alpha test's branch by comparison
+// (opacity <= 0, sample >= opacity), where one ulp can change which samples a path draws.
 __device__ __forceinline__ float texBilinear(const DTexture& t, V2 uv) {
+#pragma clang fp contract(off)
     float s = uv.x, tt = 1.f - uv.y;
     float fx = s * (float)t.w - 0.5f, fy = tt * (float)t.h - 0.5f;
     float x0f = floorf(fx), y0f = floorf(fy);
@@ -312,6 +316,130 @@ __device__ __forceinline__ V3 evalMixture(const DMat& m, V3 wo, V3 wi, float wc,
 
 __device__ __forceinline__ float matRoughness(const DScene& sc, const DMat& m, V2 uv) {
     return m.roughTex >= 0 ? texBilinear(sc.textures[m.roughTex], uv) : m.rough;
+}
+
+// Exact Fresnel reflectance of a dielectric interface, 1 under total internal reflection
+// (StandardSurface.h:84-92); eta = n_transmitted / n_incident.
+__device__ __forceinline__ float fresnelDielectric(float cosI, float eta) {
+    float sin2I = stdmax(0.f, 1.f - cosI * cosI);
+    float sin2T = sin2I / (eta * eta);
+    if (sin2T >= 1.f) return 1.f;
+    float cosT = sqrtf(1.f - sin2T);
+    float rs = (cosI - eta * cosT) / (cosI + eta * cosT);
+    float rp = (eta * cosI - cosT) / (eta * cosI + cosT);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+// IMaterial::evalOpacity (StandardSurface.h:197-201; every other material is opaque).
+__device__ __forceinline__ float matOpacity(const DScene& sc, const DMat& m, V2 uv) {
+    if (!m.alphaMask) return 1.f;
+    return m.opacityTex >= 0 ? texBilinear(sc.textures[m.opacityTex], uv) : m.opacity;
+}
+
+// IMaterial::transmittanceColor, the tint a shadow ray takes through a surface (StandardSurface.h:
+// 207-225; black = opaque, every other material): 1 - opacity for an alpha mask, base colour x
+// transmission x (1 - Fresnel at normal incidence) for glass.
+__device__ __forceinline__ V3 matTransmittance(const DScene& sc, const DMat& m, V3 base, V2 uv) {
+    if (m.type != AHIP_MAT_STANDARD) return V3{0.f, 0.f, 0.f};
+    if (m.alphaMask) {
+        float t = 1.f - matOpacity(sc, m, uv);
+        if (t <= 0.f) return V3{0.f, 0.f, 0.f};
+        return V3{t, t, t};
+    }
+    if (!m.trans) return V3{0.f, 0.f, 0.f};
+    return base * m.trScale;
+}
+
+// The dielectric glass branch of StandardSurface::sample (StandardSurface.h:248-347): smooth glass
+// (roughness < 0.001) reflects or refracts by the exact Fresnel term (a delta sample), rough glass
+// samples a GGX half-vector and then reflects (Fresnel-weighted GGX) or refracts (Walter et al.
+// 2007 microfacet transmission, no eta^2).  wol.z > 0; eta = IOR entering, 1 / IOR leaving.
+__device__ __noinline__ BSample glassSample(const Frame& fr, V3 wol, float rough, float eta, V3 base, V2 u, float uc) {
+#pragma clang fp contract(off)
+    BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
+    const float cosI = wol.z;
+    if (rough < 0.001f) {
+        const float Fr = fresnelDielectric(cosI, eta);
+        s.delta = true;
+        if (uc < Fr) {
+            s.wi = toWorld(fr, V3{-wol.x, -wol.y, wol.z});
+            s.f = base * Fr;
+            s.pdf = Fr;
+        } else {
+            float sin2T = stdmax(0.f, 1.f - cosI * cosI) / (eta * eta);
+            float cosT = sqrtf(stdmax(0.f, 1.f - sin2T));
+            s.wi = toWorld(fr, V3{-wol.x / eta, -wol.y / eta, -cosT});
+            s.f = base * (1.f - Fr);
+            s.pdf = 1.f - Fr;
+        }
+        return s;
+    }
+    const float alpha = stdmax(1e-4f, rough * rough), a2 = alpha * alpha;
+    const V3 wh = sampleHalf(u, a2);
+    if (wh.z <= 0.f) return s;
+    const float cosIH = stdmax(0.f, dot(wol, wh));
+    if (cosIH <= 0.f) return s;
+    const float Fr = fresnelDielectric(cosIH, eta);
+    if (uc < Fr) {
+        V3 wil = wh * (2.f * cosIH) - wol;
+        if (wil.z <= 0.f) return s;
+        float D = dGGX(wh.z, a2), G2 = g2Smith(wol.z, wil.z, a2);
+        float k = Fr * (D * G2 / (4.f * wol.z));
+        s.wi = toWorld(fr, wil);
+        s.f = V3{k * base.x, k * base.y, k * base.z};
+        s.pdf = Fr * pdfReflect(wh.z, a2, cosIH);
+        return s;
+    }
+    float sin2T = stdmax(0.f, 1.f - cosIH * cosIH) / (eta * eta);
+    if (sin2T >= 1.f) return s;                                              // total internal reflection
+    float cosTH = sqrtf(1.f - sin2T);
+    V3 wil = wol * (-1.f / eta) + wh * (cosIH / eta - cosTH);
+    if (wil.z >= 0.f) return s;
+    const float cosO = wol.z, cosIt = fabsf(wil.z);
+    const float D = dGGX(wh.z, a2), G2 = g2Smith(cosO, cosIt, a2);
+    const float absTH = fabsf(dot(wil, wh));
+    const float denom = cosIH + eta * absTH;
+    if (denom < 1e-6f) return s;
+    const float jac = eta * eta * absTH / (denom * denom);
+    s.wi = toWorld(fr, wil);
+    s.f = base * ((1.f - Fr) * D * G2 * cosIH * absTH / (cosO * denom * denom));
+    s.pdf = (1.f - Fr) * (D * wh.z) * jac;
+    return s;
+}
+
+// The glass branches of StandardSurface::evaluate (StandardSurface.h:410-474): across the surface
+// the rough transmission lobe (smooth glass has no area density there), on the same side the
+// Fresnel-weighted GGX reflection alone.  f without cosine; the forward pdf.
+__device__ __noinline__ V3 glassEval(const DMat& m, bool front, V3 wol, V3 wil, float rough, V3 base, float& pdf) {
+#pragma clang fp contract(off)
+    pdf = 0.f;
+    const V3 zero{0.f, 0.f, 0.f};
+    const float eta = front ? m.eta : 1.f / m.eta;
+    if (!(wol.z * wil.z > 0.f)) {
+        if (!m.trans || rough < 0.001f) return zero;
+        float a2 = rough * rough;
+        a2 *= a2;
+        V3 wh = safeNormalize(wol + wil * eta);
+        if (wh.z < 0.f) wh = -wh;
+        const float cosIH = stdmax(0.f, dot(wol, wh)), cosTH = fabsf(dot(wil, wh));
+        const float Fr = fresnelDielectric(cosIH, eta);
+        const float D = dGGX(wh.z, a2), G2 = g2Smith(wol.z, fabsf(wil.z), a2);
+        const float denom = cosIH + eta * cosTH;
+        if (denom < 1e-6f) return zero;
+        const float k = (1.f - Fr) * D * G2 * cosIH * cosTH / (wol.z * fabsf(wil.z) * denom * denom);
+        const float jac = eta * eta * cosTH / (denom * denom);
+        pdf = (1.f - Fr) * D * wh.z * jac;
+        return base * k;
+    }
+    if (rough < 0.001f) return zero;
+    const float alpha = stdmax(1e-4f, rough * rough), a2 = alpha * alpha;
+    const V3 wh = safeNormalize(wol + wil);
+    if (wh.z < 0.f) return zero;
+    const float cosIH = stdmax(0.f, dot(wol, wh));
+    const float Fr = fresnelDielectric(cosIH, eta);
+    const float D = dGGX(wh.z, a2), G2 = g2Smith(wol.z, wil.z, a2);
+    pdf = Fr > 0.f ? Fr * pdfReflect(wh.z, a2, cosIH) : 0.f;
+    return base * (Fr * D * G2 / (4.f * wol.z * wil.z));
 }
 
 // What a hit contributes to its material: the base colour (the material's, or the hit
@@ -431,6 +559,7 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, c
     V3 wol = toLocal(fr, wo);
     if (wol.z <= 0.f) return s;
     float r = mi.r;
+    if (m.glass) return glassSample(fr, wol, r, fr.front ? m.eta : 1.f / m.eta, mi.base, u, uc);
     float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
     float wc, wm, ws, wd;
     if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return s;
@@ -482,6 +611,10 @@ __device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, const Ma
     if (m.type == AHIP_MAT_HAIR) return HAIRM ? hairMarschner(m, fr, wo, wi, pdf) : hairKK(m, fr, wo, wi, pdf);
     V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
     pdf = 0.f;
+    if (m.type == AHIP_MAT_STANDARD && (m.glass || m.trans)) {
+        // transmission across the surface (m.trans), Fresnel-weighted reflection (m.glass)
+        if (!(wil.z * wol.z > 0.f) || m.glass) return glassEval(m, fr.front, wol, wil, mi.r, mi.base, pdf);
+    }
     if (!(wil.z * wol.z > 0.f)) return V3{0.f, 0.f, 0.f};
     if (m.type != AHIP_MAT_STANDARD) {
         pdf = stdmax(0.f, fabsf(wil.z)) * kInvPi;

@@ -111,6 +111,16 @@ struct DMat {
     float hairV[3];           //   longitudinal variance of R, TT, TRT,
     float hairS;              //   azimuthal logistic scale,
     float hairAlpha[3];       //   cuticle tilts (radians) (MarschnerHair.h:200-216)
+    // Transparency (StandardSurface.h:162-169, 197-225, 248-347, 410-474), flags set on the host:
+    uint32_t glass;           // transmission > 0.001 && metalness < 0.001: the dielectric branch of sample
+                              // and the Fresnel-weighted reflection of evaluate in one hemisphere
+    uint32_t trans;           // !(transmission < 0.001) && !(metalness > 0.001): evaluate's transmission
+                              // lobe across the surface and a non-black transmittanceColor
+    float eta;                // specular_IOR
+    float trScale;            // transmission * (1 - fresnelDielectric(1, IOR)): the shadow tint per unit base colour
+    uint32_t alphaMask;       // Params::alphaMask: evalOpacity = opacity, shadow tint 1 - opacity
+    float opacity;            // constant opacity (opacityTex < 0)
+    int32_t opacityTex;       // -1 or texture slot
 };
 
 struct DLight {
@@ -185,6 +195,9 @@ struct DScene {
     const float* triAlbedo;   // [3 * tris] per-triangle base colour in commit order, or null
     uint32_t hasHair;         // any hair material
     uint32_t curveDepth;      // de Casteljau depth of the ribbon test (the reference's MAX_DEPTH is 6)
+    uint32_t hasTransparent;  // any glass or alpha-masked material: shadow and BSDF-sampled NEE rays step
+                              // through transmissive surfaces (ShadowRay.h:26-61, PathIntegrator.cpp:217-236)
+    uint32_t hasAlpha;        // any alpha-masked material: the bounce loop's pass-through test runs
 };
 
 }  // namespace ahip

@@ -133,10 +133,22 @@ int makePinholeCamera(const float from[3], const float at[3], const float up[3],
     return AHIP_OK;
 }
 
+// fresnelDielectric (StandardSurface.h:84-92): exact Fresnel reflectance of a dielectric interface.
+static float fresnelDielectricHost(float cosI, float eta) {
+    float sin2I = std::max(0.f, 1.f - cosI * cosI);
+    float sin2T = sin2I / (eta * eta);
+    if (sin2T >= 1.f) return 1.f;
+    float cosT = std::sqrt(1.f - sin2T);
+    float rs = (cosI - eta * cosT) / (cosI + eta * cosT);
+    float rp = (eta * cosI - cosT) / (eta * cosI + cosT);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
 int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& D, std::string& err) {
     D = DMat{};
     D.type = m.type;
     D.roughTex = -1;
+    D.opacityTex = -1;
     std::memcpy(D.albedo, m.albedo, 12);
     if (m.type == AHIP_MAT_LAMBERT || m.type == AHIP_MAT_EMISSIVE) {
         if (m.type == AHIP_MAT_EMISSIVE) std::memcpy(D.Le, m.emission, 12);
@@ -157,10 +169,6 @@ int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& D, std::
         D.hairAlpha[1] = m.hair_alpha * kPiF / 180.f * 0.5f;
         D.hairAlpha[2] = -m.hair_alpha * kPiF / 180.f * 1.5f;
     } else if (m.type == AHIP_MAT_STANDARD) {
-        if (m.transmission > 0.001f) {
-            err = "transmissive standard_surface is not supported by this backend";
-            return AHIP_E_UNSUPPORTED;
-        }
         D.base = m.base;
         D.metal = m.metalness;
         D.spec = m.specular;
@@ -173,8 +181,28 @@ int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& D, std::
         D.f0d = r * r;
         float rc = (1.f - 1.5f) / (1.f + 1.5f);
         D.coatF0 = rc * rc;
-        float rr = m.roughness, mm = m.metalness, ss = m.specular;
-        D.isDelta = (rr < 0.001f && (mm > 0.999f || ss > 0.f)) ? 1u : 0u;
+        float rr = m.roughness, mm = m.metalness, ss = m.specular, tr = m.transmission;
+        // isDelta: smooth dielectric glass, else a smooth metal or specular surface (:162-169)
+        D.isDelta = ((rr < 0.001f && tr > 0.001f && mm < 0.001f) || (rr < 0.001f && (mm > 0.999f || ss > 0.f)))
+                        ? 1u : 0u;
+        // the glass branches (:248-347, 410-474) and transmittanceColor (:219-224) test the
+        // transmission and metalness thresholds in these two forms
+        D.glass = (tr > 0.001f && mm < 0.001f) ? 1u : 0u;
+        D.trans = (!(tr < 0.001f) && !(mm > 0.001f)) ? 1u : 0u;
+        D.eta = m.specular_ior;
+        D.trScale = tr * (1.f - fresnelDielectricHost(1.f, m.specular_ior));
+        if (m.alpha_mask) {
+            D.alphaMask = 1u;
+            D.opacity = m.opacity;
+            D.opacityTex = -1;
+            if (m.opacity_texture >= 0) {
+                if ((uint32_t)m.opacity_texture >= numTextures) {
+                    err = "opacity_texture out of range";
+                    return AHIP_E_INVALID_ARG;
+                }
+                D.opacityTex = m.opacity_texture;
+            }
+        }
         D.specularFlag = rr < 0.001f ? 1u : 0u;
         if (m.emission_weight > 0.f)
             for (int k = 0; k < 3; ++k) D.Le[k] = m.emission[k] * m.emission_weight;
@@ -223,6 +251,8 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
         if (st) return st;
         emits[i] = D.hasLe != 0u;
         if (D.type == AHIP_MAT_HAIR) out.hasHair = true;
+        if (D.trans || D.alphaMask) out.hasTransparent = true;
+        if (D.alphaMask) out.hasAlpha = true;
         out.mats.push_back(D);
     }
     out.meshMat.assign(d->mesh_material, d->mesh_material + d->num_meshes);

@@ -35,10 +35,12 @@ struct HostScene {
     bool hasDome = false;
     bool hasEmitters = false;
     bool hasHair = false;
+    bool hasTransparent = false;   // a glass or alpha-masked material
+    bool hasAlpha = false;         // an alpha-masked material
 };
 
-// One material's device record (Lambert / Emissive / opaque StandardSurface / hair); refuses
-// transmissive standard_surface.  Returns an AHIP_E_* status.
+// One material's device record (Lambert / Emissive / StandardSurface, glass and alpha masks
+// included / hair).  Returns an AHIP_E_* status.
 int convertMaterial(const ahip_material& m, uint32_t numTextures, DMat& out, std::string& err);
 
 // Validates the description and builds materials, lights, textures and the dome's sampling

@@ -71,12 +71,23 @@ __device__ __forceinline__ void blockAppend(uint32_t* const* ctrs, const bool* w
     __syncthreads();   // the LDS is reused by the next call
 }
 
-// Path flag in the dimension word of a queued ray: a padding slot outside its view.
+// Path flags in the dimension word of a queued ray: a padding slot outside its view; the last
+// bounce sampled a delta lobe; the first-hit AOVs are written (the first hit an alpha mask does
+// not pass through, PathIntegrator.cpp:104-126).
 constexpr uint32_t kDeadPath = 1u << 30;
 constexpr uint32_t kSpecularPath = 1u << 31;
+constexpr uint32_t kAovDone = 1u << 29;
+constexpr uint32_t kPathFlags = kDeadPath | kSpecularPath | kAovDone;
 
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
+
+// The hit's texture coordinates (BVHBackend.cpp:285-317), uncontracted so that a texture lookup
+// at them is the oracle's bit for bit.
+__device__ __forceinline__ V2 interpUV(const DTriShade& S, float u, float v, float w) {
+#pragma clang fp contract(off)
+    return V2{S.uv0[0] * w + S.uv1[0] * u + S.uv2[0] * v, S.uv0[1] * w + S.uv1[1] * u + S.uv2[1] * v};
+}
 
 // Slot of a view -> pixel: 64 consecutive slots cover one tileW x tileH pixel tile, so a 64-lane
 // wavefront traces one coherent tile at bounce 0.
@@ -219,13 +230,14 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
         V3 shO{0.f, 0.f, 0.f}, shD{0.f, 0.f, 0.f}, shC{0.f, 0.f, 0.f};
         float shTmax = 0.f;
         V3 miO{0.f, 0.f, 0.f}, miD{0.f, 0.f, 0.f}, miC{0.f, 0.f, 0.f};
-        bool specularNext = false;
+        bool specularNext = false, aovDone = false;
         if (active) {
             float4 ra = q.ro[cur][i], rb = q.rd[cur][i], bb = q.beta[cur][i], hh = q.hit[i];
             p0 = __float_as_uint(ra.w);
             uint32_t meta = __float_as_uint(rb.w);
-            dim = meta & ~(kDeadPath | kSpecularPath);
+            dim = meta & ~kPathFlags;
             bool specular = (meta & kSpecularPath) != 0u;
+            aovDone = (meta & kAovDone) != 0u;
             beta = v3(bb);
             V3 dir = v3(rb);
             uint32_t px, py, sidx;
@@ -257,8 +269,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     p = v0 + e1 * u + e2 * v;
                     n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
                     if (dot(n, ng) < 0.f) n = -n;
-                    uv = V2{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v,
-                            SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
+                    uv = interpUV(*SH, u, v, w);
                     fr = makeFrame(n, ng, dir);
                 } else {
                     // curve hit record (CurveBrute.cpp:218-233)
@@ -280,14 +291,33 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     fr = makeCurveFrame(n, tang, dir, V3{info[0], info[1], info[2]});
                     mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 }
-                if (mi >= 0) {
+                bool passThrough = false;
+                if (mi >= 0 && sc.hasAlpha) {
+                    // alpha test (PathIntegrator.cpp:104-118): a cut-out point, or a partly opaque one
+                    // whose opacity is below a sample (drawn only then), continues the ray past the
+                    // surface unshaded, in the same bounce state
+                    const DMat& m = sc.mats[mi];
+                    const float op = matOpacity(sc, m, uv);
+                    passThrough = op <= 0.f ||
+                                  (op < 1.f && haltonSample(c_halton, pd.seed, px, py, sidx, dim++) >= op);
+                    if (passThrough && bounce < pd.maxDepth) {
+                        V3 off = ng * 1e-4f;                     // spawnRay(si.p, si.ng, r.direction)
+                        if (dot(dir, ng) < 0.f) off = -off;
+                        rayO = p + off;
+                        rayD = dir;
+                        specularNext = specular;
+                        cont = true;
+                    }
+                }
+                if (mi >= 0 && !passThrough) {
                     const DMat& m = sc.mats[mi];
                     const MatIn mIn = matInputs(sc, m, uv, commit);
                     V3 wo = -dir;
-                    if (bounce == 0u && q.aovA) {   // first-hit AOVs (PathIntegrator.cpp:121-126)
+                    if (!aovDone && q.aovA) {   // first-hit AOVs (PathIntegrator.cpp:121-126)
                         q.aovA[p0] = f4(matReflectance(m, fr, mIn), 0.f);
                         q.aovN[p0] = f4(n, 0.f);
                     }
+                    aovDone = true;
                     if (m.hasLe && (bounce == 0u || specular)) Ladd = beta * ld3(m.Le);   // :128-132
                     if (!m.isDelta && sc.numLights > 0u) {                                // :134-144
                         float ul = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
@@ -381,7 +411,8 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
         const uint32_t sc0 = slots[0], sc1 = slots[1], sc2 = slots[2];
         if (cont) {
             q.ro[nxt][sc0] = f4(rayO, __uint_as_float(p0));
-            q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? kSpecularPath : 0u)));
+            q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? kSpecularPath : 0u) |
+                                                      (aovDone ? kAovDone : 0u)));
             q.beta[nxt][sc0] = f4(beta, 0.f);
         }
         if (wantShadow) {
@@ -948,6 +979,153 @@ __global__ __launch_bounds__(BS) void k_mis(DScene sc, WaveQueues q, uint32_t bo
     }
 }
 
+// --------------------------------------------------------------------------- transparent scenes
+// In scenes with glass or alpha-masked materials a shadow ray and a BSDF-sampled NEE ray step
+// through transmissive surfaces, closest hit by closest hit, up to 8 of them, multiplying their
+// transmittanceColor tints: shadowTransmittance (ShadowRay.h:26-61) and the step loop of
+// estimateDirect's BSDF strategy (PathIntegrator.cpp:217-236).  Each step re-spawns the ray from
+// the hit point off the geometric normal with the remaining distance less 1e-4.
+struct StepHit {
+    bool hit;
+    float t;
+    V3 p, ng;
+    V2 uv;
+    int32_t mi;          // material index, -1 = none
+    uint32_t commit;     // triangle in commit order (per-triangle albedo), ~0u for a curve
+};
+template <bool CURVES>
+__device__ __forceinline__ StepHit stepClosest(const DScene& sc, const TRay& r, TStack& S, TraceCounters& tc,
+                                               uint32_t* overflow) {
+    StepHit sh;
+    float t = 0.f, u = 0.f, v = 0.f;
+    uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, overflow);
+    if (CURVES && sc.hasCurves) {
+        float best = h != ~0u ? t : r.tMax;
+        CurveHit ch;
+        if (traverseCurves<false, false>(sc, r, S, best, ch, tc, overflow)) {
+            // the curve hit's point and facing normal (CurveBrute.cpp:218-233)
+            V3 dir = r.d, fx, fy;
+            onbNC(dir, fx, fy);
+            V3 nd = -dir, tang = ch.tang;
+            V3 nrm = nd - tang * dot(nd, tang);
+            float ln = len(nrm);
+            nrm = ln > 1e-8f ? nrm * (1.f / ln) : fx;
+            const uint32_t strand = sc.segIds[2u * ch.rec];
+            const uint32_t mesh = __float_as_uint(sc.strandInfo[4u * (size_t)strand + 3u]);
+            sh.hit = true;
+            sh.t = best;
+            sh.p = r.o + dir * best;
+            sh.ng = nrm;
+            sh.uv = V2{0.f, ch.tSeg};
+            sh.mi = mesh < sc.numMeshes ? sc.meshMat[mesh] : -1;
+            sh.commit = ~0u;
+            return sh;
+        }
+    }
+    sh.hit = h != ~0u;
+    if (!sh.hit) return sh;
+    const DTri* T = sc.tris + h;
+    const DTriShade* SH = sc.shade + h;
+    const float w = 1.f - u - v;
+    sh.t = t;
+    sh.p = ld3(T->v0) + ld3(T->e1) * u + ld3(T->e2) * v;
+    sh.ng = ld3(T->n);
+    sh.uv = interpUV(*SH, u, v, w);
+    sh.mi = T->matIndex;
+    sh.commit = T->commitIndex;
+    return sh;
+}
+// transmittanceColor of a step's surface (black: opaque, or no material)
+__device__ __forceinline__ V3 stepTint(const DScene& sc, const StepHit& h) {
+    if (h.mi < 0) return V3{0.f, 0.f, 0.f};
+    const DMat& m = sc.mats[h.mi];
+    return matTransmittance(sc, m, matInputs(sc, m, h.uv, h.commit).base, h.uv);
+}
+__device__ __forceinline__ V3 stepOrigin(const StepHit& h, V3 d) {   // spawnRay(si.p, si.ng, d)
+    V3 off = h.ng * 1e-4f;
+    if (dot(d, h.ng) < 0.f) off = -off;
+    return h.p + off;
+}
+
+template <uint32_t BS, uint32_t K, bool CURVES>
+__global__ __launch_bounds__(BS) void k_shadow_tr(DScene sc, WaveQueues q, uint32_t bounce) {
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, 1);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        const float4 a = q.so[i], b = q.sd[i];
+        V3 o = v3(a);
+        const V3 d = v3(b);
+        float tMax = b.w;
+        V3 T{1.f, 1.f, 1.f};
+        bool blocked = false;
+        for (int step = 0; step < 8; ++step) {                   // ShadowRay.h:31-58
+            const StepHit h = stepClosest<CURVES>(sc, makeTRay(o, d, 1e-4f, tMax), S, tc, ctr(q, bounce, 3));
+            if (!h.hit) break;
+            const V3 tint = stepTint(sc, h);
+            if (isBlack(tint)) { blocked = true; break; }
+            T = T * tint;
+            if (isBlack(T)) { blocked = true; break; }
+            const float remaining = tMax - h.t;
+            if (remaining <= 1e-4f) break;
+            o = stepOrigin(h, d);
+            tMax = remaining - 1e-4f;
+        }
+        if (!blocked) {
+            const uint32_t p0 = __float_as_uint(a.w);
+            const V3 add = v3(q.sc[i]) * T;
+            const float4 l = q.L[p0];
+            q.L[p0] = make_float4(l.x + add.x, l.y + add.y, l.z + add.z, 0.f);
+        }
+    }
+}
+
+template <uint32_t BS, uint32_t K, bool CURVES>
+__global__ __launch_bounds__(BS) void k_mis_tr(DScene sc, WaveQueues q, uint32_t bounce) {
+    __shared__ uint64_t lstk[K * BS];
+    TStack S{lstk, q.ovfRef, q.ovfT, threadIdx.x, BS, K, blockIdx.x * BS + threadIdx.x, gridDim.x * BS,
+             q.stackCap};
+    const uint32_t count = *ctr(q, bounce, 2);
+    TraceCounters tc{0, 0};
+    for (uint32_t base = blockIdx.x * BS; base < count; base += gridDim.x * BS) {
+        uint32_t i = base + threadIdx.x;
+        if (i >= count) break;
+        const float4 a = q.mo[i], b = q.md[i];
+        V3 o = v3(a);
+        const V3 d = v3(b);
+        float tMax = b.w;
+        V3 Tr{1.f, 1.f, 1.f};
+        StepHit h;
+        h.hit = false;
+        for (int step = 0; step < 8; ++step) {                   // PathIntegrator.cpp:217-236
+            h = stepClosest<CURVES>(sc, makeTRay(o, d, 1e-4f, tMax), S, tc, ctr(q, bounce, 3));
+            if (!h.hit) break;
+            if (h.mi < 0) { Tr = V3{0.f, 0.f, 0.f}; break; }
+            const V3 tint = stepTint(sc, h);
+            if (isBlack(tint)) break;                            // opaque or emitter: stop here
+            Tr = Tr * tint;
+            if (isBlack(Tr)) break;
+            const float remaining = tMax - h.t;
+            if (remaining <= 1e-4f) { h.hit = false; break; }
+            o = stepOrigin(h, d);
+            tMax = remaining - 1e-4f;
+        }
+        V3 Li{0.f, 0.f, 0.f};
+        if (!h.hit) Li = envLe(sc, d);
+        else if (h.mi >= 0 && sc.mats[h.mi].hasLe) Li = ld3(sc.mats[h.mi].Le);
+        if (!isBlack(Li) && !isBlack(Tr)) {
+            const uint32_t p0 = __float_as_uint(a.w);
+            const V3 add = v3(q.mc[i]) * Li * Tr;
+            const float4 l = q.L[p0];
+            q.L[p0] = make_float4(l.x + add.x, l.y + add.y, l.z + add.z, 0.f);
+        }
+    }
+}
+
 // --------------------------------------------------------------------------- accumulate
 // PathIntegrator::renderTile's per-pixel sums (PathIntegrator.cpp:27-57) for this pass.
 __global__ __launch_bounds__(256) void k_accumulate(WaveQueues q) {
@@ -1086,7 +1264,8 @@ __global__ __launch_bounds__(256) void k_material_probe(DMat m, uint32_t n, cons
     float pdf;
     V3 f = matEval<HAIRM>(sc, m, fr, mi, wo, wi, pdf);
     BSample bs = matSample<HAIRM>(sc, m, fr, mi, wo, u, uc);
-    V3 sa = m.type == AHIP_MAT_HAIR ? hairSigmaA(m, fr.color) : V3{0.f, 0.f, 0.f};
+    DScene none{};   // the probe's materials read no texture
+    V3 sa = m.type == AHIP_MAT_HAIR ? hairSigmaA(m, fr.color) : matTransmittance(none, m, mi.base, V2{0.f, 0.f});
     float* o = out + 16u * (size_t)i;
     const bool valid = bsValid(bs);
     o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = pdf; o[4] = 0.f;
@@ -1294,7 +1473,12 @@ void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, 
     }
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    if (curveRounds(sc, q)) {
+    if (sc.hasTransparent) {
+        if (sc.hasCurves)
+            hipLaunchKernelGGL((k_shadow_tr<kTraceBS, kStackKCurves, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+        else
+            hipLaunchKernelGGL((k_shadow_tr<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    } else if (curveRounds(sc, q)) {
         launchCurveQuery<kQShadow, true>(sc, q, bounce, 0u, blocks, s);
         hipLaunchKernelGGL(k_shadow_resolve, dim3(blocks), dim3(256), 0, s, q, bounce);
     } else if (sc.hasCurves)
@@ -1303,7 +1487,12 @@ void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32
         hipLaunchKernelGGL((k_shadow<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
 }
 void launchMis(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {
-    if (curveRounds(sc, q)) {
+    if (sc.hasTransparent) {
+        if (sc.hasCurves)
+            hipLaunchKernelGGL((k_mis_tr<kTraceBS, kStackKCurves, true>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+        else
+            hipLaunchKernelGGL((k_mis_tr<kTraceBS, kStackK, false>), dim3(blocks), dim3(kTraceBS), 0, s, sc, q, bounce);
+    } else if (curveRounds(sc, q)) {
         if (sc.hasEmitters) {
             launchCurveQuery<kQMis, false>(sc, q, bounce, 0u, blocks, s);
             hipLaunchKernelGGL((k_mis_resolve<true>), dim3(blocks), dim3(256), 0, s, sc, q, bounce);

@@ -178,6 +178,23 @@ def standard_surface(base_color, metalness=0.0, roughness=0.5, specular=1.0, ior
     return m
 
 
+def glass(base_color=(1.0, 1.0, 1.0), roughness=0.0, ior=1.5, transmission=1.0) -> A.Material:
+    """Dielectric glass: standard_surface with transmission, no metal (StandardSurface.h:248-347)."""
+    m = standard_surface(base_color, metalness=0.0, roughness=roughness, specular=1.0, ior=ior)
+    m.transmission = transmission
+    return m
+
+
+def alpha_masked(base_color, opacity=1.0, opacity_texture=-1, roughness=0.5) -> A.Material:
+    """standard_surface with Params::alphaMask: opacity a constant or a texture (StandardSurface.h:141-143,
+    197-217)."""
+    m = standard_surface(base_color, roughness=roughness)
+    m.alpha_mask = 1
+    m.opacity = opacity
+    m.opacity_texture = opacity_texture
+    return m
+
+
 def hair(sigma_a=(0.40, 0.65, 1.35), eta=1.55, beta_m=0.3, beta_n=0.3, alpha=2.0) -> A.Material:
     """MarschnerHairMaterial::Params defaults (src/shading/MarschnerHair.h:193-199)."""
     m = A.Material()
@@ -328,6 +345,41 @@ def cornell(width=512, height=512, spp=64, max_depth=8, seed=0, second_light=Fal
     return mb.scene("cornell", materials=mats, lights=lights, cam_from=(0.5, 0.5, -1.4),
                     cam_at=(0.5, 0.5, 0.0), cam_up=(0.0, 1.0, 0.0), vfov=40.0, width=width,
                     height=height, spp=spp, max_depth=max_depth, seed=seed)
+
+
+def cornell_glass(width=512, height=512, spp=64, max_depth=8, seed=0, alpha_texture=False) -> Scene:
+    """The Cornell box with transparency: the tall block smooth glass (IOR 1.5, a delta BSDF), the short
+    block rough tinted glass (roughness 0.3), and two alpha-masked panels between the blocks and the back
+    wall: a partly opaque one (opacity 0.45, the stochastic pass-through) and a cut-out one (opacity 0).
+    alpha_texture: the partly opaque panel reads its opacity from an 8x8 texture of 0 / 0.45 / 1 squares
+    instead (texture 0; only this build's lookup reads it)."""
+    mats = [lambert((0.73, 0.73, 0.73)), lambert((0.63, 0.065, 0.05)), lambert((0.14, 0.45, 0.09)),
+            emissive((0.0, 0.0, 0.0), (17.0, 12.0, 4.0)),
+            glass(roughness=0.0), glass((0.85, 0.95, 0.9), roughness=0.3, ior=1.45),
+            alpha_masked((0.9, 0.6, 0.2), opacity=0.45, opacity_texture=0 if alpha_texture else -1),
+            alpha_masked((0.2, 0.3, 0.9), opacity=0.0)]
+    textures = []
+    if alpha_texture:
+        ii, jj = np.meshgrid(np.arange(8), np.arange(8), indexing="ij")
+        textures.append(np.choose((ii + 2 * jj) % 3, [0.0, 0.45, 1.0]).astype(np.float32))
+    mb = _MeshBuilder()
+    _quad(mb, (0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), 0)      # floor
+    _quad(mb, (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, 1, 0), (0, -1, 0), 0)     # ceiling
+    _quad(mb, (0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1), (0, 0, -1), 0)     # back
+    _quad(mb, (0, 0, 1), (0, 1, 1), (0, 1, 0), (0, 0, 0), (1, 0, 0), 1)      # left (red)
+    _quad(mb, (1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 2)     # right (green)
+    # the glass blocks float 2 mm above the floor: a bottom face on the floor plane would tie with the
+    # floor for rays inside the glass, a tie decided by the last bit of each build's hit distance
+    _block(mb, (0.13, 0.002, 0.12), (0.43, 0.3, 0.42), -18.0, 5)            # short block: rough glass
+    _block(mb, (0.53, 0.002, 0.45), (0.83, 0.6, 0.75), 18.0, 4)             # tall block: smooth glass
+    _quad(mb, (0.08, 0.3, 0.9), (0.08, 0.8, 0.9), (0.48, 0.8, 0.9), (0.48, 0.3, 0.9), (0, 0, -1), 6)
+    _quad(mb, (0.55, 0.75, 0.95), (0.55, 0.95, 0.95), (0.9, 0.95, 0.95), (0.9, 0.75, 0.95), (0, 0, -1), 7)
+    y = 0.999
+    _quad(mb, (0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625), (0, -1, 0), 3)
+    lights = [area_light((0.5, y, 0.5), (0.125, 0.0, 0.0), (0.0, 0.0, 0.125), (17.0, 12.0, 4.0))]
+    return mb.scene("cornell_glass", materials=mats, lights=lights, cam_from=(0.5, 0.5, -1.4),
+                    cam_at=(0.5, 0.5, 0.0), cam_up=(0.0, 1.0, 0.0), vfov=40.0, width=width,
+                    height=height, spp=spp, max_depth=max_depth, seed=seed, textures=textures)
 
 
 def geodesic_sphere(freq: int):

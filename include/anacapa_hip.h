@@ -66,15 +66,22 @@ typedef struct ahip_material {
     float    specular_ior;
     float    coat;
     float    coat_roughness;
-    float    transmission;
+    float    transmission;      /* > 0.001 with metalness < 0.001: dielectric glass (smooth below
+                                   roughness 0.001, else GGX reflection + microfacet transmission,
+                                   StandardSurface.h:248-347, 410-474); shadow rays step through
+                                   it with the tint transmittanceColor gives (:207-225) */
     int32_t  roughness_texture; /* -1 = constant roughness; otherwise index of a texture */
-    uint32_t pad[2];
+    float    opacity;           /* Params::opacity (StandardSurface.h:141), read when alpha_mask */
+    int32_t  opacity_texture;   /* -1 = constant opacity; otherwise index of a texture (evalTOV) */
     float    hair_sigma_a[3];   /* MarschnerHairMaterial::Params (MarschnerHair.h:193-199) */
     float    hair_eta;
     float    hair_beta_m;
     float    hair_beta_n;
     float    hair_alpha;        /* degrees */
-    uint32_t hair_pad;
+    uint32_t alpha_mask;        /* StandardSurface Params::alphaMask (StandardSurface.h:143): evalOpacity
+                                   = opacity (:197-201), a stochastic pass-through in the bounce loop
+                                   (PathIntegrator.cpp:104-118) and a tint 1 - opacity for shadow
+                                   rays (:213-217); 0 = opaque (evalOpacity = 1) */
 } ahip_material;
 
 typedef struct ahip_light {
@@ -321,7 +328,8 @@ int ahip_halton_eval(ahip_context* ctx, uint64_t seed, const uint32_t* px, const
  *     AHIP_FLAG_HAIR_MARSCHNER as for rendering.  in[n*19] = normal, tangent, strand colour, wo, wi,
  *     u.x, u.y, uComponent, curve (0/1: a curve row takes the tangent as its fiber, ShadingContext.h:
  *     39-44); out[n*16] = evaluate(wo, wi): f rgb, pdf, 0 | sample(wo, u, uComponent): wi, f rgb, pdf,
- *     valid | hair effectiveSigmaA rgb (zero otherwise).  The frame is the integrator's for a hit
+ *     valid | hair effectiveSigmaA rgb, else transmittanceColor rgb (StandardSurface.h:207-225: glass
+ *     and alpha-masked surfaces; black for opaque ones).  The frame is the integrator's for a hit
  *     seen along -wo. --- */
 int ahip_material_probe(ahip_context* ctx, const ahip_material* mat, uint32_t flags, uint32_t n, const float* in,
                         float* out);

@@ -338,6 +338,7 @@ struct Consts {
 struct Material {
     ahip_material m;
     const Texture* roughTex = nullptr;
+    const Texture* opacityTex = nullptr;
     // derived (StandardSurface.h:152-160)
     float coatA2 = 0.f, f0d = 0.f, coatF0 = 0.f;
     bool marschner = false;   // hair: R/TT/TRT instead of the reference's runtime Kajiya-Kay
@@ -359,6 +360,121 @@ struct Material {
         return r < 0.001f && (mm > 0.999f || s > 0.f);
     }
     bool flagsSpecular() const { return m.type == AHIP_MAT_STANDARD && m.roughness < 0.001f; }
+    // the dielectric branch of sample and of same-side evaluate (StandardSurface.h:248, 457)
+    bool glass() const { return m.type == AHIP_MAT_STANDARD && m.transmission > 0.001f && m.metalness < 0.001f; }
+    // evaluate across the surface (:412-421) and transmittanceColor (:219-221) pass these tests
+    bool trans() const { return m.type == AHIP_MAT_STANDARD && !(m.transmission < 0.001f) && !(m.metalness > 0.001f); }
+    // IMaterial::evalOpacity (StandardSurface.h:197-201; 1 for every other material)
+    float opacity(V2 uv) const {
+        if (m.type != AHIP_MAT_STANDARD || !m.alpha_mask) return 1.f;
+        return opacityTex ? opacityTex->sample(uv) : m.opacity;
+    }
+    // IMaterial::transmittanceColor (StandardSurface.h:207-225; black for every other material)
+    V3 transmittance(const Ctx& c) const {
+        if (m.type != AHIP_MAT_STANDARD) return {};
+        if (m.alpha_mask) {
+            float t = 1.f - opacity(c.uv);
+            if (t <= 0.f) return {};
+            return {t, t, t};
+        }
+        if (!trans()) return {};
+        float Fr = fresnelDielectric(1.f, m.specular_ior);
+        return baseOf(c) * (m.transmission * (1.f - Fr));
+    }
+    // StandardSurface::sample, glass branch (StandardSurface.h:248-347)
+    BSDFSample glassSample(const Ctx& c, V3 wo_l, V2 u, float uc) const {
+        BSDFSample s;
+        V3 base = baseOf(c);
+        float r = rough(c.uv);
+        float eta = c.front ? m.specular_ior : (1.f / m.specular_ior);
+        float cosI = wo_l.z;
+        if (r < 0.001f) {                                                             // :257-288
+            float Fr = fresnelDielectric(cosI, eta);
+            s.delta = true;
+            if (uc < Fr) {
+                s.wi = c.toWorld(V3{-wo_l.x, -wo_l.y, wo_l.z});
+                s.f = base * Fr;
+                s.pdf = Fr;
+            } else {
+                float sin2T = fmaxs(0.f, 1.f - cosI * cosI) / (eta * eta);
+                float cosT = std::sqrt(fmaxs(0.f, 1.f - sin2T));
+                s.wi = c.toWorld(V3{-wo_l.x / eta, -wo_l.y / eta, -cosT});
+                s.f = base * (1.f - Fr);
+                s.pdf = 1.f - Fr;
+            }
+            return s;
+        }
+        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;                        // :290-346
+        V3 wh = sampleGGXh(u, a2);
+        if (wh.z <= 0.f) return {};
+        float cosIH = fmaxs(0.f, dot(wo_l, wh));
+        if (cosIH <= 0.f) return {};
+        float Fr = fresnelDielectric(cosIH, eta);
+        if (uc < Fr) {
+            V3 wil = wh * (2.f * cosIH) - wo_l;
+            if (wil.z <= 0.f) return {};
+            float D = D_GGX(wh.z, a2), G = G2(wo_l.z, wil.z, a2);
+            V3 f0{Fr, Fr, Fr};
+            V3 bsdf = f0 * (D * G / (4.f * wo_l.z));
+            s.wi = c.toWorld(wil);
+            s.f = bsdf * base;
+            s.pdf = Fr * pdfGGXr(wh.z, a2, cosIH);
+            return s;
+        }
+        float sin2T = fmaxs(0.f, 1.f - cosIH * cosIH) / (eta * eta);
+        if (sin2T >= 1.f) return {};
+        float cosTH = std::sqrt(1.f - sin2T);
+        V3 wil = wo_l * (-1.f / eta) + wh * (cosIH / eta - cosTH);
+        if (wil.z >= 0.f) return {};
+        float cosO = wo_l.z, cosIt = std::abs(wil.z);
+        float D = D_GGX(wh.z, a2), G = G2(cosO, cosIt, a2);
+        float denom = cosIH + eta * std::abs(dot(wil, wh));
+        if (denom < 1e-6f) return {};
+        float absTH = std::abs(dot(wil, wh));
+        float pdfWh = D * wh.z;
+        float jac = eta * eta * absTH / (denom * denom);
+        s.wi = c.toWorld(wil);
+        s.f = base * ((1.f - Fr) * D * G * cosIH * absTH / (cosO * denom * denom));
+        s.pdf = (1.f - Fr) * pdfWh * jac;
+        return s;
+    }
+    // StandardSurface::evaluate, glass branches (StandardSurface.h:410-474)
+    BSDFEval glassEval(const Ctx& c, V3 wol, V3 wil) const {
+        BSDFEval e;
+        float r = rough(c.uv);
+        float eta = c.front ? m.specular_ior : (1.f / m.specular_ior);
+        V3 base = baseOf(c);
+        if (!(wol.z * wil.z > 0.f)) {                                                 // :412-444
+            if (!trans() || r < 0.001f) return e;
+            float a2 = r * r;
+            a2 *= a2;
+            V3 wh = safeNormalize(wol + wil * eta);
+            if (wh.z < 0.f) wh = -wh;
+            float cosIH = fmaxs(0.f, dot(wol, wh)), cosTH = std::abs(dot(wil, wh));
+            float Fr = fresnelDielectric(cosIH, eta);
+            float D = D_GGX(wh.z, a2), G = G2(wol.z, std::abs(wil.z), a2);
+            float denom = cosIH + eta * cosTH;
+            if (denom < 1e-6f) return e;
+            float k = (1.f - Fr) * D * G * cosIH * cosTH / (wol.z * std::abs(wil.z) * denom * denom);
+            float jac = eta * eta * cosTH / (denom * denom);
+            e.f = base * k;
+            e.pdf = (1.f - Fr) * D * wh.z * jac;
+            e.pdfRev = e.pdf;
+            return e;
+        }
+        if (r < 0.001f) return e;                                                     // :457-474
+        float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
+        V3 wh = safeNormalize(wol + wil);
+        if (wh.z < 0.f) return e;
+        float cosIH = fmaxs(0.f, dot(wol, wh));
+        float Fr = fresnelDielectric(cosIH, eta);
+        float D = D_GGX(wh.z, a2), G = G2(wol.z, wil.z, a2);
+        float pdf = Fr > 0.f ? Fr * pdfGGXr(wh.z, a2, cosIH) : 0.f;
+        e.f = base * (Fr * D * G / (4.f * wol.z * wil.z));
+        e.pdf = pdf;
+        e.pdfRev = pdf;
+        return e;
+    }
     // IMaterial::reflectance, the first-hit denoising albedo: LambertianMaterial::m_albedo
     // (Lambertian.h:76-78, inherited by EmissiveMaterial), StandardSurface.h:189-195,
     // MarschnerHair.h:224-232 (exp(-effectiveSigmaA)).
@@ -537,6 +653,7 @@ struct Material {
         // StandardSurface::sample, opaque branch (StandardSurface.h:231-241, 351-405)
         V3 wo_l = c.toLocal(wo);
         if (wo_l.z <= 0.f) return {};
+        if (glass()) return glassSample(c, wo_l, u, uc);
         V3 base = baseOf(c);
         float metal = m.metalness, spec = m.specular, r = rough(c.uv);
         float alpha = fmaxs(1e-4f, r * r), a2 = alpha * alpha;
@@ -597,7 +714,9 @@ struct Material {
             e.pdfRev = cosPdf(std::abs(wol.z));
             return e;
         }
-        // StandardSurface::evaluate opaque branch (StandardSurface.h:410-417, 448-503)
+        // StandardSurface::evaluate: glass across the surface or on the same side, else the
+        // opaque branch (StandardSurface.h:410-417, 448-503)
+        if ((!(wol.z * wil.z > 0.f) && trans()) || ((wol.z * wil.z > 0.f) && glass())) return glassEval(c, wol, wil);
         if (!(wol.z * wil.z > 0.f)) return e;
         V3 base = baseOf(c);
         float metal = m.metalness, spec = m.specular, r = rough(c.uv);
@@ -1017,14 +1136,24 @@ struct Scene {
     }
     V3 envLe(V3 d) const { return hasEnv ? dome.eval(d) * dome.intensity : envRad; }
 
-    // shadowTransmittance (integrator/ShadowRay.h:26-61) for opaque materials
+    // shadowTransmittance (integrator/ShadowRay.h:26-61): step through up to 8 transmissive
+    // surfaces, multiplying their tints; black at an opaque one.
     V3 shadowTr(Ray ray) const {
         V3 T{1.f, 1.f, 1.f};
         for (int i = 0; i < 8; ++i) {
             SI si;
             if (!traceSI(ray, si)) break;
-            if (!matOf(si.meshID)) return {};
-            return {};   // transmittanceColor() is black for every supported material
+            const Material* mat = matOf(si.meshID);
+            if (!mat) return {};
+            Ctx ctx(si, ray.d);
+            V3 tint = mat->transmittance(ctx);
+            if (isBlack(tint)) return {};
+            T = T * tint;
+            if (isBlack(T)) return {};
+            float remaining = ray.tMax - si.t;
+            if (remaining <= 1e-4f) break;
+            ray = spawnRay(si.p, si.ng, ray.d);
+            ray.tMax = remaining - 1e-4f;
         }
         return T;
     }
@@ -1067,11 +1196,21 @@ struct Scene {
                     V3 Tr{1.f, 1.f, 1.f};
                     SI hs;
                     bool hit = false;
-                    for (int step = 0; step < 8; ++step) {
+                    for (int step = 0; step < 8; ++step) {                           // :217-236
                         hit = traceSI(sr, hs);
                         if (!hit) break;
-                        if (!matOf(hs.meshID)) { Tr = {}; break; }
-                        break;   // opaque / emitter
+                        const Material* m = matOf(hs.meshID);
+                        if (!m) { Tr = {}; break; }
+                        Ctx hctx(hs, sr.d);
+                        V3 tint = m->transmittance(hctx);
+                        if (isBlack(tint)) break;   // opaque / emitter
+                        Tr = Tr * tint;
+                        if (isBlack(Tr)) break;
+                        float remaining = sr.tMax - hs.t;
+                        if (remaining <= 1e-4f) { hit = false; break; }
+                        Ray nr = spawnRay(hs.p, hs.ng, sr.d);
+                        nr.tMax = remaining - 1e-4f;
+                        sr = nr;
                     }
                     V3 Li;
                     if (!hit) Li = envLe(bs.wi);
@@ -1091,7 +1230,7 @@ struct Scene {
     // camera ray misses or hits a mesh without material (PathIntegrator.cpp:121-126).
     V3 Li(Ray r, Halton& smp, uint32_t* rays = nullptr, V3* outAlb = nullptr, V3* outN = nullptr) const {
         V3 L, beta{1.f, 1.f, 1.f};
-        bool specular = false;
+        bool specular = false, firstHit = true;
         for (uint32_t bounce = 0; bounce <= maxDepth; ++bounce) {
             SI si;
             if (rays) ++*rays;
@@ -1103,10 +1242,19 @@ struct Scene {
             if (!mat) break;
             V3 wo = -r.d;
             Ctx ctx(si, r.d);
-            if (bounce == 0 && outAlb) {
+            {   // alpha test (PathIntegrator.cpp:104-118); a sample is drawn only for partial opacity
+                float op = mat->opacity(ctx.uv);
+                bool pass = op <= 0.f || (op < 1.f && smp.get1D() >= op);
+                if (pass) {
+                    r = spawnRay(si.p, si.ng, r.d);
+                    continue;
+                }
+            }
+            if (firstHit && outAlb) {
                 *outAlb = mat->reflectance(ctx);
                 *outN = si.n;
             }
+            firstHit = false;
             V3 Le = mat->Le();
             if (!isBlack(Le) && (bounce == 0 || specular)) L = L + beta * Le;
             if (!mat->isDelta() && !lights.empty()) {
@@ -1832,10 +1980,13 @@ void* orc_scene_create(const ahip_scene_desc* d) {
     for (uint32_t i = 0; i < d->num_materials; ++i) {
         Material m;
         m.m = d->materials[i];
-        if (m.m.type == AHIP_MAT_STANDARD && m.m.transmission > 0.001f) return nullptr;   // opaque only
         if (m.m.roughness_texture >= 0) {
             if ((uint32_t)m.m.roughness_texture >= d->num_textures) return nullptr;
             m.roughTex = &s->textures[m.m.roughness_texture];
+        }
+        if (m.m.type == AHIP_MAT_STANDARD && m.m.alpha_mask && m.m.opacity_texture >= 0) {
+            if ((uint32_t)m.m.opacity_texture >= d->num_textures) return nullptr;
+            m.opacityTex = &s->textures[m.m.opacity_texture];
         }
         m.init();
         s->mats.push_back(m);
@@ -2242,7 +2393,8 @@ int orc_probe(const ahip_material* mat, uint32_t flags, uint32_t n, const float*
         Ctx c(si, -wo);
         BSDFEval e = m.evaluate(c, wo, wi);
         BSDFSample bs = m.sample(c, wo, V2{q[15], q[16]}, q[17]);
-        V3 sa = m.m.type == AHIP_MAT_HAIR ? m.hairSigmaA(c) : V3{};
+        // slots 13..15: hair effectiveSigmaA, else transmittanceColor
+        V3 sa = m.m.type == AHIP_MAT_HAIR ? m.hairSigmaA(c) : m.transmittance(c);
         float* o = out + 16 * i;
         o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf; o[4] = e.pdfRev;
         o[5] = bs.wi.x; o[6] = bs.wi.y; o[7] = bs.wi.z;

@@ -142,7 +142,9 @@ int ref_generate_rays(const ahip_camera* c, uint32_t n, const uint32_t* px, cons
     return 0;
 }
 
-void* ref_scene_create(const ahip_scene_desc* d) { return buildRefScene(d).release(); }
+std::unique_ptr<IMaterial> makeHarnessStandardSurface(const ahip_material& a);
+
+void* ref_scene_create(const ahip_scene_desc* d) { return buildRefScene(d, makeHarnessStandardSurface).release(); }
 
 void ref_scene_destroy(void* h) { delete static_cast<RefScene*>(h); }
 
@@ -461,6 +463,10 @@ struct SSHolder {
         m.m_p.coat = a.coat;
         m.m_p.coat_roughness = a.coat_roughness;
         m.m_p.transmission = a.transmission;
+        m.m_p.emission = a.emission_weight;
+        m.m_p.emission_color = Spectrum{a.emission[0], a.emission[1], a.emission[2]};
+        m.m_p.opacity = a.opacity;
+        m.m_p.alphaMask = a.alpha_mask != 0u;
         // StandardSurfaceMaterial(const Params&), StandardSurface.h:152-160
         float r = std::max(1e-4f, m.m_p.roughness.value);
         m.m_alpha = r * r;
@@ -471,6 +477,20 @@ struct SSHolder {
         m.m_coatF0 = F0_fromIOR(1.5f);
     }
     ~SSHolder() { s.m.m_p.~Params(); }
+    // evalOpacity and transmittanceColor with a constant opacity (StandardSurface.h:197-225)
+    float opacity() const { return s.m.m_p.alphaMask ? s.m.m_p.opacity.value : 1.f; }
+    Spectrum transmittance() const {
+        const StandardSurfaceMaterial& m = s.m;
+        if (m.m_p.alphaMask) {
+            float t = 1.f - m.m_p.opacity.value;
+            if (t <= 0.f) return {};
+            return {t, t, t};
+        }
+        if (m.m_p.transmission < 0.001f) return {};
+        if (m.m_p.metalness.value > 0.001f) return {};
+        float Fr = fresnelDielectric(1.f, m.m_p.specular_IOR);
+        return m.m_p.base_color.value * (m.m_p.transmission * (1.f - Fr));
+    }
     SSHolder(const SSHolder&) = delete;
     SSHolder& operator=(const SSHolder&) = delete;
 
@@ -494,15 +514,56 @@ struct SSHolder {
         return true;
     }
 
-    // evaluate, opaque branch with constant parameters (StandardSurface.h:410-417, 446-503)
+    // evaluate with constant parameters: the glass branches (StandardSurface.h:410-474) on the
+    // reference's free helpers, else the opaque branch (:446-503)
     BSDFEval evaluate(const ShadingContext& ctx, Vec3f wo, Vec3f wi) const {
         const StandardSurfaceMaterial& m = s.m;
         Vec3f woLocal = ctx.toLocal(wo), wiLocal = ctx.toLocal(wi);
-        if (!sameHemisphere(woLocal, wiLocal)) return {};
         Spectrum base_color = m.m_p.base_color.value;
         float metal = m.m_p.metalness.value, spec = m.m_p.specular.value, rough = m.m_p.roughness.value;
+        if (!sameHemisphere(woLocal, wiLocal)) {
+            if (m.m_p.transmission < 0.001f) return {};
+            if (rough < 0.001f) return {};
+            if (metal > 0.001f) return {};
+            float eta = ctx.frontFace ? m.m_p.specular_IOR : (1.f / m.m_p.specular_IOR);
+            float alpha2 = rough * rough;
+            alpha2 *= alpha2;
+            Vec3f wh = safeNormalize(woLocal + wiLocal * eta);
+            if (wh.z < 0.f) wh = -wh;
+            float cosIH = std::max(0.f, dot(woLocal, wh));
+            float cosTH = std::abs(dot(wiLocal, wh));
+            float Fr = fresnelDielectric(cosIH, eta);
+            float D = D_GGX(wh.z, alpha2);
+            float G2 = G2_Smith_Separable(woLocal.z, std::abs(wiLocal.z), alpha2);
+            float denom = cosIH + eta * cosTH;
+            if (denom < 1e-6f) return {};
+            float bsdf_scalar = (1.f - Fr) * D * G2 * cosIH * cosTH / (woLocal.z * std::abs(wiLocal.z) * denom * denom);
+            float jacobian = eta * eta * cosTH / (denom * denom);
+            BSDFEval e;
+            e.f = base_color * bsdf_scalar;
+            e.pdf = (1.f - Fr) * D * wh.z * jacobian;
+            e.pdfRev = e.pdf;
+            return e;
+        }
         float alpha = std::max(1e-4f, rough * rough);
         float alpha2 = alpha * alpha;
+        if (m.m_p.transmission > 0.001f && metal < 0.001f) {
+            if (rough < 0.001f) return {};
+            float eta = ctx.frontFace ? m.m_p.specular_IOR : (1.f / m.m_p.specular_IOR);
+            Vec3f wh = safeNormalize(woLocal + wiLocal);
+            if (wh.z < 0.f) return {};
+            float cosIH = std::max(0.f, dot(woLocal, wh));
+            float Fr = fresnelDielectric(cosIH, eta);
+            float D = D_GGX(wh.z, alpha2);
+            float G2 = G2_Smith_Separable(woLocal.z, wiLocal.z, alpha2);
+            float cosR = wiLocal.z;
+            float pdf = Fr > 0.f ? Fr * pdfGGX_reflection(wh.z, alpha2, cosIH) : 0.f;
+            BSDFEval e;
+            e.f = base_color * (Fr * D * G2 / (4.f * woLocal.z * cosR));
+            e.pdf = pdf;
+            e.pdfRev = pdf;
+            return e;
+        }
         float wCoat, wMetal, wSpec, wDiff;
         if (!weights(woLocal.z, metal, spec, wCoat, wMetal, wSpec, wDiff)) return {};
         float pdfFwd, pdfRev;
@@ -515,8 +576,8 @@ struct SSHolder {
         return e;
     }
 
-    // sample, opaque branch with constant parameters and no normal map (StandardSurface.h:231-241,
-    // 351-405)
+    // sample with constant parameters and no normal map: the glass branch (StandardSurface.h:
+    // 248-347) on the reference's free helpers, else the opaque branch (:231-241, 351-405)
     BSDFSample sample(const ShadingContext& ctx, Vec3f wo, Vec2f u, float uComponent) const {
         const StandardSurfaceMaterial& m = s.m;
         const ShadingContext& sctx = ctx;
@@ -524,6 +585,70 @@ struct SSHolder {
         if (woLocal.z <= 0.f) return {};
         Spectrum base_color = m.m_p.base_color.value;
         float metal = m.m_p.metalness.value, spec = m.m_p.specular.value, rough = m.m_p.roughness.value;
+        if (m.m_p.transmission > 0.001f && metal < 0.001f) {
+            float eta = ctx.frontFace ? m.m_p.specular_IOR : (1.f / m.m_p.specular_IOR);
+            float cosI = woLocal.z;
+            if (rough < 0.001f) {
+                float Fr = fresnelDielectric(cosI, eta);
+                BSDFSample r;
+                if (uComponent < Fr) {
+                    r.wi = sctx.toWorld(Vec3f{-woLocal.x, -woLocal.y, woLocal.z});
+                    r.f = base_color * Fr;
+                    r.pdf = Fr;
+                    r.pdfRev = Fr;
+                    r.flags = BSDFFlag_Specular | BSDFFlag_Reflection;
+                } else {
+                    float sin2T = std::max(0.f, 1.f - cosI * cosI) / (eta * eta);
+                    float cosT = std::sqrt(std::max(0.f, 1.f - sin2T));
+                    r.wi = sctx.toWorld(Vec3f{-woLocal.x / eta, -woLocal.y / eta, -cosT});
+                    r.f = base_color * (1.f - Fr);
+                    r.pdf = 1.f - Fr;
+                    r.pdfRev = 1.f - Fr;
+                    r.eta = eta;
+                    r.flags = BSDFFlag_Specular | BSDFFlag_Transmission;
+                }
+                return r;
+            }
+            float alpha = std::max(1e-4f, rough * rough), alpha2 = alpha * alpha;
+            Vec3f wh = sampleGGX_halfvector(u, alpha2);
+            if (wh.z <= 0.f) return {};
+            float cosIH = std::max(0.f, dot(woLocal, wh));
+            if (cosIH <= 0.f) return {};
+            float Fr = fresnelDielectric(cosIH, eta);
+            BSDFSample r;
+            if (uComponent < Fr) {
+                Vec3f wiLocal = wh * (2.f * cosIH) - woLocal;
+                if (wiLocal.z <= 0.f) return {};
+                float D = D_GGX(wh.z, alpha2), G2 = G2_Smith_Separable(woLocal.z, wiLocal.z, alpha2);
+                Spectrum f0{Fr, Fr, Fr};
+                Spectrum bsdf = f0 * (D * G2 / (4.f * woLocal.z));
+                r.wi = sctx.toWorld(wiLocal);
+                r.f = bsdf * base_color;
+                r.pdf = Fr * pdfGGX_reflection(wh.z, alpha2, cosIH);
+                r.pdfRev = r.pdf;
+                r.flags = BSDFFlag_Glossy | BSDFFlag_Reflection;
+                return r;
+            }
+            float sin2T = std::max(0.f, 1.f - cosIH * cosIH) / (eta * eta);
+            if (sin2T >= 1.f) return {};
+            float cosT_H = std::sqrt(1.f - sin2T);
+            Vec3f wiLocal = woLocal * (-1.f / eta) + wh * (cosIH / eta - cosT_H);
+            if (wiLocal.z >= 0.f) return {};
+            float cosO = woLocal.z, cosI_t = std::abs(wiLocal.z);
+            float D = D_GGX(wh.z, alpha2), G2 = G2_Smith_Separable(cosO, cosI_t, alpha2);
+            float denom = cosIH + eta * std::abs(dot(wiLocal, wh));
+            if (denom < 1e-6f) return {};
+            float absCosT_wh = std::abs(dot(wiLocal, wh));
+            float pdf_wh = D * wh.z;
+            float jacobian = eta * eta * absCosT_wh / (denom * denom);
+            r.wi = sctx.toWorld(wiLocal);
+            r.f = base_color * ((1.f - Fr) * D * G2 * cosIH * absCosT_wh / (cosO * denom * denom));
+            r.pdf = (1.f - Fr) * pdf_wh * jacobian;
+            r.pdfRev = r.pdf;
+            r.eta = eta;
+            r.flags = BSDFFlag_Glossy | BSDFFlag_Transmission;
+            return r;
+        }
         float alpha = std::max(1e-4f, rough * rough);
         float alpha2 = alpha * alpha;
         float wCoat, wMetal, wSpec, wDiff;
@@ -552,11 +677,73 @@ struct SSHolder {
 };
 }  // namespace
 
+// A test material behind the reference's own IMaterial interface for the integrator tests.
+// StandardSurfaceMaterial itself cannot be built here (its virtual methods call evalTOV, defined in
+// the OpenImageIO-only Texture.cpp), so its methods for constant parameters are restated on SSHolder
+// (isDelta :162-169, flags :173-181, Le :183-187, reflectance :189-195, evalOpacity :197-201,
+// transmittanceColor :207-225, sample / evaluate as above).  With it the reference's unmodified
+// PathIntegrator — Li's alpha test (PathIntegrator.cpp:104-118), estimateDirect's step loop
+// (:217-236) and shadowTransmittance (ShadowRay.h:26-61) — runs on glass and alpha-masked scenes.
+class HarnessStandardSurface : public IMaterial {
+public:
+    explicit HarnessStandardSurface(const ahip_material& a) : h_(a) {}
+    bool isDelta() const override {
+        const auto& p = h_.s.m.m_p;
+        float r = p.roughness.value, m = p.metalness.value, s = p.specular.value;
+        if (r < 0.001f && p.transmission > 0.001f && m < 0.001f) return true;
+        return r < 0.001f && (m > 0.999f || s > 0.f);
+    }
+    float roughness() const override { return h_.s.m.m_p.roughness.value; }
+    float metalness() const override { return h_.s.m.m_p.metalness.value; }
+    uint32_t flags() const override {
+        const auto& p = h_.s.m.m_p;
+        uint32_t f = BSDFFlag_Reflection;
+        if (p.transmission > 0.001f) f |= BSDFFlag_Transmission;
+        if (p.roughness.value < 0.001f) f |= BSDFFlag_Specular;
+        else if (p.roughness.value < 0.3f) f |= BSDFFlag_Glossy;
+        else f |= BSDFFlag_Diffuse;
+        return f;
+    }
+    Spectrum Le(const ShadingContext&, Vec3f) const override {
+        const auto& p = h_.s.m.m_p;
+        if (p.emission > 0.f) return p.emission_color.value * p.emission;
+        return {};
+    }
+    Spectrum reflectance(const ShadingContext&) const override {
+        const auto& p = h_.s.m.m_p;
+        float metal = p.metalness.value, spec = p.specular.value;
+        float diff = (1.f - metal) * (1.f - spec * 0.5f);
+        return p.base_color.value * p.base * (diff + metal);
+    }
+    float evalOpacity(const ShadingContext&) const override { return h_.opacity(); }
+    Spectrum transmittanceColor(const ShadingContext&) const override { return h_.transmittance(); }
+    BSDFSample sample(const ShadingContext& ctx, Vec3f wo, Vec2f u, float uc) const override {
+        BSDFSample r = h_.sample(ctx, wo, u, uc);
+        if (r.isValid() && !(h_.s.m.m_p.transmission > 0.001f && h_.s.m.m_p.metalness.value < 0.001f))
+            r.flags = flags();                                           // result.flags = flags() (:403)
+        return r;
+    }
+    BSDFEval evaluate(const ShadingContext& ctx, Vec3f wo, Vec3f wi) const override {
+        return h_.evaluate(ctx, wo, wi);
+    }
+    float pdf(const ShadingContext& ctx, Vec3f wo, Vec3f wi) const override { return evaluate(ctx, wo, wi).pdf; }
+
+private:
+    SSHolder h_;
+};
+
+std::unique_ptr<IMaterial> makeHarnessStandardSurface(const ahip_material& a) {
+    if (a.roughness_texture >= 0 || (a.alpha_mask && a.opacity_texture >= 0)) return nullptr;   // constants only
+    return std::make_unique<HarnessStandardSurface>(a);
+}
+
 // The probe layout above, on a surface hit (ShadingContext from n alone: t, bt by
-// buildOrthonormalBasis, ShadingContext.h:45-47); the sigma_a slots are zero.
+// buildOrthonormalBasis, ShadingContext.h:45-47); slots 13..15 hold transmittanceColor.
 int ref_ss_probe(const ahip_material* mat, uint32_t n, const float* in, float* out) {
-    if (mat->type != AHIP_MAT_STANDARD || mat->transmission > 0.001f) return 1;
+    if (mat->type != AHIP_MAT_STANDARD || mat->roughness_texture >= 0 || (mat->alpha_mask && mat->opacity_texture >= 0))
+        return 1;
     SSHolder h(*mat);
+    const Spectrum tint = h.transmittance();
     for (uint32_t i = 0; i < n; ++i) {
         const float* q = in + 19 * i;
         SurfaceInteraction si;
@@ -570,7 +757,7 @@ int ref_ss_probe(const ahip_material* mat, uint32_t n, const float* in, float* o
         o[0] = e.f.x; o[1] = e.f.y; o[2] = e.f.z; o[3] = e.pdf; o[4] = e.pdfRev;
         o[5] = s.wi.x; o[6] = s.wi.y; o[7] = s.wi.z;
         o[8] = s.f.x; o[9] = s.f.y; o[10] = s.f.z; o[11] = s.pdf; o[12] = s.isValid() ? 1.f : 0.f;
-        o[13] = o[14] = o[15] = 0.f;
+        o[13] = tint.x; o[14] = tint.y; o[15] = tint.z;
     }
     return 0;
 }

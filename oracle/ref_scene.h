@@ -39,8 +39,10 @@ inline Camera toCamera(const ahip_camera* c) {
 
 // Meshes into a GeometryPool committed to a BVHBackend, Lambert / Emissive materials by mesh, quad
 // area lights and a constant background; nullptr for anything else (curves: CurveBrute.cpp needs
-// spdlog; StandardSurface and DomeLight need OpenImageIO-only sources).
-inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d) {
+// spdlog; StandardSurface and DomeLight need OpenImageIO-only sources).  `standard`, when given,
+// makes the IMaterial of a standard_surface material (nullptr: the scene is refused).
+using StandardFactory = std::unique_ptr<IMaterial> (*)(const ahip_material&);
+inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, StandardFactory standard = nullptr) {
     if (d->num_strands) return nullptr;
     auto s = std::make_unique<RefScene>();
     for (uint32_t m = 0; m < d->num_meshes; ++m) {
@@ -62,7 +64,11 @@ inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d) {
             s->mats.push_back(std::make_unique<LambertianMaterial>(v3(m.albedo)));
         else if (m.type == AHIP_MAT_EMISSIVE)
             s->mats.push_back(std::make_unique<EmissiveMaterial>(v3(m.albedo), v3(m.emission)));
-        else
+        else if (m.type == AHIP_MAT_STANDARD && standard && !d->tri_albedo) {
+            auto p = standard(m);
+            if (!p) return nullptr;
+            s->mats.push_back(std::move(p));
+        } else
             return nullptr;
     }
     for (uint32_t m = 0; m < d->num_meshes; ++m) {

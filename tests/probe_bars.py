@@ -12,18 +12,24 @@ def rel(a, b, floor=1e-30):
 _rel = rel
 
 
-def probe_close(got, ref, rows, hair, what="", sharp=False, sigma_rtol=0.0):
+def probe_close(got, ref, rows, hair, what="", sharp=False, sigma_rtol=0.0, floor_rel=1e-6, max_flips=0):
     """The probe bars of the module docstring; returns the number of rows compared.  sharp: the
     material has a GGX lobe of roughness < 0.18 (alpha^2 < 1e-3), whose sampled f and pdf are
     each held only within 0.15 (and 2e-2 on three rows in four); their ratio keeps the full bar.
     sigma_rtol: hair effectiveSigmaA is bit-exact between CPU builds (glibc pow / log); the device's
-    powf / logf differ by an ulp, so device comparisons pass 1e-5."""
+    powf / logf differ by an ulp, so device comparisons pass 1e-5.  floor_rel: values below this
+    fraction of the rows' largest are compared absolutely (glass across the surface: 1 - F near the
+    critical angle cancels in float, so its far-tail values keep few significant digits).
+    max_flips: rows allowed to differ in sample validity, left out of the sampled-value bars (glass:
+    its reflect-or-refract choice uComponent < F sits where F climbs steeply to 1 near total internal
+    reflection, and the refracted branch's TIR test sin^2 t >= 1 on its edge, so the last bits of F
+    can decide the branch)."""
     wo, n, t = rows[:, 9:12], rows[:, 0:3], rows[:, 3:6]
     ok = np.abs((wo * n).sum(1)) >= 1e-4
     if hair:
         ok &= np.abs((wo * t).sum(1)) <= 1 - 1e-6
     g, r = got[ok], ref[ok]
-    floor = 1e-6 * max(np.abs(r[:, 0:4]).max(), 1e-30)
+    floor = floor_rel * max(np.abs(r[:, 0:4]).max(), 1e-30)
     e_eval = _rel(g[:, 0:4], r[:, 0:4], floor).max(1)
     assert e_eval.max() <= 2e-3, f"{what}: evaluate differs by {e_eval.max():.3g}"
     if hair:
@@ -36,12 +42,15 @@ def probe_close(got, ref, rows, hair, what="", sharp=False, sigma_rtol=0.0):
     else:
         degenerate = np.zeros(len(g), bool)
     vg, vr = g[:, 12] > 0, r[:, 12] > 0
-    assert np.array_equal(vg[~degenerate], vr[~degenerate]), f"{what}: sample validity"
+    flips = (vg != vr) & ~degenerate
+    if flips.any():
+        print(f"[probe] {what}: {int(flips.sum())} of {len(vg)} rows differ in sample validity", flush=True)
+    assert flips.sum() <= max_flips, f"{what}: sample validity ({int(flips.sum())} rows)"
     both = vg & vr & ~degenerate
     assert np.abs(g[both, 5:8] - r[both, 5:8]).max(initial=0) <= 5e-4, f"{what}: sampled direction"
     if not both.any():
         return int(ok.sum())
-    sfloor = 1e-6 * max(np.abs(r[both, 8:12]).max(), 1e-30)
+    sfloor = floor_rel * max(np.abs(r[both, 8:12]).max(), 1e-30)
     e_s = _rel(g[both, 8:12], r[both, 8:12], sfloor).max(1)
     ratio = lambda x: x[:, 8:11] / x[:, 11:12]
     e_r = _rel(ratio(g[both]), ratio(r[both])).max(1)

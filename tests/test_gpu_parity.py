@@ -220,12 +220,13 @@ def test_injected_failure_then_recovery(gpu):
     assert gpu.download_accum()[..., 3].min() == 1.0
 
 
-def test_unsupported_material_is_rejected(gpu):
+def test_bad_material_is_rejected(gpu):
+    """Glass and alpha masks are served; a material naming a texture the scene lacks is refused."""
     from anacapa_amd.integrator import BackendError
     sc = scenes.cornell(width=16, height=16)
-    m = scenes.standard_surface((1, 1, 1))
-    m.transmission = 1.0
-    sc.materials[0] = m
+    sc.materials[0] = scenes.glass(roughness=0.2)
+    gpu.prepare(sc)
+    sc.materials[0] = scenes.alpha_masked((1, 1, 1), opacity=0.5, opacity_texture=3)
     with pytest.raises(BackendError) as e:
         gpu.prepare(sc)
-    assert e.value.status == A.AHIP_E_UNSUPPORTED
+    assert e.value.status == A.AHIP_E_INVALID_ARG
