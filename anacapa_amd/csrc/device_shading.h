@@ -524,8 +524,9 @@ __device__ __noinline__ BSample hairMarschnerSample(const DMat& m, const Frame& 
     return bs;
 }
 
-// IMaterial::sample.  HAIRM: hair evaluates the Marschner model instead of Kajiya-Kay.
-template <bool HAIRM = false>
+// IMaterial::sample.  HAIRM: hair evaluates the Marschner model instead of Kajiya-Kay.  GLASS: the
+// glass branch is compiled in (instances for scenes without glass leave it out).
+template <bool HAIRM = false, bool GLASS = false>
 __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V2 u,
                              float uc) {
     BSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, false};
@@ -559,7 +560,7 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, c
     V3 wol = toLocal(fr, wo);
     if (wol.z <= 0.f) return s;
     float r = mi.r;
-    if (m.glass) return glassSample(fr, wol, r, fr.front ? m.eta : 1.f / m.eta, mi.base, u, uc);
+    if (GLASS && m.glass) return glassSample(fr, wol, r, fr.front ? m.eta : 1.f / m.eta, mi.base, u, uc);
     float alpha = stdmax(1e-4f, r * r), a2 = alpha * alpha;
     float wc, wm, ws, wd;
     if (!layerWeights(m, wol.z, wc, wm, ws, wd)) return s;
@@ -605,13 +606,13 @@ __device__ BSample matSample(const DScene& sc, const DMat& m, const Frame& fr, c
     return res;
 }
 
-// IMaterial::evaluate: f without cosine, forward pdf.  HAIRM as for matSample.
-template <bool HAIRM = false>
+// IMaterial::evaluate: f without cosine, forward pdf.  HAIRM and GLASS as for matSample.
+template <bool HAIRM = false, bool GLASS = false>
 __device__ V3 matEval(const DScene& sc, const DMat& m, const Frame& fr, const MatIn& mi, V3 wo, V3 wi, float& pdf) {
     if (m.type == AHIP_MAT_HAIR) return HAIRM ? hairMarschner(m, fr, wo, wi, pdf) : hairKK(m, fr, wo, wi, pdf);
     V3 wil = toLocal(fr, wi), wol = toLocal(fr, wo);
     pdf = 0.f;
-    if (m.type == AHIP_MAT_STANDARD && (m.glass || m.trans)) {
+    if (GLASS && m.type == AHIP_MAT_STANDARD && (m.glass || m.trans)) {
         // transmission across the surface (m.trans), Fresnel-weighted reflection (m.glass)
         if (!(wil.z * wol.z > 0.f) || m.glass) return glassEval(m, fr.front, wol, wil, mi.r, mi.base, pdf);
     }

@@ -211,9 +211,10 @@ __global__ __launch_bounds__(BS) void k_extend(DScene sc, WaveQueues q, uint32_t
 }
 
 // --------------------------------------------------------------------------- shade
-// HAIRM: hair materials evaluate the Marschner model (AHIP_FLAG_HAIR_MARSCHNER); the instances
-// without it carry none of that code.
-template <bool CURVES, bool HAIRM>
+// HAIRM: hair materials evaluate the Marschner model (AHIP_FLAG_HAIR_MARSCHNER); TRANSP: the scene
+// has glass or alpha-masked materials (the alpha test, the first-hit AOV flag and the glass BSDF);
+// the instances without them carry none of that code.
+template <bool CURVES, bool HAIRM, bool TRANSP>
 __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, WaveQueues q, uint32_t bounce,
                                                uint32_t cur) {
     __shared__ uint32_t lds[3 * (4 + 1)];
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
             uint32_t meta = __float_as_uint(rb.w);
             dim = meta & ~kPathFlags;
             bool specular = (meta & kSpecularPath) != 0u;
-            aovDone = (meta & kAovDone) != 0u;
+            aovDone = TRANSP ? (meta & kAovDone) != 0u : bounce != 0u;
             beta = v3(bb);
             V3 dir = v3(rb);
             uint32_t px, py, sidx;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                     mi = meshID < sc.numMeshes ? sc.meshMat[meshID] : -1;
                 }
                 bool passThrough = false;
-                if (mi >= 0 && sc.hasAlpha) {
+                if (TRANSP && mi >= 0 && sc.hasAlpha) {
                     // alpha test (PathIntegrator.cpp:104-118): a cut-out point, or a partly opaque one
                     // whose opacity is below a sample (drawn only then), continues the ray past the
                     // surface unshaded, in the same bounce state
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         LSample ls = lightSample(sc, L, p, u2);
                         if (ls.pdf > 0.f && !isBlack(ls.Li)) {
                             float bpdf;
-                            V3 bf = matEval<HAIRM>(sc, m, fe, mIn, wo, ls.wi, bpdf);
+                            V3 bf = matEval<HAIRM, TRANSP>(sc, m, fe, mIn, wo, ls.wi, bpdf);
                             if (!isBlack(bf)) {
                                 // spawnRayTo(si.p, si.n, si.p + wi*dist) (Types.h:277-284)
                                 V3 target = p + ls.wi * ls.dist;
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample<HAIRM>(sc, m, fe, mIn, wo, ub, uc);
+                        BSample bs = matSample<HAIRM, TRANSP>(sc, m, fe, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             float lp = lightPdf(sc, L, p, bs.wi);
                             if (lp > 0.f) {
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                         V2 ub;
                         ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
                         ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample<HAIRM>(sc, m, fr, mIn, wo, ub, uc);
+                        BSample bs = matSample<HAIRM, TRANSP>(sc, m, fr, mIn, wo, ub, uc);
                         if (bsValid(bs)) {
                             specularNext = bs.delta;
                             beta = beta * (bs.f / bs.pdf);
@@ -412,7 +413,7 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
         if (cont) {
             q.ro[nxt][sc0] = f4(rayO, __uint_as_float(p0));
             q.rd[nxt][sc0] = f4(rayD, __uint_as_float(dim | (specularNext ? kSpecularPath : 0u) |
-                                                      (aovDone ? kAovDone : 0u)));
+                                                      (TRANSP && aovDone ? kAovDone : 0u)));
             q.beta[nxt][sc0] = f4(beta, 0.f);
         }
         if (wantShadow) {
@@ -1262,8 +1263,8 @@ __global__ __launch_bounds__(256) void k_material_probe(DMat m, uint32_t n, cons
     DScene sc{};
     const MatIn mi = matInputs(sc, m, V2{0.f, 0.f}, ~0u);
     float pdf;
-    V3 f = matEval<HAIRM>(sc, m, fr, mi, wo, wi, pdf);
-    BSample bs = matSample<HAIRM>(sc, m, fr, mi, wo, u, uc);
+    V3 f = matEval<HAIRM, true>(sc, m, fr, mi, wo, wi, pdf);
+    BSample bs = matSample<HAIRM, true>(sc, m, fr, mi, wo, u, uc);
     DScene none{};   // the probe's materials read no texture
     V3 sa = m.type == AHIP_MAT_HAIR ? hairSigmaA(m, fr.color) : matTransmittance(none, m, mi.base, V2{0.f, 0.f});
     float* o = out + 16u * (size_t)i;
@@ -1464,12 +1465,21 @@ void launchExtend(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32
 void launchShade(const DScene& sc, const RenderConsts& rc, const WaveQueues& q, uint32_t bounce, uint32_t cur,
                  uint32_t blocks, hipStream_t s) {
     const bool hairm = sc.hasHair && (rc.flags & AHIP_FLAG_HAIR_MARSCHNER) != 0u;
-    if (sc.hasCurves) {
-        if (hairm) hipLaunchKernelGGL((k_shade<true, true>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
-        else hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+    const dim3 g(blocks), b(256);
+    if (sc.hasTransparent) {
+        if (sc.hasCurves) {
+            if (hairm) hipLaunchKernelGGL((k_shade<true, true, true>), g, b, 0, s, sc, rc, q, bounce, cur);
+            else hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, s, sc, rc, q, bounce, cur);
+        } else {
+            if (hairm) hipLaunchKernelGGL((k_shade<false, true, true>), g, b, 0, s, sc, rc, q, bounce, cur);
+            else hipLaunchKernelGGL((k_shade<false, false, true>), g, b, 0, s, sc, rc, q, bounce, cur);
+        }
+    } else if (sc.hasCurves) {
+        if (hairm) hipLaunchKernelGGL((k_shade<true, true, false>), g, b, 0, s, sc, rc, q, bounce, cur);
+        else hipLaunchKernelGGL((k_shade<true, false, false>), g, b, 0, s, sc, rc, q, bounce, cur);
     } else {
-        if (hairm) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
-        else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(256), 0, s, sc, rc, q, bounce, cur);
+        if (hairm) hipLaunchKernelGGL((k_shade<false, true, false>), g, b, 0, s, sc, rc, q, bounce, cur);
+        else hipLaunchKernelGGL((k_shade<false, false, false>), g, b, 0, s, sc, rc, q, bounce, cur);
     }
 }
 void launchShadow(const DScene& sc, const WaveQueues& q, uint32_t bounce, uint32_t blocks, hipStream_t s) {

@@ -143,8 +143,11 @@ int ref_generate_rays(const ahip_camera* c, uint32_t n, const uint32_t* px, cons
 }
 
 std::unique_ptr<IMaterial> makeHarnessStandardSurface(const ahip_material& a);
+std::unique_ptr<ILight> makeHarnessDome(const ahip_scene_desc* d, const ahip_light& l);
 
-void* ref_scene_create(const ahip_scene_desc* d) { return buildRefScene(d, makeHarnessStandardSurface).release(); }
+void* ref_scene_create(const ahip_scene_desc* d) {
+    return buildRefScene(d, makeHarnessStandardSurface, makeHarnessDome).release();
+}
 
 void ref_scene_destroy(void* h) { delete static_cast<RefScene*>(h); }
 
@@ -270,21 +273,35 @@ int ref_render_tile_aovs(void* h, const ahip_camera* cam, uint32_t x0, uint32_t 
 // concurrency), HaltonSampler(spp, seed) cloned per tile, PathIntegrator(maxDepth) ->
 // renderTile -> TileBuffer, merged here into out[W*H*5].  *seconds = steady_clock around
 // the parallel tile loop only (RenderSession.cpp:479, 665-667).
+int ref_render_rows(void* h, const ahip_camera* cam, uint32_t spp, uint32_t maxDepth, uint64_t seed,
+                    uint32_t row0, uint32_t row1, uint32_t nthreads, uint32_t tileSize, float* out, double* seconds,
+                    uint32_t* threadsUsed);
+
 int ref_render_frame(void* h, const ahip_camera* cam, uint32_t spp, uint32_t maxDepth, uint64_t seed,
                      uint32_t nthreads, uint32_t tileSize, float* out, double* seconds,
                      uint32_t* threadsUsed) {
+    return ref_render_rows(h, cam, spp, maxDepth, seed, 0, cam->height, nthreads, tileSize, out, seconds,
+                           threadsUsed);
+}
+
+// The same tile loop over the rows [row0, row1) of the frame only (a band of the frame for a bounded
+// CPU timing sample); out still holds the whole frame.
+int ref_render_rows(void* h, const ahip_camera* cam, uint32_t spp, uint32_t maxDepth, uint64_t seed,
+                    uint32_t row0, uint32_t row1, uint32_t nthreads, uint32_t tileSize, float* out, double* seconds,
+                    uint32_t* threadsUsed) {
     auto* s = static_cast<RefScene*>(h);
     s->view.camera = toCamera(cam);
     const uint32_t W = cam->width, H = cam->height;
+    row1 = std::min(row1, H);
     PathIntegrator integ(maxDepth);
     integ.prepare(s->view);
     HaltonSampler base(spp, seed);
     ThreadPool pool(nthreads);
     if (threadsUsed) *threadsUsed = pool.numThreads();
     std::vector<TileRequest> tiles;
-    for (uint32_t y = 0; y < H; y += tileSize)
+    for (uint32_t y = row0; y < row1; y += tileSize)
         for (uint32_t x = 0; x < W; x += tileSize)
-            tiles.push_back({x, y, std::min(tileSize, W - x), std::min(tileSize, H - y), 0, spp});
+            tiles.push_back({x, y, std::min(tileSize, W - x), std::min(tileSize, row1 - y), 0, spp});
     std::memset(out, 0, sizeof(float) * W * H * 5);
     std::mutex mergeMutex;
     auto t0 = std::chrono::steady_clock::now();
@@ -818,6 +835,31 @@ struct DomeHolder {
     DomeHolder& operator=(const DomeHolder&) = delete;
 };
 }  // namespace
+
+// The dome as the renderer's environment light for the integrator runs: an ILight that forwards to
+// the reference DomeLight's own methods (all header code, DomeLight.h:117-183), called on the
+// holder's object whose constructor (with its OpenImageIO loader) is not run.
+namespace {
+class HarnessDome : public ILight {
+public:
+    HarnessDome(uint32_t w, uint32_t h, const float* rgb, float intensity, const float* rot)
+        : h_(w, h, rgb, intensity, rot) {}
+    LightSample sample(Vec3f p, Vec3f n, Vec2f u) const override { return h_.s.d.DomeLight::sample(p, n, u); }
+    float pdf(Vec3f p, Vec3f wi) const override { return h_.s.d.DomeLight::pdf(p, wi); }
+    LightLeSample sampleLe(Vec2f a, Vec2f b) const override { return h_.s.d.DomeLight::sampleLe(a, b); }
+    Spectrum Le(Vec3f pos, Vec3f n, Vec3f wo) const override { return h_.s.d.DomeLight::Le(pos, n, wo); }
+    float power() const override { return h_.s.d.DomeLight::power(); }
+    bool isInfinite() const override { return true; }
+    bool isDelta() const override { return false; }
+
+private:
+    DomeHolder h_;
+};
+}  // namespace
+
+std::unique_ptr<ILight> makeHarnessDome(const ahip_scene_desc* d, const ahip_light& l) {
+    return std::make_unique<HarnessDome>(d->env_width, d->env_height, d->env_pixels, l.intensity, l.rot);
+}
 
 // A dome over an RGB f32 equirect image (row 0 = zenith) and a world->envmap rotation (3 rows).
 void* ref_dome_create(uint32_t w, uint32_t h, const float* rgb, float intensity, const float* rot) {

@@ -41,8 +41,11 @@ inline Camera toCamera(const ahip_camera* c) {
 // area lights and a constant background; nullptr for anything else (curves: CurveBrute.cpp needs
 // spdlog; StandardSurface and DomeLight need OpenImageIO-only sources).  `standard`, when given,
 // makes the IMaterial of a standard_surface material (nullptr: the scene is refused).
+// `dome`, when given, makes the ILight of the environment dome from the descriptor's pixels.
 using StandardFactory = std::unique_ptr<IMaterial> (*)(const ahip_material&);
-inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, StandardFactory standard = nullptr) {
+using DomeFactory = std::unique_ptr<ILight> (*)(const ahip_scene_desc*, const ahip_light&);
+inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, StandardFactory standard = nullptr,
+                                               DomeFactory dome = nullptr) {
     if (d->num_strands) return nullptr;
     auto s = std::make_unique<RefScene>();
     for (uint32_t m = 0; m < d->num_meshes; ++m) {
@@ -77,11 +80,17 @@ inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, Standar
     }
     for (uint32_t i = 0; i < d->num_lights; ++i) {
         const ahip_light& l = d->lights[i];
+        if (l.type == AHIP_LIGHT_DOME && dome && d->env_light == (int32_t)i && d->env_pixels) {
+            s->lights.push_back(dome(d, l));
+            s->view.lights.push_back(s->lights.back().get());
+            s->view.envLight = s->lights.back().get();   // the dome is also the environment
+            continue;
+        }
         if (l.type != AHIP_LIGHT_AREA) return nullptr;
         s->lights.push_back(std::make_unique<AreaLight>(v3(l.center), v3(l.u_axis), v3(l.v_axis), v3(l.Le)));
         s->view.lights.push_back(s->lights.back().get());
     }
-    if (d->env_light >= 0) return nullptr;
+    if (d->env_light >= 0 && !s->view.envLight) return nullptr;
     s->view.envRadiance = v3(d->env_radiance);
     s->accel = std::make_unique<BVHBackend>(s->pool);
     s->accel->commit();

@@ -396,6 +396,8 @@ class Reference(_ShadingProbes):
                                              C.c_uint32, C.c_uint32, VP, VP, VP])
         L.ref_render_frame.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
                                        C.c_uint32, C.c_uint32, VP, VP, VP]
+        L.ref_render_rows.argtypes = [VP, C.POINTER(A.Camera), C.c_uint32, C.c_uint32, C.c_uint64,
+                                      C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, VP]
         L.ref_distribution1d.argtypes = [VP, C.c_uint32, C.c_uint32, VP, VP, VP, VP, VP, VP]
         L.ref_ggx_helper.argtypes = [C.c_int, C.c_uint32, VP, VP, VP, VP]
         L.ref_lambert.argtypes = [VP, C.c_uint32, VP, VP]
@@ -563,13 +565,16 @@ class RefScene:
                                            self.sc.seed, clamp, threads, tile, P(out), P(sec), P(used))
         return out, float(sec[0]), int(used[0])
 
-    def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None):
+    def render_frame(self, spp, threads=0, tile=64, max_depth=None, cam=None, row0=0, row1=None):
+        """RenderSession's non-adaptive tile loop (64x64 tiles on the reference ThreadPool) over the
+        rows [row0, row1) (the whole frame by default): (accum, seconds of the tile loop, threads)."""
         cam = cam or self.cam
         out = np.zeros((cam.height, cam.width, 5), np.float32)
         sec = np.zeros(1, np.float64)
         used = np.zeros(1, np.uint32)
-        self.ref.lib.ref_render_frame(self.h, C.byref(cam), spp, self.sc.max_depth if max_depth is None else max_depth,
-                                      self.sc.seed, threads, tile, P(out), P(sec), P(used))
+        self.ref.lib.ref_render_rows(self.h, C.byref(cam), spp, self.sc.max_depth if max_depth is None else max_depth,
+                                     self.sc.seed, row0, cam.height if row1 is None else row1, threads, tile,
+                                     P(out), P(sec), P(used))
         return out, float(sec[0]), int(used[0])
 
 
