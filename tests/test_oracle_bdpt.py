@@ -57,9 +57,36 @@ def test_oracle_bdpt_clamp_caps_strategies(oracle):
     assert lum(c).mean() < lum(a).mean()
 
 
+def test_oracle_bdpt_glossy_vs_reference(oracle, reference):
+    """StandardSurface in BDPT against the reference's own BDPTIntegrator::renderTile: the Cornell box
+    with rough gold metal, a coated dielectric block and a smooth mirror panel, two lights.  The
+    reference's StandardSurfaceMaterial is not buildable (its virtual methods call the OpenImageIO-only
+    evalTOV), so its side runs the test material of oracle/ref_driver.cpp, which restates the class's
+    methods for constant parameters on the reference's own helpers; BDPTIntegrator and MISWeight are the
+    reference's code.  Per sample: every sample that neither flag marks (connection visibility on the
+    last bits of a hit distance; a subpath through the smooth mirror) within 2e-4 (measured: 53 of
+    16384 beyond, all flagged; the largest unflagged 1.4e-4), sample counts and AOVs equal."""
+    sc = scenes.cornell(width=64, height=64, second_light=True, glossy=True)
+    os_, rs = oracle.scene(sc), reference.scene(sc)
+    a, aa, an, bd = os_.render_tile_bdpt(0, 0, 64, 64, 0, NS, 10.0, border=True)
+    b, ba, bn = rs.render_tile_bdpt(0, 0, 64, 64, 0, NS, 10.0)
+    assert np.array_equal(a[..., 3], b[..., 3])
+    np.testing.assert_array_equal(aa, ba)
+    np.testing.assert_allclose(an, bn, rtol=0, atol=1e-6)
+    assert abs(a[..., :3].astype(np.float64).mean() / b[..., :3].astype(np.float64).mean() - 1) < 1e-3
+    g = np.stack([os_.render_tile_bdpt(0, 0, 64, 64, s, 1)[0][..., :3] for s in range(NS)]).astype(np.float64)
+    r = np.stack([rs.render_tile_bdpt(0, 0, 64, 64, s, 1)[0][..., :3] for s in range(NS)]).astype(np.float64)
+    flagged = np.moveaxis(bd, -1, 0) > 0
+    e = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(np.abs(r).mean(), 1e-6))
+    off = e > 2e-4
+    print(f"[bdpt] glossy vs reference: {int(off.sum())} of {e.size} samples beyond 2e-4, "
+          f"{int((off & flagged).sum())} of them flagged; largest unflagged {e[~flagged].max():.3g}", flush=True)
+    assert not (off & ~flagged).any()
+    assert off.sum() <= 5e-3 * e.size
+
+
 def test_oracle_bdpt_smooth_surface_flag():
-    """StandardSurface in BDPT (no reference build: StandardSurface.h needs OpenImageIO-only sources).
-    The Cornell box with rough gold metal and coated dielectric blocks and a smooth mirror panel,
+    """StandardSurface in BDPT: the build dependence the smooth-surface flag marks.  The Cornell box with rough gold metal and coated dielectric blocks and a smooth mirror panel,
     rendered by the oracle built twice: uncontracted (liboracle.so) and FMA-contracted at the
     reference's Release flags (liboracle_release.so).  On every sample that neither flag marks, the
     two builds agree within 2e-4; the smooth-surface flag (bit 1: a subpath sampled the mirror, whose
