@@ -29,6 +29,7 @@ AHIP_MAT_STANDARD = 2
 AHIP_MAT_HAIR = 3
 AHIP_LIGHT_AREA = 0
 AHIP_LIGHT_DOME = 1
+AHIP_LIGHT_DIRECTIONAL = 2
 AHIP_FLAG_FIX_DOUBLE_COSINE = 1
 AHIP_FLAG_HAIR_MARSCHNER = 2
 AHIP_INTEGRATOR_PATH = 0
@@ -55,7 +56,7 @@ class Material(C.Structure):
 class Light(C.Structure):
     _fields_ = [
         ("type", C.c_uint32), ("center", F3), ("u_axis", F3), ("v_axis", F3), ("Le", F3),
-        ("intensity", C.c_float), ("rot", C.c_float * 9),
+        ("intensity", C.c_float), ("rot", C.c_float * 9), ("direction", F3), ("half_angle", C.c_float),
     ]
 
 

@@ -9,6 +9,8 @@
 #include <anacapa/shading/IMaterial.h>
 #include <anacapa/shading/ShadingContext.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -16,6 +18,7 @@
 #include "shading/Lambertian.h"
 #include "shading/StandardSurface.h"
 #include "shading/lights/AreaLight.h"
+#include "shading/lights/DirectionalLight.h"
 #include "shading/lights/DomeLight.h"
 
 namespace anacapa {
@@ -156,8 +159,20 @@ void HipPathIntegrator::prepare(const SceneView& scene) {
             d->getRotation(r0, r1, r2);
             put3(al.rot, r0); put3(al.rot + 3, r1); put3(al.rot + 6, r2);
             envIndex = (int32_t)lights.size();
+        } else if (auto* dl = dynamic_cast<const DirectionalLight*>(l)) {
+            // direction and radiance from its own sample at the cone's axis (u.x = 0 gives
+            // cos theta = 1); the angular radius from the sample at the cone's rim (u.x = 1), 0 when
+            // the light is hard and returns its axis for every u (DirectionalLight.h:50-76)
+            const LightSample axis = dl->sample({0.f, 0.f, 0.f}, {0.f, 0.f, 1.f}, {0.f, 0.f});
+            const LightSample rim = dl->sample({0.f, 0.f, 0.f}, {0.f, 0.f, 1.f}, {1.f, 0.f});
+            al.type = AHIP_LIGHT_DIRECTIONAL;
+            put3(al.direction, axis.wi);
+            put3(al.Le, axis.Li);
+            const float c = std::min(1.f, std::max(-1.f, dot(axis.wi, rim.wi)));
+            al.half_angle = (rim.wi.x == axis.wi.x && rim.wi.y == axis.wi.y && rim.wi.z == axis.wi.z)
+                                ? 0.f : std::acos(c) * 180.f / 3.14159265358979323846f;
         } else {
-            return;   // directional lights stay on the CPU path
+            return;   // other lights stay on the CPU path
         }
         lights.push_back(al);
     }

@@ -726,6 +726,8 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     S.hasEmitters = hs.hasEmitters ? 1u : 0u;
     S.hasTransparent = hs.hasTransparent ? 1u : 0u;
     S.hasAlpha = hs.hasAlpha ? 1u : 0u;
+    S.hasDirectional = 0u;
+    for (const DLight& L : hs.lights) S.hasDirectional |= L.type == AHIP_LIGHT_DIRECTIONAL ? 1u : 0u;
     std::memcpy(S.envRadiance, d->env_radiance, 12);
     if (hs.hasDome) {
         const HostDome& D = hs.dome;
@@ -1464,6 +1466,8 @@ int replayArgs(ahip_context* ctx, uint32_t s0, uint32_t ns, ReplayArgs& a, uint3
     if (ctx->dsc.hasCurves) return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support triangle scenes only");
     if (ctx->dsc.hasTransparent)
         return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support opaque scenes only (no glass or alpha masks)");
+    if (ctx->dsc.hasDirectional)
+        return fail(ctx, AHIP_E_UNSUPPORTED, "replay / adjoint support area and dome lights only");
     if (ns == 0) return fail(ctx, AHIP_E_INVALID_ARG, "sample_count must be positive");
     a = ReplayArgs{};
     a.cam = toDCam(ctx->cam);

@@ -819,6 +819,27 @@ struct LSample {
 // ILight::sample
 __device__ LSample lightSample(const DScene& sc, const DLight& L, V3 p, V2 u) {
     LSample s{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f};
+    if (L.type == AHIP_LIGHT_DIRECTIONAL) {
+        // DirectionalLight::sample (DirectionalLight.h:50-76): a delta sample of pdf 1 toward the
+        // light, jittered within the cone when it has an angular radius
+        s.Li = ld3(L.Le);
+        s.pdf = 1.f;
+        s.dist = 1e10f;
+        const V3 dl = ld3(L.dir);
+        if (L.cosCone >= 1.f - 1e-6f) {
+            s.wi = dl;
+        } else {
+            const float cosT = 1.f - u.x * (1.f - L.cosCone);
+            const float sinT = sqrtf(stdmax(0.f, 1.f - cosT * cosT));
+            const float phi = 2.f * kPi * u.y;
+            float sp, cp;
+            sincosf(phi, &sp, &cp);
+            V3 t, bt;
+            onb(dl, t, bt);
+            s.wi = safeNormalize(t * (sinT * cp) + bt * (sinT * sp) + dl * cosT);
+        }
+        return s;
+    }
     if (L.type == AHIP_LIGHT_DOME) {
         float pdf;
         V3 wi = domeSampleDir(sc.dome, u, pdf);
@@ -843,8 +864,9 @@ __device__ LSample lightSample(const DScene& sc, const DLight& L, V3 p, V2 u) {
     return s;
 }
 
-// ILight::pdf
+// ILight::pdf (0 for the delta directional light, DirectionalLight.h:78-80)
 __device__ float lightPdf(const DScene& sc, const DLight& L, V3 p, V3 wi) {
+    if (L.type == AHIP_LIGHT_DIRECTIONAL) return 0.f;
     if (L.type == AHIP_LIGHT_DOME) return domePdf(sc.dome, wi);
     V3 nn = -ld3(L.normal);
     float den = dot(wi, nn);

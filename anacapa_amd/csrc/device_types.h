@@ -127,6 +127,8 @@ struct DLight {
     uint32_t type;            // AHIP_LIGHT_*
     float center[3], uA[3], vA[3], Le[3], normal[3];
     float area, uLen, vLen;
+    float dir[3];             // directional: unit dirToLight
+    float cosCone;            // directional: cos of the angular radius (1 = hard)
 };
 
 struct DTexture {
@@ -198,6 +200,7 @@ struct DScene {
     uint32_t hasTransparent;  // any glass or alpha-masked material: shadow and BSDF-sampled NEE rays step
                               // through transmissive surfaces (ShadowRay.h:26-61, PathIntegrator.cpp:217-236)
     uint32_t hasAlpha;        // any alpha-masked material: the bounce loop's pass-through test runs
+    uint32_t hasDirectional;  // any directional (delta) light
 };
 
 }  // namespace ahip

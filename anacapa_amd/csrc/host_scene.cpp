@@ -278,6 +278,21 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
             L.area = lenv(c) * 4.f;
             L.uLen = lenv(ld(l.u_axis));
             L.vLen = lenv(ld(l.v_axis));
+        } else if (l.type == AHIP_LIGHT_DIRECTIONAL) {
+            // DirectionalLight(dirToLight, Le, ..., halfAngleDeg) (DirectionalLight.h:33-45)
+            const V dl = ld(l.direction);
+            const float len = lenv(dl);
+            if (!(len > 1e-8f)) {
+                L.dir[0] = 0.f; L.dir[1] = 0.f; L.dir[2] = 1.f;   // safeNormalize's fallback
+            } else {
+                st(L.dir, V{dl.x * (1.f / len), dl.y * (1.f / len), dl.z * (1.f / len)});
+            }
+            std::memcpy(L.Le, l.Le, 12);
+            if (!(l.half_angle >= 0.f) || !(l.half_angle <= 90.f)) {
+                err = "directional light half_angle must be in [0, 90] degrees";
+                return AHIP_E_INVALID_ARG;
+            }
+            L.cosCone = std::cos(l.half_angle * 3.14159265358979323846f / 180.f);
         } else if (l.type == AHIP_LIGHT_DOME) {
             if ((int32_t)i != d->env_light) {
                 err = "a dome light must be the scene's env_light";

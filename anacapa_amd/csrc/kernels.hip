@@ -342,8 +342,9 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                                 sd = sd * (1.f / dist);
                                 V3 off = n * 1e-4f;
                                 if (dot(sd, n) < 0.f) off = -off;
+                                // a delta light's sample has weight 1 (PathIntegrator.cpp:185-187)
                                 float f = ls.pdf, g = bpdf;
-                                float wgt = (f * f) / (f * f + g * g);
+                                float wgt = L.type == AHIP_LIGHT_DIRECTIONAL ? 1.f : (f * f) / (f * f + g * g);
                                 // hair: sin of the angle to the fiber (PathIntegrator.cpp:189-193)
                                 float dt = dot(ls.wi, fe.t);
                                 float cosI = isCurve ? sqrtf(fmaxf(0.f, 1.f - dt * dt)) : fabsf(dot(ls.wi, n));
@@ -355,25 +356,28 @@ __global__ __launch_bounds__(256, 4) void k_shade(DScene sc, RenderConsts pd, Wa
                                 wantShadow = true;
                             }
                         }
-                        // BSDF-sampling strategy (PathIntegrator.cpp:200-262); g++ evaluates
-                        // sample(ctx, wo, get2D(), get1D()) right to left: uComponent first.
-                        float uc = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        V2 ub;
-                        ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
-                        BSample bs = matSample<HAIRM, TRANSP>(sc, m, fe, mIn, wo, ub, uc);
-                        if (bsValid(bs)) {
-                            float lp = lightPdf(sc, L, p, bs.wi);
-                            if (lp > 0.f) {
-                                float f = bs.pdf, g = lp;
-                                float wgt = bs.delta ? 1.f : (f * f) / (f * f + g * g);
-                                V3 off = ng * 1e-4f;
-                                if (dot(bs.wi, ng) < 0.f) off = -off;
-                                float cosI = (pd.flags & AHIP_FLAG_FIX_DOUBLE_COSINE) ? 1.f : fabsf(dot(bs.wi, n));
-                                miO = p + off;
-                                miD = bs.wi;
-                                miC = beta * (bs.f * cosI * wgt / bs.pdf) * nL;
-                                wantMis = true;
+                        // BSDF-sampling strategy (PathIntegrator.cpp:200-262), none for a delta
+                        // light; g++ evaluates sample(ctx, wo, get2D(), get1D()) right to left:
+                        // uComponent first.
+                        if (L.type != AHIP_LIGHT_DIRECTIONAL) {
+                            float uc = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                            V2 ub;
+                            ub.x = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                            ub.y = haltonSample(c_halton, pd.seed, px, py, sidx, dim++);
+                            BSample bs = matSample<HAIRM, TRANSP>(sc, m, fe, mIn, wo, ub, uc);
+                            if (bsValid(bs)) {
+                                float lp = lightPdf(sc, L, p, bs.wi);
+                                if (lp > 0.f) {
+                                    float f = bs.pdf, g = lp;
+                                    float wgt = bs.delta ? 1.f : (f * f) / (f * f + g * g);
+                                    V3 off = ng * 1e-4f;
+                                    if (dot(bs.wi, ng) < 0.f) off = -off;
+                                    float cosI = (pd.flags & AHIP_FLAG_FIX_DOUBLE_COSINE) ? 1.f : fabsf(dot(bs.wi, n));
+                                    miO = p + off;
+                                    miD = bs.wi;
+                                    miC = beta * (bs.f * cosI * wgt / bs.pdf) * nL;
+                                    wantMis = true;
+                                }
                             }
                         }
                     }

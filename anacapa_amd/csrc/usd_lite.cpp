@@ -13,12 +13,13 @@
 //     Lambert(0.5) at index 0, glass by name when no surface shader (USDLoader.cpp:386-405,
 //     526-779);
 //   - RectLight / SphereLight / DiskLight as area lights, emissive meshes auto-registered as
-//     area lights over their bounding box (USDLoader.cpp:1433-1545, 1748-1826);
+//     area lights over their bounding box (USDLoader.cpp:1433-1545, 1748-1826); DistantLight as
+//     a directional light (USDLoader.cpp:1553-1575);
 //   - Camera: pinhole from focal length and horizontal aperture, selection by explicit path,
 //     RenderSettings `camera`, or first camera (USDLoader.cpp:1084-1161, 1635-1722);
 //   - upAxis "Z" rotates every point, normal and light axis to Y-up (USDLoader.cpp:58-67).
 // Composition (references, payloads, sublayers, inherits, variants), animation (time-sampled
-// transforms or geometry), texture-driven material inputs, DistantLight and DomeLight, OpenPBR
+// transforms or geometry), texture-driven material inputs, DomeLight, OpenPBR
 // networks and thin-lens cameras have no counterpart on the device path; a layer that uses any
 // of them is refused with AHIP_E_UNSUPPORTED and a message naming the prim, never loaded in part.
 #include "../../include/anacapa_hip.h"
@@ -905,9 +906,9 @@ private:
         else if (p.type == "RectLight") rectLight(p);
         else if (p.type == "SphereLight") sphereLight(p);
         else if (p.type == "DiskLight") diskLight(p);
-        else if (p.type == "DistantLight" || p.type == "DomeLight")
-            unsupported(p.type + " " + p.path + " (the device path has area lights and an HDRI dome from the "
-                                                "scene description only)");
+        else if (p.type == "DistantLight") distantLight(p);
+        else if (p.type == "DomeLight")
+            unsupported(p.type + " " + p.path + " (an HDRI dome comes from the scene description only)");
         else if (p.type == "Camera") cameras.push_back(&p);
         for (const auto& c : p.children) visit(*c);
     }
@@ -1234,6 +1235,29 @@ private:
         const V3 u = transformPoint(x, w * 0.5, 0, 0, S.zUp) - c;
         const V3 v = transformPoint(x, 0, h * 0.5, 0, S.zUp) - c;
         pushArea(c, u, v, Le);
+    }
+    // USDLoader.cpp:1553-1575: the light shines along its local -Z, so the direction toward it is the
+    // local +Z in world space; radiance colour x intensity x 2^exposure; a hard (delta) direction.
+    void distantLight(const Prim& p) {
+        const M4 x = S.localToWorld(p, S.startTime);
+        const V3 o = transformPoint(x, 0, 0, 0, S.zUp), z = transformPoint(x, 0, 0, 1, S.zUp);
+        const V3 d = z - o;
+        const float len = d.length();
+        float color[3] = {1.f, 1.f, 1.f};
+        S.getVec3(p, "inputs:color", color);
+        const float intensity = exposedIntensity(p);
+        ahip_light l;
+        std::memset(&l, 0, sizeof l);
+        l.type = AHIP_LIGHT_DIRECTIONAL;
+        const V3 n = len > 1e-8f ? d * (1.f / len) : V3{0.f, 0.f, 1.f};
+        l.direction[0] = n.x;
+        l.direction[1] = n.y;
+        l.direction[2] = n.z;
+        for (int i = 0; i < 3; ++i) l.Le[i] = color[i] * intensity;
+        l.intensity = 1.f;
+        l.rot[0] = l.rot[4] = l.rot[8] = 1.f;
+        l.half_angle = 0.f;
+        lights.push_back(l);
     }
     void sphereLight(const Prim& p) {
         const M4 x = S.localToWorld(p, S.startTime);

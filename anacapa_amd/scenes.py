@@ -234,6 +234,26 @@ def dome_light(intensity=1.0) -> A.Light:
     return l
 
 
+def directional_light(direction, Le, half_angle=0.0) -> A.Light:
+    """DirectionalLight(dirToLight, Le, ..., halfAngleDeg) (src/shading/lights/DirectionalLight.h:33-45):
+    direction points from the scene toward the light."""
+    l = A.Light()
+    l.type = A.AHIP_LIGHT_DIRECTIONAL
+    l.direction = A.F3(*direction)
+    l.Le = A.F3(*Le)
+    l.half_angle = half_angle
+    l.intensity = 1.0
+    l.rot = (C.c_float * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    return l
+
+
+def with_sun(sc: Scene, direction=(0.35, 0.8, -0.5), Le=(3.0, 2.8, 2.5), half_angle=0.0) -> Scene:
+    """The scene with a directional light added to its lights (the light pick is uniform over them,
+    PathIntegrator.cpp:134-144)."""
+    sc.lights = list(sc.lights) + [directional_light(direction, Le, half_angle)]
+    return sc
+
+
 class _MeshBuilder:
     def __init__(self):
         self.pos, self.nrm, self.uv, self.idx = [], [], [], []

@@ -43,6 +43,8 @@ extern "C" {
  * src/shading/lights/DomeLight.h:78-388) */
 #define AHIP_LIGHT_AREA 0
 #define AHIP_LIGHT_DOME 1
+#define AHIP_LIGHT_DIRECTIONAL 2   /* DirectionalLight (src/shading/lights/DirectionalLight.h:24-160): a
+                                      delta light at infinity; the path tracer only (BDPT refuses it) */
 
 /* ahip_render_params.flags */
 #define AHIP_FLAG_FIX_DOUBLE_COSINE 1u  /* opt out of the reference's extra cos in NEE's BSDF branch
@@ -92,6 +94,10 @@ typedef struct ahip_light {
     float    Le[3];             /* area: emitted radiance */
     float    intensity;         /* dome: radiance scale */
     float    rot[9];            /* dome: world->envmap rows (DomeLight.h:105-112) */
+    float    direction[3];      /* directional: dirToLight, from a surface toward the light (normalised
+                                   on prepare as the constructor does); Le: its radiance */
+    float    half_angle;        /* directional: angular radius in degrees (0 = a hard, exact direction;
+                                   DirectionalLight.h:33-45, 55-74) */
 } ahip_light;
 
 /* Flat scene.  Mirrors GeometryPool / SceneView (include/anacapa/accel/GeometryPool.h:118-197,

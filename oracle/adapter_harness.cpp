@@ -33,6 +33,7 @@
 #include "sampling/HaltonSampler.h"
 #include "shading/Lambertian.h"
 #include "shading/lights/AreaLight.h"
+#include "shading/lights/DirectionalLight.h"
 #include <anacapa/film/Film.h>
 
 #include "../anacapa_amd/adapter/HipPathIntegrator.h"

@@ -933,12 +933,31 @@ struct Light {
     V3 center, uA, vA, Le, normal;
     float area = 1.f;
     const Dome* dome = nullptr;
+    V3 dir;                    // directional: unit dirToLight
+    float cosCone = 1.f;       // directional: cos of the angular radius
+    bool isDelta() const { return type == AHIP_LIGHT_DIRECTIONAL; }                   // DirectionalLight.h:145
     void init() {                                                                     // AreaLight.h:123-131
         normal = safeNormalize(cross(uA, vA));
         area = length(cross(uA, vA)) * 4.f;
     }
     LightSample sample(V3 p, V2 u) const {
         LightSample s;
+        if (type == AHIP_LIGHT_DIRECTIONAL) {                                         // DirectionalLight.h:50-76
+            s.dist = 1e10f;
+            s.Li = Le;
+            s.pdf = 1.f;
+            if (cosCone >= 1.f - 1e-6f) {
+                s.wi = dir;
+            } else {
+                float cosT = 1.f - u.x * (1.f - cosCone);
+                float sinT = std::sqrt(fmaxs(0.f, 1.f - cosT * cosT));
+                float phi = 2.f * kPi * u.y;
+                V3 t, bt;
+                onb(dir, t, bt);
+                s.wi = safeNormalize(t * (sinT * std::cos(phi)) + bt * (sinT * std::sin(phi)) + dir * cosT);
+            }
+            return s;
+        }
         if (type == AHIP_LIGHT_DOME) {                                                // DomeLight.h:117-129
             float pdf;
             V3 wi = dome->sampleDir(u, pdf);
@@ -957,6 +976,7 @@ struct Light {
         return s;
     }
     float pdf(V3 p, V3 wi) const {
+        if (type == AHIP_LIGHT_DIRECTIONAL) return 0.f;                               // DirectionalLight.h:78-80
         if (type == AHIP_LIGHT_DOME) return dome->evalPdf(wi);
         float den = dot(wi, -normal);                                                 // AreaLight.h:159-180
         if (den < 1e-6f) return 0.f;
@@ -1173,7 +1193,7 @@ struct Scene {
                     Ray sr = spawnRayTo(si.p, si.n, si.p + ls.wi * ls.dist);
                     V3 Tr = shadowTr(sr);
                     if (!isBlack(Tr)) {
-                        float w = power(ls.pdf, be.pdf);
+                        float w = light.isDelta() ? 1.f : power(ls.pdf, be.pdf);
                         // hair: sin of the angle to the fiber (PathIntegrator.cpp:189-193)
                         float cosI = si.isCurve ? std::sqrt(fmaxs(0.f, 1.f - dot(ls.wi, ctx.t) * dot(ls.wi, ctx.t)))
                                                 : absDot(ls.wi, si.n);
@@ -1182,7 +1202,7 @@ struct Scene {
                 }
             }
         }
-        {
+        if (!light.isDelta()) {
             // mat.sample(ctx, wo, get2D(), get1D()): g++ evaluates the arguments right to left.
             float uc = smp.get1D();
             V2 u = smp.get2D();
@@ -2013,6 +2033,10 @@ void* orc_scene_create(const ahip_scene_desc* d) {
             L.vA = {l.v_axis[0], l.v_axis[1], l.v_axis[2]};
             L.Le = {l.Le[0], l.Le[1], l.Le[2]};
             L.init();
+        } else if (l.type == AHIP_LIGHT_DIRECTIONAL) {                                // DirectionalLight.h:33-45
+            L.dir = safeNormalize(V3{l.direction[0], l.direction[1], l.direction[2]});
+            L.Le = {l.Le[0], l.Le[1], l.Le[2]};
+            L.cosCone = std::cos(l.half_angle * kPi / 180.f);
         } else {
             if ((int32_t)i != d->env_light) return nullptr;
             L.dome = &s->dome;

@@ -58,6 +58,7 @@
 #include "sampling/PCGRng.h"
 #include "shading/Lambertian.h"
 #include "shading/lights/AreaLight.h"
+#include "shading/lights/DirectionalLight.h"
 #include "render/ThreadPool.h"
 #include <anacapa/film/Film.h>
 

@@ -1,8 +1,8 @@
 // ref_scene.h — ORACLE / TEST INFRASTRUCTURE ONLY: a reference SceneView built from an
 // ahip_scene_desc with the reference's own classes (GeometryPool, BVHBackend, LambertianMaterial,
-// EmissiveMaterial, AreaLight), shared by the reference drivers in oracle/.  Include after the
-// reference headers those classes come from (accel/BVHBackend.h, shading/Lambertian.h,
-// shading/lights/AreaLight.h).
+// EmissiveMaterial, AreaLight, DirectionalLight), shared by the reference drivers in oracle/.
+// Include after the reference headers those classes come from (accel/BVHBackend.h,
+// shading/Lambertian.h, shading/lights/AreaLight.h, shading/lights/DirectionalLight.h).
 #pragma once
 #include <memory>
 #include <vector>
@@ -84,6 +84,12 @@ inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, Standar
             s->lights.push_back(dome(d, l));
             s->view.lights.push_back(s->lights.back().get());
             s->view.envLight = s->lights.back().get();   // the dome is also the environment
+            continue;
+        }
+        if (l.type == AHIP_LIGHT_DIRECTIONAL) {   // the scene bounds only place BDPT's emission disk
+            s->lights.push_back(std::make_unique<DirectionalLight>(v3(l.direction), v3(l.Le), 1.f, Vec3f{},
+                                                                   l.half_angle));
+            s->view.lights.push_back(s->lights.back().get());
             continue;
         }
         if (l.type != AHIP_LIGHT_AREA) return nullptr;

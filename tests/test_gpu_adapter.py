@@ -16,12 +16,18 @@ from anacapa_amd import scenes
 pytestmark = pytest.mark.gpu
 
 
-def test_adapter_in_reference_render_loop():
+@pytest.mark.parametrize("sun", [False, True])
+def test_adapter_in_reference_render_loop(sun):
+    """sun: a DirectionalLight of 1-degree angular radius shining in through the box's open side as well,
+    which the adapter reads back through the ILight interface (direction and radiance at the cone's axis,
+    the radius from its rim sample)."""
     from adapter_harness import AdapterHarness, available
     from oracle_bindings import Oracle
     if not available():
         pytest.fail("oracle/_ref/libadapter_harness.so not built")
     sc = scenes.cornell(width=200, height=136)    # tiles of 64: ragged right and bottom edges
+    if sun:
+        sc = scenes.with_sun(sc, direction=(0.3, 0.4, -1.0), Le=(4.0, 4.0, 4.0), half_angle=1.0)
     cam = Oracle().pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height)
     h = AdapterHarness(sc, cam, sc.max_depth)
     st = h.status()
@@ -32,7 +38,7 @@ def test_adapter_in_reference_render_loop():
     assert np.array_equal(got[..., 3], ref[..., 3]) and (got[..., 3] == 8).all()
     g, r = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(r.mean(), 1e-6))
-    print(f"[adapter] cornell {sc.width}x{sc.height} x 8 spp: {(err > 1e-3).sum()} pixels beyond 1e-3 "
+    print(f"[adapter] cornell{' + sun' if sun else ''} {sc.width}x{sc.height} x 8 spp: {(err > 1e-3).sum()} pixels beyond 1e-3 "
           f"(max {err.max():.3g}); means {g.mean():.6f} / {r.mean():.6f}", flush=True)
     assert (err <= 1e-3).all()
     assert abs(g.mean() / r.mean() - 1) < 1e-4

@@ -465,7 +465,23 @@ def test_emissive_meshes_become_area_lights():
     assert list(l.u_axis) == [0, 0.5, 0] and list(l.v_axis) == [0, 0, 1.5]   # thin along x
 
 
-@pytest.mark.parametrize("kind", ["DistantLight", "DomeLight"])
+def test_distant_light_is_directional():
+    """DistantLight (USDLoader.cpp:1553-1575): the direction toward the light is the prim's local +Z in
+    world space (rotated here by 90 degrees about x: +Z turns to -Y), radiance colour x intensity x
+    2^exposure, a hard direction; with upAxis Z the direction is turned to Y-up like every light axis."""
+    body = ('def DistantLight "Sun"\n{\n    color3f inputs:color = (1, 0.5, 0.25)\n    float inputs:intensity = 3\n'
+            "    float inputs:exposure = 1\n    double3 xformOp:rotateXYZ = (90, 0, 0)\n"
+            '    uniform token[] xformOpOrder = ["xformOp:rotateXYZ"]\n}\n')
+    (l,) = _lights(body)
+    assert l.type == A.AHIP_LIGHT_DIRECTIONAL and l.half_angle == 0.0
+    assert np.allclose(list(l.Le), [6, 3, 1.5], rtol=1e-6)
+    assert np.allclose(list(l.direction), [0, -1, 0], atol=1e-6)
+    (z,) = load(layer('def DistantLight "Sun"\n{\n    float inputs:intensity = 2\n}\n', meta='upAxis = "Z"')).lights
+    assert list(z.Le) == [2, 2, 2]
+    assert np.allclose(list(z.direction), [0, 1, 0], atol=1e-6)   # local +Z is up in a Z-up layer
+
+
+@pytest.mark.parametrize("kind", ["DomeLight"])
 def test_unsupported_lights_are_refused(kind):
     with pytest.raises(UsdLoadError) as e:
         load(layer(f'def {kind} "L"\n{{\n    float inputs:intensity = 1\n}}\n'))
