@@ -763,10 +763,9 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     else if (hs.hasDome || d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
         ctx->bdptRefusal = "an environment";
     if (ctx->bdptRefusal.empty() && d->tri_albedo) ctx->bdptRefusal = "per-triangle albedo";
-    if (ctx->bdptRefusal.empty() && hs.hasTransparent) ctx->bdptRefusal = "a glass or alpha-masked material";
     for (const DMat& m : hs.mats)
         if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE &&
-            !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0))
+            !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0 && m.opacityTex < 0))
             ctx->bdptRefusal = "a material other than Lambert / Emissive / untextured StandardSurface";
     for (const DLight& L : hs.lights)
         if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA) ctx->bdptRefusal = "a light other than an area light";

@@ -26,6 +26,7 @@
 #include "device_math.h"
 #include "device_sampling.h"
 #include "device_shading.h"
+#include "device_transmit.h"
 #include "device_traverse.h"
 #include "device_types.h"
 #include "kernels.h"
@@ -38,6 +39,8 @@ namespace {
 
 constexpr uint32_t kVCamera = 0u, kVSurface = 1u, kVLight = 2u, kVDelta = 4u;   // PathVertex.h:24-26
 constexpr uint32_t kBdptBS = 128u, kBdptK = 8u;
+// term states (the w slot of a term): counted, waiting for its visibility, waiting and clamped then
+constexpr float kTermPending = 0.f, kTermCounted = 1.f, kTermClamp = 2.f;
 
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3(float4 a) { return V3{a.x, a.y, a.z}; }
@@ -63,17 +66,23 @@ __device__ __forceinline__ MatIn vertexInputs(const DMat& m) { return MatIn{ld3(
 
 // BDPTIntegrator::evalVertex (BDPTIntegrator.cpp:40-63) at a surface vertex of material m, normal
 // n, outgoing wo: f(wo, wi) |cos wi| and the forward pdf of IMaterial::evaluate.  The reverse pdf is
-// read by the subpaths at wi = -wo only, where both LambertianMaterial::evaluate (Lambertian.h:53-66)
-// and the opaque StandardSurface (StandardSurface.h:410-417) reject the pair as opposite hemispheres
-// and give 0; at other pairs it is formed for Lambert only.
+// read by the subpaths at wi = -wo only, where LambertianMaterial::evaluate (Lambertian.h:53-66) and
+// the opaque StandardSurface (StandardSurface.h:410-417) reject the pair as opposite hemispheres and
+// give 0, and rough glass answers with its transmission lobe, whose reverse pdf is its forward one
+// (StandardSurface.h:444, 475); at other pairs it is formed for Lambert and glass only.
 __device__ __forceinline__ V3 evalVertex(const DScene& sc, int32_t mi, V3 n, V3 wo, V3 wi, float& fwd, float& rev) {
     fwd = rev = 0.f;
     if (mi < 0) return V3{0.f, 0.f, 0.f};
     const DMat& m = sc.mats[mi];
     const Frame fr = makeFrame(n, n, -wo);
-    const V3 f = matEval(sc, m, fr, vertexInputs(m), wo, wi, fwd);
+    const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), wo, wi, fwd);
     if (isBlack(f)) return f;
-    if (m.type != AHIP_MAT_STANDARD) rev = stdmax(0.f, fabsf(toLocal(fr, wo).z)) * kInvPi;
+    if (m.type != AHIP_MAT_STANDARD) {
+        rev = stdmax(0.f, fabsf(toLocal(fr, wo).z)) * kInvPi;
+    } else {
+        const bool across = !(toLocal(fr, wo).z * toLocal(fr, wi).z > 0.f);
+        if ((across && m.trans) || (!across && m.glass)) rev = fwd;
+    }
     return f * fabsf(dot(wi, n));
 }
 
@@ -99,11 +108,23 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         const V3 p = v0 + e1 * u + e2 * v;
         V3 n = safeNormalize(ld3(SH->sn0) * w + ld3(SH->sn1) * u + ld3(SH->sn2) * v);
         if (dot(n, ng) < 0.f) n = -n;
-        const V2 uv{SH->uv0[0] * w + SH->uv1[0] * u + SH->uv2[0] * v, SH->uv0[1] * w + SH->uv1[1] * u + SH->uv2[1] * v};
+        const V2 uv = interpUV(*SH, u, v, w);
         const int32_t mi = T->matIndex;
         if (mi < 0) break;
         const DMat& m = sc.mats[mi];
         const V3 dir = r.d, wo = -dir;
+        if (!light && sc.hasAlpha) {
+            // the camera subpath's alpha test (BDPTIntegrator.cpp:130-143): a pass-through is not a
+            // bounce; a sample is drawn only for a partial opacity
+            const float op = matOpacity(sc, m, uv);
+            if (op <= 0.f || (op < 1.f && hs(a, px, py, sidx, dim) >= op)) {
+                V3 off = ng * 1e-4f;                              // spawnRay(si.p, si.ng, dir)
+                if (dot(dir, ng) < 0.f) off = -off;
+                r = makeTRay(p + off, dir, 1e-4f, __int_as_float(0x7f800000));
+                --depth;
+                continue;
+            }
+        }
         const Frame fr = makeFrame(n, ng, dir);
         const float pdfFwdArea = toArea(pdfFwd, prevP, p, n);
         const uint32_t vi = count++;
@@ -132,7 +153,7 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         ub.x = hs(a, px, py, sidx, dim);
         ub.y = hs(a, px, py, sidx, dim);
         const MatIn mIn = matInputs(sc, m, uv, T->commitIndex);
-        const BSample bs = matSample(sc, m, fr, mIn, wo, ub, uc);
+        const BSample bs = matSample<false, true>(sc, m, fr, mIn, wo, ub, uc);
         if (!bsValid(bs)) break;
         beta = beta * (bs.f / bs.pdf);
         pdfFwd = bs.pdf;
@@ -292,7 +313,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                             float pdf;
                             const DMat& m = sc.mats[cm];
                             const Frame fr = makeFrame(cn, cn, -cwo);
-                            const V3 f = matEval(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
+                            const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
                             const V3 ln = v3(l1);
                             const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
                             if (!isBlack(f) && !isBlack(Le)) {
@@ -338,14 +359,13 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                 if (keep) {
                     c = C * misWeight(a, i, nl, nc, s, t);
                     keep = finite3(c);
-                    if (keep && a.clamp > 0.f && s >= 2u && t >= 2u) {
-                        const float l = luminance(c);
-                        if (l > a.clamp) c = c * (a.clamp / l);
-                    }
                 }
                 if (keep) {
+                    // a term without a visibility test counts at once (s = 0: never clamped); one with
+                    // a test counts once its ray is traced, which multiplies in the connection's
+                    // transmittance and then applies the firefly clamp of s, t >= 2 (kTermClamp)
                     const uint32_t idx = k * a.cap + i;
-                    a.terms[idx] = f4(c, ray ? 0.f : 1.f);
+                    a.terms[idx] = f4(c, !ray ? kTermCounted : (s >= 2u && t >= 2u) ? kTermClamp : kTermPending);
                     if (ray) {   // staged in the slot's own rows until the wave's queue space is known
                         // spawnRayTo(from, fromN, to) (Types.h:277-284)
                         V3 dir = to - from;
@@ -380,8 +400,11 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
     }
 }
 
-// The connection queue's visibility: any hit in [1e-4, tMax) (shadowTransmittance of an opaque scene,
-// ShadowRay.h:26-61); an unoccluded term gets weight 1.
+// The connection queue's visibility, shadowTransmittance (ShadowRay.h:26-61): in an opaque scene any
+// hit in [1e-4, tMax) blocks the connection; with glass or alpha masks (TRANSP) the ray steps through
+// transmissive surfaces and their tint multiplies the term.  A visible term then gets the firefly
+// clamp its strategy takes (BDPTIntegrator.cpp:547-556) and counts.
+template <bool TRANSP>
 __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_trace(DScene sc, BdptArgs a) {
     __shared__ uint64_t lstk[kBdptK * kBdptBS];
     TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
@@ -390,9 +413,23 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_trace(DScene sc, BdptArgs
     const uint32_t count = *a.connCount;
     for (uint32_t j = blockIdx.x * kBdptBS + threadIdx.x; j < count; j += gridDim.x * kBdptBS) {
         const float4 o = a.connO[j], d = a.connD[j];
-        float t, u, v;
-        if (traverse<true, false>(sc, makeTRay(v3(o), v3(d), 1e-4f, d.w), S, t, u, v, tc, a.overflow) == ~0u)
-            a.terms[__float_as_uint(o.w)].w = 1.f;
+        V3 T{1.f, 1.f, 1.f};
+        bool visible;
+        if constexpr (TRANSP) {
+            visible = shadowTransmittance<false>(sc, v3(o), v3(d), d.w, S, tc, a.overflow, T);
+        } else {
+            float t, u, v;
+            visible = traverse<true, false>(sc, makeTRay(v3(o), v3(d), 1e-4f, d.w), S, t, u, v, tc, a.overflow) == ~0u;
+        }
+        if (!visible) continue;
+        const uint32_t idx = __float_as_uint(o.w);
+        const float4 x = a.terms[idx];
+        V3 c = TRANSP ? v3(x) * T : v3(x);
+        if (x.w == kTermClamp && a.clamp > 0.f) {
+            const float l = luminance(c);
+            if (l > a.clamp) c = c * (a.clamp / l);
+        }
+        a.terms[idx] = f4(c, kTermCounted);
     }
 }
 
@@ -403,7 +440,7 @@ __global__ __launch_bounds__(256) void k_bdpt_conn_resolve(BdptArgs a) {
         V3 sum{0.f, 0.f, 0.f};
         for (uint32_t k = 0; k < n; ++k) {
             const float4 x = a.terms[k * a.cap + i];
-            if (x.w != 0.f) sum = sum + v3(x);
+            if (x.w == kTermCounted) sum = sum + v3(x);
         }
         a.sampleL[i] = f4(sum, 0.f);
     }
@@ -462,7 +499,8 @@ void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, uint32_t t
     hipLaunchKernelGGL(k_bdpt_zero, dim3(1), dim3(1), 0, s, a.connCount);
     hipLaunchKernelGGL(k_bdpt_paths, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
     hipLaunchKernelGGL(k_bdpt_conn_gen, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
-    hipLaunchKernelGGL(k_bdpt_conn_trace, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
+    if (sc.hasTransparent) hipLaunchKernelGGL(k_bdpt_conn_trace<true>, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
+    else hipLaunchKernelGGL(k_bdpt_conn_trace<false>, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
     hipLaunchKernelGGL(k_bdpt_conn_resolve, dim3((a.slots + 255u) / 256u), dim3(256), 0, s, a);
     const uint32_t npix = a.slots / a.ns;
     hipLaunchKernelGGL(k_bdpt_accum, dim3((npix + 255u) / 256u), dim3(256), 0, s, a);

@@ -168,9 +168,9 @@ typedef struct ahip_camera {
 #define AHIP_INTEGRATOR_PATH 0u   /* PathIntegrator (src/integrator/PathIntegrator.cpp) */
 #define AHIP_INTEGRATOR_BDPT 1u   /* BDPTIntegrator (src/integrator/BDPTIntegrator.cpp): scenes of
                                      triangles with Lambert / Emissive / untextured StandardSurface
-                                     materials and area lights, no environment, no per-triangle
-                                     albedo; ahip_render_tile, ahip_render_tile_aovs and
-                                     ahip_render_frame (without AOVs) */
+                                     materials (glass and constant alpha masks included) and area
+                                     lights, no environment, no per-triangle albedo; ahip_render_tile,
+                                     ahip_render_tile_aovs and ahip_render_frame (without AOVs) */
 
 typedef struct ahip_render_params {
     uint32_t max_depth;   /* PathIntegrator maxDepth, inclusive (PathIntegrator.cpp:80); BDPT: bounces

@@ -1646,6 +1646,16 @@ struct Scene {
             if (!mat) break;
             V3 wo = -ray.d;
             Ctx ctx(si, ray.d);
+            if (!light) {
+                // the camera subpath's alpha test (BDPTIntegrator.cpp:130-143): a pass-through is
+                // not a bounce; a sample is drawn only for a partial opacity
+                float op = mat->opacity(ctx.uv);
+                if (op <= 0.f || (op < 1.f && smp.get1D() >= op)) {
+                    ray = spawnRay(si.p, si.ng, ray.d);
+                    --depth;
+                    continue;
+                }
+            }
             float pdfFwdArea = toArea(pdfFwd, P.v[P.count - 1].p, si.p, si.n);
             uint32_t vi = P.count++;
             PVert& V = P.v[vi];
@@ -1676,7 +1686,7 @@ struct Scene {
             float uc = smp.get1D();
             V2 u = smp.get2D();
             BSDFSample bs = mat->sample(ctx, wo, u, uc);
-            if (mat->isDelta()) ++smooth;
+            if (mat->isDelta() && !mat->glass()) ++smooth;   // smooth glass samples the exact Fresnel
             if (!bs.valid()) break;
             beta = beta * (bs.f / bs.pdf);
             pdfFwd = bs.pdf;
@@ -1759,13 +1769,15 @@ struct Scene {
     // between tMax * (1 - 4e-7) and tMax * (1 + 4e-7), where builds that round differently can
     // disagree.
     mutable uint32_t border = 0;
-    bool connVisible(const Ray& r) const {
-        bool vis = !isBlack(shadowTr(r));
+    // shadowTransmittance of a connection (ShadowRay.h:26-61), flagging it (border) when the verdict
+    // changes between tMax (1 -/+ 4e-7)
+    V3 connTr(const Ray& r) const {
+        V3 T = shadowTr(r);
         Ray a = r, b = r;
         a.tMax = r.tMax * (1.f - 4e-7f);
         b.tMax = r.tMax * (1.f + 4e-7f);
         if (isBlack(shadowTr(a)) != isBlack(shadowTr(b))) ++border;
-        return vis;
+        return T;
     }
     // A smooth StandardSurface (roughness < 0.001: isDelta) is sampled as GGX at alpha = 1e-4,
     // alpha^2 = 1e-8, where alpha^2 - 1 rounds to -1 in float: the sampled half-vector is the normal
@@ -1795,7 +1807,8 @@ struct Scene {
             if (dist < 1e-6f) return {};
             V3 wi = toL * (1.f / dist);
             Ray sr = spawnRayTo(cp.v[ct].p, cp.v[ct].n, cp.v[ct].p + wi * dist);
-            if (!connVisible(sr)) return {};
+            const V3 Tr1 = connTr(sr);
+            if (isBlack(Tr1)) return {};
             SI si;
             si.p = cp.v[ct].p;
             si.n = si.ng = cp.v[ct].n;
@@ -1807,7 +1820,7 @@ struct Scene {
             float cosI = std::abs(dot(wi, cp.v[ct].n));
             float cosL = std::abs(dot(-wi, lp.v[0].n));
             float d2 = dist * dist;
-            return cp.v[ct].beta * be.f * cosI * Le * cosL * V3{1.f, 1.f, 1.f} / (d2 * lp.v[0].pdfFwd);
+            return cp.v[ct].beta * be.f * cosI * Le * cosL * Tr1 / (d2 * lp.v[0].pdfFwd);
         }
         if (t == 1) return {};
         uint32_t ls = s - 1, ct = t - 1;
@@ -1815,7 +1828,8 @@ struct Scene {
         if (!lp.v[ls].mat || !cp.v[ct].mat) return {};
         if (lp.v[ls].minRough < 0.08f || cp.v[ct].minRough < 0.08f) return {};
         Ray sr = spawnRayTo(lp.v[ls].p, lp.v[ls].n, cp.v[ct].p);
-        if (!connVisible(sr)) return {};
+        const V3 Tr2 = connTr(sr);
+        if (isBlack(Tr2)) return {};
         V3 d = cp.v[ct].p - lp.v[ls].p;
         float dist = length(d);
         if (dist < 1e-6f) return {};
@@ -1826,11 +1840,11 @@ struct Scene {
         V3 fC = evalVertex(cp, ct, -wi, fC1, rC1);
         if (isBlack(fC)) return {};
         float invD2 = 1.f / (dist * dist);
-        return lp.v[ls].beta * fL * invD2 * fC * cp.v[ct].beta * V3{1.f, 1.f, 1.f};
+        return lp.v[ls].beta * fL * invD2 * fC * cp.v[ct].beta * Tr2;
     }
     // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581): out[w*h*5] as renderTile above
     // border (optional, [w*h*ns]): bit 0 where a connection's visibility of the sample hangs on the
-    // last bits of its hit distance (connVisible), bit 1 where a subpath sampled a smooth
+    // last bits of its hit distance (connTr), bit 1 where a subpath sampled a smooth
     // StandardSurface (`smooth`).
     void renderTileBDPT(uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t s0, uint32_t ns, float clamp,
                         float* out, float* albedo = nullptr, float* normal = nullptr,
@@ -2155,7 +2169,7 @@ int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
 
 // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581) with firefly clamp `clamp` (0 = off):
 // out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and the per-sample flags border [w*h*ns]
-// (bit 0: Scene::connVisible's borderline visibility, bit 1: Scene::smooth) may be null.  Scenes of triangles with
+// (bit 0: Scene::connTr's borderline visibility, bit 1: Scene::smooth) may be null.  Scenes of triangles with
 // Lambert / Emissive / untextured StandardSurface materials and area lights only (returns 1 otherwise).
 int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
                          float clamp, float* out, float* albedo, float* normal, uint8_t* border) {
@@ -2166,7 +2180,7 @@ int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
     if (!s->triAlbedo.empty()) return 1;
     for (const Material& m : s->mats)
         if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE &&
-            !(m.m.type == AHIP_MAT_STANDARD && !m.roughTex))
+            !(m.m.type == AHIP_MAT_STANDARD && !m.roughTex && !m.opacityTex))
             return 1;
     s->renderTileBDPT(x0, y0, w, hh, s0, ns, clamp, out, albedo, normal, border);
     return 0;

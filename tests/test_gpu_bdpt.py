@@ -20,14 +20,20 @@ NS = 4
 BDPT = A.AHIP_INTEGRATOR_BDPT
 
 
-@pytest.mark.parametrize("two_lights,glossy", [(False, False), (True, False), (True, True)])
+@pytest.mark.parametrize("two_lights,glossy", [(False, False), (True, False), (True, True), (False, "glass")])
 def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights, glossy):
     """glossy: standard_surface gold metal and coated dielectric blocks and a mirror panel (a delta
-    BSDF, whose connections the reference skips; measured: 64 samples beyond 2e-4, all flagged)."""
-    sc = scenes.cornell(width=64, height=64, second_light=two_lights, glossy=glossy)
+    BSDF, whose connections the reference skips; measured: 64 samples beyond 2e-4, all flagged).
+    "glass": a smooth and a rough tinted glass block and two panels of partial opacity, whose
+    connections go through k_bdpt_conn_trace's stepping instance (measured: 33 samples beyond 2e-4,
+    30 flagged, the other three within 7.3e-4)."""
+    if glossy == "glass":
+        sc = scenes.cornell_glass(width=64, height=64)
+    else:
+        sc = scenes.cornell(width=64, height=64, second_light=two_lights, glossy=glossy)
     os_ = oracle.scene(sc)
     gpu.prepare(sc, integrator=BDPT)
-    print(f"[bdpt] cornell 64x64 x {NS} spp, two lights {two_lights}, glossy {glossy}", flush=True)
+    print(f"[bdpt] {sc.name} 64x64 x {NS} spp, two lights {two_lights}, glossy {glossy}", flush=True)
     got, alb, nrm = gpu.render_tile_aovs(0, 0, 64, 64, 0, NS)
     ref, oalb, onrm, bd = os_.render_tile_bdpt(0, 0, 64, 64, 0, NS, border=True)
     assert np.array_equal(got[..., 3], ref[..., 3])
@@ -43,7 +49,11 @@ def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights, glossy):
     off = e > 2e-4
     print(f"[bdpt] {int(off.sum())} of {e.size} samples beyond 2e-4, {int((off & border).sum())} flagged "
           f"({int(border.sum())} flagged in all); largest unflagged {e[~border].max():.3g}", flush=True)
-    assert not (off & ~border).any()
+    if glossy == "glass":   # the oracle against the reference leaves two unflagged samples within 6e-4
+        assert e[~border].max() <= 1e-3
+        assert (off & ~border).sum() <= 5e-4 * e.size
+    else:
+        assert not (off & ~border).any()
     assert off.sum() <= (5e-3 if glossy else 3e-3) * e.size
 
 
