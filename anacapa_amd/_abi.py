@@ -57,6 +57,7 @@ class Light(C.Structure):
     _fields_ = [
         ("type", C.c_uint32), ("center", F3), ("u_axis", F3), ("v_axis", F3), ("Le", F3),
         ("intensity", C.c_float), ("rot", C.c_float * 9), ("direction", F3), ("half_angle", C.c_float),
+        ("scene_center", F3), ("scene_radius", C.c_float),
     ]
 
 

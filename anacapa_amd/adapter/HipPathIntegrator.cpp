@@ -159,6 +159,17 @@ void HipPathIntegrator::prepare(const SceneView& scene) {
             d->getRotation(r0, r1, r2);
             put3(al.rot, r0); put3(al.rot + 3, r1); put3(al.rot + 6, r2);
             envIndex = (int32_t)lights.size();
+            // the scene sphere of sampleLe (DomeLight.h:142-168): at uPos = (0, 0) the origin is the
+            // disk's centre, centre + envDir * radius, and at (1, 0) one radius from it along t
+            for (float v : {0.5f, 0.25f, 0.75f, 0.125f}) {
+                const LightLeSample a = d->sampleLe({0.f, 0.f}, {0.5f, v});
+                const LightLeSample b = d->sampleLe({1.f, 0.f}, {0.5f, v});
+                if (!(a.pdfPos > 0.f) || !(a.pdfDir > 0.f)) continue;
+                const float radius = (b.pos - a.pos).length();
+                put3(al.scene_center, a.pos + a.dir * radius);   // dir = -envDir
+                al.scene_radius = radius;
+                break;
+            }
         } else if (auto* dl = dynamic_cast<const DirectionalLight*>(l)) {
             // direction and radiance from its own sample at the cone's axis (u.x = 0 gives
             // cos theta = 1); the angular radius from the sample at the cone's rim (u.x = 1), 0 when
@@ -171,6 +182,13 @@ void HipPathIntegrator::prepare(const SceneView& scene) {
             const float c = std::min(1.f, std::max(-1.f, dot(axis.wi, rim.wi)));
             al.half_angle = (rim.wi.x == axis.wi.x && rim.wi.y == axis.wi.y && rim.wi.z == axis.wi.z)
                                 ? 0.f : std::acos(c) * 180.f / 3.14159265358979323846f;
+            // the scene sphere of sampleLe (DirectionalLight.h:88-125): at uPos = (0, 0) the origin is
+            // the disk's centre, centre + dirToLight * 100 radii, and at (1, 0) one radius from it
+            const LightLeSample a = dl->sampleLe({0.f, 0.f}, {0.f, 0.f});
+            const LightLeSample b = dl->sampleLe({1.f, 0.f}, {0.f, 0.f});
+            const float radius = (b.pos - a.pos).length();
+            put3(al.scene_center, a.pos - axis.wi * (radius * 100.f));
+            al.scene_radius = radius;
         } else {
             return;   // other lights stay on the CPU path
         }

@@ -756,22 +756,25 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         S.dome.intensity = D.intensity;
         std::memcpy(S.dome.rot, D.rot, sizeof(D.rot));
     }
-    // BDPT takes triangles with Lambert / Emissive / untextured StandardSurface materials lit by quad
-    // area lights, no environment (the device forms no per-vertex uv, which textures would need)
+    // BDPT takes triangles with Lambert / Emissive / StandardSurface materials without textures
+    // (the device forms no per-vertex uv) lit by quad area lights, the dome and directional lights
     ctx->bdptRefusal.clear();
     if (!ctx->curves.nodes.empty()) ctx->bdptRefusal = "curves";
-    else if (hs.hasDome || d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
-        ctx->bdptRefusal = "an environment";
+    else if (d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
+        ctx->bdptRefusal = "a constant environment radiance (no light)";
     if (ctx->bdptRefusal.empty() && d->tri_albedo) ctx->bdptRefusal = "per-triangle albedo";
     for (const DMat& m : hs.mats)
         if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE &&
             !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0 && m.opacityTex < 0))
             ctx->bdptRefusal = "a material other than Lambert / Emissive / untextured StandardSurface";
     for (const DLight& L : hs.lights)
-        if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA) ctx->bdptRefusal = "a light other than an area light";
+        if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA && !(L.sceneR > 0.f))
+            ctx->bdptRefusal = "a dome or directional light without a scene_radius";
     if (ctx->bdptRefusal.empty() && !hs.lights.empty()) {
-        // LightSampler::build (LightSampler.h:28-69): Vose's alias table over the lights' power,
-        // AreaLight::power (AreaLight.h:114-117)
+        // LightSampler::build (LightSampler.h:28-69): Vose's alias table over the lights' power:
+        // AreaLight::power (AreaLight.h:114-117), DomeLight::power (DomeLight.h:176-179, its
+        // integral from :255-257), DirectionalLight::power (DirectionalLight.h:136-139, 41)
+        constexpr float kPiF = 3.14159265358979323846f, k2PiF = 6.28318530717958647692f;
         const uint32_t n = (uint32_t)hs.lights.size();
         std::vector<float> w(n), prob(n);
         std::vector<uint32_t> alias(n, 0u), small, large;
@@ -779,7 +782,17 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         for (uint32_t i = 0; i < n; ++i) {
             const DLight& L = hs.lights[i];
             const float lum = 0.2722287f * L.Le[0] + 0.6740818f * L.Le[1] + 0.0536895f * L.Le[2];
-            w[i] = std::max(0.f, lum * L.area * 3.14159265358979323846f);
+            float pw;
+            if (L.type == AHIP_LIGHT_DOME) {
+                const float envIntegral = hs.dome.margInt * k2PiF * kPiF / (float)(hs.dome.W * hs.dome.H);
+                pw = envIntegral * 4.f * kPiF * hs.dome.intensity;
+            } else if (L.type == AHIP_LIGHT_DIRECTIONAL) {
+                const float diskArea = kPiF * L.sceneR * L.sceneR;
+                pw = lum * diskArea * kPiF;
+            } else {
+                pw = lum * L.area * kPiF;
+            }
+            w[i] = std::max(0.f, pw);
             total += w[i];
         }
         if (total <= 0.f) {
@@ -846,7 +859,7 @@ int ahip_set_params(ahip_context* ctx, const ahip_render_params* p) {
         return fail(ctx, AHIP_E_INVALID_ARG, "firefly_clamp must be finite and >= 0");
     if (p->integrator == AHIP_INTEGRATOR_BDPT && ctx->prepared && !ctx->bdptRefusal.empty())
         return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive / StandardSurface materials "
-                                             "and area lights; the scene has " + ctx->bdptRefusal);
+                                             "and area / dome / directional lights; the scene has " + ctx->bdptRefusal);
     ctx->params = *p;
     ctx->dsc.curveDepth = p->curve_depth ? p->curve_depth : kMaxCurveDepth;
     return AHIP_OK;
@@ -903,7 +916,7 @@ static int bdptRender(ahip_context* ctx, uint32_t x0, uint32_t y0, uint32_t w, u
                       uint32_t outY0) {
     if (!ctx->bdptRefusal.empty())
         return fail(ctx, AHIP_E_UNSUPPORTED, "BDPT takes triangles with Lambert / Emissive / StandardSurface materials "
-                                             "and area lights; the scene has " + ctx->bdptRefusal);
+                                             "and area / dome / directional lights; the scene has " + ctx->bdptRefusal);
     if (ns == 0u) return AHIP_OK;
     int st = checkInjected(ctx);
     if (st) return st;

@@ -37,7 +37,7 @@ __constant__ HaltonTables c_haltonB;
 
 namespace {
 
-constexpr uint32_t kVCamera = 0u, kVSurface = 1u, kVLight = 2u, kVDelta = 4u;   // PathVertex.h:24-26
+constexpr uint32_t kVCamera = 0u, kVSurface = 1u, kVLight = 2u, kVDelta = 4u, kVInf = 8u;   // PathVertex.h:24-29
 constexpr uint32_t kBdptBS = 128u, kBdptK = 8u;
 // term states (the w slot of a term): counted, waiting for its visibility, waiting and clamped then
 constexpr float kTermPending = 0.f, kTermCounted = 1.f, kTermClamp = 2.f;
@@ -99,7 +99,19 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
     for (uint32_t depth = 0; depth < a.maxDepth && count < a.maxV; ++depth) {
         float t = 0.f, u = 0.f, v = 0.f;
         const uint32_t h = traverse<false, false>(sc, r, S, t, u, v, tc, a.overflow);
-        if (h == ~0u) break;
+        if (h == ~0u) {
+            // the camera subpath's environment vertex (BDPTIntegrator.cpp:100-118): the escaped
+            // direction's radiance, an area pdf of 0
+            if (!light && sc.hasEnv && count < a.maxV) {
+                const uint32_t vi = count++;
+                a.verts[vAt(a, sub, vi, 0, i)] = f4(r.o + r.d * 1e6f, 0.f);
+                a.verts[vAt(a, sub, vi, 1, i)] = f4(-r.d, 0.f);
+                a.verts[vAt(a, sub, vi, 2, i)] = f4(-r.d, __uint_as_float(kVLight | kVInf));
+                a.verts[vAt(a, sub, vi, 3, i)] = f4(beta, 1.f);
+                a.verts[vAt(a, sub, vi, 4, i)] = f4(domeEval(sc.dome, r.d), __int_as_float(-1));
+            }
+            break;
+        }
         // fillSurfaceInteraction (BVHBackend.cpp:285-317)
         const DTri* T = sc.tris + h;
         const float w = 1.f - u - v;
@@ -168,6 +180,84 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
 
 }  // namespace
 
+namespace {
+
+struct LeSample {
+    V3 Le, pos, n, dir;
+    float pdfPos, pdfDir;
+};
+
+// ILight::sampleLe: AreaLight.h:81-105 (cosine-weighted from a point of the quad), DomeLight.h:142-168
+// (a direction from the sky's distribution, the origin on a disk of the scene sphere's radius facing
+// the scene at its far side), DirectionalLight.h:88-125 (a disk 100 radii toward the light, the
+// direction jittered within the cone).  A zero pdfPos marks no sample.
+__device__ LeSample lightSampleLe(const DScene& sc, const DLight& L, V2 uPos, V2 uDir) {
+    LeSample s{};
+    if (L.type == AHIP_LIGHT_DOME) {
+        float pdfDir;
+        const V3 envDir = domeSampleDir(sc.dome, uDir, pdfDir);
+        if (pdfDir <= 0.f) return s;
+        V3 t, bt;
+        onb(-envDir, t, bt);
+        const float R = L.sceneR, r = R * sqrtf(uPos.x), phi = 2.f * kPi * uPos.y;
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        const V3 dc = ld3(L.sceneC) + envDir * R;
+        s.pos = dc + t * (r * cp) + bt * (r * sp);
+        s.Le = domeEval(sc.dome, envDir);
+        s.n = -envDir;
+        s.dir = -envDir;
+        s.pdfPos = 1.f / (kPi * R * R);
+        s.pdfDir = pdfDir;
+        return s;
+    }
+    if (L.type == AHIP_LIGHT_DIRECTIONAL) {
+        const float R = L.sceneR, r = R * sqrtf(uPos.x), phi = 2.f * kPi * uPos.y;
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        const V3 dl = ld3(L.dir);
+        V3 t, bt;
+        onb(dl, t, bt);
+        const V3 dc = ld3(L.sceneC) + dl * (R * 100.f);
+        s.pos = dc + t * (r * cp) + bt * (r * sp);
+        V3 dir;
+        if (L.cosCone >= 1.f - 1e-6f) {
+            dir = -dl;
+        } else {
+            const float cosT = 1.f - uDir.x * (1.f - L.cosCone);
+            const float sinT = sqrtf(stdmax(0.f, 1.f - cosT * cosT));
+            float sq, cq;
+            sincosf(2.f * kPi * uDir.y, &sq, &cq);
+            const V3 nd = -dl;
+            V3 dt, dbt;
+            onb(nd, dt, dbt);
+            dir = safeNormalize(dt * (sinT * cq) + dbt * (sinT * sq) + nd * cosT);
+        }
+        s.Le = ld3(L.Le);
+        s.n = dir;
+        s.dir = dir;
+        s.pdfPos = 1.f / (kPi * R * R);
+        s.pdfDir = 1.f;
+        return s;
+    }
+    const V3 ln = ld3(L.normal);
+    s.pos = ld3(L.center) + ld3(L.uA) * (2.f * uPos.x - 1.f) + ld3(L.vA) * (2.f * uPos.y - 1.f);
+    const float phi = 2.f * kPi * uDir.x;
+    const float cosT = sqrtf(uDir.y), sinT = sqrtf(1.f - uDir.y);
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    V3 t, bt;
+    onb(ln, t, bt);
+    s.dir = t * (sinT * cp) + bt * (sinT * sp) + ln * cosT;
+    s.n = ln;
+    s.Le = ld3(L.Le);
+    s.pdfPos = 1.f / L.area;
+    s.pdfDir = cosT * kInvPi;
+    return s;
+}
+
+}  // namespace
+
 __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
     __shared__ uint64_t lstk[kBdptK * kBdptBS];
     TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
@@ -194,7 +284,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
                                  V3{1.f, 1.f, 1.f}, 1.f, px, py, sidx, dim, false, tc);
         // first-hit AOVs (BDPTIntegrator.cpp:502-512): IMaterial::reflectance at the first vertex
         float4 alb = make_float4(0.f, 0.f, 0.f, 0.f), nrm = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (nc >= 2u) {
+        if (nc >= 2u && (__float_as_uint(a.verts[vAt(a, 0, 1, 2, i)].w) & 3u) == kVSurface) {
             const int32_t mi = __float_as_int(a.verts[vAt(a, 0, 1, 4, i)].w);
             const DMat& m = sc.mats[mi];
             const V3 n1 = v3(a.verts[vAt(a, 0, 1, 1, i)]);
@@ -218,26 +308,18 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
             const V2 uDir{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
             const V2 uPos{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
             if (lpdf > 0.f) {
-                // AreaLight::sampleLe (AreaLight.h:81-105)
-                const V3 ln = ld3(L.normal), Le = ld3(L.Le);
-                const V3 pos = ld3(L.center) + ld3(L.uA) * (2.f * uPos.x - 1.f) + ld3(L.vA) * (2.f * uPos.y - 1.f);
-                const float phi = 2.f * kPi * uDir.x;
-                const float cosT = sqrtf(uDir.y), sinT = sqrtf(1.f - uDir.y);
-                float sp, cp;
-                sincosf(phi, &sp, &cp);
-                V3 t, bt;
-                onb(ln, t, bt);
-                const V3 dirL = t * (sinT * cp) + bt * (sinT * sp) + ln * cosT;
-                const float pdfPos = 1.f / L.area, pdfDir = cosT * kInvPi;
-                if (pdfPos > 0.f && pdfDir > 0.f) {
-                    const V3 beta = Le * fabsf(dot(dirL, ln)) / (lpdf * pdfPos * pdfDir);
-                    a.verts[vAt(a, 1, 0, 0, i)] = f4(pos, lpdf * pdfPos);
-                    a.verts[vAt(a, 1, 0, 1, i)] = f4(ln, 0.f);
-                    a.verts[vAt(a, 1, 0, 2, i)] = f4(dirL, __uint_as_float(kVLight));
-                    a.verts[vAt(a, 1, 0, 3, i)] = f4(Le / (lpdf * pdfPos), 1.f);
-                    a.verts[vAt(a, 1, 0, 4, i)] = f4(Le, __int_as_float((int32_t)li));
-                    nl = walk(sc, a, S, i, 1u, 1u, pos, makeTRay(pos, dirL, 1e-4f, __int_as_float(0x7f800000)), beta,
-                              pdfDir, px, py, sidx, dim, true, tc);
+                LeSample ls = lightSampleLe(sc, L, uPos, uDir);
+                if (ls.pdfPos > 0.f && ls.pdfDir > 0.f) {
+                    const V3 beta = ls.Le * fabsf(dot(ls.dir, ls.n)) / (lpdf * ls.pdfPos * ls.pdfDir);
+                    const uint32_t fl = kVLight | (L.type == AHIP_LIGHT_DIRECTIONAL ? kVDelta : 0u) |
+                                        (L.type != AHIP_LIGHT_AREA ? kVInf : 0u);
+                    a.verts[vAt(a, 1, 0, 0, i)] = f4(ls.pos, lpdf * ls.pdfPos);
+                    a.verts[vAt(a, 1, 0, 1, i)] = f4(ls.n, 0.f);
+                    a.verts[vAt(a, 1, 0, 2, i)] = f4(ls.dir, __uint_as_float(fl));
+                    a.verts[vAt(a, 1, 0, 3, i)] = f4(ls.Le / (lpdf * ls.pdfPos), 1.f);
+                    a.verts[vAt(a, 1, 0, 4, i)] = f4(ls.Le, __int_as_float((int32_t)li));
+                    nl = walk(sc, a, S, i, 1u, 1u, ls.pos, makeTRay(ls.pos, ls.dir, 1e-4f, __int_as_float(0x7f800000)),
+                              beta, ls.pdfDir, px, py, sidx, dim, true, tc);
                 }
             }
         }
@@ -299,8 +381,40 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                 bool ray = false;
                 V3 from{0.f, 0.f, 0.f}, fromN{0.f, 0.f, 0.f}, to{0.f, 0.f, 0.f};
                 if (s == 0u) {
-                    // (0, t): emission at the last camera vertex (BDPTIntegrator.cpp:318-343)
-                    if (t >= 2u && cSurf) C = cbeta * v3(c4);
+                    // (0, t): emission at the last camera vertex (BDPTIntegrator.cpp:318-343); an
+                    // environment vertex only straight after the camera (t = 2)
+                    if (t >= 2u && (cSurf || (t == 2u && (cflags & kVInf)))) C = cbeta * v3(c4);
+                } else if (s == 1u && (__float_as_uint(a.verts[vAt(a, 1, 0, 2, i)].w) & kVInf)) {
+                    // (1, t) from the dome or a directional light: along the light subpath's own
+                    // direction, a shadow ray 1e10 long, Le / (light pdf x directional pdf)
+                    // (:357-365, :395-402; DirectionalLight.h:78-80: pdf 0, taken as 1)
+                    if (!(cflags & kVDelta) && cm >= 0) {
+                        const float4 l2 = a.verts[vAt(a, 1, 0, 2, i)], l4 = a.verts[vAt(a, 1, 0, 4, i)];
+                        const int32_t li = __float_as_int(l4.w);
+                        const DLight& L = sc.lights[li];
+                        const V3 wi = safeNormalize(-v3(l2));
+                        float pdf;
+                        const DMat& m = sc.mats[cm];
+                        const Frame fr = makeFrame(cn, cn, -cwo);
+                        const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
+                        V3 Le{0.f, 0.f, 0.f};
+                        float pdfDir = 1.f;
+                        if (L.type == AHIP_LIGHT_DOME) {
+                            Le = domeEval(sc.dome, wi);
+                            pdfDir = domePdf(sc.dome, wi);
+                            if (pdfDir <= 0.f) pdfDir = 1.f;
+                        } else if (dot(wi, ld3(L.dir)) > L.cosCone - 1e-4f) {   // DirectionalLight::Le (:131-134)
+                            Le = ld3(L.Le);
+                        }
+                        if (!isBlack(f) && !isBlack(Le)) {
+                            const float cosI = fabsf(dot(wi, cn));
+                            C = cbeta * f * cosI * Le / (a.lightW[li] * pdfDir);
+                            ray = true;
+                            from = cp;
+                            fromN = cn;
+                            to = cp + wi * 1e10f;
+                        }
+                    }
                 } else if (s == 1u) {
                     // (1, t): the light endpoint toward camera vertex t-1 (:345-410)
                     if (!(cflags & kVDelta) && cm >= 0) {

@@ -129,6 +129,8 @@ struct DLight {
     float area, uLen, vLen;
     float dir[3];             // directional: unit dirToLight
     float cosCone;            // directional: cos of the angular radius (1 = hard)
+    float sceneC[3];          // dome, directional: centre and radius of the scene sphere where
+    float sceneR;             //   BDPT's light subpaths start (sampleLe)
 };
 
 struct DTexture {

@@ -302,6 +302,15 @@ int buildHostScene(const ahip_scene_desc* d, HostScene& out, std::string& err) {
             err = "unknown light type";
             return AHIP_E_INVALID_ARG;
         }
+        if (l.type != AHIP_LIGHT_AREA) {
+            if (!(l.scene_radius >= 0.f) || !std::isfinite(l.scene_radius) || !std::isfinite(l.scene_center[0]) ||
+                !std::isfinite(l.scene_center[1]) || !std::isfinite(l.scene_center[2])) {
+                err = "a light's scene_radius must be finite and >= 0 and its scene_center finite";
+                return AHIP_E_INVALID_ARG;
+            }
+            std::memcpy(L.sceneC, l.scene_center, 12);
+            L.sceneR = l.scene_radius;
+        }
         out.lights.push_back(L);
     }
     if (d->env_light >= 0) {

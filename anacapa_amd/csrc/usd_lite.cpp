@@ -1431,6 +1431,23 @@ ahip_usd_scene* build(const std::string& text, uint32_t w, uint32_t h, const std
     }
     for (const MaterialOut& m : ld.mats) s->materials.push_back(m.m);
     s->lights = ld.lights;
+    // the lights' scene sphere once every mesh is loaded (USDLoader.cpp:1722-1744): the bounds'
+    // centroid and 1.5 x the half diagonal
+    if (!s->positions.empty()) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = 0; i < s->positions.size(); i += 3)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], s->positions[i + k]);
+                hi[k] = std::max(hi[k], s->positions[i + k]);
+            }
+        const V3 diag{hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        const float radius = diag.length() * 0.5f * 1.5f;
+        for (ahip_light& l : s->lights)
+            if (l.type == AHIP_LIGHT_DIRECTIONAL || l.type == AHIP_LIGHT_DOME) {
+                for (int k = 0; k < 3; ++k) l.scene_center[k] = (lo[k] + hi[k]) * 0.5f;
+                l.scene_radius = radius;
+            }
+    }
     s->hasCamera = ld.hasCamera;
     std::memcpy(s->from, ld.camFrom, sizeof s->from);
     std::memcpy(s->at, ld.camAt, sizeof s->at);

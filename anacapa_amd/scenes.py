@@ -62,6 +62,22 @@ class Scene:
     def num_triangles(self) -> int:
         return int(len(self.indices) // 3)
 
+    def _with_sphere(self, light):
+        """A dome or directional light left without a scene sphere gets the one its reference loader
+        gives it: the triangle bounds' centroid, and the half diagonal x 2 for the dome
+        (RenderSession.cpp:407-416) or x 1.5 for a directional light (USDLoader.cpp:1722-1744)."""
+        if light.type not in (A.AHIP_LIGHT_DOME, A.AHIP_LIGHT_DIRECTIONAL) or light.scene_radius != 0 \
+                or len(self.positions) == 0:
+            return light
+        p = np.asarray(self.positions, np.float32).reshape(-1, 3)
+        lo, hi = p.min(0), p.max(0)
+        dg = hi - lo
+        half = np.float32(np.sqrt(np.float32(dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2]))) * np.float32(0.5)
+        l = A.Light.from_buffer_copy(light)
+        l.scene_center = A.F3(*((lo + hi) * np.float32(0.5)).tolist())
+        l.scene_radius = float(half * np.float32(2.0 if light.type == A.AHIP_LIGHT_DOME else 1.5))
+        return l
+
     def desc(self):
         """Build an ahip_scene_desc; returns (desc, keepalive) — keep both alive while in use."""
         keep = []
@@ -83,7 +99,7 @@ class Scene:
         uc = arr(self.uv_count, np.uint32)
         mm = arr(self.mesh_material, np.int32)
         mats = (A.Material * max(1, len(self.materials)))(*self.materials)
-        lts = (A.Light * max(1, len(self.lights)))(*self.lights)
+        lts = (A.Light * max(1, len(self.lights)))(*[self._with_sphere(l) for l in self.lights])
         keep += [mats, lts]
         d = A.SceneDesc()
         d.num_meshes = self.num_meshes

@@ -98,6 +98,14 @@ typedef struct ahip_light {
                                    on prepare as the constructor does); Le: its radiance */
     float    half_angle;        /* directional: angular radius in degrees (0 = a hard, exact direction;
                                    DirectionalLight.h:33-45, 55-74) */
+    float    scene_center[3];   /* dome, directional: the scene's bounding sphere, where BDPT's light
+                                   subpaths start (sampleLe: a disk of radius scene_radius facing the
+                                   scene; DomeLight.h:84-92, 142-168, DirectionalLight.h:28-41, 88-125);
+                                   centre = the geometry bounds' centroid, radius = the half
+                                   diagonal x 2 for RenderSession's dome (RenderSession.cpp:407-416),
+                                   x 1.5 for the USD loader's lights (USDLoader.cpp:1730-1744).
+                                   The path tracer does not read them. */
+    float    scene_radius;      /* > 0 for BDPT with a dome or directional light */
 } ahip_light;
 
 /* Flat scene.  Mirrors GeometryPool / SceneView (include/anacapa/accel/GeometryPool.h:118-197,
@@ -168,8 +176,10 @@ typedef struct ahip_camera {
 #define AHIP_INTEGRATOR_PATH 0u   /* PathIntegrator (src/integrator/PathIntegrator.cpp) */
 #define AHIP_INTEGRATOR_BDPT 1u   /* BDPTIntegrator (src/integrator/BDPTIntegrator.cpp): scenes of
                                      triangles with Lambert / Emissive / untextured StandardSurface
-                                     materials (glass and constant alpha masks included) and area
-                                     lights, no environment, no per-triangle albedo; ahip_render_tile,
+                                     materials (glass and constant alpha masks included) lit by area
+                                     lights, the dome and directional lights (each with its
+                                     scene_radius > 0); no constant env_radiance, no per-triangle
+                                     albedo; ahip_render_tile,
                                      ahip_render_tile_aovs and ahip_render_frame (without AOVs) */
 
 typedef struct ahip_render_params {

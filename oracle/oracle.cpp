@@ -935,6 +935,8 @@ struct Light {
     const Dome* dome = nullptr;
     V3 dir;                    // directional: unit dirToLight
     float cosCone = 1.f;       // directional: cos of the angular radius
+    V3 sceneC;                 // dome, directional: the scene sphere of sampleLe
+    float sceneR = 0.f;
     bool isDelta() const { return type == AHIP_LIGHT_DIRECTIONAL; }                   // DirectionalLight.h:145
     void init() {                                                                     // AreaLight.h:123-131
         normal = safeNormalize(cross(uA, vA));
@@ -1554,8 +1556,8 @@ struct Scene {
     }
 
     // ------------------------------------------------------------ BDPT (BDPTIntegrator.cpp, MISWeight.cpp)
-    // Scenes of triangles with Lambert / Emissive / untextured opaque StandardSurface materials and
-    // quad area lights, no environment: the scope the device implements.
+    // Scenes of triangles with Lambert / Emissive / untextured StandardSurface materials, lit by quad
+    // area lights, the dome and directional lights: the scope the device implements.
     struct PVert {                                                                    // PathVertex.h:28-60
         V3 p, n, wo, beta, Le;
         float pdfFwd = 0.f, pdfRev = 0.f, minRough = 1.f;
@@ -1582,7 +1584,18 @@ struct Scene {
             std::vector<float> wt(n);
             float total = 0.f;
             for (uint32_t i = 0; i < n; ++i) {
-                wt[i] = fmaxs(0.f, luminance(L[i].Le) * L[i].area * kPi);          // AreaLight.h:114-117
+                float pw;
+                if (L[i].type == AHIP_LIGHT_DOME) {                                   // DomeLight.h:176-179, 255-257
+                    const Dome& D = *L[i].dome;
+                    float envIntegral = D.marg.integral * k2Pi * kPi / (float)(D.W * D.H);
+                    pw = envIntegral * 4.f * kPi * D.intensity;
+                } else if (L[i].type == AHIP_LIGHT_DIRECTIONAL) {                     // DirectionalLight.h:41, 136-139
+                    float diskArea = kPi * L[i].sceneR * L[i].sceneR;
+                    pw = luminance(L[i].Le) * diskArea * kPi;
+                } else {
+                    pw = luminance(L[i].Le) * L[i].area * kPi;                         // AreaLight.h:114-117
+                }
+                wt[i] = fmaxs(0.f, pw);
                 total += wt[i];
             }
             if (total <= 0.f) {
@@ -1641,7 +1654,20 @@ struct Scene {
     void walk(Ray ray, V3 beta, float pdfFwd, Halton& smp, PPath& P, bool light) const {
         for (uint32_t depth = 0; depth < maxDepth && !P.full(); ++depth) {
             SI si;
-            if (!traceSI(ray, si)) break;   // no environment in the supported scenes
+            if (!traceSI(ray, si)) {
+                // the camera subpath's environment vertex (BDPTIntegrator.cpp:100-118)
+                if (!light && hasEnv && !P.full()) {
+                    PVert& V = P.v[P.count++];
+                    V = PVert{};
+                    V.p = ray.o + ray.d * 1e6f;
+                    V.n = -ray.d;
+                    V.wo = -ray.d;
+                    V.beta = beta;
+                    V.Le = dome.eval(ray.d) * dome.intensity;
+                    V.flags = kVLight | kVInf;
+                }
+                break;
+            }
             const Material* mat = matOf(si.meshID);
             if (!mat) break;
             V3 wo = -ray.d;
@@ -1706,7 +1732,71 @@ struct Scene {
         V.flags = kVCamera;
         walk(primary, {1.f, 1.f, 1.f}, 1.f, smp, P, false);
     }
-    // traceLightSubpath (BDPTIntegrator.cpp:199-303); AreaLight::sampleLe (AreaLight.h:81-105)
+    // ILight::sampleLe: AreaLight.h:81-105, DomeLight.h:142-168, DirectionalLight.h:88-125.  pdfPos 0:
+    // no sample.
+    struct LeSmp {
+        V3 Le, pos, n, dir;
+        float pdfPos = 0.f, pdfDir = 0.f;
+    };
+    static LeSmp sampleLe(const Light& L, V2 uPos, V2 uDir) {
+        LeSmp s;
+        if (L.type == AHIP_LIGHT_DOME) {
+            float pdfDir;
+            V3 envDir = L.dome->sampleDir(uDir, pdfDir);
+            if (pdfDir <= 0.f) return s;
+            V3 t, bt;
+            onb(-envDir, t, bt);
+            float r = L.sceneR * std::sqrt(uPos.x);
+            float phi = k2Pi * uPos.y;
+            V3 dc = L.sceneC + envDir * L.sceneR;
+            s.pos = dc + t * (r * std::cos(phi)) + bt * (r * std::sin(phi));
+            float diskArea = kPi * L.sceneR * L.sceneR;
+            s.Le = L.dome->eval(envDir) * L.dome->intensity;
+            s.n = -envDir;
+            s.dir = -envDir;
+            s.pdfPos = 1.f / diskArea;
+            s.pdfDir = pdfDir;
+            return s;
+        }
+        if (L.type == AHIP_LIGHT_DIRECTIONAL) {
+            float r = L.sceneR * std::sqrt(uPos.x);
+            float phi = 2.f * kPi * uPos.y;
+            V3 t, bt;
+            onb(L.dir, t, bt);
+            V3 dc = L.sceneC + L.dir * (L.sceneR * 100.f);
+            s.pos = dc + t * (r * std::cos(phi)) + bt * (r * std::sin(phi));
+            V3 dir;
+            if (L.cosCone >= 1.f - 1e-6f) {
+                dir = -L.dir;
+            } else {
+                float cosT = 1.f - uDir.x * (1.f - L.cosCone);
+                float sinT = std::sqrt(fmaxs(0.f, 1.f - cosT * cosT));
+                float p = 2.f * kPi * uDir.y;
+                V3 nd = -L.dir, dt, dbt;
+                onb(nd, dt, dbt);
+                dir = safeNormalize(dt * (sinT * std::cos(p)) + dbt * (sinT * std::sin(p)) + nd * cosT);
+            }
+            float diskArea = kPi * L.sceneR * L.sceneR;
+            s.Le = L.Le;
+            s.n = dir;
+            s.dir = dir;
+            s.pdfPos = 1.f / diskArea;
+            s.pdfDir = 1.f;
+            return s;
+        }
+        s.pos = L.center + L.uA * (2.f * uPos.x - 1.f) + L.vA * (2.f * uPos.y - 1.f);
+        float phi = 2.f * kPi * uDir.x;
+        float cosT = std::sqrt(uDir.y), sinT = std::sqrt(1.f - uDir.y);
+        V3 t, bt;
+        onb(L.normal, t, bt);
+        s.dir = t * (sinT * std::cos(phi)) + bt * (sinT * std::sin(phi)) + L.normal * cosT;
+        s.n = L.normal;
+        s.Le = L.Le;
+        s.pdfPos = 1.f / L.area;
+        s.pdfDir = cosT * kInvPi;
+        return s;
+    }
+    // traceLightSubpath (BDPTIntegrator.cpp:199-303)
     void lightSubpath(uint32_t li, float lpdf, Halton& smp, PPath& P) const {
         P.count = 0;
         P.lightPdf = lpdf;
@@ -1715,28 +1805,22 @@ struct Scene {
         // light->sampleLe(get2D(), get2D()): g++ evaluates the arguments right to left
         V2 uDir = smp.get2D();
         V2 uPos = smp.get2D();
-        V3 pos = L.center + L.uA * (2.f * uPos.x - 1.f) + L.vA * (2.f * uPos.y - 1.f);
-        float phi = 2.f * kPi * uDir.x;
-        float cosT = std::sqrt(uDir.y), sinT = std::sqrt(1.f - uDir.y);
-        V3 t, bt;
-        onb(L.normal, t, bt);
-        V3 dir = t * (sinT * std::cos(phi)) + bt * (sinT * std::sin(phi)) + L.normal * cosT;
-        float pdfPos = 1.f / L.area, pdfDir = cosT * kInvPi;
-        if (pdfPos <= 0.f || pdfDir <= 0.f) return;
-        V3 beta = L.Le * std::abs(dot(dir, L.normal)) / (lpdf * pdfPos * pdfDir);
+        LeSmp ls = sampleLe(L, uPos, uDir);
+        if (ls.pdfPos <= 0.f || ls.pdfDir <= 0.f) return;
+        V3 beta = ls.Le * std::abs(dot(ls.dir, ls.n)) / (lpdf * ls.pdfPos * ls.pdfDir);
         PVert& V = P.v[P.count++];
         V = PVert{};
-        V.p = pos;
-        V.n = L.normal;
-        V.wo = dir;
-        V.beta = L.Le / (lpdf * pdfPos);
-        V.Le = L.Le;
-        V.pdfFwd = lpdf * pdfPos;
-        V.flags = kVLight;
+        V.p = ls.pos;
+        V.n = ls.n;
+        V.wo = ls.dir;
+        V.beta = ls.Le / (lpdf * ls.pdfPos);
+        V.Le = ls.Le;
+        V.pdfFwd = lpdf * ls.pdfPos;
+        V.flags = kVLight | (L.isDelta() ? kVDelta : 0u) | (L.type != AHIP_LIGHT_AREA ? kVInf : 0u);
         V.light = &L;
-        Ray ray{pos, dir};
+        Ray ray{ls.pos, ls.dir};
         ray.tMin = 1e-4f;
-        walk(ray, beta, pdfDir, smp, P, true);
+        walk(ray, beta, ls.pdfDir, smp, P, true);
     }
     // bdptMISWeight (MISWeight.cpp:14-72), in double as there
     static float misWeight(const PPath& lp, const PPath& cp, uint32_t s, uint32_t t) {
@@ -1795,7 +1879,40 @@ struct Scene {
             if (t < 2) return {};
             uint32_t last = t - 1;
             if (cp.type(last) == kVSurface) return cp.v[last].beta * cp.v[last].Le;
-            return {};   // light-type camera vertices are environment vertices: none here
+            // light-type camera vertices are environment vertices, counted right after the camera
+            if ((cp.v[last].flags & kVInf) && t == 2) return cp.v[last].beta * cp.v[last].Le;
+            return {};
+        }
+        if (s == 1 && (lp.v[0].flags & kVInf)) {
+            // the dome or a directional light: along the light subpath's own direction
+            // (BDPTIntegrator.cpp:357-365, 395-402)
+            uint32_t ct = t - 1;
+            if (cp.delta(ct) || !cp.v[ct].mat) return {};
+            const Light* light = lp.v[0].light;
+            if (!light) return {};
+            V3 wi = safeNormalize(-lp.v[0].wo);
+            Ray sr = spawnRayTo(cp.v[ct].p, cp.v[ct].n, cp.v[ct].p + wi * 1e10f);
+            const V3 Tr1 = connTr(sr);
+            if (isBlack(Tr1)) return {};
+            SI si;
+            si.p = cp.v[ct].p;
+            si.n = si.ng = cp.v[ct].n;
+            Ctx ctx(si, -cp.v[ct].wo);
+            BSDFEval be = cp.v[ct].mat->evaluate(ctx, cp.v[ct].wo, wi);
+            if (isBlack(be.f)) return {};
+            V3 Le;
+            float pdfDir;
+            if (light->type == AHIP_LIGHT_DOME) {                                          // DomeLight.h:131-133, 171-173
+                Le = light->dome->eval(wi) * light->dome->intensity;
+                pdfDir = light->dome->evalPdf(wi);
+            } else {                                                                   // DirectionalLight.h:78-80, 131-134
+                Le = dot(wi, light->dir) > light->cosCone - 1e-4f ? light->Le : V3{};
+                pdfDir = 0.f;
+            }
+            if (isBlack(Le)) return {};
+            float cosI = std::abs(dot(wi, cp.v[ct].n));
+            if (pdfDir <= 0.f) pdfDir = 1.f;
+            return cp.v[ct].beta * be.f * cosI * Le * Tr1 / (lp.lightPdf * pdfDir);
         }
         if (s == 1) {
             uint32_t ct = t - 1;
@@ -2055,6 +2172,10 @@ void* orc_scene_create(const ahip_scene_desc* d) {
             if ((int32_t)i != d->env_light) return nullptr;
             L.dome = &s->dome;
         }
+        if (l.type != AHIP_LIGHT_AREA) {
+            L.sceneC = V3{l.scene_center[0], l.scene_center[1], l.scene_center[2]};
+            L.sceneR = l.scene_radius;
+        }
         s->lights.push_back(L);
     }
     return s.release();
@@ -2170,13 +2291,14 @@ int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
 // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581) with firefly clamp `clamp` (0 = off):
 // out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and the per-sample flags border [w*h*ns]
 // (bit 0: Scene::connTr's borderline visibility, bit 1: Scene::smooth) may be null.  Scenes of triangles with
-// Lambert / Emissive / untextured StandardSurface materials and area lights only (returns 1 otherwise).
+// Lambert / Emissive / untextured StandardSurface materials lit by area lights, the dome and directional
+// lights with a scene sphere only (returns 1 otherwise).
 int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
                          float clamp, float* out, float* albedo, float* normal, uint8_t* border) {
     const Scene* s = static_cast<Scene*>(h);
-    if (!s->curves.nodes.empty() || s->hasEnv || !isBlack(s->envRad)) return 1;
+    if (!s->curves.nodes.empty() || !isBlack(s->envRad)) return 1;
     for (const Light& L : s->lights)
-        if (L.type != AHIP_LIGHT_AREA) return 1;
+        if (L.type != AHIP_LIGHT_AREA && !(L.sceneR > 0.f)) return 1;
     if (!s->triAlbedo.empty()) return 1;
     for (const Material& m : s->mats)
         if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE &&

@@ -852,6 +852,11 @@ public:
     float power() const override { return h_.s.d.DomeLight::power(); }
     bool isInfinite() const override { return true; }
     bool isDelta() const override { return false; }
+    // the scene sphere of sampleLe, as RenderSession / the USD loader set it (DomeLight.h:187-188)
+    void setSphere(float radius, Vec3f center) {
+        h_.s.d.setSceneRadius(radius);
+        h_.s.d.setSceneCenter(center);
+    }
 
 private:
     DomeHolder h_;
@@ -859,7 +864,9 @@ private:
 }  // namespace
 
 std::unique_ptr<ILight> makeHarnessDome(const ahip_scene_desc* d, const ahip_light& l) {
-    return std::make_unique<HarnessDome>(d->env_width, d->env_height, d->env_pixels, l.intensity, l.rot);
+    auto dome = std::make_unique<HarnessDome>(d->env_width, d->env_height, d->env_pixels, l.intensity, l.rot);
+    if (l.scene_radius > 0.f) dome->setSphere(l.scene_radius, v3(l.scene_center));
+    return dome;
 }
 
 // A dome over an RGB f32 equirect image (row 0 = zenith) and a world->envmap rotation (3 rows).

@@ -86,9 +86,9 @@ inline std::unique_ptr<RefScene> buildRefScene(const ahip_scene_desc* d, Standar
             s->view.envLight = s->lights.back().get();   // the dome is also the environment
             continue;
         }
-        if (l.type == AHIP_LIGHT_DIRECTIONAL) {   // the scene bounds only place BDPT's emission disk
-            s->lights.push_back(std::make_unique<DirectionalLight>(v3(l.direction), v3(l.Le), 1.f, Vec3f{},
-                                                                   l.half_angle));
+        if (l.type == AHIP_LIGHT_DIRECTIONAL) {   // the scene sphere only places BDPT's emission disk
+            s->lights.push_back(std::make_unique<DirectionalLight>(v3(l.direction), v3(l.Le), l.scene_radius,
+                                                                   v3(l.scene_center), l.half_angle));
             s->view.lights.push_back(s->lights.back().get());
             continue;
         }

@@ -316,7 +316,8 @@ class OracleScene:
         bd = np.zeros((h, w, ns), np.uint8)
         st = self.orc.lib.orc_render_tile_bdpt(self.h, x0, y0, w, h, s0, ns, clamp, P(out), P(alb), P(nrm), P(bd))
         if st:
-            raise ValueError("the oracle's BDPT takes Lambert / Emissive triangles and area lights only")
+            raise ValueError("the oracle's BDPT takes Lambert / Emissive / untextured StandardSurface triangles "
+                             "lit by area lights, the dome and directional lights with a scene sphere")
         return (out, alb, nrm, bd) if border else (out, alb, nrm)
 
     def set_eval(self, tri_albedo=None, texture=0, texels=None):

@@ -80,9 +80,14 @@ def test_bdpt_frame_equals_tile_and_clamp(gpu, oracle):
 def test_bdpt_refusals(gpu):
     """BDPT is refused for scenes it does not take (at ahip_set_params once a scene is prepared, and
     at render time), and by the path-tracer-only entry points."""
-    sc = scenes.ggx_spheres(width=16, height=16, spp=1, grid=2, freq=2, hdri_size=(16, 8))   # a dome
+    sc = scenes.cornell(width=16, height=16)
+    sc.env_radiance = (0.1, 0.1, 0.1)   # a constant environment radiance is no light of BDPT's
     with pytest.raises(Exception):
         gpu.prepare(sc, integrator=BDPT)
+    sun = scenes.with_sun(scenes.cornell(width=16, height=16))
+    sun.lights[-1].scene_radius = -1.0   # invalid for any integrator
+    with pytest.raises(Exception):
+        gpu.prepare(sun)
     textured = scenes.cornell(width=16, height=16, glossy=True)
     textured.materials[4].roughness_texture = 0
     textured.textures = [np.full((4, 4), 0.3, np.float32)]
@@ -125,7 +130,8 @@ def test_bdpt_cornell_frame_bands_vs_oracle(gpu, oracle):
         assert abs(m / mr - 1) < 1e-3
 
 
-def test_bdpt_adapter_in_reference_render_loop():
+@pytest.mark.parametrize("sun", [False, True])
+def test_bdpt_adapter_in_reference_render_loop(sun):
     """The reference-side adapter in BDPT mode (HipPathIntegrator(device, maxDepth, BDPT, clamp), the
     drop-in for RenderSession's BDPTIntegrator, RenderSession.cpp:459-461) in the reference's tile
     loop, against the reference's own BDPTIntegrator in the same loop on the same SceneView: sample
@@ -134,12 +140,16 @@ def test_bdpt_adapter_in_reference_render_loop():
     distance, 0.2%: at 8 spp about 1.6% of pixels; measured 0.6%, means within 6e-6).  Pixels whose
     samples hit the ceiling and the light quad, whose normals face opposite ways, have a zero mean
     normal but a record: the adapter tells records by a nonzero albedo or normal.  A scene the
-    device BDPT does not take leaves the adapter unprepared."""
+    device BDPT does not take leaves the adapter unprepared.  sun: a directional light through the
+    box's open side besides the quad, whose scene sphere the adapter reads back from
+    DirectionalLight::sampleLe (to ~1e-7 of the radius)."""
     from adapter_harness import AdapterHarness, available
     from oracle_bindings import Oracle
     if not available():
         pytest.fail("oracle/_ref/libadapter_harness.so not built")
     sc = scenes.cornell(width=200, height=136)
+    if sun:
+        sc = scenes.with_sun(sc)
     cam = Oracle().pinhole(sc.cam_from, sc.cam_at, sc.cam_up, sc.vfov, sc.width, sc.height)
     h = AdapterHarness(sc, cam, sc.max_depth, integrator=BDPT, clamp=10.0)
     st = h.status()
@@ -150,7 +160,7 @@ def test_bdpt_adapter_in_reference_render_loop():
     assert np.array_equal(got[..., 3], ref[..., 3]) and (got[..., 3] == 8).all()
     g, r = got[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     err = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(r.mean(), 1e-6))
-    print(f"[bdpt-adapter] cornell {sc.width}x{sc.height} x 8 spp: {(err > 1e-3).sum()} of {err.size} pixels "
+    print(f"[bdpt-adapter] cornell sun={sun} {sc.width}x{sc.height} x 8 spp: {(err > 1e-3).sum()} of {err.size} pixels "
           f"beyond 1e-3; means {g.mean():.6f} / {r.mean():.6f}", flush=True)
     assert (err > 1e-3).mean() < 0.02
     assert abs(g.mean() / r.mean() - 1) < 2e-4
@@ -162,3 +172,33 @@ def test_bdpt_adapter_in_reference_render_loop():
                         lit.max_depth, integrator=BDPT)
     st2 = h2.status()
     assert st2["valid"] and not st2["prepared"], st2
+
+
+@pytest.mark.parametrize("name", ["dome", "dome_sun", "cornell_sun"])
+def test_bdpt_infinite_lights_vs_oracle(gpu, oracle, name):
+    """The dome and directional lights in BDPT (the oracle is held to the reference's BDPTIntegrator
+    on the same three scenes by tests/test_oracle_bdpt.py): environment vertices, light subpaths from
+    the sky's and the sun's emission disks, the (1, t) strategy along the light subpath's direction.
+    Per sample against the oracle: sample counts and AOVs equal (normals within 1e-4: the FMA build
+    interpolates sphere normals differently), means within 1e-4, at most 0.5% of samples beyond 2e-4
+    and every one within 2e-2 — the oracle built uncontracted differs from the reference's own build
+    by up to 6e-3 on these scenes, the device's contraction pattern is a third one."""
+    from test_oracle_bdpt import INFINITE_SCENES
+    sc = INFINITE_SCENES[name]()
+    os_ = oracle.scene(sc)
+    gpu.prepare(sc, integrator=BDPT)
+    print(f"[bdpt] {name} 64x64 x {NS} spp", flush=True)
+    got, alb, nrm = gpu.render_tile_aovs(0, 0, 64, 64, 0, NS)
+    ref, oalb, onrm = os_.render_tile_bdpt(0, 0, 64, 64, 0, NS)
+    assert np.array_equal(got[..., 3], ref[..., 3])
+    np.testing.assert_allclose(alb, oalb, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(nrm, onrm, rtol=0, atol=1e-4)
+    gm, rm = got[..., :3].astype(np.float64).mean(), ref[..., :3].astype(np.float64).mean()
+    g = np.stack([gpu.render_tile(0, 0, 64, 64, s, 1)[..., :3] for s in range(NS)]).astype(np.float64)
+    r = np.stack([os_.render_tile_bdpt(0, 0, 64, 64, s, 1)[0][..., :3] for s in range(NS)]).astype(np.float64)
+    e = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(np.abs(r).mean(), 1e-6))
+    print(f"[bdpt] {name}: means {gm:.7g} / {rm:.7g}; {int((e > 2e-4).sum())} of {e.size} samples beyond "
+          f"2e-4, largest {e.max():.3g}", flush=True)
+    assert abs(gm / rm - 1) < 1e-4
+    assert (e > 2e-4).sum() <= 5e-3 * e.size
+    assert e.max() <= 2e-2
