@@ -756,8 +756,10 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
         S.dome.intensity = D.intensity;
         std::memcpy(S.dome.rot, D.rot, sizeof(D.rot));
     }
-    // BDPT takes triangles with Lambert / Emissive / StandardSurface materials without textures
-    // (the device forms no per-vertex uv) lit by quad area lights, the dome and directional lights
+    // BDPT takes triangles with Lambert / Emissive / StandardSurface materials without a roughness
+    // texture (the vertices keep no uv, and connections evaluate the BSDF at stored vertices; an
+    // opacity texture is read only at fresh hits: the camera subpath's alpha test and the
+    // connections' transmittance) lit by quad area lights, the dome and directional lights
     ctx->bdptRefusal.clear();
     if (!ctx->curves.nodes.empty()) ctx->bdptRefusal = "curves";
     else if (d->env_radiance[0] != 0.f || d->env_radiance[1] != 0.f || d->env_radiance[2] != 0.f)
@@ -765,8 +767,8 @@ int ahip_prepare(ahip_context* ctx, const ahip_scene_desc* d) {
     if (ctx->bdptRefusal.empty() && d->tri_albedo) ctx->bdptRefusal = "per-triangle albedo";
     for (const DMat& m : hs.mats)
         if (ctx->bdptRefusal.empty() && m.type != AHIP_MAT_LAMBERT && m.type != AHIP_MAT_EMISSIVE &&
-            !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0 && m.opacityTex < 0))
-            ctx->bdptRefusal = "a material other than Lambert / Emissive / untextured StandardSurface";
+            !(m.type == AHIP_MAT_STANDARD && m.roughTex < 0))
+            ctx->bdptRefusal = "a material other than Lambert / Emissive / StandardSurface without a roughness texture";
     for (const DLight& L : hs.lights)
         if (ctx->bdptRefusal.empty() && L.type != AHIP_LIGHT_AREA && !(L.sceneR > 0.f))
             ctx->bdptRefusal = "a dome or directional light without a scene_radius";

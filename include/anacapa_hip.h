@@ -175,8 +175,8 @@ typedef struct ahip_camera {
 /* ahip_render_params.integrator (RenderSettings::integrator, src/render/RenderSession.h:45-47) */
 #define AHIP_INTEGRATOR_PATH 0u   /* PathIntegrator (src/integrator/PathIntegrator.cpp) */
 #define AHIP_INTEGRATOR_BDPT 1u   /* BDPTIntegrator (src/integrator/BDPTIntegrator.cpp): scenes of
-                                     triangles with Lambert / Emissive / untextured StandardSurface
-                                     materials (glass and constant alpha masks included) lit by area
+                                     triangles with Lambert / Emissive / StandardSurface materials
+                                     without a roughness texture (glass and alpha masks included) lit by area
                                      lights, the dome and directional lights (each with its
                                      scene_radius > 0); no constant env_radiance, no per-triangle
                                      albedo; ahip_render_tile,

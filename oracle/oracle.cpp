@@ -1556,7 +1556,7 @@ struct Scene {
     }
 
     // ------------------------------------------------------------ BDPT (BDPTIntegrator.cpp, MISWeight.cpp)
-    // Scenes of triangles with Lambert / Emissive / untextured StandardSurface materials, lit by quad
+    // Scenes of triangles with Lambert / Emissive / StandardSurface materials without a roughness texture, lit by quad
     // area lights, the dome and directional lights: the scope the device implements.
     struct PVert {                                                                    // PathVertex.h:28-60
         V3 p, n, wo, beta, Le;
@@ -2291,7 +2291,7 @@ int orc_render_tile_aovs(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
 // BDPTIntegrator::renderTile (BDPTIntegrator.cpp:464-581) with firefly clamp `clamp` (0 = off):
 // out[w*h*5] as orc_render_tile; albedo / normal [w*h*3] and the per-sample flags border [w*h*ns]
 // (bit 0: Scene::connTr's borderline visibility, bit 1: Scene::smooth) may be null.  Scenes of triangles with
-// Lambert / Emissive / untextured StandardSurface materials lit by area lights, the dome and directional
+// Lambert / Emissive / StandardSurface materials (no roughness texture) lit by area lights, the dome and directional
 // lights with a scene sphere only (returns 1 otherwise).
 int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t hh, uint32_t s0, uint32_t ns,
                          float clamp, float* out, float* albedo, float* normal, uint8_t* border) {
@@ -2302,7 +2302,7 @@ int orc_render_tile_bdpt(void* h, uint32_t x0, uint32_t y0, uint32_t w, uint32_t
     if (!s->triAlbedo.empty()) return 1;
     for (const Material& m : s->mats)
         if (m.m.type != AHIP_MAT_LAMBERT && m.m.type != AHIP_MAT_EMISSIVE &&
-            !(m.m.type == AHIP_MAT_STANDARD && !m.roughTex && !m.opacityTex))
+            !(m.m.type == AHIP_MAT_STANDARD && !m.roughTex))
             return 1;
     s->renderTileBDPT(x0, y0, w, hh, s0, ns, clamp, out, albedo, normal, border);
     return 0;

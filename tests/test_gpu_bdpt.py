@@ -20,15 +20,19 @@ NS = 4
 BDPT = A.AHIP_INTEGRATOR_BDPT
 
 
-@pytest.mark.parametrize("two_lights,glossy", [(False, False), (True, False), (True, True), (False, "glass")])
+@pytest.mark.parametrize("two_lights,glossy", [(False, False), (True, False), (True, True), (False, "glass"),
+                                               (False, "glass_texture")])
 def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights, glossy):
     """glossy: standard_surface gold metal and coated dielectric blocks and a mirror panel (a delta
     BSDF, whose connections the reference skips; measured: 64 samples beyond 2e-4, all flagged).
     "glass": a smooth and a rough tinted glass block and two panels of partial opacity, whose
     connections go through k_bdpt_conn_trace's stepping instance (measured: 33 samples beyond 2e-4,
-    30 flagged, the other three within 7.3e-4)."""
-    if glossy == "glass":
-        sc = scenes.cornell_glass(width=64, height=64)
+    30 flagged, the other three within 7.3e-4).  "glass_texture": the first panel's opacity from a
+    texture, read at the camera subpath's hits and at the connections' stepping hits (no reference
+    pin: the reference's side takes constant opacity only; measured: 33 beyond 2e-4, 30 flagged, the
+    others within 8.6e-4)."""
+    if glossy in ("glass", "glass_texture"):
+        sc = scenes.cornell_glass(width=64, height=64, alpha_texture=glossy == "glass_texture")
     else:
         sc = scenes.cornell(width=64, height=64, second_light=two_lights, glossy=glossy)
     os_ = oracle.scene(sc)
@@ -49,7 +53,7 @@ def test_bdpt_tile_vs_oracle(gpu, oracle, two_lights, glossy):
     off = e > 2e-4
     print(f"[bdpt] {int(off.sum())} of {e.size} samples beyond 2e-4, {int((off & border).sum())} flagged "
           f"({int(border.sum())} flagged in all); largest unflagged {e[~border].max():.3g}", flush=True)
-    if glossy == "glass":   # the oracle against the reference leaves two unflagged samples within 6e-4
+    if glossy in ("glass", "glass_texture"):   # the oracle against the reference leaves two unflagged within 6e-4
         assert e[~border].max() <= 1e-3
         assert (off & ~border).sum() <= 5e-4 * e.size
     else:

@@ -95,8 +95,7 @@ def test_glass_with_curves_vs_oracle(gpu, oracle):
 
 
 def test_transparent_scenes_refused_where_unsupported(gpu):
-    """Replay / adjoint and the traffic counter take opaque scenes only; BDPT refuses an opacity
-    texture (its vertices keep no uv)."""
+    """Replay / adjoint and the traffic counter take opaque scenes only."""
     from anacapa_amd.integrator import BackendError
     sc = scenes.cornell_glass(width=16, height=16, spp=1)
     gpu.prepare(sc)
@@ -105,8 +104,4 @@ def test_transparent_scenes_refused_where_unsupported(gpu):
     assert e.value.status == A.AHIP_E_UNSUPPORTED
     with pytest.raises(BackendError) as e:
         gpu.replay(0, 1)
-    assert e.value.status == A.AHIP_E_UNSUPPORTED
-    gpu.prepare(scenes.cornell_glass(width=16, height=16, spp=1, alpha_texture=True))
-    with pytest.raises(BackendError) as e:
-        gpu.set_params(integrator=A.AHIP_INTEGRATOR_BDPT)
     assert e.value.status == A.AHIP_E_UNSUPPORTED
