@@ -70,18 +70,19 @@ __device__ __forceinline__ MatIn vertexInputs(const DMat& m) { return MatIn{ld3(
 // the opaque StandardSurface (StandardSurface.h:410-417) reject the pair as opposite hemispheres and
 // give 0, and rough glass answers with its transmission lobe, whose reverse pdf is its forward one
 // (StandardSurface.h:444, 475); at other pairs it is formed for Lambert and glass only.
+template <bool GLASS>
 __device__ __forceinline__ V3 evalVertex(const DScene& sc, int32_t mi, V3 n, V3 wo, V3 wi, float& fwd, float& rev) {
     fwd = rev = 0.f;
     if (mi < 0) return V3{0.f, 0.f, 0.f};
     const DMat& m = sc.mats[mi];
     const Frame fr = makeFrame(n, n, -wo);
-    const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), wo, wi, fwd);
+    const V3 f = matEval<false, GLASS>(sc, m, fr, vertexInputs(m), wo, wi, fwd);
     if (isBlack(f)) return f;
     if (m.type != AHIP_MAT_STANDARD) {
         rev = stdmax(0.f, fabsf(toLocal(fr, wo).z)) * kInvPi;
     } else {
         const bool across = !(toLocal(fr, wo).z * toLocal(fr, wi).z > 0.f);
-        if ((across && m.trans) || (!across && m.glass)) rev = fwd;
+        if (GLASS && ((across && m.trans) || (!across && m.glass))) rev = fwd;
     }
     return f * fabsf(dot(wi, n));
 }
@@ -93,6 +94,7 @@ __device__ __forceinline__ float hs(const BdptArgs& a, uint32_t px, uint32_t py,
 // The bounce loop shared by the two subpaths (BDPTIntegrator.cpp:97-190 / 240-300): from vertex
 // `count - 1` at prevP along `r`, recording surface vertices until a miss, roulette or an invalid
 // sample.  `light`: the light subpath (no emission recorded).  Returns the vertex count.
+template <bool TRANSP, bool INF>
 __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, uint32_t i, uint32_t sub,
                          uint32_t count, V3 prevP, TRay r, V3 beta, float pdfFwd, uint32_t px, uint32_t py,
                          uint32_t sidx, uint32_t& dim, bool light, TraceCounters& tc) {
@@ -102,7 +104,7 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         if (h == ~0u) {
             // the camera subpath's environment vertex (BDPTIntegrator.cpp:100-118): the escaped
             // direction's radiance, an area pdf of 0
-            if (!light && sc.hasEnv && count < a.maxV) {
+            if (INF && !light && sc.hasEnv && count < a.maxV) {
                 const uint32_t vi = count++;
                 a.verts[vAt(a, sub, vi, 0, i)] = f4(r.o + r.d * 1e6f, 0.f);
                 a.verts[vAt(a, sub, vi, 1, i)] = f4(-r.d, 0.f);
@@ -125,7 +127,7 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         if (mi < 0) break;
         const DMat& m = sc.mats[mi];
         const V3 dir = r.d, wo = -dir;
-        if (!light && sc.hasAlpha) {
+        if (TRANSP && !light && sc.hasAlpha) {
             // the camera subpath's alpha test (BDPTIntegrator.cpp:130-143): a pass-through is not a
             // bounce; a sample is drawn only for a partial opacity
             const float op = matOpacity(sc, m, uv);
@@ -151,7 +153,7 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         a.verts[vAt(a, sub, vi, 4, i)] = f4((!light && m.hasLe) ? ld3(m.Le) : V3{0.f, 0.f, 0.f}, __int_as_float(mi));
         if (vi > 0u) {   // the previous vertex's reverse pdf, from this one
             float dummy, revSA;
-            evalVertex(sc, mi, n, wo, -wo, dummy, revSA);
+            evalVertex<TRANSP>(sc, mi, n, wo, -wo, dummy, revSA);
             const float4 pv = a.verts[vAt(a, sub, vi - 1u, 0, i)], pn = a.verts[vAt(a, sub, vi - 1u, 1, i)];
             a.verts[vAt(a, sub, vi - 1u, 1, i)] = make_float4(pn.x, pn.y, pn.z, toArea(revSA, p, v3(pv), v3(pn)));
         }
@@ -165,7 +167,7 @@ __device__ uint32_t walk(const DScene& sc, const BdptArgs& a, const TStack& S, u
         ub.x = hs(a, px, py, sidx, dim);
         ub.y = hs(a, px, py, sidx, dim);
         const MatIn mIn = matInputs(sc, m, uv, T->commitIndex);
-        const BSample bs = matSample<false, true>(sc, m, fr, mIn, wo, ub, uc);
+        const BSample bs = matSample<false, TRANSP>(sc, m, fr, mIn, wo, ub, uc);
         if (!bsValid(bs)) break;
         beta = beta * (bs.f / bs.pdf);
         pdfFwd = bs.pdf;
@@ -191,9 +193,10 @@ struct LeSample {
 // (a direction from the sky's distribution, the origin on a disk of the scene sphere's radius facing
 // the scene at its far side), DirectionalLight.h:88-125 (a disk 100 radii toward the light, the
 // direction jittered within the cone).  A zero pdfPos marks no sample.
+template <bool INF>
 __device__ LeSample lightSampleLe(const DScene& sc, const DLight& L, V2 uPos, V2 uDir) {
     LeSample s{};
-    if (L.type == AHIP_LIGHT_DOME) {
+    if (INF && L.type == AHIP_LIGHT_DOME) {
         float pdfDir;
         const V3 envDir = domeSampleDir(sc.dome, uDir, pdfDir);
         if (pdfDir <= 0.f) return s;
@@ -211,7 +214,7 @@ __device__ LeSample lightSampleLe(const DScene& sc, const DLight& L, V2 uPos, V2
         s.pdfDir = pdfDir;
         return s;
     }
-    if (L.type == AHIP_LIGHT_DIRECTIONAL) {
+    if (INF && L.type == AHIP_LIGHT_DIRECTIONAL) {
         const float R = L.sceneR, r = R * sqrtf(uPos.x), phi = 2.f * kPi * uPos.y;
         float sp, cp;
         sincosf(phi, &sp, &cp);
@@ -258,6 +261,8 @@ __device__ LeSample lightSampleLe(const DScene& sc, const DLight& L, V2 uPos, V2
 
 }  // namespace
 
+// TRANSP: a scene with glass or alpha masks; INF: a scene with the dome or a directional light
+template <bool TRANSP, bool INF>
 __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
     __shared__ uint64_t lstk[kBdptK * kBdptBS];
     TStack S{lstk, a.ovfRef, a.ovfT, threadIdx.x, kBdptBS, kBdptK, blockIdx.x * kBdptBS + threadIdx.x,
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
         a.verts[vAt(a, 0, 0, 2, i)] = f4(-d, __uint_as_float(kVCamera));
         a.verts[vAt(a, 0, 0, 3, i)] = make_float4(1.f, 1.f, 1.f, 1.f);
         a.verts[vAt(a, 0, 0, 4, i)] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-        const uint32_t nc = walk(sc, a, S, i, 0u, 1u, o, makeTRay(o, d, 1e-4f, __int_as_float(0x7f800000)),
+        const uint32_t nc = walk<TRANSP, INF>(sc, a, S, i, 0u, 1u, o, makeTRay(o, d, 1e-4f, __int_as_float(0x7f800000)),
                                  V3{1.f, 1.f, 1.f}, 1.f, px, py, sidx, dim, false, tc);
         // first-hit AOVs (BDPTIntegrator.cpp:502-512): IMaterial::reflectance at the first vertex
         float4 alb = make_float4(0.f, 0.f, 0.f, 0.f), nrm = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -308,7 +313,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
             const V2 uDir{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
             const V2 uPos{hs(a, px, py, sidx, dim), hs(a, px, py, sidx, dim)};
             if (lpdf > 0.f) {
-                LeSample ls = lightSampleLe(sc, L, uPos, uDir);
+                LeSample ls = lightSampleLe<INF>(sc, L, uPos, uDir);
                 if (ls.pdfPos > 0.f && ls.pdfDir > 0.f) {
                     const V3 beta = ls.Le * fabsf(dot(ls.dir, ls.n)) / (lpdf * ls.pdfPos * ls.pdfDir);
                     const uint32_t fl = kVLight | (L.type == AHIP_LIGHT_DIRECTIONAL ? kVDelta : 0u) |
@@ -318,7 +323,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_paths(DScene sc, BdptArgs a) {
                     a.verts[vAt(a, 1, 0, 2, i)] = f4(ls.dir, __uint_as_float(fl));
                     a.verts[vAt(a, 1, 0, 3, i)] = f4(ls.Le / (lpdf * ls.pdfPos), 1.f);
                     a.verts[vAt(a, 1, 0, 4, i)] = f4(ls.Le, __int_as_float((int32_t)li));
-                    nl = walk(sc, a, S, i, 1u, 1u, ls.pos, makeTRay(ls.pos, ls.dir, 1e-4f, __int_as_float(0x7f800000)),
+                    nl = walk<TRANSP, INF>(sc, a, S, i, 1u, 1u, ls.pos, makeTRay(ls.pos, ls.dir, 1e-4f, __int_as_float(0x7f800000)),
                               beta, ls.pdfDir, px, py, sidx, dim, true, tc);
                 }
             }
@@ -359,6 +364,7 @@ __device__ float misWeight(const BdptArgs& a, uint32_t i, uint32_t nl, uint32_t 
 // visibility: the weighted, clamped contribution goes to the slot's term list (term k of slot i at
 // terms[k * cap + i]); a strategy that needs a visibility test leaves its term at weight 0 and
 // appends its shadow ray (spawnRayTo, Types.h:277-284) to the connection queue.
+template <bool TRANSP, bool INF>
 __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a) {
     const uint32_t lane = __lane_id();
     for (uint32_t base0 = blockIdx.x * kBdptBS; base0 < a.slots; base0 += gridDim.x * kBdptBS) {
@@ -384,7 +390,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                     // (0, t): emission at the last camera vertex (BDPTIntegrator.cpp:318-343); an
                     // environment vertex only straight after the camera (t = 2)
                     if (t >= 2u && (cSurf || (t == 2u && (cflags & kVInf)))) C = cbeta * v3(c4);
-                } else if (s == 1u && (__float_as_uint(a.verts[vAt(a, 1, 0, 2, i)].w) & kVInf)) {
+                } else if (INF && s == 1u && (__float_as_uint(a.verts[vAt(a, 1, 0, 2, i)].w) & kVInf)) {
                     // (1, t) from the dome or a directional light: along the light subpath's own
                     // direction, a shadow ray 1e10 long, Le / (light pdf x directional pdf)
                     // (:357-365, :395-402; DirectionalLight.h:78-80: pdf 0, taken as 1)
@@ -396,7 +402,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                         float pdf;
                         const DMat& m = sc.mats[cm];
                         const Frame fr = makeFrame(cn, cn, -cwo);
-                        const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
+                        const V3 f = matEval<false, TRANSP>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
                         V3 Le{0.f, 0.f, 0.f};
                         float pdfDir = 1.f;
                         if (L.type == AHIP_LIGHT_DOME) {
@@ -427,7 +433,7 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                             float pdf;
                             const DMat& m = sc.mats[cm];
                             const Frame fr = makeFrame(cn, cn, -cwo);
-                            const V3 f = matEval<false, true>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
+                            const V3 f = matEval<false, TRANSP>(sc, m, fr, vertexInputs(m), cwo, wi, pdf);
                             const V3 ln = v3(l1);
                             const V3 Le = dot(wi, ln) < 0.f ? v3(l4) : V3{0.f, 0.f, 0.f};   // AreaLight.h:110-112
                             if (!isBlack(f) && !isBlack(Le)) {
@@ -456,8 +462,8 @@ __global__ __launch_bounds__(kBdptBS) void k_bdpt_conn_gen(DScene sc, BdptArgs a
                         if (dist >= 1e-6f) {
                             const V3 wi = d * (1.f / dist);
                             float f1, r1, f2, r2;
-                            const V3 fL = evalVertex(sc, lm, v3(l1), v3(l2), wi, f1, r1);
-                            const V3 fC = isBlack(fL) ? V3{0.f, 0.f, 0.f} : evalVertex(sc, cm, cn, cwo, -wi, f2, r2);
+                            const V3 fL = evalVertex<TRANSP>(sc, lm, v3(l1), v3(l2), wi, f1, r1);
+                            const V3 fC = isBlack(fL) ? V3{0.f, 0.f, 0.f} : evalVertex<TRANSP>(sc, cm, cn, cwo, -wi, f2, r2);
                             if (!isBlack(fC)) {
                                 C = v3(l3) * fL * (1.f / (dist * dist)) * fC * cbeta;
                                 ray = true;
@@ -611,8 +617,20 @@ __global__ void k_bdpt_zero(uint32_t* p) { *p = 0u; }
 
 void launchBdpt(const DScene& sc, const BdptArgs& a, uint32_t blocks, uint32_t traceBlocks, hipStream_t s) {
     hipLaunchKernelGGL(k_bdpt_zero, dim3(1), dim3(1), 0, s, a.connCount);
-    hipLaunchKernelGGL(k_bdpt_paths, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
-    hipLaunchKernelGGL(k_bdpt_conn_gen, dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    const bool inf = sc.hasEnv || sc.hasDirectional;
+    if (sc.hasTransparent && inf) {
+        hipLaunchKernelGGL((k_bdpt_paths<true, true>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+        hipLaunchKernelGGL((k_bdpt_conn_gen<true, true>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    } else if (sc.hasTransparent) {
+        hipLaunchKernelGGL((k_bdpt_paths<true, false>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+        hipLaunchKernelGGL((k_bdpt_conn_gen<true, false>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    } else if (inf) {
+        hipLaunchKernelGGL((k_bdpt_paths<false, true>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+        hipLaunchKernelGGL((k_bdpt_conn_gen<false, true>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    } else {
+        hipLaunchKernelGGL((k_bdpt_paths<false, false>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+        hipLaunchKernelGGL((k_bdpt_conn_gen<false, false>), dim3(blocks), dim3(kBdptBS), 0, s, sc, a);
+    }
     if (sc.hasTransparent) hipLaunchKernelGGL(k_bdpt_conn_trace<true>, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
     else hipLaunchKernelGGL(k_bdpt_conn_trace<false>, dim3(traceBlocks), dim3(kBdptBS), 0, s, sc, a);
     hipLaunchKernelGGL(k_bdpt_conn_resolve, dim3((a.slots + 255u) / 256u), dim3(256), 0, s, a);

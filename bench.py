@@ -451,6 +451,20 @@ def bdpt_form(be, args, log):
            "ms_per_frame": round(t, 3), "msamples_per_s": round(samples / t / 1e3, 2),
            "published_cpu": {"ms_per_frame": 1168, "host": "Apple M3 Pro, 11 threads", "source": "README.md:444-448"}}
     log(f"[bench] bdpt: {t:.1f} ms/frame, {samples / t / 1e3:.1f} Msamples/s")
+    # the same integrator on the 1M-triangle workload frame under its sky (one warm-up, one timed)
+    wl = scenes.ggx_spheres()
+    be.prepare(wl, integrator=A.AHIP_INTEGRATOR_BDPT, firefly_clamp=10.0)
+    be.clear_accum()
+    be.render_frame(0, wl.spp)
+    be.reset_stats()
+    be.clear_accum()
+    be.render_frame(0, wl.spp)
+    tw = be.stats()["render_ms"]
+    ws = wl.width * wl.height * wl.spp
+    out["workload_1m"] = {"workload": f"ggx_spheres: {wl.num_triangles} tris, {wl.width}x{wl.height}, {wl.spp} spp, "
+                                     f"depth {wl.max_depth}, the 2048x1024 sky (dome) light, BDPTIntegrator",
+                          "ms_per_frame": round(tw, 2), "msamples_per_s": round(ws / tw / 1e3, 2)}
+    log(f"[bench] bdpt workload: {tw:.1f} ms/frame, {ws / tw / 1e3:.1f} Msamples/s")
     be.set_params(sc.max_depth, sc.min_depth, sc.seed)   # back to the path tracer
     if args.no_cpu_baseline:
         return out

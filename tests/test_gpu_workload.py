@@ -178,3 +178,36 @@ def test_batched_workload_views(gpu, ggx):
           f"equal {tot['equal']}, divergent {tot['divergent']} of {tot['samples']}", flush=True)
     # measured on the MI355X: equal 99.67%, divergent 0.012%, pixels 99.22%
     _assert_stats(tot, "64 view tiles", equal=0.995, divergent=5e-4, pixels=0.99, mean=1e-3)
+
+
+def test_ggx_frame_bdpt_bands_vs_oracle(gpu, ggx):
+    """BDPT on the 1M-triangle workload (1920x1080 x 16 spp under the 2048x1024 sky with its sun,
+    3x3 StandardSurface spheres; environment vertices, light subpaths from the sky's emission disk):
+    the whole frame through ahip_render_frame, two 8-row bands against the oracle's
+    BDPTIntegrator, each sample on its own: sample counts equal, band means within 1e-3, and the
+    samples in the classes of the path-tracer test above (equal within 1e-3, divergent beyond 0.1)."""
+    from anacapa_amd import _abi as A
+    sc, os_ = ggx
+    gpu.prepare(sc, integrator=A.AHIP_INTEGRATOR_BDPT)
+    gpu.clear_accum()
+    gpu.render_frame(0, sc.spp)
+    frame = gpu.download_accum()
+    assert np.isfinite(frame).all()
+    for y0 in (480, 864):
+        print(f"[workload] bdpt rows {y0}..{y0 + 7}", flush=True)
+        ref = os_.render_tile_bdpt(0, y0, sc.width, 8, 0, sc.spp)[0]
+        band = frame[y0:y0 + 8]
+        assert np.array_equal(band[..., 3], ref[..., 3])
+        gm, rm = band[..., :3].astype(np.float64).mean(), ref[..., :3].astype(np.float64).mean()
+        g = np.stack([gpu.render_tile(0, y0, sc.width, 8, s, 1)[..., :3] for s in range(sc.spp)]).astype(np.float64)
+        r = np.stack([os_.render_tile_bdpt(0, y0, sc.width, 8, s, 1)[0][..., :3]
+                      for s in range(sc.spp)]).astype(np.float64)
+        np.testing.assert_allclose(g.sum(0), band[..., :3], rtol=1e-5, atol=1e-5)
+        e = np.abs(g - r).max(-1) / np.maximum(np.abs(r).max(-1), 1e-3 * max(np.abs(r).mean(), 1e-6))
+        equal, divergent = (e <= 1e-3).mean(), (e > 0.1).mean()
+        print(f"[workload] bdpt rows {y0}..{y0 + 7}: means {gm:.6g} / {rm:.6g}; samples equal {equal:.5f}, "
+              f"divergent {divergent:.5f}", flush=True)
+        assert abs(gm / rm - 1) < 1e-3
+        assert equal >= 0.99
+        assert divergent <= 1e-3
+    gpu.set_params(sc.max_depth, sc.min_depth, sc.seed)
