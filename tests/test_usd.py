@@ -481,6 +481,16 @@ def test_distant_light_is_directional():
     assert np.allclose(list(z.direction), [0, 1, 0], atol=1e-6)   # local +Z is up in a Z-up layer
 
 
+def test_distant_light_scene_sphere():
+    """Once every mesh is loaded the loader gives its directional lights the scene's bounding sphere
+    (USDLoader.cpp:1730-1744): the bounds' centroid and 1.5 x the half diagonal, where BDPT's light
+    subpaths start."""
+    body = tri_mesh("T", pts=((0, 0, 0), (4, 0, 0), (0, 2, 6))) + 'def DistantLight "Sun"\n{\n}\n'
+    (l,) = load(layer(body)).lights
+    assert list(l.scene_center) == [2, 1, 3]
+    assert l.scene_radius == pytest.approx(0.5 * 1.5 * np.sqrt(16 + 4 + 36), rel=1e-6)
+
+
 @pytest.mark.parametrize("kind", ["DomeLight"])
 def test_unsupported_lights_are_refused(kind):
     with pytest.raises(UsdLoadError) as e:
